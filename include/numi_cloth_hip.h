@@ -1,0 +1,202 @@
+#pragma once
+
+/*
+ * C ABI of libnumi_cloth_hip.so: the XPBD cloth substep loop on AMD Instinct
+ * MI355X (gfx950).
+ *
+ * Every entry point takes plain pointers and sizes; no C++ or HIP type crosses
+ * the boundary. Functions return 0 (NUMI_CLOTH_HIP_OK) or a positive status;
+ * numi_cloth_hip_last_error() then holds a message for the calling thread.
+ * Device-side failures are reported the way the reference reports them: as
+ * the sticky NUMI_CLOTH_BAG_GPU_FAILURE_* word returned by read().
+ *
+ * The reference has no library; its de-facto host entry point is
+ *   runGPU(device, queue, pipelines, InitialState, iterations, strainSweeps,
+ *          perSubstepConfigs) -> GPUResult
+ * (reference tools/cloth_bag_metal.mm:4010-4018, result :3951-3967).
+ * create/step/read/destroy below split that call so that device state stays
+ * resident between calls; step() over N substeps followed by step() over M
+ * substeps gives the same bytes as one step() over N+M substeps.
+ */
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "numi/cloth_bag_gpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NUMI_CLOTH_HIP_EXTENSION_VERSION 1u
+
+enum NumiClothHipStatus {
+    NUMI_CLOTH_HIP_OK = 0,
+    NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT = 1,
+    NUMI_CLOTH_HIP_ERROR_TOPOLOGY = 2,
+    NUMI_CLOTH_HIP_ERROR_DEVICE = 3,
+    NUMI_CLOTH_HIP_ERROR_ALLOCATION = 4,
+    NUMI_CLOTH_HIP_ERROR_UNSUPPORTED = 5,
+    NUMI_CLOTH_HIP_ERROR_PARSE = 6
+};
+
+/*
+ * Scene facts that ABI v13 has no field for. Zero-initialise for the
+ * reference behaviour (no static colliders, no kinematic drive, hash
+ * self-contact off).
+ */
+typedef struct NumiClothHipSceneExtension {
+    /* Static spheres: xyz centre, w radius. Cloth knots are pushed to
+     * radius + yarn radius in the same pass as the ground. */
+    const mr_float4* staticSpheres;
+    uint32_t staticSphereCount;
+    /* 1 enables yarn/yarn self-contact through the device uniform hash. */
+    uint32_t selfContact;
+    /* Broad-phase margin added to the swept segment bounds once per
+     * substep; pairs beyond it are not tested again until the next substep. */
+    float selfContactMargin;
+    /* Maximum candidate pairs kept per substep (0 selects 8 per segment). */
+    uint32_t selfCandidateCapacity;
+    /* Kinematic drive of particles with zero inverse mass:
+     * z(t) = anchor.z + amplitude * sin(omega * t + phase), anchor = the
+     * position given at create, t = time at the end of the substep. */
+    float driveAmplitude;
+    float driveOmega;
+    float drivePhase;
+    /* Linear velocity damping coefficient c >= 0 (1/s): at the start of a
+     * substep v <- v / (1 + c dt) + g dt, so damping never scales the
+     * gravity increment and static equilibria are unchanged.
+     * 0 reproduces the reference. */
+    float damping;
+} NumiClothHipSceneExtension;
+
+/*
+ * Everything the reference's InitialState carries
+ * (reference tools/cloth_bag_metal.mm:466-488), as ABI v13 arrays.
+ * Arrays of count 0 may be NULL.
+ */
+typedef struct NumiClothHipSceneDesc {
+    const NumiClothBagGPUConfig* config;
+    const NumiClothBagGPUParticle* particles;
+    const NumiClothBagGPUDistance* distances;
+    const NumiClothBagGPUBatch* distanceBatches;
+    const NumiClothBagGPUGrip* grips;
+    const NumiClothBagGPUKnot* knots;
+    const NumiClothBagGPUBatch* knotBatches;
+    const NumiClothBagGPUBend* bends;
+    const NumiClothBagGPUBatch* bendBatches;
+    const NumiClothBagGPUFruit* fruits;
+    const NumiClothBagGPUFruitPair* fruitPairs;
+    const NumiClothBagGPUBatch* fruitPairBatches;
+    const NumiClothBagGPUYarnContact* yarnContacts;
+    const uint32_t* mouthRimParticles;
+    const uint32_t* cuffParticles;
+    uint32_t particleCount;
+    uint32_t distanceCount;
+    uint32_t distanceBatchCount;
+    uint32_t gripCount;
+    uint32_t knotCount;
+    uint32_t knotBatchCount;
+    uint32_t bendCount;
+    uint32_t bendBatchCount;
+    uint32_t fruitCount;
+    uint32_t fruitPairCount;
+    uint32_t fruitPairBatchCount;
+    uint32_t yarnContactCount;
+    uint32_t mouthRimCount;
+    uint32_t cuffCount;
+    uint32_t initialCandidateMask;
+    uint32_t initialReleasedMask;
+    NumiClothHipSceneExtension extension;
+} NumiClothHipSceneDesc;
+
+/* Destination arrays for read(); any pointer may be NULL to skip it. */
+typedef struct NumiClothHipState {
+    NumiClothBagGPUParticle* particles;
+    NumiClothBagGPUDistance* distances;
+    NumiClothBagGPUGrip* grips;
+    NumiClothBagGPUKnot* knots;
+    NumiClothBagGPUBend* bends;
+    NumiClothBagGPUFruit* fruits;
+    NumiClothBagGPUFruitPair* fruitPairs;
+    NumiClothBagGPUYarnContact* yarnContacts;
+    NumiClothBagGPUSelfStatus* selfStatus;
+    NumiClothBagGPUFrictionStatus* frictionStatus;
+    NumiClothBagGPUAerodynamicsStatus* aerodynamicsStatus;
+    NumiClothBagGPUReleaseStatus* releaseStatus;
+    uint32_t* failure;
+} NumiClothHipState;
+
+/* Residual diagnostics, reduced on the device without float atomics
+ * (replaces the host loops of reference tools/cloth_bag_metal.mm:5176-5276). */
+typedef struct NumiClothHipDiagnostics {
+    float maximumStrainViolation;   /* max(len - rest*(1+limit), 0) */
+    float maximumGroundPenetration; /* max(r - z, 0) over knots */
+    float maximumSpherePenetration; /* over static spheres */
+    float maximumSelfPenetration;   /* over current self candidates */
+    uint32_t selfCandidateCount;
+    uint32_t selfCandidateOverflow;
+    uint32_t reserved0;
+    uint32_t reserved1;
+} NumiClothHipDiagnostics;
+
+typedef struct NumiClothHipSolver NumiClothHipSolver;
+
+uint32_t numi_cloth_hip_abi_version(void);
+const char* numi_cloth_hip_last_error(void);
+
+/* Host->device boundary of runGPU (reference tools/cloth_bag_metal.mm:4019-4177):
+ * validates topology (ranges, colour batches) on the host, converts the
+ * records to device structure-of-arrays buffers and keeps them resident. */
+int numi_cloth_hip_create(const NumiClothHipSceneDesc* scene,
+                          NumiClothHipSolver** out);
+
+/* The per-substep schedule of runGPU (reference
+ * tools/cloth_bag_metal.mm:4211-4709): one substep per config. Configs may
+ * change only what the reference lets a trajectory change (:4044-4090);
+ * anything else returns NUMI_CLOTH_HIP_ERROR_TOPOLOGY. Asynchronous on the
+ * solver's stream. */
+int numi_cloth_hip_step(NumiClothHipSolver* solver,
+                        const NumiClothBagGPUConfig* configs,
+                        uint32_t substeps,
+                        uint32_t iterations,
+                        uint32_t strainSweeps);
+
+/* Device->host readback of runGPU (reference :4720-4757); synchronous. */
+int numi_cloth_hip_read(NumiClothHipSolver* solver,
+                        const NumiClothHipState* out);
+
+int numi_cloth_hip_diagnostics(NumiClothHipSolver* solver,
+                               NumiClothHipDiagnostics* out);
+
+int numi_cloth_hip_synchronize(NumiClothHipSolver* solver);
+
+/* Passes whose launches can be timed in isolation. */
+enum NumiClothHipPass {
+    NUMI_CLOTH_HIP_PASS_DISTANCE = 0,  /* one distance colour batch */
+    NUMI_CLOTH_HIP_PASS_BEND = 1,      /* one bend colour batch */
+    NUMI_CLOTH_HIP_PASS_STRAIN = 2,    /* one strain-limit colour batch */
+    NUMI_CLOTH_HIP_PASS_COLLIDE = 3    /* ground + static spheres over all knots */
+};
+
+typedef struct NumiClothHipPassTiming {
+    double averageMilliseconds;        /* per launch, from HIP events */
+    double algorithmicBytes;           /* unique bytes one launch must move */
+    uint32_t launches;
+    uint32_t itemsPerLaunch;
+} NumiClothHipPassTiming;
+
+/* Times `repeats` back-to-back launches of one pass on the solver's stream
+ * (colour batch `colour` where the pass has colours) between two HIP events.
+ * The launches act on the live state, as they would inside step(). */
+int numi_cloth_hip_time_pass(NumiClothHipSolver* solver, uint32_t pass, uint32_t colour,
+                             uint32_t repeats, NumiClothHipPassTiming* out);
+
+/* The hipStream_t all of the solver's work is queued on. */
+void* numi_cloth_hip_stream(NumiClothHipSolver* solver);
+
+int numi_cloth_hip_destroy(NumiClothHipSolver* solver);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,470 @@
+"""ctypes bindings of libnumi_cloth_hip.so.
+
+The C ABI (include/numi_cloth_hip.h, include/numi_cloth_scene.h) is the
+product boundary; this module only mirrors it for Python callers and tests.
+ABI v13 records are exposed as numpy structured dtypes with the field names
+of the reference header (reference include/numi/cloth_bag_gpu.h:19-190).
+
+There is no CPU fallback: if the shared library is missing, loading fails.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+PACKAGE = Path(__file__).resolve().parent
+LIBRARY_PATH = PACKAGE / "libnumi_cloth_hip.so"
+
+ABI_VERSION = 13
+FAILURE_ABI = 1
+FAILURE_RANGE = 2
+FAILURE_BATCH = 4
+FAILURE_NONFINITE = 8
+FAILURE_GRIP_CAPTURE = 16
+
+STATUS_NAMES = {0: "ok", 1: "invalid argument", 2: "topology", 3: "device", 4: "allocation",
+                5: "unsupported", 6: "parse"}
+
+_u4 = ("<u4", (4,))
+_f4 = ("<f4", (4,))
+
+
+def _dtype(*fields):
+    return np.dtype([(name, *kind) for name, kind in fields], align=True)
+
+
+CONFIG = _dtype(("control", _u4), ("constraintCounts", _u4), ("contactCounts", _u4),
+                ("gravityAndTimestep", _f4), ("gripTargetAndActive", _f4), ("gripOrientation", _f4),
+                ("gripControl", _u4), ("gripMaterial", _f4), ("clothMaterial", _f4), ("fruitMaterial", _f4),
+                ("airVelocityAndDensity", _f4), ("aerodynamicCoefficients", _f4), ("mouthControl", _u4),
+                ("mouthMaterial", _f4))
+PARTICLE = _dtype(("positionAndInverseMass", _f4), ("previousAndMass", _f4), ("velocity", _f4))
+DISTANCE = _dtype(("particlesAndColor", _u4), ("material", _f4))
+GRIP = _dtype(("particle", _u4), ("targetOffsetAndCompliance", _f4), ("lambda", _f4))
+KNOT = _dtype(("particles", _u4), ("control", _u4), ("material", _f4))
+BEND = _dtype(("particlesAndColor", _u4), ("material", _f4))
+FRUIT = _dtype(("positionAndInverseMass", _f4), ("previousAndRadius", _f4), ("velocityAndGroundImpulse", _f4),
+               ("angularVelocity", _f4), ("orientation", _f4), ("identity", _u4))
+FRUIT_PAIR = _dtype(("fruitsAndColor", _u4), ("contact", _f4))
+YARN_CONTACT = _dtype(("identity", _u4), ("currentNormalAndSeparation", _f4), ("sweptNormalAndAdvance", _f4),
+                      ("weightsAndTime", _f4), ("control", _u4), ("fruitNormalAndImpulse", _f4),
+                      ("segmentImpulse", _f4))
+BATCH = _dtype(("control", _u4))
+SELF_STATUS = _dtype(("counters", _u4))
+FRICTION_STATUS = _dtype(("counters", _u4), ("metrics", _u4))
+AERO_STATUS = _dtype(("metrics", _u4))
+RELEASE_STATUS = _dtype(("masks", _u4))
+
+RECORD_SIZES = {"config": 224, "particle": 48, "distance": 32, "grip": 48, "knot": 48, "bend": 32,
+                "fruit": 96, "fruit_pair": 32, "yarn_contact": 112, "batch": 16}
+for _name, _dt in [("config", CONFIG), ("particle", PARTICLE), ("distance", DISTANCE), ("grip", GRIP),
+                   ("knot", KNOT), ("bend", BEND), ("fruit", FRUIT), ("fruit_pair", FRUIT_PAIR),
+                   ("yarn_contact", YARN_CONTACT), ("batch", BATCH)]:
+    assert _dt.itemsize == RECORD_SIZES[_name], _name
+
+
+class Float4(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float), ("w", C.c_float)]
+
+
+class SceneExtension(C.Structure):
+    _fields_ = [("staticSpheres", C.c_void_p), ("staticSphereCount", C.c_uint32), ("selfContact", C.c_uint32),
+                ("selfContactMargin", C.c_float), ("selfCandidateCapacity", C.c_uint32),
+                ("driveAmplitude", C.c_float), ("driveOmega", C.c_float), ("drivePhase", C.c_float),
+                ("damping", C.c_float)]
+
+
+_DESC_POINTERS = ["config", "particles", "distances", "distanceBatches", "grips", "knots", "knotBatches", "bends",
+                  "bendBatches", "fruits", "fruitPairs", "fruitPairBatches", "yarnContacts", "mouthRimParticles",
+                  "cuffParticles"]
+_DESC_COUNTS = ["particleCount", "distanceCount", "distanceBatchCount", "gripCount", "knotCount",
+                "knotBatchCount", "bendCount", "bendBatchCount", "fruitCount", "fruitPairCount",
+                "fruitPairBatchCount", "yarnContactCount", "mouthRimCount", "cuffCount", "initialCandidateMask",
+                "initialReleasedMask"]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = ([(name, C.c_void_p) for name in _DESC_POINTERS] + [(name, C.c_uint32) for name in _DESC_COUNTS]
+                + [("extension", SceneExtension)])
+
+
+class State(C.Structure):
+    _fields_ = [(name, C.c_void_p) for name in
+                ["particles", "distances", "grips", "knots", "bends", "fruits", "fruitPairs", "yarnContacts",
+                 "selfStatus", "frictionStatus", "aerodynamicsStatus", "releaseStatus", "failure"]]
+
+
+class Diagnostics(C.Structure):
+    _fields_ = [("maximumStrainViolation", C.c_float), ("maximumGroundPenetration", C.c_float),
+                ("maximumSpherePenetration", C.c_float), ("maximumSelfPenetration", C.c_float),
+                ("selfCandidateCount", C.c_uint32), ("selfCandidateOverflow", C.c_uint32),
+                ("reserved0", C.c_uint32), ("reserved1", C.c_uint32)]
+
+
+MATERIAL_FIELDS = ["ordinaryNodeMassKg", "hemNodeMassKg", "yarnRadiusM", "axialBodyComplianceMPerN",
+                   "axialCuffComplianceMPerN", "knotCompliance", "bendBodyComplianceMPerN",
+                   "bendCuffComplianceMPerN", "gripComplianceMPerN", "clothGroundFriction", "clothSelfFriction",
+                   "fruitYarnFriction", "fruitGroundFriction", "fruitPairFriction", "fruitRollingResistance",
+                   "yarnCrossflowDrag", "yarnSkinFriction", "fruitDrag", "fruitRotationalDrag"]
+
+
+class PassTiming(C.Structure):
+    _fields_ = [("averageMilliseconds", C.c_double), ("algorithmicBytes", C.c_double), ("launches", C.c_uint32),
+                ("itemsPerLaunch", C.c_uint32)]
+
+
+PASS_DISTANCE, PASS_BEND, PASS_STRAIN, PASS_COLLIDE = 0, 1, 2, 3
+
+
+class MaterialValues(C.Structure):
+    _fields_ = [(name, C.c_double) for name in MATERIAL_FIELDS]
+
+
+class MaterialArtifact(C.Structure):
+    _fields_ = [("values", MaterialValues), ("loaded", C.c_uint32), ("parametersHash", C.c_char * 128),
+                ("observationsHash", C.c_char * 128)]
+
+
+class GripPose(C.Structure):
+    _fields_ = [("timeSeconds", C.c_double), ("translation", C.c_double * 3), ("orientation", C.c_double * 4),
+                ("active", C.c_uint32), ("attachmentGeneration", C.c_uint32)]
+
+
+class GripTrajectoryInfo(C.Structure):
+    _fields_ = [("poseCount", C.c_uint32), ("selectNearestCuffPatch", C.c_uint32),
+                ("attachmentGenerations", C.c_uint32), ("reserved", C.c_uint32), ("durationSeconds", C.c_double),
+                ("maximumRotation", C.c_double), ("schema", C.c_char * 32), ("contentFingerprint", C.c_char * 32)]
+
+
+class SheetSpec(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("spacing", C.c_float), ("elevation", C.c_float),
+                ("jitter", C.c_float), ("arealDensity", C.c_float), ("yarnRadius", C.c_float),
+                ("stretchCompliance", C.c_float), ("bendCompliance", C.c_float), ("knotCompliance", C.c_float),
+                ("strainLimit", C.c_float), ("cellSize", C.c_float), ("timestep", C.c_float),
+                ("gravityZ", C.c_float), ("groundFriction", C.c_float), ("selfFriction", C.c_float),
+                ("crossingKnots", C.c_uint32), ("ground", C.c_uint32), ("pinTopRow", C.c_uint32),
+                ("selfContact", C.c_uint32), ("sphereMode", C.c_uint32), ("sphereCount", C.c_uint32),
+                ("sphereRadiusMin", C.c_float), ("sphereRadiusMax", C.c_float), ("fixedSphere", C.c_float * 4),
+                ("seed", C.c_uint64)]
+
+
+class BagSpec(C.Structure):
+    _fields_ = [("material", MaterialValues), ("stretchComplianceOverride", C.c_float),
+                ("bendComplianceOverride", C.c_float), ("strainLimitOverride", C.c_float),
+                ("fruitRadiusMin", C.c_float), ("fruitRadiusMax", C.c_float), ("selfContact", C.c_uint32),
+                ("reserved", C.c_uint32), ("seed", C.c_uint64)]
+
+
+class ClothError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+        self.message = message
+
+
+_lib = None
+
+
+def library() -> C.CDLL:
+    """Load the in-tree shared library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIBRARY_PATH.exists():
+        raise RuntimeError(f"{LIBRARY_PATH} is missing: build it with __graft_entry__.build()")
+    lib = C.CDLL(str(LIBRARY_PATH))
+    vp, u32, i32 = C.c_void_p, C.c_uint32, C.c_int
+    signatures = {
+        "numi_cloth_hip_abi_version": ([], u32),
+        "numi_cloth_hip_last_error": ([], C.c_char_p),
+        "numi_cloth_hip_create": ([vp, C.POINTER(vp)], i32),
+        "numi_cloth_hip_step": ([vp, vp, u32, u32, u32], i32),
+        "numi_cloth_hip_read": ([vp, C.POINTER(State)], i32),
+        "numi_cloth_hip_diagnostics": ([vp, C.POINTER(Diagnostics)], i32),
+        "numi_cloth_hip_synchronize": ([vp], i32),
+        "numi_cloth_hip_stream": ([vp], vp),
+        "numi_cloth_hip_destroy": ([vp], i32),
+        "numi_cloth_hip_time_pass": ([vp, u32, u32, u32, C.POINTER(PassTiming)], i32),
+        "numi_cloth_material_defaults": ([C.POINTER(MaterialArtifact)], i32),
+        "numi_cloth_material_load": ([C.c_char_p, C.POINTER(MaterialArtifact)], i32),
+        "numi_cloth_grip_trajectory_load": ([C.c_char_p, C.POINTER(vp)], i32),
+        "numi_cloth_grip_trajectory_info": ([vp, C.POINTER(GripTrajectoryInfo)], i32),
+        "numi_cloth_grip_trajectory_pose": ([vp, u32, C.POINTER(GripPose)], i32),
+        "numi_cloth_grip_trajectory_sample": ([vp, C.c_double, C.POINTER(GripPose)], i32),
+        "numi_cloth_grip_trajectory_destroy": ([vp], None),
+        "numi_cloth_scene_sheet": ([C.POINTER(SheetSpec), C.POINTER(vp)], i32),
+        "numi_cloth_scene_bag": ([C.POINTER(BagSpec), C.POINTER(vp)], i32),
+        "numi_cloth_scene_desc": ([vp], C.POINTER(SceneDesc)),
+        "numi_cloth_scene_destroy": ([vp], None),
+        "numi_cloth_splitmix64": ([C.POINTER(C.c_uint64)], C.c_uint64),
+        "numi_cloth_fnv1a64": ([vp, C.c_size_t, C.c_uint64], C.c_uint64),
+    }
+    for name, (args, result) in signatures.items():
+        function = getattr(lib, name)
+        function.argtypes = args
+        function.restype = result
+    _lib = lib
+    return lib
+
+
+def _check(status: int) -> None:
+    if status != 0:
+        raise ClothError(status, library().numi_cloth_hip_last_error().decode())
+
+
+def _ptr(array) -> int | None:
+    return None if array is None or len(array) == 0 else array.ctypes.data
+
+
+# ---------------------------------------------------------------- artifacts --
+
+def material_defaults() -> MaterialArtifact:
+    artifact = MaterialArtifact()
+    _check(library().numi_cloth_material_defaults(C.byref(artifact)))
+    return artifact
+
+
+def load_material(path) -> MaterialArtifact:
+    artifact = MaterialArtifact()
+    _check(library().numi_cloth_material_load(str(path).encode(), C.byref(artifact)))
+    return artifact
+
+
+class GripTrajectory:
+    """numi.grip.trajectory v1/v2/v3 (reference include/numi/grip_trajectory.h:189-362)."""
+
+    def __init__(self, path):
+        handle = C.c_void_p()
+        _check(library().numi_cloth_grip_trajectory_load(str(path).encode(), C.byref(handle)))
+        self._handle = handle
+
+    def __del__(self):
+        if getattr(self, "_handle", None):
+            library().numi_cloth_grip_trajectory_destroy(self._handle)
+            self._handle = None
+
+    def info(self) -> GripTrajectoryInfo:
+        out = GripTrajectoryInfo()
+        _check(library().numi_cloth_grip_trajectory_info(self._handle, C.byref(out)))
+        return out
+
+    def pose(self, index: int) -> GripPose:
+        out = GripPose()
+        _check(library().numi_cloth_grip_trajectory_pose(self._handle, index, C.byref(out)))
+        return out
+
+    def sample(self, time_seconds: float) -> GripPose:
+        out = GripPose()
+        _check(library().numi_cloth_grip_trajectory_sample(self._handle, time_seconds, C.byref(out)))
+        return out
+
+
+# -------------------------------------------------------------------- scene --
+
+class Scene:
+    """ABI v13 arrays plus the extension block, as numpy arrays.
+
+    Built either by a C builder (sheet(), bag()) or directly from arrays
+    (probe scenes). `desc` is the C descriptor pointing into these arrays.
+    """
+
+    ARRAYS = [("particles", PARTICLE, "particleCount"), ("distances", DISTANCE, "distanceCount"),
+              ("distanceBatches", BATCH, "distanceBatchCount"), ("grips", GRIP, "gripCount"),
+              ("knots", KNOT, "knotCount"), ("knotBatches", BATCH, "knotBatchCount"),
+              ("bends", BEND, "bendCount"), ("bendBatches", BATCH, "bendBatchCount"),
+              ("fruits", FRUIT, "fruitCount"), ("fruitPairs", FRUIT_PAIR, "fruitPairCount"),
+              ("fruitPairBatches", BATCH, "fruitPairBatchCount"), ("yarnContacts", YARN_CONTACT, "yarnContactCount"),
+              ("mouthRimParticles", np.dtype("<u4"), "mouthRimCount"), ("cuffParticles", np.dtype("<u4"), "cuffCount")]
+
+    def __init__(self, config, **arrays):
+        self.config = np.array(config, dtype=CONFIG).reshape(1)
+        for name, dtype, _ in self.ARRAYS:
+            value = arrays.pop(name, None)
+            setattr(self, name, np.ascontiguousarray(np.zeros(0, dtype) if value is None else value, dtype=dtype))
+        spheres = arrays.pop("staticSpheres", None)
+        self.staticSpheres = np.ascontiguousarray(np.zeros((0, 4), np.float32) if spheres is None else spheres,
+                                                  dtype=np.float32).reshape(-1, 4)
+        self.selfContact = int(arrays.pop("selfContact", 0))
+        self.selfContactMargin = float(arrays.pop("selfContactMargin", 0.0))
+        self.selfCandidateCapacity = int(arrays.pop("selfCandidateCapacity", 0))
+        self.driveAmplitude = float(arrays.pop("driveAmplitude", 0.0))
+        self.driveOmega = float(arrays.pop("driveOmega", 0.0))
+        self.drivePhase = float(arrays.pop("drivePhase", 0.0))
+        self.damping = float(arrays.pop("damping", 0.0))
+        self.initialCandidateMask = int(arrays.pop("initialCandidateMask", 0))
+        self.initialReleasedMask = int(arrays.pop("initialReleasedMask", 0))
+        if arrays:
+            raise TypeError(f"unknown scene arrays: {sorted(arrays)}")
+        self.desc = self._describe()
+
+    def _describe(self) -> SceneDesc:
+        desc = SceneDesc()
+        desc.config = self.config.ctypes.data
+        for name, _, count in self.ARRAYS:
+            array = getattr(self, name)
+            setattr(desc, name, _ptr(array))
+            setattr(desc, count, len(array))
+        desc.initialCandidateMask = self.initialCandidateMask
+        desc.initialReleasedMask = self.initialReleasedMask
+        e = desc.extension
+        e.staticSpheres = _ptr(self.staticSpheres)
+        e.staticSphereCount = len(self.staticSpheres)
+        e.selfContact = self.selfContact
+        e.selfContactMargin = self.selfContactMargin
+        e.selfCandidateCapacity = self.selfCandidateCapacity
+        e.driveAmplitude = self.driveAmplitude
+        e.driveOmega = self.driveOmega
+        e.drivePhase = self.drivePhase
+        e.damping = self.damping
+        return desc
+
+    def refresh(self) -> None:
+        """Re-point the descriptor after arrays or extension fields changed."""
+        self.desc = self._describe()
+
+    @classmethod
+    def _from_builder(cls, handle) -> "Scene":
+        lib = library()
+        d = lib.numi_cloth_scene_desc(handle).contents
+        arrays = {}
+        for name, dtype, count in cls.ARRAYS:
+            n = getattr(d, count)
+            pointer = getattr(d, name)
+            if n == 0 or not pointer:
+                arrays[name] = np.zeros(0, dtype)
+            else:
+                raw = (C.c_ubyte * (n * dtype.itemsize)).from_address(pointer)
+                arrays[name] = np.frombuffer(raw, dtype=dtype, count=n).copy()
+        config = np.frombuffer((C.c_ubyte * CONFIG.itemsize).from_address(d.config), dtype=CONFIG).copy()
+        e = d.extension
+        spheres = np.zeros((0, 4), np.float32)
+        if e.staticSphereCount:
+            raw = (C.c_float * (4 * e.staticSphereCount)).from_address(e.staticSpheres)
+            spheres = np.frombuffer(raw, dtype=np.float32).reshape(-1, 4).copy()
+        scene = cls(config, staticSpheres=spheres, selfContact=e.selfContact, selfContactMargin=e.selfContactMargin,
+                    selfCandidateCapacity=e.selfCandidateCapacity, driveAmplitude=e.driveAmplitude,
+                    driveOmega=e.driveOmega, drivePhase=e.drivePhase, damping=e.damping,
+                    initialCandidateMask=d.initialCandidateMask, initialReleasedMask=d.initialReleasedMask, **arrays)
+        lib.numi_cloth_scene_destroy(handle)
+        return scene
+
+    @classmethod
+    def sheet(cls, **fields) -> "Scene":
+        spec = SheetSpec()
+        defaults = dict(spacing=0.01, elevation=0.3, jitter=0.0, arealDensity=0.2, yarnRadius=0.004,
+                        stretchCompliance=1.0e-7, bendCompliance=1.0e-4, knotCompliance=2.0e-6, strainLimit=0.10,
+                        cellSize=0.01, timestep=1.0 / 1920.0, gravityZ=-9.81, groundFriction=0.45,
+                        selfFriction=0.34, ground=1, seed=1)
+        defaults.update(fields)
+        for key, value in defaults.items():
+            if key == "fixedSphere":
+                spec.fixedSphere = (C.c_float * 4)(*value)
+            else:
+                setattr(spec, key, value)
+        handle = C.c_void_p()
+        _check(library().numi_cloth_scene_sheet(C.byref(spec), C.byref(handle)))
+        return cls._from_builder(handle)
+
+    @classmethod
+    def bag(cls, material: MaterialArtifact | None = None, **fields) -> "Scene":
+        spec = BagSpec()
+        spec.material = (material or material_defaults()).values
+        for key, value in fields.items():
+            setattr(spec, key, value)
+        handle = C.c_void_p()
+        _check(library().numi_cloth_scene_bag(C.byref(spec), C.byref(handle)))
+        return cls._from_builder(handle)
+
+    def configs(self, substeps: int) -> np.ndarray:
+        """`substeps` copies of the initial configuration."""
+        return np.repeat(self.config, substeps)
+
+
+# ------------------------------------------------------------------- solver --
+
+class Solver:
+    """Device-resident cloth state on the current HIP device."""
+
+    def __init__(self, scene: Scene):
+        self.scene = scene
+        handle = C.c_void_p()
+        _check(library().numi_cloth_hip_create(C.byref(scene.desc), C.byref(handle)))
+        self._handle = handle
+
+    def close(self) -> None:
+        if getattr(self, "_handle", None):
+            _check(library().numi_cloth_hip_destroy(self._handle))
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def stream(self) -> int:
+        return library().numi_cloth_hip_stream(self._handle)
+
+    def step(self, configs: np.ndarray, iterations: int = 32, strain_sweeps: int = 3) -> None:
+        configs = np.ascontiguousarray(configs, dtype=CONFIG)
+        _check(library().numi_cloth_hip_step(self._handle, configs.ctypes.data, len(configs), iterations,
+                                             strain_sweeps))
+
+    def synchronize(self) -> None:
+        _check(library().numi_cloth_hip_synchronize(self._handle))
+
+    def read(self) -> dict:
+        s = self.scene
+        out = {
+            "particles": np.zeros(len(s.particles), PARTICLE),
+            "distances": np.zeros(len(s.distances), DISTANCE),
+            "grips": np.zeros(len(s.grips), GRIP),
+            "knots": np.zeros(len(s.knots), KNOT),
+            "bends": np.zeros(len(s.bends), BEND),
+            "fruits": np.zeros(len(s.fruits), FRUIT),
+            "fruitPairs": np.zeros(len(s.fruitPairs), FRUIT_PAIR),
+            "yarnContacts": np.zeros(len(s.yarnContacts), YARN_CONTACT),
+            "selfStatus": np.zeros(1, SELF_STATUS),
+            "frictionStatus": np.zeros(1, FRICTION_STATUS),
+            "aerodynamicsStatus": np.zeros(1, AERO_STATUS),
+            "releaseStatus": np.zeros(1, RELEASE_STATUS),
+            "failure": np.zeros(1, np.uint32),
+        }
+        state = State()
+        for name, array in out.items():
+            setattr(state, name, _ptr(array))
+        _check(library().numi_cloth_hip_read(self._handle, C.byref(state)))
+        out["failure"] = int(out["failure"][0])
+        return out
+
+    def time_pass(self, kind: int, colour: int = 0, repeats: int = 50) -> PassTiming:
+        out = PassTiming()
+        _check(library().numi_cloth_hip_time_pass(self._handle, kind, colour, repeats, C.byref(out)))
+        return out
+
+    def diagnostics(self) -> Diagnostics:
+        out = Diagnostics()
+        _check(library().numi_cloth_hip_diagnostics(self._handle, C.byref(out)))
+        return out
+
+
+def state_hash(arrays: dict) -> int:
+    """FNV-1a 64 over the raw bytes of the state arrays, in a fixed order
+    (reference tools/cloth_bag_metal.mm:4821-4860 hashes its raw buffers)."""
+    lib = library()
+    value = 1469598103934665603
+    for name in ["particles", "distances", "grips", "knots", "bends", "fruits", "fruitPairs", "yarnContacts"]:
+        array = arrays.get(name)
+        if array is not None and len(array):
+            value = lib.numi_cloth_fnv1a64(array.ctypes.data, array.nbytes, value)
+    return int(value)

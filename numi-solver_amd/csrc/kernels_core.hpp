@@ -1,0 +1,527 @@
+#pragma once
+
+// Core XPBD kernels of the cloth substep on gfx950: configuration check,
+// predict, distance / crossing-knot / chord-bend / grip projections, ground
+// and static-sphere contact, the unilateral strain limiter and finalize with
+// cloth/ground Coulomb friction.
+//
+// Device layout (structure of arrays, one buffer per field, 16-byte records
+// where the field is a vector so every access is one coalesced dwordx4):
+//   pos[P]  float4  position xyz, inverse mass w
+//   prev[P] float4  substep-start position xyz, mass w
+//   vel[P]  float4  velocity xyz, predicted vz (during the substep) or
+//                   accumulated ground normal impulse (after finalize) w
+//   distances: ij[D] uint2, restCompliance[D] float2, lambda[D] float,
+//              maxLength[D] float = rest * (1 + strain limit)
+//   bends:     ends[B] uint2 (first, third), chordCompliance[B] float2,
+//              lambda[B] float
+//   knots:     quad[K] uint4, cosineCompliance[K] float2, lambda[K] float
+// Colour batches are contiguous index ranges, so one launch per colour reads
+// each record exactly once.
+//
+// Every kernel gates on ok[s], the per-substep result of validConfig
+// (reference src/metal/ClothBag.metal:16-55), which the reference
+// re-evaluates in every thread of every dispatch.
+
+#include "device_math.hpp"
+
+namespace numi_dev {
+
+constexpr uint32_t kBlock = 256;
+constexpr uint32_t kMaxWaves = 16;
+
+// --------------------------------------------------------------- predict --
+
+struct BeginArgs {
+    const NumiClothBagGPUConfig* cfg;
+    const uint32_t* ok;
+    float4* pos;
+    float4* prev;
+    float4* vel;
+    const float4* anchor;       // kinematic drive anchors, or null
+    float* distanceLambda;
+    float* bendLambda;
+    float* knotLambda;
+    NumiClothBagGPUGrip* grips;
+    uint32_t particles, distances, bends, knots, grips_;
+    const float* driveOffset;   // z offset of driven knots at the substep end
+    float dampingFactor;        // 1 / (1 + c dt), 1 when undamped
+    uint32_t advance;           // 1: also x += v dt (nothing runs in between)
+    uint32_t* failure;
+};
+
+// begin_substep (reference src/metal/ClothBag.metal:491-575) for knots and
+// the constraint multipliers, optionally fused with advance_positions
+// (:907-938) when no grip re-attachment or air load runs between them.
+__global__ void k_begin(BeginArgs a) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (*a.ok == 0u) return;
+    if (i < a.particles) {
+        float4 p = a.pos[i];
+        float4 v = a.vel[i];
+        float4 q = a.prev[i];
+        const V3 x = xyz(p);
+        const V3 u = xyz(v);
+        if (!finite3(x) || !finite3(u) || !isfinite(p.w) || p.w < 0.0f) {
+            recordFailure(a.failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
+        } else {
+            q = withXyz(q, x);
+            v.w = 0.0f;
+            const float dt = a.cfg->gravityAndTimestep.w;
+            if (p.w > 0.0f) {
+                // Damping acts on the carried velocity only, so a resting
+                // knot still receives the full gravity increment.
+                const V3 carried = a.dampingFactor != 1.0f ? u * a.dampingFactor : u;
+                const V3 nu = fma3(xyz(a.cfg->gravityAndTimestep), dt, carried);
+                v = withXyz(v, nu);
+                v.w = v.z;
+                if (a.advance) p = withXyz(p, fma3(nu, dt, x));
+            } else if (a.anchor != nullptr) {
+                const float4 anchor = a.anchor[i];
+                p = withXyz(p, v3(anchor.x, anchor.y, anchor.z + *a.driveOffset));
+            }
+            a.prev[i] = q;
+            a.vel[i] = v;
+            a.pos[i] = p;
+        }
+    }
+    if (i < a.distances) a.distanceLambda[i] = 0.0f;
+    if (i < a.bends) a.bendLambda[i] = 0.0f;
+    if (i < a.knots) a.knotLambda[i] = 0.0f;
+    if (i < a.grips_) {
+        a.grips[i].lambda.x = 0.0f;
+        a.grips[i].lambda.y = 0.0f;
+        a.grips[i].lambda.z = 0.0f;
+    }
+}
+
+// advance_positions (reference src/metal/ClothBag.metal:907-938), cloth part.
+__global__ void k_advance(const NumiClothBagGPUConfig* __restrict__ cfg, const uint32_t* __restrict__ ok,
+                          float4* __restrict__ pos, const float4* __restrict__ vel, uint32_t particles) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= particles || *ok == 0u) return;
+    float4 p = pos[i];
+    if (p.w > 0.0f) {
+        const float4 v = vel[i];
+        pos[i] = withXyz(p, fma3(xyz(v), cfg->gravityAndTimestep.w, xyz(p)));
+    }
+}
+
+// ------------------------------------------------------------ distances --
+
+// solve_distance (reference src/metal/ClothBag.metal:1148-1213) over one
+// colour batch [first, first + count).
+__global__ void k_distance(const NumiClothBagGPUConfig* __restrict__ cfg, const uint32_t* __restrict__ ok,
+                           uint32_t first, uint32_t count, float4* __restrict__ pos,
+                           const uint2* __restrict__ ij, const float2* __restrict__ restCompliance,
+                           float* __restrict__ lambda, uint32_t* __restrict__ failure) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count || *ok == 0u) return;
+    const uint32_t c = first + t;
+    const uint2 e = ij[c];
+    const float2 rc = restCompliance[c];
+    float lam = lambda[c];
+    float4 a = pos[e.x];
+    float4 b = pos[e.y];
+    const V3 d = xyz(b) - xyz(a);
+    const float len = length(d);
+    const float dt = cfg->gravityAndTimestep.w;
+    const float alpha = rc.y / (dt * dt);
+    const float denom = a.w + b.w + alpha;
+    if (!isfinite(len) || !isfinite(alpha) || !isfinite(denom) || !isfinite(rc.x) || !isfinite(lam)) {
+        recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
+        return;
+    }
+    if (!(len > 1.0e-12f) || !(denom > 0.0f)) return;
+    const float value = len - rc.x;
+    const float dl = (-value - alpha * lam) / denom;
+    const V3 corr = d * (dl / len);
+    lam += dl;
+    const V3 na = xyz(a) - corr * a.w;
+    const V3 nb = xyz(b) + corr * b.w;
+    if (!finite3(na) || !finite3(nb) || !isfinite(lam)) {
+        recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
+        return;
+    }
+    pos[e.x] = withXyz(a, na);
+    pos[e.y] = withXyz(b, nb);
+    lambda[c] = lam;
+}
+
+// limit_strain (reference src/metal/ClothBag.metal:1616-1673) over one colour.
+__global__ void k_strain(const uint32_t* __restrict__ ok, uint32_t first, uint32_t count,
+                         float4* __restrict__ pos, const uint2* __restrict__ ij,
+                         const float* __restrict__ maxLength, uint32_t* __restrict__ failure) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count || *ok == 0u) return;
+    const uint32_t c = first + t;
+    const uint2 e = ij[c];
+    const float limit = maxLength[c];
+    const float4 a = pos[e.x];
+    const float4 b = pos[e.y];
+    const V3 d = xyz(b) - xyz(a);
+    const float len = length(d);
+    if (!isfinite(len) || !isfinite(limit)) {
+        recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
+        return;
+    }
+    if (!(len > limit) || !(len > 1.0e-12f)) return;
+    const float denom = a.w + b.w;
+    if (!(denom > 0.0f) || !isfinite(denom)) return;
+    const float excess = len - limit;
+    const V3 dir = d / len;
+    pos[e.x] = withXyz(a, xyz(a) + dir * (a.w * excess / denom));
+    pos[e.y] = withXyz(b, xyz(b) - dir * (b.w * excess / denom));
+}
+
+// ----------------------------------------------------------------- knots --
+
+// solve_knot (reference src/metal/ClothBag.metal:1215-1313): crossing-angle
+// cosine constraint on the warp and weft chords through a knot.
+__global__ void k_knot(const NumiClothBagGPUConfig* __restrict__ cfg, const uint32_t* __restrict__ ok,
+                       uint32_t first, uint32_t count, float4* __restrict__ pos,
+                       const uint4* __restrict__ quad, const float2* __restrict__ cosineCompliance,
+                       float* __restrict__ lambda, uint32_t* __restrict__ failure) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count || *ok == 0u) return;
+    const uint32_t c = first + t;
+    const uint4 q = quad[c];
+    const float2 cc = cosineCompliance[c];
+    float lam = lambda[c];
+    float4 p0 = pos[q.x], p1 = pos[q.y], p2 = pos[q.z], p3 = pos[q.w];
+    const V3 warpV = xyz(p1) - xyz(p0);
+    const V3 weftV = xyz(p3) - xyz(p2);
+    const float warpL = length(warpV);
+    const float weftL = length(weftV);
+    if (!isfinite(warpL) || !isfinite(weftL) || !isfinite(cc.x) || !isfinite(cc.y) || !isfinite(lam)) {
+        recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
+        return;
+    }
+    if (!(warpL > 1.0e-12f) || !(weftL > 1.0e-12f)) return;
+    const V3 warp = warpV / warpL;
+    const V3 weft = weftV / weftL;
+    const float cosine = clampf(dot(warp, weft), -1.0f, 1.0f);
+    const float value = cosine - cc.x;
+    const V3 gWarp = (weft - warp * cosine) / warpL;
+    const V3 gWeft = (warp - weft * cosine) / weftL;
+    const V3 g[4] = {-gWarp, gWarp, -gWeft, gWeft};
+    const float w[4] = {p0.w, p1.w, p2.w, p3.w};
+    const float dt = cfg->gravityAndTimestep.w;
+    const float alpha = cc.y / (dt * dt);
+    float denom = alpha;
+    for (int k = 0; k < 4; ++k) denom += w[k] * dot(g[k], g[k]);
+    if (!(denom > 0.0f) || !isfinite(denom)) return;
+    const float dl = (-value - alpha * lam) / denom;
+    lam += dl;
+    const V3 n0 = xyz(p0) + g[0] * (w[0] * dl);
+    const V3 n1 = xyz(p1) + g[1] * (w[1] * dl);
+    const V3 n2 = xyz(p2) + g[2] * (w[2] * dl);
+    const V3 n3 = xyz(p3) + g[3] * (w[3] * dl);
+    if (!finite3(n0) || !finite3(n1) || !finite3(n2) || !finite3(n3) || !isfinite(lam)) {
+        recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
+        return;
+    }
+    pos[q.x] = withXyz(p0, n0);
+    pos[q.y] = withXyz(p1, n1);
+    pos[q.z] = withXyz(p2, n2);
+    pos[q.w] = withXyz(p3, n3);
+    lambda[c] = lam;
+}
+
+// ----------------------------------------------------------------- bends --
+
+// solve_bend (reference src/metal/ClothBag.metal:1315-1447): chord-length
+// XPBD between the outer knots with a ground active set (a grounded endpoint
+// that would be pushed down keeps its height) and a step fraction that stops
+// free endpoints at the ground.
+__global__ void k_bend(const NumiClothBagGPUConfig* __restrict__ cfg, const uint32_t* __restrict__ ok,
+                       uint32_t first, uint32_t count, float4* __restrict__ pos,
+                       const uint2* __restrict__ ends, const float2* __restrict__ chordCompliance,
+                       float* __restrict__ lambda, uint32_t* __restrict__ failure) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count || *ok == 0u) return;
+    const uint32_t c = first + t;
+    const uint2 e = ends[c];
+    const float2 cc = chordCompliance[c];
+    float lam = lambda[c];
+    float4 a = pos[e.x];
+    float4 b = pos[e.y];
+    const V3 d = xyz(b) - xyz(a);
+    const float chord = length(d);
+    if (!isfinite(chord) || !isfinite(cc.x) || !isfinite(cc.y) || !isfinite(lam)) {
+        recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
+        return;
+    }
+    if (!(chord > 1.0e-12f)) return;
+    const float value = chord - cc.x;
+    const V3 dir = d / chord;
+    const V3 g[2] = {-dir, dir};
+    const float w[2] = {a.w, b.w};
+    const float dt = cfg->gravityAndTimestep.w;
+    const float alpha = cc.y / (dt * dt);
+    float freeDenom = alpha;
+    for (int k = 0; k < 2; ++k) freeDenom += w[k] * dot(g[k], g[k]);
+    if (!(freeDenom >= 1.0e-16f) || !isfinite(freeDenom)) return;
+    const float numerator = -value - alpha * lam;
+    const float freeDl = numerator / freeDenom;
+    const bool ground = cfg->constraintCounts.z != 0u;
+    const float floorZ = cfg->clothMaterial.x;
+    const float z[2] = {a.z, b.z};
+    bool active[2];
+    float denom = alpha;
+    for (int k = 0; k < 2; ++k) {
+        active[k] = ground && z[k] <= floorZ + 1.0e-9f && g[k].z * freeDl < 0.0f;
+        denom += w[k] * (dot(g[k], g[k]) - (active[k] ? g[k].z * g[k].z : 0.0f));
+    }
+    if (!(denom >= 1.0e-16f) || !isfinite(denom)) return;
+    const float dl = numerator / denom;
+    float fraction = 1.0f;
+    if (ground) {
+        for (int k = 0; k < 2; ++k) {
+            if (active[k]) continue;
+            const float down = g[k].z * w[k] * dl;
+            if (down < 0.0f) fraction = fminf(fraction, fmaxf(0.0f, z[k] - floorZ) / -down);
+        }
+    }
+    const float applied = dl * fraction;
+    lam += applied;
+    V3 ca = g[0] * (w[0] * applied);
+    V3 cb = g[1] * (w[1] * applied);
+    if (active[0]) ca.z = 0.0f;
+    if (active[1]) cb.z = 0.0f;
+    V3 na = xyz(a) + ca;
+    V3 nb = xyz(b) + cb;
+    if (ground) {
+        na.z = fmaxf(na.z, floorZ);
+        nb.z = fmaxf(nb.z, floorZ);
+    }
+    if (!finite3(na) || !finite3(nb) || !isfinite(lam)) {
+        recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
+        return;
+    }
+    pos[e.x] = withXyz(a, na);
+    pos[e.y] = withXyz(b, nb);
+    lambda[c] = lam;
+}
+
+// ------------------------------------------------------------------ grip --
+
+// update_grip_attachment (reference src/metal/ClothBag.metal:577-673).
+__global__ void k_grip_attach(const NumiClothBagGPUConfig* __restrict__ cfg, const uint32_t* __restrict__ ok,
+                              const float4* __restrict__ pos, NumiClothBagGPUGrip* __restrict__ grips,
+                              uint32_t gripCount, const uint32_t* __restrict__ cuff, uint32_t particles,
+                              uint32_t* __restrict__ failure) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (*ok == 0u || i >= gripCount || i >= cfg->control.w || !(cfg->gripTargetAndActive.w > 0.0f)) return;
+    NumiClothBagGPUGrip grip = grips[i];
+    if (grip.particle.y == cfg->gripControl.x) return;
+    if (grip.particle.y == 0xffffffffu || grip.particle.y + 1u != cfg->gripControl.x) {
+        recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_GRIP_CAPTURE);
+        return;
+    }
+    const V3 target = xyz(cfg->gripTargetAndActive);
+    uint32_t particle = grip.particle.x;
+    if (cfg->gripControl.y == 1u) {
+        const uint32_t rings = cfg->gripControl.z;
+        uint32_t centre = 0u;
+        float nearest = INFINITY;
+        for (uint32_t ring = 0u; ring < rings; ++ring) {
+            const uint32_t candidate = cuff[rings + ring];
+            if (candidate >= particles) {
+                recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_RANGE);
+                return;
+            }
+            const V3 sep = xyz(pos[candidate]) - target;
+            const float d2 = dot(sep, sep);
+            if (d2 < nearest) {
+                nearest = d2;
+                centre = ring;
+            }
+        }
+        const uint32_t width = 2u * cfg->gripControl.w + 1u;
+        const uint32_t row = i / width;
+        const uint32_t slot = i % width;
+        const uint32_t ring = (centre + rings + slot - cfg->gripControl.w) % rings;
+        particle = cuff[row * rings + ring];
+        grip.particle.x = particle;
+        grip.particle.w = centre;
+    }
+    if (particle >= particles) {
+        recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_RANGE);
+        return;
+    }
+    const V3 x = xyz(pos[particle]);
+    const V3 sep = x - target;
+    const float capture = length(sep);
+    if (!finite3(x) || !isfinite(capture)) {
+        recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
+        return;
+    }
+    if (capture > cfg->gripMaterial.x) {
+        recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_GRIP_CAPTURE);
+        return;
+    }
+    const mr_float4 q = cfg->gripOrientation;
+    const mr_float4 inverse = {-q.x, -q.y, -q.z, q.w};
+    const V3 local = rotate(inverse, sep);
+    const V3 rebuilt = target + rotate(q, local);
+    const float error = length(x - rebuilt);
+    grip.targetOffsetAndCompliance.x = local.x;
+    grip.targetOffsetAndCompliance.y = local.y;
+    grip.targetOffsetAndCompliance.z = local.z;
+    grip.particle.y = cfg->gripControl.x;
+    grip.particle.z = __float_as_uint(fmaxf(__uint_as_float(grip.particle.z), capture));
+    grip.lambda.x = 0.0f;
+    grip.lambda.y = 0.0f;
+    grip.lambda.z = 0.0f;
+    grip.lambda.w = fmaxf(grip.lambda.w, error);
+    if (!finite4(grip.targetOffsetAndCompliance) || !finite4(grip.lambda)) {
+        recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
+        return;
+    }
+    grips[i] = grip;
+}
+
+// solve_grip (reference src/metal/ClothBag.metal:1449-1501): vector XPBD
+// towards the handle-frame target g + q o.
+__global__ void k_grip(const NumiClothBagGPUConfig* __restrict__ cfg, const uint32_t* __restrict__ ok,
+                       float4* __restrict__ pos, NumiClothBagGPUGrip* __restrict__ grips, uint32_t gripCount,
+                       uint32_t particles, uint32_t* __restrict__ failure) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (*ok == 0u || i >= gripCount || i >= cfg->control.w || !(cfg->gripTargetAndActive.w > 0.0f)) return;
+    NumiClothBagGPUGrip grip = grips[i];
+    if (grip.particle.y != cfg->gripControl.x) return;
+    const uint32_t index = grip.particle.x;
+    if (index >= particles) {
+        recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_RANGE);
+        return;
+    }
+    float4 p = pos[index];
+    const float dt = cfg->gravityAndTimestep.w;
+    const float alpha = grip.targetOffsetAndCompliance.w / (dt * dt);
+    const float denom = p.w + alpha;
+    if (!isfinite(alpha) || !isfinite(denom) || !finite4(grip.targetOffsetAndCompliance) ||
+        !finite4(grip.lambda)) {
+        recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
+        return;
+    }
+    if (!(denom > 0.0f)) return;
+    const V3 target = xyz(cfg->gripTargetAndActive) + rotate(cfg->gripOrientation, xyz(grip.targetOffsetAndCompliance));
+    const V3 value = xyz(p) - target;
+    const V3 lam = xyz(grip.lambda);
+    const V3 dl = (-value - lam * alpha) / denom;
+    const V3 nl = lam + dl;
+    const V3 np = xyz(p) + dl * p.w;
+    if (!finite3(np) || !finite3(nl)) {
+        recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
+        return;
+    }
+    grip.lambda.x = nl.x;
+    grip.lambda.y = nl.y;
+    grip.lambda.z = nl.z;
+    pos[index] = withXyz(p, np);
+    grips[i] = grip;
+}
+
+// ------------------------------------------------------ ground + spheres --
+
+// Cloth part of solve_ground (reference src/metal/ClothBag.metal:1575-1614)
+// followed by static sphere contact, which the reference does not have
+// (its spheres must be dynamic fruits): a dynamic knot inside a sphere grown
+// by the yarn radius is moved radially onto its surface.
+__global__ void k_collide(const NumiClothBagGPUConfig* __restrict__ cfg, const uint32_t* __restrict__ ok,
+                          float4* __restrict__ pos, uint32_t particles, const float4* __restrict__ spheres,
+                          uint32_t sphereCount, uint32_t* __restrict__ failure) {
+    extern __shared__ float4 sphereCache[];
+    for (uint32_t k = threadIdx.x; k < sphereCount; k += blockDim.x) sphereCache[k] = spheres[k];
+    __syncthreads();
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= particles || *ok == 0u) return;
+    float4 p = pos[i];
+    if (!isfinite(p.z)) {
+        recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
+        return;
+    }
+    const float r = cfg->clothMaterial.x;
+    bool changed = false;
+    if (cfg->constraintCounts.z != 0u && !(p.z >= r)) {
+        p.z = fmaxf(p.z, r);
+        changed = true;
+    }
+    if (p.w > 0.0f) {
+        for (uint32_t k = 0; k < sphereCount; ++k) {
+            const float4 s = sphereCache[k];
+            const V3 d = xyz(p) - xyz(s);
+            const float dist = length(d);
+            const float reach = s.w + r;
+            if (dist < reach && dist > 1.0e-12f) {
+                p = withXyz(p, xyz(s) + d * (reach / dist));
+                changed = true;
+            }
+        }
+    }
+    if (changed) pos[i] = p;
+}
+
+// ------------------------------------------------------------- finalize --
+
+// finalize_substep (reference src/metal/ClothBag.metal:2890-2921) fused with
+// apply_cloth_ground_friction (:2976-3022); nothing runs between them on the
+// cloth knots. Friction contact counts and the maximum cone ratio are reduced
+// per wavefront before one integer atomic per wave.
+__global__ void k_finalize(const NumiClothBagGPUConfig* __restrict__ cfg, const uint32_t* __restrict__ ok,
+                           float4* __restrict__ pos, const float4* __restrict__ prev, float4* __restrict__ vel,
+                           uint32_t particles, NumiClothBagGPUFrictionStatus* __restrict__ friction,
+                           uint32_t* __restrict__ failure) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (*ok == 0u) return;
+    bool contact = false;
+    float ratio = 0.0f;
+    if (i < particles) {
+        const float4 p = pos[i];
+        float4 v = vel[i];
+        const float4 q = prev[i];
+        const float inverseDt = 1.0f / cfg->gravityAndTimestep.w;
+        const float predicted = v.w;
+        V3 u = (xyz(p) - xyz(q)) * inverseDt;
+        float jn = 0.0f;
+        const bool ground = cfg->constraintCounts.z != 0u;
+        if (ground && p.w > 0.0f && p.z <= cfg->clothMaterial.x + 1.0e-6f) {
+            jn = fmaxf(0.0f, (u.z - predicted) / p.w);
+        }
+        bool good = finite3(u);
+        if (good && ground) {
+            const float mu = cfg->clothMaterial.z;
+            if (!(mu >= 0.0f) || !isfinite(mu)) {
+                good = false;
+            } else if (jn > 0.0f && p.w > 0.0f) {
+                const V3 tangent = v3(u.x, u.y, 0.0f);
+                const float slip = length(tangent);
+                if (slip > 1.0e-10f) {
+                    const float limit = mu * jn;
+                    const float jt = fminf(slip / p.w, limit);
+                    if (jt > 0.0f) {
+                        u = u - tangent * (jt * p.w / slip);
+                        contact = true;
+                        ratio = limit > 0.0f ? jt / limit : 0.0f;
+                        good = finite3(u) && isfinite(jn);
+                    }
+                }
+            }
+        }
+        if (!good) {
+            recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
+        } else {
+            vel[i] = make_float4(u.x, u.y, u.z, jn);
+        }
+    }
+    const unsigned long long ballot = __ballot(contact);
+    float waveMax = ratio;
+    for (int offset = 32; offset > 0; offset >>= 1) waveMax = fmaxf(waveMax, __shfl_xor(waveMax, offset));
+    if ((threadIdx.x & 63u) == 0u && ballot != 0ull) {
+        atomicAdd(&friction->counters.z, static_cast<uint32_t>(__popcll(ballot)));
+        atomicMax(&friction->metrics.x, __float_as_uint(waveMax));
+    }
+}
+
+}  // namespace numi_dev

@@ -1,0 +1,779 @@
+// Host runtime and C ABI of the MI355X cloth substep loop.
+//
+// create(): host validation of the ABI v13 arrays, AoS -> SoA conversion,
+//           device-resident buffers (reference runGPU allocates and copies
+//           per call, tools/cloth_bag_metal.mm:4019-4177; here state stays
+//           on the device between calls).
+// step():   uploads the per-substep configs, resets the device substep
+//           cursor and replays one captured hipGraph per substep. The graph
+//           holds the reference's per-substep dispatch order
+//           (tools/cloth_bag_metal.mm:4211-4709) for the families the scene
+//           has; empty families are left out of the graph.
+// read():   SoA -> AoS readback (reference :4720-4757).
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "kernels_core.hpp"
+#include "kernels_diag.hpp"
+#include "numi_cloth_hip.h"
+
+using namespace numi_hip;
+using namespace numi_dev;
+
+namespace {
+
+#define HIP_OK(expr)                                                                          \
+    do {                                                                                      \
+        const hipError_t status_ = (expr);                                                    \
+        if (status_ != hipSuccess) {                                                          \
+            fail(NUMI_CLOTH_HIP_ERROR_DEVICE, std::string(#expr " failed: ") +                \
+                                                  hipGetErrorString(status_));                \
+        }                                                                                     \
+    } while (0)
+
+// Fixed number of reconciliation / final / certificate passes of the
+// reference schedule (tools/cloth_bag_metal.mm:47-49).
+constexpr uint32_t kReconciliationPasses = 8;
+constexpr uint32_t kFinalContactPasses = 2;
+constexpr uint32_t kCertificatePasses = 8;
+
+template <typename T>
+struct DeviceArray {
+    T* ptr = nullptr;
+    std::size_t count = 0;
+    void allocate(std::size_t n) {
+        release();
+        count = n;
+        if (n == 0) return;
+        HIP_OK(hipMalloc(&ptr, n * sizeof(T)));
+    }
+    void upload(const T* host, std::size_t n, hipStream_t stream) {
+        if (n == 0) return;
+        HIP_OK(hipMemcpyAsync(ptr, host, n * sizeof(T), hipMemcpyHostToDevice, stream));
+    }
+    void download(T* host, std::size_t n, hipStream_t stream) const {
+        if (n == 0) return;
+        HIP_OK(hipMemcpyAsync(host, ptr, n * sizeof(T), hipMemcpyDeviceToHost, stream));
+    }
+    void release() {
+        if (ptr != nullptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        count = 0;
+    }
+    ~DeviceArray() { release(); }
+};
+
+struct Range {
+    uint32_t first;
+    uint32_t count;
+};
+
+uint32_t blocksFor(uint32_t n) { return std::max(1u, (n + kBlock - 1) / kBlock); }
+
+}  // namespace
+
+struct NumiClothHipSolver {
+    hipStream_t stream = nullptr;
+    int device = 0;
+
+    // Counts and topology (host copies keep the static record fields).
+    NumiClothBagGPUConfig initialConfig{};
+    uint32_t P = 0, D = 0, G = 0, K = 0, B = 0, S = 0;
+    std::vector<NumiClothBagGPUDistance> hostDistances;
+    std::vector<NumiClothBagGPUBend> hostBends;
+    std::vector<NumiClothBagGPUKnot> hostKnots;
+    std::vector<Range> distanceColours, bendColours, knotColours;
+    std::vector<uint32_t> cuff;
+    NumiClothHipSceneExtension extension{};
+    bool driven = false;
+
+    // Device state.
+    DeviceArray<float4> pos, prev, vel, anchor;
+    DeviceArray<uint2> dIJ, bEnds;
+    DeviceArray<float2> dRestCompliance, bChordCompliance, kCosineCompliance;
+    DeviceArray<float> dLambda, dMaxLength, bLambda, kLambda;
+    DeviceArray<uint4> kQuad;
+    DeviceArray<NumiClothBagGPUGrip> grips;
+    DeviceArray<uint32_t> dCuff;
+    DeviceArray<float4> spheres;
+    DeviceArray<NumiClothBagGPUConfig> configs;
+    DeviceArray<float> drive;
+    DeviceArray<SubstepSlot> slot;
+    DeviceArray<uint32_t> failure;
+    DeviceArray<NumiClothBagGPUSelfStatus> selfStatus;
+    DeviceArray<NumiClothBagGPUFrictionStatus> frictionStatus;
+    DeviceArray<NumiClothBagGPUAerodynamicsStatus> aeroStatus;
+    DeviceArray<NumiClothBagGPUReleaseStatus> releaseStatus;
+    DeviceArray<float4> diagPartials, diagResult;
+
+    // Pinned staging for per-step uploads.
+    NumiClothBagGPUConfig* stagingConfigs = nullptr;
+    float* stagingDrive = nullptr;
+    uint32_t stagingCapacity = 0;
+    double time = 0.0;  // simulated time at the end of the last substep
+
+    // One captured substep; rebuilt when (iterations, strainSweeps) change.
+    hipGraphExec_t graph = nullptr;
+    uint32_t graphIterations = ~0u;
+    uint32_t graphSweeps = ~0u;
+
+    ~NumiClothHipSolver() {
+        if (graph != nullptr) (void)hipGraphExecDestroy(graph);
+        if (stagingConfigs != nullptr) (void)hipHostFree(stagingConfigs);
+        if (stagingDrive != nullptr) (void)hipHostFree(stagingDrive);
+        if (stream != nullptr) (void)hipStreamDestroy(stream);
+    }
+
+    const NumiClothBagGPUConfig* cfg() const { return &slot.ptr->config; }
+    const uint32_t* ok() const { return &slot.ptr->ok; }
+
+    // ---------------------------------------------------------- launches --
+
+    void launchBegin(bool fuseAdvance, float dampingFactor) {
+        BeginArgs a{};
+        a.cfg = cfg();
+        a.ok = ok();
+        a.pos = pos.ptr;
+        a.prev = prev.ptr;
+        a.vel = vel.ptr;
+        a.anchor = driven ? anchor.ptr : nullptr;
+        a.distanceLambda = dLambda.ptr;
+        a.bendLambda = bLambda.ptr;
+        a.knotLambda = kLambda.ptr;
+        a.grips = grips.ptr;
+        a.particles = P;
+        a.distances = D;
+        a.bends = B;
+        a.knots = K;
+        a.grips_ = G;
+        a.driveOffset = &slot.ptr->driveOffset;
+        a.dampingFactor = dampingFactor;
+        a.advance = fuseAdvance ? 1u : 0u;
+        a.failure = failure.ptr;
+        const uint32_t n = std::max({P, D, B, K, G});
+        k_begin<<<blocksFor(n), kBlock, 0, stream>>>(a);
+    }
+
+    void launchDistances() {
+        for (const Range& r : distanceColours) {
+            k_distance<<<blocksFor(r.count), kBlock, 0, stream>>>(cfg(), ok(), r.first, r.count, pos.ptr, dIJ.ptr,
+                                                                  dRestCompliance.ptr, dLambda.ptr, failure.ptr);
+        }
+    }
+
+    void launchKnots() {
+        for (const Range& r : knotColours) {
+            k_knot<<<blocksFor(r.count), kBlock, 0, stream>>>(cfg(), ok(), r.first, r.count, pos.ptr, kQuad.ptr,
+                                                              kCosineCompliance.ptr, kLambda.ptr, failure.ptr);
+        }
+    }
+
+    void launchBends() {
+        for (const Range& r : bendColours) {
+            k_bend<<<blocksFor(r.count), kBlock, 0, stream>>>(cfg(), ok(), r.first, r.count, pos.ptr, bEnds.ptr,
+                                                              bChordCompliance.ptr, bLambda.ptr, failure.ptr);
+        }
+    }
+
+    void launchStrain(uint32_t sweeps) {
+        for (uint32_t sweep = 0; sweep < sweeps; ++sweep) {
+            for (const Range& r : distanceColours) {
+                k_strain<<<blocksFor(r.count), kBlock, 0, stream>>>(ok(), r.first, r.count, pos.ptr, dIJ.ptr,
+                                                                    dMaxLength.ptr, failure.ptr);
+            }
+        }
+    }
+
+    bool groundOrColliders() const {
+        return initialConfig.constraintCounts.z != 0u || extension.staticSphereCount != 0u;
+    }
+
+    void launchCollide() {
+        if (!groundOrColliders()) return;
+        const uint32_t n = extension.staticSphereCount;
+        k_collide<<<blocksFor(P), kBlock, n * sizeof(float4), stream>>>(cfg(), ok(), pos.ptr, P, spheres.ptr, n,
+                                                                        failure.ptr);
+    }
+
+    void launchGrips() {
+        if (G == 0) return;
+        k_grip<<<blocksFor(G), kBlock, 0, stream>>>(cfg(), ok(), pos.ptr, grips.ptr, G, P, failure.ptr);
+    }
+
+    // The substep in the reference's dispatch order
+    // (tools/cloth_bag_metal.mm:4211-4709), families without records omitted.
+    void recordSubstep(uint32_t iterations, uint32_t sweeps) {
+        k_prepare<<<1, 64, 0, stream>>>(configs.ptr, drive.ptr, slot.ptr, failure.ptr);
+        const float damping = extension.damping;
+        // The damping factor depends on dt; dt is constant over a scene's
+        // trajectories (the reference never changes it), so it is folded here.
+        const float dt = initialConfig.gravityAndTimestep.w;
+        const float factor = damping > 0.0f ? 1.0f / (1.0f + damping * dt) : 1.0f;
+        const bool attach = G != 0;
+        launchBegin(!attach, factor);
+        if (attach) {
+            k_grip_attach<<<blocksFor(G), kBlock, 0, stream>>>(cfg(), ok(), pos.ptr, grips.ptr, G, dCuff.ptr, P,
+                                                               failure.ptr);
+            k_advance<<<blocksFor(P), kBlock, 0, stream>>>(cfg(), ok(), pos.ptr, vel.ptr, P);
+        }
+        for (uint32_t it = 0; it < iterations; ++it) {
+            launchDistances();
+            launchKnots();
+            if (it % 2 == 0) launchBends();
+            launchGrips();
+            launchCollide();
+        }
+        launchStrain(sweeps);
+        if (sweeps != 0) {
+            for (uint32_t pass = 0; pass < kReconciliationPasses; ++pass) {
+                launchStrain(sweeps);
+                launchCollide();
+            }
+            launchStrain(sweeps);
+            for (uint32_t pass = 0; pass < kFinalContactPasses; ++pass) launchCollide();
+            for (uint32_t certificate = 0; certificate < kCertificatePasses; ++certificate) {
+                launchStrain(sweeps);
+                for (uint32_t pass = 0; pass < kFinalContactPasses; ++pass) launchCollide();
+            }
+        }
+        k_finalize<<<blocksFor(P), kBlock, 0, stream>>>(cfg(), ok(), pos.ptr, prev.ptr, vel.ptr, P,
+                                                        frictionStatus.ptr, failure.ptr);
+    }
+
+    void buildGraph(uint32_t iterations, uint32_t sweeps) {
+        if (graph != nullptr && graphIterations == iterations && graphSweeps == sweeps) return;
+        if (graph != nullptr) {
+            HIP_OK(hipGraphExecDestroy(graph));
+            graph = nullptr;
+        }
+        HIP_OK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        recordSubstep(iterations, sweeps);
+        hipGraph_t captured = nullptr;
+        const hipError_t launchStatus = hipGetLastError();
+        HIP_OK(hipStreamEndCapture(stream, &captured));
+        if (launchStatus != hipSuccess) {
+            (void)hipGraphDestroy(captured);
+            fail(NUMI_CLOTH_HIP_ERROR_DEVICE, std::string("substep capture failed: ") + hipGetErrorString(launchStatus));
+        }
+        const hipError_t instantiate = hipGraphInstantiate(&graph, captured, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(captured);
+        if (instantiate != hipSuccess) {
+            graph = nullptr;
+            fail(NUMI_CLOTH_HIP_ERROR_DEVICE, std::string("hipGraphInstantiate failed: ") + hipGetErrorString(instantiate));
+        }
+        graphIterations = iterations;
+        graphSweeps = sweeps;
+    }
+
+    void ensureStaging(uint32_t substeps) {
+        if (substeps <= stagingCapacity) return;
+        const uint32_t capacity = std::max(substeps, 64u);
+        if (stagingConfigs != nullptr) HIP_OK(hipHostFree(stagingConfigs));
+        if (stagingDrive != nullptr) HIP_OK(hipHostFree(stagingDrive));
+        stagingConfigs = nullptr;
+        stagingDrive = nullptr;
+        HIP_OK(hipStreamSynchronize(stream));
+        HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&stagingConfigs), capacity * sizeof(NumiClothBagGPUConfig)));
+        HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&stagingDrive), capacity * sizeof(float)));
+        configs.allocate(capacity);
+        drive.allocate(capacity);
+        stagingCapacity = capacity;
+    }
+};
+
+namespace {
+
+void requireEqual(uint32_t a, uint32_t b, const char* what) {
+    if (a != b) fail(NUMI_CLOTH_HIP_ERROR_TOPOLOGY, std::string("scene count mismatch: ") + what);
+}
+
+// Colour batches must tile their array in order, each record carrying the
+// batch colour (the reference checks this per dispatch and raises
+// FAILURE_BATCH, src/metal/ClothBag.metal:1172-1175; topology is immutable
+// after create, so it is checked once here).
+template <typename Colour>
+std::vector<Range> checkBatches(const NumiClothBagGPUBatch* batches, uint32_t batchCount, uint32_t records,
+                                Colour colourOf, const char* what) {
+    std::vector<Range> ranges;
+    if (batchCount != 0 && batches == nullptr) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, std::string("null batches: ") + what);
+    for (uint32_t b = 0; b < batchCount; ++b) {
+        const mr_uint4 c = batches[b].control;
+        if (static_cast<uint64_t>(c.x) + c.y > records) {
+            fail(NUMI_CLOTH_HIP_ERROR_TOPOLOGY, std::string("batch out of range (FAILURE_RANGE): ") + what);
+        }
+        for (uint32_t k = c.x; k < c.x + c.y; ++k) {
+            if (colourOf(k) != c.z) {
+                fail(NUMI_CLOTH_HIP_ERROR_TOPOLOGY, std::string("record colour differs from its batch (FAILURE_BATCH): ") + what);
+            }
+        }
+        if (c.y != 0) ranges.push_back({c.x, c.y});
+    }
+    return ranges;
+}
+
+// Within a colour no two records may share a particle; this is what makes a
+// colour launch race-free (reference verifyColoring,
+// tools/cloth_bag_metal.mm:1705-1819).
+template <typename Participants>
+void checkColouring(const std::vector<Range>& ranges, uint32_t particles, Participants participantsOf,
+                    const char* what) {
+    std::vector<uint32_t> stamp(particles, ~0u);
+    for (uint32_t r = 0; r < ranges.size(); ++r) {
+        for (uint32_t k = ranges[r].first; k < ranges[r].first + ranges[r].count; ++k) {
+            uint32_t ids[4];
+            const uint32_t n = participantsOf(k, ids);
+            for (uint32_t m = 0; m < n; ++m) {
+                if (ids[m] >= particles) fail(NUMI_CLOTH_HIP_ERROR_TOPOLOGY, std::string("particle index out of range (FAILURE_RANGE): ") + what);
+                if (stamp[ids[m]] == r) fail(NUMI_CLOTH_HIP_ERROR_TOPOLOGY, std::string("colour batch shares a particle: ") + what);
+                stamp[ids[m]] = r;
+            }
+        }
+    }
+}
+
+void createSolver(const NumiClothHipSceneDesc& scene, NumiClothHipSolver& s) {
+    if (scene.config == nullptr) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "scene config is null");
+    const NumiClothBagGPUConfig& c = *scene.config;
+    if (c.control.x != NUMI_CLOTH_BAG_GPU_ABI_VERSION) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "config ABI version is not 13");
+    requireEqual(c.control.y, scene.particleCount, "particles");
+    requireEqual(c.control.z, scene.distanceCount, "distances");
+    requireEqual(c.control.w, scene.gripCount, "grips");
+    requireEqual(c.constraintCounts.x, scene.knotCount, "knots");
+    requireEqual(c.constraintCounts.y, scene.bendCount, "bends");
+    requireEqual(c.constraintCounts.w, scene.fruitCount, "fruits");
+    requireEqual(c.contactCounts.x, scene.fruitPairCount, "fruit pairs");
+    requireEqual(c.contactCounts.y, scene.yarnContactCount, "yarn contacts");
+    if (scene.fruitCount != 0 || scene.fruitPairCount != 0 || scene.yarnContactCount != 0) {
+        fail(NUMI_CLOTH_HIP_ERROR_UNSUPPORTED, "fruit and fruit/yarn families are not built into this library");
+    }
+    if (scene.extension.selfContact != 0) {
+        fail(NUMI_CLOTH_HIP_ERROR_UNSUPPORTED, "hash self-contact is not built into this library");
+    }
+    s.initialConfig = c;
+    s.P = scene.particleCount;
+    s.D = scene.distanceCount;
+    s.G = scene.gripCount;
+    s.K = scene.knotCount;
+    s.B = scene.bendCount;
+    s.extension = scene.extension;
+    s.driven = scene.extension.driveAmplitude != 0.0f;
+    if (s.P == 0 || scene.particles == nullptr) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "scene has no particles");
+    if (s.D != 0 && scene.distances == nullptr) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "null distances");
+    if (s.B != 0 && scene.bends == nullptr) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "null bends");
+    if (s.K != 0 && scene.knots == nullptr) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "null knots");
+    if (s.G != 0 && scene.grips == nullptr) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "null grips");
+    if (scene.extension.staticSphereCount > 4096) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "at most 4096 static spheres");
+
+    s.hostDistances.assign(scene.distances, scene.distances + s.D);
+    s.hostBends.assign(scene.bends, scene.bends + s.B);
+    s.hostKnots.assign(scene.knots, scene.knots + s.K);
+    s.distanceColours = checkBatches(scene.distanceBatches, scene.distanceBatchCount, s.D,
+                                     [&](uint32_t k) { return s.hostDistances[k].particlesAndColor.z; }, "distance");
+    s.bendColours = checkBatches(scene.bendBatches, scene.bendBatchCount, s.B,
+                                 [&](uint32_t k) { return s.hostBends[k].particlesAndColor.w; }, "bend");
+    s.knotColours = checkBatches(scene.knotBatches, scene.knotBatchCount, s.K,
+                                 [&](uint32_t k) { return s.hostKnots[k].control.x; }, "knot");
+    checkColouring(s.distanceColours, s.P, [&](uint32_t k, uint32_t* ids) {
+        ids[0] = s.hostDistances[k].particlesAndColor.x;
+        ids[1] = s.hostDistances[k].particlesAndColor.y;
+        return 2u;
+    }, "distance");
+    checkColouring(s.bendColours, s.P, [&](uint32_t k, uint32_t* ids) {
+        ids[0] = s.hostBends[k].particlesAndColor.x;
+        ids[1] = s.hostBends[k].particlesAndColor.z;
+        return 2u;
+    }, "bend");
+    checkColouring(s.knotColours, s.P, [&](uint32_t k, uint32_t* ids) {
+        const mr_uint4 q = s.hostKnots[k].particles;
+        ids[0] = q.x;
+        ids[1] = q.y;
+        ids[2] = q.z;
+        ids[3] = q.w;
+        return 4u;
+    }, "knot");
+    for (const NumiClothBagGPUBend& b : s.hostBends) {
+        if (b.particlesAndColor.y >= s.P) fail(NUMI_CLOTH_HIP_ERROR_TOPOLOGY, "bend middle index out of range");
+    }
+    if (scene.cuffCount != 0) {
+        if (scene.cuffParticles == nullptr) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "null cuff");
+        s.cuff.assign(scene.cuffParticles, scene.cuffParticles + scene.cuffCount);
+        for (uint32_t p : s.cuff) {
+            if (p >= s.P) fail(NUMI_CLOTH_HIP_ERROR_TOPOLOGY, "cuff topology is out of range");
+        }
+    }
+
+    HIP_OK(hipGetDevice(&s.device));
+    HIP_OK(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
+    hipStream_t st = s.stream;
+
+    std::vector<float4> hp(s.P), hq(s.P), hv(s.P);
+    for (uint32_t i = 0; i < s.P; ++i) {
+        const NumiClothBagGPUParticle& p = scene.particles[i];
+        hp[i] = make_float4(p.positionAndInverseMass.x, p.positionAndInverseMass.y, p.positionAndInverseMass.z,
+                            p.positionAndInverseMass.w);
+        hq[i] = make_float4(p.previousAndMass.x, p.previousAndMass.y, p.previousAndMass.z, p.previousAndMass.w);
+        hv[i] = make_float4(p.velocity.x, p.velocity.y, p.velocity.z, p.velocity.w);
+    }
+    s.pos.allocate(s.P);
+    s.prev.allocate(s.P);
+    s.vel.allocate(s.P);
+    s.pos.upload(hp.data(), s.P, st);
+    s.prev.upload(hq.data(), s.P, st);
+    s.vel.upload(hv.data(), s.P, st);
+    if (s.driven) {
+        s.anchor.allocate(s.P);
+        s.anchor.upload(hp.data(), s.P, st);
+    }
+
+    std::vector<uint2> ij(s.D);
+    std::vector<float2> rc(s.D);
+    std::vector<float> lam(s.D), maxLen(s.D);
+    for (uint32_t k = 0; k < s.D; ++k) {
+        const NumiClothBagGPUDistance& d = s.hostDistances[k];
+        ij[k] = make_uint2(d.particlesAndColor.x, d.particlesAndColor.y);
+        rc[k] = make_float2(d.material.x, d.material.y);
+        lam[k] = d.material.z;
+        maxLen[k] = d.material.x * (1.0f + d.material.w);
+    }
+    s.dIJ.allocate(s.D);
+    s.dRestCompliance.allocate(s.D);
+    s.dLambda.allocate(s.D);
+    s.dMaxLength.allocate(s.D);
+    s.dIJ.upload(ij.data(), s.D, st);
+    s.dRestCompliance.upload(rc.data(), s.D, st);
+    s.dLambda.upload(lam.data(), s.D, st);
+    s.dMaxLength.upload(maxLen.data(), s.D, st);
+
+    std::vector<uint2> ends(s.B);
+    std::vector<float2> cc(s.B);
+    std::vector<float> bl(s.B);
+    for (uint32_t k = 0; k < s.B; ++k) {
+        const NumiClothBagGPUBend& b = s.hostBends[k];
+        ends[k] = make_uint2(b.particlesAndColor.x, b.particlesAndColor.z);
+        cc[k] = make_float2(b.material.x, b.material.z);
+        bl[k] = b.material.w;
+    }
+    s.bEnds.allocate(s.B);
+    s.bChordCompliance.allocate(s.B);
+    s.bLambda.allocate(s.B);
+    s.bEnds.upload(ends.data(), s.B, st);
+    s.bChordCompliance.upload(cc.data(), s.B, st);
+    s.bLambda.upload(bl.data(), s.B, st);
+
+    std::vector<uint4> quad(s.K);
+    std::vector<float2> kc(s.K);
+    std::vector<float> kl(s.K);
+    for (uint32_t k = 0; k < s.K; ++k) {
+        const NumiClothBagGPUKnot& n = s.hostKnots[k];
+        quad[k] = make_uint4(n.particles.x, n.particles.y, n.particles.z, n.particles.w);
+        kc[k] = make_float2(n.material.x, n.material.y);
+        kl[k] = n.material.z;
+    }
+    s.kQuad.allocate(s.K);
+    s.kCosineCompliance.allocate(s.K);
+    s.kLambda.allocate(s.K);
+    s.kQuad.upload(quad.data(), s.K, st);
+    s.kCosineCompliance.upload(kc.data(), s.K, st);
+    s.kLambda.upload(kl.data(), s.K, st);
+
+    s.grips.allocate(s.G);
+    s.grips.upload(scene.grips, s.G, st);
+    s.dCuff.allocate(s.cuff.size());
+    s.dCuff.upload(s.cuff.data(), s.cuff.size(), st);
+    const uint32_t nSpheres = scene.extension.staticSphereCount;
+    s.spheres.allocate(nSpheres);
+    if (nSpheres != 0) {
+        std::vector<float4> sp(nSpheres);
+        for (uint32_t k = 0; k < nSpheres; ++k) {
+            const mr_float4& v = scene.extension.staticSpheres[k];
+            sp[k] = make_float4(v.x, v.y, v.z, v.w);
+        }
+        s.spheres.upload(sp.data(), nSpheres, st);
+        HIP_OK(hipStreamSynchronize(st));
+    }
+    s.extension.staticSpheres = nullptr;  // never dereference caller memory later
+
+    s.slot.allocate(1);
+    s.failure.allocate(1);
+    s.selfStatus.allocate(1);
+    s.frictionStatus.allocate(1);
+    s.aeroStatus.allocate(1);
+    s.releaseStatus.allocate(1);
+    HIP_OK(hipMemsetAsync(s.slot.ptr, 0, sizeof(SubstepSlot), st));
+    HIP_OK(hipMemsetAsync(s.failure.ptr, 0, sizeof(uint32_t), st));
+    HIP_OK(hipMemsetAsync(s.selfStatus.ptr, 0, sizeof(NumiClothBagGPUSelfStatus), st));
+    HIP_OK(hipMemsetAsync(s.frictionStatus.ptr, 0, sizeof(NumiClothBagGPUFrictionStatus), st));
+    HIP_OK(hipMemsetAsync(s.aeroStatus.ptr, 0, sizeof(NumiClothBagGPUAerodynamicsStatus), st));
+    HIP_OK(hipMemsetAsync(s.releaseStatus.ptr, 0, sizeof(NumiClothBagGPUReleaseStatus), st));
+    s.diagPartials.allocate(1024);
+    s.diagResult.allocate(1);
+    s.ensureStaging(64);
+    HIP_OK(hipStreamSynchronize(st));
+}
+
+// The reference lets a trajectory change only grip target/orientation/
+// generation/selection mode and materials (tools/cloth_bag_metal.mm:4044-4090).
+void checkTrajectoryConfig(const NumiClothHipSolver& s, const NumiClothBagGPUConfig& c) {
+    const NumiClothBagGPUConfig& i = s.initialConfig;
+    if (c.control.x != NUMI_CLOTH_BAG_GPU_ABI_VERSION || c.control.y != i.control.y || c.control.z != i.control.z ||
+        c.control.w != i.control.w || c.gripControl.z != i.gripControl.z || c.gripControl.w != i.gripControl.w ||
+        std::memcmp(&c.constraintCounts, &i.constraintCounts, sizeof(mr_uint4)) != 0 ||
+        std::memcmp(&c.contactCounts, &i.contactCounts, sizeof(mr_uint4)) != 0 ||
+        std::memcmp(&c.mouthControl, &i.mouthControl, sizeof(mr_uint4)) != 0) {
+        fail(NUMI_CLOTH_HIP_ERROR_TOPOLOGY, "cloth trajectory changes fixed topology");
+    }
+    if (c.gripControl.y == 1u && s.cuff.size() != 2ull * c.gripControl.z) {
+        fail(NUMI_CLOTH_HIP_ERROR_TOPOLOGY, "cloth trajectory has incomplete cuff topology");
+    }
+    if (c.gravityAndTimestep.w != i.gravityAndTimestep.w && s.extension.damping > 0.0f) {
+        fail(NUMI_CLOTH_HIP_ERROR_TOPOLOGY, "a damped scene keeps its timestep");
+    }
+    const float density = c.airVelocityAndDensity.w;
+    if (density > 0.0f && (c.aerodynamicCoefficients.x > 0.0f || c.aerodynamicCoefficients.y > 0.0f)) {
+        fail(NUMI_CLOTH_HIP_ERROR_UNSUPPORTED, "yarn air loads are not built into this library");
+    }
+}
+
+void stepSolver(NumiClothHipSolver& s, const NumiClothBagGPUConfig* configs, uint32_t substeps,
+                uint32_t iterations, uint32_t sweeps) {
+    if (substeps == 0) return;
+    if (configs == nullptr) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "null configs");
+    for (uint32_t k = 0; k < substeps; ++k) checkTrajectoryConfig(s, configs[k]);
+    s.ensureStaging(substeps);
+    // The pinned staging buffers are reused: the previous upload must be done.
+    HIP_OK(hipStreamSynchronize(s.stream));
+    std::memcpy(s.stagingConfigs, configs, substeps * sizeof(NumiClothBagGPUConfig));
+    double t = s.time;
+    for (uint32_t k = 0; k < substeps; ++k) {
+        t += configs[k].gravityAndTimestep.w;
+        s.stagingDrive[k] = s.driven ? static_cast<float>(s.extension.driveAmplitude *
+                                                          std::sin(s.extension.driveOmega * t + s.extension.drivePhase))
+                                     : 0.0f;
+    }
+    s.time = t;
+    s.configs.upload(s.stagingConfigs, substeps, s.stream);
+    s.drive.upload(s.stagingDrive, substeps, s.stream);
+    s.buildGraph(iterations, sweeps);
+    const uint32_t zero = 0;
+    (void)zero;
+    k_reset_cursor<<<1, 64, 0, s.stream>>>(s.slot.ptr);
+    // Status records count per step() call, like the reference's per-call buffers.
+    k_fill_u32<<<1, 64, 0, s.stream>>>(reinterpret_cast<uint32_t*>(s.frictionStatus.ptr), 8, 0u);
+    k_fill_u32<<<1, 64, 0, s.stream>>>(reinterpret_cast<uint32_t*>(s.selfStatus.ptr), 4, 0u);
+    HIP_OK(hipGetLastError());
+    for (uint32_t k = 0; k < substeps; ++k) HIP_OK(hipGraphLaunch(s.graph, s.stream));
+}
+
+void readSolver(NumiClothHipSolver& s, const NumiClothHipState& out) {
+    hipStream_t st = s.stream;
+    std::vector<float4> hp, hq, hv;
+    if (out.particles != nullptr) {
+        hp.resize(s.P);
+        hq.resize(s.P);
+        hv.resize(s.P);
+        s.pos.download(hp.data(), s.P, st);
+        s.prev.download(hq.data(), s.P, st);
+        s.vel.download(hv.data(), s.P, st);
+    }
+    std::vector<float> dl, bl, kl;
+    if (out.distances != nullptr) {
+        dl.resize(s.D);
+        s.dLambda.download(dl.data(), s.D, st);
+    }
+    if (out.bends != nullptr) {
+        bl.resize(s.B);
+        s.bLambda.download(bl.data(), s.B, st);
+    }
+    if (out.knots != nullptr) {
+        kl.resize(s.K);
+        s.kLambda.download(kl.data(), s.K, st);
+    }
+    if (out.grips != nullptr) s.grips.download(out.grips, s.G, st);
+    if (out.selfStatus != nullptr) s.selfStatus.download(out.selfStatus, 1, st);
+    if (out.frictionStatus != nullptr) s.frictionStatus.download(out.frictionStatus, 1, st);
+    if (out.aerodynamicsStatus != nullptr) s.aeroStatus.download(out.aerodynamicsStatus, 1, st);
+    if (out.releaseStatus != nullptr) s.releaseStatus.download(out.releaseStatus, 1, st);
+    if (out.failure != nullptr) s.failure.download(out.failure, 1, st);
+    HIP_OK(hipStreamSynchronize(st));
+    if (out.particles != nullptr) {
+        for (uint32_t i = 0; i < s.P; ++i) {
+            NumiClothBagGPUParticle& p = out.particles[i];
+            p.positionAndInverseMass = {hp[i].x, hp[i].y, hp[i].z, hp[i].w};
+            p.previousAndMass = {hq[i].x, hq[i].y, hq[i].z, hq[i].w};
+            p.velocity = {hv[i].x, hv[i].y, hv[i].z, hv[i].w};
+        }
+    }
+    if (out.distances != nullptr) {
+        for (uint32_t k = 0; k < s.D; ++k) {
+            out.distances[k] = s.hostDistances[k];
+            out.distances[k].material.z = dl[k];
+        }
+    }
+    if (out.bends != nullptr) {
+        for (uint32_t k = 0; k < s.B; ++k) {
+            out.bends[k] = s.hostBends[k];
+            out.bends[k].material.w = bl[k];
+        }
+    }
+    if (out.knots != nullptr) {
+        for (uint32_t k = 0; k < s.K; ++k) {
+            out.knots[k] = s.hostKnots[k];
+            out.knots[k].material.z = kl[k];
+        }
+    }
+}
+
+void diagnoseSolver(NumiClothHipSolver& s, NumiClothHipDiagnostics* out) {
+    const uint32_t n = std::max(s.P, s.D);
+    const uint32_t blocks = std::min<uint32_t>(1024, blocksFor(n));
+    const float radius = s.initialConfig.clothMaterial.x;
+    k_diag_partial<<<blocks, kBlock, 0, s.stream>>>(s.pos.ptr, s.P, s.dIJ.ptr, s.dMaxLength.ptr, s.D, s.spheres.ptr,
+                                                    s.extension.staticSphereCount, radius, s.diagPartials.ptr);
+    k_diag_final<<<1, kBlock, 0, s.stream>>>(s.diagPartials.ptr, blocks, s.diagResult.ptr);
+    HIP_OK(hipGetLastError());
+    float4 r{};
+    s.diagResult.download(&r, 1, s.stream);
+    HIP_OK(hipStreamSynchronize(s.stream));
+    std::memset(out, 0, sizeof(*out));
+    out->maximumStrainViolation = r.x;
+    out->maximumGroundPenetration = s.initialConfig.constraintCounts.z != 0u ? r.y : 0.0f;
+    out->maximumSpherePenetration = r.z;
+}
+
+void timePass(NumiClothHipSolver& s, uint32_t pass, uint32_t colour, uint32_t repeats,
+              NumiClothHipPassTiming* out) {
+    if (repeats == 0) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "repeats must be positive");
+    const std::vector<Range>* colours = nullptr;
+    // Algorithmic bytes per item: distance 84 (ij, rest, compliance 20; lambda
+    // read+write 8; two endpoints read 32, written 24), bend 80, strain check 48,
+    // collide 28 (read xyzw 16, write xyz 12).
+    double perItem = 0.0;
+    if (pass == NUMI_CLOTH_HIP_PASS_DISTANCE) {
+        colours = &s.distanceColours;
+        perItem = 84.0;
+    } else if (pass == NUMI_CLOTH_HIP_PASS_BEND) {
+        colours = &s.bendColours;
+        perItem = 80.0;
+    } else if (pass == NUMI_CLOTH_HIP_PASS_STRAIN) {
+        colours = &s.distanceColours;
+        perItem = 48.0;
+    } else if (pass != NUMI_CLOTH_HIP_PASS_COLLIDE) {
+        fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "unknown pass");
+    }
+    Range range{0, s.P};
+    if (colours != nullptr) {
+        if (colour >= colours->size()) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "colour out of range");
+        range = (*colours)[colour];
+    } else {
+        perItem = 28.0;
+    }
+    hipEvent_t start = nullptr, stop = nullptr;
+    HIP_OK(hipEventCreate(&start));
+    HIP_OK(hipEventCreate(&stop));
+    HIP_OK(hipEventRecord(start, s.stream));
+    for (uint32_t k = 0; k < repeats; ++k) {
+        if (pass == NUMI_CLOTH_HIP_PASS_DISTANCE) {
+            k_distance<<<blocksFor(range.count), kBlock, 0, s.stream>>>(s.cfg(), s.ok(), range.first, range.count,
+                                                                        s.pos.ptr, s.dIJ.ptr, s.dRestCompliance.ptr,
+                                                                        s.dLambda.ptr, s.failure.ptr);
+        } else if (pass == NUMI_CLOTH_HIP_PASS_BEND) {
+            k_bend<<<blocksFor(range.count), kBlock, 0, s.stream>>>(s.cfg(), s.ok(), range.first, range.count,
+                                                                    s.pos.ptr, s.bEnds.ptr, s.bChordCompliance.ptr,
+                                                                    s.bLambda.ptr, s.failure.ptr);
+        } else if (pass == NUMI_CLOTH_HIP_PASS_STRAIN) {
+            k_strain<<<blocksFor(range.count), kBlock, 0, s.stream>>>(s.ok(), range.first, range.count, s.pos.ptr,
+                                                                      s.dIJ.ptr, s.dMaxLength.ptr, s.failure.ptr);
+        } else {
+            const uint32_t n = s.extension.staticSphereCount;
+            k_collide<<<blocksFor(s.P), kBlock, n * sizeof(float4), s.stream>>>(s.cfg(), s.ok(), s.pos.ptr, s.P,
+                                                                               s.spheres.ptr, n, s.failure.ptr);
+        }
+    }
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipEventRecord(stop, s.stream));
+    HIP_OK(hipEventSynchronize(stop));
+    float ms = 0.0f;
+    HIP_OK(hipEventElapsedTime(&ms, start, stop));
+    (void)hipEventDestroy(start);
+    (void)hipEventDestroy(stop);
+    out->averageMilliseconds = static_cast<double>(ms) / repeats;
+    out->algorithmicBytes = perItem * range.count;
+    out->launches = repeats;
+    out->itemsPerLaunch = range.count;
+}
+
+}  // namespace
+
+extern "C" {
+
+int numi_cloth_hip_time_pass(NumiClothHipSolver* solver, uint32_t pass, uint32_t colour, uint32_t repeats,
+                             NumiClothHipPassTiming* out) {
+    return guarded([&] {
+        if (solver == nullptr || out == nullptr) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "null argument");
+        timePass(*solver, pass, colour, repeats, out);
+    });
+}
+
+
+uint32_t numi_cloth_hip_abi_version(void) { return NUMI_CLOTH_BAG_GPU_ABI_VERSION; }
+
+int numi_cloth_hip_create(const NumiClothHipSceneDesc* scene, NumiClothHipSolver** out) {
+    return guarded([&] {
+        if (scene == nullptr || out == nullptr) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "null argument");
+        *out = nullptr;
+        auto solver = std::make_unique<NumiClothHipSolver>();
+        createSolver(*scene, *solver);
+        *out = solver.release();
+    });
+}
+
+int numi_cloth_hip_step(NumiClothHipSolver* solver, const NumiClothBagGPUConfig* configs, uint32_t substeps,
+                        uint32_t iterations, uint32_t strainSweeps) {
+    return guarded([&] {
+        if (solver == nullptr) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "null solver");
+        stepSolver(*solver, configs, substeps, iterations, strainSweeps);
+    });
+}
+
+int numi_cloth_hip_read(NumiClothHipSolver* solver, const NumiClothHipState* out) {
+    return guarded([&] {
+        if (solver == nullptr || out == nullptr) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "null argument");
+        readSolver(*solver, *out);
+    });
+}
+
+int numi_cloth_hip_diagnostics(NumiClothHipSolver* solver, NumiClothHipDiagnostics* out) {
+    return guarded([&] {
+        if (solver == nullptr || out == nullptr) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "null argument");
+        diagnoseSolver(*solver, out);
+    });
+}
+
+int numi_cloth_hip_synchronize(NumiClothHipSolver* solver) {
+    return guarded([&] {
+        if (solver == nullptr) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "null solver");
+        HIP_OK(hipStreamSynchronize(solver->stream));
+    });
+}
+
+void* numi_cloth_hip_stream(NumiClothHipSolver* solver) {
+    return solver == nullptr ? nullptr : static_cast<void*>(solver->stream);
+}
+
+int numi_cloth_hip_destroy(NumiClothHipSolver* solver) {
+    return guarded([&] {
+        if (solver == nullptr) return;
+        (void)hipStreamSynchronize(solver->stream);
+        delete solver;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,63 @@
+"""ctypes front end of the CPU FP64 oracle (test infrastructure only).
+
+Takes the same C scene descriptor as the product library (the SceneDesc
+ctypes structure of numi-solver_amd/cloth.py) and returns double-precision
+state after a run of substeps.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIBRARY_PATH = Path(__file__).resolve().parent / "libnumi_cloth_oracle.so"
+
+
+class OracleOutput(C.Structure):
+    _fields_ = [("positions", C.c_void_p), ("previous", C.c_void_p), ("velocities", C.c_void_p),
+                ("groundImpulse", C.c_void_p), ("distanceLambda", C.c_void_p), ("bendLambda", C.c_void_p),
+                ("knotLambda", C.c_void_p), ("gripLambda", C.c_void_p), ("gripParticle", C.c_void_p),
+                ("failure", C.c_uint32), ("frictionContacts", C.c_uint32), ("maximumConeRatio", C.c_double)]
+
+
+_lib = None
+
+
+def library() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIBRARY_PATH.exists():
+            from . import build as oracle_build
+            oracle_build.build()
+        lib = C.CDLL(str(LIBRARY_PATH))
+        lib.numi_cloth_oracle_run.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
+                                              C.POINTER(OracleOutput)]
+        lib.numi_cloth_oracle_run.restype = C.c_int
+        _lib = lib
+    return _lib
+
+
+def run(scene, configs: np.ndarray, iterations: int, strain_sweeps: int) -> dict:
+    """Run len(configs) substeps of `scene` (an object with .desc and the
+    ABI arrays) in FP64."""
+    configs = np.ascontiguousarray(configs)
+    P = len(scene.particles)
+    out = {
+        "positions": np.zeros((P, 3)), "previous": np.zeros((P, 3)), "velocities": np.zeros((P, 3)),
+        "groundImpulse": np.zeros(P), "distanceLambda": np.zeros(len(scene.distances)),
+        "bendLambda": np.zeros(len(scene.bends)), "knotLambda": np.zeros(len(scene.knots)),
+        "gripLambda": np.zeros((len(scene.grips), 3)), "gripParticle": np.zeros(len(scene.grips), np.uint32),
+    }
+    result = OracleOutput()
+    for name, array in out.items():
+        setattr(result, name, array.ctypes.data if array.size else None)
+    status = library().numi_cloth_oracle_run(C.addressof(scene.desc), configs.ctypes.data, len(configs),
+                                             iterations, strain_sweeps, C.byref(result))
+    if status != 0:
+        raise RuntimeError(f"oracle run failed with status {status}")
+    out["failure"] = int(result.failure)
+    out["frictionContacts"] = int(result.frictionContacts)
+    out["maximumConeRatio"] = float(result.maximumConeRatio)
+    return out
