@@ -57,7 +57,7 @@ def build_scene(cloth, name: str):
     w = WORKLOADS[name]
     fields = {k: v for k, v in w.items() if k not in ("substeps", "frame")}
     scene = cloth.Scene.sheet(spacing=0.01, elevation=0.3, jitter=1.0e-4, seed=SEED, cellSize=0.01,
-                              timestep=w["frame"] / w["substeps"], **fields)
+                              selfContact=1, timestep=w["frame"] / w["substeps"], **fields)
     return scene, scene.configs(w["substeps"])
 
 
@@ -123,6 +123,7 @@ def main() -> int:
         print(f"device failure bits {state['failure']}", file=sys.stderr)
         return 1
     diagnostics = solver.diagnostics()
+    self_status = state["selfStatus"]["counters"][0]
 
     # Dominant kernel: the distance colour pass, timed with HIP events on the
     # solver's stream right after the timed region (same state, same shape).
@@ -155,7 +156,11 @@ def main() -> int:
         "cpu_baseline": baseline,
         "diagnostics": {"max_strain_violation": diagnostics.maximumStrainViolation,
                         "max_ground_penetration": diagnostics.maximumGroundPenetration,
-                        "max_sphere_penetration": diagnostics.maximumSpherePenetration},
+                        "max_sphere_penetration": diagnostics.maximumSpherePenetration,
+                        "max_self_penetration": diagnostics.maximumSelfPenetration,
+                        "self_candidates_last_substep": diagnostics.selfCandidateCount,
+                        "self_candidate_overflow": bool(diagnostics.selfCandidateOverflow),
+                        "self_contacts_last_step": {"present": int(self_status[0]), "swept": int(self_status[1])}},
     }
     print(json.dumps(line))
     return 0

@@ -192,6 +192,27 @@ typedef struct NumiClothHipPassTiming {
 int numi_cloth_hip_time_pass(NumiClothHipSolver* solver, uint32_t pass, uint32_t colour,
                              uint32_t repeats, NumiClothHipPassTiming* out);
 
+/* Broad phase of the hash self-contact run on the live state (positions and
+ * substep-start positions as they are now) with the scene's initial
+ * configuration, for inspection. Outputs: the (cell id, segment) entries in
+ * radix-sorted order, the candidate pairs (s < t) in emission order, and the
+ * grid. Pointers may be NULL; counts are always written. */
+typedef struct NumiClothHipSelfBroadPhase {
+    uint32_t* pairs;           /* 2 x pairCapacity: s, t */
+    uint32_t* cellIds;         /* entryCapacity */
+    uint32_t* segments;        /* entryCapacity */
+    uint32_t pairCapacity;
+    uint32_t entryCapacity;
+    uint32_t pairCount;        /* out: candidates found */
+    uint32_t entryCount;       /* out: entries written */
+    float origin[3];           /* out: grid origin */
+    float cell;                /* out: cell size */
+    uint32_t cells[3];         /* out: nx, ny, nz */
+    uint32_t valid;            /* out: 0 if the grid was rejected */
+} NumiClothHipSelfBroadPhase;
+
+int numi_cloth_hip_self_broad_phase(NumiClothHipSolver* solver, NumiClothHipSelfBroadPhase* io);
+
 /* The hipStream_t all of the solver's work is queued on. */
 void* numi_cloth_hip_stream(NumiClothHipSolver* solver);
 

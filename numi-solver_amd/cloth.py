@@ -111,6 +111,13 @@ MATERIAL_FIELDS = ["ordinaryNodeMassKg", "hemNodeMassKg", "yarnRadiusM", "axialB
                    "yarnCrossflowDrag", "yarnSkinFriction", "fruitDrag", "fruitRotationalDrag"]
 
 
+class SelfBroadPhase(C.Structure):
+    _fields_ = [("pairs", C.c_void_p), ("cellIds", C.c_void_p), ("segments", C.c_void_p),
+                ("pairCapacity", C.c_uint32), ("entryCapacity", C.c_uint32), ("pairCount", C.c_uint32),
+                ("entryCount", C.c_uint32), ("origin", C.c_float * 3), ("cell", C.c_float),
+                ("cells", C.c_uint32 * 3), ("valid", C.c_uint32)]
+
+
 class PassTiming(C.Structure):
     _fields_ = [("averageMilliseconds", C.c_double), ("algorithmicBytes", C.c_double), ("launches", C.c_uint32),
                 ("itemsPerLaunch", C.c_uint32)]
@@ -188,6 +195,7 @@ def library() -> C.CDLL:
         "numi_cloth_hip_stream": ([vp], vp),
         "numi_cloth_hip_destroy": ([vp], i32),
         "numi_cloth_hip_time_pass": ([vp, u32, u32, u32, C.POINTER(PassTiming)], i32),
+        "numi_cloth_hip_self_broad_phase": ([vp, C.POINTER(SelfBroadPhase)], i32),
         "numi_cloth_material_defaults": ([C.POINTER(MaterialArtifact)], i32),
         "numi_cloth_material_load": ([C.c_char_p, C.POINTER(MaterialArtifact)], i32),
         "numi_cloth_grip_trajectory_load": ([C.c_char_p, C.POINTER(vp)], i32),
@@ -451,6 +459,21 @@ class Solver:
         out = PassTiming()
         _check(library().numi_cloth_hip_time_pass(self._handle, kind, colour, repeats, C.byref(out)))
         return out
+
+    def self_broad_phase(self) -> dict:
+        """Run the self-contact broad phase on the live state; returns the
+        sorted (cell id, segment) entries, the candidate pairs and the grid."""
+        io = SelfBroadPhase()
+        _check(library().numi_cloth_hip_self_broad_phase(self._handle, C.byref(io)))
+        pairs = np.zeros((max(io.pairCount, 1), 2), np.uint32)
+        cells = np.zeros(max(io.entryCount, 1), np.uint32)
+        segments = np.zeros(max(io.entryCount, 1), np.uint32)
+        io.pairs, io.cellIds, io.segments = pairs.ctypes.data, cells.ctypes.data, segments.ctypes.data
+        io.pairCapacity, io.entryCapacity = io.pairCount, io.entryCount
+        _check(library().numi_cloth_hip_self_broad_phase(self._handle, C.byref(io)))
+        return {"pairs": pairs[:io.pairCount], "cellIds": cells[:io.entryCount],
+                "segments": segments[:io.entryCount], "origin": np.array(io.origin[:], np.float32),
+                "cell": np.float32(io.cell), "cells": tuple(io.cells), "valid": io.valid}
 
     def diagnostics(self) -> Diagnostics:
         out = Diagnostics()

@@ -1,0 +1,546 @@
+#pragma once
+
+// Yarn/yarn self-contact on a uniform grid built by a device radix sort.
+//
+// The reference keeps a static all-pairs schedule (E(E-1)/2 pairs in
+// colour batches, reference tools/cloth_bag_metal.mm:1154-1272) and walks the
+// active batches in one threadgroup (reference src/metal/ClothBag.metal:
+// 2125-2448); neither scales past a few thousand segments. Here the contact
+// semantics are defined once, for the GPU and the CPU oracle alike:
+//
+// Broad phase, once per substep after the positions are predicted:
+//   box(s) = [min, max] over both endpoints at the substep start (prev) and
+//            at the predicted position, grown by e = yarn radius + margin.
+//   candidates = every pair s < t whose boxes overlap (closed intervals on
+//            all three axes) and whose segments are not topologically local
+//            (no endpoint of t within two distance hops of an endpoint of s,
+//            the locality rule of reference tools/cloth_bag_metal.mm:1107-1152).
+// The grid only accelerates that set: cell = floor((v - origin) / h) with
+// origin = the componentwise minimum box corner and h the configured cell
+// size; each segment is entered once per cell its box covers, (cell id,
+// segment) entries are sorted by a stable LSD radix sort, and a pair is
+// reported only from its owner cell, the cell of max(lo_s, lo_t), so each
+// pair appears exactly once. Entries and pairs are written through
+// count / exclusive scan / emit passes, so the lists are identical on every
+// run.
+//
+// Narrow phase (reference solve_self_contact arithmetic, present and swept
+// modes) runs on all candidates in parallel from one position snapshot
+// (Jacobi). Each endpoint correction is converted to 2^-32 m fixed point and
+// summed with 64-bit integer atomics, which are order independent; every
+// touched knot then moves by the mean of its corrections. No float atomics.
+
+#include "device_math.hpp"
+#include "kernels_scan.hpp"
+
+namespace numi_dev {
+
+constexpr uint32_t kMaxCellsPerSegment = 512;
+constexpr double kFixedScale = 4294967296.0;          // 2^32 per metre
+constexpr double kFixedInverse = 1.0 / 4294967296.0;
+
+struct SelfGrid {
+    float4 origin;           // xyz origin of cell (0,0,0)
+    float cell;
+    uint32_t nx, ny, nz;
+    uint32_t segments;       // D, scan length for the per-segment passes
+    uint32_t entryCount;     // total entries requested by the boxes
+    uint32_t sortCount;      // min(entryCount, capacity)
+    uint32_t pairTotal;      // candidate pairs found
+    uint32_t pairCount;      // min(pairTotal, capacity)
+    uint32_t valid;          // 0 when the grid would exceed 2^32 cells
+    uint32_t pad[2];
+};
+
+struct FixedAccumulator {
+    long long x, y, z;
+    uint32_t count;
+    uint32_t pad;
+};
+
+// ---------------------------------------------------------- broad phase --
+
+__global__ void k_self_boxes(const NumiClothBagGPUConfig* __restrict__ cfg, float margin,
+                             const float4* __restrict__ pos, const float4* __restrict__ prev,
+                             const uint2* __restrict__ ij, uint32_t segments,
+                             float4* __restrict__ boxLo, float4* __restrict__ boxHi,
+                             float4* __restrict__ partialLo, float4* __restrict__ partialHi) {
+    __shared__ float4 waveLo[kScanThreads / 64];
+    __shared__ float4 waveHi[kScanThreads / 64];
+    const float expand = cfg->clothMaterial.x + margin;
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    float lo[3] = {INFINITY, INFINITY, INFINITY};
+    float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    if (s < segments) {
+        const uint2 e = ij[s];
+        const float4 p[4] = {pos[e.x], pos[e.y], prev[e.x], prev[e.y]};
+        for (int k = 0; k < 4; ++k) {
+            lo[0] = fminf(lo[0], p[k].x);
+            lo[1] = fminf(lo[1], p[k].y);
+            lo[2] = fminf(lo[2], p[k].z);
+            hi[0] = fmaxf(hi[0], p[k].x);
+            hi[1] = fmaxf(hi[1], p[k].y);
+            hi[2] = fmaxf(hi[2], p[k].z);
+        }
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = lo[k] - expand;
+            hi[k] = hi[k] + expand;
+        }
+        boxLo[s] = make_float4(lo[0], lo[1], lo[2], 0.0f);
+        boxHi[s] = make_float4(hi[0], hi[1], hi[2], 0.0f);
+    }
+    for (int k = 0; k < 3; ++k) {
+        for (int offset = 32; offset > 0; offset >>= 1) {
+            lo[k] = fminf(lo[k], __shfl_xor(lo[k], offset));
+            hi[k] = fmaxf(hi[k], __shfl_xor(hi[k], offset));
+        }
+    }
+    if ((threadIdx.x & 63u) == 0) {
+        waveLo[threadIdx.x >> 6] = make_float4(lo[0], lo[1], lo[2], 0.0f);
+        waveHi[threadIdx.x >> 6] = make_float4(hi[0], hi[1], hi[2], 0.0f);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float4 a = waveLo[0], b = waveHi[0];
+        for (uint32_t w = 1; w < blockDim.x / 64; ++w) {
+            a = make_float4(fminf(a.x, waveLo[w].x), fminf(a.y, waveLo[w].y), fminf(a.z, waveLo[w].z), 0.0f);
+            b = make_float4(fmaxf(b.x, waveHi[w].x), fmaxf(b.y, waveHi[w].y), fmaxf(b.z, waveHi[w].z), 0.0f);
+        }
+        partialLo[blockIdx.x] = a;
+        partialHi[blockIdx.x] = b;
+    }
+}
+
+__device__ __forceinline__ int cellOf(float v, float origin, float cell) {
+    return static_cast<int>(floorf((v - origin) / cell));
+}
+
+// One workgroup: the grid origin and extent from the per-block box bounds.
+__global__ void k_self_grid(const float4* __restrict__ partialLo, const float4* __restrict__ partialHi,
+                            uint32_t blocks, const NumiClothBagGPUConfig* __restrict__ cfg,
+                            const uint32_t* __restrict__ ok, uint32_t segments, SelfGrid* __restrict__ grid,
+                            uint32_t* __restrict__ failure) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    float4 lo = partialLo[0], hi = partialHi[0];
+    for (uint32_t b = 1; b < blocks; ++b) {
+        const float4 l = partialLo[b], h = partialHi[b];
+        lo = make_float4(fminf(lo.x, l.x), fminf(lo.y, l.y), fminf(lo.z, l.z), 0.0f);
+        hi = make_float4(fmaxf(hi.x, h.x), fmaxf(hi.y, h.y), fmaxf(hi.z, h.z), 0.0f);
+    }
+    const float cell = cfg->clothMaterial.y;
+    SelfGrid g{};
+    g.origin = lo;
+    g.cell = cell;
+    g.segments = segments;
+    bool valid = *ok != 0u && cell > 0.0f && isfinite(cell) && finite3(xyz(lo)) && finite3(xyz(hi));
+    if (valid) {
+        const double nx = static_cast<double>(cellOf(hi.x, lo.x, cell)) + 1.0;
+        const double ny = static_cast<double>(cellOf(hi.y, lo.y, cell)) + 1.0;
+        const double nz = static_cast<double>(cellOf(hi.z, lo.z, cell)) + 1.0;
+        valid = nx * ny * nz < 4294967296.0;
+        if (valid) {
+            g.nx = static_cast<uint32_t>(nx);
+            g.ny = static_cast<uint32_t>(ny);
+            g.nz = static_cast<uint32_t>(nz);
+        }
+    }
+    if (!valid && *ok != 0u) recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_RANGE);
+    g.valid = valid ? 1u : 0u;
+    *grid = g;
+}
+
+struct CellRange {
+    int lo[3], hi[3];
+    uint32_t count;
+};
+
+__device__ __forceinline__ CellRange cellRange(const float4& lo, const float4& hi, const SelfGrid& g) {
+    CellRange r;
+    r.lo[0] = cellOf(lo.x, g.origin.x, g.cell);
+    r.lo[1] = cellOf(lo.y, g.origin.y, g.cell);
+    r.lo[2] = cellOf(lo.z, g.origin.z, g.cell);
+    r.hi[0] = cellOf(hi.x, g.origin.x, g.cell);
+    r.hi[1] = cellOf(hi.y, g.origin.y, g.cell);
+    r.hi[2] = cellOf(hi.z, g.origin.z, g.cell);
+    const uint64_t n = static_cast<uint64_t>(r.hi[0] - r.lo[0] + 1) * (r.hi[1] - r.lo[1] + 1) *
+                       (r.hi[2] - r.lo[2] + 1);
+    r.count = n > kMaxCellsPerSegment ? 0u : static_cast<uint32_t>(n);
+    return r;
+}
+
+__device__ __forceinline__ uint32_t cellId(int x, int y, int z, const SelfGrid& g) {
+    return (static_cast<uint32_t>(z) * g.ny + static_cast<uint32_t>(y)) * g.nx + static_cast<uint32_t>(x);
+}
+
+__global__ void k_self_cell_counts(const float4* __restrict__ boxLo, const float4* __restrict__ boxHi,
+                                   const SelfGrid* __restrict__ grid, uint32_t* __restrict__ counts,
+                                   uint32_t* __restrict__ failure) {
+    const SelfGrid g = *grid;
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= g.segments) return;
+    uint32_t n = 0;
+    if (g.valid) {
+        const CellRange r = cellRange(boxLo[s], boxHi[s], g);
+        n = r.count;
+        if (n == 0) recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_RANGE);
+    }
+    counts[s] = n;
+}
+
+__global__ void k_self_emit_entries(const float4* __restrict__ boxLo, const float4* __restrict__ boxHi,
+                                    SelfGrid* __restrict__ grid, const uint32_t* __restrict__ offsets,
+                                    const uint32_t* __restrict__ entryTotal, uint32_t capacity,
+                                    uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                                    uint32_t* __restrict__ failure) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    const SelfGrid g = *grid;
+    if (s == 0) {
+        grid->entryCount = *entryTotal;
+        grid->sortCount = min(*entryTotal, capacity);
+        if (*entryTotal > capacity) recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_RANGE);
+    }
+    if (s >= g.segments || !g.valid) return;
+    const CellRange r = cellRange(boxLo[s], boxHi[s], g);
+    uint32_t out = offsets[s];
+    if (r.count == 0) return;
+    for (int z = r.lo[2]; z <= r.hi[2]; ++z)
+        for (int y = r.lo[1]; y <= r.hi[1]; ++y)
+            for (int x = r.lo[0]; x <= r.hi[0]; ++x, ++out)
+                if (out < capacity) {
+                    keys[out] = cellId(x, y, z, g);
+                    vals[out] = s;
+                }
+}
+
+// Two knots are local when one lies in the other's two-hop neighbourhood
+// (sorted CSR lists that include the knot itself).
+__device__ __forceinline__ bool inList(const uint32_t* __restrict__ list, uint32_t begin, uint32_t end,
+                                       uint32_t value) {
+    while (begin < end) {
+        const uint32_t mid = begin + (end - begin) / 2;
+        const uint32_t v = list[mid];
+        if (v == value) return true;
+        if (v < value) begin = mid + 1;
+        else end = mid;
+    }
+    return false;
+}
+
+__device__ __forceinline__ bool localPair(uint2 a, uint2 b, const uint32_t* __restrict__ start,
+                                          const uint32_t* __restrict__ list) {
+    const uint32_t a0 = start[a.x], a1 = start[a.x + 1];
+    const uint32_t b0 = start[a.y], b1 = start[a.y + 1];
+    return inList(list, a0, a1, b.x) || inList(list, a0, a1, b.y) || inList(list, b0, b1, b.x) ||
+           inList(list, b0, b1, b.y);
+}
+
+__device__ __forceinline__ bool boxesOverlap(const float4& aLo, const float4& aHi, const float4& bLo,
+                                             const float4& bHi) {
+    return aLo.x <= bHi.x && aLo.y <= bHi.y && aLo.z <= bHi.z && aHi.x >= bLo.x && aHi.y >= bLo.y &&
+           aHi.z >= bLo.z;
+}
+
+// Visits the candidates of segment s in a fixed order (cells z, y, x; entries
+// in sorted order); `emit` is called with each partner t.
+template <typename Emit>
+__device__ void forEachCandidate(uint32_t s, const SelfGrid& g, const float4* __restrict__ boxLo,
+                                 const float4* __restrict__ boxHi, const uint2* __restrict__ ij,
+                                 const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                                 const uint32_t* __restrict__ localStart, const uint32_t* __restrict__ localList,
+                                 Emit emit) {
+    const float4 lo = boxLo[s], hi = boxHi[s];
+    const CellRange r = cellRange(lo, hi, g);
+    if (r.count == 0) return;
+    const uint2 es = ij[s];
+    const uint32_t n = g.sortCount;
+    for (int z = r.lo[2]; z <= r.hi[2]; ++z) {
+        for (int y = r.lo[1]; y <= r.hi[1]; ++y) {
+            for (int x = r.lo[0]; x <= r.hi[0]; ++x) {
+                const uint32_t id = cellId(x, y, z, g);
+                uint32_t first = 0, last = n;
+                while (first < last) {
+                    const uint32_t mid = first + (last - first) / 2;
+                    if (keys[mid] < id) first = mid + 1;
+                    else last = mid;
+                }
+                for (uint32_t k = first; k < n && keys[k] == id; ++k) {
+                    const uint32_t t = vals[k];
+                    if (t <= s) continue;
+                    const float4 tLo = boxLo[t], tHi = boxHi[t];
+                    if (!boxesOverlap(lo, hi, tLo, tHi)) continue;
+                    if (cellOf(fmaxf(lo.x, tLo.x), g.origin.x, g.cell) != x ||
+                        cellOf(fmaxf(lo.y, tLo.y), g.origin.y, g.cell) != y ||
+                        cellOf(fmaxf(lo.z, tLo.z), g.origin.z, g.cell) != z) {
+                        continue;
+                    }
+                    if (localPair(es, ij[t], localStart, localList)) continue;
+                    emit(t);
+                }
+            }
+        }
+    }
+}
+
+__global__ void k_self_pair_counts(const float4* __restrict__ boxLo, const float4* __restrict__ boxHi,
+                                   const uint2* __restrict__ ij, const SelfGrid* __restrict__ grid,
+                                   const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                                   const uint32_t* __restrict__ localStart, const uint32_t* __restrict__ localList,
+                                   uint32_t* __restrict__ counts) {
+    const SelfGrid g = *grid;
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= g.segments) return;
+    uint32_t n = 0;
+    if (g.valid) forEachCandidate(s, g, boxLo, boxHi, ij, keys, vals, localStart, localList, [&](uint32_t) { ++n; });
+    counts[s] = n;
+}
+
+__global__ void k_self_emit_pairs(const float4* __restrict__ boxLo, const float4* __restrict__ boxHi,
+                                  const uint2* __restrict__ ij, SelfGrid* __restrict__ grid,
+                                  const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                                  const uint32_t* __restrict__ localStart, const uint32_t* __restrict__ localList,
+                                  const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ pairTotal,
+                                  uint32_t capacity, uint2* __restrict__ pairs, uint32_t* __restrict__ failure) {
+    const SelfGrid g = *grid;
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s == 0) {
+        grid->pairTotal = *pairTotal;
+        grid->pairCount = min(*pairTotal, capacity);
+        if (*pairTotal > capacity) recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_RANGE);
+    }
+    if (s >= g.segments || !g.valid) return;
+    uint32_t out = offsets[s];
+    forEachCandidate(s, g, boxLo, boxHi, ij, keys, vals, localStart, localList, [&](uint32_t t) {
+        if (out < capacity) pairs[out] = make_uint2(s, t);
+        ++out;
+    });
+}
+
+// ---------------------------------------------------------- narrow phase --
+
+struct SegmentSample {
+    V3 first, second;
+    float firstWeight, secondWeight, distance;
+};
+
+// Closest points of two segments (reference sampleSegments,
+// src/metal/ClothBag.metal:135-211).
+__device__ SegmentSample sampleSegments(V3 a0, V3 a1, V3 b0, V3 b1) {
+    const V3 da = a1 - a0;
+    const V3 db = b1 - b0;
+    const V3 offset = a0 - b0;
+    const float la = dot(da, da);
+    const float lb = dot(db, db);
+    const float pb = dot(db, offset);
+    float wa = 0.0f, wb = 0.0f;
+    if (la <= 1.0e-20f && lb <= 1.0e-20f) return {a0, b0, 0.0f, 0.0f, length(b0 - a0)};
+    if (la <= 1.0e-20f) {
+        wb = clampf(pb / lb, 0.0f, 1.0f);
+    } else {
+        const float pa = dot(da, offset);
+        if (lb <= 1.0e-20f) {
+            wa = clampf(-pa / la, 0.0f, 1.0f);
+        } else {
+            const float cross_ = dot(da, db);
+            const float denom = la * lb - cross_ * cross_;
+            if (denom > 1.0e-20f) wa = clampf((cross_ * pb - pa * lb) / denom, 0.0f, 1.0f);
+            wb = (cross_ * wa + pb) / lb;
+            if (wb < 0.0f) {
+                wb = 0.0f;
+                wa = clampf(-pa / la, 0.0f, 1.0f);
+            } else if (wb > 1.0f) {
+                wb = 1.0f;
+                wa = clampf((cross_ - pa) / la, 0.0f, 1.0f);
+            }
+        }
+    }
+    const V3 pA = fma3(da, wa, a0);
+    const V3 pB = fma3(db, wb, b0);
+    return {pA, pB, wa, wb, length(pB - pA)};
+}
+
+__device__ V3 contactNormal(V3 separation, V3 da, V3 db, V3 previousOffset) {
+    const float len = length(separation);
+    if (len > 1.0e-12f) return separation / len;
+    V3 n = safeNormalized(cross(da, db));
+    if (dot(previousOffset, n) > 0.0f) n = -n;
+    return n;
+}
+
+__device__ __forceinline__ void addFixed(FixedAccumulator* acc, V3 d) {
+    atomicAdd(reinterpret_cast<unsigned long long*>(&acc->x),
+              static_cast<unsigned long long>(__double2ll_rn(static_cast<double>(d.x) * kFixedScale)));
+    atomicAdd(reinterpret_cast<unsigned long long*>(&acc->y),
+              static_cast<unsigned long long>(__double2ll_rn(static_cast<double>(d.y) * kFixedScale)));
+    atomicAdd(reinterpret_cast<unsigned long long*>(&acc->z),
+              static_cast<unsigned long long>(__double2ll_rn(static_cast<double>(d.z) * kFixedScale)));
+    atomicAdd(&acc->count, 1u);
+}
+
+// solve_self_contact (reference src/metal/ClothBag.metal:2125-2448) for every
+// candidate against one snapshot; mode 0 present overlap, mode 1 swept
+// conservative advancement (80 steps, 0.9 safety, 1e-10 minimum advance).
+__global__ void k_self_solve(const NumiClothBagGPUConfig* __restrict__ cfg, const uint32_t* __restrict__ ok,
+                             uint32_t mode, const SelfGrid* __restrict__ grid, const uint2* __restrict__ pairs,
+                             const uint2* __restrict__ ij, const float4* __restrict__ pos,
+                             const float4* __restrict__ prev, FixedAccumulator* __restrict__ acc,
+                             NumiClothBagGPUSelfStatus* __restrict__ status) {
+    if (*ok == 0u) return;
+    const uint32_t count = grid->valid ? grid->pairCount : 0u;
+    const float r = cfg->clothMaterial.x;
+    const float target = 2.0f * r;
+    const float tolerance = 1.0e-9f;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t rounds = (count + stride - 1) / stride;
+    for (uint32_t round = 0; round < rounds; ++round) {
+        const uint32_t k = round * stride + blockIdx.x * blockDim.x + threadIdx.x;
+        bool accepted = false;
+        float correction = 0.0f;
+        if (k < count) {
+            const uint2 pr = pairs[k];
+            const uint2 ea = ij[pr.x];
+            const uint2 eb = ij[pr.y];
+            const float4 A0 = pos[ea.x], A1 = pos[ea.y], B0 = pos[eb.x], B1 = pos[eb.y];
+            const float4 Q0 = prev[ea.x], Q1 = prev[ea.y], R0 = prev[eb.x], R1 = prev[eb.y];
+            // edgeBoundsOverlap (reference :213-274) with expansion r.
+            float aLo[3] = {fminf(A0.x, A1.x), fminf(A0.y, A1.y), fminf(A0.z, A1.z)};
+            float aHi[3] = {fmaxf(A0.x, A1.x), fmaxf(A0.y, A1.y), fmaxf(A0.z, A1.z)};
+            float bLo[3] = {fminf(B0.x, B1.x), fminf(B0.y, B1.y), fminf(B0.z, B1.z)};
+            float bHi[3] = {fmaxf(B0.x, B1.x), fmaxf(B0.y, B1.y), fmaxf(B0.z, B1.z)};
+            if (mode == 1u) {
+                const float qa[2][3] = {{Q0.x, Q0.y, Q0.z}, {Q1.x, Q1.y, Q1.z}};
+                const float qb[2][3] = {{R0.x, R0.y, R0.z}, {R1.x, R1.y, R1.z}};
+                for (int a = 0; a < 3; ++a) {
+                    aLo[a] = fminf(aLo[a], fminf(qa[0][a], qa[1][a]));
+                    aHi[a] = fmaxf(aHi[a], fmaxf(qa[0][a], qa[1][a]));
+                    bLo[a] = fminf(bLo[a], fminf(qb[0][a], qb[1][a]));
+                    bHi[a] = fmaxf(bHi[a], fmaxf(qb[0][a], qb[1][a]));
+                }
+            }
+            bool overlap = true;
+            for (int a = 0; a < 3; ++a) overlap = overlap && (aLo[a] - r <= bHi[a] + r) && (aHi[a] + r >= bLo[a] - r);
+            float wa = 0.0f, wb = 0.0f;
+            V3 n = v3(0.0f, 0.0f, 0.0f);
+            if (overlap) {
+                if (mode == 0u) {
+                    const SegmentSample c = sampleSegments(xyz(A0), xyz(A1), xyz(B0), xyz(B1));
+                    if (c.distance < target) {
+                        wa = c.firstWeight;
+                        wb = c.secondWeight;
+                        const V3 pa = mix3(xyz(Q0), xyz(Q1), wa);
+                        const V3 pb = mix3(xyz(R0), xyz(R1), wb);
+                        n = contactNormal(c.second - c.first, xyz(A1) - xyz(A0), xyz(B1) - xyz(B0), pa - pb);
+                        correction = target - c.distance;
+                    }
+                } else {
+                    const float bound = length(xyz(A0) - xyz(Q0)) + length(xyz(A1) - xyz(Q1)) +
+                                        length(xyz(B0) - xyz(R0)) + length(xyz(B1) - xyz(R1));
+                    if (bound >= 1.0e-14f) {
+                        float t = 0.0f;
+                        auto at = [&](float time) {
+                            return sampleSegments(mix3(xyz(Q0), xyz(A0), time), mix3(xyz(Q1), xyz(A1), time),
+                                                  mix3(xyz(R0), xyz(B0), time), mix3(xyz(R1), xyz(B1), time));
+                        };
+                        SegmentSample hit = at(0.0f);
+                        bool found = hit.distance <= target + tolerance;
+                        if (!found) {
+                            for (int step = 0; step < 80; ++step) {
+                                hit = at(t);
+                                const float gap = hit.distance - target;
+                                if (gap <= tolerance) {
+                                    found = true;
+                                    break;
+                                }
+                                const float advance = 0.9f * gap / bound;
+                                if (!isfinite(advance) || !(advance > 0.0f) || t + advance >= 1.0f) break;
+                                t += fmaxf(advance, 1.0e-10f);
+                            }
+                        }
+                        if (found) {
+                            wa = hit.firstWeight;
+                            wb = hit.secondWeight;
+                            const V3 pa = mix3(xyz(Q0), xyz(Q1), wa);
+                            const V3 pb = mix3(xyz(R0), xyz(R1), wb);
+                            const V3 a0t = mix3(xyz(Q0), xyz(A0), t), a1t = mix3(xyz(Q1), xyz(A1), t);
+                            const V3 b0t = mix3(xyz(R0), xyz(B0), t), b1t = mix3(xyz(R1), xyz(B1), t);
+                            n = contactNormal(hit.second - hit.first, a1t - a0t, b1t - b0t, pa - pb);
+                            const V3 restA = (xyz(A0) - a0t) * (1.0f - wa) + (xyz(A1) - a1t) * wa;
+                            const V3 restB = (xyz(B0) - b0t) * (1.0f - wb) + (xyz(B1) - b1t) * wb;
+                            correction = dot(restA - restB, n);
+                            if (!(correction > 0.0f)) correction = 0.0f;
+                        }
+                    }
+                }
+            }
+            if (correction > 0.0f) {
+                const float ua = 1.0f - wa, ub = 1.0f - wb;
+                const float denom = A0.w * ua * ua + A1.w * wa * wa + B0.w * ub * ub + B1.w * wb * wb;
+                if (denom > 0.0f && isfinite(denom)) {
+                    const float lambda = correction / denom;
+                    addFixed(acc + ea.x, n * -(A0.w * ua * lambda));
+                    addFixed(acc + ea.y, n * -(A1.w * wa * lambda));
+                    addFixed(acc + eb.x, n * (B0.w * ub * lambda));
+                    addFixed(acc + eb.y, n * (B1.w * wb * lambda));
+                    accepted = true;
+                }
+            }
+        }
+        const unsigned long long ballot = __ballot(accepted);
+        float m = accepted ? correction : 0.0f;
+        for (int offset = 32; offset > 0; offset >>= 1) m = fmaxf(m, __shfl_xor(m, offset));
+        if ((threadIdx.x & 63u) == 0 && ballot != 0ull) {
+            atomicAdd(mode == 0u ? &status->counters.x : &status->counters.y, static_cast<uint32_t>(__popcll(ballot)));
+            atomicMax(&status->counters.z, __float_as_uint(m));
+        }
+    }
+}
+
+// Moves every touched knot by the mean of its fixed-point corrections and
+// clears its accumulator for the next call.
+__global__ void k_self_apply(const uint32_t* __restrict__ ok, float4* __restrict__ pos,
+                             FixedAccumulator* __restrict__ acc, uint32_t particles, uint32_t* __restrict__ failure) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= particles || *ok == 0u) return;
+    const uint32_t n = acc[i].count;
+    if (n == 0) return;
+    const FixedAccumulator a = acc[i];
+    const double scale = kFixedInverse / static_cast<double>(n);
+    float4 p = pos[i];
+    const V3 d = v3(static_cast<float>(static_cast<double>(a.x) * scale),
+                    static_cast<float>(static_cast<double>(a.y) * scale),
+                    static_cast<float>(static_cast<double>(a.z) * scale));
+    const V3 np = xyz(p) + d;
+    if (!finite3(np)) {
+        recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
+    } else {
+        pos[i] = withXyz(p, np);
+    }
+    acc[i] = FixedAccumulator{0, 0, 0, 0u, 0u};
+}
+
+// Largest capsule overlap 2r - distance over the current candidates
+// (reference maximumSelfPenetration, tools/cloth_bag_metal.mm:5216-5276),
+// per-block partial maxima written into the w lane of the diagnostic
+// partials.
+__global__ void k_self_penetration_into(const SelfGrid* __restrict__ grid, const uint2* __restrict__ pairs,
+                                        const uint2* __restrict__ ij, const float4* __restrict__ pos, float radius,
+                                        float4* __restrict__ partial) {
+    __shared__ float waveMaxima[kScanThreads / 64];
+    const uint32_t count = grid->valid ? grid->pairCount : 0u;
+    float m = 0.0f;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < count; k += gridDim.x * blockDim.x) {
+        const uint2 pr = pairs[k];
+        const uint2 a = ij[pr.x], b = ij[pr.y];
+        const SegmentSample c = sampleSegments(xyz(pos[a.x]), xyz(pos[a.y]), xyz(pos[b.x]), xyz(pos[b.y]));
+        m = fmaxf(m, 2.0f * radius - c.distance);
+    }
+    for (int offset = 32; offset > 0; offset >>= 1) m = fmaxf(m, __shfl_xor(m, offset));
+    if ((threadIdx.x & 63u) == 0) waveMaxima[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float r = 0.0f;
+        for (uint32_t w = 0; w < blockDim.x / 64; ++w) r = fmaxf(r, waveMaxima[w]);
+        partial[blockIdx.x].w = r;
+    }
+}
+
+}  // namespace numi_dev

@@ -303,7 +303,8 @@ void buildSheet(const NumiClothHipSheetSpec& spec, NumiClothHipScene& s) {
     }
     s.publish();
     s.desc.extension.selfContact = spec.selfContact;
-    s.desc.extension.selfContactMargin = spec.yarnRadius;
+    // Broad-phase margin: a quarter yarn radius of travel within a substep.
+    s.desc.extension.selfContactMargin = 0.25f * spec.yarnRadius;
 }
 
 // ------------------------------------------------------------------ bag ----
@@ -573,7 +574,7 @@ void buildBag(const NumiClothHipBagSpec& spec, NumiClothHipScene& s) {
     }
     s.publish();
     s.desc.extension.selfContact = spec.selfContact;
-    s.desc.extension.selfContactMargin = yarnRadius;
+    s.desc.extension.selfContactMargin = 0.25f * yarnRadius;
 }
 
 }  // namespace

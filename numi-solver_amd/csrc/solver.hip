@@ -23,6 +23,7 @@
 #include "common.hpp"
 #include "kernels_core.hpp"
 #include "kernels_diag.hpp"
+#include "kernels_self.hpp"
 #include "numi_cloth_hip.h"
 
 using namespace numi_hip;
@@ -114,6 +115,23 @@ struct NumiClothHipSolver {
     DeviceArray<NumiClothBagGPUReleaseStatus> releaseStatus;
     DeviceArray<float4> diagPartials, diagResult;
 
+    // Hash self-contact (present when extension.selfContact != 0).
+    bool self = false;
+    uint32_t entryCapacity = 0, pairCapacity = 0, entryTiles = 0, segmentBlocks = 0;
+    DeviceArray<uint32_t> localStart, localList;
+    DeviceArray<float4> boxLo, boxHi, partialLo, partialHi;
+    DeviceArray<SelfGrid> grid;
+    DeviceArray<uint32_t> segmentCounts, scanSums, totals, constants;
+    DeviceArray<uint32_t> keysA, valsA, keysB, valsB, histogram;
+    DeviceArray<uint2> pairs;
+    DeviceArray<FixedAccumulator> accumulator;
+    DeviceArray<SubstepSlot> debugSlot;
+
+    // Graphs are the default; NUMI_CLOTH_HIP_GRAPHS=0 launches the same
+    // kernels directly on the stream (used under profilers that do not
+    // follow graph launches).
+    bool useGraphs = true;
+
     // Pinned staging for per-step uploads.
     NumiClothBagGPUConfig* stagingConfigs = nullptr;
     float* stagingDrive = nullptr;
@@ -203,6 +221,60 @@ struct NumiClothHipSolver {
                                                                         failure.ptr);
     }
 
+    // Exclusive scan of `capacity`-sized data whose live length is *count.
+    void launchScan(uint32_t* data, const uint32_t* count, uint32_t capacity, uint32_t* total) {
+        const uint32_t tiles = std::max(1u, (capacity + kScanTile - 1) / kScanTile);
+        k_scan_tiles<<<tiles, kScanThreads, 0, stream>>>(data, count, scanSums.ptr);
+        k_scan_sums<<<1, kScanThreads, 0, stream>>>(scanSums.ptr, count, total);
+        k_scan_add<<<tiles, kScanThreads, 0, stream>>>(data, count, scanSums.ptr);
+    }
+
+    // Broad phase: boxes, grid, cell entries, radix sort, candidate pairs.
+    void launchBroadPhase(const NumiClothBagGPUConfig* config, const uint32_t* valid) {
+        const float margin = extension.selfContactMargin;
+        SelfGrid* g = grid.ptr;
+        k_self_boxes<<<segmentBlocks, kScanThreads, 0, stream>>>(config, margin, pos.ptr, prev.ptr, dIJ.ptr, D,
+                                                                 boxLo.ptr, boxHi.ptr, partialLo.ptr,
+                                                                 partialHi.ptr);
+        k_self_grid<<<1, 64, 0, stream>>>(partialLo.ptr, partialHi.ptr, segmentBlocks, config, valid, D, g,
+                                          failure.ptr);
+        k_self_cell_counts<<<segmentBlocks, kScanThreads, 0, stream>>>(boxLo.ptr, boxHi.ptr, g, segmentCounts.ptr,
+                                                                       failure.ptr);
+        launchScan(segmentCounts.ptr, &g->segments, D, totals.ptr);
+        k_self_emit_entries<<<segmentBlocks, kScanThreads, 0, stream>>>(boxLo.ptr, boxHi.ptr, g, segmentCounts.ptr,
+                                                                        totals.ptr, entryCapacity, keysA.ptr,
+                                                                        valsA.ptr, failure.ptr);
+        uint32_t* keysIn = keysA.ptr;
+        uint32_t* valsIn = valsA.ptr;
+        uint32_t* keysOut = keysB.ptr;
+        uint32_t* valsOut = valsB.ptr;
+        for (uint32_t shift = 0; shift < 32; shift += kRadixBits) {
+            k_radix_histogram<<<entryTiles, kScanThreads, 0, stream>>>(keysIn, &g->sortCount, shift, entryTiles,
+                                                                        histogram.ptr);
+            launchScan(histogram.ptr, constants.ptr, kRadixBins * entryTiles, totals.ptr + 2);
+            k_radix_scatter<<<entryTiles, kScanThreads, 0, stream>>>(keysIn, valsIn, &g->sortCount, shift,
+                                                                      entryTiles, histogram.ptr, keysOut, valsOut);
+            std::swap(keysIn, keysOut);
+            std::swap(valsIn, valsOut);
+        }
+        k_self_pair_counts<<<segmentBlocks, kScanThreads, 0, stream>>>(boxLo.ptr, boxHi.ptr, dIJ.ptr, g, keysIn,
+                                                                       valsIn, localStart.ptr, localList.ptr,
+                                                                       segmentCounts.ptr);
+        launchScan(segmentCounts.ptr, &g->segments, D, totals.ptr + 1);
+        k_self_emit_pairs<<<segmentBlocks, kScanThreads, 0, stream>>>(boxLo.ptr, boxHi.ptr, dIJ.ptr, g, keysIn,
+                                                                      valsIn, localStart.ptr, localList.ptr,
+                                                                      segmentCounts.ptr, totals.ptr + 1,
+                                                                      pairCapacity, pairs.ptr, failure.ptr);
+    }
+
+    void launchSelf(uint32_t mode) {
+        if (!self) return;
+        const uint32_t blocks = std::min<uint32_t>(2048, std::max(1u, (pairCapacity + kBlock - 1) / kBlock));
+        k_self_solve<<<blocks, kBlock, 0, stream>>>(cfg(), ok(), mode, grid.ptr, pairs.ptr, dIJ.ptr, pos.ptr,
+                                                     prev.ptr, accumulator.ptr, selfStatus.ptr);
+        k_self_apply<<<blocksFor(P), kBlock, 0, stream>>>(ok(), pos.ptr, accumulator.ptr, P, failure.ptr);
+    }
+
     void launchGrips() {
         if (G == 0) return;
         k_grip<<<blocksFor(G), kBlock, 0, stream>>>(cfg(), ok(), pos.ptr, grips.ptr, G, P, failure.ptr);
@@ -224,11 +296,16 @@ struct NumiClothHipSolver {
                                                                failure.ptr);
             k_advance<<<blocksFor(P), kBlock, 0, stream>>>(cfg(), ok(), pos.ptr, vel.ptr, P);
         }
+        if (self) {
+            launchBroadPhase(cfg(), ok());
+            launchSelf(1);
+        }
         for (uint32_t it = 0; it < iterations; ++it) {
             launchDistances();
             launchKnots();
             if (it % 2 == 0) launchBends();
             launchGrips();
+            launchSelf(0);
             launchCollide();
         }
         launchStrain(sweeps);
@@ -237,9 +314,11 @@ struct NumiClothHipSolver {
                 launchStrain(sweeps);
                 launchCollide();
             }
+            launchSelf(0);
             launchStrain(sweeps);
             for (uint32_t pass = 0; pass < kFinalContactPasses; ++pass) launchCollide();
             for (uint32_t certificate = 0; certificate < kCertificatePasses; ++certificate) {
+                launchSelf(0);
                 launchStrain(sweeps);
                 for (uint32_t pass = 0; pass < kFinalContactPasses; ++pass) launchCollide();
             }
@@ -249,6 +328,7 @@ struct NumiClothHipSolver {
     }
 
     void buildGraph(uint32_t iterations, uint32_t sweeps) {
+        if (!useGraphs) return;
         if (graph != nullptr && graphIterations == iterations && graphSweeps == sweeps) return;
         if (graph != nullptr) {
             HIP_OK(hipGraphExecDestroy(graph));
@@ -339,6 +419,8 @@ void checkColouring(const std::vector<Range>& ranges, uint32_t particles, Partic
     }
 }
 
+void setupSelfContact(const NumiClothHipSceneDesc& scene, NumiClothHipSolver& s);
+
 void createSolver(const NumiClothHipSceneDesc& scene, NumiClothHipSolver& s) {
     if (scene.config == nullptr) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "scene config is null");
     const NumiClothBagGPUConfig& c = *scene.config;
@@ -353,9 +435,6 @@ void createSolver(const NumiClothHipSceneDesc& scene, NumiClothHipSolver& s) {
     requireEqual(c.contactCounts.y, scene.yarnContactCount, "yarn contacts");
     if (scene.fruitCount != 0 || scene.fruitPairCount != 0 || scene.yarnContactCount != 0) {
         fail(NUMI_CLOTH_HIP_ERROR_UNSUPPORTED, "fruit and fruit/yarn families are not built into this library");
-    }
-    if (scene.extension.selfContact != 0) {
-        fail(NUMI_CLOTH_HIP_ERROR_UNSUPPORTED, "hash self-contact is not built into this library");
     }
     s.initialConfig = c;
     s.P = scene.particleCount;
@@ -514,8 +593,77 @@ void createSolver(const NumiClothHipSceneDesc& scene, NumiClothHipSolver& s) {
     HIP_OK(hipMemsetAsync(s.aeroStatus.ptr, 0, sizeof(NumiClothBagGPUAerodynamicsStatus), st));
     HIP_OK(hipMemsetAsync(s.releaseStatus.ptr, 0, sizeof(NumiClothBagGPUReleaseStatus), st));
     s.diagPartials.allocate(1024);
+    if (const char* graphs = std::getenv("NUMI_CLOTH_HIP_GRAPHS")) s.useGraphs = std::strcmp(graphs, "0") != 0;
+    if (scene.extension.selfContact != 0) setupSelfContact(scene, s);
     s.diagResult.allocate(1);
     s.ensureStaging(64);
+    HIP_OK(hipStreamSynchronize(st));
+}
+
+// Two-hop neighbourhoods over the distance graph (sorted CSR, each list
+// includes its own knot): the locality rule that keeps adjacent yarn out of
+// self-contact (reference tools/cloth_bag_metal.mm:1107-1152).
+void setupSelfContact(const NumiClothHipSceneDesc& scene, NumiClothHipSolver& s) {
+    if (s.D == 0) return;
+    if (!(scene.config->clothMaterial.y > 0.0f)) {
+        fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "self-contact needs a positive cell size (clothMaterial.y)");
+    }
+    std::vector<std::vector<uint32_t>> direct(s.P);
+    for (const NumiClothBagGPUDistance& d : s.hostDistances) {
+        direct[d.particlesAndColor.x].push_back(d.particlesAndColor.y);
+        direct[d.particlesAndColor.y].push_back(d.particlesAndColor.x);
+    }
+    std::vector<uint32_t> start(s.P + 1, 0), list;
+    for (uint32_t i = 0; i < s.P; ++i) {
+        std::vector<uint32_t> near{i};
+        for (uint32_t a : direct[i]) {
+            near.push_back(a);
+            for (uint32_t b : direct[a]) near.push_back(b);
+        }
+        std::sort(near.begin(), near.end());
+        near.erase(std::unique(near.begin(), near.end()), near.end());
+        start[i + 1] = start[i] + static_cast<uint32_t>(near.size());
+        list.insert(list.end(), near.begin(), near.end());
+    }
+    hipStream_t st = s.stream;
+    s.self = true;
+    s.localStart.allocate(start.size());
+    s.localStart.upload(start.data(), start.size(), st);
+    s.localList.allocate(list.size());
+    s.localList.upload(list.data(), list.size(), st);
+    const uint64_t entries = static_cast<uint64_t>(s.D) * 16u;
+    const uint64_t pairCapacity = scene.extension.selfCandidateCapacity != 0
+                                      ? scene.extension.selfCandidateCapacity
+                                      : static_cast<uint64_t>(s.D) * 16u;
+    if (entries > 0x7fffffffull || pairCapacity > 0x7fffffffull) {
+        fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "self-contact capacities exceed 2^31");
+    }
+    s.entryCapacity = static_cast<uint32_t>(entries);
+    s.pairCapacity = static_cast<uint32_t>(pairCapacity);
+    s.entryTiles = (s.entryCapacity + kScanTile - 1) / kScanTile;
+    s.segmentBlocks = (s.D + kScanThreads - 1) / kScanThreads;
+    s.boxLo.allocate(s.D);
+    s.boxHi.allocate(s.D);
+    s.partialLo.allocate(s.segmentBlocks);
+    s.partialHi.allocate(s.segmentBlocks);
+    s.grid.allocate(1);
+    s.segmentCounts.allocate(s.D);
+    const uint32_t maxScanLength = std::max({s.D, kRadixBins * s.entryTiles, 1u});
+    s.scanSums.allocate((maxScanLength + kScanTile - 1) / kScanTile + 1);
+    s.totals.allocate(4);
+    s.constants.allocate(1);
+    const uint32_t histLength = kRadixBins * s.entryTiles;
+    s.constants.upload(&histLength, 1, st);
+    s.keysA.allocate(s.entryCapacity);
+    s.valsA.allocate(s.entryCapacity);
+    s.keysB.allocate(s.entryCapacity);
+    s.valsB.allocate(s.entryCapacity);
+    s.histogram.allocate(histLength);
+    s.pairs.allocate(s.pairCapacity);
+    s.accumulator.allocate(s.P);
+    HIP_OK(hipMemsetAsync(s.accumulator.ptr, 0, s.P * sizeof(FixedAccumulator), st));
+    HIP_OK(hipMemsetAsync(s.grid.ptr, 0, sizeof(SelfGrid), st));
+    s.debugSlot.allocate(1);
     HIP_OK(hipStreamSynchronize(st));
 }
 
@@ -569,7 +717,12 @@ void stepSolver(NumiClothHipSolver& s, const NumiClothBagGPUConfig* configs, uin
     k_fill_u32<<<1, 64, 0, s.stream>>>(reinterpret_cast<uint32_t*>(s.frictionStatus.ptr), 8, 0u);
     k_fill_u32<<<1, 64, 0, s.stream>>>(reinterpret_cast<uint32_t*>(s.selfStatus.ptr), 4, 0u);
     HIP_OK(hipGetLastError());
-    for (uint32_t k = 0; k < substeps; ++k) HIP_OK(hipGraphLaunch(s.graph, s.stream));
+    if (s.useGraphs) {
+        for (uint32_t k = 0; k < substeps; ++k) HIP_OK(hipGraphLaunch(s.graph, s.stream));
+    } else {
+        for (uint32_t k = 0; k < substeps; ++k) s.recordSubstep(iterations, sweeps);
+        HIP_OK(hipGetLastError());
+    }
 }
 
 void readSolver(NumiClothHipSolver& s, const NumiClothHipState& out) {
@@ -637,6 +790,10 @@ void diagnoseSolver(NumiClothHipSolver& s, NumiClothHipDiagnostics* out) {
     const float radius = s.initialConfig.clothMaterial.x;
     k_diag_partial<<<blocks, kBlock, 0, s.stream>>>(s.pos.ptr, s.P, s.dIJ.ptr, s.dMaxLength.ptr, s.D, s.spheres.ptr,
                                                     s.extension.staticSphereCount, radius, s.diagPartials.ptr);
+    if (s.self) {
+        k_self_penetration_into<<<blocks, kBlock, 0, s.stream>>>(s.grid.ptr, s.pairs.ptr, s.dIJ.ptr, s.pos.ptr, radius,
+                                                                 s.diagPartials.ptr);
+    }
     k_diag_final<<<1, kBlock, 0, s.stream>>>(s.diagPartials.ptr, blocks, s.diagResult.ptr);
     HIP_OK(hipGetLastError());
     float4 r{};
@@ -646,6 +803,14 @@ void diagnoseSolver(NumiClothHipSolver& s, NumiClothHipDiagnostics* out) {
     out->maximumStrainViolation = r.x;
     out->maximumGroundPenetration = s.initialConfig.constraintCounts.z != 0u ? r.y : 0.0f;
     out->maximumSpherePenetration = r.z;
+    out->maximumSelfPenetration = r.w;
+    if (s.self) {
+        SelfGrid g{};
+        s.grid.download(&g, 1, s.stream);
+        HIP_OK(hipStreamSynchronize(s.stream));
+        out->selfCandidateCount = g.valid ? g.pairCount : 0u;
+        out->selfCandidateOverflow = g.pairTotal > g.pairCount || g.entryCount > g.sortCount ? 1u : 0u;
+    }
 }
 
 void timePass(NumiClothHipSolver& s, uint32_t pass, uint32_t colour, uint32_t repeats,
@@ -710,9 +875,47 @@ void timePass(NumiClothHipSolver& s, uint32_t pass, uint32_t colour, uint32_t re
     out->itemsPerLaunch = range.count;
 }
 
+void selfBroadPhase(NumiClothHipSolver& s, NumiClothHipSelfBroadPhase* io) {
+    if (!s.self) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "the scene has no hash self-contact");
+    SubstepSlot slot{};
+    slot.config = s.initialConfig;
+    slot.ok = 1u;
+    s.debugSlot.upload(&slot, 1, s.stream);
+    s.launchBroadPhase(&s.debugSlot.ptr->config, &s.debugSlot.ptr->ok);
+    HIP_OK(hipGetLastError());
+    SelfGrid g{};
+    s.grid.download(&g, 1, s.stream);
+    HIP_OK(hipStreamSynchronize(s.stream));
+    io->pairCount = g.valid ? g.pairCount : 0u;
+    io->entryCount = g.valid ? g.sortCount : 0u;
+    io->origin[0] = g.origin.x;
+    io->origin[1] = g.origin.y;
+    io->origin[2] = g.origin.z;
+    io->cell = g.cell;
+    io->cells[0] = g.nx;
+    io->cells[1] = g.ny;
+    io->cells[2] = g.nz;
+    io->valid = g.valid;
+    if (io->pairs != nullptr) {
+        const uint32_t n = std::min(io->pairCount, io->pairCapacity);
+        s.pairs.download(reinterpret_cast<uint2*>(io->pairs), n, s.stream);
+    }
+    // After the four ping-pong radix passes the sorted entries are in A.
+    if (io->cellIds != nullptr) s.keysA.download(io->cellIds, std::min(io->entryCount, io->entryCapacity), s.stream);
+    if (io->segments != nullptr) s.valsA.download(io->segments, std::min(io->entryCount, io->entryCapacity), s.stream);
+    HIP_OK(hipStreamSynchronize(s.stream));
+}
+
 }  // namespace
 
 extern "C" {
+
+int numi_cloth_hip_self_broad_phase(NumiClothHipSolver* solver, NumiClothHipSelfBroadPhase* io) {
+    return guarded([&] {
+        if (solver == nullptr || io == nullptr) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "null argument");
+        selfBroadPhase(*solver, io);
+    });
+}
 
 int numi_cloth_hip_time_pass(NumiClothHipSolver* solver, uint32_t pass, uint32_t colour, uint32_t repeats,
                              NumiClothHipPassTiming* out) {

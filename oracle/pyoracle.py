@@ -19,7 +19,9 @@ class OracleOutput(C.Structure):
     _fields_ = [("positions", C.c_void_p), ("previous", C.c_void_p), ("velocities", C.c_void_p),
                 ("groundImpulse", C.c_void_p), ("distanceLambda", C.c_void_p), ("bendLambda", C.c_void_p),
                 ("knotLambda", C.c_void_p), ("gripLambda", C.c_void_p), ("gripParticle", C.c_void_p),
-                ("failure", C.c_uint32), ("frictionContacts", C.c_uint32), ("maximumConeRatio", C.c_double)]
+                ("failure", C.c_uint32), ("frictionContacts", C.c_uint32), ("maximumConeRatio", C.c_double),
+                ("presentContacts", C.c_uint32), ("sweptContacts", C.c_uint32), ("candidateCount", C.c_uint32),
+                ("reserved", C.c_uint32)]
 
 
 _lib = None
@@ -35,13 +37,19 @@ def library() -> C.CDLL:
         lib.numi_cloth_oracle_run.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
                                               C.POINTER(OracleOutput)]
         lib.numi_cloth_oracle_run.restype = C.c_int
+        lib.numi_cloth_oracle_run_fp32.argtypes = lib.numi_cloth_oracle_run.argtypes
+        lib.numi_cloth_oracle_run_fp32.restype = C.c_int
+        lib.numi_cloth_oracle_self_candidates_fp32.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                                                C.c_uint32]
+        lib.numi_cloth_oracle_self_candidates_fp32.restype = C.c_int64
         _lib = lib
     return _lib
 
 
-def run(scene, configs: np.ndarray, iterations: int, strain_sweeps: int) -> dict:
+def run(scene, configs: np.ndarray, iterations: int, strain_sweeps: int, precision: str = "fp64") -> dict:
     """Run len(configs) substeps of `scene` (an object with .desc and the
-    ABI arrays) in FP64."""
+    ABI arrays) in FP64, or with precision="fp32" as the FP32 twin that
+    evaluates every expression in the device kernels' order."""
     configs = np.ascontiguousarray(configs)
     P = len(scene.particles)
     out = {
@@ -53,11 +61,27 @@ def run(scene, configs: np.ndarray, iterations: int, strain_sweeps: int) -> dict
     result = OracleOutput()
     for name, array in out.items():
         setattr(result, name, array.ctypes.data if array.size else None)
-    status = library().numi_cloth_oracle_run(C.addressof(scene.desc), configs.ctypes.data, len(configs),
-                                             iterations, strain_sweeps, C.byref(result))
+    entry = library().numi_cloth_oracle_run if precision == "fp64" else library().numi_cloth_oracle_run_fp32
+    status = entry(C.addressof(scene.desc), configs.ctypes.data, len(configs), iterations, strain_sweeps,
+                   C.byref(result))
     if status != 0:
         raise RuntimeError(f"oracle run failed with status {status}")
     out["failure"] = int(result.failure)
     out["frictionContacts"] = int(result.frictionContacts)
     out["maximumConeRatio"] = float(result.maximumConeRatio)
+    out["presentContacts"] = int(result.presentContacts)
+    out["sweptContacts"] = int(result.sweptContacts)
+    out["candidateCount"] = int(result.candidateCount)
     return out
+
+
+def self_candidates_fp32(scene, particles: np.ndarray, config: np.ndarray) -> np.ndarray:
+    """Sorted (s, t) candidate pairs of the broad phase on an FP32 snapshot."""
+    particles = np.ascontiguousarray(particles)
+    config = np.ascontiguousarray(config)
+    total = library().numi_cloth_oracle_self_candidates_fp32(C.addressof(scene.desc), particles.ctypes.data,
+                                                             config.ctypes.data, None, 0)
+    pairs = np.zeros((max(total, 1), 2), np.uint32)
+    library().numi_cloth_oracle_self_candidates_fp32(C.addressof(scene.desc), particles.ctypes.data,
+                                                     config.ctypes.data, pairs.ctypes.data, total)
+    return pairs[:total]

@@ -74,6 +74,26 @@ def _sheet(cloth, **overrides):
     return cloth.Scene.sheet(**fields)
 
 
+def _check_frame(oracle, scene, state, configs, iterations, sweeps, label):
+    """GPU vs the FP32 twin (same schedule, same FP32 operation order) and vs
+    FP64. The FP64 gate is 1e-5 of the cloth extent; a stiff pinned sheet
+    amplifies FP32 rounding beyond it, and then the GPU must be as close to
+    FP64 as the exact FP32 restatement is."""
+    gpu = _positions(state)
+    twin = oracle.run(scene, configs, iterations, sweeps, precision="fp32")
+    fp64 = oracle.run(scene, configs, iterations, sweeps)
+    x = scene.particles["positionAndInverseMass"][:, :3]
+    extent = float(np.max(x.max(axis=0) - x.min(axis=0)))
+    e_twin = np.max(np.abs(gpu - twin["positions"]))
+    e_fp64 = np.max(np.abs(gpu - fp64["positions"]))
+    e_twin_fp64 = np.max(np.abs(twin["positions"] - fp64["positions"]))
+    print(f"{label}: |gpu-twin| {e_twin:.3e}  |gpu-fp64| {e_fp64:.3e}  |twin-fp64| {e_twin_fp64:.3e}  "
+          f"gate {1e-5 * extent:.3e}")
+    assert e_twin <= 1e-6 * extent
+    assert e_fp64 <= max(1e-5 * extent, 1.5 * e_twin_fp64)
+    return fp64, e_fp64, extent
+
+
 @pytest.mark.parametrize("overrides", [
     {},
     {"crossingKnots": 1},
@@ -82,15 +102,19 @@ def _sheet(cloth, **overrides):
 def test_sheet_frame_parity(cloth, oracle, overrides):
     scene = _sheet(cloth, **overrides)
     substeps, iterations, sweeps = 32, 16, 3
+    configs = scene.configs(substeps)
     state = _run(cloth, scene, substeps, iterations, sweeps)
-    reference = oracle.run(scene, scene.configs(substeps), iterations, sweeps)
     assert state["failure"] == 0
-    extent = 47 * 0.01
-    error = np.max(np.abs(_positions(state) - reference["positions"]))
-    print(f"sheet {overrides}: max |dx| = {error:.3e} m, gate {1e-5 * extent:.3e}")
-    assert error <= 1.0e-5 * extent
+    reference, error, extent = _check_frame(oracle, scene, state, configs, iterations, sweeps, f"sheet {overrides}")
+    if overrides.get("ground", 1):
+        # Draped on ground and sphere (the benchmark's class): the plain gate.
+        assert error <= 1.0e-5 * extent
     lam = state["distances"]["material"][:, 2].astype(np.float64)
-    assert np.max(np.abs(lam - reference["distanceLambda"])) <= 1e-3 * max(1e-12, np.max(np.abs(reference["distanceLambda"])))
+    # The reference's multiplier gate for its GPU-vs-FP64 check is 2e-8
+    # (reference tools/cloth_bag_metal.mm:7656-7661).
+    assert np.max(np.abs(lam - reference["distanceLambda"])) <= 2e-8
+    bend = state["bends"]["material"][:, 3].astype(np.float64)
+    assert np.max(np.abs(bend - reference["bendLambda"])) <= 2e-8
 
 
 def test_runs_are_bit_identical(cloth):
@@ -122,8 +146,7 @@ def test_kinematic_drive_parity(cloth, oracle):
     scene.damping = 0.5
     scene.refresh()
     state = _run(cloth, scene, 48, 8, 2)
-    reference = oracle.run(scene, scene.configs(48), 8, 2)
-    assert np.max(np.abs(_positions(state) - reference["positions"])) < 1e-5 * 0.23
+    _check_frame(oracle, scene, state, scene.configs(48), 8, 2, "driven strip")
 
 
 def test_diagnostics_equal_host_reduction(cloth):
