@@ -125,15 +125,25 @@ def main() -> int:
     diagnostics = solver.diagnostics()
     self_status = state["selfStatus"]["counters"][0]
 
-    # Dominant kernel: the distance colour pass, timed with HIP events on the
-    # solver's stream right after the timed region (same state, same shape).
-    timing = solver.time_pass(cloth.cloth.PASS_DISTANCE, 0, repeats=200)
+    # Dominant kernel, timed with HIP events on the solver's stream right after
+    # the timed region (same state, same launch shape): the LDS tile program of
+    # an even iteration when the solver runs tiles, else one distance colour.
+    tiles = solver.tile_info()
+    if tiles.enabled:
+        timing = solver.time_pass(cloth.cloth.PASS_TILE_ITERATION, 0, repeats=50)
+        kernel = "k_tile_program (even iteration: distance + bend colours fused in LDS)"
+    else:
+        timing = solver.time_pass(cloth.cloth.PASS_DISTANCE, 0, repeats=200)
+        kernel = "k_distance (one colour batch)"
     achieved = timing.algorithmicBytes / (timing.averageMilliseconds * 1.0e-3) / 1.0e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBPS, "unit": "GB/s",
-                "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": None,
-                "kernel": "k_distance (one colour batch)", "bytes_per_launch": timing.algorithmicBytes,
+                "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": None, "kernel": kernel,
+                "bytes_per_launch": timing.algorithmicBytes,
                 "avg_launch_us": round(timing.averageMilliseconds * 1000.0, 3),
                 "items_per_launch": timing.itemsPerLaunch}
+    tile_config = {"enabled": bool(tiles.enabled), "tiles": tiles.tiles, "owned_per_tile": tiles.ownedPerTile,
+                   "max_loaded": tiles.maxLoaded, "halo_hops": tiles.halo, "workgroup": tiles.workgroupSize,
+                   "passes_fused": tiles.passes, "redundancy": round(tiles.redundancy, 3)}
     solver.close()
 
     baseline = None
@@ -151,7 +161,7 @@ def main() -> int:
                    "bends": len(scene.bends), "substeps_per_frame": substeps, "iterations": ITERATIONS,
                    "strain_sweeps": STRAIN_SWEEPS, "static_spheres": len(scene.staticSpheres),
                    "self_contact": bool(scene.selfContact), "frame_seconds": WORKLOADS[args.workload]["frame"],
-                   "wall_seconds_timed": round(wall, 3)},
+                   "wall_seconds_timed": round(wall, 3), "tile_programs": tile_config},
         "roofline": roofline,
         "cpu_baseline": baseline,
         "diagnostics": {"max_strain_violation": diagnostics.maximumStrainViolation,

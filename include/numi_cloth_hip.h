@@ -176,8 +176,24 @@ enum NumiClothHipPass {
     NUMI_CLOTH_HIP_PASS_DISTANCE = 0,  /* one distance colour batch */
     NUMI_CLOTH_HIP_PASS_BEND = 1,      /* one bend colour batch */
     NUMI_CLOTH_HIP_PASS_STRAIN = 2,    /* one strain-limit colour batch */
-    NUMI_CLOTH_HIP_PASS_COLLIDE = 3    /* ground + static spheres over all knots */
+    NUMI_CLOTH_HIP_PASS_COLLIDE = 3,   /* ground + static spheres over all knots */
+    NUMI_CLOTH_HIP_PASS_TILE_ITERATION = 4 /* LDS tile program of an even iteration */
 };
+
+/* How the solver maps the substep onto the device (chosen at create). */
+typedef struct NumiClothHipTileInfo {
+    uint32_t enabled;          /* 1 when iterations run as LDS tile programs */
+    uint32_t tiles;            /* workgroups per tile-program launch */
+    uint32_t ownedPerTile;     /* knots written back per tile */
+    uint32_t maxLoaded;        /* largest tile + halo held in LDS */
+    uint32_t halo;             /* halo depth in constraint hops */
+    uint32_t workgroupSize;
+    uint32_t passes;           /* colour passes fused per even-iteration launch */
+    uint32_t reserved;
+    double redundancy;         /* constraint solves per launch / unique ones */
+} NumiClothHipTileInfo;
+
+int numi_cloth_hip_tile_info(NumiClothHipSolver* solver, NumiClothHipTileInfo* out);
 
 typedef struct NumiClothHipPassTiming {
     double averageMilliseconds;        /* per launch, from HIP events */

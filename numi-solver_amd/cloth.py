@@ -123,7 +123,13 @@ class PassTiming(C.Structure):
                 ("itemsPerLaunch", C.c_uint32)]
 
 
-PASS_DISTANCE, PASS_BEND, PASS_STRAIN, PASS_COLLIDE = 0, 1, 2, 3
+PASS_DISTANCE, PASS_BEND, PASS_STRAIN, PASS_COLLIDE, PASS_TILE_ITERATION = 0, 1, 2, 3, 4
+
+
+class TileInfo(C.Structure):
+    _fields_ = [("enabled", C.c_uint32), ("tiles", C.c_uint32), ("ownedPerTile", C.c_uint32),
+                ("maxLoaded", C.c_uint32), ("halo", C.c_uint32), ("workgroupSize", C.c_uint32),
+                ("passes", C.c_uint32), ("reserved", C.c_uint32), ("redundancy", C.c_double)]
 
 
 class MaterialValues(C.Structure):
@@ -196,6 +202,7 @@ def library() -> C.CDLL:
         "numi_cloth_hip_destroy": ([vp], i32),
         "numi_cloth_hip_time_pass": ([vp, u32, u32, u32, C.POINTER(PassTiming)], i32),
         "numi_cloth_hip_self_broad_phase": ([vp, C.POINTER(SelfBroadPhase)], i32),
+        "numi_cloth_hip_tile_info": ([vp, C.POINTER(TileInfo)], i32),
         "numi_cloth_material_defaults": ([C.POINTER(MaterialArtifact)], i32),
         "numi_cloth_material_load": ([C.c_char_p, C.POINTER(MaterialArtifact)], i32),
         "numi_cloth_grip_trajectory_load": ([C.c_char_p, C.POINTER(vp)], i32),
@@ -474,6 +481,11 @@ class Solver:
         return {"pairs": pairs[:io.pairCount], "cellIds": cells[:io.entryCount],
                 "segments": segments[:io.entryCount], "origin": np.array(io.origin[:], np.float32),
                 "cell": np.float32(io.cell), "cells": tuple(io.cells), "valid": io.valid}
+
+    def tile_info(self) -> TileInfo:
+        out = TileInfo()
+        _check(library().numi_cloth_hip_tile_info(self._handle, C.byref(out)))
+        return out
 
     def diagnostics(self) -> Diagnostics:
         out = Diagnostics()

@@ -109,8 +109,37 @@ __global__ void k_advance(const NumiClothBagGPUConfig* __restrict__ cfg, const u
 
 // ------------------------------------------------------------ distances --
 
-// solve_distance (reference src/metal/ClothBag.metal:1148-1213) over one
-// colour batch [first, first + count).
+// One XPBD distance projection (reference solve_distance,
+// src/metal/ClothBag.metal:1148-1213) on values already loaded: updates a, b
+// and lambda in place; returns false when nothing is written.
+__device__ __forceinline__ bool projectDistance(float4& a, float4& b, float2 rc, float& lam, float dt,
+                                                uint32_t* failure) {
+    const V3 d = xyz(b) - xyz(a);
+    const float len = length(d);
+    const float alpha = rc.y / (dt * dt);
+    const float denom = a.w + b.w + alpha;
+    if (!isfinite(len) || !isfinite(alpha) || !isfinite(denom) || !isfinite(rc.x) || !isfinite(lam)) {
+        recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
+        return false;
+    }
+    if (!(len > 1.0e-12f) || !(denom > 0.0f)) return false;
+    const float value = len - rc.x;
+    const float dl = (-value - alpha * lam) / denom;
+    const V3 corr = d * (dl / len);
+    const float nl = lam + dl;
+    const V3 na = xyz(a) - corr * a.w;
+    const V3 nb = xyz(b) + corr * b.w;
+    if (!finite3(na) || !finite3(nb) || !isfinite(nl)) {
+        recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
+        return false;
+    }
+    a = withXyz(a, na);
+    b = withXyz(b, nb);
+    lam = nl;
+    return true;
+}
+
+// solve_distance over one colour batch [first, first + count).
 __global__ void k_distance(const NumiClothBagGPUConfig* __restrict__ cfg, const uint32_t* __restrict__ ok,
                            uint32_t first, uint32_t count, float4* __restrict__ pos,
                            const uint2* __restrict__ ij, const float2* __restrict__ restCompliance,
@@ -119,36 +148,34 @@ __global__ void k_distance(const NumiClothBagGPUConfig* __restrict__ cfg, const 
     if (t >= count || *ok == 0u) return;
     const uint32_t c = first + t;
     const uint2 e = ij[c];
-    const float2 rc = restCompliance[c];
     float lam = lambda[c];
     float4 a = pos[e.x];
     float4 b = pos[e.y];
-    const V3 d = xyz(b) - xyz(a);
-    const float len = length(d);
-    const float dt = cfg->gravityAndTimestep.w;
-    const float alpha = rc.y / (dt * dt);
-    const float denom = a.w + b.w + alpha;
-    if (!isfinite(len) || !isfinite(alpha) || !isfinite(denom) || !isfinite(rc.x) || !isfinite(lam)) {
-        recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
-        return;
-    }
-    if (!(len > 1.0e-12f) || !(denom > 0.0f)) return;
-    const float value = len - rc.x;
-    const float dl = (-value - alpha * lam) / denom;
-    const V3 corr = d * (dl / len);
-    lam += dl;
-    const V3 na = xyz(a) - corr * a.w;
-    const V3 nb = xyz(b) + corr * b.w;
-    if (!finite3(na) || !finite3(nb) || !isfinite(lam)) {
-        recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
-        return;
-    }
-    pos[e.x] = withXyz(a, na);
-    pos[e.y] = withXyz(b, nb);
+    if (!projectDistance(a, b, restCompliance[c], lam, cfg->gravityAndTimestep.w, failure)) return;
+    pos[e.x] = a;
+    pos[e.y] = b;
     lambda[c] = lam;
 }
 
-// limit_strain (reference src/metal/ClothBag.metal:1616-1673) over one colour.
+// One unilateral strain limit (reference limit_strain,
+// src/metal/ClothBag.metal:1616-1673) on loaded values.
+__device__ __forceinline__ bool limitStrainPair(float4& a, float4& b, float limit, uint32_t* failure) {
+    const V3 d = xyz(b) - xyz(a);
+    const float len = length(d);
+    if (!isfinite(len) || !isfinite(limit)) {
+        recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
+        return false;
+    }
+    if (!(len > limit) || !(len > 1.0e-12f)) return false;
+    const float denom = a.w + b.w;
+    if (!(denom > 0.0f) || !isfinite(denom)) return false;
+    const float excess = len - limit;
+    const V3 dir = d / len;
+    a = withXyz(a, xyz(a) + dir * (a.w * excess / denom));
+    b = withXyz(b, xyz(b) - dir * (b.w * excess / denom));
+    return true;
+}
+
 __global__ void k_strain(const uint32_t* __restrict__ ok, uint32_t first, uint32_t count,
                          float4* __restrict__ pos, const uint2* __restrict__ ij,
                          const float* __restrict__ maxLength, uint32_t* __restrict__ failure) {
@@ -156,48 +183,28 @@ __global__ void k_strain(const uint32_t* __restrict__ ok, uint32_t first, uint32
     if (t >= count || *ok == 0u) return;
     const uint32_t c = first + t;
     const uint2 e = ij[c];
-    const float limit = maxLength[c];
-    const float4 a = pos[e.x];
-    const float4 b = pos[e.y];
-    const V3 d = xyz(b) - xyz(a);
-    const float len = length(d);
-    if (!isfinite(len) || !isfinite(limit)) {
-        recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
-        return;
-    }
-    if (!(len > limit) || !(len > 1.0e-12f)) return;
-    const float denom = a.w + b.w;
-    if (!(denom > 0.0f) || !isfinite(denom)) return;
-    const float excess = len - limit;
-    const V3 dir = d / len;
-    pos[e.x] = withXyz(a, xyz(a) + dir * (a.w * excess / denom));
-    pos[e.y] = withXyz(b, xyz(b) - dir * (b.w * excess / denom));
+    float4 a = pos[e.x];
+    float4 b = pos[e.y];
+    if (!limitStrainPair(a, b, maxLength[c], failure)) return;
+    pos[e.x] = a;
+    pos[e.y] = b;
 }
 
 // ----------------------------------------------------------------- knots --
 
-// solve_knot (reference src/metal/ClothBag.metal:1215-1313): crossing-angle
-// cosine constraint on the warp and weft chords through a knot.
-__global__ void k_knot(const NumiClothBagGPUConfig* __restrict__ cfg, const uint32_t* __restrict__ ok,
-                       uint32_t first, uint32_t count, float4* __restrict__ pos,
-                       const uint4* __restrict__ quad, const float2* __restrict__ cosineCompliance,
-                       float* __restrict__ lambda, uint32_t* __restrict__ failure) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= count || *ok == 0u) return;
-    const uint32_t c = first + t;
-    const uint4 q = quad[c];
-    const float2 cc = cosineCompliance[c];
-    float lam = lambda[c];
-    float4 p0 = pos[q.x], p1 = pos[q.y], p2 = pos[q.z], p3 = pos[q.w];
+// One crossing-angle cosine constraint on the warp and weft chords through a
+// knot (reference solve_knot, src/metal/ClothBag.metal:1215-1313).
+__device__ __forceinline__ bool projectKnot(float4& p0, float4& p1, float4& p2, float4& p3, float2 cc, float& lam,
+                                            float dt, uint32_t* failure) {
     const V3 warpV = xyz(p1) - xyz(p0);
     const V3 weftV = xyz(p3) - xyz(p2);
     const float warpL = length(warpV);
     const float weftL = length(weftV);
     if (!isfinite(warpL) || !isfinite(weftL) || !isfinite(cc.x) || !isfinite(cc.y) || !isfinite(lam)) {
         recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
-        return;
+        return false;
     }
-    if (!(warpL > 1.0e-12f) || !(weftL > 1.0e-12f)) return;
+    if (!(warpL > 1.0e-12f) || !(weftL > 1.0e-12f)) return false;
     const V3 warp = warpV / warpL;
     const V3 weft = weftV / weftL;
     const float cosine = clampf(dot(warp, weft), -1.0f, 1.0f);
@@ -206,66 +213,71 @@ __global__ void k_knot(const NumiClothBagGPUConfig* __restrict__ cfg, const uint
     const V3 gWeft = (warp - weft * cosine) / weftL;
     const V3 g[4] = {-gWarp, gWarp, -gWeft, gWeft};
     const float w[4] = {p0.w, p1.w, p2.w, p3.w};
-    const float dt = cfg->gravityAndTimestep.w;
     const float alpha = cc.y / (dt * dt);
     float denom = alpha;
     for (int k = 0; k < 4; ++k) denom += w[k] * dot(g[k], g[k]);
-    if (!(denom > 0.0f) || !isfinite(denom)) return;
+    if (!(denom > 0.0f) || !isfinite(denom)) return false;
     const float dl = (-value - alpha * lam) / denom;
-    lam += dl;
+    const float nl = lam + dl;
     const V3 n0 = xyz(p0) + g[0] * (w[0] * dl);
     const V3 n1 = xyz(p1) + g[1] * (w[1] * dl);
     const V3 n2 = xyz(p2) + g[2] * (w[2] * dl);
     const V3 n3 = xyz(p3) + g[3] * (w[3] * dl);
-    if (!finite3(n0) || !finite3(n1) || !finite3(n2) || !finite3(n3) || !isfinite(lam)) {
+    if (!finite3(n0) || !finite3(n1) || !finite3(n2) || !finite3(n3) || !isfinite(nl)) {
         recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
-        return;
+        return false;
     }
-    pos[q.x] = withXyz(p0, n0);
-    pos[q.y] = withXyz(p1, n1);
-    pos[q.z] = withXyz(p2, n2);
-    pos[q.w] = withXyz(p3, n3);
+    p0 = withXyz(p0, n0);
+    p1 = withXyz(p1, n1);
+    p2 = withXyz(p2, n2);
+    p3 = withXyz(p3, n3);
+    lam = nl;
+    return true;
+}
+
+__global__ void k_knot(const NumiClothBagGPUConfig* __restrict__ cfg, const uint32_t* __restrict__ ok,
+                       uint32_t first, uint32_t count, float4* __restrict__ pos,
+                       const uint4* __restrict__ quad, const float2* __restrict__ cosineCompliance,
+                       float* __restrict__ lambda, uint32_t* __restrict__ failure) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count || *ok == 0u) return;
+    const uint32_t c = first + t;
+    const uint4 q = quad[c];
+    float lam = lambda[c];
+    float4 p0 = pos[q.x], p1 = pos[q.y], p2 = pos[q.z], p3 = pos[q.w];
+    if (!projectKnot(p0, p1, p2, p3, cosineCompliance[c], lam, cfg->gravityAndTimestep.w, failure)) return;
+    pos[q.x] = p0;
+    pos[q.y] = p1;
+    pos[q.z] = p2;
+    pos[q.w] = p3;
     lambda[c] = lam;
 }
 
 // ----------------------------------------------------------------- bends --
 
-// solve_bend (reference src/metal/ClothBag.metal:1315-1447): chord-length
-// XPBD between the outer knots with a ground active set (a grounded endpoint
-// that would be pushed down keeps its height) and a step fraction that stops
-// free endpoints at the ground.
-__global__ void k_bend(const NumiClothBagGPUConfig* __restrict__ cfg, const uint32_t* __restrict__ ok,
-                       uint32_t first, uint32_t count, float4* __restrict__ pos,
-                       const uint2* __restrict__ ends, const float2* __restrict__ chordCompliance,
-                       float* __restrict__ lambda, uint32_t* __restrict__ failure) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= count || *ok == 0u) return;
-    const uint32_t c = first + t;
-    const uint2 e = ends[c];
-    const float2 cc = chordCompliance[c];
-    float lam = lambda[c];
-    float4 a = pos[e.x];
-    float4 b = pos[e.y];
+// One chord bend (reference solve_bend, src/metal/ClothBag.metal:1315-1447):
+// chord-length XPBD between the outer knots with a ground active set (a
+// grounded endpoint that would be pushed down keeps its height) and a step
+// fraction that stops free endpoints at the ground.
+__device__ __forceinline__ bool projectBend(float4& a, float4& b, float2 cc, float& lam, float dt, bool ground,
+                                            float floorZ, uint32_t* failure) {
     const V3 d = xyz(b) - xyz(a);
     const float chord = length(d);
     if (!isfinite(chord) || !isfinite(cc.x) || !isfinite(cc.y) || !isfinite(lam)) {
         recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
-        return;
+        return false;
     }
-    if (!(chord > 1.0e-12f)) return;
+    if (!(chord > 1.0e-12f)) return false;
     const float value = chord - cc.x;
     const V3 dir = d / chord;
     const V3 g[2] = {-dir, dir};
     const float w[2] = {a.w, b.w};
-    const float dt = cfg->gravityAndTimestep.w;
     const float alpha = cc.y / (dt * dt);
     float freeDenom = alpha;
     for (int k = 0; k < 2; ++k) freeDenom += w[k] * dot(g[k], g[k]);
-    if (!(freeDenom >= 1.0e-16f) || !isfinite(freeDenom)) return;
+    if (!(freeDenom >= 1.0e-16f) || !isfinite(freeDenom)) return false;
     const float numerator = -value - alpha * lam;
     const float freeDl = numerator / freeDenom;
-    const bool ground = cfg->constraintCounts.z != 0u;
-    const float floorZ = cfg->clothMaterial.x;
     const float z[2] = {a.z, b.z};
     bool active[2];
     float denom = alpha;
@@ -273,7 +285,7 @@ __global__ void k_bend(const NumiClothBagGPUConfig* __restrict__ cfg, const uint
         active[k] = ground && z[k] <= floorZ + 1.0e-9f && g[k].z * freeDl < 0.0f;
         denom += w[k] * (dot(g[k], g[k]) - (active[k] ? g[k].z * g[k].z : 0.0f));
     }
-    if (!(denom >= 1.0e-16f) || !isfinite(denom)) return;
+    if (!(denom >= 1.0e-16f) || !isfinite(denom)) return false;
     const float dl = numerator / denom;
     float fraction = 1.0f;
     if (ground) {
@@ -284,7 +296,7 @@ __global__ void k_bend(const NumiClothBagGPUConfig* __restrict__ cfg, const uint
         }
     }
     const float applied = dl * fraction;
-    lam += applied;
+    const float nl = lam + applied;
     V3 ca = g[0] * (w[0] * applied);
     V3 cb = g[1] * (w[1] * applied);
     if (active[0]) ca.z = 0.0f;
@@ -295,12 +307,33 @@ __global__ void k_bend(const NumiClothBagGPUConfig* __restrict__ cfg, const uint
         na.z = fmaxf(na.z, floorZ);
         nb.z = fmaxf(nb.z, floorZ);
     }
-    if (!finite3(na) || !finite3(nb) || !isfinite(lam)) {
+    if (!finite3(na) || !finite3(nb) || !isfinite(nl)) {
         recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
+        return false;
+    }
+    a = withXyz(a, na);
+    b = withXyz(b, nb);
+    lam = nl;
+    return true;
+}
+
+__global__ void k_bend(const NumiClothBagGPUConfig* __restrict__ cfg, const uint32_t* __restrict__ ok,
+                       uint32_t first, uint32_t count, float4* __restrict__ pos,
+                       const uint2* __restrict__ ends, const float2* __restrict__ chordCompliance,
+                       float* __restrict__ lambda, uint32_t* __restrict__ failure) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count || *ok == 0u) return;
+    const uint32_t c = first + t;
+    const uint2 e = ends[c];
+    float lam = lambda[c];
+    float4 a = pos[e.x];
+    float4 b = pos[e.y];
+    if (!projectBend(a, b, chordCompliance[c], lam, cfg->gravityAndTimestep.w, cfg->constraintCounts.z != 0u,
+                     cfg->clothMaterial.x, failure)) {
         return;
     }
-    pos[e.x] = withXyz(a, na);
-    pos[e.y] = withXyz(b, nb);
+    pos[e.x] = a;
+    pos[e.y] = b;
     lambda[c] = lam;
 }
 
@@ -428,7 +461,30 @@ __global__ void k_grip(const NumiClothBagGPUConfig* __restrict__ cfg, const uint
 // Cloth part of solve_ground (reference src/metal/ClothBag.metal:1575-1614)
 // followed by static sphere contact, which the reference does not have
 // (its spheres must be dynamic fruits): a dynamic knot inside a sphere grown
-// by the yarn radius is moved radially onto its surface.
+// by the yarn radius is moved radially onto its surface. Returns true when
+// the knot moved.
+__device__ __forceinline__ bool collideKnot(float4& p, bool ground, float r, const float4* spheres,
+                                            uint32_t sphereCount) {
+    bool changed = false;
+    if (ground && !(p.z >= r)) {
+        p.z = fmaxf(p.z, r);
+        changed = true;
+    }
+    if (p.w > 0.0f) {
+        for (uint32_t k = 0; k < sphereCount; ++k) {
+            const float4 s = spheres[k];
+            const V3 d = xyz(p) - xyz(s);
+            const float dist = length(d);
+            const float reach = s.w + r;
+            if (dist < reach && dist > 1.0e-12f) {
+                p = withXyz(p, xyz(s) + d * (reach / dist));
+                changed = true;
+            }
+        }
+    }
+    return changed;
+}
+
 __global__ void k_collide(const NumiClothBagGPUConfig* __restrict__ cfg, const uint32_t* __restrict__ ok,
                           float4* __restrict__ pos, uint32_t particles, const float4* __restrict__ spheres,
                           uint32_t sphereCount, uint32_t* __restrict__ failure) {
@@ -442,25 +498,7 @@ __global__ void k_collide(const NumiClothBagGPUConfig* __restrict__ cfg, const u
         recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
         return;
     }
-    const float r = cfg->clothMaterial.x;
-    bool changed = false;
-    if (cfg->constraintCounts.z != 0u && !(p.z >= r)) {
-        p.z = fmaxf(p.z, r);
-        changed = true;
-    }
-    if (p.w > 0.0f) {
-        for (uint32_t k = 0; k < sphereCount; ++k) {
-            const float4 s = sphereCache[k];
-            const V3 d = xyz(p) - xyz(s);
-            const float dist = length(d);
-            const float reach = s.w + r;
-            if (dist < reach && dist > 1.0e-12f) {
-                p = withXyz(p, xyz(s) + d * (reach / dist));
-                changed = true;
-            }
-        }
-    }
-    if (changed) pos[i] = p;
+    if (collideKnot(p, cfg->constraintCounts.z != 0u, cfg->clothMaterial.x, sphereCache, sphereCount)) pos[i] = p;
 }
 
 // ------------------------------------------------------------- finalize --

@@ -20,9 +20,9 @@
 // size; each segment is entered once per cell its box covers, (cell id,
 // segment) entries are sorted by a stable LSD radix sort, and a pair is
 // reported only from its owner cell, the cell of max(lo_s, lo_t), so each
-// pair appears exactly once. Entries and pairs are written through
-// count / exclusive scan / emit passes, so the lists are identical on every
-// run.
+// pair appears exactly once, as (min, max). Entries and pairs are written
+// through count / exclusive scan / emit passes, so the lists are identical
+// on every run.
 //
 // Narrow phase (reference solve_self_contact arithmetic, present and swept
 // modes) runs on all candidates in parallel from one position snapshot
@@ -115,17 +115,40 @@ __device__ __forceinline__ int cellOf(float v, float origin, float cell) {
     return static_cast<int>(floorf((v - origin) / cell));
 }
 
-// One workgroup: the grid origin and extent from the per-block box bounds.
+// One workgroup: the grid origin and extent from the per-block box bounds
+// (min/max are exact, so the reduction order does not matter).
 __global__ void k_self_grid(const float4* __restrict__ partialLo, const float4* __restrict__ partialHi,
                             uint32_t blocks, const NumiClothBagGPUConfig* __restrict__ cfg,
                             const uint32_t* __restrict__ ok, uint32_t segments, SelfGrid* __restrict__ grid,
                             uint32_t* __restrict__ failure) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    float4 lo = partialLo[0], hi = partialHi[0];
-    for (uint32_t b = 1; b < blocks; ++b) {
-        const float4 l = partialLo[b], h = partialHi[b];
-        lo = make_float4(fminf(lo.x, l.x), fminf(lo.y, l.y), fminf(lo.z, l.z), 0.0f);
-        hi = make_float4(fmaxf(hi.x, h.x), fmaxf(hi.y, h.y), fmaxf(hi.z, h.z), 0.0f);
+    __shared__ float4 waveLo[kScanThreads / 64];
+    __shared__ float4 waveHi[kScanThreads / 64];
+    float l[3] = {INFINITY, INFINITY, INFINITY};
+    float h[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t b = threadIdx.x; b < blocks; b += blockDim.x) {
+        const float4 a = partialLo[b], c = partialHi[b];
+        l[0] = fminf(l[0], a.x);
+        l[1] = fminf(l[1], a.y);
+        l[2] = fminf(l[2], a.z);
+        h[0] = fmaxf(h[0], c.x);
+        h[1] = fmaxf(h[1], c.y);
+        h[2] = fmaxf(h[2], c.z);
+    }
+    for (int k = 0; k < 3; ++k)
+        for (int offset = 32; offset > 0; offset >>= 1) {
+            l[k] = fminf(l[k], __shfl_xor(l[k], offset));
+            h[k] = fmaxf(h[k], __shfl_xor(h[k], offset));
+        }
+    if ((threadIdx.x & 63u) == 0) {
+        waveLo[threadIdx.x >> 6] = make_float4(l[0], l[1], l[2], 0.0f);
+        waveHi[threadIdx.x >> 6] = make_float4(h[0], h[1], h[2], 0.0f);
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    float4 lo = waveLo[0], hi = waveHi[0];
+    for (uint32_t w = 1; w < blockDim.x / 64; ++w) {
+        lo = make_float4(fminf(lo.x, waveLo[w].x), fminf(lo.y, waveLo[w].y), fminf(lo.z, waveLo[w].z), 0.0f);
+        hi = make_float4(fmaxf(hi.x, waveHi[w].x), fmaxf(hi.y, waveHi[w].y), fmaxf(hi.z, waveHi[w].z), 0.0f);
     }
     const float cell = cfg->clothMaterial.y;
     SelfGrid g{};
@@ -240,44 +263,37 @@ __device__ __forceinline__ bool boxesOverlap(const float4& aLo, const float4& aH
            aHi.z >= bLo.z;
 }
 
-// Visits the candidates of segment s in a fixed order (cells z, y, x; entries
-// in sorted order); `emit` is called with each partner t.
+// Candidates of sorted entry k = (cell, s): the later entries (cell, t) of
+// the same cell run, kept when the boxes overlap, this cell is the pair's
+// owner cell (the cell of max(lo_s, lo_t)) and the segments are not local.
+// Each pair is therefore visited exactly once, from one entry, and comes out
+// as (min, max). Runs are short (the segments whose grown boxes cover one
+// cell), so the work is spread over all entries without searching.
 template <typename Emit>
-__device__ void forEachCandidate(uint32_t s, const SelfGrid& g, const float4* __restrict__ boxLo,
-                                 const float4* __restrict__ boxHi, const uint2* __restrict__ ij,
-                                 const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
-                                 const uint32_t* __restrict__ localStart, const uint32_t* __restrict__ localList,
-                                 Emit emit) {
-    const float4 lo = boxLo[s], hi = boxHi[s];
-    const CellRange r = cellRange(lo, hi, g);
-    if (r.count == 0) return;
-    const uint2 es = ij[s];
+__device__ void forEntryCandidates(uint32_t k, const SelfGrid& g, const float4* __restrict__ boxLo,
+                                   const float4* __restrict__ boxHi, const uint2* __restrict__ ij,
+                                   const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                                   const uint32_t* __restrict__ localStart, const uint32_t* __restrict__ localList,
+                                   Emit emit) {
     const uint32_t n = g.sortCount;
-    for (int z = r.lo[2]; z <= r.hi[2]; ++z) {
-        for (int y = r.lo[1]; y <= r.hi[1]; ++y) {
-            for (int x = r.lo[0]; x <= r.hi[0]; ++x) {
-                const uint32_t id = cellId(x, y, z, g);
-                uint32_t first = 0, last = n;
-                while (first < last) {
-                    const uint32_t mid = first + (last - first) / 2;
-                    if (keys[mid] < id) first = mid + 1;
-                    else last = mid;
-                }
-                for (uint32_t k = first; k < n && keys[k] == id; ++k) {
-                    const uint32_t t = vals[k];
-                    if (t <= s) continue;
-                    const float4 tLo = boxLo[t], tHi = boxHi[t];
-                    if (!boxesOverlap(lo, hi, tLo, tHi)) continue;
-                    if (cellOf(fmaxf(lo.x, tLo.x), g.origin.x, g.cell) != x ||
-                        cellOf(fmaxf(lo.y, tLo.y), g.origin.y, g.cell) != y ||
-                        cellOf(fmaxf(lo.z, tLo.z), g.origin.z, g.cell) != z) {
-                        continue;
-                    }
-                    if (localPair(es, ij[t], localStart, localList)) continue;
-                    emit(t);
-                }
-            }
+    const uint32_t id = keys[k];
+    const uint32_t s = vals[k];
+    const float4 lo = boxLo[s], hi = boxHi[s];
+    const uint2 es = ij[s];
+    const uint32_t x = id % g.nx;
+    const uint32_t y = (id / g.nx) % g.ny;
+    const uint32_t z = id / (g.nx * g.ny);
+    for (uint32_t j = k + 1; j < n && keys[j] == id; ++j) {
+        const uint32_t t = vals[j];
+        const float4 tLo = boxLo[t], tHi = boxHi[t];
+        if (!boxesOverlap(lo, hi, tLo, tHi)) continue;
+        if (cellOf(fmaxf(lo.x, tLo.x), g.origin.x, g.cell) != static_cast<int>(x) ||
+            cellOf(fmaxf(lo.y, tLo.y), g.origin.y, g.cell) != static_cast<int>(y) ||
+            cellOf(fmaxf(lo.z, tLo.z), g.origin.z, g.cell) != static_cast<int>(z)) {
+            continue;
         }
+        if (localPair(es, ij[t], localStart, localList)) continue;
+        emit(min(s, t), max(s, t));
     }
 }
 
@@ -287,11 +303,11 @@ __global__ void k_self_pair_counts(const float4* __restrict__ boxLo, const float
                                    const uint32_t* __restrict__ localStart, const uint32_t* __restrict__ localList,
                                    uint32_t* __restrict__ counts) {
     const SelfGrid g = *grid;
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= g.segments) return;
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!g.valid || k >= g.sortCount) return;
     uint32_t n = 0;
-    if (g.valid) forEachCandidate(s, g, boxLo, boxHi, ij, keys, vals, localStart, localList, [&](uint32_t) { ++n; });
-    counts[s] = n;
+    forEntryCandidates(k, g, boxLo, boxHi, ij, keys, vals, localStart, localList, [&](uint32_t, uint32_t) { ++n; });
+    counts[k] = n;
 }
 
 __global__ void k_self_emit_pairs(const float4* __restrict__ boxLo, const float4* __restrict__ boxHi,
@@ -301,16 +317,17 @@ __global__ void k_self_emit_pairs(const float4* __restrict__ boxLo, const float4
                                   const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ pairTotal,
                                   uint32_t capacity, uint2* __restrict__ pairs, uint32_t* __restrict__ failure) {
     const SelfGrid g = *grid;
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s == 0) {
-        grid->pairTotal = *pairTotal;
-        grid->pairCount = min(*pairTotal, capacity);
-        if (*pairTotal > capacity) recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_RANGE);
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k == 0) {
+        const uint32_t total = g.valid ? *pairTotal : 0u;
+        grid->pairTotal = total;
+        grid->pairCount = min(total, capacity);
+        if (total > capacity) recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_RANGE);
     }
-    if (s >= g.segments || !g.valid) return;
-    uint32_t out = offsets[s];
-    forEachCandidate(s, g, boxLo, boxHi, ij, keys, vals, localStart, localList, [&](uint32_t t) {
-        if (out < capacity) pairs[out] = make_uint2(s, t);
+    if (!g.valid || k >= g.sortCount) return;
+    uint32_t out = offsets[k];
+    forEntryCandidates(k, g, boxLo, boxHi, ij, keys, vals, localStart, localList, [&](uint32_t a, uint32_t b) {
+        if (out < capacity) pairs[out] = make_uint2(a, b);
         ++out;
     });
 }

@@ -24,6 +24,8 @@
 #include "kernels_core.hpp"
 #include "kernels_diag.hpp"
 #include "kernels_self.hpp"
+#include "kernels_tile.hpp"
+#include "tile_plan.hpp"
 #include "numi_cloth_hip.h"
 
 using namespace numi_hip;
@@ -121,11 +123,34 @@ struct NumiClothHipSolver {
     DeviceArray<uint32_t> localStart, localList;
     DeviceArray<float4> boxLo, boxHi, partialLo, partialHi;
     DeviceArray<SelfGrid> grid;
-    DeviceArray<uint32_t> segmentCounts, scanSums, totals, constants;
+    DeviceArray<uint32_t> segmentCounts, entryCounts, scanSums, totals, constants;
     DeviceArray<uint32_t> keysA, valsA, keysB, valsB, histogram;
     DeviceArray<uint2> pairs;
     DeviceArray<FixedAccumulator> accumulator;
     DeviceArray<SubstepSlot> debugSlot;
+
+    // LDS tile programs (NUMI_CLOTH_HIP_TILES=0 keeps the per-colour launches).
+    struct TileProgram {
+        bool valid = false;
+        uint32_t tiles = 0, passes = 0, maxLoaded = 0, block = 256, owned = 0, halo = 0;
+        bool hasDistance = false, hasBend = false, hasKnot = false;
+        std::vector<uint32_t> kinds;
+        DeviceArray<uint32_t> loadStart, loadIds, ownedCount, itemStart;
+        DeviceArray<uint4> items;
+        double algorithmicBytes = 0.0;
+        uint64_t computed = 0, unique = 0;
+    };
+    bool useTiles = false;
+    TileProgram tileEven, tileOdd, tileStrain;
+    uint32_t tileOwned = 0, tileBlock = 256, tileCapacity = 0, tileStrainSweeps = ~0u;
+    std::vector<float4> initialPositions;  // create-time layout used to order tiles
+    DeviceArray<float4> posAlt;
+    DeviceArray<float> dLambdaAlt, bLambdaAlt, kLambdaAlt;
+    // Buffers holding the live state while a substep is recorded.
+    float4* posCur = nullptr;
+    float* dLamCur = nullptr;
+    float* bLamCur = nullptr;
+    float* kLamCur = nullptr;
 
     // Graphs are the default; NUMI_CLOTH_HIP_GRAPHS=0 launches the same
     // kernels directly on the stream (used under profilers that do not
@@ -159,13 +184,13 @@ struct NumiClothHipSolver {
         BeginArgs a{};
         a.cfg = cfg();
         a.ok = ok();
-        a.pos = pos.ptr;
+        a.pos = posCur;
         a.prev = prev.ptr;
         a.vel = vel.ptr;
         a.anchor = driven ? anchor.ptr : nullptr;
-        a.distanceLambda = dLambda.ptr;
-        a.bendLambda = bLambda.ptr;
-        a.knotLambda = kLambda.ptr;
+        a.distanceLambda = dLamCur;
+        a.bendLambda = bLamCur;
+        a.knotLambda = kLamCur;
         a.grips = grips.ptr;
         a.particles = P;
         a.distances = D;
@@ -182,29 +207,29 @@ struct NumiClothHipSolver {
 
     void launchDistances() {
         for (const Range& r : distanceColours) {
-            k_distance<<<blocksFor(r.count), kBlock, 0, stream>>>(cfg(), ok(), r.first, r.count, pos.ptr, dIJ.ptr,
-                                                                  dRestCompliance.ptr, dLambda.ptr, failure.ptr);
+            k_distance<<<blocksFor(r.count), kBlock, 0, stream>>>(cfg(), ok(), r.first, r.count, posCur, dIJ.ptr,
+                                                                  dRestCompliance.ptr, dLamCur, failure.ptr);
         }
     }
 
     void launchKnots() {
         for (const Range& r : knotColours) {
-            k_knot<<<blocksFor(r.count), kBlock, 0, stream>>>(cfg(), ok(), r.first, r.count, pos.ptr, kQuad.ptr,
-                                                              kCosineCompliance.ptr, kLambda.ptr, failure.ptr);
+            k_knot<<<blocksFor(r.count), kBlock, 0, stream>>>(cfg(), ok(), r.first, r.count, posCur, kQuad.ptr,
+                                                              kCosineCompliance.ptr, kLamCur, failure.ptr);
         }
     }
 
     void launchBends() {
         for (const Range& r : bendColours) {
-            k_bend<<<blocksFor(r.count), kBlock, 0, stream>>>(cfg(), ok(), r.first, r.count, pos.ptr, bEnds.ptr,
-                                                              bChordCompliance.ptr, bLambda.ptr, failure.ptr);
+            k_bend<<<blocksFor(r.count), kBlock, 0, stream>>>(cfg(), ok(), r.first, r.count, posCur, bEnds.ptr,
+                                                              bChordCompliance.ptr, bLamCur, failure.ptr);
         }
     }
 
     void launchStrain(uint32_t sweeps) {
         for (uint32_t sweep = 0; sweep < sweeps; ++sweep) {
             for (const Range& r : distanceColours) {
-                k_strain<<<blocksFor(r.count), kBlock, 0, stream>>>(ok(), r.first, r.count, pos.ptr, dIJ.ptr,
+                k_strain<<<blocksFor(r.count), kBlock, 0, stream>>>(ok(), r.first, r.count, posCur, dIJ.ptr,
                                                                     dMaxLength.ptr, failure.ptr);
             }
         }
@@ -217,7 +242,7 @@ struct NumiClothHipSolver {
     void launchCollide() {
         if (!groundOrColliders()) return;
         const uint32_t n = extension.staticSphereCount;
-        k_collide<<<blocksFor(P), kBlock, n * sizeof(float4), stream>>>(cfg(), ok(), pos.ptr, P, spheres.ptr, n,
+        k_collide<<<blocksFor(P), kBlock, n * sizeof(float4), stream>>>(cfg(), ok(), posCur, P, spheres.ptr, n,
                                                                         failure.ptr);
     }
 
@@ -233,10 +258,10 @@ struct NumiClothHipSolver {
     void launchBroadPhase(const NumiClothBagGPUConfig* config, const uint32_t* valid) {
         const float margin = extension.selfContactMargin;
         SelfGrid* g = grid.ptr;
-        k_self_boxes<<<segmentBlocks, kScanThreads, 0, stream>>>(config, margin, pos.ptr, prev.ptr, dIJ.ptr, D,
+        k_self_boxes<<<segmentBlocks, kScanThreads, 0, stream>>>(config, margin, posCur, prev.ptr, dIJ.ptr, D,
                                                                  boxLo.ptr, boxHi.ptr, partialLo.ptr,
                                                                  partialHi.ptr);
-        k_self_grid<<<1, 64, 0, stream>>>(partialLo.ptr, partialHi.ptr, segmentBlocks, config, valid, D, g,
+        k_self_grid<<<1, kScanThreads, 0, stream>>>(partialLo.ptr, partialHi.ptr, segmentBlocks, config, valid, D, g,
                                           failure.ptr);
         k_self_cell_counts<<<segmentBlocks, kScanThreads, 0, stream>>>(boxLo.ptr, boxHi.ptr, g, segmentCounts.ptr,
                                                                        failure.ptr);
@@ -257,32 +282,96 @@ struct NumiClothHipSolver {
             std::swap(keysIn, keysOut);
             std::swap(valsIn, valsOut);
         }
-        k_self_pair_counts<<<segmentBlocks, kScanThreads, 0, stream>>>(boxLo.ptr, boxHi.ptr, dIJ.ptr, g, keysIn,
-                                                                       valsIn, localStart.ptr, localList.ptr,
-                                                                       segmentCounts.ptr);
-        launchScan(segmentCounts.ptr, &g->segments, D, totals.ptr + 1);
-        k_self_emit_pairs<<<segmentBlocks, kScanThreads, 0, stream>>>(boxLo.ptr, boxHi.ptr, dIJ.ptr, g, keysIn,
-                                                                      valsIn, localStart.ptr, localList.ptr,
-                                                                      segmentCounts.ptr, totals.ptr + 1,
-                                                                      pairCapacity, pairs.ptr, failure.ptr);
+        const uint32_t entryBlocks = (entryCapacity + kScanThreads - 1) / kScanThreads;
+        k_self_pair_counts<<<entryBlocks, kScanThreads, 0, stream>>>(boxLo.ptr, boxHi.ptr, dIJ.ptr, g, keysIn,
+                                                                     valsIn, localStart.ptr, localList.ptr,
+                                                                     entryCounts.ptr);
+        launchScan(entryCounts.ptr, &g->sortCount, entryCapacity, totals.ptr + 1);
+        k_self_emit_pairs<<<entryBlocks, kScanThreads, 0, stream>>>(boxLo.ptr, boxHi.ptr, dIJ.ptr, g, keysIn,
+                                                                    valsIn, localStart.ptr, localList.ptr,
+                                                                    entryCounts.ptr, totals.ptr + 1, pairCapacity,
+                                                                    pairs.ptr, failure.ptr);
     }
 
     void launchSelf(uint32_t mode) {
         if (!self) return;
         const uint32_t blocks = std::min<uint32_t>(2048, std::max(1u, (pairCapacity + kBlock - 1) / kBlock));
-        k_self_solve<<<blocks, kBlock, 0, stream>>>(cfg(), ok(), mode, grid.ptr, pairs.ptr, dIJ.ptr, pos.ptr,
+        k_self_solve<<<blocks, kBlock, 0, stream>>>(cfg(), ok(), mode, grid.ptr, pairs.ptr, dIJ.ptr, posCur,
                                                      prev.ptr, accumulator.ptr, selfStatus.ptr);
-        k_self_apply<<<blocksFor(P), kBlock, 0, stream>>>(ok(), pos.ptr, accumulator.ptr, P, failure.ptr);
+        k_self_apply<<<blocksFor(P), kBlock, 0, stream>>>(ok(), posCur, accumulator.ptr, P, failure.ptr);
+    }
+
+    void launchTileProgram(TileProgram& t, uint32_t trailingCollides, const NumiClothBagGPUConfig* config,
+                           const uint32_t* valid, bool scratch = false) {
+        TileProgramArgs a{};
+        a.cfg = config;
+        a.ok = valid;
+        a.posIn = posCur;
+        a.posOut = posCur == pos.ptr ? posAlt.ptr : pos.ptr;
+        a.loadStart = t.loadStart.ptr;
+        a.loadIds = t.loadIds.ptr;
+        a.ownedCount = t.ownedCount.ptr;
+        a.itemStart = t.itemStart.ptr;
+        a.items = t.items.ptr;
+        a.distanceRestCompliance = dRestCompliance.ptr;
+        a.distanceMaxLength = dMaxLength.ptr;
+        a.distanceLambdaIn = dLamCur;
+        a.distanceLambdaOut = dLamCur == dLambda.ptr ? dLambdaAlt.ptr : dLambda.ptr;
+        a.bendChordCompliance = bChordCompliance.ptr;
+        a.bendLambdaIn = bLamCur;
+        a.bendLambdaOut = bLamCur == bLambda.ptr ? bLambdaAlt.ptr : bLambda.ptr;
+        a.knotCosineCompliance = kCosineCompliance.ptr;
+        a.knotLambdaIn = kLamCur;
+        a.knotLambdaOut = kLamCur == kLambda.ptr ? kLambdaAlt.ptr : kLambda.ptr;
+        a.spheres = spheres.ptr;
+        a.sphereCount = extension.staticSphereCount;
+        a.passes = t.passes;
+        a.trailingCollides = groundOrColliders() ? trailingCollides : 0u;
+        a.failure = failure.ptr;
+        for (uint32_t k = 0; k < t.passes; ++k) a.kinds[k] = t.kinds[k];
+        const size_t shared = (t.maxLoaded + (a.trailingCollides ? a.sphereCount : 0u)) * sizeof(float4);
+        k_tile_program<<<t.tiles, t.block, shared, stream>>>(a);
+        if (scratch) return;
+        posCur = a.posOut;
+        if (t.hasDistance) dLamCur = a.distanceLambdaOut;
+        if (t.hasBend) bLamCur = a.bendLambdaOut;
+        if (t.hasKnot) kLamCur = a.knotLambdaOut;
+    }
+
+    // Moves the live state back to the primary buffers at the end of a substep.
+    void restorePrimaryBuffers() {
+        if (posCur != pos.ptr) k_copy_float4<<<blocksFor(P), kBlock, 0, stream>>>(posCur, pos.ptr, P);
+        if (D != 0 && dLamCur != dLambda.ptr) k_copy_float<<<blocksFor(D), kBlock, 0, stream>>>(dLamCur, dLambda.ptr, D);
+        if (B != 0 && bLamCur != bLambda.ptr) k_copy_float<<<blocksFor(B), kBlock, 0, stream>>>(bLamCur, bLambda.ptr, B);
+        if (K != 0 && kLamCur != kLambda.ptr) k_copy_float<<<blocksFor(K), kBlock, 0, stream>>>(kLamCur, kLambda.ptr, K);
+        posCur = pos.ptr;
+        dLamCur = dLambda.ptr;
+        bLamCur = bLambda.ptr;
+        kLamCur = kLambda.ptr;
+    }
+
+    // Strain sweeps followed by `collides` ground/sphere passes.
+    void launchStrainAndCollide(uint32_t sweeps, uint32_t collides) {
+        if (useTiles && sweeps != 0 && tileStrain.valid) {
+            launchTileProgram(tileStrain, collides, cfg(), ok());
+            return;
+        }
+        launchStrain(sweeps);
+        for (uint32_t k = 0; k < collides; ++k) launchCollide();
     }
 
     void launchGrips() {
         if (G == 0) return;
-        k_grip<<<blocksFor(G), kBlock, 0, stream>>>(cfg(), ok(), pos.ptr, grips.ptr, G, P, failure.ptr);
+        k_grip<<<blocksFor(G), kBlock, 0, stream>>>(cfg(), ok(), posCur, grips.ptr, G, P, failure.ptr);
     }
 
     // The substep in the reference's dispatch order
     // (tools/cloth_bag_metal.mm:4211-4709), families without records omitted.
     void recordSubstep(uint32_t iterations, uint32_t sweeps) {
+        posCur = pos.ptr;
+        dLamCur = dLambda.ptr;
+        bLamCur = bLambda.ptr;
+        kLamCur = kLambda.ptr;
         k_prepare<<<1, 64, 0, stream>>>(configs.ptr, drive.ptr, slot.ptr, failure.ptr);
         const float damping = extension.damping;
         // The damping factor depends on dt; dt is constant over a scene's
@@ -292,42 +381,46 @@ struct NumiClothHipSolver {
         const bool attach = G != 0;
         launchBegin(!attach, factor);
         if (attach) {
-            k_grip_attach<<<blocksFor(G), kBlock, 0, stream>>>(cfg(), ok(), pos.ptr, grips.ptr, G, dCuff.ptr, P,
+            k_grip_attach<<<blocksFor(G), kBlock, 0, stream>>>(cfg(), ok(), posCur, grips.ptr, G, dCuff.ptr, P,
                                                                failure.ptr);
-            k_advance<<<blocksFor(P), kBlock, 0, stream>>>(cfg(), ok(), pos.ptr, vel.ptr, P);
+            k_advance<<<blocksFor(P), kBlock, 0, stream>>>(cfg(), ok(), posCur, vel.ptr, P);
         }
         if (self) {
             launchBroadPhase(cfg(), ok());
             launchSelf(1);
         }
         for (uint32_t it = 0; it < iterations; ++it) {
-            launchDistances();
-            launchKnots();
-            if (it % 2 == 0) launchBends();
+            TileProgram& program = it % 2 == 0 ? tileEven : tileOdd;
+            if (useTiles && program.valid) {
+                launchTileProgram(program, 0, cfg(), ok());
+            } else {
+                launchDistances();
+                launchKnots();
+                if (it % 2 == 0) launchBends();
+            }
             launchGrips();
             launchSelf(0);
             launchCollide();
         }
-        launchStrain(sweeps);
+        launchStrainAndCollide(sweeps, 0);
         if (sweeps != 0) {
-            for (uint32_t pass = 0; pass < kReconciliationPasses; ++pass) {
-                launchStrain(sweeps);
-                launchCollide();
-            }
+            for (uint32_t pass = 0; pass < kReconciliationPasses; ++pass) launchStrainAndCollide(sweeps, 1);
             launchSelf(0);
-            launchStrain(sweeps);
-            for (uint32_t pass = 0; pass < kFinalContactPasses; ++pass) launchCollide();
+            launchStrainAndCollide(sweeps, kFinalContactPasses);
             for (uint32_t certificate = 0; certificate < kCertificatePasses; ++certificate) {
                 launchSelf(0);
-                launchStrain(sweeps);
-                for (uint32_t pass = 0; pass < kFinalContactPasses; ++pass) launchCollide();
+                launchStrainAndCollide(sweeps, kFinalContactPasses);
             }
         }
+        restorePrimaryBuffers();
         k_finalize<<<blocksFor(P), kBlock, 0, stream>>>(cfg(), ok(), pos.ptr, prev.ptr, vel.ptr, P,
                                                         frictionStatus.ptr, failure.ptr);
     }
 
+    void ensureStrainProgram(uint32_t sweeps);
+
     void buildGraph(uint32_t iterations, uint32_t sweeps) {
+        ensureStrainProgram(sweeps);
         if (!useGraphs) return;
         if (graph != nullptr && graphIterations == iterations && graphSweeps == sweeps) return;
         if (graph != nullptr) {
@@ -420,6 +513,8 @@ void checkColouring(const std::vector<Range>& ranges, uint32_t particles, Partic
 }
 
 void setupSelfContact(const NumiClothHipSceneDesc& scene, NumiClothHipSolver& s);
+void setupTiles(NumiClothHipSolver& s);
+double timeTileProgram(NumiClothHipSolver& s, NumiClothHipSolver::TileProgram& t, uint32_t repeats);
 
 void createSolver(const NumiClothHipSceneDesc& scene, NumiClothHipSolver& s) {
     if (scene.config == nullptr) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "scene config is null");
@@ -594,7 +689,16 @@ void createSolver(const NumiClothHipSceneDesc& scene, NumiClothHipSolver& s) {
     HIP_OK(hipMemsetAsync(s.releaseStatus.ptr, 0, sizeof(NumiClothBagGPUReleaseStatus), st));
     s.diagPartials.allocate(1024);
     if (const char* graphs = std::getenv("NUMI_CLOTH_HIP_GRAPHS")) s.useGraphs = std::strcmp(graphs, "0") != 0;
+    s.posCur = s.pos.ptr;
+    s.dLamCur = s.dLambda.ptr;
+    s.bLamCur = s.bLambda.ptr;
+    s.kLamCur = s.kLambda.ptr;
+    s.debugSlot.allocate(1);
+    s.initialPositions = hp;
     if (scene.extension.selfContact != 0) setupSelfContact(scene, s);
+    bool tiles = true;
+    if (const char* env = std::getenv("NUMI_CLOTH_HIP_TILES")) tiles = std::strcmp(env, "0") != 0;
+    if (tiles) setupTiles(s);
     s.diagResult.allocate(1);
     s.ensureStaging(64);
     HIP_OK(hipStreamSynchronize(st));
@@ -648,7 +752,8 @@ void setupSelfContact(const NumiClothHipSceneDesc& scene, NumiClothHipSolver& s)
     s.partialHi.allocate(s.segmentBlocks);
     s.grid.allocate(1);
     s.segmentCounts.allocate(s.D);
-    const uint32_t maxScanLength = std::max({s.D, kRadixBins * s.entryTiles, 1u});
+    s.entryCounts.allocate(s.entryCapacity);
+    const uint32_t maxScanLength = std::max({s.D, kRadixBins * s.entryTiles, s.entryCapacity, 1u});
     s.scanSums.allocate((maxScanLength + kScanTile - 1) / kScanTile + 1);
     s.totals.allocate(4);
     s.constants.allocate(1);
@@ -663,9 +768,172 @@ void setupSelfContact(const NumiClothHipSceneDesc& scene, NumiClothHipSolver& s)
     s.accumulator.allocate(s.P);
     HIP_OK(hipMemsetAsync(s.accumulator.ptr, 0, s.P * sizeof(FixedAccumulator), st));
     HIP_OK(hipMemsetAsync(s.grid.ptr, 0, sizeof(SelfGrid), st));
-    s.debugSlot.allocate(1);
     HIP_OK(hipStreamSynchronize(st));
 }
+
+// ---- tile programs -------------------------------------------------------
+
+TileTopology tileTopology(const NumiClothHipSolver& s, std::vector<uint32_t>& dPairs, std::vector<uint32_t>& bPairs,
+                          std::vector<uint32_t>& kQuads) {
+    dPairs.resize(2 * s.D);
+    for (uint32_t k = 0; k < s.D; ++k) {
+        dPairs[2 * k] = s.hostDistances[k].particlesAndColor.x;
+        dPairs[2 * k + 1] = s.hostDistances[k].particlesAndColor.y;
+    }
+    bPairs.resize(2 * s.B);
+    for (uint32_t k = 0; k < s.B; ++k) {
+        bPairs[2 * k] = s.hostBends[k].particlesAndColor.x;
+        bPairs[2 * k + 1] = s.hostBends[k].particlesAndColor.z;
+    }
+    kQuads.resize(4 * s.K);
+    for (uint32_t k = 0; k < s.K; ++k) {
+        const mr_uint4 q = s.hostKnots[k].particles;
+        kQuads[4 * k] = q.x;
+        kQuads[4 * k + 1] = q.y;
+        kQuads[4 * k + 2] = q.z;
+        kQuads[4 * k + 3] = q.w;
+    }
+    TileTopology t;
+    t.particles = s.P;
+    t.distancePairs = dPairs.data();
+    t.bendPairs = bPairs.data();
+    t.knotQuads = kQuads.data();
+    t.positions = reinterpret_cast<const float*>(s.initialPositions.data());
+    return t;
+}
+
+// The passes of one tile program: an iteration (distance, knot and, on even
+// iterations, bend colours, in the reference's order) or a strain-limit
+// block of `strainSweeps` sweeps over the distance colours.
+std::vector<TilePass> tilePasses(const NumiClothHipSolver& s, bool bends, uint32_t strainSweeps) {
+    std::vector<TilePass> passes;
+    if (strainSweeps != 0) {
+        for (uint32_t sweep = 0; sweep < strainSweeps; ++sweep)
+            for (const Range& r : s.distanceColours) passes.push_back({kTileStrain, r.first, r.count});
+        return passes;
+    }
+    for (const Range& r : s.distanceColours) passes.push_back({kTileDistance, r.first, r.count});
+    for (const Range& r : s.knotColours) passes.push_back({kTileKnot, r.first, r.count});
+    if (bends)
+        for (const Range& r : s.bendColours) passes.push_back({kTileBend, r.first, r.count});
+    return passes;
+}
+
+bool buildTileProgram(NumiClothHipSolver& s, const std::vector<TilePass>& passes, uint32_t owned,
+                      NumiClothHipSolver::TileProgram& out) {
+    out = NumiClothHipSolver::TileProgram{};
+    if (passes.empty() || passes.size() > kMaxTilePasses) return false;
+    std::vector<uint32_t> dPairs, bPairs, kQuads;
+    const TileTopology topo = tileTopology(s, dPairs, bPairs, kQuads);
+    TilePlan plan;
+    if (!planTiles(topo, passes, owned, s.tileCapacity, 32, &plan)) return false;
+    out.valid = true;
+    out.tiles = plan.tiles;
+    out.passes = static_cast<uint32_t>(passes.size());
+    out.maxLoaded = plan.maxLoaded;
+    out.owned = owned;
+    out.halo = plan.halo;
+    out.computed = plan.computed;
+    out.unique = plan.unique;
+    out.block = s.tileBlock;
+    double bytes = s.P * (16.0 + 12.0);
+    bool strain = false;
+    for (const TilePass& pass : passes) {
+        out.kinds.push_back(pass.kind);
+        out.hasDistance |= pass.kind == kTileDistance;
+        out.hasBend |= pass.kind == kTileBend;
+        out.hasKnot |= pass.kind == kTileKnot;
+        strain |= pass.kind == kTileStrain;
+    }
+    // Unique bytes one launch must move: knots read (xyzw) and written (xyz)
+    // once, each constraint's static record and multiplier once.
+    if (out.hasDistance) bytes += s.D * (20.0 + 8.0);
+    if (out.hasBend) bytes += s.B * (16.0 + 8.0);
+    if (out.hasKnot) bytes += s.K * (24.0 + 8.0);
+    if (strain) bytes += s.D * 12.0;
+    out.algorithmicBytes = bytes;
+    hipStream_t st = s.stream;
+    out.loadStart.allocate(plan.loadStart.size());
+    out.loadStart.upload(plan.loadStart.data(), plan.loadStart.size(), st);
+    out.loadIds.allocate(plan.loadIds.size());
+    out.loadIds.upload(plan.loadIds.data(), plan.loadIds.size(), st);
+    out.ownedCount.allocate(plan.ownedCount.size());
+    out.ownedCount.upload(plan.ownedCount.data(), plan.ownedCount.size(), st);
+    out.itemStart.allocate(plan.itemStart.size());
+    out.itemStart.upload(plan.itemStart.data(), plan.itemStart.size(), st);
+    out.items.allocate(plan.items.size() / 4);
+    out.items.upload(reinterpret_cast<const uint4*>(plan.items.data()), plan.items.size() / 4, st);
+    HIP_OK(hipStreamSynchronize(st));
+    return true;
+}
+
+// Launch time of one tile program writing to the alternate buffers only, so
+// the state is not changed.
+double timeTileProgram(NumiClothHipSolver& s, NumiClothHipSolver::TileProgram& t, uint32_t repeats) {
+    hipEvent_t start = nullptr, stop = nullptr;
+    HIP_OK(hipEventCreate(&start));
+    HIP_OK(hipEventCreate(&stop));
+    const NumiClothBagGPUConfig* config = &s.debugSlot.ptr->config;
+    const uint32_t* valid = &s.debugSlot.ptr->ok;
+    s.launchTileProgram(t, 0, config, valid, true);
+    HIP_OK(hipEventRecord(start, s.stream));
+    for (uint32_t k = 0; k < repeats; ++k) s.launchTileProgram(t, 0, config, valid, true);
+    HIP_OK(hipEventRecord(stop, s.stream));
+    HIP_OK(hipEventSynchronize(stop));
+    HIP_OK(hipGetLastError());
+    float ms = 0.0f;
+    HIP_OK(hipEventElapsedTime(&ms, start, stop));
+    (void)hipEventDestroy(start);
+    (void)hipEventDestroy(stop);
+    return ms / repeats;
+}
+
+// Autotuner: picks the tile size (owned knots per workgroup) and the
+// workgroup size by timing the even-iteration program, the most frequent
+// launch of a substep, over a small candidate grid.
+void setupTiles(NumiClothHipSolver& s) {
+    if (s.D == 0 && s.B == 0 && s.K == 0) return;
+    int maxShared = 0;
+    HIP_OK(hipDeviceGetAttribute(&maxShared, hipDeviceAttributeMaxSharedMemoryPerBlock, s.device));
+    const uint32_t spheres = s.extension.staticSphereCount;
+    if (static_cast<uint64_t>(spheres + 64) * sizeof(float4) >= static_cast<uint64_t>(maxShared)) return;
+    s.tileCapacity = static_cast<uint32_t>(maxShared / sizeof(float4)) - spheres;
+    s.posAlt.allocate(s.P);
+    s.dLambdaAlt.allocate(s.D);
+    s.bLambdaAlt.allocate(s.B);
+    s.kLambdaAlt.allocate(s.K);
+    SubstepSlot slot{};
+    slot.config = s.initialConfig;
+    slot.ok = 1u;
+    s.debugSlot.upload(&slot, 1, s.stream);
+    const std::vector<TilePass> even = tilePasses(s, true, 0);
+    double best = 1e30;
+    uint32_t bestOwned = 0, bestBlock = 256;
+    std::vector<uint32_t> sizes;
+    if (s.P <= 4096) sizes.push_back(s.P);
+    for (uint32_t owned : {4096u, 2048u, 1024u, 512u, 256u})
+        if (owned < s.P) sizes.push_back(owned);
+    for (uint32_t owned : sizes) {
+        NumiClothHipSolver::TileProgram candidate;
+        if (!buildTileProgram(s, even, owned, candidate)) continue;
+        for (uint32_t block : {256u, 512u, 1024u}) {
+            candidate.block = block;
+            const double ms = timeTileProgram(s, candidate, 5);
+            if (ms < best) {
+                best = ms;
+                bestOwned = owned;
+                bestBlock = block;
+            }
+        }
+    }
+    if (bestOwned == 0) return;
+    s.tileOwned = bestOwned;
+    s.tileBlock = bestBlock;
+    if (!buildTileProgram(s, even, bestOwned, s.tileEven)) return;
+    buildTileProgram(s, tilePasses(s, false, 0), bestOwned, s.tileOdd);
+    s.useTiles = true;
+}
+
 
 // The reference lets a trajectory change only grip target/orientation/
 // generation/selection mode and materials (tools/cloth_bag_metal.mm:4044-4090).
@@ -830,8 +1098,16 @@ void timePass(NumiClothHipSolver& s, uint32_t pass, uint32_t colour, uint32_t re
     } else if (pass == NUMI_CLOTH_HIP_PASS_STRAIN) {
         colours = &s.distanceColours;
         perItem = 48.0;
-    } else if (pass != NUMI_CLOTH_HIP_PASS_COLLIDE) {
+    } else if (pass != NUMI_CLOTH_HIP_PASS_COLLIDE && pass != NUMI_CLOTH_HIP_PASS_TILE_ITERATION) {
         fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "unknown pass");
+    }
+    if (pass == NUMI_CLOTH_HIP_PASS_TILE_ITERATION) {
+        if (!s.useTiles || !s.tileEven.valid) fail(NUMI_CLOTH_HIP_ERROR_UNSUPPORTED, "no tile program on this solver");
+        out->averageMilliseconds = timeTileProgram(s, s.tileEven, repeats);
+        out->algorithmicBytes = s.tileEven.algorithmicBytes;
+        out->launches = repeats;
+        out->itemsPerLaunch = static_cast<uint32_t>(s.tileEven.unique);
+        return;
     }
     Range range{0, s.P};
     if (colours != nullptr) {
@@ -908,12 +1184,41 @@ void selfBroadPhase(NumiClothHipSolver& s, NumiClothHipSelfBroadPhase* io) {
 
 }  // namespace
 
+void NumiClothHipSolver::ensureStrainProgram(uint32_t sweeps) {
+    if (!useTiles || sweeps == tileStrainSweeps) return;
+    tileStrainSweeps = sweeps;
+    if (sweeps == 0 || !buildTileProgram(*this, tilePasses(*this, false, sweeps), tileOwned, tileStrain)) {
+        tileStrain.valid = false;
+    }
+    if (graph != nullptr) {
+        (void)hipGraphExecDestroy(graph);
+        graph = nullptr;
+    }
+}
+
 extern "C" {
 
 int numi_cloth_hip_self_broad_phase(NumiClothHipSolver* solver, NumiClothHipSelfBroadPhase* io) {
     return guarded([&] {
         if (solver == nullptr || io == nullptr) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "null argument");
         selfBroadPhase(*solver, io);
+    });
+}
+
+int numi_cloth_hip_tile_info(NumiClothHipSolver* solver, NumiClothHipTileInfo* out) {
+    return guarded([&] {
+        if (solver == nullptr || out == nullptr) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "null argument");
+        std::memset(out, 0, sizeof(*out));
+        const auto& t = solver->tileEven;
+        out->enabled = solver->useTiles && t.valid ? 1u : 0u;
+        if (!out->enabled) return;
+        out->tiles = t.tiles;
+        out->ownedPerTile = t.owned;
+        out->maxLoaded = t.maxLoaded;
+        out->halo = t.halo;
+        out->workgroupSize = t.block;
+        out->passes = t.passes;
+        out->redundancy = t.unique != 0 ? static_cast<double>(t.computed) / static_cast<double>(t.unique) : 0.0;
     });
 }
 
