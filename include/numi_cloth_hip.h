@@ -125,6 +125,9 @@ typedef struct NumiClothHipState {
     NumiClothBagGPUAerodynamicsStatus* aerodynamicsStatus;
     NumiClothBagGPUReleaseStatus* releaseStatus;
     uint32_t* failure;
+    /* Per fruit: largest mouth-release clearance seen since create, metres
+     * (reference maximumMouthClearanceBits, tools/cloth_bag_metal.mm:7585-7591). */
+    float* mouthClearance;
 } NumiClothHipState;
 
 /* Residual diagnostics, reduced on the device without float atomics

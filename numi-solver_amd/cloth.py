@@ -94,7 +94,8 @@ class SceneDesc(C.Structure):
 class State(C.Structure):
     _fields_ = [(name, C.c_void_p) for name in
                 ["particles", "distances", "grips", "knots", "bends", "fruits", "fruitPairs", "yarnContacts",
-                 "selfStatus", "frictionStatus", "aerodynamicsStatus", "releaseStatus", "failure"]]
+                 "selfStatus", "frictionStatus", "aerodynamicsStatus", "releaseStatus", "failure",
+                 "mouthClearance"]]
 
 
 class Diagnostics(C.Structure):
@@ -454,6 +455,7 @@ class Solver:
             "aerodynamicsStatus": np.zeros(1, AERO_STATUS),
             "releaseStatus": np.zeros(1, RELEASE_STATUS),
             "failure": np.zeros(1, np.uint32),
+            "mouthClearance": np.zeros(len(s.fruits), np.float32),
         }
         state = State()
         for name, array in out.items():

@@ -25,6 +25,7 @@
 #include "kernels_diag.hpp"
 #include "kernels_self.hpp"
 #include "kernels_tile.hpp"
+#include "kernels_bag.hpp"
 #include "tile_plan.hpp"
 #include "numi_cloth_hip.h"
 
@@ -116,6 +117,18 @@ struct NumiClothHipSolver {
     DeviceArray<NumiClothBagGPUAerodynamicsStatus> aeroStatus;
     DeviceArray<NumiClothBagGPUReleaseStatus> releaseStatus;
     DeviceArray<float4> diagPartials, diagResult;
+
+    // Produce-bag families (fruits, fruit pairs, fruit/yarn records, air).
+    uint32_t F = 0, FP = 0, Y = 0;
+    bool yarnAir = false;
+    std::vector<Range> pairColours;
+    std::vector<uint32_t> rimHost;
+    DeviceArray<NumiClothBagGPUFruit> fruits;
+    DeviceArray<NumiClothBagGPUFruitPair> fruitPairs;
+    DeviceArray<NumiClothBagGPUYarnContact> yarnContacts;
+    DeviceArray<uint4> distanceBatchesDevice;
+    DeviceArray<float4> aeroForces, aeroReduction;
+    DeviceArray<uint32_t> rim, mouthClearance;
 
     // Hash self-contact (present when extension.selfContact != 0).
     bool self = false;
@@ -237,6 +250,41 @@ struct NumiClothHipSolver {
 
     bool groundOrColliders() const {
         return initialConfig.constraintCounts.z != 0u || extension.staticSphereCount != 0u;
+    }
+
+    // ---- bag families (no-ops when the scene has no fruits) ----
+    uint32_t fruitLanes() const { return std::max(64u, (F + 63u) / 64u * 64u); }
+
+    void launchFruitPairs() {
+        for (const Range& r : pairColours)
+            k_fruit_pair<<<blocksFor(r.count), kBlock, 0, stream>>>(cfg(), ok(), r.first, r.count, fruits.ptr,
+                                                                    fruitPairs.ptr, failure.ptr);
+    }
+
+    void launchYarn(uint32_t mode) {
+        if (Y == 0) return;
+        k_yarn_batches<<<1, std::max(256u, fruitLanes()), P * sizeof(float4), stream>>>(
+            cfg(), ok(), mode, distanceBatchesDevice.ptr, static_cast<uint32_t>(distanceColours.size()), posCur, P,
+            dIJ.ptr, D, fruits.ptr, F, yarnContacts.ptr, failure.ptr);
+    }
+
+    void launchBuildYarnContacts() {
+        if (Y == 0) return;
+        k_yarn_contacts_build<<<blocksFor(Y), kBlock, 0, stream>>>(cfg(), ok(), posCur, prev.ptr, dIJ.ptr, D,
+                                                                   fruits.ptr, F, yarnContacts.ptr, failure.ptr);
+    }
+
+    void launchFruitGround() {
+        if (F == 0 || initialConfig.constraintCounts.z == 0u) return;
+        k_fruit_ground<<<blocksFor(F), kBlock, 0, stream>>>(cfg(), ok(), fruits.ptr, F, failure.ptr);
+    }
+
+    // solveFinalContacts of the reference: fruit pairs, fruit/yarn, ground.
+    void launchFinalContacts() {
+        launchFruitPairs();
+        launchYarn(0);
+        launchCollide();
+        launchFruitGround();
     }
 
     void launchCollide() {
@@ -379,16 +427,38 @@ struct NumiClothHipSolver {
         const float dt = initialConfig.gravityAndTimestep.w;
         const float factor = damping > 0.0f ? 1.0f / (1.0f + damping * dt) : 1.0f;
         const bool attach = G != 0;
-        launchBegin(!attach, factor);
+        const bool separateAdvance = attach || yarnAir || F != 0;
+        launchBegin(!separateAdvance, factor);
+        if (F != 0) {
+            k_fruit_begin<<<blocksFor(std::max({F, FP, Y})), kBlock, 0, stream>>>(
+                cfg(), ok(), fruits.ptr, F, fruitPairs.ptr, FP, yarnContacts.ptr, Y, failure.ptr);
+        }
         if (attach) {
             k_grip_attach<<<blocksFor(G), kBlock, 0, stream>>>(cfg(), ok(), posCur, grips.ptr, G, dCuff.ptr, P,
                                                                failure.ptr);
+        }
+        if (yarnAir) {
+            k_aero_clear<<<blocksFor(P), kBlock, 0, stream>>>(ok(), aeroForces.ptr, P);
+            for (const Range& r : distanceColours)
+                k_aero_accumulate<<<blocksFor(r.count), kBlock, 0, stream>>>(cfg(), ok(), r.first, r.count, posCur,
+                                                                             vel.ptr, dIJ.ptr, aeroForces.ptr,
+                                                                             aeroStatus.ptr, failure.ptr);
+            k_aero_reduce<<<1, 256, 0, stream>>>(cfg(), ok(), posCur, vel.ptr, aeroForces.ptr, P, aeroReduction.ptr);
+            k_aero_apply<<<blocksFor(P), kBlock, 0, stream>>>(cfg(), ok(), posCur, vel.ptr, aeroForces.ptr,
+                                                              aeroReduction.ptr, P, failure.ptr);
+        }
+        if (F != 0) k_fruit_aero<<<blocksFor(F), kBlock, 0, stream>>>(cfg(), ok(), fruits.ptr, F, aeroStatus.ptr,
+                                                                      failure.ptr);
+        if (separateAdvance) {
             k_advance<<<blocksFor(P), kBlock, 0, stream>>>(cfg(), ok(), posCur, vel.ptr, P);
+            if (F != 0) k_fruit_advance<<<blocksFor(F), kBlock, 0, stream>>>(cfg(), ok(), fruits.ptr, F);
         }
         if (self) {
             launchBroadPhase(cfg(), ok());
             launchSelf(1);
         }
+        launchBuildYarnContacts();
+        launchYarn(1);
         for (uint32_t it = 0; it < iterations; ++it) {
             TileProgram& program = it % 2 == 0 ? tileEven : tileOdd;
             if (useTiles && program.valid) {
@@ -399,22 +469,57 @@ struct NumiClothHipSolver {
                 if (it % 2 == 0) launchBends();
             }
             launchGrips();
+            launchFruitPairs();
+            launchYarn(0);
             launchSelf(0);
             launchCollide();
+            launchFruitGround();
         }
         launchStrainAndCollide(sweeps, 0);
         if (sweeps != 0) {
-            for (uint32_t pass = 0; pass < kReconciliationPasses; ++pass) launchStrainAndCollide(sweeps, 1);
-            launchSelf(0);
-            launchStrainAndCollide(sweeps, kFinalContactPasses);
-            for (uint32_t certificate = 0; certificate < kCertificatePasses; ++certificate) {
+            // Reconciliation, endpoint and certificate passes
+            // (reference tools/cloth_bag_metal.mm:4557-4590).
+            for (uint32_t pass = 0; pass < kReconciliationPasses; ++pass) {
+                launchFruitPairs();
+                launchYarn(0);
+                launchStrainAndCollide(sweeps, 1);
+                launchFruitGround();
+            }
+            for (uint32_t round = 0; round <= kCertificatePasses; ++round) {
                 launchSelf(0);
-                launchStrainAndCollide(sweeps, kFinalContactPasses);
+                if (F == 0) {
+                    // Without fruits the final contact passes are ground
+                    // passes only and fuse into the strain program.
+                    launchStrainAndCollide(sweeps, kFinalContactPasses);
+                } else {
+                    launchStrainAndCollide(sweeps, 0);
+                    for (uint32_t pass = 0; pass < kFinalContactPasses; ++pass) launchFinalContacts();
+                }
             }
         }
         restorePrimaryBuffers();
+        launchBuildYarnContacts();
         k_finalize<<<blocksFor(P), kBlock, 0, stream>>>(cfg(), ok(), pos.ptr, prev.ptr, vel.ptr, P,
                                                         frictionStatus.ptr, failure.ptr);
+        if (F != 0) {
+            k_fruit_finalize<<<blocksFor(F), kBlock, 0, stream>>>(cfg(), ok(), fruits.ptr, F, failure.ptr);
+            if (Y != 0) {
+                k_yarn_friction<<<1, std::max(256u, fruitLanes()), P * sizeof(float4), stream>>>(
+                    cfg(), ok(), distanceBatchesDevice.ptr, static_cast<uint32_t>(distanceColours.size()), pos.ptr,
+                    vel.ptr, P, dIJ.ptr, D, fruits.ptr, F, yarnContacts.ptr, frictionStatus.ptr, failure.ptr);
+            }
+            for (const Range& r : pairColours)
+                k_fruit_pair_friction<<<blocksFor(r.count), kBlock, 0, stream>>>(cfg(), ok(), r.first, r.count,
+                                                                                 fruits.ptr, fruitPairs.ptr,
+                                                                                 frictionStatus.ptr, failure.ptr);
+            k_fruit_ground_friction<<<blocksFor(F), kBlock, 0, stream>>>(cfg(), ok(), fruits.ptr, F,
+                                                                         frictionStatus.ptr, failure.ptr);
+            k_fruit_orientation<<<blocksFor(F), kBlock, 0, stream>>>(cfg(), ok(), fruits.ptr, F, failure.ptr);
+            if (!rimHost.empty()) {
+                k_release<<<1, fruitLanes(), 0, stream>>>(cfg(), ok(), pos.ptr, rim.ptr, fruits.ptr, F,
+                                                          releaseStatus.ptr, mouthClearance.ptr, failure.ptr);
+            }
+        }
     }
 
     void ensureStrainProgram(uint32_t sweeps);
@@ -528,9 +633,10 @@ void createSolver(const NumiClothHipSceneDesc& scene, NumiClothHipSolver& s) {
     requireEqual(c.constraintCounts.w, scene.fruitCount, "fruits");
     requireEqual(c.contactCounts.x, scene.fruitPairCount, "fruit pairs");
     requireEqual(c.contactCounts.y, scene.yarnContactCount, "yarn contacts");
-    if (scene.fruitCount != 0 || scene.fruitPairCount != 0 || scene.yarnContactCount != 0) {
-        fail(NUMI_CLOTH_HIP_ERROR_UNSUPPORTED, "fruit and fruit/yarn families are not built into this library");
+    if (scene.yarnContactCount != 0 && scene.yarnContactCount != scene.fruitCount * scene.distanceCount) {
+        fail(NUMI_CLOTH_HIP_ERROR_TOPOLOGY, "fruit/yarn records must be fruits x distances (FAILURE_RANGE)");
     }
+    if (scene.fruitCount > 32) fail(NUMI_CLOTH_HIP_ERROR_UNSUPPORTED, "at most 32 fruits (release masks are 32-bit)");
     s.initialConfig = c;
     s.P = scene.particleCount;
     s.D = scene.distanceCount;
@@ -539,7 +645,9 @@ void createSolver(const NumiClothHipSceneDesc& scene, NumiClothHipSolver& s) {
     s.B = scene.bendCount;
     s.extension = scene.extension;
     s.driven = scene.extension.driveAmplitude != 0.0f;
-    if (s.P == 0 || scene.particles == nullptr) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "scene has no particles");
+    // Fruit-only scenes (no knots) are valid, as in the reference's fruit probes.
+    if (s.P == 0 && scene.fruitCount == 0) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "scene has no particles");
+    if (s.P != 0 && scene.particles == nullptr) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "null particles");
     if (s.D != 0 && scene.distances == nullptr) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "null distances");
     if (s.B != 0 && scene.bends == nullptr) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "null bends");
     if (s.K != 0 && scene.knots == nullptr) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "null knots");
@@ -660,6 +768,60 @@ void createSolver(const NumiClothHipSceneDesc& scene, NumiClothHipSolver& s) {
 
     s.grips.allocate(s.G);
     s.grips.upload(scene.grips, s.G, st);
+    s.F = scene.fruitCount;
+    s.FP = scene.fruitPairCount;
+    s.Y = scene.yarnContactCount;
+    if (s.F != 0) {
+        if (scene.fruits == nullptr) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "null fruits");
+        s.fruits.allocate(s.F);
+        s.fruits.upload(scene.fruits, s.F, st);
+        std::vector<NumiClothBagGPUFruitPair> pairs(scene.fruitPairs, scene.fruitPairs + s.FP);
+        s.pairColours = checkBatches(scene.fruitPairBatches, scene.fruitPairBatchCount, s.FP,
+                                     [&](uint32_t k) { return pairs[k].fruitsAndColor.z; }, "fruit pair");
+        std::vector<uint32_t> pairStamp(s.F, ~0u);
+        for (uint32_t r = 0; r < s.pairColours.size(); ++r) {
+            for (uint32_t k = s.pairColours[r].first; k < s.pairColours[r].first + s.pairColours[r].count; ++k) {
+                const mr_uint4 p = pairs[k].fruitsAndColor;
+                if (p.x >= s.F || p.y >= s.F || p.x == p.y) fail(NUMI_CLOTH_HIP_ERROR_TOPOLOGY, "fruit pair index (FAILURE_RANGE)");
+                if (pairStamp[p.x] == r || pairStamp[p.y] == r) fail(NUMI_CLOTH_HIP_ERROR_TOPOLOGY, "fruit pair colour shares a fruit");
+                pairStamp[p.x] = r;
+                pairStamp[p.y] = r;
+            }
+        }
+        s.fruitPairs.allocate(s.FP);
+        s.fruitPairs.upload(pairs.data(), s.FP, st);
+        s.yarnContacts.allocate(s.Y);
+        s.yarnContacts.upload(scene.yarnContacts, s.Y, st);
+        if (s.Y != 0) {
+            if (s.F > 1024) fail(NUMI_CLOTH_HIP_ERROR_UNSUPPORTED, "fruit/yarn solve runs one fruit per lane");
+            int maxShared = 0;
+            HIP_OK(hipDeviceGetAttribute(&maxShared, hipDeviceAttributeMaxSharedMemoryPerBlock, s.device));
+            if (static_cast<uint64_t>(s.P) * sizeof(float4) > static_cast<uint64_t>(maxShared)) {
+                fail(NUMI_CLOTH_HIP_ERROR_UNSUPPORTED, "fruit/yarn contact stages every knot in LDS");
+            }
+            std::vector<uint4> batches;
+            for (const Range& r : s.distanceColours) batches.push_back(make_uint4(r.first, r.count, 0u, 0u));
+            s.distanceBatchesDevice.allocate(batches.size());
+            s.distanceBatchesDevice.upload(batches.data(), batches.size(), st);
+        }
+        if (scene.mouthRimCount != 0) {
+            if (scene.mouthRimParticles == nullptr) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "null mouth rim");
+            s.rimHost.assign(scene.mouthRimParticles, scene.mouthRimParticles + scene.mouthRimCount);
+            for (uint32_t p : s.rimHost)
+                if (p >= s.P) fail(NUMI_CLOTH_HIP_ERROR_TOPOLOGY, "mouth rim index (FAILURE_RANGE)");
+            if (c.mouthControl.x > s.rimHost.size()) fail(NUMI_CLOTH_HIP_ERROR_TOPOLOGY, "mouth rim shorter than mouthControl.x");
+            s.rim.allocate(s.rimHost.size());
+            s.rim.upload(s.rimHost.data(), s.rimHost.size(), st);
+        }
+    }
+    s.yarnAir = c.airVelocityAndDensity.w > 0.0f &&
+                (c.aerodynamicCoefficients.x > 0.0f || c.aerodynamicCoefficients.y > 0.0f) && s.D != 0;
+    if (s.yarnAir) {
+        s.aeroForces.allocate(s.P);
+        s.aeroReduction.allocate(1);
+    }
+    s.mouthClearance.allocate(32);
+    HIP_OK(hipMemsetAsync(s.mouthClearance.ptr, 0, 32 * sizeof(uint32_t), st));
     s.dCuff.allocate(s.cuff.size());
     s.dCuff.upload(s.cuff.data(), s.cuff.size(), st);
     const uint32_t nSpheres = scene.extension.staticSphereCount;
@@ -686,7 +848,10 @@ void createSolver(const NumiClothHipSceneDesc& scene, NumiClothHipSolver& s) {
     HIP_OK(hipMemsetAsync(s.selfStatus.ptr, 0, sizeof(NumiClothBagGPUSelfStatus), st));
     HIP_OK(hipMemsetAsync(s.frictionStatus.ptr, 0, sizeof(NumiClothBagGPUFrictionStatus), st));
     HIP_OK(hipMemsetAsync(s.aeroStatus.ptr, 0, sizeof(NumiClothBagGPUAerodynamicsStatus), st));
-    HIP_OK(hipMemsetAsync(s.releaseStatus.ptr, 0, sizeof(NumiClothBagGPUReleaseStatus), st));
+    // Release masks carry over between calls (reference
+    // tools/cloth_bag_metal.mm:4161-4162, :4779-4780).
+    const NumiClothBagGPUReleaseStatus masks{{scene.initialCandidateMask, scene.initialReleasedMask, 0u, 0u}};
+    s.releaseStatus.upload(&masks, 1, st);
     s.diagPartials.allocate(1024);
     if (const char* graphs = std::getenv("NUMI_CLOTH_HIP_GRAPHS")) s.useGraphs = std::strcmp(graphs, "0") != 0;
     s.posCur = s.pos.ptr;
@@ -952,9 +1117,10 @@ void checkTrajectoryConfig(const NumiClothHipSolver& s, const NumiClothBagGPUCon
     if (c.gravityAndTimestep.w != i.gravityAndTimestep.w && s.extension.damping > 0.0f) {
         fail(NUMI_CLOTH_HIP_ERROR_TOPOLOGY, "a damped scene keeps its timestep");
     }
-    const float density = c.airVelocityAndDensity.w;
-    if (density > 0.0f && (c.aerodynamicCoefficients.x > 0.0f || c.aerodynamicCoefficients.y > 0.0f)) {
-        fail(NUMI_CLOTH_HIP_ERROR_UNSUPPORTED, "yarn air loads are not built into this library");
+    const bool air = c.airVelocityAndDensity.w > 0.0f &&
+                     (c.aerodynamicCoefficients.x > 0.0f || c.aerodynamicCoefficients.y > 0.0f) && s.D != 0;
+    if (air && !s.yarnAir) {
+        fail(NUMI_CLOTH_HIP_ERROR_TOPOLOGY, "yarn air loads must be enabled by the scene's initial configuration");
     }
 }
 
@@ -984,6 +1150,7 @@ void stepSolver(NumiClothHipSolver& s, const NumiClothBagGPUConfig* configs, uin
     // Status records count per step() call, like the reference's per-call buffers.
     k_fill_u32<<<1, 64, 0, s.stream>>>(reinterpret_cast<uint32_t*>(s.frictionStatus.ptr), 8, 0u);
     k_fill_u32<<<1, 64, 0, s.stream>>>(reinterpret_cast<uint32_t*>(s.selfStatus.ptr), 4, 0u);
+    k_fill_u32<<<1, 64, 0, s.stream>>>(reinterpret_cast<uint32_t*>(s.aeroStatus.ptr), 4, 0u);
     HIP_OK(hipGetLastError());
     if (s.useGraphs) {
         for (uint32_t k = 0; k < substeps; ++k) HIP_OK(hipGraphLaunch(s.graph, s.stream));
@@ -1018,12 +1185,21 @@ void readSolver(NumiClothHipSolver& s, const NumiClothHipState& out) {
         s.kLambda.download(kl.data(), s.K, st);
     }
     if (out.grips != nullptr) s.grips.download(out.grips, s.G, st);
+    if (out.fruits != nullptr) s.fruits.download(out.fruits, s.F, st);
+    if (out.fruitPairs != nullptr) s.fruitPairs.download(out.fruitPairs, s.FP, st);
+    if (out.yarnContacts != nullptr) s.yarnContacts.download(out.yarnContacts, s.Y, st);
     if (out.selfStatus != nullptr) s.selfStatus.download(out.selfStatus, 1, st);
     if (out.frictionStatus != nullptr) s.frictionStatus.download(out.frictionStatus, 1, st);
     if (out.aerodynamicsStatus != nullptr) s.aeroStatus.download(out.aerodynamicsStatus, 1, st);
     if (out.releaseStatus != nullptr) s.releaseStatus.download(out.releaseStatus, 1, st);
     if (out.failure != nullptr) s.failure.download(out.failure, 1, st);
+    std::vector<uint32_t> clearance;
+    if (out.mouthClearance != nullptr && s.F != 0) {
+        clearance.resize(s.F);
+        s.mouthClearance.download(clearance.data(), s.F, st);
+    }
     HIP_OK(hipStreamSynchronize(st));
+    for (uint32_t i = 0; i < clearance.size(); ++i) std::memcpy(out.mouthClearance + i, &clearance[i], sizeof(float));
     if (out.particles != nullptr) {
         for (uint32_t i = 0; i < s.P; ++i) {
             NumiClothBagGPUParticle& p = out.particles[i];

@@ -145,6 +145,7 @@ struct Oracle {
             gripCentre[i] = g.particle.w;
         }
         if (d.extension.selfContact) buildLocality();
+        initBag();
     }
 
     // Two-hop neighbourhoods (including the knot) over the distance graph.
@@ -666,6 +667,620 @@ struct Oracle {
         }
     }
 
+    // ============================================================== bag ==
+    // Fruit spheres, fruit/yarn records and air loads (reference
+    // src/metal/ClothBag.metal:491-575, :675-905, :1025-1146, :1503-1614,
+    // :1675-1786, :2733-2974, :3024-3335), in the order of the device
+    // kernels in numi-solver_amd/csrc/kernels_bag.hpp.
+    struct Fruit {
+        V<R> x, prev, v, w;
+        R q[4];
+        R invMass = 0, radius = 0, groundImpulse = 0;
+    };
+    struct PairContact {
+        V<R> n;
+        R j = 0;
+    };
+    struct Yarn {
+        V<R> normal, sweptNormal, fruitN;
+        R separation = 0, advance = 0, wCurrent = 0, wSwept = 0, time = 0, target = 0;
+        R fruitJ = 0, segA = 0, segB = 0, firstTime = 0, sweptAdvance = 0;
+        uint32_t overlap = 0, impact = 0, degenerate = 0, responses = 0;
+    };
+    std::vector<Fruit> fruit;
+    std::vector<PairContact> pairContact;
+    std::vector<Yarn> yarn;
+    std::vector<V<R>> airForce;
+    bool yarnAir = false;
+    uint32_t candidateMask = 0, releasedMask = 0;
+    uint32_t pairFrictionContacts = 0, yarnFrictionContacts = 0, fruitGroundContacts = 0, rollingContacts = 0;
+    double maximumRollingRatio = 0;
+    double maximumYarnAeroForce = 0, maximumFruitAeroForce = 0, maximumFruitAeroTorque = 0;
+    std::vector<double> mouthClearance;
+
+    void initBag() {
+        const uint32_t F = d.fruitCount;
+        fruit.resize(F);
+        for (uint32_t i = 0; i < F; ++i) {
+            const NumiClothBagGPUFruit& s = d.fruits[i];
+            Fruit& f = fruit[i];
+            f.x = v3<R>(s.positionAndInverseMass);
+            f.prev = v3<R>(s.previousAndRadius);
+            f.v = v3<R>(s.velocityAndGroundImpulse);
+            f.w = v3<R>(s.angularVelocity);
+            f.q[0] = R(s.orientation.x);
+            f.q[1] = R(s.orientation.y);
+            f.q[2] = R(s.orientation.z);
+            f.q[3] = R(s.orientation.w);
+            f.invMass = R(s.positionAndInverseMass.w);
+            f.radius = R(s.previousAndRadius.w);
+            f.groundImpulse = R(s.velocityAndGroundImpulse.w);
+        }
+        pairContact.assign(d.fruitPairCount, PairContact{});
+        yarn.assign(d.yarnContactCount, Yarn{});
+        const NumiClothBagGPUConfig& c = *d.config;
+        yarnAir = c.airVelocityAndDensity.w > 0.0f &&
+                  (c.aerodynamicCoefficients.x > 0.0f || c.aerodynamicCoefficients.y > 0.0f) && D != 0;
+        candidateMask = d.initialCandidateMask;
+        releasedMask = d.initialReleasedMask;
+        mouthClearance.assign(F, 0.0);
+    }
+
+    R inverseInertia(const Fruit& f) const { return R(2.5) * f.invMass / (f.radius * f.radius); }
+
+    void applyImpulse(Fruit& f, V<R> impulse, V<R> offset) {
+        f.v = f.v + impulse * f.invMass;
+        f.w = f.w + cross(offset, impulse) * inverseInertia(f);
+    }
+
+    void fruitBegin(const NumiClothBagGPUConfig& c) {
+        const R dt = R(c.gravityAndTimestep.w);
+        for (Fruit& f : fruit) {
+            f.prev = f.x;
+            f.v = fma3(v3<R>(c.gravityAndTimestep), dt, f.v);
+            f.groundImpulse = 0;
+        }
+        for (PairContact& p : pairContact) p = PairContact{};
+        for (Yarn& y : yarn) {
+            y.responses = 0;
+            y.fruitN = V<R>{};
+            y.fruitJ = 0;
+            y.segA = y.segB = y.firstTime = y.sweptAdvance = 0;
+        }
+    }
+
+    void yarnAirLoads(const NumiClothBagGPUConfig& c) {
+        if (!yarnAir) return;
+        airForce.assign(P, V<R>{});
+        const V<R> air = v3<R>(c.airVelocityAndDensity);
+        const R density = R(c.airVelocityAndDensity.w);
+        const R pi = R(3.14159265358979323846);
+        const R radius = R(c.clothMaterial.x);
+        for (uint32_t b = 0; b < d.distanceBatchCount; ++b) {
+            const mr_uint4 batch = d.distanceBatches[b].control;
+            for (uint32_t k = batch.x; k < batch.x + batch.y; ++k) {
+                const mr_uint4 e = d.distances[k].particlesAndColor;
+                const V<R> span = p[e.y].x - p[e.x].x;
+                const R len = norm(span);
+                if (!(len > R(1.0e-12))) continue;
+                const V<R> axis = span / len;
+                const V<R> rel = (p[e.x].v + p[e.y].v) * R(0.5) - air;
+                const V<R> axial = axis * dot(rel, axis);
+                const V<R> crossflow = rel - axial;
+                const R crossCoeff = R(0.5) * density * R(c.aerodynamicCoefficients.x) * (R(2) * radius * len);
+                const R skinCoeff = R(0.5) * density * R(c.aerodynamicCoefficients.y) * (R(2) * pi * radius * len);
+                const V<R> force = crossflow * (-crossCoeff * norm(crossflow)) - axial * (skinCoeff * norm(axial));
+                maximumYarnAeroForce = std::max(maximumYarnAeroForce, static_cast<double>(norm(force)));
+                airForce[e.x] = airForce[e.x] + force * R(0.5);
+                airForce[e.y] = airForce[e.y] + force * R(0.5);
+            }
+        }
+        // 256 strided lane sums, then a pairwise tree (the device order).
+        R power[256] = {}, weighted[256] = {};
+        for (uint32_t lane = 0; lane < 256; ++lane) {
+            for (uint32_t i = lane; i < P; i += 256) {
+                if (p[i].w > R(0)) {
+                    power[lane] += dot(airForce[i], p[i].v - air);
+                    weighted[lane] += p[i].w * dot(airForce[i], airForce[i]);
+                }
+            }
+        }
+        for (uint32_t stride = 128; stride > 0; stride >>= 1)
+            for (uint32_t lane = 0; lane < stride; ++lane) {
+                power[lane] += power[lane + stride];
+                weighted[lane] += weighted[lane + stride];
+            }
+        R attenuation = 1;
+        if (weighted[0] > R(0)) {
+            attenuation = power[0] < R(0) ? R(1) / (R(1) + R(c.gravityAndTimestep.w) * weighted[0] / -power[0]) : R(0);
+        }
+        const R dt = R(c.gravityAndTimestep.w);
+        for (uint32_t i = 0; i < P; ++i) {
+            if (p[i].w > R(0)) {
+                p[i].v = p[i].v + airForce[i] * (attenuation * p[i].w * dt);
+                p[i].predictedVz = p[i].v.z;
+            }
+        }
+    }
+
+    void fruitAirLoads(const NumiClothBagGPUConfig& c) {
+        const R pi = R(3.14159265358979323846);
+        const R density = R(c.airVelocityAndDensity.w);
+        const R dt = R(c.gravityAndTimestep.w);
+        const V<R> air = v3<R>(c.airVelocityAndDensity);
+        for (Fruit& f : fruit) {
+            const V<R> rel = f.v - air;
+            const R speed = norm(rel);
+            const R transCoeff = R(0.5) * density * R(c.aerodynamicCoefficients.z) * pi * f.radius * f.radius;
+            const R transAtt = R(1) / (R(1) + f.invMass * transCoeff * speed * dt);
+            f.v = air + rel * transAtt;
+            const R spin = norm(f.w);
+            const R r = f.radius;
+            const R rotCoeff = (R(3) * pi * pi / R(8)) * density * R(c.aerodynamicCoefficients.w) * r * r * r * r * r;
+            const R rotAtt = R(1) / (R(1) + inverseInertia(f) * rotCoeff * spin * dt);
+            f.w = f.w * rotAtt;
+            maximumFruitAeroForce = std::max(maximumFruitAeroForce, static_cast<double>(transCoeff * speed * speed));
+            maximumFruitAeroTorque = std::max(maximumFruitAeroTorque, static_cast<double>(rotCoeff * spin * spin));
+        }
+    }
+
+    void fruitAdvance(R dt) {
+        for (Fruit& f : fruit) f.x = fma3(f.v, dt, f.x);
+    }
+
+    void fruitPairs(const NumiClothBagGPUConfig& c) {
+        for (uint32_t b = 0; b < d.fruitPairBatchCount; ++b) {
+            const mr_uint4 batch = d.fruitPairBatches[b].control;
+            for (uint32_t k = batch.x; k < batch.x + batch.y; ++k) {
+                const mr_uint4 pr = d.fruitPairs[k].fruitsAndColor;
+                Fruit& a = fruit[pr.x];
+                Fruit& bf = fruit[pr.y];
+                const V<R> diff = bf.x - a.x;
+                const R len = norm(diff);
+                const R target = a.radius + bf.radius;
+                if (!(len < target) || !(len > R(1.0e-12))) continue;
+                const R denom = a.invMass + bf.invMass;
+                if (!(denom > R(0))) continue;
+                const R lambda = (target - len) / denom;
+                const V<R> n = diff / len;
+                const V<R> corr = n * lambda;
+                a.x = a.x - corr * a.invMass;
+                bf.x = bf.x + corr * bf.invMass;
+                const R impulse = lambda / R(c.gravityAndTimestep.w);
+                pairContact[k].n = pairContact[k].n + n * impulse;
+                pairContact[k].j += impulse;
+            }
+        }
+    }
+
+    void fruitGround(const NumiClothBagGPUConfig& c) {
+        if (c.constraintCounts.z == 0u) return;
+        for (Fruit& f : fruit) {
+            const R penetration = f.radius - f.x.z;
+            if (penetration > R(0)) {
+                f.groundImpulse += penetration / (f.invMass * R(c.gravityAndTimestep.w));
+                f.x.z = f.radius;
+            }
+        }
+    }
+
+    struct PointSample {
+        V<R> closest;
+        R weight, distance;
+    };
+
+    static PointSample pointSegment(V<R> point, V<R> a, V<R> b) {
+        const V<R> dd = b - a;
+        const R l2 = dot(dd, dd);
+        const R w = l2 > R(1.0e-20) ? clampR(dot(dd, point - a) / l2, R(0), R(1)) : R(0);
+        const V<R> cl = fma3(dd, w, a);
+        return {cl, w, norm(cl - point)};
+    }
+
+    static V<R> safeUnit(V<R> v) {
+        const R m = norm(v);
+        return m > R(1.0e-14) ? v / m : V<R>{R(1), R(0), R(0)};
+    }
+
+    static V<R> degenerateNormal(V<R> a, V<R> b) {
+        const V<R> dir = safeUnit(b - a);
+        const V<R> axis = std::abs(dir.z) < R(0.9) ? V<R>{R(0), R(0), R(1)} : V<R>{R(1), R(0), R(0)};
+        return safeUnit(cross(dir, axis));
+    }
+
+    void buildYarnContacts(const NumiClothBagGPUConfig& c) {
+        const uint32_t F = d.fruitCount;
+        if (yarn.empty()) return;
+        for (uint32_t fi = 0; fi < F; ++fi) {
+            const Fruit& f = fruit[fi];
+            for (uint32_t si = 0; si < D; ++si) {
+                Yarn& y = yarn[fi * D + si];
+                const mr_uint4 e = d.distances[si].particlesAndColor;
+                const V<R> a = p[e.x].x, b = p[e.y].x, a0 = p[e.x].prev, b0 = p[e.y].prev;
+                const R target = f.radius + R(c.clothMaterial.x);
+                const PointSample now = pointSegment(f.x, a, b);
+                const bool degenerate = now.distance < R(1.0e-12);
+                const V<R> normal = degenerate ? degenerateNormal(a, b) : (now.closest - f.x) / now.distance;
+                R time = 0, sweptWeight = 0, removed = 0;
+                V<R> sweptNormal{};
+                bool impact = false;
+                const R bound = norm(f.x - f.prev) + norm(a - a0) + norm(b - b0);
+                if (bound >= R(1.0e-14)) {
+                    auto at = [&](R t) { return pointSegment(mix3(f.prev, f.x, t), mix3(a0, a, t), mix3(b0, b, t)); };
+                    PointSample hit = at(R(0));
+                    bool found = hit.distance <= target + R(1.0e-9);
+                    if (!found) {
+                        for (int step = 0; step < 80; ++step) {
+                            hit = at(time);
+                            const R gap = hit.distance - target;
+                            if (gap <= R(1.0e-9)) {
+                                found = true;
+                                break;
+                            }
+                            const R adv = R(0.9) * gap / bound;
+                            if (!std::isfinite(adv) || !(adv > R(0)) || time + adv >= R(1)) break;
+                            time += std::max(adv, R(1.0e-10));
+                        }
+                    }
+                    if (found && hit.distance >= R(1.0e-12)) {
+                        const V<R> ct = mix3(f.prev, f.x, time), at_ = mix3(a0, a, time), bt = mix3(b0, b, time);
+                        sweptNormal = (hit.closest - ct) / hit.distance;
+                        sweptWeight = hit.weight;
+                        const V<R> segRest = (a - at_) * (R(1) - sweptWeight) + (b - bt) * sweptWeight;
+                        removed = dot((f.x - ct) - segRest, sweptNormal);
+                        impact = removed > R(0);
+                        if (!impact) removed = 0;
+                    }
+                }
+                y.normal = normal;
+                y.separation = now.distance - target;
+                y.sweptNormal = sweptNormal;
+                y.advance = removed;
+                y.wCurrent = now.weight;
+                y.wSwept = sweptWeight;
+                y.time = time;
+                y.target = target;
+                y.overlap = now.distance < target ? 1u : 0u;
+                y.impact = impact ? 1u : 0u;
+                y.degenerate = degenerate ? 1u : 0u;
+            }
+        }
+    }
+
+    R yarnCorrection(Particle<R>& first, Particle<R>& second, Fruit& f, R wa, R wb, V<R> n, R distance,
+                     bool groundEnabled, R floorZ) {
+        V<R> a = first.x, b = second.x;
+        const R ia = first.w, ib = second.w;
+        bool aActive = groundEnabled && n.z < R(0) && wa > R(0) && a.z <= floorZ + R(1.0e-9);
+        bool bActive = groundEnabled && n.z < R(0) && wb > R(0) && b.z <= floorZ + R(1.0e-9);
+        if (aActive) a.z = floorZ;
+        if (bActive) b.z = floorZ;
+        R remaining = distance, total = 0;
+        for (int k = 0; k < 3 && remaining > R(1.0e-14); ++k) {
+            V<R> ra = n * ia, rb = n * ib;
+            if (aActive) ra.z = 0;
+            if (bActive) rb.z = 0;
+            const R denom = f.invMass + dot(n, ra) * wa * wa + dot(n, rb) * wb * wb;
+            if (!(denom > R(0))) break;
+            const R freeStep = remaining / denom;
+            R step = freeStep;
+            if (groundEnabled && n.z < R(0)) {
+                const R va = ra.z * wa;
+                if (!aActive && va < R(0)) step = std::min(step, std::max(R(0), a.z - floorZ) / -va);
+                const R vb = rb.z * wb;
+                if (!bActive && vb < R(0)) step = std::min(step, std::max(R(0), b.z - floorZ) / -vb);
+            }
+            f.x = f.x - n * (f.invMass * step);
+            a = a + ra * (wa * step);
+            b = b + rb * (wb * step);
+            total += step;
+            remaining = std::max(R(0), remaining - denom * step);
+            if (step >= freeStep - R(1.0e-14)) break;
+            bool activated = false;
+            if (!aActive && wa > R(0) && a.z <= floorZ + R(1.0e-9)) {
+                a.z = floorZ;
+                aActive = true;
+                activated = true;
+            }
+            if (!bActive && wb > R(0) && b.z <= floorZ + R(1.0e-9)) {
+                b.z = floorZ;
+                bActive = true;
+                activated = true;
+            }
+            if (!activated) break;
+        }
+        first.x = a;
+        second.x = b;
+        return total;
+    }
+
+    // Phase order of the reference: fruit f visits segment (phase + f) mod n;
+    // within a phase the fruits touch disjoint segments of one colour.
+    void yarnBatches(const NumiClothBagGPUConfig& c, uint32_t mode) {
+        if (yarn.empty()) return;
+        const uint32_t F = d.fruitCount;
+        const bool ground = c.constraintCounts.z != 0u;
+        const R floorZ = R(c.clothMaterial.x);
+        const R dt = R(c.gravityAndTimestep.w);
+        for (uint32_t bi = 0; bi < d.distanceBatchCount; ++bi) {
+            const mr_uint4 batch = d.distanceBatches[bi].control;
+            for (uint32_t phase = 0; phase < batch.y; ++phase) {
+                for (uint32_t fi = 0; fi < F; ++fi) {
+                    const uint32_t segment = batch.x + (phase + fi) % batch.y;
+                    const mr_uint4 e = d.distances[segment].particlesAndColor;
+                    Yarn& y = yarn[fi * D + segment];
+                    Fruit& f = fruit[fi];
+                    V<R> n{};
+                    R wa = 0, wb = 0, correction = 0;
+                    if (mode == 1) {
+                        if (y.impact) {
+                            n = y.sweptNormal;
+                            wb = y.wSwept;
+                            wa = R(1) - wb;
+                            correction = y.advance;
+                        }
+                    } else {
+                        const PointSample s = pointSegment(f.x, p[e.x].x, p[e.y].x);
+                        const R target = f.radius + R(c.clothMaterial.x);
+                        if (s.distance < target) {
+                            n = s.distance < R(1.0e-12) ? degenerateNormal(p[e.x].x, p[e.y].x)
+                                                        : (s.closest - f.x) / s.distance;
+                            wb = s.weight;
+                            wa = R(1) - wb;
+                            correction = target - s.distance;
+                        }
+                    }
+                    if (!(correction > R(0))) continue;
+                    const R lambda = yarnCorrection(p[e.x], p[e.y], f, wa, wb, n, correction, ground, floorZ);
+                    if (lambda > R(0)) {
+                        const R impulse = lambda / dt;
+                        y.fruitN = y.fruitN - n * impulse;
+                        y.fruitJ += impulse;
+                        y.segA += wa * impulse;
+                        y.segB += wb * impulse;
+                        if (mode == 1) {
+                            y.firstTime = y.time;
+                            y.sweptAdvance += correction;
+                        }
+                        y.responses += 1;
+                    }
+                }
+            }
+        }
+    }
+
+    void finalizeFruit(const NumiClothBagGPUConfig& c) {
+        for (Fruit& f : fruit) f.v = (f.x - f.prev) / R(c.gravityAndTimestep.w);
+    }
+
+    void yarnFriction(const NumiClothBagGPUConfig& c) {
+        if (yarn.empty()) return;
+        const uint32_t F = d.fruitCount;
+        const R mu = R(c.fruitMaterial.w);
+        const bool ground = c.constraintCounts.z != 0u;
+        const R band = R(c.clothMaterial.x) + R(1.0e-6);
+        for (uint32_t bi = 0; bi < d.distanceBatchCount; ++bi) {
+            const mr_uint4 batch = d.distanceBatches[bi].control;
+            for (uint32_t phase = 0; phase < batch.y; ++phase) {
+                for (uint32_t fi = 0; fi < F; ++fi) {
+                    const uint32_t segment = batch.x + (phase + fi) % batch.y;
+                    const mr_uint4 e = d.distances[segment].particlesAndColor;
+                    const Yarn& y = yarn[fi * D + segment];
+                    Fruit& f = fruit[fi];
+                    const R jn = y.fruitJ;
+                    const R nl = norm(y.fruitN);
+                    if (!(jn > R(0)) || !(nl > R(1.0e-10))) continue;
+                    const V<R> n = y.fruitN / nl;
+                    R wx = y.segA / jn, wy = y.segB / jn;
+                    const R ws = wx + wy;
+                    if (!(ws > R(1.0e-12))) continue;
+                    wx = wx / ws;
+                    wy = wy / ws;
+                    Particle<R>& a = p[e.x];
+                    Particle<R>& b = p[e.y];
+                    const V<R> yarnV = a.v * wx + b.v * wy;
+                    const V<R> offset = n * -f.radius;
+                    const V<R> rel = (f.v + cross(f.w, offset)) - yarnV;
+                    const V<R> tv = rel - n * dot(rel, n);
+                    const R slip = norm(tv);
+                    if (!(slip > R(1.0e-10))) continue;
+                    const V<R> t = tv / slip;
+                    const V<R> lever = cross(offset, t);
+                    R denom = f.invMass + inverseInertia(f) * dot(lever, lever);
+                    V<R> ra = t * a.w, rb = t * b.w;
+                    if (ground) {
+                        if (a.x.z <= band && ra.z < R(0)) ra.z = 0;
+                        if (b.x.z <= band && rb.z < R(0)) rb.z = 0;
+                    }
+                    denom += dot(t, ra) * wx * wx + dot(t, rb) * wy * wy;
+                    if (!(denom > R(0))) continue;
+                    const R limit = mu * jn;
+                    const R jt = std::min(slip / denom, limit);
+                    if (!(jt > R(0))) continue;
+                    applyImpulse(f, t * -jt, offset);
+                    a.v = a.v + ra * (jt * wx);
+                    b.v = b.v + rb * (jt * wy);
+                    yarnFrictionContacts += 1;
+                    if (limit > R(0)) maximumConeRatio = std::max(maximumConeRatio, static_cast<double>(jt / limit));
+                }
+            }
+        }
+    }
+
+    void pairFriction(const NumiClothBagGPUConfig& c) {
+        const R mu = R(c.fruitMaterial.x);
+        for (uint32_t bi = 0; bi < d.fruitPairBatchCount; ++bi) {
+            const mr_uint4 batch = d.fruitPairBatches[bi].control;
+            for (uint32_t k = batch.x; k < batch.x + batch.y; ++k) {
+                const mr_uint4 pr = d.fruitPairs[k].fruitsAndColor;
+                const R jn = pairContact[k].j;
+                const R nl = norm(pairContact[k].n);
+                if (!(jn > R(0)) || !(nl > R(1.0e-10))) continue;
+                Fruit& a = fruit[pr.x];
+                Fruit& b = fruit[pr.y];
+                const V<R> n = pairContact[k].n / nl;
+                const V<R> oa = n * a.radius, ob = n * -b.radius;
+                const V<R> rel = (b.v + cross(b.w, ob)) - (a.v + cross(a.w, oa));
+                const V<R> tv = rel - n * dot(rel, n);
+                const R slip = norm(tv);
+                if (!(slip > R(1.0e-10))) continue;
+                const V<R> t = tv / slip;
+                const V<R> la = cross(oa, t), lb = cross(ob, t);
+                const R denom = a.invMass + b.invMass + inverseInertia(a) * dot(la, la) + inverseInertia(b) * dot(lb, lb);
+                if (!(denom > R(0))) continue;
+                const R limit = mu * jn;
+                const R jt = std::min(slip / denom, limit);
+                if (!(jt > R(0))) continue;
+                const V<R> onB = t * -jt;
+                applyImpulse(b, onB, ob);
+                applyImpulse(a, -onB, oa);
+                pairFrictionContacts += 1;
+                if (limit > R(0)) maximumConeRatio = std::max(maximumConeRatio, static_cast<double>(jt / limit));
+            }
+        }
+    }
+
+    void fruitGroundFriction(const NumiClothBagGPUConfig& c) {
+        if (c.constraintCounts.z == 0u) return;
+        const R mu = R(c.fruitMaterial.y), rolling = R(c.fruitMaterial.z);
+        for (Fruit& f : fruit) {
+            const R jn = f.groundImpulse;
+            if (!(jn > R(0))) continue;
+            const V<R> n{R(0), R(0), R(1)};
+            const V<R> offset{R(0), R(0), -f.radius};
+            const V<R> cv = f.v + cross(f.w, offset);
+            const V<R> tv = cv - n * dot(cv, n);
+            const R slip = norm(tv);
+            if (slip > R(1.0e-10)) {
+                const V<R> t = tv / slip;
+                const V<R> lever = cross(offset, t);
+                const R denom = f.invMass + inverseInertia(f) * dot(lever, lever);
+                const R limit = mu * jn;
+                const R jt = std::min(slip / denom, limit);
+                if (jt > R(0)) {
+                    applyImpulse(f, t * -jt, offset);
+                    fruitGroundContacts += 1;
+                    if (limit > R(0)) maximumConeRatio = std::max(maximumConeRatio, static_cast<double>(jt / limit));
+                }
+            }
+            const V<R> roll{f.w.x, f.w.y, R(0)};
+            const R rollSpeed = norm(roll);
+            if (rollSpeed > R(1.0e-12) && rolling > R(0)) {
+                const R inertia = inverseInertia(f);
+                const R limit = rolling * f.radius * jn;
+                const R angular = std::min(rollSpeed / inertia, limit);
+                f.w = f.w - roll * (angular * inertia / rollSpeed);
+                rollingContacts += 1;
+                if (limit > R(0)) maximumRollingRatio = std::max(maximumRollingRatio, static_cast<double>(angular / limit));
+            }
+        }
+    }
+
+    void orientation(const NumiClothBagGPUConfig& c) {
+        const R h = R(0.5) * R(c.gravityAndTimestep.w);
+        for (Fruit& f : fruit) {
+            const V<R> qv{f.q[0], f.q[1], f.q[2]};
+            const V<R> dv = f.w * f.q[3] + cross(f.w, qv);
+            const R dw = -dot(f.w, qv);
+            const R a[4] = {std::fma(dv.x, h, f.q[0]), std::fma(dv.y, h, f.q[1]), std::fma(dv.z, h, f.q[2]),
+                            std::fma(dw, h, f.q[3])};
+            const R m2 = a[0] * a[0] + a[1] * a[1] + a[2] * a[2] + a[3] * a[3];
+            const R s = R(1) / std::sqrt(m2);
+            for (int k = 0; k < 4; ++k) f.q[k] = a[k] * s;
+        }
+    }
+
+    bool insideMouth(V<R> centre, V<R> tangent, V<R> bitangent, V<R> point, R clearance) const {
+        const uint32_t n = d.config->mouthControl.x;
+        const V<R> rp = point - centre;
+        const R px = dot(rp, tangent), py = dot(rp, bitangent);
+        bool inside = false;
+        R nearest = R(INFINITY);
+        uint32_t second = n - 1;
+        for (uint32_t first = 0; first < n; ++first) {
+            const V<R> a = p[d.mouthRimParticles[first]].x - centre;
+            const V<R> b = p[d.mouthRimParticles[second]].x - centre;
+            const R ax = dot(a, tangent), ay = dot(a, bitangent), bx = dot(b, tangent), by = dot(b, bitangent);
+            const R ex = bx - ax, ey = by - ay;
+            const R l2 = ex * ex + ey * ey;
+            const R t = l2 > R(1.0e-20) ? clampR(((px - ax) * ex + (py - ay) * ey) / l2, R(0), R(1)) : R(0);
+            const R sx = px - (ax + t * ex), sy = py - (ay + t * ey);
+            nearest = std::min(nearest, sx * sx + sy * sy);
+            if ((ay > py) != (by > py)) {
+                const R cx = ax + (bx - ax) * (py - ay) / (by - ay);
+                if (px < cx) inside = !inside;
+            }
+            second = first;
+        }
+        return inside || nearest <= clearance * clearance;
+    }
+
+    R rimDistance(V<R> point) const {
+        const uint32_t n = d.config->mouthControl.x;
+        R nearest = R(INFINITY);
+        for (uint32_t k = 0; k < n; ++k) {
+            const V<R> a = p[d.mouthRimParticles[k]].x;
+            const V<R> e = p[d.mouthRimParticles[(k + 1) % n]].x - a;
+            const R l2 = dot(e, e);
+            const R t = l2 > R(1.0e-20) ? clampR(dot(point - a, e) / l2, R(0), R(1)) : R(0);
+            const V<R> s = point - (a + e * t);
+            nearest = std::min(nearest, dot(s, s));
+        }
+        return std::sqrt(nearest);
+    }
+
+    void release(const NumiClothBagGPUConfig& c) {
+        const uint32_t n = c.mouthControl.x;
+        if (n == 0 || d.mouthRimCount == 0) return;
+        V<R> centre{};
+        for (uint32_t k = 0; k < n; ++k) centre = centre + p[d.mouthRimParticles[k]].x;
+        centre = centre / R(n);
+        V<R> area{};
+        for (uint32_t k = 0; k < n; ++k)
+            area = area + cross(p[d.mouthRimParticles[k]].x - centre, p[d.mouthRimParticles[(k + 1) % n]].x - centre);
+        if (!(norm(area) > R(1.0e-8))) return;
+        V<R> normal = area / norm(area);
+        if (dot(normal, centre - p[c.mouthControl.y].x) < R(0)) normal = -normal;
+        V<R> tangent = p[d.mouthRimParticles[0]].x - centre;
+        tangent = tangent - normal * dot(tangent, normal);
+        if (!(norm(tangent) > R(1.0e-8))) {
+            tangent = cross(normal, std::abs(normal.z) < R(0.9) ? V<R>{R(0), R(0), R(1)} : V<R>{R(1), R(0), R(0)});
+        }
+        tangent = tangent / norm(tangent);
+        V<R> bitangent = cross(normal, tangent);
+        bitangent = bitangent / norm(bitangent);
+        // Candidate bits set this substep are visible to the rim test of the
+        // same fruit only (each fruit reads its own bit).
+        for (uint32_t i = 0; i < fruit.size(); ++i) {
+            const Fruit& f = fruit[i];
+            const R required = f.radius + R(c.clothMaterial.x);
+            const bool insideProjection = insideMouth(centre, tangent, bitangent, f.x, R(0));
+            const bool nearProjection = insideMouth(centre, tangent, bitangent, f.x, required);
+            const R outward = dot(f.x - centre, normal);
+            const uint32_t mask = 1u << i;
+            if (insideProjection && outward > R(0)) candidateMask |= mask;
+            const R axial = outward - required;
+            const R rimClear = rimDistance(f.x) - required;
+            const bool throughCap = nearProjection && axial > R(c.mouthMaterial.x);
+            const bool aroundRim = (candidateMask & mask) != 0u && !insideProjection && outward > R(0) &&
+                                   rimClear > R(c.mouthMaterial.x);
+            const R clearance = throughCap ? axial : (aroundRim ? rimClear : axial);
+            mouthClearance[i] = std::max(mouthClearance[i], static_cast<double>(std::max(R(0), clearance)));
+            if (throughCap || aroundRim) releasedMask |= mask;
+        }
+    }
+
+    void ground(const NumiClothBagGPUConfig& c, bool colliders) {
+        if (colliders) collide(c);
+        fruitGround(c);
+    }
+
+    void finalContacts(const NumiClothBagGPUConfig& c, bool colliders) {
+        fruitPairs(c);
+        yarnBatches(c, 0);
+        ground(c, colliders);
+    }
+
     void substep(const NumiClothBagGPUConfig& c, uint32_t iterations, uint32_t sweeps) {
         const R dt = R(c.gravityAndTimestep.w);
         time += static_cast<double>(c.gravityAndTimestep.w);
@@ -674,40 +1289,52 @@ struct Oracle {
                                   ? R(static_cast<float>(e.driveAmplitude * std::sin(e.driveOmega * time + e.drivePhase)))
                                   : R(0);
         begin(c, driveOffset);
+        fruitBegin(c);
         attachGrips(c);
+        yarnAirLoads(c);
+        fruitAirLoads(c);
         advance(dt);
+        fruitAdvance(dt);
         const bool self = e.selfContact != 0;
         if (self) {
             broadPhase(c);
             selfContact(c, 1);
         }
+        buildYarnContacts(c);
+        yarnBatches(c, 1);
         const bool colliders = d.config->constraintCounts.z != 0u || e.staticSphereCount != 0u;
         for (uint32_t it = 0; it < iterations; ++it) {
             distances(dt);
             knots(dt);
             if (it % 2 == 0) bends(c, dt);
             grips(c, dt);
+            fruitPairs(c);
+            yarnBatches(c, 0);
             if (self) selfContact(c, 0);
-            if (colliders) collide(c);
+            ground(c, colliders);
         }
         strain(sweeps);
         if (sweeps != 0) {
             for (int pass = 0; pass < 8; ++pass) {
+                fruitPairs(c);
+                yarnBatches(c, 0);
                 strain(sweeps);
-                if (colliders) collide(c);
+                ground(c, colliders);
             }
-            if (self) selfContact(c, 0);
-            strain(sweeps);
-            for (int pass = 0; pass < 2; ++pass)
-                if (colliders) collide(c);
-            for (int certificate = 0; certificate < 8; ++certificate) {
+            for (int round = 0; round <= 8; ++round) {
                 if (self) selfContact(c, 0);
                 strain(sweeps);
-                for (int pass = 0; pass < 2; ++pass)
-                    if (colliders) collide(c);
+                for (int pass = 0; pass < 2; ++pass) finalContacts(c, colliders);
             }
         }
+        buildYarnContacts(c);
         finalize(c);
+        finalizeFruit(c);
+        yarnFriction(c);
+        pairFriction(c);
+        fruitGroundFriction(c);
+        orientation(c);
+        release(c);
     }
 };
 
@@ -733,6 +1360,20 @@ typedef struct NumiClothOracleOutput {
     uint32_t sweptContacts;
     uint32_t candidateCount;  /* candidates of the last broad phase */
     uint32_t reserved;
+    double* fruits;           /* F x 20: the five float4 fields of a fruit record */
+    double* fruitPairs;       /* FP x 4: contact (normal impulse vector, impulse) */
+    double* yarnContacts;     /* Y x 24: the six float4/uint4 fields after identity */
+    double* mouthClearance;   /* F: largest release clearance over the run */
+    double maximumYarnAeroForce;
+    double maximumFruitAeroForce;
+    double maximumFruitAeroTorque;
+    uint32_t candidateMask;
+    uint32_t releasedMask;
+    uint32_t pairFrictionContacts;
+    uint32_t yarnFrictionContacts;
+    uint32_t fruitGroundContacts;
+    uint32_t rollingContacts;
+    double maximumRollingRatio;
 } NumiClothOracleOutput;
 
 }  // extern "C"
@@ -741,7 +1382,6 @@ template <typename R>
 static int runOracle(const NumiClothHipSceneDesc* scene, const NumiClothBagGPUConfig* configs, uint32_t substeps,
                      uint32_t iterations, uint32_t sweeps, NumiClothOracleOutput* out) {
     if (scene == nullptr || configs == nullptr || out == nullptr) return 1;
-    if (scene->fruitCount != 0 || scene->yarnContactCount != 0) return 5;
     Oracle<R> o(*scene);
     for (uint32_t k = 0; k < substeps; ++k) o.substep(configs[k], iterations, sweeps);
     for (uint32_t i = 0; i < o.P; ++i) {
@@ -780,6 +1420,40 @@ static int runOracle(const NumiClothHipSceneDesc* scene, const NumiClothBagGPUCo
     out->presentContacts = o.presentContacts;
     out->sweptContacts = o.sweptContacts;
     out->candidateCount = static_cast<uint32_t>(o.candidates.size());
+    for (uint32_t i = 0; i < o.fruit.size() && out->fruits; ++i) {
+        const auto& f = o.fruit[i];
+        const double r[20] = {f.x.x,    f.x.y,    f.x.z,    f.invMass, f.prev.x, f.prev.y, f.prev.z,
+                              f.radius, f.v.x,    f.v.y,    f.v.z,     f.groundImpulse, f.w.x, f.w.y,
+                              f.w.z,    0.0,      f.q[0],   f.q[1],    f.q[2],   f.q[3]};
+        std::copy(r, r + 20, out->fruits + 20 * i);
+    }
+    for (uint32_t k = 0; k < o.pairContact.size() && out->fruitPairs; ++k) {
+        const auto& c = o.pairContact[k];
+        const double r[4] = {c.n.x, c.n.y, c.n.z, c.j};
+        std::copy(r, r + 4, out->fruitPairs + 4 * k);
+    }
+    for (uint32_t k = 0; k < o.yarn.size() && out->yarnContacts; ++k) {
+        const auto& y = o.yarn[k];
+        const double r[24] = {y.normal.x, y.normal.y, y.normal.z, y.separation,
+                              y.sweptNormal.x, y.sweptNormal.y, y.sweptNormal.z, y.advance,
+                              y.wCurrent, y.wSwept, y.time, y.target,
+                              double(y.overlap), double(y.impact), double(y.degenerate), double(y.responses),
+                              y.fruitN.x, y.fruitN.y, y.fruitN.z, y.fruitJ,
+                              y.segA, y.segB, y.firstTime, y.sweptAdvance};
+        std::copy(r, r + 24, out->yarnContacts + 24 * k);
+    }
+    for (uint32_t i = 0; i < o.mouthClearance.size() && out->mouthClearance; ++i)
+        out->mouthClearance[i] = o.mouthClearance[i];
+    out->maximumYarnAeroForce = o.maximumYarnAeroForce;
+    out->maximumFruitAeroForce = o.maximumFruitAeroForce;
+    out->maximumFruitAeroTorque = o.maximumFruitAeroTorque;
+    out->candidateMask = o.candidateMask;
+    out->releasedMask = o.releasedMask;
+    out->pairFrictionContacts = o.pairFrictionContacts;
+    out->yarnFrictionContacts = o.yarnFrictionContacts;
+    out->fruitGroundContacts = o.fruitGroundContacts;
+    out->rollingContacts = o.rollingContacts;
+    out->maximumRollingRatio = o.maximumRollingRatio;
     return 0;
 }
 

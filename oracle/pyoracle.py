@@ -21,7 +21,13 @@ class OracleOutput(C.Structure):
                 ("knotLambda", C.c_void_p), ("gripLambda", C.c_void_p), ("gripParticle", C.c_void_p),
                 ("failure", C.c_uint32), ("frictionContacts", C.c_uint32), ("maximumConeRatio", C.c_double),
                 ("presentContacts", C.c_uint32), ("sweptContacts", C.c_uint32), ("candidateCount", C.c_uint32),
-                ("reserved", C.c_uint32)]
+                ("reserved", C.c_uint32), ("fruits", C.c_void_p), ("fruitPairs", C.c_void_p),
+                ("yarnContacts", C.c_void_p), ("mouthClearance", C.c_void_p), ("maximumYarnAeroForce", C.c_double),
+                ("maximumFruitAeroForce", C.c_double), ("maximumFruitAeroTorque", C.c_double),
+                ("candidateMask", C.c_uint32),
+                ("releasedMask", C.c_uint32), ("pairFrictionContacts", C.c_uint32),
+                ("yarnFrictionContacts", C.c_uint32), ("fruitGroundContacts", C.c_uint32),
+                ("rollingContacts", C.c_uint32), ("maximumRollingRatio", C.c_double)]
 
 
 _lib = None
@@ -57,6 +63,8 @@ def run(scene, configs: np.ndarray, iterations: int, strain_sweeps: int, precisi
         "groundImpulse": np.zeros(P), "distanceLambda": np.zeros(len(scene.distances)),
         "bendLambda": np.zeros(len(scene.bends)), "knotLambda": np.zeros(len(scene.knots)),
         "gripLambda": np.zeros((len(scene.grips), 3)), "gripParticle": np.zeros(len(scene.grips), np.uint32),
+        "fruits": np.zeros((len(scene.fruits), 20)), "fruitPairs": np.zeros((len(scene.fruitPairs), 4)),
+        "yarnContacts": np.zeros((len(scene.yarnContacts), 24)), "mouthClearance": np.zeros(len(scene.fruits)),
     }
     result = OracleOutput()
     for name, array in out.items():
@@ -72,6 +80,11 @@ def run(scene, configs: np.ndarray, iterations: int, strain_sweeps: int, precisi
     out["presentContacts"] = int(result.presentContacts)
     out["sweptContacts"] = int(result.sweptContacts)
     out["candidateCount"] = int(result.candidateCount)
+    for name in ["candidateMask", "releasedMask", "pairFrictionContacts", "yarnFrictionContacts",
+                 "fruitGroundContacts", "rollingContacts"]:
+        out[name] = int(getattr(result, name))
+    for name in ["maximumRollingRatio", "maximumYarnAeroForce", "maximumFruitAeroForce", "maximumFruitAeroTorque"]:
+        out[name] = float(getattr(result, name))
     return out
 
 

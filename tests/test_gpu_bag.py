@@ -1,0 +1,175 @@
+"""Produce-bag families on the HIP path (needs a GPU): fruit pairs, fruit
+ground contact and friction, fruit/yarn swept contact, air loads, mouth
+release, and one frame of the 1,465-knot bag with 12 fruits.
+
+Each probe is compared with the FP32 twin of the oracle (same schedule and
+FP32 operation order; 1e-6 of the probe's scale) and with the FP32 values
+the reference publishes for the same probe (reference
+docs/METAL_CLOTH.md:366-406). The bag frame uses the sheet frame gate.
+"""
+
+import numpy as np
+import pytest
+
+import probes
+from test_gpu_core import _check_frame, _positions
+
+pytestmark = pytest.mark.gpu
+
+
+def _fruits(state):
+    f = state["fruits"]
+    return np.concatenate([f[name].astype(np.float64) for name in
+                           ("positionAndInverseMass", "previousAndRadius", "velocityAndGroundImpulse",
+                            "angularVelocity", "orientation")], axis=1)
+
+
+def _yarn(state):
+    y = state["yarnContacts"]
+    return np.concatenate([y[name].astype(np.float64) for name in
+                           ("currentNormalAndSeparation", "sweptNormalAndAdvance", "weightsAndTime", "control",
+                            "fruitNormalAndImpulse", "segmentImpulse")], axis=1)
+
+
+def _bits(word):
+    return float(np.array([word], np.uint32).view(np.float32)[0])
+
+
+def _run(cloth, scene, substeps, iterations, sweeps):
+    with cloth.Solver(scene) as solver:
+        solver.step(scene.configs(substeps), iterations, sweeps)
+        return solver.read()
+
+
+def _compare(label, gpu, twin, tolerance):
+    error = float(np.max(np.abs(gpu - twin))) if gpu.size else 0.0
+    print(f"{label}: max |gpu - twin| = {error:.3e}")
+    assert error <= tolerance
+
+
+def test_fruit_pair_probe(cloth, oracle):
+    scene = probes.fruit_pair_probe(cloth)
+    state = _run(cloth, scene, 1, 1, 0)
+    twin = oracle.run(scene, scene.configs(1), 1, 0, precision="fp32")
+    assert state["failure"] == 0
+    fruits = _fruits(state)
+    _compare("fruit pair fruits", fruits, twin["fruits"], 4e-6)
+    _compare("fruit pair contact", state["fruitPairs"]["contact"].astype(np.float64), twin["fruitPairs"], 4e-6)
+    separation = np.linalg.norm(fruits[1, :3] - fruits[0, :3])
+    assert separation == pytest.approx(1.999999962672, abs=1e-7)
+    assert state["fruitPairs"]["contact"][0, 3] == pytest.approx(16.656669616699, abs=4e-6)
+    counters = state["frictionStatus"]["counters"][0]
+    assert counters[0] == 1
+    assert _bits(state["frictionStatus"]["metrics"][0, 0]) == pytest.approx(0.057165719569, abs=1e-6)
+
+
+def test_fruit_ground_probe(cloth, oracle):
+    scene = probes.ground_contact_probe(cloth)
+    state = _run(cloth, scene, 1, 1, 0)
+    twin = oracle.run(scene, scene.configs(1), 1, 0, precision="fp32")
+    assert state["failure"] == 0
+    fruits = _fruits(state)
+    _compare("fruit ground", fruits, twin["fruits"], 1e-6)
+    assert fruits[0, 2] == 1.0
+    assert fruits[0, 11] == pytest.approx(25.0, abs=1e-5)
+    counters = state["frictionStatus"]["counters"][0]
+    metrics = state["frictionStatus"]["metrics"][0]
+    assert tuple(counters[2:4]) == (1, 1) and metrics[2] == 1
+    assert _bits(metrics[0]) == pytest.approx(1.0, abs=1e-6)
+    assert _bits(metrics[1]) == pytest.approx(1.0, abs=1e-6)
+    omega = fruits[0, 12:15]
+    # Published rolling_speed_after=0.342739083985.
+    assert np.hypot(omega[0], omega[1]) == pytest.approx(0.342739083985, abs=1e-6)
+
+
+def test_yarn_ccd_probe(cloth, oracle):
+    scene = probes.yarn_ccd_probe(cloth)
+    state = _run(cloth, scene, 1, 0, 0)
+    twin = oracle.run(scene, scene.configs(1), 0, 0, precision="fp32")
+    assert state["failure"] == 0
+    yarn = _yarn(state)
+    _compare("yarn ccd record", yarn, twin["yarnContacts"], 2e-6)
+    _compare("yarn ccd fruit", _fruits(state), twin["fruits"], 2e-6)
+    # Published impact_time=0.380000025034 removed_advance=0.123999990523
+    # normal_impulse=12.399999618530 final_fruit_x=-0.023999996483.
+    assert yarn[0, 22] == pytest.approx(0.380000025034, abs=1e-7)
+    assert yarn[0, 23] == pytest.approx(0.123999990523, abs=1e-7)
+    assert yarn[0, 19] == pytest.approx(12.399999618530, abs=2e-6)
+    assert yarn[0, 15] == 1
+    assert state["fruits"]["positionAndInverseMass"][0, 0] == pytest.approx(-0.023999996483, abs=1e-7)
+    assert state["frictionStatus"]["counters"][0, 1] == 1
+
+
+@pytest.mark.parametrize("refined, co_moving", [(False, False), (True, False), (False, True)])
+def test_yarn_air_probe(cloth, oracle, refined, co_moving):
+    scene = probes.yarn_aero_probe(cloth, refined, co_moving)
+    state = _run(cloth, scene, 1, 0, 0)
+    twin = oracle.run(scene, scene.configs(1), 0, 0, precision="fp32")
+    assert state["failure"] == 0
+    velocity = state["particles"]["velocity"][:, :3].astype(np.float64)
+    _compare(f"yarn air refined={refined} co_moving={co_moving}", velocity, twin["velocities"], 4e-6)
+    force = _bits(state["aerodynamicsStatus"]["metrics"][0, 0])
+    assert force == pytest.approx(twin["maximumYarnAeroForce"], rel=1e-6, abs=1e-12)
+    if not refined and not co_moving:
+        assert force == pytest.approx(0.172502279282, abs=1e-7)
+
+
+def test_fruit_air_probe(cloth, oracle):
+    scene = probes.fruit_aero_probe(cloth)
+    state = _run(cloth, scene, 1, 0, 0)
+    twin = oracle.run(scene, scene.configs(1), 0, 0, precision="fp32")
+    assert state["failure"] == 0
+    _compare("fruit air", _fruits(state), twin["fruits"], 2e-6)
+    metrics = state["aerodynamicsStatus"]["metrics"][0]
+    assert _bits(metrics[1]) == pytest.approx(0.132944747806, abs=1e-7)
+    assert _bits(metrics[2]) == pytest.approx(0.000002209812, abs=1e-11)
+
+
+@pytest.mark.parametrize("case", sorted(probes.MOUTH_CASES))
+def test_mouth_release_probe(cloth, oracle, case):
+    (position, rotated, initial), released = probes.MOUTH_CASES[case]
+    scene = probes.mouth_probe(cloth, position, rotated, initial)
+    state = _run(cloth, scene, 1, 0, 0)
+    twin = oracle.run(scene, scene.configs(1), 0, 0, precision="fp32")
+    assert state["failure"] == 0
+    masks = state["releaseStatus"]["masks"][0]
+    assert masks[1] == released == twin["releasedMask"]
+    assert masks[0] == twin["candidateMask"]
+    # The reference's mouth gate: clearance within 2 um of FP64.
+    fp64 = oracle.run(scene, scene.configs(1), 0, 0)
+    assert abs(float(state["mouthClearance"][0]) - fp64["mouthClearance"][0]) <= 2.0e-6
+
+
+def _bag(cloth):
+    return cloth.Scene.bag()
+
+
+def test_bag_frame_parity(cloth, oracle):
+    """One 1/120 s frame (48 substeps) of the 1,465-knot bag holding 12 fruits."""
+    scene = _bag(cloth)
+    substeps, iterations, sweeps = 48, 32, 3
+    configs = scene.configs(substeps)
+    state = _run(cloth, scene, substeps, iterations, sweeps)
+    assert state["failure"] == 0
+    _check_frame(oracle, scene, state, configs, iterations, sweeps, "bag")
+    twin = oracle.run(scene, configs, iterations, sweeps, precision="fp32")
+    fp64 = oracle.run(scene, configs, iterations, sweeps)
+    fruits = _fruits(state)
+    x = scene.particles["positionAndInverseMass"][:, :3]
+    extent = float(np.max(x.max(axis=0) - x.min(axis=0)))
+    e_twin = np.max(np.abs(fruits[:, :3] - twin["fruits"][:, :3]))
+    e_fp64 = np.max(np.abs(fruits[:, :3] - fp64["fruits"][:, :3]))
+    e_ref = np.max(np.abs(twin["fruits"][:, :3] - fp64["fruits"][:, :3]))
+    print(f"bag fruits: |gpu-twin| {e_twin:.3e} |gpu-fp64| {e_fp64:.3e} |twin-fp64| {e_ref:.3e}")
+    assert e_twin <= 1e-6 * extent
+    assert e_fp64 <= max(1e-5 * extent, 1.5 * e_ref)
+    masks = state["releaseStatus"]["masks"][0]
+    assert masks[0] == twin["candidateMask"] and masks[1] == twin["releasedMask"]
+
+
+def test_bag_runs_are_bit_identical(cloth):
+    scene = _bag(cloth)
+    first = _run(cloth, scene, 24, 32, 3)
+    second = _run(cloth, scene, 24, 32, 3)
+    for name in ("particles", "fruits", "fruitPairs", "yarnContacts", "frictionStatus", "aerodynamicsStatus"):
+        assert first[name].tobytes() == second[name].tobytes(), name
