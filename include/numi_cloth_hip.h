@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define NUMI_CLOTH_HIP_EXTENSION_VERSION 1u
+#define NUMI_CLOTH_HIP_EXTENSION_VERSION 2u
 
 enum NumiClothHipStatus {
     NUMI_CLOTH_HIP_OK = 0,
@@ -68,6 +68,11 @@ typedef struct NumiClothHipSceneExtension {
      * gravity increment and static equilibria are unchanged.
      * 0 reproduces the reference. */
     float damping;
+    /* Optional, one per particle: (drive amplitude, drive omega, drive
+     * phase, damping). When set it replaces the four scalars above for
+     * every particle, so independent specimens merged into one scene keep
+     * their own drive and damping. NULL uses the scalars. */
+    const mr_float4* particleMotion;
 } NumiClothHipSceneExtension;
 
 /*
@@ -236,6 +241,37 @@ int numi_cloth_hip_self_broad_phase(NumiClothHipSolver* solver, NumiClothHipSelf
 void* numi_cloth_hip_stream(NumiClothHipSolver* solver);
 
 int numi_cloth_hip_destroy(NumiClothHipSolver* solver);
+
+/*
+ * Batched specimens (no counterpart in the reference, which steps one scene
+ * per runGPU call): many independent scenes merged into one device scene
+ * (numi_cloth_scene_merge) and stepped together, so that every colour
+ * launch, LDS tile program and hipGraph replay covers all specimens. Each
+ * specimen evolves bit for bit as it would alone.
+ */
+typedef struct NumiClothHipBatch NumiClothHipBatch;
+
+int numi_cloth_hip_batch_create(const NumiClothHipSceneDesc* const* specimens,
+                                uint32_t count, NumiClothHipBatch** out);
+
+/* configs[k] drives substep k of every specimen; its count fields are
+ * replaced by the batch totals. */
+int numi_cloth_hip_batch_step(NumiClothHipBatch* batch,
+                              const NumiClothBagGPUConfig* configs,
+                              uint32_t substeps, uint32_t iterations,
+                              uint32_t strainSweeps);
+
+/* Specimen `specimen`'s particles and constraint records in its own order and
+ * indexing; failure and friction status are those of the whole batch. */
+int numi_cloth_hip_batch_read(NumiClothHipBatch* batch, uint32_t specimen,
+                              const NumiClothHipState* out);
+
+/* Borrowed solver of the merged scene (synchronize, diagnostics, tile_info,
+ * time_pass, stream) and its scene; valid until batch_destroy. */
+NumiClothHipSolver* numi_cloth_hip_batch_solver(NumiClothHipBatch* batch);
+const NumiClothHipSceneDesc* numi_cloth_hip_batch_scene(const NumiClothHipBatch* batch);
+
+int numi_cloth_hip_batch_destroy(NumiClothHipBatch* batch);
 
 #ifdef __cplusplus
 }

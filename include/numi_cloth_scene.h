@@ -149,8 +149,61 @@ typedef struct NumiClothHipBagSpec {
     uint64_t seed;
 } NumiClothHipBagSpec;
 
+/* A hanging strip specimen (batched calibration and adjoint workloads): a
+ * width x height knot grid in the x-z plane, top row (j = 0) pinned and
+ * driven vertically by amplitude * sin(2 pi frequency t + phase), no
+ * ground, warp/weft distances and chord bends. Knot (i, j) sits at
+ * origin + (i spacing, jitter, -j spacing), row-major. */
+typedef struct NumiClothHipStripSpec {
+    uint32_t width;
+    uint32_t height;
+    float spacing;
+    float arealDensity;         /* kg/m^2; knot mass = density * spacing^2 */
+    float yarnRadius;
+    float stretchCompliance;
+    float bendCompliance;
+    float strainLimit;
+    float timestep;
+    float gravityZ;
+    float driveAmplitude;       /* m */
+    float driveFrequency;       /* Hz */
+    float drivePhase;           /* rad */
+    float damping;              /* 1/s, see NumiClothHipSceneExtension */
+    float origin[3];
+    float jitter;               /* y offsets uniform in [-jitter, jitter] */
+    uint64_t seed;
+} NumiClothHipStripSpec;
+
 int numi_cloth_scene_sheet(const NumiClothHipSheetSpec* spec,
                            NumiClothHipScene** out);
+int numi_cloth_scene_strip(const NumiClothHipStripSpec* spec,
+                           NumiClothHipScene** out);
+
+/* Merges independent specimens into one scene that a single solver steps:
+ * particles are concatenated in specimen order; the constraints of colour c
+ * of every specimen form colour batch c of the merged scene (specimen order,
+ * then each specimen's own batch order), so one colour launch covers all
+ * specimens and every specimen evolves exactly as it would alone. Each
+ * specimen's drive and damping become per-particle motion. All specimens
+ * must share timestep, gravity, yarn radius and the ground flag, and have no
+ * grips, fruits, mouth, static spheres, air loads or self-contact (those
+ * couple particles beyond a specimen). */
+int numi_cloth_scene_merge(const NumiClothHipSceneDesc* const* specimens,
+                           uint32_t count, NumiClothHipScene** out);
+
+enum NumiClothHipMergeFamily {
+    NUMI_CLOTH_HIP_MERGE_PARTICLE = 0,
+    NUMI_CLOTH_HIP_MERGE_DISTANCE = 1,
+    NUMI_CLOTH_HIP_MERGE_BEND = 2,
+    NUMI_CLOTH_HIP_MERGE_KNOT = 3
+};
+
+/* Where specimen `specimen`'s records of `family` went: *indices points at
+ * *count merged indices, one per record of the specimen in its own order
+ * (borrowed; valid until the merged scene is destroyed). */
+int numi_cloth_scene_merge_indices(const NumiClothHipScene* merged, uint32_t family,
+                                   uint32_t specimen, const uint32_t** indices,
+                                   uint32_t* count);
 int numi_cloth_scene_bag(const NumiClothHipBagSpec* spec,
                          NumiClothHipScene** out);
 /* Borrowed view; valid until numi_cloth_scene_destroy. */

@@ -74,7 +74,7 @@ class SceneExtension(C.Structure):
     _fields_ = [("staticSpheres", C.c_void_p), ("staticSphereCount", C.c_uint32), ("selfContact", C.c_uint32),
                 ("selfContactMargin", C.c_float), ("selfCandidateCapacity", C.c_uint32),
                 ("driveAmplitude", C.c_float), ("driveOmega", C.c_float), ("drivePhase", C.c_float),
-                ("damping", C.c_float)]
+                ("damping", C.c_float), ("particleMotion", C.c_void_p)]
 
 
 _DESC_POINTERS = ["config", "particles", "distances", "distanceBatches", "grips", "knots", "knotBatches", "bends",
@@ -165,6 +165,14 @@ class SheetSpec(C.Structure):
                 ("seed", C.c_uint64)]
 
 
+class StripSpec(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("spacing", C.c_float), ("arealDensity", C.c_float),
+                ("yarnRadius", C.c_float), ("stretchCompliance", C.c_float), ("bendCompliance", C.c_float),
+                ("strainLimit", C.c_float), ("timestep", C.c_float), ("gravityZ", C.c_float),
+                ("driveAmplitude", C.c_float), ("driveFrequency", C.c_float), ("drivePhase", C.c_float),
+                ("damping", C.c_float), ("origin", C.c_float * 3), ("jitter", C.c_float), ("seed", C.c_uint64)]
+
+
 class BagSpec(C.Structure):
     _fields_ = [("material", MaterialValues), ("stretchComplianceOverride", C.c_float),
                 ("bendComplianceOverride", C.c_float), ("strainLimitOverride", C.c_float),
@@ -213,6 +221,15 @@ def library() -> C.CDLL:
         "numi_cloth_grip_trajectory_destroy": ([vp], None),
         "numi_cloth_scene_sheet": ([C.POINTER(SheetSpec), C.POINTER(vp)], i32),
         "numi_cloth_scene_bag": ([C.POINTER(BagSpec), C.POINTER(vp)], i32),
+        "numi_cloth_scene_strip": ([C.POINTER(StripSpec), C.POINTER(vp)], i32),
+        "numi_cloth_scene_merge": ([C.POINTER(vp), u32, C.POINTER(vp)], i32),
+        "numi_cloth_scene_merge_indices": ([vp, u32, u32, C.POINTER(C.POINTER(u32)), C.POINTER(u32)], i32),
+        "numi_cloth_hip_batch_create": ([C.POINTER(vp), u32, C.POINTER(vp)], i32),
+        "numi_cloth_hip_batch_step": ([vp, vp, u32, u32, u32], i32),
+        "numi_cloth_hip_batch_read": ([vp, u32, C.POINTER(State)], i32),
+        "numi_cloth_hip_batch_solver": ([vp], vp),
+        "numi_cloth_hip_batch_scene": ([vp], C.POINTER(SceneDesc)),
+        "numi_cloth_hip_batch_destroy": ([vp], i32),
         "numi_cloth_scene_desc": ([vp], C.POINTER(SceneDesc)),
         "numi_cloth_scene_destroy": ([vp], None),
         "numi_cloth_splitmix64": ([C.POINTER(C.c_uint64)], C.c_uint64),
@@ -310,6 +327,8 @@ class Scene:
         self.driveOmega = float(arrays.pop("driveOmega", 0.0))
         self.drivePhase = float(arrays.pop("drivePhase", 0.0))
         self.damping = float(arrays.pop("damping", 0.0))
+        motion = arrays.pop("particleMotion", None)
+        self.particleMotion = None if motion is None else np.ascontiguousarray(motion, np.float32).reshape(-1, 4)
         self.initialCandidateMask = int(arrays.pop("initialCandidateMask", 0))
         self.initialReleasedMask = int(arrays.pop("initialReleasedMask", 0))
         if arrays:
@@ -335,6 +354,7 @@ class Scene:
         e.driveOmega = self.driveOmega
         e.drivePhase = self.drivePhase
         e.damping = self.damping
+        e.particleMotion = _ptr(self.particleMotion)
         return desc
 
     def refresh(self) -> None:
@@ -342,7 +362,7 @@ class Scene:
         self.desc = self._describe()
 
     @classmethod
-    def _from_builder(cls, handle) -> "Scene":
+    def _from_builder(cls, handle, keep: bool = False):
         lib = library()
         d = lib.numi_cloth_scene_desc(handle).contents
         arrays = {}
@@ -360,10 +380,16 @@ class Scene:
         if e.staticSphereCount:
             raw = (C.c_float * (4 * e.staticSphereCount)).from_address(e.staticSpheres)
             spheres = np.frombuffer(raw, dtype=np.float32).reshape(-1, 4).copy()
+        motion = None
+        if e.particleMotion:
+            raw = (C.c_float * (4 * d.particleCount)).from_address(e.particleMotion)
+            motion = np.frombuffer(raw, dtype=np.float32).reshape(-1, 4).copy()
         scene = cls(config, staticSpheres=spheres, selfContact=e.selfContact, selfContactMargin=e.selfContactMargin,
                     selfCandidateCapacity=e.selfCandidateCapacity, driveAmplitude=e.driveAmplitude,
-                    driveOmega=e.driveOmega, drivePhase=e.drivePhase, damping=e.damping,
+                    driveOmega=e.driveOmega, drivePhase=e.drivePhase, damping=e.damping, particleMotion=motion,
                     initialCandidateMask=d.initialCandidateMask, initialReleasedMask=d.initialReleasedMask, **arrays)
+        if keep:
+            return scene, handle
         lib.numi_cloth_scene_destroy(handle)
         return scene
 
@@ -394,12 +420,125 @@ class Scene:
         _check(library().numi_cloth_scene_bag(C.byref(spec), C.byref(handle)))
         return cls._from_builder(handle)
 
+    @classmethod
+    def strip(cls, **fields) -> "Scene":
+        """A hanging strip specimen (see NumiClothHipStripSpec)."""
+        spec = StripSpec()
+        defaults = dict(spacing=0.01, arealDensity=0.2, yarnRadius=0.002, stretchCompliance=1.0e-7,
+                        bendCompliance=1.0e-5, strainLimit=0.10, timestep=1.0 / 2880.0, gravityZ=-9.81,
+                        driveAmplitude=0.01, driveFrequency=1.0, drivePhase=0.0, damping=0.0, origin=(0.0, 0.0, 1.0),
+                        jitter=1.0e-4, seed=1)
+        defaults.update(fields)
+        for key, value in defaults.items():
+            if key == "origin":
+                spec.origin = (C.c_float * 3)(*value)
+            else:
+                setattr(spec, key, value)
+        handle = C.c_void_p()
+        _check(library().numi_cloth_scene_strip(C.byref(spec), C.byref(handle)))
+        return cls._from_builder(handle)
+
+    @classmethod
+    def merge(cls, specimens: list["Scene"]) -> tuple["Scene", list[dict]]:
+        """The merged scene of independent specimens and, per specimen, the
+        merged index of each of its particles / distances / bends / knots."""
+        lib = library()
+        pointers = (C.c_void_p * len(specimens))(*[C.addressof(s.desc) for s in specimens])
+        handle = C.c_void_p()
+        _check(lib.numi_cloth_scene_merge(pointers, len(specimens), C.byref(handle)))
+        maps = []
+        for k in range(len(specimens)):
+            entry = {}
+            for family, name in enumerate(["particles", "distances", "bends", "knots"]):
+                indices = C.POINTER(C.c_uint32)()
+                count = C.c_uint32()
+                _check(lib.numi_cloth_scene_merge_indices(handle, family, k, C.byref(indices), C.byref(count)))
+                entry[name] = np.ctypeslib.as_array(indices, (count.value,)).copy() if count.value else \
+                    np.zeros(0, np.uint32)
+            maps.append(entry)
+        scene = cls._from_builder(handle)
+        return scene, maps
+
     def configs(self, substeps: int) -> np.ndarray:
         """`substeps` copies of the initial configuration."""
         return np.repeat(self.config, substeps)
 
 
 # ------------------------------------------------------------------- solver --
+
+def _state_arrays(particles, distances, grips, knots, bends, fruits, fruit_pairs, yarn_contacts) -> dict:
+    return {
+        "particles": np.zeros(particles, PARTICLE),
+        "distances": np.zeros(distances, DISTANCE),
+        "grips": np.zeros(grips, GRIP),
+        "knots": np.zeros(knots, KNOT),
+        "bends": np.zeros(bends, BEND),
+        "fruits": np.zeros(fruits, FRUIT),
+        "fruitPairs": np.zeros(fruit_pairs, FRUIT_PAIR),
+        "yarnContacts": np.zeros(yarn_contacts, YARN_CONTACT),
+        "selfStatus": np.zeros(1, SELF_STATUS),
+        "frictionStatus": np.zeros(1, FRICTION_STATUS),
+        "aerodynamicsStatus": np.zeros(1, AERO_STATUS),
+        "releaseStatus": np.zeros(1, RELEASE_STATUS),
+        "failure": np.zeros(1, np.uint32),
+        "mouthClearance": np.zeros(fruits, np.float32),
+    }
+
+
+class Batch:
+    """Independent specimens stepped together by one device solver."""
+
+    def __init__(self, specimens: list[Scene]):
+        self.specimens = specimens
+        pointers = (C.c_void_p * len(specimens))(*[C.addressof(s.desc) for s in specimens])
+        handle = C.c_void_p()
+        _check(library().numi_cloth_hip_batch_create(pointers, len(specimens), C.byref(handle)))
+        self._handle = handle
+
+    def close(self) -> None:
+        if getattr(self, "_handle", None):
+            _check(library().numi_cloth_hip_batch_destroy(self._handle))
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def solver_handle(self) -> int:
+        return library().numi_cloth_hip_batch_solver(self._handle)
+
+    def step(self, configs: np.ndarray, iterations: int = 32, strain_sweeps: int = 3) -> None:
+        configs = np.ascontiguousarray(configs, dtype=CONFIG)
+        _check(library().numi_cloth_hip_batch_step(self._handle, configs.ctypes.data, len(configs), iterations,
+                                                   strain_sweeps))
+
+    def synchronize(self) -> None:
+        _check(library().numi_cloth_hip_synchronize(self.solver_handle))
+
+    def tile_info(self) -> "TileInfo":
+        info = TileInfo()
+        _check(library().numi_cloth_hip_tile_info(self.solver_handle, C.byref(info)))
+        return info
+
+    def read(self, specimen: int) -> dict:
+        s = self.specimens[specimen]
+        out = _state_arrays(len(s.particles), len(s.distances), 0, len(s.knots), len(s.bends), 0, 0, 0)
+        state = State()
+        for name, array in out.items():
+            setattr(state, name, _ptr(array))
+        _check(library().numi_cloth_hip_batch_read(self._handle, specimen, C.byref(state)))
+        out["failure"] = int(out["failure"][0])
+        return out
+
 
 class Solver:
     """Device-resident cloth state on the current HIP device."""
@@ -441,22 +580,8 @@ class Solver:
 
     def read(self) -> dict:
         s = self.scene
-        out = {
-            "particles": np.zeros(len(s.particles), PARTICLE),
-            "distances": np.zeros(len(s.distances), DISTANCE),
-            "grips": np.zeros(len(s.grips), GRIP),
-            "knots": np.zeros(len(s.knots), KNOT),
-            "bends": np.zeros(len(s.bends), BEND),
-            "fruits": np.zeros(len(s.fruits), FRUIT),
-            "fruitPairs": np.zeros(len(s.fruitPairs), FRUIT_PAIR),
-            "yarnContacts": np.zeros(len(s.yarnContacts), YARN_CONTACT),
-            "selfStatus": np.zeros(1, SELF_STATUS),
-            "frictionStatus": np.zeros(1, FRICTION_STATUS),
-            "aerodynamicsStatus": np.zeros(1, AERO_STATUS),
-            "releaseStatus": np.zeros(1, RELEASE_STATUS),
-            "failure": np.zeros(1, np.uint32),
-            "mouthClearance": np.zeros(len(s.fruits), np.float32),
-        }
+        out = _state_arrays(len(s.particles), len(s.distances), len(s.grips), len(s.knots), len(s.bends),
+                            len(s.fruits), len(s.fruitPairs), len(s.yarnContacts))
         state = State()
         for name, array in out.items():
             setattr(state, name, _ptr(array))

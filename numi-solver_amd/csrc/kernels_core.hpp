@@ -39,12 +39,15 @@ struct BeginArgs {
     float4* prev;
     float4* vel;
     const float4* anchor;       // kinematic drive anchors, or null
+    const float4* motion;       // per knot (amplitude, omega, phase, damping), or null
+    float4 drive;               // (amplitude, omega, phase, -) when motion is null
+    const double* time;         // time at the end of the substep
+    float dampingDt;            // timestep the per-knot damping factors use
     float* distanceLambda;
     float* bendLambda;
     float* knotLambda;
     NumiClothBagGPUGrip* grips;
     uint32_t particles, distances, bends, knots, grips_;
-    const float* driveOffset;   // z offset of driven knots at the substep end
     float dampingFactor;        // 1 / (1 + c dt), 1 when undamped
     uint32_t advance;           // 1: also x += v dt (nothing runs in between)
     uint32_t* failure;
@@ -71,14 +74,24 @@ __global__ void k_begin(BeginArgs a) {
             if (p.w > 0.0f) {
                 // Damping acts on the carried velocity only, so a resting
                 // knot still receives the full gravity increment.
-                const V3 carried = a.dampingFactor != 1.0f ? u * a.dampingFactor : u;
+                float factor = a.dampingFactor;
+                if (a.motion != nullptr) {
+                    const float c = a.motion[i].w;
+                    factor = c > 0.0f ? 1.0f / (1.0f + c * a.dampingDt) : 1.0f;
+                }
+                const V3 carried = factor != 1.0f ? u * factor : u;
                 const V3 nu = fma3(xyz(a.cfg->gravityAndTimestep), dt, carried);
                 v = withXyz(v, nu);
                 v.w = v.z;
                 if (a.advance) p = withXyz(p, fma3(nu, dt, x));
             } else if (a.anchor != nullptr) {
+                // Kinematic drive, evaluated in double like the host
+                // reference: z = anchor.z + A sin(omega t + phase).
                 const float4 anchor = a.anchor[i];
-                p = withXyz(p, v3(anchor.x, anchor.y, anchor.z + *a.driveOffset));
+                const float4 m = a.motion != nullptr ? a.motion[i] : a.drive;
+                const float offset = static_cast<float>(
+                    static_cast<double>(m.x) * sin(static_cast<double>(m.y) * *a.time + static_cast<double>(m.z)));
+                p = withXyz(p, v3(anchor.x, anchor.y, anchor.z + offset));
             }
             a.prev[i] = q;
             a.vel[i] = v;

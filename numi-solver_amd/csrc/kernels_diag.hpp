@@ -20,8 +20,9 @@ struct SubstepSlot {
     NumiClothBagGPUConfig config;   // live configuration
     uint32_t ok;                    // validConfig result
     uint32_t cursor;                // next substep index
-    float driveOffset;              // kinematic drive z offset
     uint32_t substep;               // index of the live substep
+    uint32_t reserved;
+    double time;                    // time at the end of the live substep
 };
 
 __device__ bool configValid(const NumiClothBagGPUConfig& c, uint32_t* failure) {
@@ -56,7 +57,7 @@ __device__ bool configValid(const NumiClothBagGPUConfig& c, uint32_t* failure) {
 
 // One thread. An ABI-mismatched config is not copied (the previous live
 // configuration stays), exactly as the reference's prepare kernel does.
-__global__ void k_prepare(const NumiClothBagGPUConfig* __restrict__ configs, const float* __restrict__ drive,
+__global__ void k_prepare(const NumiClothBagGPUConfig* __restrict__ configs, const double* __restrict__ times,
                           SubstepSlot* __restrict__ slot, uint32_t* __restrict__ failure) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     const uint32_t s = slot->cursor;
@@ -67,7 +68,7 @@ __global__ void k_prepare(const NumiClothBagGPUConfig* __restrict__ configs, con
         slot->config = next;
     }
     slot->ok = configValid(slot->config, failure) ? 1u : 0u;
-    slot->driveOffset = drive[s];
+    slot->time = times[s];
     slot->substep = s;
     slot->cursor = s + 1u;
 }

@@ -16,61 +16,8 @@
 
 #include "common.hpp"
 #include "numi_cloth_scene.h"
+#include "scene_internal.hpp"
 
-struct NumiClothHipScene {
-    NumiClothBagGPUConfig config{};
-    std::vector<NumiClothBagGPUParticle> particles;
-    std::vector<NumiClothBagGPUDistance> distances;
-    std::vector<NumiClothBagGPUBatch> distanceBatches;
-    std::vector<NumiClothBagGPUGrip> grips;
-    std::vector<NumiClothBagGPUKnot> knots;
-    std::vector<NumiClothBagGPUBatch> knotBatches;
-    std::vector<NumiClothBagGPUBend> bends;
-    std::vector<NumiClothBagGPUBatch> bendBatches;
-    std::vector<NumiClothBagGPUFruit> fruits;
-    std::vector<NumiClothBagGPUFruitPair> fruitPairs;
-    std::vector<NumiClothBagGPUBatch> fruitPairBatches;
-    std::vector<NumiClothBagGPUYarnContact> yarnContacts;
-    std::vector<uint32_t> mouthRim;
-    std::vector<uint32_t> cuff;
-    std::vector<mr_float4> spheres;
-    NumiClothHipSceneDesc desc{};
-
-    void publish() {
-        desc = NumiClothHipSceneDesc{};
-        desc.config = &config;
-        desc.particles = particles.data();
-        desc.distances = distances.data();
-        desc.distanceBatches = distanceBatches.data();
-        desc.grips = grips.data();
-        desc.knots = knots.data();
-        desc.knotBatches = knotBatches.data();
-        desc.bends = bends.data();
-        desc.bendBatches = bendBatches.data();
-        desc.fruits = fruits.data();
-        desc.fruitPairs = fruitPairs.data();
-        desc.fruitPairBatches = fruitPairBatches.data();
-        desc.yarnContacts = yarnContacts.data();
-        desc.mouthRimParticles = mouthRim.data();
-        desc.cuffParticles = cuff.data();
-        desc.particleCount = static_cast<uint32_t>(particles.size());
-        desc.distanceCount = static_cast<uint32_t>(distances.size());
-        desc.distanceBatchCount = static_cast<uint32_t>(distanceBatches.size());
-        desc.gripCount = static_cast<uint32_t>(grips.size());
-        desc.knotCount = static_cast<uint32_t>(knots.size());
-        desc.knotBatchCount = static_cast<uint32_t>(knotBatches.size());
-        desc.bendCount = static_cast<uint32_t>(bends.size());
-        desc.bendBatchCount = static_cast<uint32_t>(bendBatches.size());
-        desc.fruitCount = static_cast<uint32_t>(fruits.size());
-        desc.fruitPairCount = static_cast<uint32_t>(fruitPairs.size());
-        desc.fruitPairBatchCount = static_cast<uint32_t>(fruitPairBatches.size());
-        desc.yarnContactCount = static_cast<uint32_t>(yarnContacts.size());
-        desc.mouthRimCount = static_cast<uint32_t>(mouthRim.size());
-        desc.cuffCount = static_cast<uint32_t>(cuff.size());
-        desc.extension.staticSpheres = spheres.data();
-        desc.extension.staticSphereCount = static_cast<uint32_t>(spheres.size());
-    }
-};
 
 namespace numi_hip {
 namespace {
@@ -305,6 +252,226 @@ void buildSheet(const NumiClothHipSheetSpec& spec, NumiClothHipScene& s) {
     s.desc.extension.selfContact = spec.selfContact;
     // Broad-phase margin: a quarter yarn radius of travel within a substep.
     s.desc.extension.selfContactMargin = 0.25f * spec.yarnRadius;
+}
+
+// ---------------------------------------------------------------- strip ----
+
+constexpr double kPiScene = 3.14159265358979323846;
+
+void buildStrip(const NumiClothHipStripSpec& spec, NumiClothHipScene& s) {
+    if (spec.width < 2 || spec.height < 2 || !(spec.spacing > 0.0f) || !(spec.arealDensity > 0.0f) ||
+        !(spec.timestep > 0.0f) || !(spec.damping >= 0.0f)) {
+        fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT,
+             "strip spec needs width,height >= 2, positive spacing, density, timestep and damping >= 0");
+    }
+    const uint32_t W = spec.width;
+    const uint32_t H = spec.height;
+    const uint64_t count = static_cast<uint64_t>(W) * H;
+    if (count > (1ull << 26)) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "strip too large");
+    uint64_t rng = spec.seed;
+    const double h = spec.spacing;
+    const float mass = static_cast<float>(static_cast<double>(spec.arealDensity) * h * h);
+    s.particles.reserve(count);
+    for (uint32_t j = 0; j < H; ++j) {
+        for (uint32_t i = 0; i < W; ++i) {
+            const double jitter = spec.jitter * (2.0 * unitDouble(&rng) - 1.0);
+            const D3 p{spec.origin[0] + i * h, spec.origin[1] + jitter, spec.origin[2] - j * h};
+            s.particles.push_back(makeParticle(p, mass, j == 0));
+        }
+    }
+    auto at = [W](uint32_t i, uint32_t j) { return j * W + i; };
+    std::vector<EdgeItem> edges;
+    edges.reserve(2 * count);
+    for (uint32_t j = 0; j < H; ++j)
+        for (uint32_t i = 0; i + 1 < W; ++i) edges.push_back({at(i, j), at(i + 1, j), 1u, spec.stretchCompliance});
+    for (uint32_t j = 0; j + 1 < H; ++j)
+        for (uint32_t i = 0; i < W; ++i) edges.push_back({at(i, j), at(i, j + 1), 0u, spec.stretchCompliance});
+    emitDistances(s, edges, spec.strainLimit);
+    std::vector<BendItem> bends;
+    bends.reserve(2 * count);
+    for (uint32_t j = 0; j < H; ++j)
+        for (uint32_t i = 1; i + 1 < W; ++i) bends.push_back({at(i - 1, j), at(i, j), at(i + 1, j), spec.bendCompliance});
+    for (uint32_t j = 1; j + 1 < H; ++j)
+        for (uint32_t i = 0; i < W; ++i) bends.push_back({at(i, j - 1), at(i, j), at(i, j + 1), spec.bendCompliance});
+    emitBends(s, bends);
+
+    NumiClothBagGPUConfig& c = s.config;
+    c.control = u4(NUMI_CLOTH_BAG_GPU_ABI_VERSION, static_cast<uint32_t>(s.particles.size()),
+                   static_cast<uint32_t>(s.distances.size()), 0u);
+    c.constraintCounts = u4(0u, static_cast<uint32_t>(s.bends.size()), 0u, 0u);
+    c.gravityAndTimestep = f4(0.0f, 0.0f, spec.gravityZ, spec.timestep);
+    c.gripOrientation = f4(0.0f, 0.0f, 0.0f, 1.0f);
+    c.clothMaterial = f4(spec.yarnRadius, 0.0f, 0.0f, 0.0f);
+    s.publish();
+    s.desc.extension.driveAmplitude = spec.driveAmplitude;
+    s.desc.extension.driveOmega = static_cast<float>(2.0 * kPiScene * spec.driveFrequency);
+    s.desc.extension.drivePhase = spec.drivePhase;
+    s.desc.extension.damping = spec.damping;
+}
+
+// ---------------------------------------------------------------- merge ----
+
+template <typename Record, typename ColourOf, typename Shift>
+void mergeFamily(const std::vector<const NumiClothHipSceneDesc*>& specimens, const std::vector<uint32_t>& offsets,
+                 const Record* (*records)(const NumiClothHipSceneDesc&), uint32_t (*countOf)(const NumiClothHipSceneDesc&),
+                 const NumiClothBagGPUBatch* (*batchesOf)(const NumiClothHipSceneDesc&),
+                 uint32_t (*batchCountOf)(const NumiClothHipSceneDesc&), ColourOf colourOf, Shift shift,
+                 std::vector<Record>& merged, std::vector<NumiClothBagGPUBatch>& mergedBatches,
+                 std::vector<uint32_t>& start, std::vector<uint32_t>& index, const char* what) {
+    const uint32_t n = static_cast<uint32_t>(specimens.size());
+    start.assign(n + 1, 0);
+    for (uint32_t s = 0; s < n; ++s) start[s + 1] = start[s] + countOf(*specimens[s]);
+    index.assign(start[n], ~0u);
+    uint32_t colours = 0;
+    for (uint32_t s = 0; s < n; ++s) {
+        const uint32_t batches = batchCountOf(*specimens[s]);
+        const NumiClothBagGPUBatch* b = batchesOf(*specimens[s]);
+        if (batches != 0 && b == nullptr) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, std::string("null batches: ") + what);
+        for (uint32_t k = 0; k < batches; ++k) colours = std::max(colours, b[k].control.z + 1);
+    }
+    merged.clear();
+    mergedBatches.clear();
+    for (uint32_t colour = 0; colour < colours; ++colour) {
+        const uint32_t first = static_cast<uint32_t>(merged.size());
+        for (uint32_t s = 0; s < n; ++s) {
+            const NumiClothHipSceneDesc& d = *specimens[s];
+            const Record* r = records(d);
+            const NumiClothBagGPUBatch* b = batchesOf(d);
+            for (uint32_t k = 0; k < batchCountOf(d); ++k) {
+                if (b[k].control.z != colour) continue;
+                if (static_cast<uint64_t>(b[k].control.x) + b[k].control.y > countOf(d)) {
+                    fail(NUMI_CLOTH_HIP_ERROR_TOPOLOGY, std::string("batch out of range (FAILURE_RANGE): ") + what);
+                }
+                for (uint32_t j = b[k].control.x; j < b[k].control.x + b[k].control.y; ++j) {
+                    if (colourOf(r[j]) != colour) {
+                        fail(NUMI_CLOTH_HIP_ERROR_TOPOLOGY,
+                             std::string("record colour differs from its batch (FAILURE_BATCH): ") + what);
+                    }
+                    if (index[start[s] + j] != ~0u) {
+                        fail(NUMI_CLOTH_HIP_ERROR_TOPOLOGY, std::string("record in two batches: ") + what);
+                    }
+                    index[start[s] + j] = static_cast<uint32_t>(merged.size());
+                    Record copy = r[j];
+                    shift(copy, offsets[s]);
+                    merged.push_back(copy);
+                }
+            }
+        }
+        const uint32_t count = static_cast<uint32_t>(merged.size()) - first;
+        if (count != 0) mergedBatches.push_back({u4(first, count, colour, 0u)});
+    }
+    for (uint32_t k = 0; k < index.size(); ++k) {
+        if (index[k] == ~0u) fail(NUMI_CLOTH_HIP_ERROR_TOPOLOGY, std::string("record in no batch: ") + what);
+    }
+}
+
+void mergeScenes(const std::vector<const NumiClothHipSceneDesc*>& specimens, NumiClothHipScene& m) {
+    if (specimens.empty()) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "no specimens");
+    const NumiClothBagGPUConfig* c0 = specimens[0]->config;
+    for (const NumiClothHipSceneDesc* d : specimens) {
+        if (d == nullptr || d->config == nullptr) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "null specimen");
+        const NumiClothBagGPUConfig& c = *d->config;
+        if (d->gripCount || d->fruitCount || d->fruitPairCount || d->yarnContactCount || d->mouthRimCount ||
+            d->cuffCount || d->extension.staticSphereCount || d->extension.selfContact) {
+            fail(NUMI_CLOTH_HIP_ERROR_UNSUPPORTED,
+                 "merged specimens may not have grips, fruits, mouth, cuff, static spheres or self-contact");
+        }
+        if (c.airVelocityAndDensity.w > 0.0f) {
+            fail(NUMI_CLOTH_HIP_ERROR_UNSUPPORTED, "merged specimens may not have air loads (their reduction is global)");
+        }
+        if (std::memcmp(&c.gravityAndTimestep, &c0->gravityAndTimestep, sizeof(mr_float4)) != 0 ||
+            std::memcmp(&c.clothMaterial, &c0->clothMaterial, sizeof(mr_float4)) != 0 ||
+            c.constraintCounts.z != c0->constraintCounts.z) {
+            fail(NUMI_CLOTH_HIP_ERROR_UNSUPPORTED,
+                 "merged specimens must share timestep, gravity, cloth material and the ground flag");
+        }
+        if (d->particleCount != 0 && d->particles == nullptr) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "null particles");
+    }
+    const uint32_t n = static_cast<uint32_t>(specimens.size());
+    std::vector<uint32_t> offsets(n + 1, 0);
+    for (uint32_t s = 0; s < n; ++s) {
+        const uint64_t next = static_cast<uint64_t>(offsets[s]) + specimens[s]->particleCount;
+        if (next >= (1ull << 31)) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "merged scene too large");
+        offsets[s + 1] = static_cast<uint32_t>(next);
+    }
+    m.particles.clear();
+    m.motion.clear();
+    m.mergeStart[0] = offsets;
+    m.mergeIndex[0].resize(offsets[n]);
+    for (uint32_t s = 0; s < n; ++s) {
+        const NumiClothHipSceneDesc& d = *specimens[s];
+        const NumiClothHipSceneExtension& e = d.extension;
+        for (uint32_t k = 0; k < d.particleCount; ++k) {
+            m.mergeIndex[0][offsets[s] + k] = offsets[s] + k;
+            m.particles.push_back(d.particles[k]);
+            m.motion.push_back(e.particleMotion != nullptr ? e.particleMotion[k]
+                                                           : f4(e.driveAmplitude, e.driveOmega, e.drivePhase, e.damping));
+        }
+    }
+    auto checkIndex = [&](uint32_t index, uint32_t s) {
+        if (index >= specimens[s]->particleCount) fail(NUMI_CLOTH_HIP_ERROR_TOPOLOGY, "particle index out of range");
+    };
+    mergeFamily<NumiClothBagGPUDistance>(
+        specimens, offsets, [](const NumiClothHipSceneDesc& d) { return d.distances; },
+        [](const NumiClothHipSceneDesc& d) { return d.distanceCount; },
+        [](const NumiClothHipSceneDesc& d) { return d.distanceBatches; },
+        [](const NumiClothHipSceneDesc& d) { return d.distanceBatchCount; },
+        [](const NumiClothBagGPUDistance& r) { return r.particlesAndColor.z; },
+        [](NumiClothBagGPUDistance& r, uint32_t o) {
+            r.particlesAndColor.x += o;
+            r.particlesAndColor.y += o;
+        },
+        m.distances, m.distanceBatches, m.mergeStart[1], m.mergeIndex[1], "distance");
+    mergeFamily<NumiClothBagGPUBend>(
+        specimens, offsets, [](const NumiClothHipSceneDesc& d) { return d.bends; },
+        [](const NumiClothHipSceneDesc& d) { return d.bendCount; },
+        [](const NumiClothHipSceneDesc& d) { return d.bendBatches; },
+        [](const NumiClothHipSceneDesc& d) { return d.bendBatchCount; },
+        [](const NumiClothBagGPUBend& r) { return r.particlesAndColor.w; },
+        [](NumiClothBagGPUBend& r, uint32_t o) {
+            r.particlesAndColor.x += o;
+            r.particlesAndColor.y += o;
+            r.particlesAndColor.z += o;
+        },
+        m.bends, m.bendBatches, m.mergeStart[2], m.mergeIndex[2], "bend");
+    mergeFamily<NumiClothBagGPUKnot>(
+        specimens, offsets, [](const NumiClothHipSceneDesc& d) { return d.knots; },
+        [](const NumiClothHipSceneDesc& d) { return d.knotCount; },
+        [](const NumiClothHipSceneDesc& d) { return d.knotBatches; },
+        [](const NumiClothHipSceneDesc& d) { return d.knotBatchCount; },
+        [](const NumiClothBagGPUKnot& r) { return r.control.x; },
+        [](NumiClothBagGPUKnot& r, uint32_t o) {
+            r.particles.x += o;
+            r.particles.y += o;
+            r.particles.z += o;
+            r.particles.w += o;
+        },
+        m.knots, m.knotBatches, m.mergeStart[3], m.mergeIndex[3], "knot");
+    // Participant indices must stay inside their own specimen.
+    for (uint32_t s = 0; s < n; ++s) {
+        const NumiClothHipSceneDesc& d = *specimens[s];
+        for (uint32_t k = 0; k < d.distanceCount; ++k) {
+            checkIndex(d.distances[k].particlesAndColor.x, s);
+            checkIndex(d.distances[k].particlesAndColor.y, s);
+        }
+        for (uint32_t k = 0; k < d.bendCount; ++k) {
+            checkIndex(d.bends[k].particlesAndColor.x, s);
+            checkIndex(d.bends[k].particlesAndColor.y, s);
+            checkIndex(d.bends[k].particlesAndColor.z, s);
+        }
+        for (uint32_t k = 0; k < d.knotCount; ++k) {
+            checkIndex(d.knots[k].particles.x, s);
+            checkIndex(d.knots[k].particles.y, s);
+            checkIndex(d.knots[k].particles.z, s);
+            checkIndex(d.knots[k].particles.w, s);
+        }
+    }
+    m.config = *c0;
+    m.config.control = u4(NUMI_CLOTH_BAG_GPU_ABI_VERSION, offsets[n], static_cast<uint32_t>(m.distances.size()), 0u);
+    m.config.constraintCounts = u4(static_cast<uint32_t>(m.knots.size()), static_cast<uint32_t>(m.bends.size()),
+                                   c0->constraintCounts.z, 0u);
+    m.config.contactCounts = u4(0u, 0u, 0u, 0u);
+    m.publish();
 }
 
 // ------------------------------------------------------------------ bag ----
@@ -608,6 +775,42 @@ int numi_cloth_scene_bag(const NumiClothHipBagSpec* spec, NumiClothHipScene** ou
         auto scene = std::make_unique<NumiClothHipScene>();
         buildBag(*spec, *scene);
         *out = scene.release();
+    });
+}
+
+int numi_cloth_scene_strip(const NumiClothHipStripSpec* spec, NumiClothHipScene** out) {
+    return guarded([&] {
+        if (spec == nullptr || out == nullptr) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "null argument");
+        *out = nullptr;
+        auto scene = std::make_unique<NumiClothHipScene>();
+        buildStrip(*spec, *scene);
+        *out = scene.release();
+    });
+}
+
+int numi_cloth_scene_merge(const NumiClothHipSceneDesc* const* specimens, uint32_t count, NumiClothHipScene** out) {
+    return guarded([&] {
+        if (specimens == nullptr || out == nullptr || count == 0) {
+            fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "null argument or no specimens");
+        }
+        *out = nullptr;
+        auto scene = std::make_unique<NumiClothHipScene>();
+        mergeScenes(std::vector<const NumiClothHipSceneDesc*>(specimens, specimens + count), *scene);
+        *out = scene.release();
+    });
+}
+
+int numi_cloth_scene_merge_indices(const NumiClothHipScene* merged, uint32_t family, uint32_t specimen,
+                                   const uint32_t** indices, uint32_t* count) {
+    return guarded([&] {
+        if (merged == nullptr || indices == nullptr || count == nullptr || family > 3) {
+            fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "null argument or unknown family");
+        }
+        const std::vector<uint32_t>& start = merged->mergeStart[family];
+        if (start.empty()) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "scene was not merged");
+        if (specimen + 1 >= start.size()) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "specimen out of range");
+        *indices = merged->mergeIndex[family].data() + start[specimen];
+        *count = start[specimen + 1] - start[specimen];
     });
 }
 

@@ -28,6 +28,8 @@
 #include "kernels_bag.hpp"
 #include "tile_plan.hpp"
 #include "numi_cloth_hip.h"
+#include "numi_cloth_scene.h"
+#include "scene_internal.hpp"
 
 using namespace numi_hip;
 using namespace numi_dev;
@@ -67,6 +69,10 @@ struct DeviceArray {
         if (n == 0) return;
         HIP_OK(hipMemcpyAsync(host, ptr, n * sizeof(T), hipMemcpyDeviceToHost, stream));
     }
+    void downloadRange(T* host, std::size_t first, std::size_t n, hipStream_t stream) const {
+        if (n == 0) return;
+        HIP_OK(hipMemcpyAsync(host, ptr + first, n * sizeof(T), hipMemcpyDeviceToHost, stream));
+    }
     void release() {
         if (ptr != nullptr) (void)hipFree(ptr);
         ptr = nullptr;
@@ -100,7 +106,7 @@ struct NumiClothHipSolver {
     bool driven = false;
 
     // Device state.
-    DeviceArray<float4> pos, prev, vel, anchor;
+    DeviceArray<float4> pos, prev, vel, anchor, motion;
     DeviceArray<uint2> dIJ, bEnds;
     DeviceArray<float2> dRestCompliance, bChordCompliance, kCosineCompliance;
     DeviceArray<float> dLambda, dMaxLength, bLambda, kLambda;
@@ -109,7 +115,7 @@ struct NumiClothHipSolver {
     DeviceArray<uint32_t> dCuff;
     DeviceArray<float4> spheres;
     DeviceArray<NumiClothBagGPUConfig> configs;
-    DeviceArray<float> drive;
+    DeviceArray<double> times;      // per substep: time at the substep end
     DeviceArray<SubstepSlot> slot;
     DeviceArray<uint32_t> failure;
     DeviceArray<NumiClothBagGPUSelfStatus> selfStatus;
@@ -172,7 +178,7 @@ struct NumiClothHipSolver {
 
     // Pinned staging for per-step uploads.
     NumiClothBagGPUConfig* stagingConfigs = nullptr;
-    float* stagingDrive = nullptr;
+    double* stagingTimes = nullptr;
     uint32_t stagingCapacity = 0;
     double time = 0.0;  // simulated time at the end of the last substep
 
@@ -184,7 +190,7 @@ struct NumiClothHipSolver {
     ~NumiClothHipSolver() {
         if (graph != nullptr) (void)hipGraphExecDestroy(graph);
         if (stagingConfigs != nullptr) (void)hipHostFree(stagingConfigs);
-        if (stagingDrive != nullptr) (void)hipHostFree(stagingDrive);
+        if (stagingTimes != nullptr) (void)hipHostFree(stagingTimes);
         if (stream != nullptr) (void)hipStreamDestroy(stream);
     }
 
@@ -201,6 +207,10 @@ struct NumiClothHipSolver {
         a.prev = prev.ptr;
         a.vel = vel.ptr;
         a.anchor = driven ? anchor.ptr : nullptr;
+        a.motion = motion.ptr;
+        a.drive = make_float4(extension.driveAmplitude, extension.driveOmega, extension.drivePhase, 0.0f);
+        a.time = &slot.ptr->time;
+        a.dampingDt = initialConfig.gravityAndTimestep.w;
         a.distanceLambda = dLamCur;
         a.bendLambda = bLamCur;
         a.knotLambda = kLamCur;
@@ -210,7 +220,6 @@ struct NumiClothHipSolver {
         a.bends = B;
         a.knots = K;
         a.grips_ = G;
-        a.driveOffset = &slot.ptr->driveOffset;
         a.dampingFactor = dampingFactor;
         a.advance = fuseAdvance ? 1u : 0u;
         a.failure = failure.ptr;
@@ -420,7 +429,7 @@ struct NumiClothHipSolver {
         dLamCur = dLambda.ptr;
         bLamCur = bLambda.ptr;
         kLamCur = kLambda.ptr;
-        k_prepare<<<1, 64, 0, stream>>>(configs.ptr, drive.ptr, slot.ptr, failure.ptr);
+        k_prepare<<<1, 64, 0, stream>>>(configs.ptr, times.ptr, slot.ptr, failure.ptr);
         const float damping = extension.damping;
         // The damping factor depends on dt; dt is constant over a scene's
         // trajectories (the reference never changes it), so it is folded here.
@@ -555,15 +564,20 @@ struct NumiClothHipSolver {
         if (substeps <= stagingCapacity) return;
         const uint32_t capacity = std::max(substeps, 64u);
         if (stagingConfigs != nullptr) HIP_OK(hipHostFree(stagingConfigs));
-        if (stagingDrive != nullptr) HIP_OK(hipHostFree(stagingDrive));
+        if (stagingTimes != nullptr) HIP_OK(hipHostFree(stagingTimes));
         stagingConfigs = nullptr;
-        stagingDrive = nullptr;
+        stagingTimes = nullptr;
         HIP_OK(hipStreamSynchronize(stream));
         HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&stagingConfigs), capacity * sizeof(NumiClothBagGPUConfig)));
-        HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&stagingDrive), capacity * sizeof(float)));
+        HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&stagingTimes), capacity * sizeof(double)));
         configs.allocate(capacity);
-        drive.allocate(capacity);
+        times.allocate(capacity);
         stagingCapacity = capacity;
+        // The captured substep reads the per-substep arrays by address.
+        if (graph != nullptr) {
+            HIP_OK(hipGraphExecDestroy(graph));
+            graph = nullptr;
+        }
     }
 };
 
@@ -644,7 +658,7 @@ void createSolver(const NumiClothHipSceneDesc& scene, NumiClothHipSolver& s) {
     s.K = scene.knotCount;
     s.B = scene.bendCount;
     s.extension = scene.extension;
-    s.driven = scene.extension.driveAmplitude != 0.0f;
+    s.driven = scene.extension.driveAmplitude != 0.0f || scene.extension.particleMotion != nullptr;
     // Fruit-only scenes (no knots) are valid, as in the reference's fruit probes.
     if (s.P == 0 && scene.fruitCount == 0) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "scene has no particles");
     if (s.P != 0 && scene.particles == nullptr) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "null particles");
@@ -710,6 +724,19 @@ void createSolver(const NumiClothHipSceneDesc& scene, NumiClothHipSolver& s) {
     s.pos.upload(hp.data(), s.P, st);
     s.prev.upload(hq.data(), s.P, st);
     s.vel.upload(hv.data(), s.P, st);
+    if (scene.extension.particleMotion != nullptr) {
+        std::vector<float4> motion(s.P);
+        for (uint32_t i = 0; i < s.P; ++i) {
+            const mr_float4 m = scene.extension.particleMotion[i];
+            if (!std::isfinite(m.x) || !std::isfinite(m.y) || !std::isfinite(m.z) || !(m.w >= 0.0f) ||
+                !std::isfinite(m.w)) {
+                fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "particle motion must be finite with damping >= 0");
+            }
+            motion[i] = make_float4(m.x, m.y, m.z, m.w);
+        }
+        s.motion.allocate(s.P);
+        s.motion.upload(motion.data(), s.P, st);
+    }
     if (s.driven) {
         s.anchor.allocate(s.P);
         s.anchor.upload(hp.data(), s.P, st);
@@ -1136,13 +1163,11 @@ void stepSolver(NumiClothHipSolver& s, const NumiClothBagGPUConfig* configs, uin
     double t = s.time;
     for (uint32_t k = 0; k < substeps; ++k) {
         t += configs[k].gravityAndTimestep.w;
-        s.stagingDrive[k] = s.driven ? static_cast<float>(s.extension.driveAmplitude *
-                                                          std::sin(s.extension.driveOmega * t + s.extension.drivePhase))
-                                     : 0.0f;
+        s.stagingTimes[k] = t;
     }
     s.time = t;
     s.configs.upload(s.stagingConfigs, substeps, s.stream);
-    s.drive.upload(s.stagingDrive, substeps, s.stream);
+    s.times.upload(s.stagingTimes, substeps, s.stream);
     s.buildGraph(iterations, sweeps);
     const uint32_t zero = 0;
     (void)zero;
@@ -1372,6 +1397,124 @@ void NumiClothHipSolver::ensureStrainProgram(uint32_t sweeps) {
     }
 }
 
+// ------------------------------------------------------------------ batch --
+// Many independent specimens stepped by one solver over their merged scene:
+// every colour launch, tile program and hipGraph replay covers all specimens.
+
+struct NumiClothHipBatch {
+    NumiClothHipScene* merged = nullptr;
+    std::unique_ptr<NumiClothHipSolver> solver;
+    uint32_t specimens = 0;
+    std::vector<NumiClothBagGPUConfig> configs;
+    uint64_t version = 0;          // step() calls so far
+    uint64_t lambdaVersion = ~0ull; // version of the cached multipliers
+    std::vector<float> dl, bl, kl;
+    ~NumiClothHipBatch() {
+        solver.reset();
+        if (merged != nullptr) numi_cloth_scene_destroy(merged);
+    }
+};
+
+namespace {
+
+void stepBatch(NumiClothHipBatch& b, const NumiClothBagGPUConfig* configs, uint32_t substeps, uint32_t iterations,
+               uint32_t sweeps) {
+    if (substeps == 0) return;
+    if (configs == nullptr) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "null configs");
+    const NumiClothBagGPUConfig& totals = b.merged->config;
+    b.configs.assign(configs, configs + substeps);
+    for (NumiClothBagGPUConfig& c : b.configs) {
+        c.control.y = totals.control.y;
+        c.control.z = totals.control.z;
+        c.control.w = totals.control.w;
+        c.constraintCounts = totals.constraintCounts;
+        c.contactCounts = totals.contactCounts;
+    }
+    stepSolver(*b.solver, b.configs.data(), substeps, iterations, sweeps);
+    ++b.version;
+}
+
+const uint32_t* mergeIndices(const NumiClothHipBatch& b, uint32_t family, uint32_t specimen, uint32_t* count) {
+    const uint32_t* indices = nullptr;
+    if (numi_cloth_scene_merge_indices(b.merged, family, specimen, &indices, count) != 0) {
+        fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "specimen out of range");
+    }
+    return indices;
+}
+
+void readBatch(NumiClothHipBatch& b, uint32_t specimen, const NumiClothHipState& out) {
+    if (specimen >= b.specimens) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "specimen out of range");
+    NumiClothHipSolver& s = *b.solver;
+    hipStream_t st = s.stream;
+    uint32_t particles = 0;
+    const uint32_t* particleIndex = mergeIndices(b, NUMI_CLOTH_HIP_MERGE_PARTICLE, specimen, &particles);
+    const uint32_t first = particles != 0 ? particleIndex[0] : 0u;
+    std::vector<float4> hp(particles), hq(particles), hv(particles);
+    if (out.particles != nullptr) {
+        s.pos.downloadRange(hp.data(), first, particles, st);
+        s.prev.downloadRange(hq.data(), first, particles, st);
+        s.vel.downloadRange(hv.data(), first, particles, st);
+    }
+    // Multipliers are gathered from full downloads, cached per step() call.
+    const bool needLambda = out.distances != nullptr || out.bends != nullptr || out.knots != nullptr;
+    if (needLambda && b.lambdaVersion != b.version) {
+        b.dl.resize(s.D);
+        b.bl.resize(s.B);
+        b.kl.resize(s.K);
+        s.dLambda.download(b.dl.data(), s.D, st);
+        s.bLambda.download(b.bl.data(), s.B, st);
+        s.kLambda.download(b.kl.data(), s.K, st);
+        b.lambdaVersion = b.version;
+    }
+    if (out.failure != nullptr) s.failure.download(out.failure, 1, st);
+    if (out.frictionStatus != nullptr) s.frictionStatus.download(out.frictionStatus, 1, st);
+    HIP_OK(hipStreamSynchronize(st));
+    if (out.particles != nullptr) {
+        for (uint32_t i = 0; i < particles; ++i) {
+            NumiClothBagGPUParticle& p = out.particles[i];
+            p.positionAndInverseMass = {hp[i].x, hp[i].y, hp[i].z, hp[i].w};
+            p.previousAndMass = {hq[i].x, hq[i].y, hq[i].z, hq[i].w};
+            p.velocity = {hv[i].x, hv[i].y, hv[i].z, hv[i].w};
+        }
+    }
+    uint32_t count = 0;
+    if (out.distances != nullptr) {
+        const uint32_t* index = mergeIndices(b, NUMI_CLOTH_HIP_MERGE_DISTANCE, specimen, &count);
+        for (uint32_t k = 0; k < count; ++k) {
+            NumiClothBagGPUDistance d = b.merged->distances[index[k]];
+            d.particlesAndColor.x -= first;
+            d.particlesAndColor.y -= first;
+            d.material.z = b.dl[index[k]];
+            out.distances[k] = d;
+        }
+    }
+    if (out.bends != nullptr) {
+        const uint32_t* index = mergeIndices(b, NUMI_CLOTH_HIP_MERGE_BEND, specimen, &count);
+        for (uint32_t k = 0; k < count; ++k) {
+            NumiClothBagGPUBend d = b.merged->bends[index[k]];
+            d.particlesAndColor.x -= first;
+            d.particlesAndColor.y -= first;
+            d.particlesAndColor.z -= first;
+            d.material.w = b.bl[index[k]];
+            out.bends[k] = d;
+        }
+    }
+    if (out.knots != nullptr) {
+        const uint32_t* index = mergeIndices(b, NUMI_CLOTH_HIP_MERGE_KNOT, specimen, &count);
+        for (uint32_t k = 0; k < count; ++k) {
+            NumiClothBagGPUKnot d = b.merged->knots[index[k]];
+            d.particles.x -= first;
+            d.particles.y -= first;
+            d.particles.z -= first;
+            d.particles.w -= first;
+            d.material.z = b.kl[index[k]];
+            out.knots[k] = d;
+        }
+    }
+}
+
+}  // namespace
+
 extern "C" {
 
 int numi_cloth_hip_self_broad_phase(NumiClothHipSolver* solver, NumiClothHipSelfBroadPhase* io) {
@@ -1457,6 +1600,56 @@ int numi_cloth_hip_destroy(NumiClothHipSolver* solver) {
         if (solver == nullptr) return;
         (void)hipStreamSynchronize(solver->stream);
         delete solver;
+    });
+}
+
+int numi_cloth_hip_batch_create(const NumiClothHipSceneDesc* const* specimens, uint32_t count,
+                                NumiClothHipBatch** out) {
+    return guarded([&] {
+        if (specimens == nullptr || out == nullptr || count == 0) {
+            fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "null argument or no specimens");
+        }
+        *out = nullptr;
+        auto batch = std::make_unique<NumiClothHipBatch>();
+        if (numi_cloth_scene_merge(specimens, count, &batch->merged) != 0) {
+            fail(NUMI_CLOTH_HIP_ERROR_TOPOLOGY, numi_cloth_hip_last_error());
+        }
+        batch->specimens = count;
+        batch->solver = std::make_unique<NumiClothHipSolver>();
+        createSolver(batch->merged->desc, *batch->solver);
+        *out = batch.release();
+    });
+}
+
+int numi_cloth_hip_batch_step(NumiClothHipBatch* batch, const NumiClothBagGPUConfig* configs, uint32_t substeps,
+                              uint32_t iterations, uint32_t strainSweeps) {
+    return guarded([&] {
+        if (batch == nullptr) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "null batch");
+        stepBatch(*batch, configs, substeps, iterations, strainSweeps);
+    });
+}
+
+int numi_cloth_hip_batch_read(NumiClothHipBatch* batch, uint32_t specimen, const NumiClothHipState* out) {
+    return guarded([&] {
+        if (batch == nullptr || out == nullptr) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "null argument");
+        readBatch(*batch, specimen, *out);
+    });
+}
+
+NumiClothHipSolver* numi_cloth_hip_batch_solver(NumiClothHipBatch* batch) {
+    return batch == nullptr ? nullptr : batch->solver.get();
+}
+
+const NumiClothHipSceneDesc* numi_cloth_hip_batch_scene(const NumiClothHipBatch* batch) {
+    return batch == nullptr ? nullptr : &batch->merged->desc;
+}
+
+int numi_cloth_hip_batch_destroy(NumiClothHipBatch* batch) {
+    return guarded([&] {
+        if (batch != nullptr) {
+            (void)hipStreamSynchronize(batch->solver->stream);
+            delete batch;
+        }
     });
 }
 

@@ -175,27 +175,46 @@ struct Oracle {
     }
 
     // ------------------------------------------------------------ begin --
-    void begin(const NumiClothBagGPUConfig& c, R driveOffset) {
+    // Damping factor 1 / (1 + c dt0) of a damping coefficient c, in the
+    // arithmetic of each precision (FP32 twin: float operations).
+    static R dampingFactor(float damping, float dt0) {
+        if (!(damping > 0.0f)) return R(1);
+        return sizeof(R) == sizeof(float) ? R(1.0f / (1.0f + damping * dt0)) : R(1) / (R(1) + R(damping) * R(dt0));
+    }
+
+    // Kinematic drive offset A sin(omega t + phase), evaluated in double
+    // and rounded to float as the device does.
+    static R driveOffset(float amplitude, float omega, float phase, double time) {
+        return R(static_cast<float>(static_cast<double>(amplitude) *
+                                    std::sin(static_cast<double>(omega) * time + static_cast<double>(phase))));
+    }
+
+    void begin(const NumiClothBagGPUConfig& c, double endTime) {
         const R dt = R(c.gravityAndTimestep.w);
         const V<R> g = v3<R>(c.gravityAndTimestep);
-        const float damping = d.extension.damping;
+        const NumiClothHipSceneExtension& e = d.extension;
         const float dt0 = d.config->gravityAndTimestep.w;
-        const R factor = damping > 0.0f
-                             ? (sizeof(R) == sizeof(float) ? R(1.0f / (1.0f + damping * dt0))
-                                                           : R(1) / (R(1) + R(damping) * R(dt0)))
-                             : R(1);
-        const bool driven = d.extension.driveAmplitude != 0.0f;
+        const R factor = dampingFactor(e.damping, dt0);
+        const bool driven = e.driveAmplitude != 0.0f || e.particleMotion != nullptr;
+        const R globalOffset = e.driveAmplitude != 0.0f
+                                   ? driveOffset(e.driveAmplitude, e.driveOmega, e.drivePhase, endTime)
+                                   : R(0);
         for (uint32_t i = 0; i < P; ++i) {
             Particle<R>& q = p[i];
             q.prev = q.x;
             q.groundImpulse = 0;
             q.predictedVz = 0;
             if (q.w > R(0)) {
-                const V<R> carried = factor != R(1) ? q.v * factor : q.v;
+                const R f = e.particleMotion != nullptr ? dampingFactor(e.particleMotion[i].w, dt0) : factor;
+                const V<R> carried = f != R(1) ? q.v * f : q.v;
                 q.v = fma3(g, dt, carried);
                 q.predictedVz = q.v.z;
             } else if (driven) {
-                q.x = {anchor[i].x, anchor[i].y, anchor[i].z + driveOffset};
+                const R offset = e.particleMotion != nullptr
+                                     ? driveOffset(e.particleMotion[i].x, e.particleMotion[i].y,
+                                                   e.particleMotion[i].z, endTime)
+                                     : globalOffset;
+                q.x = {anchor[i].x, anchor[i].y, anchor[i].z + offset};
             }
         }
         std::fill(dLambda.begin(), dLambda.end(), R(0));
@@ -1285,10 +1304,7 @@ struct Oracle {
         const R dt = R(c.gravityAndTimestep.w);
         time += static_cast<double>(c.gravityAndTimestep.w);
         const NumiClothHipSceneExtension& e = d.extension;
-        const R driveOffset = e.driveAmplitude != 0.0f
-                                  ? R(static_cast<float>(e.driveAmplitude * std::sin(e.driveOmega * time + e.drivePhase)))
-                                  : R(0);
-        begin(c, driveOffset);
+        begin(c, time);
         fruitBegin(c);
         attachGrips(c);
         yarnAirLoads(c);
