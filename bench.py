@@ -5,18 +5,31 @@ A step is one frame of the workload (all of its substeps, each with the full
 iteration / strain-limit / reconciliation schedule) on inputs already resident
 in HBM. Prints one JSON line.
 
-Workloads (synthetic, seeded):
+Workloads (synthetic, seeded with SplitMix64 / numpy PCG64 from SEED):
   sheet256   256 x 256 woven sheet (65,536 knots), 1 cm spacing, z jitter
              +-0.1 mm, dropped from 0.3 m onto a fixed r = 0.25 m sphere and
-             the ground, 32 substeps per 1/60 s frame (default).
+             the ground, hash self-contact with 1 cm cells, 32 substeps per
+             1/60 s frame (default).
   sheet1024  1024 x 1024 sheet (1,048,576 knots) over 64 static spheres of
-             radius 2-10 cm at seeded positions, 16 substeps per 1/60 s frame.
-Both use 32 XPBD iterations and 3 strain-limit sweeps per substep.
+             radius 2-10 cm at seeded positions, self-contact, 16 substeps
+             per 1/60 s frame.
+  bag        the produce bag (1,465 knots, 12 fruits of radius 2-3 cm),
+             stretch compliance 1e-7, bend 1e-4, strain limit 0.10, ground
+             on, cuff grip lifted 0.4 m over 2 s (smoothstep) then held,
+             48 substeps per 1/120 s frame.
+  strips512  512 hanging strips, width U[8,64] x height U[64,256] knots,
+             compliances log-U[1e-8,1e-5], density U[0.1,0.4] kg/m^2, top row
+             driven by a 0.5-2 Hz vertical sinusoid, one batch (merged scene,
+             one launch per pass, hipGraph substeps), 48 substeps per 1/60 s
+             frame; reports specimen-frames/s and the speed-up over stepping
+             the specimens one solver at a time.
+All use 32 XPBD iterations and 3 strain-limit sweeps per substep.
 """
 
 from __future__ import annotations
 
 import argparse
+import concurrent.futures
 import ctypes
 import json
 import os
@@ -25,6 +38,8 @@ import sys
 import time
 from pathlib import Path
 
+import numpy as np
+
 REPO = Path(__file__).resolve().parent
 sys.path.insert(0, str(REPO))
 
@@ -32,15 +47,16 @@ import __graft_entry__  # noqa: E402
 
 SEED = 20261203
 PEAK_HBM_GBPS = 8000.0
+ITERATIONS = 32
+STRAIN_SWEEPS = 3
 
-WORKLOADS = {
+SHEETS = {
     "sheet256": dict(width=256, height=256, substeps=32, frame=1.0 / 60.0, sphereMode=1,
                      fixedSphere=(0.0, 0.0, 0.0, 0.25)),
     "sheet1024": dict(width=1024, height=1024, substeps=16, frame=1.0 / 60.0, sphereMode=2, sphereCount=64,
                       sphereRadiusMin=0.02, sphereRadiusMax=0.10),
 }
-ITERATIONS = 32
-STRAIN_SWEEPS = 3
+WORKLOADS = sorted([*SHEETS, "bag", "strips512"])
 
 
 def _hip():
@@ -53,13 +69,80 @@ def _hip():
     return hip
 
 
-def build_scene(cloth, name: str):
-    w = WORKLOADS[name]
+class Timer:
+    """HIP events on one stream around a region."""
+
+    def __init__(self, hip, stream: int):
+        self.hip = hip
+        self.stream = ctypes.c_void_p(stream)
+        self.start, self.stop = ctypes.c_void_p(), ctypes.c_void_p()
+        hip.hipEventCreate(ctypes.byref(self.start))
+        hip.hipEventCreate(ctypes.byref(self.stop))
+
+    def begin(self):
+        self.wall = time.perf_counter()
+        self.hip.hipEventRecord(self.start, self.stream)
+
+    def end(self) -> float:
+        self.hip.hipEventRecord(self.stop, self.stream)
+        self.hip.hipEventSynchronize(self.stop)
+        self.wall = time.perf_counter() - self.wall
+        elapsed = ctypes.c_float()
+        self.hip.hipEventElapsedTime(ctypes.byref(elapsed), self.start, self.stop)
+        return elapsed.value / 1000.0
+
+
+# ---------------------------------------------------------------- scenes --
+
+def sheet_scene(cloth, name: str):
+    w = SHEETS[name]
     fields = {k: v for k, v in w.items() if k not in ("substeps", "frame")}
     scene = cloth.Scene.sheet(spacing=0.01, elevation=0.3, jitter=1.0e-4, seed=SEED, cellSize=0.01,
                               selfContact=1, timestep=w["frame"] / w["substeps"], **fields)
-    return scene, scene.configs(w["substeps"])
+    return scene, [scene.configs(w["substeps"])], w["frame"]
 
+
+def bag_scene(cloth, frames: int):
+    """Produce bag with the stated materials; frame f's 48 configs lift the
+    cuff grip by 0.4 m * smoothstep(t / 2 s)."""
+    scene = cloth.Scene.bag(stretchComplianceOverride=1.0e-7, bendComplianceOverride=1.0e-4,
+                            strainLimitOverride=0.10, fruitRadiusMin=0.02, fruitRadiusMax=0.03, seed=SEED)
+    scene.config["constraintCounts"][0, 2] = 1
+    scene.refresh()
+    substeps, frame = 48, 1.0 / 120.0
+    base = scene.config[0]["gripTargetAndActive"].astype(np.float64)
+    dt = frame / substeps
+    out = []
+    for f in range(frames):
+        configs = scene.configs(substeps)
+        t = (f * substeps + np.arange(1, substeps + 1)) * dt
+        u = np.clip(t / 2.0, 0.0, 1.0)
+        configs["gripTargetAndActive"][:, 2] = base[2] + 0.4 * u * u * (3.0 - 2.0 * u)
+        configs["gravityAndTimestep"][:, 3] = dt
+        out.append(configs)
+    scene.config["gravityAndTimestep"][0, 3] = dt
+    scene.refresh()
+    return scene, out, frame
+
+
+def strip_specimens(cloth, count: int = 512):
+    rng = np.random.default_rng(SEED)
+    specimens = []
+    x = 0.0
+    for k in range(count):
+        width = int(rng.integers(8, 65))
+        height = int(rng.integers(64, 257))
+        specimens.append(cloth.Scene.strip(
+            width=width, height=height, spacing=0.01, arealDensity=float(rng.uniform(0.1, 0.4)),
+            stretchCompliance=float(10.0 ** rng.uniform(-8, -5)), bendCompliance=float(10.0 ** rng.uniform(-8, -5)),
+            strainLimit=0.10, timestep=1.0 / (60.0 * 48.0), driveAmplitude=0.01,
+            driveFrequency=float(rng.uniform(0.5, 2.0)), drivePhase=float(rng.uniform(0.0, 2.0 * np.pi)),
+            origin=(x, 0.0, 3.0), jitter=1.0e-4, seed=SEED + k))
+        x += width * 0.01 + 0.05
+    return specimens
+
+
+# ------------------------------------------------------------ baselines --
 
 def cpu_baseline(scene, configs, substeps: int) -> dict:
     """FP64 oracle, one core, on the first `substeps` substeps of the frame."""
@@ -70,8 +153,159 @@ def cpu_baseline(scene, configs, substeps: int) -> dict:
     seconds = time.perf_counter() - start
     rate = len(scene.particles) * substeps / seconds
     return {"value": rate, "unit": "particle-substeps/s", "cores": 1, "kind": "port",
-            "sample": f"{substeps} substep(s) of the same frame, FP64 oracle (g++ -O3 -fno-fast-math "
+            "sample": f"{substeps} substep(s) of the same workload, FP64 oracle (g++ -O3 -fno-fast-math "
                       f"-ffp-contract=off), {seconds:.2f} s on {platform.processor() or platform.machine()}"}
+
+
+def cpu_baseline_strips(specimens, count: int) -> dict:
+    """FP64 oracle, one specimen per thread (ctypes releases the GIL), one
+    frame of each of the first `count` specimens."""
+    from oracle import pyoracle
+    pyoracle.library()
+    threads = max(1, min(16, os.cpu_count() or 1))
+    sample = specimens[:count]
+    start = time.perf_counter()
+    with concurrent.futures.ThreadPoolExecutor(max_workers=threads) as pool:
+        list(pool.map(lambda s: pyoracle.run(s, s.configs(48), ITERATIONS, STRAIN_SWEEPS), sample))
+    seconds = time.perf_counter() - start
+    return {"value": len(sample) / seconds, "unit": "specimen-frames/s", "cores": threads, "kind": "port",
+            "sample": f"1 frame (48 substeps) of {len(sample)} strips, FP64 oracle, {threads} threads, "
+                      f"{seconds:.2f} s on {platform.processor() or platform.machine()}"}
+
+
+# -------------------------------------------------------------- roofline --
+
+def roofline(cloth, solver) -> tuple[dict, dict]:
+    """Dominant kernel, timed with HIP events on the solver's stream after the
+    timed region (same state, same launch shape): the LDS tile program of an
+    even iteration when the solver runs tiles, else one distance colour."""
+    tiles = solver.tile_info()
+    if tiles.enabled:
+        timing = solver.time_pass(cloth.cloth.PASS_TILE_ITERATION, 0, repeats=50)
+        kernel = "k_tile_program (even iteration: distance + bend colours fused in LDS)"
+    else:
+        timing = solver.time_pass(cloth.cloth.PASS_DISTANCE, 0, repeats=200)
+        kernel = "k_distance (one colour batch)"
+    achieved = timing.algorithmicBytes / (timing.averageMilliseconds * 1.0e-3) / 1.0e9
+    line = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBPS, "unit": "GB/s",
+            "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": None, "kernel": kernel,
+            "bytes_per_launch": timing.algorithmicBytes,
+            "avg_launch_us": round(timing.averageMilliseconds * 1000.0, 3),
+            "items_per_launch": timing.itemsPerLaunch}
+    tile_config = {"enabled": bool(tiles.enabled), "tiles": tiles.tiles, "owned_per_tile": tiles.ownedPerTile,
+                   "max_loaded": tiles.maxLoaded, "halo_hops": tiles.halo, "workgroup": tiles.workgroupSize,
+                   "passes_fused": tiles.passes, "redundancy": round(tiles.redundancy, 3)}
+    return line, tile_config
+
+
+def diagnostics_of(solver, state) -> dict:
+    d = solver.diagnostics()
+    self_status = state["selfStatus"]["counters"][0]
+    return {"max_strain_violation": d.maximumStrainViolation, "max_ground_penetration": d.maximumGroundPenetration,
+            "max_sphere_penetration": d.maximumSpherePenetration, "max_self_penetration": d.maximumSelfPenetration,
+            "self_candidates_last_substep": d.selfCandidateCount, "self_candidate_overflow": bool(d.selfCandidateOverflow),
+            "self_contacts_last_step": {"present": int(self_status[0]), "swept": int(self_status[1])}}
+
+
+# ------------------------------------------------------------- workloads --
+
+def run_plain(args, cloth, hip) -> dict:
+    if args.workload == "bag":
+        scene, frames, frame_seconds = bag_scene(cloth, args.warmup + args.steps)
+    else:
+        scene, frames, frame_seconds = sheet_scene(cloth, args.workload)
+    def frame(k):
+        return frames[k % len(frames)]
+    substeps = len(frames[0])
+    particles = len(scene.particles)
+    solver = cloth.Solver(scene)
+    for k in range(args.warmup):
+        solver.step(frame(k), ITERATIONS, STRAIN_SWEEPS)
+    solver.synchronize()
+    assert hip.hipDeviceSynchronize() == 0
+    timer = Timer(hip, solver.stream)
+    timer.begin()
+    for k in range(args.steps):
+        solver.step(frame(args.warmup + k), ITERATIONS, STRAIN_SWEEPS)
+    seconds = timer.end()
+    solver.synchronize()
+    assert hip.hipDeviceSynchronize() == 0
+    state = solver.read()
+    if state["failure"] != 0:
+        raise RuntimeError(f"device failure bits {state['failure']}")
+    diagnostics = diagnostics_of(solver, state)
+    line, tile_config = roofline(cloth, solver)
+    released = int(state["releaseStatus"]["masks"][0, 1])
+    solver.close()
+    baseline = None
+    if not args.no_cpu_baseline:
+        budget = 32 * 65536 if args.workload != "bag" else 48 * 1465 * 4
+        cpu_substeps = args.cpu_substeps or max(1, min(substeps, int(budget / particles)))
+        baseline = cpu_baseline(scene, frames[0], cpu_substeps)
+    value = particles * substeps * args.steps / seconds
+    config = {"workload": args.workload, "particles": particles, "distances": len(scene.distances),
+              "bends": len(scene.bends), "knots": len(scene.knots), "fruits": len(scene.fruits),
+              "substeps_per_frame": substeps, "iterations": ITERATIONS, "strain_sweeps": STRAIN_SWEEPS,
+              "static_spheres": len(scene.staticSpheres), "self_contact": bool(scene.selfContact),
+              "frame_seconds": frame_seconds, "wall_seconds_timed": round(timer.wall, 3),
+              "tile_programs": tile_config}
+    if args.workload == "bag":
+        config["released_mask"] = released
+    return {"metric": "particle-substeps/s", "value": value, "unit": "particle-substeps/s",
+            "ms_per_step": seconds * 1000.0 / args.steps, "dtype": "f32", "config": config, "roofline": line,
+            "cpu_baseline": baseline, "diagnostics": diagnostics,
+            "data": "synthetic (seeded scene, SplitMix64 seed %d)" % SEED}
+
+
+def run_strips(args, cloth, hip) -> dict:
+    specimens = strip_specimens(cloth)
+    configs = specimens[0].configs(48)
+    particles = sum(len(s.particles) for s in specimens)
+    batch = cloth.Batch(specimens)
+    for _ in range(args.warmup):
+        batch.step(configs, ITERATIONS, STRAIN_SWEEPS)
+    batch.synchronize()
+    assert hip.hipDeviceSynchronize() == 0
+    solver_handle = batch.solver_handle
+    stream = cloth.cloth.library().numi_cloth_hip_stream(solver_handle)
+    timer = Timer(hip, stream)
+    timer.begin()
+    for _ in range(args.steps):
+        batch.step(configs, ITERATIONS, STRAIN_SWEEPS)
+    seconds = timer.end()
+    batch.synchronize()
+    failure = batch.read(0)["failure"]
+    if failure != 0:
+        raise RuntimeError(f"device failure bits {failure}")
+    view = cloth.Solver.borrow(solver_handle, specimens[0])
+    line, tile_config = roofline(cloth, view)
+    batch.close()
+    batched = len(specimens) * args.steps / seconds
+
+    # One specimen per solver: the plain single-specimen path, launched once
+    # per specimen, on a seeded sample of the same specimens.
+    sample = [specimens[k] for k in np.random.default_rng(SEED).choice(len(specimens), 16, replace=False)]
+    total = 0.0
+    for scene in sample:
+        with cloth.Solver(scene) as solver:
+            solver.step(configs, ITERATIONS, STRAIN_SWEEPS)
+            solver.synchronize()
+            t = Timer(hip, solver.stream)
+            t.begin()
+            for _ in range(2):
+                solver.step(configs, ITERATIONS, STRAIN_SWEEPS)
+            total += t.end()
+    single = 2 * len(sample) / total
+    baseline = None if args.no_cpu_baseline else cpu_baseline_strips(specimens, 16)
+    config = {"workload": "strips512", "specimens": len(specimens), "particles": particles,
+              "substeps_per_frame": 48, "iterations": ITERATIONS, "strain_sweeps": STRAIN_SWEEPS,
+              "frame_seconds": 1.0 / 60.0, "wall_seconds_timed": round(timer.wall, 3),
+              "one_per_launch_specimen_frames_per_s": single, "speedup_vs_one_per_launch": batched / single,
+              "one_per_launch_sample": "16 seeded specimens, 2 frames each, own solver each",
+              "particle_substeps_per_s": particles * 48 * args.steps / seconds, "tile_programs": tile_config}
+    return {"metric": "batched specimen-frames/s", "value": batched, "unit": "specimen-frames/s",
+            "ms_per_step": seconds * 1000.0 / args.steps, "dtype": "f32", "config": config, "roofline": line,
+            "cpu_baseline": baseline, "data": "synthetic (seeded strips, PCG64 seed %d)" % SEED}
 
 
 def main() -> int:
@@ -79,7 +313,7 @@ def main() -> int:
     parser.add_argument("--gpus", type=int, default=1)
     parser.add_argument("--steps", type=int, default=10)
     parser.add_argument("--warmup", type=int, default=2)
-    parser.add_argument("--workload", choices=sorted(WORKLOADS), default="sheet256")
+    parser.add_argument("--workload", choices=WORKLOADS, default="sheet256")
     parser.add_argument("--cpu-substeps", type=int, default=None,
                         help="substeps of the CPU FP64 baseline sample (default: sized to ~15 s)")
     parser.add_argument("--no-cpu-baseline", action="store_true")
@@ -89,89 +323,16 @@ def main() -> int:
         print("bench.py measures one GPU only: the cloth substep loop of this library runs on a single "
               "MI355X (multi-GPU sharding is a design note, not built); rerun with --gpus 1", file=sys.stderr)
         return 2
-
     cloth = __graft_entry__.load_package()
     hip = _hip()
-    scene, configs = build_scene(cloth, args.workload)
-    substeps = len(configs)
-    particles = len(scene.particles)
-    solver = cloth.Solver(scene)
-    stream = ctypes.c_void_p(solver.stream)
-
-    for _ in range(args.warmup):
-        solver.step(configs, ITERATIONS, STRAIN_SWEEPS)
-    solver.synchronize()
-    assert hip.hipDeviceSynchronize() == 0
-
-    start, stop = ctypes.c_void_p(), ctypes.c_void_p()
-    hip.hipEventCreate(ctypes.byref(start))
-    hip.hipEventCreate(ctypes.byref(stop))
-    wall = time.perf_counter()
-    hip.hipEventRecord(start, stream)
-    for _ in range(args.steps):
-        solver.step(configs, ITERATIONS, STRAIN_SWEEPS)
-    hip.hipEventRecord(stop, stream)
-    hip.hipEventSynchronize(stop)
-    solver.synchronize()
-    assert hip.hipDeviceSynchronize() == 0
-    wall = time.perf_counter() - wall
-    elapsed = ctypes.c_float()
-    hip.hipEventElapsedTime(ctypes.byref(elapsed), start, stop)
-    seconds = elapsed.value / 1000.0
-    state = solver.read()
-    if state["failure"] != 0:
-        print(f"device failure bits {state['failure']}", file=sys.stderr)
-        return 1
-    diagnostics = solver.diagnostics()
-    self_status = state["selfStatus"]["counters"][0]
-
-    # Dominant kernel, timed with HIP events on the solver's stream right after
-    # the timed region (same state, same launch shape): the LDS tile program of
-    # an even iteration when the solver runs tiles, else one distance colour.
-    tiles = solver.tile_info()
-    if tiles.enabled:
-        timing = solver.time_pass(cloth.cloth.PASS_TILE_ITERATION, 0, repeats=50)
-        kernel = "k_tile_program (even iteration: distance + bend colours fused in LDS)"
-    else:
-        timing = solver.time_pass(cloth.cloth.PASS_DISTANCE, 0, repeats=200)
-        kernel = "k_distance (one colour batch)"
-    achieved = timing.algorithmicBytes / (timing.averageMilliseconds * 1.0e-3) / 1.0e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBPS, "unit": "GB/s",
-                "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": None, "kernel": kernel,
-                "bytes_per_launch": timing.algorithmicBytes,
-                "avg_launch_us": round(timing.averageMilliseconds * 1000.0, 3),
-                "items_per_launch": timing.itemsPerLaunch}
-    tile_config = {"enabled": bool(tiles.enabled), "tiles": tiles.tiles, "owned_per_tile": tiles.ownedPerTile,
-                   "max_loaded": tiles.maxLoaded, "halo_hops": tiles.halo, "workgroup": tiles.workgroupSize,
-                   "passes_fused": tiles.passes, "redundancy": round(tiles.redundancy, 3)}
-    solver.close()
-
-    baseline = None
-    if not args.no_cpu_baseline:
-        cpu_substeps = args.cpu_substeps or max(1, min(substeps, int(32 * 65536 / particles)))
-        baseline = cpu_baseline(scene, configs, cpu_substeps)
-
-    value = particles * substeps * args.steps / seconds
-    line = {
-        "metric": "particle-substeps/s", "value": value, "unit": "particle-substeps/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": seconds * 1000.0 / args.steps,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-        "data": "synthetic (seeded sheet, SplitMix64 seed %d)" % SEED,
-        "config": {"workload": args.workload, "particles": particles, "distances": len(scene.distances),
-                   "bends": len(scene.bends), "substeps_per_frame": substeps, "iterations": ITERATIONS,
-                   "strain_sweeps": STRAIN_SWEEPS, "static_spheres": len(scene.staticSpheres),
-                   "self_contact": bool(scene.selfContact), "frame_seconds": WORKLOADS[args.workload]["frame"],
-                   "wall_seconds_timed": round(wall, 3), "tile_programs": tile_config},
-        "roofline": roofline,
-        "cpu_baseline": baseline,
-        "diagnostics": {"max_strain_violation": diagnostics.maximumStrainViolation,
-                        "max_ground_penetration": diagnostics.maximumGroundPenetration,
-                        "max_sphere_penetration": diagnostics.maximumSpherePenetration,
-                        "max_self_penetration": diagnostics.maximumSelfPenetration,
-                        "self_candidates_last_substep": diagnostics.selfCandidateCount,
-                        "self_candidate_overflow": bool(diagnostics.selfCandidateOverflow),
-                        "self_contacts_last_step": {"present": int(self_status[0]), "swept": int(self_status[1])}},
-    }
+    result = run_strips(args, cloth, hip) if args.workload == "strips512" else run_plain(args, cloth, hip)
+    line = {"metric": result["metric"], "value": result["value"], "unit": result["unit"], "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": result["ms_per_step"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": result["dtype"],
+            "data": result["data"], "config": result["config"], "roofline": result["roofline"],
+            "cpu_baseline": result["cpu_baseline"]}
+    if "diagnostics" in result:
+        line["diagnostics"] = result["diagnostics"]
     print(json.dumps(line))
     return 0
 

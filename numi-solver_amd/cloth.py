@@ -549,9 +549,19 @@ class Solver:
         _check(library().numi_cloth_hip_create(C.byref(scene.desc), C.byref(handle)))
         self._handle = handle
 
+    @classmethod
+    def borrow(cls, handle: int, scene: Scene) -> "Solver":
+        """A non-owning view of a solver created elsewhere (a batch's)."""
+        view = cls.__new__(cls)
+        view.scene = scene
+        view._handle = C.c_void_p(handle)
+        view._borrowed = True
+        return view
+
     def close(self) -> None:
         if getattr(self, "_handle", None):
-            _check(library().numi_cloth_hip_destroy(self._handle))
+            if not getattr(self, "_borrowed", False):
+                _check(library().numi_cloth_hip_destroy(self._handle))
             self._handle = None
 
     def __del__(self):

@@ -237,14 +237,12 @@ __global__ void k_fruit_aero(const NumiClothBagGPUConfig* __restrict__ cfg, cons
 // ------------------------------------------------------------- contacts --
 
 // Positional sphere/sphere separation of one colour batch of fruit pairs.
-__global__ void k_fruit_pair(const NumiClothBagGPUConfig* __restrict__ cfg, const uint32_t* __restrict__ ok,
-                             uint32_t first, uint32_t count, NumiClothBagGPUFruit* __restrict__ fruits,
-                             NumiClothBagGPUFruitPair* __restrict__ pairs, uint32_t* __restrict__ failure) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= count || *ok == 0u) return;
-    NumiClothBagGPUFruitPair pair = pairs[first + t];
-    NumiClothBagGPUFruit a = fruits[pair.fruitsAndColor.x];
-    NumiClothBagGPUFruit b = fruits[pair.fruitsAndColor.y];
+// One sphere/sphere separation (reference solve_fruit_pair,
+// src/metal/ClothBag.metal:1503-1573) on fruits held by the caller.
+__device__ __forceinline__ void projectFruitPair(NumiClothBagGPUFruit& a, NumiClothBagGPUFruit& b,
+                                                 NumiClothBagGPUFruitPair& pair, float dt, uint32_t* failure,
+                                                 bool& wrote) {
+    wrote = false;
     const V3 d = xyz(b.positionAndInverseMass) - xyz(a.positionAndInverseMass);
     const float len = length(d);
     const float target = a.previousAndRadius.w + b.previousAndRadius.w;
@@ -261,20 +259,53 @@ __global__ void k_fruit_pair(const NumiClothBagGPUConfig* __restrict__ cfg, cons
     const float lambda = (target - len) / denom;
     const V3 n = d / len;
     const V3 c = n * lambda;
-    setXyz(a.positionAndInverseMass, xyz(a.positionAndInverseMass) - c * a.positionAndInverseMass.w);
-    setXyz(b.positionAndInverseMass, xyz(b.positionAndInverseMass) + c * b.positionAndInverseMass.w);
-    const float impulse = lambda / cfg->gravityAndTimestep.w;
-    const V3 nc = xyz(pair.contact) + n * impulse;
-    setXyz(pair.contact, nc);
-    pair.contact.w += impulse;
-    if (!finite3(xyz(a.positionAndInverseMass)) || !finite3(xyz(b.positionAndInverseMass)) ||
-        !finite4(pair.contact)) {
+    NumiClothBagGPUFruit na = a, nb = b;
+    setXyz(na.positionAndInverseMass, xyz(a.positionAndInverseMass) - c * a.positionAndInverseMass.w);
+    setXyz(nb.positionAndInverseMass, xyz(b.positionAndInverseMass) + c * b.positionAndInverseMass.w);
+    const float impulse = lambda / dt;
+    NumiClothBagGPUFruitPair np = pair;
+    setXyz(np.contact, xyz(pair.contact) + n * impulse);
+    np.contact.w += impulse;
+    if (!finite3(xyz(na.positionAndInverseMass)) || !finite3(xyz(nb.positionAndInverseMass)) ||
+        !finite4(np.contact)) {
         recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
         return;
     }
-    fruits[pair.fruitsAndColor.x] = a;
-    fruits[pair.fruitsAndColor.y] = b;
-    pairs[first + t] = pair;
+    a = na;
+    b = nb;
+    pair = np;
+    wrote = true;
+}
+
+// All fruit-pair colours of one solve pass in one workgroup: fruits staged
+// in LDS, a barrier between colours (the reference launches one dispatch per
+// colour, :4519-4528).
+__global__ void k_fruit_pairs(const NumiClothBagGPUConfig* __restrict__ cfg, const uint32_t* __restrict__ ok,
+                              const uint2* __restrict__ colours, uint32_t colourCount,
+                              NumiClothBagGPUFruit* __restrict__ fruits, uint32_t fruitCount,
+                              NumiClothBagGPUFruitPair* __restrict__ pairs, uint32_t* __restrict__ failure) {
+    __shared__ NumiClothBagGPUFruit local[32];
+    if (*ok == 0u) return;
+    for (uint32_t i = threadIdx.x; i < fruitCount; i += blockDim.x) local[i] = fruits[i];
+    __syncthreads();
+    const float dt = cfg->gravityAndTimestep.w;
+    for (uint32_t c = 0; c < colourCount; ++c) {
+        const uint2 range = colours[c];
+        for (uint32_t t = threadIdx.x; t < range.y; t += blockDim.x) {
+            NumiClothBagGPUFruitPair pair = pairs[range.x + t];
+            NumiClothBagGPUFruit a = local[pair.fruitsAndColor.x];
+            NumiClothBagGPUFruit b = local[pair.fruitsAndColor.y];
+            bool wrote = false;
+            projectFruitPair(a, b, pair, dt, failure, wrote);
+            if (wrote) {
+                local[pair.fruitsAndColor.x] = a;
+                local[pair.fruitsAndColor.y] = b;
+                pairs[range.x + t] = pair;
+            }
+        }
+        __syncthreads();
+    }
+    for (uint32_t i = threadIdx.x; i < fruitCount; i += blockDim.x) fruits[i] = local[i];
 }
 
 // Fruit part of solve_ground: z >= radius with the impulse that removes the
@@ -441,93 +472,6 @@ __device__ float yarnCorrection(float4& first, float4& second, NumiClothBagGPUFr
     return total;
 }
 
-// All colour batches of one solve_yarn_batch call in one workgroup (one lane
-// per fruit, fruits <= workgroup size), knots staged in LDS. mode 1 uses the
-// swept impacts of the contact records, mode 0 the present overlap.
-__global__ void k_yarn_batches(const NumiClothBagGPUConfig* __restrict__ cfg, const uint32_t* __restrict__ ok,
-                               uint32_t mode, const uint4* __restrict__ batches, uint32_t batchCount,
-                               float4* __restrict__ pos, uint32_t particles, const uint2* __restrict__ ij,
-                               uint32_t segments, NumiClothBagGPUFruit* __restrict__ fruits, uint32_t fruitCount,
-                               NumiClothBagGPUYarnContact* __restrict__ contacts, uint32_t* __restrict__ failure) {
-    extern __shared__ float4 knots[];
-    if (*ok == 0u) return;
-    for (uint32_t i = threadIdx.x; i < particles; i += blockDim.x) knots[i] = pos[i];
-    __syncthreads();
-    const uint32_t f = threadIdx.x;
-    const bool lane = f < fruitCount;
-    NumiClothBagGPUFruit fruit{};
-    if (lane) fruit = fruits[f];
-    const bool ground = cfg->constraintCounts.z != 0u;
-    const float floorZ = cfg->clothMaterial.x;
-    const float dt = cfg->gravityAndTimestep.w;
-    for (uint32_t batch = 0; batch < batchCount; ++batch) {
-        const uint4 b = batches[batch];
-        // Fruits f and f + b.y would meet the same segment in one phase:
-        // such fruits take turns (rounds), lower fruit first.
-        const uint32_t rounds = (fruitCount + b.y - 1) / b.y;
-        for (uint32_t step = 0; step < b.y * rounds; ++step) {
-            const uint32_t phase = step / rounds, round = step % rounds;
-            if (lane && f / b.y == round) {
-                const uint32_t segment = b.x + (phase + f) % b.y;
-                const uint2 e = ij[segment];
-                const uint32_t index = f * segments + segment;
-                NumiClothBagGPUYarnContact c = contacts[index];
-                V3 n = v3(0.0f, 0.0f, 0.0f);
-                float wa = 0.0f, wb = 0.0f, correction = 0.0f;
-                if (mode == 1u) {
-                    if (c.control.y != 0u) {
-                        n = xyz(c.sweptNormalAndAdvance);
-                        wb = c.weightsAndTime.y;
-                        wa = 1.0f - wb;
-                        correction = c.sweptNormalAndAdvance.w;
-                    }
-                } else {
-                    const V3 a = xyz(knots[e.x]), bb = xyz(knots[e.y]);
-                    const V3 centre = xyz(fruit.positionAndInverseMass);
-                    const PointSample s = samplePointSegment(centre, a, bb);
-                    const float target = fruit.previousAndRadius.w + cfg->clothMaterial.x;
-                    if (s.distance < target) {
-                        n = s.distance < 1.0e-12f ? degenerateYarnNormal(a, bb) : (s.closest - centre) / s.distance;
-                        wb = s.weight;
-                        wa = 1.0f - wb;
-                        correction = target - s.distance;
-                    }
-                }
-                if (correction > 0.0f) {
-                    bool wrote = false;
-                    float4 ka = knots[e.x], kb = knots[e.y];
-                    const float lambda = yarnCorrection(ka, kb, fruit, wa, wb, n, correction, ground, floorZ, failure,
-                                                        wrote);
-                    if (wrote) {
-                        knots[e.x] = ka;
-                        knots[e.y] = kb;
-                    }
-                    if (lambda > 0.0f) {
-                        const float impulse = lambda / dt;
-                        const V3 fn = xyz(c.fruitNormalAndImpulse) - n * impulse;
-                        setXyz(c.fruitNormalAndImpulse, fn);
-                        c.fruitNormalAndImpulse.w += impulse;
-                        c.segmentImpulse.x += wa * impulse;
-                        c.segmentImpulse.y += wb * impulse;
-                        if (mode == 1u) {
-                            c.segmentImpulse.z = c.weightsAndTime.z;
-                            c.segmentImpulse.w += correction;
-                        }
-                        c.control.w += 1u;
-                    }
-                }
-                contacts[index] = c;
-            }
-            __syncthreads();
-        }
-    }
-    if (lane) {
-        if (!finite3(xyz(fruit.positionAndInverseMass))) recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
-        fruits[f] = fruit;
-    }
-    for (uint32_t i = threadIdx.x; i < particles; i += blockDim.x) pos[i] = knots[i];
-}
-
 // ------------------------------------------------------------- finalize --
 
 __global__ void k_fruit_finalize(const NumiClothBagGPUConfig* __restrict__ cfg, const uint32_t* __restrict__ ok,
@@ -545,102 +489,13 @@ __global__ void k_fruit_finalize(const NumiClothBagGPUConfig* __restrict__ cfg, 
     fruits[i] = f;
 }
 
-// Coulomb friction between fruits and yarn segments, with the same phase
-// order as the yarn batch solve; knot velocities staged in LDS.
-__global__ void k_yarn_friction(const NumiClothBagGPUConfig* __restrict__ cfg, const uint32_t* __restrict__ ok,
-                                const uint4* __restrict__ batches, uint32_t batchCount, const float4* __restrict__ pos,
-                                float4* __restrict__ vel, uint32_t particles, const uint2* __restrict__ ij,
-                                uint32_t segments, NumiClothBagGPUFruit* __restrict__ fruits, uint32_t fruitCount,
-                                const NumiClothBagGPUYarnContact* __restrict__ contacts,
-                                NumiClothBagGPUFrictionStatus* __restrict__ status, uint32_t* __restrict__ failure) {
-    extern __shared__ float4 velocities[];
-    if (*ok == 0u) return;
-    const float mu = cfg->fruitMaterial.w;
-    if (!(mu >= 0.0f) || !isfinite(mu)) {
-        if (threadIdx.x == 0) recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
-        return;
-    }
-    for (uint32_t i = threadIdx.x; i < particles; i += blockDim.x) velocities[i] = vel[i];
-    __syncthreads();
-    const uint32_t f = threadIdx.x;
-    const bool lane = f < fruitCount;
-    NumiClothBagGPUFruit fruit{};
-    if (lane) fruit = fruits[f];
-    const bool ground = cfg->constraintCounts.z != 0u;
-    for (uint32_t batch = 0; batch < batchCount; ++batch) {
-        const uint4 b = batches[batch];
-        const uint32_t rounds = (fruitCount + b.y - 1) / b.y;
-        for (uint32_t step = 0; step < b.y * rounds; ++step) {
-            const uint32_t phase = step / rounds, round = step % rounds;
-            if (lane && f / b.y == round) {
-                const uint32_t segment = b.x + (phase + f) % b.y;
-                const uint2 e = ij[segment];
-                const NumiClothBagGPUYarnContact& c = contacts[f * segments + segment];
-                const float jn = c.fruitNormalAndImpulse.w;
-                const float nl = length(xyz(c.fruitNormalAndImpulse));
-                if (jn > 0.0f && nl > 1.0e-10f) {
-                    const V3 n = xyz(c.fruitNormalAndImpulse) / nl;
-                    float wx = c.segmentImpulse.x / jn, wy = c.segmentImpulse.y / jn;
-                    const float ws = wx + wy;
-                    if (ws > 1.0e-12f) {
-                        wx = wx / ws;
-                        wy = wy / ws;
-                        float4 va = velocities[e.x], vb = velocities[e.y];
-                        const float4 xa = pos[e.x], xb = pos[e.y];
-                        const V3 yarnV = xyz(va) * wx + xyz(vb) * wy;
-                        const V3 offset = n * -fruit.previousAndRadius.w;
-                        const V3 ballV = xyz(fruit.velocityAndGroundImpulse) + cross(xyz(fruit.angularVelocity), offset);
-                        const V3 rel = ballV - yarnV;
-                        const V3 tv = rel - n * dot(rel, n);
-                        const float slip = length(tv);
-                        if (slip > 1.0e-10f) {
-                            const V3 t = tv / slip;
-                            const V3 lever = cross(offset, t);
-                            float denom = fruit.positionAndInverseMass.w + fruitInverseInertia(fruit) * dot(lever, lever);
-                            V3 ra = t * xa.w;
-                            V3 rb = t * xb.w;
-                            if (ground) {
-                                if (xa.z <= cfg->clothMaterial.x + 1.0e-6f && ra.z < 0.0f) ra.z = 0.0f;
-                                if (xb.z <= cfg->clothMaterial.x + 1.0e-6f && rb.z < 0.0f) rb.z = 0.0f;
-                            }
-                            denom += dot(t, ra) * wx * wx + dot(t, rb) * wy * wy;
-                            if (denom > 0.0f && isfinite(denom)) {
-                                const float limit = mu * jn;
-                                const float jt = fminf(slip / denom, limit);
-                                if (jt > 0.0f) {
-                                    applyFruitImpulse(fruit, t * -jt, offset);
-                                    va = withXyz(va, xyz(va) + ra * (jt * wx));
-                                    vb = withXyz(vb, xyz(vb) + rb * (jt * wy));
-                                    velocities[e.x] = va;
-                                    velocities[e.y] = vb;
-                                    recordFriction(status, 1, jt, limit);
-                                }
-                            }
-                        }
-                    }
-                }
-            }
-            __syncthreads();
-        }
-    }
-    if (lane) {
-        if (!finite4(fruit.velocityAndGroundImpulse) || !finite4(fruit.angularVelocity)) {
-            recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
-        } else {
-            fruits[f] = fruit;
-        }
-    }
-    for (uint32_t i = threadIdx.x; i < particles; i += blockDim.x) vel[i] = velocities[i];
-}
-
-__global__ void k_fruit_pair_friction(const NumiClothBagGPUConfig* __restrict__ cfg, const uint32_t* __restrict__ ok,
-                                      uint32_t first, uint32_t count, NumiClothBagGPUFruit* __restrict__ fruits,
-                                      const NumiClothBagGPUFruitPair* __restrict__ pairs,
-                                      NumiClothBagGPUFrictionStatus* __restrict__ status,
-                                      uint32_t* __restrict__ failure) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= count || *ok == 0u) return;
-    const NumiClothBagGPUFruitPair pair = pairs[first + t];
+// Coulomb friction of one fruit pair (reference apply_fruit_pair_friction,
+// src/metal/ClothBag.metal:3160-3248) on fruits held by the caller.
+__device__ __forceinline__ void fruitPairFriction(const NumiClothBagGPUConfig* cfg, NumiClothBagGPUFruit& a,
+                                                  NumiClothBagGPUFruit& b, const NumiClothBagGPUFruitPair& pair,
+                                                  NumiClothBagGPUFrictionStatus* status, uint32_t* failure,
+                                                  bool& wrote) {
+    wrote = false;
     const float mu = cfg->fruitMaterial.x;
     const float jn = pair.contact.w;
     const float nl = length(xyz(pair.contact));
@@ -649,8 +504,6 @@ __global__ void k_fruit_pair_friction(const NumiClothBagGPUConfig* __restrict__ 
         return;
     }
     if (!(jn > 0.0f) || !(nl > 1.0e-10f)) return;
-    NumiClothBagGPUFruit a = fruits[pair.fruitsAndColor.x];
-    NumiClothBagGPUFruit b = fruits[pair.fruitsAndColor.y];
     const V3 n = xyz(pair.contact) / nl;
     const V3 oa = n * a.previousAndRadius.w;
     const V3 ob = n * -b.previousAndRadius.w;
@@ -669,17 +522,48 @@ __global__ void k_fruit_pair_friction(const NumiClothBagGPUConfig* __restrict__ 
     const float limit = mu * jn;
     const float jt = fminf(slip / denom, limit);
     if (!(jt > 0.0f)) return;
+    NumiClothBagGPUFruit na = a, nb = b;
     const V3 onB = tangent * -jt;
-    applyFruitImpulse(b, onB, ob);
-    applyFruitImpulse(a, -onB, oa);
-    if (!finite4(a.velocityAndGroundImpulse) || !finite4(b.velocityAndGroundImpulse) ||
-        !finite4(a.angularVelocity) || !finite4(b.angularVelocity)) {
+    applyFruitImpulse(nb, onB, ob);
+    applyFruitImpulse(na, -onB, oa);
+    if (!finite4(na.velocityAndGroundImpulse) || !finite4(nb.velocityAndGroundImpulse) ||
+        !finite4(na.angularVelocity) || !finite4(nb.angularVelocity)) {
         recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
         return;
     }
-    fruits[pair.fruitsAndColor.x] = a;
-    fruits[pair.fruitsAndColor.y] = b;
+    a = na;
+    b = nb;
+    wrote = true;
     recordFriction(status, 0, jt, limit);
+}
+
+// All fruit-pair friction colours in one workgroup, fruits in LDS.
+__global__ void k_fruit_pairs_friction(const NumiClothBagGPUConfig* __restrict__ cfg, const uint32_t* __restrict__ ok,
+                                       const uint2* __restrict__ colours, uint32_t colourCount,
+                                       NumiClothBagGPUFruit* __restrict__ fruits, uint32_t fruitCount,
+                                       const NumiClothBagGPUFruitPair* __restrict__ pairs,
+                                       NumiClothBagGPUFrictionStatus* __restrict__ status,
+                                       uint32_t* __restrict__ failure) {
+    __shared__ NumiClothBagGPUFruit local[32];
+    if (*ok == 0u) return;
+    for (uint32_t i = threadIdx.x; i < fruitCount; i += blockDim.x) local[i] = fruits[i];
+    __syncthreads();
+    for (uint32_t c = 0; c < colourCount; ++c) {
+        const uint2 range = colours[c];
+        for (uint32_t t = threadIdx.x; t < range.y; t += blockDim.x) {
+            const NumiClothBagGPUFruitPair pair = pairs[range.x + t];
+            NumiClothBagGPUFruit a = local[pair.fruitsAndColor.x];
+            NumiClothBagGPUFruit b = local[pair.fruitsAndColor.y];
+            bool wrote = false;
+            fruitPairFriction(cfg, a, b, pair, status, failure, wrote);
+            if (wrote) {
+                local[pair.fruitsAndColor.x] = a;
+                local[pair.fruitsAndColor.y] = b;
+            }
+        }
+        __syncthreads();
+    }
+    for (uint32_t i = threadIdx.x; i < fruitCount; i += blockDim.x) fruits[i] = local[i];
 }
 
 __global__ void k_fruit_ground_friction(const NumiClothBagGPUConfig* __restrict__ cfg, const uint32_t* __restrict__ ok,

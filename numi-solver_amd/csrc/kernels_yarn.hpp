@@ -1,0 +1,473 @@
+#pragma once
+
+// Fruit/yarn round-robin solves (reference solve_yarn_batch,
+// src/metal/ClothBag.metal:1675-1786, and apply_yarn_friction, :3024-3158).
+//
+// The reference order: for each distance colour batch of n segments, for
+// phase p = 0..n-1, fruit f visits segment first + (p + f) mod n; within a
+// phase the fruits touch distinct segments of one colour (disjoint knots),
+// so the phase is parallel and the phases are serial. Fruits f and f + n
+// would meet the same segment in one phase; they take turns (rounds), lower
+// fruit first.
+//
+// Almost every visit is a no-op (the fruit is nowhere near the segment), and
+// a no-op changes nothing, so the same result comes from running only the
+// visits that can act, in the same order:
+//   1. candidates: every (fruit, segment) visit that can act, counted per
+//      phase and scattered into phase order (count / scan / scatter in LDS);
+//   2. wave 0 runs the candidate visits phase by phase (one lane per fruit),
+//      knots in LDS, records staged in LDS;
+//   3. commit.
+// For the swept mode and for friction a visit acts exactly when its record
+// says so (impact flag, stored impulse), so the candidate set is exact. For
+// the present mode a visit acts when the fruit overlaps the segment at that
+// moment; candidates are the visits within `margin` of contact at the start
+// of the call, and after the run every other visit is checked to have stayed
+// out of contact: its start distance minus the distance its fruit and its
+// knots have moved in this call (sums of correction lengths) must still be
+// >= contact. If any visit fails that check, or the candidates overflow the
+// LDS staging, the call is run again visit by visit from the unchanged
+// global state. Either way the result is the reference's sequential one.
+
+#include "kernels_bag.hpp"
+
+namespace numi_dev {
+
+constexpr uint32_t kYarnSlots = 1024;   // staged candidate visits per call
+constexpr uint32_t kYarnThreads = 1024;
+
+// Orders LDS accesses of one wavefront (lanes of wave 0 exchange knots
+// through LDS between phases).
+__device__ __forceinline__ void waveSync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Fields of a fruit/yarn record a visit updates.
+struct YarnStage {
+    mr_float4 normalImpulse;  // fruitNormalAndImpulse
+    mr_float4 segment;        // segmentImpulse
+    uint32_t responses;     // control.w
+    uint32_t acted;
+};
+
+struct YarnArgs {
+    const NumiClothBagGPUConfig* cfg;
+    const uint32_t* ok;
+    uint32_t mode;                       // 1 swept impacts, 0 present overlap
+    const uint4* batches;                // (first, count) per distance colour
+    uint32_t batchCount;
+    const uint2* segmentBatch;           // per segment: (first, count) of its colour
+    float4* pos;
+    float4* vel;                         // friction only
+    uint32_t particles;
+    const uint2* ij;
+    uint32_t segments;
+    NumiClothBagGPUFruit* fruits;
+    uint32_t fruitCount;
+    NumiClothBagGPUYarnContact* contacts;
+    NumiClothBagGPUFrictionStatus* status;  // friction only
+    float margin;                        // present mode candidate margin
+    uint32_t* failure;
+};
+
+__device__ __forceinline__ uint32_t phaseBit(uint32_t f, uint32_t s, uint2 batch) {
+    const uint32_t j = s - batch.x;
+    const uint32_t fm = f % batch.y;
+    return batch.x + (j + batch.y - fm) % batch.y;
+}
+
+// The visit (fruit, segment) of the solve: updates fruit, knots a and b (in
+// LDS) and the staged record; returns the correction lengths it applied.
+__device__ __forceinline__ float yarnVisit(const YarnArgs& a, NumiClothBagGPUFruit& fruit,
+                                           const NumiClothBagGPUYarnContact& record, YarnStage& stage, float4* knots,
+                                           uint2 e, float* moved, bool& acted) {
+    acted = false;
+    const bool ground = a.cfg->constraintCounts.z != 0u;
+    const float floorZ = a.cfg->clothMaterial.x;
+    const float dt = a.cfg->gravityAndTimestep.w;
+    V3 n = v3(0.0f, 0.0f, 0.0f);
+    float wa = 0.0f, wb = 0.0f, correction = 0.0f;
+    if (a.mode == 1u) {
+        if (record.control.y != 0u) {
+            n = xyz(record.sweptNormalAndAdvance);
+            wb = record.weightsAndTime.y;
+            wa = 1.0f - wb;
+            correction = record.sweptNormalAndAdvance.w;
+        }
+    } else {
+        const V3 ka = xyz(knots[e.x]), kb = xyz(knots[e.y]);
+        const V3 centre = xyz(fruit.positionAndInverseMass);
+        const PointSample s = samplePointSegment(centre, ka, kb);
+        const float target = fruit.previousAndRadius.w + a.cfg->clothMaterial.x;
+        if (s.distance < target) {
+            n = s.distance < 1.0e-12f ? degenerateYarnNormal(ka, kb) : (s.closest - centre) / s.distance;
+            wb = s.weight;
+            wa = 1.0f - wb;
+            correction = target - s.distance;
+        }
+    }
+    if (!(correction > 0.0f)) return 0.0f;
+    bool wrote = false;
+    float4 ka = knots[e.x], kb = knots[e.y];
+    const V3 c0 = xyz(fruit.positionAndInverseMass);
+    const float lambda = yarnCorrection(ka, kb, fruit, wa, wb, n, correction, ground, floorZ, a.failure, wrote);
+    float travel = length(xyz(fruit.positionAndInverseMass) - c0);
+    if (wrote) {
+        moved[e.x] += length(xyz(ka) - xyz(knots[e.x]));
+        moved[e.y] += length(xyz(kb) - xyz(knots[e.y]));
+        knots[e.x] = ka;
+        knots[e.y] = kb;
+    }
+    if (lambda > 0.0f) {
+        const float impulse = lambda / dt;
+        setXyz(stage.normalImpulse, xyz(stage.normalImpulse) - n * impulse);
+        stage.normalImpulse.w += impulse;
+        stage.segment.x += wa * impulse;
+        stage.segment.y += wb * impulse;
+        if (a.mode == 1u) {
+            stage.segment.z = record.weightsAndTime.z;
+            stage.segment.w += correction;
+        }
+        stage.responses += 1u;
+        acted = true;
+    }
+    return travel;
+}
+
+// Can visit (f, s) act? Swept mode: its impact flag. Present mode: the fruit
+// is within margin of contact at the start of the call.
+__device__ __forceinline__ bool yarnCandidate(const YarnArgs& a, uint32_t f, uint32_t s, const float4* knotsStart,
+                                              const NumiClothBagGPUFruit* fruitsStart) {
+    if (a.mode == 1u) return a.contacts[f * a.segments + s].control.y != 0u;
+    const uint2 e = a.ij[s];
+    const NumiClothBagGPUFruit fr = fruitsStart[f];
+    const PointSample ps = samplePointSegment(xyz(fr.positionAndInverseMass), xyz(knotsStart[e.x]), xyz(knotsStart[e.y]));
+    return ps.distance < fr.previousAndRadius.w + a.cfg->clothMaterial.x + a.margin;
+}
+
+// Exclusive scan of counts[0..n) in place by the whole workgroup; returns
+// the total (same value in every thread).
+__device__ uint32_t blockExclusiveScan(uint32_t* counts, uint32_t n, uint32_t* scratch) {
+    const uint32_t tid = threadIdx.x, threads = blockDim.x;
+    const uint32_t chunk = (n + threads - 1) / threads;
+    const uint32_t begin = min(n, tid * chunk), end = min(n, begin + chunk);
+    uint32_t sum = 0;
+    for (uint32_t i = begin; i < end; ++i) sum += counts[i];
+    scratch[tid] = sum;
+    __syncthreads();
+    for (uint32_t offset = 1; offset < threads; offset <<= 1) {
+        const uint32_t add = tid >= offset ? scratch[tid - offset] : 0u;
+        __syncthreads();
+        scratch[tid] += add;
+        __syncthreads();
+    }
+    uint32_t running = scratch[tid] - sum;
+    const uint32_t total = scratch[threads - 1];
+    for (uint32_t i = begin; i < end; ++i) {
+        const uint32_t c = counts[i];
+        counts[i] = running;
+        running += c;
+    }
+    __syncthreads();
+    return total;
+}
+
+// The visit-by-visit schedule on wave 0 (the fallback, and the reference's
+// literal order): knots in LDS, records read and written in global memory.
+__device__ void yarnDense(const YarnArgs& a, float4* knots, float* moved) {
+    const uint32_t lane = threadIdx.x;
+    const bool live = lane < a.fruitCount;
+    NumiClothBagGPUFruit fruit{};
+    if (live) fruit = a.fruits[lane];
+    for (uint32_t batch = 0; batch < a.batchCount; ++batch) {
+        const uint4 b = a.batches[batch];
+        const uint32_t rounds = (a.fruitCount + b.y - 1) / b.y;
+        for (uint32_t step = 0; step < b.y * rounds; ++step) {
+            const uint32_t phase = step / rounds, round = step % rounds;
+            if (live && lane / b.y == round) {
+                const uint32_t segment = b.x + (phase + lane) % b.y;
+                const uint32_t index = lane * a.segments + segment;
+                NumiClothBagGPUYarnContact c = a.contacts[index];
+                YarnStage stage{c.fruitNormalAndImpulse, c.segmentImpulse, c.control.w, 0u};
+                bool acted = false;
+                yarnVisit(a, fruit, c, stage, knots, a.ij[segment], moved, acted);
+                if (acted) {
+                    c.fruitNormalAndImpulse = stage.normalImpulse;
+                    c.segmentImpulse = stage.segment;
+                    c.control.w = stage.responses;
+                    a.contacts[index] = c;
+                }
+            }
+            waveSync();
+        }
+    }
+    if (live) {
+        if (!finite3(xyz(fruit.positionAndInverseMass))) recordFailure(a.failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
+        a.fruits[lane] = fruit;
+    }
+}
+
+// One solve_yarn_batch call over all distance colours. One workgroup of
+// kYarnThreads; dynamic LDS: knots[P], moved[P], counts[D], slot arrays.
+__global__ void __launch_bounds__(kYarnThreads) k_yarn_solve(YarnArgs a) {
+    extern __shared__ float4 yarnLds[];
+    __shared__ uint32_t scanScratch[kYarnThreads];
+    __shared__ uint32_t slotPair[kYarnSlots];
+    __shared__ YarnStage slotStage[kYarnSlots];
+    __shared__ float fruitMoved[32];
+    __shared__ mr_float4 fruitStaged[32];
+    __shared__ uint32_t verdict;  // 0 sparse result stands, 1 rerun visit by visit
+    if (*a.ok == 0u) return;
+    const uint32_t tid = threadIdx.x, threads = blockDim.x;
+    const uint32_t P = a.particles, D = a.segments, F = a.fruitCount;
+    float4* knots = yarnLds;
+    float* moved = reinterpret_cast<float*>(knots + P);
+    uint32_t* counts = reinterpret_cast<uint32_t*>(moved + P);
+    for (uint32_t i = tid; i < P; i += threads) {
+        knots[i] = a.pos[i];
+        moved[i] = 0.0f;
+    }
+    for (uint32_t i = tid; i < D; i += threads) counts[i] = 0u;
+    if (tid < 32) fruitMoved[tid] = 0.0f;
+    if (tid == 0) verdict = 0u;
+    __syncthreads();
+    // 1. count candidate visits per phase
+    const uint32_t pairs = F * D;
+    for (uint32_t idx = tid; idx < pairs; idx += threads) {
+        const uint32_t f = idx / D, s = idx % D;
+        if (a.segmentBatch[s].y != 0u && yarnCandidate(a, f, s, a.pos, a.fruits))
+            atomicAdd(&counts[phaseBit(f, s, a.segmentBatch[s])], 1u);
+    }
+    __syncthreads();
+    const uint32_t total = blockExclusiveScan(counts, D, scanScratch);
+    if (total > kYarnSlots) {
+        if (tid == 0) verdict = 1u;
+    } else {
+        // scatter: counts[b] becomes the end of phase b's group
+        for (uint32_t idx = tid; idx < pairs; idx += threads) {
+            const uint32_t f = idx / D, s = idx % D;
+            if (a.segmentBatch[s].y == 0u || !yarnCandidate(a, f, s, a.pos, a.fruits)) continue;
+            const uint32_t slot = atomicAdd(&counts[phaseBit(f, s, a.segmentBatch[s])], 1u);
+            slotPair[slot] = idx;
+            const NumiClothBagGPUYarnContact& c = a.contacts[idx];
+            slotStage[slot] = YarnStage{c.fruitNormalAndImpulse, c.segmentImpulse, c.control.w, 0u};
+        }
+    }
+    __syncthreads();
+    // 2. candidate visits in phase order on wave 0
+    if (verdict == 0u && tid < 64) {
+        const uint32_t lane = tid;
+        const bool live = lane < F;
+        NumiClothBagGPUFruit fruit{};
+        if (live) fruit = a.fruits[lane];
+        float travel = 0.0f;
+        for (uint32_t batch = 0; batch < a.batchCount; ++batch) {
+            const uint4 b = a.batches[batch];
+            const uint32_t rounds = (F + b.y - 1) / b.y;
+            for (uint32_t bit = b.x; bit < b.x + b.y; ++bit) {
+                const uint32_t first = bit == 0 ? 0u : counts[bit - 1];
+                const uint32_t last = counts[bit];
+                if (first == last) continue;
+                for (uint32_t round = 0; round < rounds; ++round) {
+                    if (live && lane / b.y == round) {
+                        for (uint32_t k = first; k < last; ++k) {
+                            const uint32_t idx = slotPair[k];
+                            if (idx / D != lane) continue;
+                            const uint32_t s = idx % D;
+                            YarnStage stage = slotStage[k];
+                            bool acted = false;
+                            travel += yarnVisit(a, fruit, a.contacts[idx], stage, knots, a.ij[s], moved, acted);
+                            stage.acted = acted ? 1u : 0u;
+                            slotStage[k] = stage;
+                        }
+                    }
+                    waveSync();
+                }
+            }
+        }
+        if (live) {
+            fruitMoved[lane] = travel;
+            fruitStaged[lane] = fruit.positionAndInverseMass;  // committed after the check below
+        }
+    }
+    __syncthreads();
+    // 3. present mode: every visit that was not run must have stayed a no-op
+    if (verdict == 0u && a.mode == 0u) {
+        bool bad = false;
+        for (uint32_t idx = tid; idx < pairs; idx += threads) {
+            const uint32_t f = idx / D, s = idx % D;
+            if (a.segmentBatch[s].y == 0u || yarnCandidate(a, f, s, a.pos, a.fruits)) continue;
+            const uint2 e = a.ij[s];
+            const NumiClothBagGPUFruit fr = a.fruits[f];
+            const PointSample ps = samplePointSegment(xyz(fr.positionAndInverseMass), xyz(a.pos[e.x]), xyz(a.pos[e.y]));
+            const float target = fr.previousAndRadius.w + a.cfg->clothMaterial.x;
+            const float reach = fruitMoved[f] + fmaxf(moved[e.x], moved[e.y]);
+            if (!(ps.distance - reach >= target)) bad = true;
+        }
+        if (bad) verdict = 1u;
+    }
+    __syncthreads();
+    if (verdict == 0u) {
+        for (uint32_t i = tid; i < P; i += threads) a.pos[i] = knots[i];
+        for (uint32_t k = tid; k < total; k += threads) {
+            const YarnStage st = slotStage[k];
+            if (st.acted == 0u) continue;
+            NumiClothBagGPUYarnContact& c = a.contacts[slotPair[k]];
+            c.fruitNormalAndImpulse = st.normalImpulse;
+            c.segmentImpulse = st.segment;
+            c.control.w = st.responses;
+        }
+        if (tid < F) {
+            NumiClothBagGPUFruit fr = a.fruits[tid];
+            fr.positionAndInverseMass = fruitStaged[tid];
+            if (!finite3(xyz(fr.positionAndInverseMass))) recordFailure(a.failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
+            a.fruits[tid] = fr;
+        }
+        return;
+    }
+    // Rerun visit by visit from the unchanged global state.
+    for (uint32_t i = tid; i < P; i += threads) {
+        knots[i] = a.pos[i];
+        moved[i] = 0.0f;
+    }
+    __syncthreads();
+    if (tid < 64) yarnDense(a, knots, moved);
+    __syncthreads();
+    for (uint32_t i = tid; i < P; i += threads) a.pos[i] = knots[i];
+}
+
+// Friction of one fruit/yarn record (reference apply_yarn_friction,
+// src/metal/ClothBag.metal:3024-3158); velocities in LDS.
+__device__ __forceinline__ void yarnFrictionVisit(const YarnArgs& a, NumiClothBagGPUFruit& fruit,
+                                                  const NumiClothBagGPUYarnContact& c, float4* velocities, uint2 e) {
+    const float mu = a.cfg->fruitMaterial.w;
+    const bool ground = a.cfg->constraintCounts.z != 0u;
+    const float jn = c.fruitNormalAndImpulse.w;
+    const float nl = length(xyz(c.fruitNormalAndImpulse));
+    if (!(jn > 0.0f) || !(nl > 1.0e-10f)) return;
+    const V3 n = xyz(c.fruitNormalAndImpulse) / nl;
+    float wx = c.segmentImpulse.x / jn, wy = c.segmentImpulse.y / jn;
+    const float ws = wx + wy;
+    if (!(ws > 1.0e-12f)) return;
+    wx = wx / ws;
+    wy = wy / ws;
+    float4 va = velocities[e.x], vb = velocities[e.y];
+    const float4 xa = a.pos[e.x], xb = a.pos[e.y];
+    const V3 yarnV = xyz(va) * wx + xyz(vb) * wy;
+    const V3 offset = n * -fruit.previousAndRadius.w;
+    const V3 ballV = xyz(fruit.velocityAndGroundImpulse) + cross(xyz(fruit.angularVelocity), offset);
+    const V3 rel = ballV - yarnV;
+    const V3 tv = rel - n * dot(rel, n);
+    const float slip = length(tv);
+    if (!(slip > 1.0e-10f)) return;
+    const V3 tangent = tv / slip;
+    const V3 lever = cross(offset, tangent);
+    float denom = fruit.positionAndInverseMass.w + fruitInverseInertia(fruit) * dot(lever, lever);
+    V3 ra = tangent * xa.w;
+    V3 rb = tangent * xb.w;
+    if (ground) {
+        if (xa.z <= a.cfg->clothMaterial.x + 1.0e-6f && ra.z < 0.0f) ra.z = 0.0f;
+        if (xb.z <= a.cfg->clothMaterial.x + 1.0e-6f && rb.z < 0.0f) rb.z = 0.0f;
+    }
+    denom += dot(tangent, ra) * wx * wx + dot(tangent, rb) * wy * wy;
+    if (!(denom > 0.0f) || !isfinite(denom)) return;
+    const float limit = mu * jn;
+    const float jt = fminf(slip / denom, limit);
+    if (!(jt > 0.0f)) return;
+    applyFruitImpulse(fruit, tangent * -jt, offset);
+    velocities[e.x] = withXyz(va, xyz(va) + ra * (jt * wx));
+    velocities[e.y] = withXyz(vb, xyz(vb) + rb * (jt * wy));
+    recordFriction(a.status, 1, jt, limit);
+}
+
+// Yarn friction over all colours: a visit acts only if its record carries a
+// normal impulse, so exactly those visits run, in phase order.
+__global__ void __launch_bounds__(kYarnThreads) k_yarn_friction_solve(YarnArgs a) {
+    extern __shared__ float4 yarnLds[];
+    __shared__ uint32_t scanScratch[kYarnThreads];
+    __shared__ uint32_t slotPair[kYarnSlots];
+    __shared__ uint32_t overflow;
+    if (*a.ok == 0u) return;
+    const float mu = a.cfg->fruitMaterial.w;
+    if (!(mu >= 0.0f) || !isfinite(mu)) {
+        if (threadIdx.x == 0) recordFailure(a.failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
+        return;
+    }
+    const uint32_t tid = threadIdx.x, threads = blockDim.x;
+    const uint32_t P = a.particles, D = a.segments, F = a.fruitCount;
+    float4* velocities = yarnLds;
+    uint32_t* counts = reinterpret_cast<uint32_t*>(velocities + P);
+    for (uint32_t i = tid; i < P; i += threads) velocities[i] = a.vel[i];
+    for (uint32_t i = tid; i < D; i += threads) counts[i] = 0u;
+    if (tid == 0) overflow = 0u;
+    __syncthreads();
+    const uint32_t pairs = F * D;
+    auto acts = [&](uint32_t idx) {
+        return a.segmentBatch[idx % D].y != 0u && a.contacts[idx].fruitNormalAndImpulse.w > 0.0f;
+    };
+    for (uint32_t idx = tid; idx < pairs; idx += threads) {
+        const uint32_t f = idx / D, s = idx % D;
+        if (acts(idx)) atomicAdd(&counts[phaseBit(f, s, a.segmentBatch[s])], 1u);
+    }
+    __syncthreads();
+    const uint32_t total = blockExclusiveScan(counts, D, scanScratch);
+    if (total > kYarnSlots) {
+        if (tid == 0) overflow = 1u;
+    } else {
+        for (uint32_t idx = tid; idx < pairs; idx += threads) {
+            const uint32_t f = idx / D, s = idx % D;
+            if (acts(idx)) slotPair[atomicAdd(&counts[phaseBit(f, s, a.segmentBatch[s])], 1u)] = idx;
+        }
+    }
+    __syncthreads();
+    if (tid < 64) {
+        const uint32_t lane = tid;
+        const bool live = lane < F;
+        NumiClothBagGPUFruit fruit{};
+        if (live) fruit = a.fruits[lane];
+        for (uint32_t batch = 0; batch < a.batchCount; ++batch) {
+            const uint4 b = a.batches[batch];
+            const uint32_t rounds = (F + b.y - 1) / b.y;
+            if (overflow != 0u) {
+                // every visit, in the reference order
+                for (uint32_t step = 0; step < b.y * rounds; ++step) {
+                    const uint32_t phase = step / rounds, round = step % rounds;
+                    if (live && lane / b.y == round) {
+                        const uint32_t segment = b.x + (phase + lane) % b.y;
+                        yarnFrictionVisit(a, fruit, a.contacts[lane * D + segment], velocities, a.ij[segment]);
+                    }
+                    waveSync();
+                }
+                continue;
+            }
+            for (uint32_t bit = b.x; bit < b.x + b.y; ++bit) {
+                const uint32_t first = bit == 0 ? 0u : counts[bit - 1];
+                const uint32_t last = counts[bit];
+                if (first == last) continue;
+                for (uint32_t round = 0; round < rounds; ++round) {
+                    if (live && lane / b.y == round) {
+                        for (uint32_t k = first; k < last; ++k) {
+                            const uint32_t idx = slotPair[k];
+                            if (idx / D != lane) continue;
+                            yarnFrictionVisit(a, fruit, a.contacts[idx], velocities, a.ij[idx % D]);
+                        }
+                    }
+                    waveSync();
+                }
+            }
+        }
+        if (live) {
+            if (!finite4(fruit.velocityAndGroundImpulse) || !finite4(fruit.angularVelocity)) {
+                recordFailure(a.failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
+            } else {
+                a.fruits[lane] = fruit;
+            }
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < P; i += threads) a.vel[i] = velocities[i];
+}
+
+}  // namespace numi_dev

@@ -26,6 +26,7 @@
 #include "kernels_self.hpp"
 #include "kernels_tile.hpp"
 #include "kernels_bag.hpp"
+#include "kernels_yarn.hpp"
 #include "tile_plan.hpp"
 #include "numi_cloth_hip.h"
 #include "numi_cloth_scene.h"
@@ -128,6 +129,8 @@ struct NumiClothHipSolver {
     uint32_t F = 0, FP = 0, Y = 0;
     bool yarnAir = false;
     std::vector<Range> pairColours;
+    DeviceArray<uint2> pairColourRanges;
+    DeviceArray<uint2> segmentBatch;  // per distance: (first, count) of its colour
     std::vector<uint32_t> rimHost;
     DeviceArray<NumiClothBagGPUFruit> fruits;
     DeviceArray<NumiClothBagGPUFruitPair> fruitPairs;
@@ -265,16 +268,39 @@ struct NumiClothHipSolver {
     uint32_t fruitLanes() const { return std::max(64u, (F + 63u) / 64u * 64u); }
 
     void launchFruitPairs() {
-        for (const Range& r : pairColours)
-            k_fruit_pair<<<blocksFor(r.count), kBlock, 0, stream>>>(cfg(), ok(), r.first, r.count, fruits.ptr,
-                                                                    fruitPairs.ptr, failure.ptr);
+        if (pairColours.empty()) return;
+        k_fruit_pairs<<<1, 64, 0, stream>>>(cfg(), ok(), pairColourRanges.ptr,
+                                            static_cast<uint32_t>(pairColours.size()), fruits.ptr, F,
+                                            fruitPairs.ptr, failure.ptr);
+    }
+
+    YarnArgs yarnArgs(uint32_t mode, float4* positions) const {
+        YarnArgs a{};
+        a.cfg = cfg();
+        a.ok = ok();
+        a.mode = mode;
+        a.batches = distanceBatchesDevice.ptr;
+        a.batchCount = static_cast<uint32_t>(distanceColours.size());
+        a.segmentBatch = segmentBatch.ptr;
+        a.pos = positions;
+        a.vel = vel.ptr;
+        a.particles = P;
+        a.ij = dIJ.ptr;
+        a.segments = D;
+        a.fruits = fruits.ptr;
+        a.fruitCount = F;
+        a.contacts = yarnContacts.ptr;
+        a.status = frictionStatus.ptr;
+        // Present-mode candidates: within one yarn radius of contact.
+        a.margin = initialConfig.clothMaterial.x;
+        a.failure = failure.ptr;
+        return a;
     }
 
     void launchYarn(uint32_t mode) {
         if (Y == 0) return;
-        k_yarn_batches<<<1, std::max(256u, fruitLanes()), P * sizeof(float4), stream>>>(
-            cfg(), ok(), mode, distanceBatchesDevice.ptr, static_cast<uint32_t>(distanceColours.size()), posCur, P,
-            dIJ.ptr, D, fruits.ptr, F, yarnContacts.ptr, failure.ptr);
+        const size_t lds = static_cast<size_t>(P) * (sizeof(float4) + sizeof(float)) + static_cast<size_t>(D) * 4;
+        k_yarn_solve<<<1, kYarnThreads, lds, stream>>>(yarnArgs(mode, posCur));
     }
 
     void launchBuildYarnContacts() {
@@ -513,14 +539,14 @@ struct NumiClothHipSolver {
         if (F != 0) {
             k_fruit_finalize<<<blocksFor(F), kBlock, 0, stream>>>(cfg(), ok(), fruits.ptr, F, failure.ptr);
             if (Y != 0) {
-                k_yarn_friction<<<1, std::max(256u, fruitLanes()), P * sizeof(float4), stream>>>(
-                    cfg(), ok(), distanceBatchesDevice.ptr, static_cast<uint32_t>(distanceColours.size()), pos.ptr,
-                    vel.ptr, P, dIJ.ptr, D, fruits.ptr, F, yarnContacts.ptr, frictionStatus.ptr, failure.ptr);
+                const size_t lds = static_cast<size_t>(P) * sizeof(float4) + static_cast<size_t>(D) * 4;
+                k_yarn_friction_solve<<<1, kYarnThreads, lds, stream>>>(yarnArgs(0u, pos.ptr));
             }
-            for (const Range& r : pairColours)
-                k_fruit_pair_friction<<<blocksFor(r.count), kBlock, 0, stream>>>(cfg(), ok(), r.first, r.count,
-                                                                                 fruits.ptr, fruitPairs.ptr,
-                                                                                 frictionStatus.ptr, failure.ptr);
+            if (!pairColours.empty()) {
+                k_fruit_pairs_friction<<<1, 64, 0, stream>>>(cfg(), ok(), pairColourRanges.ptr,
+                                                             static_cast<uint32_t>(pairColours.size()), fruits.ptr, F,
+                                                             fruitPairs.ptr, frictionStatus.ptr, failure.ptr);
+            }
             k_fruit_ground_friction<<<blocksFor(F), kBlock, 0, stream>>>(cfg(), ok(), fruits.ptr, F,
                                                                          frictionStatus.ptr, failure.ptr);
             k_fruit_orientation<<<blocksFor(F), kBlock, 0, stream>>>(cfg(), ok(), fruits.ptr, F, failure.ptr);
@@ -817,19 +843,33 @@ void createSolver(const NumiClothHipSceneDesc& scene, NumiClothHipSolver& s) {
         }
         s.fruitPairs.allocate(s.FP);
         s.fruitPairs.upload(pairs.data(), s.FP, st);
+        std::vector<uint2> ranges;
+        for (const Range& r : s.pairColours) ranges.push_back(make_uint2(r.first, r.count));
+        s.pairColourRanges.allocate(ranges.size());
+        s.pairColourRanges.upload(ranges.data(), ranges.size(), st);
         s.yarnContacts.allocate(s.Y);
         s.yarnContacts.upload(scene.yarnContacts, s.Y, st);
         if (s.Y != 0) {
-            if (s.F > 1024) fail(NUMI_CLOTH_HIP_ERROR_UNSUPPORTED, "fruit/yarn solve runs one fruit per lane");
+            if (s.F > 32) fail(NUMI_CLOTH_HIP_ERROR_UNSUPPORTED, "fruit/yarn solve runs one fruit per lane of one wave");
             int maxShared = 0;
             HIP_OK(hipDeviceGetAttribute(&maxShared, hipDeviceAttributeMaxSharedMemoryPerBlock, s.device));
-            if (static_cast<uint64_t>(s.P) * sizeof(float4) > static_cast<uint64_t>(maxShared)) {
+            // Static LDS of the yarn kernels (scan scratch, staged visits,
+            // fruits) plus the knots, their travel and the phase counts.
+            const uint64_t fixed = 4ull * kYarnThreads + (4ull + sizeof(YarnStage)) * kYarnSlots + 1024;
+            const uint64_t dynamic = static_cast<uint64_t>(s.P) * 20 + static_cast<uint64_t>(s.D) * 4;
+            if (fixed + dynamic > static_cast<uint64_t>(maxShared)) {
                 fail(NUMI_CLOTH_HIP_ERROR_UNSUPPORTED, "fruit/yarn contact stages every knot in LDS");
             }
             std::vector<uint4> batches;
-            for (const Range& r : s.distanceColours) batches.push_back(make_uint4(r.first, r.count, 0u, 0u));
+            std::vector<uint2> segmentBatch(s.D, make_uint2(0u, 0u));
+            for (const Range& r : s.distanceColours) {
+                batches.push_back(make_uint4(r.first, r.count, 0u, 0u));
+                for (uint32_t k = r.first; k < r.first + r.count; ++k) segmentBatch[k] = make_uint2(r.first, r.count);
+            }
             s.distanceBatchesDevice.allocate(batches.size());
             s.distanceBatchesDevice.upload(batches.data(), batches.size(), st);
+            s.segmentBatch.allocate(s.D);
+            s.segmentBatch.upload(segmentBatch.data(), s.D, st);
         }
         if (scene.mouthRimCount != 0) {
             if (scene.mouthRimParticles == nullptr) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "null mouth rim");
@@ -1103,8 +1143,11 @@ void setupTiles(NumiClothHipSolver& s) {
     uint32_t bestOwned = 0, bestBlock = 256;
     std::vector<uint32_t> sizes;
     if (s.P <= 4096) sizes.push_back(s.P);
+    // Smaller tiles only add halo work once the tiles of 4096 knots already
+    // give every CU a workgroup, so large scenes try fewer sizes.
+    const uint32_t smallest = s.P >= 1024u * 4096u ? 2048u : (s.P >= 256u * 4096u ? 1024u : 256u);
     for (uint32_t owned : {4096u, 2048u, 1024u, 512u, 256u})
-        if (owned < s.P) sizes.push_back(owned);
+        if (owned < s.P && owned >= smallest) sizes.push_back(owned);
     for (uint32_t owned : sizes) {
         NumiClothHipSolver::TileProgram candidate;
         if (!buildTileProgram(s, even, owned, candidate)) continue;

@@ -108,11 +108,15 @@ inline bool planTiles(const TileTopology& topo, const std::vector<TilePass>& pro
             lo[a] = std::min(lo[a], topo.positions[4 * i + a]);
             hi[a] = std::max(hi[a], topo.positions[4 * i + a]);
         }
+    // One scale for all axes (the largest extent), so that a thin axis such
+    // as the jitter of a flat sheet contributes only its true share of bits
+    // and consecutive codes form compact patches.
+    double span = 0.0;
+    for (int a = 0; a < 3; ++a) span = std::max(span, static_cast<double>(hi[a]) - lo[a]);
     std::vector<std::pair<uint64_t, uint32_t>> keyed(P);
     for (uint32_t i = 0; i < P; ++i) {
         uint64_t code = 0;
         for (int a = 0; a < 3; ++a) {
-            const double span = static_cast<double>(hi[a]) - lo[a];
             const double u = span > 0.0 ? (static_cast<double>(topo.positions[4 * i + a]) - lo[a]) / span : 0.0;
             code |= spreadBits21(static_cast<uint64_t>(u * 2097151.0)) << a;
         }
@@ -158,11 +162,15 @@ inline bool planTiles(const TileTopology& topo, const std::vector<TilePass>& pro
     std::vector<uint32_t> loaded;
     std::vector<uint8_t> exact;
     std::vector<uint32_t> tileItems;
+    // Tiles of one mesh need about the same halo: each search starts at the
+    // previous tile's (any sufficient halo is exact; the smallest only saves
+    // redundant work).
+    uint32_t startHalo = 0;
     for (uint32_t t = 0; t < tiles; ++t) {
         const uint32_t begin = t * owned;
         const uint32_t end = std::min(P, begin + owned);
         bool done = false;
-        for (uint32_t halo = 0; halo <= maxHalo && !done; ++halo) {
+        for (uint32_t halo = startHalo; halo <= maxHalo && !done; ++halo) {
             // Owned knots first (Morton order), then the halo by BFS depth.
             loaded.clear();
             for (uint32_t k = begin; k < end; ++k) loaded.push_back(keyed[k].second);
@@ -254,6 +262,7 @@ inline bool planTiles(const TileTopology& topo, const std::vector<TilePass>& pro
             plan->ownedCount.push_back(end - begin);
             plan->maxLoaded = std::max<uint32_t>(plan->maxLoaded, static_cast<uint32_t>(loaded.size()));
             plan->halo = std::max(plan->halo, halo);
+            startHalo = halo;
             plan->computed += tileItems.size() / 4;
             for (uint32_t p : loaded) stamp[p] = ~0u;
             done = true;

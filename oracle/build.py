@@ -14,7 +14,7 @@ from pathlib import Path
 ORACLE = Path(__file__).resolve().parent
 REPO = ORACLE.parent
 LIBRARY = ORACLE / "libnumi_cloth_oracle.so"
-SOURCES = ["cloth_oracle.cpp"]
+SOURCES = ["cloth_oracle.cpp", "cloth_adjoint.cpp"]
 
 
 def build(verbose: bool = False) -> Path:

@@ -98,3 +98,53 @@ def self_candidates_fp32(scene, particles: np.ndarray, config: np.ndarray) -> np
     library().numi_cloth_oracle_self_candidates_fp32(C.addressof(scene.desc), particles.ctypes.data,
                                                      config.ctypes.data, pairs.ctypes.data, total)
     return pairs[:total]
+
+
+class AdjointProblem(C.Structure):
+    _fields_ = [("tracked", C.c_void_p), ("trackedCount", C.c_uint32), ("reserved", C.c_uint32),
+                ("targets", C.c_void_p), ("distanceComplianceScale", C.c_void_p),
+                ("bendComplianceScale", C.c_void_p), ("inverseMassScale", C.c_void_p), ("damping", C.c_void_p)]
+
+
+class Gradients(C.Structure):
+    _fields_ = [("loss", C.c_double), ("distanceCompliance", C.c_void_p), ("bendCompliance", C.c_void_p),
+                ("inverseMass", C.c_void_p), ("damping", C.c_void_p), ("positions", C.c_void_p)]
+
+
+def _array_pointer(array):
+    return None if array is None else array.ctypes.data
+
+
+def adjoint(scene, configs: np.ndarray, iterations: int, strain_sweeps: int, tracked: np.ndarray,
+            targets: np.ndarray, gradients: bool = True, distance_scale=None, bend_scale=None,
+            inverse_mass_scale=None, damping=None) -> dict:
+    """FP64 loss sum_s sum_k |x_k(s) - target_k(s)|^2 over the substeps and,
+    with gradients=True, its gradient with respect to distance and bend
+    compliance, inverse mass and damping coefficient (per record / knot).
+    The *_scale / damping arrays override parameters in double."""
+    lib = library()
+    if not hasattr(lib, "_adjoint_ready"):
+        lib.numi_cloth_oracle_adjoint.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
+                                                  C.POINTER(AdjointProblem), C.POINTER(Gradients)]
+        lib.numi_cloth_oracle_adjoint.restype = C.c_int
+        lib._adjoint_ready = True
+    configs = np.ascontiguousarray(configs)
+    tracked = np.ascontiguousarray(tracked, np.uint32)
+    targets = np.ascontiguousarray(targets, np.float64)
+    assert targets.shape == (len(configs), len(tracked), 3)
+    keep = [np.ascontiguousarray(a, np.float64) if a is not None else None
+            for a in (distance_scale, bend_scale, inverse_mass_scale, damping)]
+    problem = AdjointProblem(tracked.ctypes.data, len(tracked), 0, targets.ctypes.data,
+                             *[_array_pointer(a) for a in keep])
+    P, D, B = len(scene.particles), len(scene.distances), len(scene.bends)
+    out = {"distanceCompliance": np.zeros(D), "bendCompliance": np.zeros(B), "inverseMass": np.zeros(P),
+           "damping": np.zeros(P), "positions": np.zeros((P, 3))}
+    g = Gradients(0.0, *(out[name].ctypes.data if gradients and out[name].size else None
+                         for name in ("distanceCompliance", "bendCompliance", "inverseMass", "damping")),
+                  out["positions"].ctypes.data)
+    status = lib.numi_cloth_oracle_adjoint(C.addressof(scene.desc), configs.ctypes.data, len(configs), iterations,
+                                           strain_sweeps, C.byref(problem), C.byref(g))
+    if status != 0:
+        raise RuntimeError(f"oracle adjoint failed with status {status}")
+    out["loss"] = g.loss
+    return out
