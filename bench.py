@@ -23,6 +23,14 @@ Workloads (synthetic, seeded with SplitMix64 / numpy PCG64 from SEED):
              one launch per pass, hipGraph substeps), 48 substeps per 1/60 s
              frame; reports specimen-frames/s and the speed-up over stepping
              the specimens one solver at a time.
+  adjoint64  64 strips of 64 x 64 knots (compliances log-U[1e-8,1e-5],
+             density U[0.1,0.4], damping U[0.5,2] 1/s), loss = L2 distance of
+             32 tracked knots per strip to a seeded target trajectory over 200
+             substeps of 1/960 s; gradients with respect to every distance and
+             bend compliance, every knot's inverse mass (density) and damping,
+             checkpoints every 16 substeps; reports gradients/s (one gradient
+             = one specimen's full parameter gradient) and the cost relative
+             to the plain forward of the same 200 substeps.
 All use 32 XPBD iterations and 3 strain-limit sweeps per substep.
 """
 
@@ -56,7 +64,7 @@ SHEETS = {
     "sheet1024": dict(width=1024, height=1024, substeps=16, frame=1.0 / 60.0, sphereMode=2, sphereCount=64,
                       sphereRadiusMin=0.02, sphereRadiusMax=0.10),
 }
-WORKLOADS = sorted([*SHEETS, "bag", "strips512"])
+WORKLOADS = sorted([*SHEETS, "bag", "strips512", "adjoint64"])
 
 
 def _hip():
@@ -142,6 +150,26 @@ def strip_specimens(cloth, count: int = 512):
     return specimens
 
 
+def adjoint_problem(cloth, specimens_count: int = 64, substeps: int = 200, tracked: int = 32):
+    rng = np.random.default_rng(SEED + 1)
+    dt = 1.0 / 960.0
+    specimens = []
+    for k in range(specimens_count):
+        specimens.append(cloth.Scene.strip(
+            width=64, height=64, spacing=0.01, arealDensity=float(rng.uniform(0.1, 0.4)),
+            stretchCompliance=float(10.0 ** rng.uniform(-8, -5)), bendCompliance=float(10.0 ** rng.uniform(-8, -5)),
+            strainLimit=0.10, timestep=dt, driveAmplitude=0.02, driveFrequency=float(rng.uniform(0.5, 2.0)),
+            drivePhase=float(rng.uniform(0.0, 2.0 * np.pi)), damping=float(rng.uniform(0.5, 2.0)),
+            origin=(0.7 * k, 0.0, 1.0), jitter=1.0e-4, seed=SEED + 1000 + k))
+    local = np.stack([rng.choice(np.arange(64, 64 * 64), tracked, replace=False) for _ in specimens]).astype(np.uint32)
+    x0 = np.stack([s.particles["positionAndInverseMass"][local[k], :3] for k, s in enumerate(specimens)])
+    amplitude = rng.uniform(-0.02, 0.02, x0.shape)
+    frequency = rng.uniform(0.5, 2.0, x0.shape[:2] + (1,))
+    t = (np.arange(1, substeps + 1) * np.float32(dt))[:, None, None, None]
+    targets = (x0[None] + amplitude[None] * np.sin(2 * np.pi * frequency[None] * t)).astype(np.float32)
+    return specimens, local, targets, specimens[0].configs(substeps)
+
+
 # ------------------------------------------------------------ baselines --
 
 def cpu_baseline(scene, configs, substeps: int) -> dict:
@@ -171,6 +199,20 @@ def cpu_baseline_strips(specimens, count: int) -> dict:
     return {"value": len(sample) / seconds, "unit": "specimen-frames/s", "cores": threads, "kind": "port",
             "sample": f"1 frame (48 substeps) of {len(sample)} strips, FP64 oracle, {threads} threads, "
                       f"{seconds:.2f} s on {platform.processor() or platform.machine()}"}
+
+
+def cpu_baseline_adjoint(specimens, local, targets, configs) -> dict:
+    """FP64 adjoint oracle, one specimen, all 200 substeps, one core."""
+    from oracle import pyoracle
+    pyoracle.library()
+    scene = specimens[0]
+    start = time.perf_counter()
+    pyoracle.adjoint(scene, configs if len(configs) else scene.configs(1), ITERATIONS, STRAIN_SWEEPS,
+                     local[0], targets[:, 0].astype(np.float64))
+    seconds = time.perf_counter() - start
+    return {"value": 1.0 / seconds, "unit": "gradients/s", "cores": 1, "kind": "port",
+            "sample": f"one 64x64 strip, {len(configs)} substeps, FP64 adjoint oracle (tape recomputed per "
+                      f"substep), {seconds:.2f} s on {platform.processor() or platform.machine()}"}
 
 
 # -------------------------------------------------------------- roofline --
@@ -287,13 +329,14 @@ def run_strips(args, cloth, hip) -> dict:
     sample = [specimens[k] for k in np.random.default_rng(SEED).choice(len(specimens), 16, replace=False)]
     total = 0.0
     for scene in sample:
+        own = scene.configs(48)
         with cloth.Solver(scene) as solver:
-            solver.step(configs, ITERATIONS, STRAIN_SWEEPS)
+            solver.step(own, ITERATIONS, STRAIN_SWEEPS)
             solver.synchronize()
             t = Timer(hip, solver.stream)
             t.begin()
             for _ in range(2):
-                solver.step(configs, ITERATIONS, STRAIN_SWEEPS)
+                solver.step(own, ITERATIONS, STRAIN_SWEEPS)
             total += t.end()
     single = 2 * len(sample) / total
     baseline = None if args.no_cpu_baseline else cpu_baseline_strips(specimens, 16)
@@ -306,6 +349,52 @@ def run_strips(args, cloth, hip) -> dict:
     return {"metric": "batched specimen-frames/s", "value": batched, "unit": "specimen-frames/s",
             "ms_per_step": seconds * 1000.0 / args.steps, "dtype": "f32", "config": config, "roofline": line,
             "cpu_baseline": baseline, "data": "synthetic (seeded strips, PCG64 seed %d)" % SEED}
+
+
+def run_adjoint(args, cloth, hip) -> dict:
+    specimens, local, targets, configs = adjoint_problem(cloth)
+    substeps = len(configs)
+    particles = sum(len(s.particles) for s in specimens)
+    # Plain forward of the same trajectory (the library's step path).
+    forward = cloth.Batch(specimens)
+    forward.step(configs[:16], ITERATIONS, STRAIN_SWEEPS)
+    forward.synchronize()
+    stream = cloth.cloth.library().numi_cloth_hip_stream(forward.solver_handle)
+    timer = Timer(hip, stream)
+    timer.begin()
+    for _ in range(args.steps):
+        forward.step(configs, ITERATIONS, STRAIN_SWEEPS)
+    forward_seconds = timer.end() / args.steps
+    view = cloth.Solver.borrow(forward.solver_handle, specimens[0])
+    line, tile_config = roofline(cloth, view)
+    forward.close()
+    # Adjoint: forward with checkpoints, then the reverse; the library call
+    # returns when the gradients are in device memory.
+    batch = cloth.Batch(specimens)
+    for _ in range(args.warmup):
+        batch.adjoint(configs, ITERATIONS, STRAIN_SWEEPS, local, targets, 16)
+    stream = cloth.cloth.library().numi_cloth_hip_stream(batch.solver_handle)
+    timer = Timer(hip, stream)
+    timer.begin()
+    for _ in range(args.steps):
+        batch.adjoint(configs, ITERATIONS, STRAIN_SWEEPS, local, targets, 16)
+    adjoint_seconds = timer.end() / args.steps
+    grads = batch.gradients(0)
+    failure = batch.read(0)["failure"]
+    batch.close()
+    if failure != 0:
+        raise RuntimeError(f"device failure bits {failure}")
+    config = {"workload": "adjoint64", "specimens": len(specimens), "particles": particles, "substeps": substeps,
+              "checkpoint_interval": 16, "tracked_per_specimen": int(local.shape[1]), "iterations": ITERATIONS,
+              "strain_sweeps": STRAIN_SWEEPS, "forward_seconds": forward_seconds,
+              "adjoint_seconds": adjoint_seconds, "adjoint_over_forward": adjoint_seconds / forward_seconds,
+              "gradient_entries_per_specimen": int(grads["distanceCompliance"].size + grads["bendCompliance"].size
+                                                   + grads["inverseMass"].size + grads["damping"].size),
+              "loss_specimen0": grads["loss"], "tile_programs": tile_config}
+    return {"metric": "adjoint gradients/s", "value": len(specimens) / adjoint_seconds, "unit": "gradients/s",
+            "ms_per_step": adjoint_seconds * 1000.0, "dtype": "f32", "config": config, "roofline": line,
+            "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline_adjoint(specimens, local, targets, configs),
+            "data": "synthetic (seeded strips and targets, PCG64 seed %d)" % (SEED + 1)}
 
 
 def main() -> int:
@@ -325,7 +414,12 @@ def main() -> int:
         return 2
     cloth = __graft_entry__.load_package()
     hip = _hip()
-    result = run_strips(args, cloth, hip) if args.workload == "strips512" else run_plain(args, cloth, hip)
+    if args.workload == "strips512":
+        result = run_strips(args, cloth, hip)
+    elif args.workload == "adjoint64":
+        result = run_adjoint(args, cloth, hip)
+    else:
+        result = run_plain(args, cloth, hip)
     line = {"metric": result["metric"], "value": result["value"], "unit": result["unit"], "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": result["ms_per_step"],
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": result["dtype"],

@@ -271,6 +271,48 @@ int numi_cloth_hip_batch_read(NumiClothHipBatch* batch, uint32_t specimen,
 NumiClothHipSolver* numi_cloth_hip_batch_solver(NumiClothHipBatch* batch);
 const NumiClothHipSceneDesc* numi_cloth_hip_batch_scene(const NumiClothHipBatch* batch);
 
+/*
+ * Adjoint of a batch trajectory (no counterpart in the reference, whose
+ * calibration fitter takes finite differences of whole runs,
+ * tools/cloth_calibration.cpp:568-650). For contact-free specimens
+ * (distances, bends, strain limits, damping, driven pinned knots) it runs
+ * `substeps` substeps forward from the batch's current state (the state
+ * advances exactly as numi_cloth_hip_batch_step would leave it, through the
+ * specimen kernels: one workgroup per specimen, knots and multipliers in
+ * LDS), storing (x, v) every `checkpointInterval` substeps; then, segment by
+ * segment from the last, recomputes the segment from its checkpoint with a
+ * tape of every pass and runs it backwards. The loss of specimen s is
+ *   L_s = sum over substeps t and tracked knots k of |x_k(t) - target_k(t)|^2
+ * with x_k(t) the position after substep t.
+ */
+typedef struct NumiClothHipAdjointProblem {
+    const NumiClothBagGPUConfig* configs;  /* one per substep; counts replaced */
+    uint32_t substeps;
+    uint32_t iterations;
+    uint32_t strainSweeps;
+    uint32_t checkpointInterval;           /* substeps per checkpoint segment */
+    const uint32_t* tracked;               /* specimens x trackedPerSpecimen, specimen-local knots */
+    uint32_t trackedPerSpecimen;
+    uint32_t reserved;
+    const float* targets;                  /* substeps x specimens x trackedPerSpecimen x 3 */
+} NumiClothHipAdjointProblem;
+
+/* Gradients of one specimen's loss, in the specimen's own record order:
+ * dL/d(compliance) per distance and per bend, dL/dw per knot (inverse mass;
+ * for an areal density rho, dL/drho = -sum_i dL/dw_i * w_i / rho over the
+ * free knots) and dL/dc per knot for the damping coefficient c. */
+typedef struct NumiClothHipSpecimenGradients {
+    double* loss;
+    float* distanceCompliance;
+    float* bendCompliance;
+    float* inverseMass;
+    float* damping;
+} NumiClothHipSpecimenGradients;
+
+int numi_cloth_hip_batch_adjoint(NumiClothHipBatch* batch, const NumiClothHipAdjointProblem* problem);
+int numi_cloth_hip_batch_gradients(NumiClothHipBatch* batch, uint32_t specimen,
+                                   const NumiClothHipSpecimenGradients* out);
+
 int numi_cloth_hip_batch_destroy(NumiClothHipBatch* batch);
 
 #ifdef __cplusplus

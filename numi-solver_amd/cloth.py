@@ -173,6 +173,17 @@ class StripSpec(C.Structure):
                 ("damping", C.c_float), ("origin", C.c_float * 3), ("jitter", C.c_float), ("seed", C.c_uint64)]
 
 
+class AdjointProblem(C.Structure):
+    _fields_ = [("configs", C.c_void_p), ("substeps", C.c_uint32), ("iterations", C.c_uint32),
+                ("strainSweeps", C.c_uint32), ("checkpointInterval", C.c_uint32), ("tracked", C.c_void_p),
+                ("trackedPerSpecimen", C.c_uint32), ("reserved", C.c_uint32), ("targets", C.c_void_p)]
+
+
+class SpecimenGradients(C.Structure):
+    _fields_ = [("loss", C.c_void_p), ("distanceCompliance", C.c_void_p), ("bendCompliance", C.c_void_p),
+                ("inverseMass", C.c_void_p), ("damping", C.c_void_p)]
+
+
 class BagSpec(C.Structure):
     _fields_ = [("material", MaterialValues), ("stretchComplianceOverride", C.c_float),
                 ("bendComplianceOverride", C.c_float), ("strainLimitOverride", C.c_float),
@@ -230,6 +241,8 @@ def library() -> C.CDLL:
         "numi_cloth_hip_batch_solver": ([vp], vp),
         "numi_cloth_hip_batch_scene": ([vp], C.POINTER(SceneDesc)),
         "numi_cloth_hip_batch_destroy": ([vp], i32),
+        "numi_cloth_hip_batch_adjoint": ([vp, C.POINTER(AdjointProblem)], i32),
+        "numi_cloth_hip_batch_gradients": ([vp, u32, C.POINTER(SpecimenGradients)], i32),
         "numi_cloth_scene_desc": ([vp], C.POINTER(SceneDesc)),
         "numi_cloth_scene_destroy": ([vp], None),
         "numi_cloth_splitmix64": ([C.POINTER(C.c_uint64)], C.c_uint64),
@@ -528,6 +541,32 @@ class Batch:
         info = TileInfo()
         _check(library().numi_cloth_hip_tile_info(self.solver_handle, C.byref(info)))
         return info
+
+    def adjoint(self, configs: np.ndarray, iterations: int, strain_sweeps: int, tracked: np.ndarray,
+                targets: np.ndarray, checkpoint_interval: int = 16) -> None:
+        """Run len(configs) substeps with the specimen kernels and the reverse
+        pass; `tracked` is specimens x T specimen-local knots, `targets`
+        substeps x specimens x T x 3."""
+        configs = np.ascontiguousarray(configs, dtype=CONFIG)
+        tracked = np.ascontiguousarray(tracked, np.uint32)
+        targets = np.ascontiguousarray(targets, np.float32)
+        assert tracked.shape[0] == len(self.specimens)
+        assert targets.shape == (len(configs), len(self.specimens), tracked.shape[1], 3)
+        problem = AdjointProblem(configs.ctypes.data, len(configs), iterations, strain_sweeps, checkpoint_interval,
+                                 tracked.ctypes.data, tracked.shape[1], 0, targets.ctypes.data)
+        _check(library().numi_cloth_hip_batch_adjoint(self._handle, C.byref(problem)))
+
+    def gradients(self, specimen: int) -> dict:
+        s = self.specimens[specimen]
+        out = {"loss": np.zeros(1), "distanceCompliance": np.zeros(len(s.distances), np.float32),
+               "bendCompliance": np.zeros(len(s.bends), np.float32),
+               "inverseMass": np.zeros(len(s.particles), np.float32),
+               "damping": np.zeros(len(s.particles), np.float32)}
+        g = SpecimenGradients(*(_ptr(out[k]) for k in ("loss", "distanceCompliance", "bendCompliance",
+                                                        "inverseMass", "damping")))
+        _check(library().numi_cloth_hip_batch_gradients(self._handle, specimen, C.byref(g)))
+        out["loss"] = float(out["loss"][0])
+        return out
 
     def read(self, specimen: int) -> dict:
         s = self.specimens[specimen]

@@ -27,6 +27,7 @@
 #include "kernels_tile.hpp"
 #include "kernels_bag.hpp"
 #include "kernels_yarn.hpp"
+#include "kernels_adjoint.hpp"
 #include "tile_plan.hpp"
 #include "numi_cloth_hip.h"
 #include "numi_cloth_scene.h"
@@ -1444,6 +1445,28 @@ void NumiClothHipSolver::ensureStrainProgram(uint32_t sweeps) {
 // Many independent specimens stepped by one solver over their merged scene:
 // every colour launch, tile program and hipGraph replay covers all specimens.
 
+// Device buffers of the specimen adjoint (created on first use).
+struct AdjointBuffers {
+    uint32_t distanceColours = 0, bendColours = 0, maxKnots = 0, maxConstraints = 0;
+    size_t ldsBytes = 0;
+    DeviceArray<SpecimenInfo> info;
+    DeviceArray<uint4> distanceRanges, bendRanges;
+    DeviceArray<NumiClothBagGPUConfig> configs;
+    DeviceArray<double> times;
+    DeviceArray<uint32_t> tracked;
+    DeviceArray<float> targets;
+    DeviceArray<float4> checkpointPos, checkpointVel, scratchPos, scratchPrev, scratchVel;
+    DeviceArray<float> scratchLambdaD, scratchLambdaB;
+    DeviceArray<float4> tapeX, tapeV0;
+    DeviceArray<float> tapeLambdaD, tapeLambdaB, lossGrad;
+    DeviceArray<uint32_t> tapeActive;
+    DeviceArray<float4> gxState, gvState;
+    DeviceArray<float> gradInverseMass, gradDamping, gradDistance, gradBend;
+    DeviceArray<double> loss;
+    std::vector<uint32_t> trackedFirst, trackedCount;
+    bool ready = false;
+};
+
 struct NumiClothHipBatch {
     NumiClothHipScene* merged = nullptr;
     std::unique_ptr<NumiClothHipSolver> solver;
@@ -1452,6 +1475,7 @@ struct NumiClothHipBatch {
     uint64_t version = 0;          // step() calls so far
     uint64_t lambdaVersion = ~0ull; // version of the cached multipliers
     std::vector<float> dl, bl, kl;
+    AdjointBuffers adjoint;
     ~NumiClothHipBatch() {
         solver.reset();
         if (merged != nullptr) numi_cloth_scene_destroy(merged);
@@ -1553,6 +1577,295 @@ void readBatch(NumiClothHipBatch& b, uint32_t specimen, const NumiClothHipState&
             d.material.z = b.kl[index[k]];
             out.knots[k] = d;
         }
+    }
+}
+
+// ---------------------------------------------------------------- adjoint --
+
+// Per-specimen colour ranges of the merged scene (colour c of specimen s is
+// one contiguous range of merged colour batch c) and the LDS footprint.
+void setupAdjoint(NumiClothHipBatch& b) {
+    AdjointBuffers& A = b.adjoint;
+    if (A.ready) return;
+    NumiClothHipSolver& s = *b.solver;
+    const NumiClothHipSceneDesc& d = b.merged->desc;
+    const NumiClothBagGPUConfig& c = *d.config;
+    if (d.knotCount || d.gripCount || d.fruitCount || d.extension.staticSphereCount || d.extension.selfContact ||
+        c.constraintCounts.z != 0u || c.airVelocityAndDensity.w > 0.0f) {
+        fail(NUMI_CLOTH_HIP_ERROR_UNSUPPORTED,
+             "the adjoint covers distances, bends, strain limits, drive and damping (no contacts, grips, "
+             "crossing knots, fruits or air)");
+    }
+    const uint32_t S = b.specimens;
+    std::vector<uint32_t> owner(s.P);
+    std::vector<SpecimenInfo> info(S);
+    for (uint32_t k = 0; k < S; ++k) {
+        uint32_t count = 0;
+        const uint32_t* idx = mergeIndices(b, NUMI_CLOTH_HIP_MERGE_PARTICLE, k, &count);
+        info[k] = SpecimenInfo{count ? idx[0] : 0u, count, 0u, 0u, 0u, 0u, {0u, 0u}};
+        for (uint32_t i = 0; i < count; ++i) owner[idx[i]] = k;
+        if (count > kSpecimenOwn * kSpecimenThreads) {
+            fail(NUMI_CLOTH_HIP_ERROR_UNSUPPORTED, "adjoint specimens hold at most 5120 knots");
+        }
+    }
+    auto ranges = [&](const NumiClothBagGPUBatch* batches, uint32_t batchCount, auto endsOf, uint32_t* colours,
+                      bool distance) {
+        uint32_t maxColour = 0;
+        for (uint32_t k = 0; k < batchCount; ++k) maxColour = std::max(maxColour, batches[k].control.z + 1);
+        *colours = maxColour;
+        std::vector<uint4> out(static_cast<size_t>(S) * maxColour, make_uint4(0u, 0u, 0u, 0u));
+        std::vector<uint32_t> local(S, 0u);
+        for (uint32_t k = 0; k < batchCount; ++k) {
+            const mr_uint4 batch = batches[k].control;
+            uint32_t j = batch.x;
+            while (j < batch.x + batch.y) {
+                const uint32_t spec = owner[endsOf(j)];
+                uint32_t end = j;
+                while (end < batch.x + batch.y && owner[endsOf(end)] == spec) ++end;
+                uint4& r = out[static_cast<size_t>(spec) * maxColour + batch.z];
+                if (r.y != 0) fail(NUMI_CLOTH_HIP_ERROR_TOPOLOGY, "specimen colour split across a batch");
+                r = make_uint4(j, end - j, local[spec], 0u);
+                local[spec] += end - j;
+                j = end;
+            }
+        }
+        for (uint32_t k = 0; k < S; ++k) (distance ? info[k].distanceCount : info[k].bendCount) = local[k];
+        return out;
+    };
+    const auto dr = ranges(d.distanceBatches, d.distanceBatchCount,
+                           [&](uint32_t j) { return d.distances[j].particlesAndColor.x; }, &A.distanceColours, true);
+    const auto br = ranges(d.bendBatches, d.bendBatchCount,
+                           [&](uint32_t j) { return d.bends[j].particlesAndColor.x; }, &A.bendColours, false);
+    A.maxKnots = 0;
+    A.maxConstraints = 0;
+    for (const SpecimenInfo& i : info) {
+        A.maxKnots = std::max(A.maxKnots, i.knotCount);
+        A.maxConstraints = std::max(A.maxConstraints, i.distanceCount + i.bendCount);
+    }
+    int maxShared = 0;
+    HIP_OK(hipDeviceGetAttribute(&maxShared, hipDeviceAttributeMaxSharedMemoryPerBlock, s.device));
+    A.ldsBytes = static_cast<size_t>(A.maxKnots) * sizeof(float4) + static_cast<size_t>(A.maxConstraints) * 4;
+    if (A.ldsBytes + 1024 > static_cast<size_t>(maxShared)) {
+        fail(NUMI_CLOTH_HIP_ERROR_UNSUPPORTED, "adjoint specimen does not fit in LDS");
+    }
+    hipStream_t st = s.stream;
+    A.info.allocate(S);
+    A.info.upload(info.data(), S, st);
+    A.distanceRanges.allocate(dr.size());
+    A.distanceRanges.upload(dr.data(), dr.size(), st);
+    A.bendRanges.allocate(std::max<size_t>(br.size(), 1));
+    A.bendRanges.upload(br.data(), br.size(), st);
+    A.gxState.allocate(s.P);
+    A.gvState.allocate(s.P);
+    A.gradInverseMass.allocate(s.P);
+    A.gradDamping.allocate(s.P);
+    A.gradDistance.allocate(std::max(s.D, 1u));
+    A.gradBend.allocate(std::max(s.B, 1u));
+    A.loss.allocate(S);
+    A.scratchPos.allocate(s.P);
+    A.scratchPrev.allocate(s.P);
+    A.scratchVel.allocate(s.P);
+    A.scratchLambdaD.allocate(std::max(s.D, 1u));
+    A.scratchLambdaB.allocate(std::max(s.B, 1u));
+    HIP_OK(hipStreamSynchronize(st));
+    A.ready = true;
+}
+
+uint32_t adjointPasses(const AdjointBuffers& A, uint32_t iterations, uint32_t sweeps) {
+    return A.distanceColours * iterations + A.bendColours * ((iterations + 1) / 2) +
+           (sweeps == 0 ? 0u : 18u * sweeps * A.distanceColours);
+}
+
+void runAdjoint(NumiClothHipBatch& b, const NumiClothHipAdjointProblem& pr) {
+    NumiClothHipSolver& s = *b.solver;
+    setupAdjoint(b);
+    AdjointBuffers& A = b.adjoint;
+    const uint32_t T = pr.substeps, L = pr.checkpointInterval, S = b.specimens;
+    if (T == 0 || L == 0 || pr.configs == nullptr || (pr.trackedPerSpecimen != 0 && pr.tracked == nullptr) ||
+        (pr.trackedPerSpecimen != 0 && pr.targets == nullptr)) {
+        fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "adjoint problem needs substeps, a checkpoint interval, configs, "
+                                                    "tracked knots and targets");
+    }
+    hipStream_t st = s.stream;
+    HIP_OK(hipStreamSynchronize(st));
+    // configs with the batch totals, times, tracked knots (merged indices)
+    std::vector<NumiClothBagGPUConfig> configs(pr.configs, pr.configs + T);
+    std::vector<double> times(T);
+    double t = s.time;
+    const NumiClothBagGPUConfig& totals = b.merged->config;
+    for (uint32_t k = 0; k < T; ++k) {
+        NumiClothBagGPUConfig& c = configs[k];
+        c.control.y = totals.control.y;
+        c.control.z = totals.control.z;
+        c.control.w = totals.control.w;
+        c.constraintCounts = totals.constraintCounts;
+        c.contactCounts = totals.contactCounts;
+        checkTrajectoryConfig(s, c);
+        if (c.control.x != NUMI_CLOTH_BAG_GPU_ABI_VERSION || !(c.gravityAndTimestep.w > 0.0f) ||
+            !std::isfinite(c.gravityAndTimestep.w) || !std::isfinite(c.gravityAndTimestep.z)) {
+            fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "adjoint configs must be valid (ABI version, finite dt > 0)");
+        }
+        t += c.gravityAndTimestep.w;
+        times[k] = t;
+    }
+    const uint32_t Tk = pr.trackedPerSpecimen;
+    std::vector<uint32_t> tracked(static_cast<size_t>(S) * Tk);
+    std::vector<SpecimenInfo> info(S);
+    A.info.download(info.data(), S, st);
+    HIP_OK(hipStreamSynchronize(st));
+    for (uint32_t k = 0; k < S; ++k) {
+        for (uint32_t q = 0; q < Tk; ++q) {
+            const uint32_t local = pr.tracked[static_cast<size_t>(k) * Tk + q];
+            if (local >= info[k].knotCount) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "tracked knot out of range");
+            tracked[static_cast<size_t>(k) * Tk + q] = info[k].knotFirst + local;
+        }
+        info[k].trackedFirst = k * Tk;
+        info[k].trackedCount = Tk;
+    }
+    A.info.upload(info.data(), S, st);
+    A.configs.allocate(T);
+    A.configs.upload(configs.data(), T, st);
+    A.times.allocate(T);
+    A.times.upload(times.data(), T, st);
+    A.tracked.allocate(std::max<size_t>(tracked.size(), 1));
+    A.tracked.upload(tracked.data(), tracked.size(), st);
+    const size_t targetCount = static_cast<size_t>(T) * S * Tk * 3;
+    A.targets.allocate(std::max<size_t>(targetCount, 1));
+    A.targets.upload(pr.targets, targetCount, st);
+    const uint32_t segments = (T + L - 1) / L;
+    const uint32_t passes = adjointPasses(A, pr.iterations, pr.strainSweeps);
+    A.checkpointPos.allocate(static_cast<size_t>(segments) * s.P);
+    A.checkpointVel.allocate(static_cast<size_t>(segments) * s.P);
+    A.tapeX.allocate(static_cast<size_t>(L) * passes * s.P);
+    A.tapeV0.allocate(static_cast<size_t>(L) * s.P);
+    A.tapeLambdaD.allocate(std::max<size_t>(static_cast<size_t>(L) * pr.iterations * s.D, 1));
+    A.tapeLambdaB.allocate(std::max<size_t>(static_cast<size_t>(L) * ((pr.iterations + 1) / 2) * s.B, 1));
+    A.tapeActive.allocate(std::max<size_t>(static_cast<size_t>(L) * passes * S, 1));
+    A.lossGrad.allocate(std::max<size_t>(static_cast<size_t>(L) * S * Tk * 3, 1));
+    HIP_OK(hipMemsetAsync(A.loss.ptr, 0, S * sizeof(double), st));
+    HIP_OK(hipMemsetAsync(A.gxState.ptr, 0, s.P * sizeof(float4), st));
+    HIP_OK(hipMemsetAsync(A.gvState.ptr, 0, s.P * sizeof(float4), st));
+    HIP_OK(hipMemsetAsync(A.gradInverseMass.ptr, 0, s.P * sizeof(float), st));
+    HIP_OK(hipMemsetAsync(A.gradDamping.ptr, 0, s.P * sizeof(float), st));
+    HIP_OK(hipMemsetAsync(A.gradDistance.ptr, 0, std::max(s.D, 1u) * sizeof(float), st));
+    HIP_OK(hipMemsetAsync(A.gradBend.ptr, 0, std::max(s.B, 1u) * sizeof(float), st));
+
+    SpecimenArgs a{};
+    a.specimens = A.info.ptr;
+    a.distanceRanges = A.distanceRanges.ptr;
+    a.bendRanges = A.bendRanges.ptr;
+    a.distanceColours = A.distanceColours;
+    a.bendColours = A.bendColours;
+    a.configs = A.configs.ptr;
+    a.times = A.times.ptr;
+    a.iterations = pr.iterations;
+    a.sweeps = pr.strainSweeps;
+    a.anchor = s.driven ? s.anchor.ptr : nullptr;
+    a.motion = s.motion.ptr;
+    a.drive = make_float4(s.extension.driveAmplitude, s.extension.driveOmega, s.extension.drivePhase, 0.0f);
+    const float dt0 = s.initialConfig.gravityAndTimestep.w;
+    a.dampingFactor = s.extension.damping > 0.0f ? 1.0f / (1.0f + s.extension.damping * dt0) : 1.0f;
+    a.dampingDt = dt0;
+    a.ij = s.dIJ.ptr;
+    a.restCompliance = s.dRestCompliance.ptr;
+    a.maxLength = s.dMaxLength.ptr;
+    a.bEnds = s.bEnds.ptr;
+    a.chordCompliance = s.bChordCompliance.ptr;
+    a.tracked = A.tracked.ptr;
+    a.targets = A.targets.ptr;
+    a.trackedTotal = S * Tk;
+    a.loss = A.loss.ptr;
+    a.passes = passes;
+    a.particles = s.P;
+    a.distances = s.D;
+    a.bends = s.B;
+    a.tapeX = A.tapeX.ptr;
+    a.tapeLambdaD = A.tapeLambdaD.ptr;
+    a.tapeLambdaB = A.tapeLambdaB.ptr;
+    a.tapeV0 = A.tapeV0.ptr;
+    a.tapeActive = A.tapeActive.ptr;
+    a.lossGrad = A.lossGrad.ptr;
+    a.specimenCount = S;
+    a.failure = s.failure.ptr;
+    a.gxState = A.gxState.ptr;
+    a.gvState = A.gvState.ptr;
+    a.gradInverseMass = A.gradInverseMass.ptr;
+    a.gradDamping = A.gradDamping.ptr;
+    a.gradDistance = A.gradDistance.ptr;
+    a.gradBend = A.gradBend.ptr;
+
+    // Forward over the trajectory on the solver's own state, a checkpoint of
+    // (x, v) at the start of every segment.
+    for (uint32_t k = 0; k < segments; ++k) {
+        HIP_OK(hipMemcpyAsync(A.checkpointPos.ptr + static_cast<size_t>(k) * s.P, s.pos.ptr, s.P * sizeof(float4),
+                              hipMemcpyDeviceToDevice, st));
+        HIP_OK(hipMemcpyAsync(A.checkpointVel.ptr + static_cast<size_t>(k) * s.P, s.vel.ptr, s.P * sizeof(float4),
+                              hipMemcpyDeviceToDevice, st));
+        SpecimenArgs f = a;
+        f.firstSubstep = k * L;
+        f.substeps = std::min(L, T - k * L);
+        f.record = 0u;
+        f.pos = s.pos.ptr;
+        f.prev = s.prev.ptr;
+        f.vel = s.vel.ptr;
+        f.dLambda = s.dLambda.ptr;
+        f.bLambda = s.bLambda.ptr;
+        k_specimen_forward<<<S, kSpecimenThreads, A.ldsBytes, st>>>(f);
+        HIP_OK(hipGetLastError());
+    }
+    // Reverse, segment by segment: recompute the segment from its checkpoint
+    // with the tape on (scratch state), then run it backwards.
+    for (uint32_t k = segments; k-- > 0;) {
+        HIP_OK(hipMemcpyAsync(A.scratchPos.ptr, A.checkpointPos.ptr + static_cast<size_t>(k) * s.P,
+                              s.P * sizeof(float4), hipMemcpyDeviceToDevice, st));
+        HIP_OK(hipMemcpyAsync(A.scratchVel.ptr, A.checkpointVel.ptr + static_cast<size_t>(k) * s.P,
+                              s.P * sizeof(float4), hipMemcpyDeviceToDevice, st));
+        HIP_OK(hipMemcpyAsync(A.scratchPrev.ptr, s.prev.ptr, s.P * sizeof(float4), hipMemcpyDeviceToDevice, st));
+        SpecimenArgs f = a;
+        f.firstSubstep = k * L;
+        f.substeps = std::min(L, T - k * L);
+        f.record = 1u;
+        f.pos = A.scratchPos.ptr;
+        f.prev = A.scratchPrev.ptr;
+        f.vel = A.scratchVel.ptr;
+        f.dLambda = A.scratchLambdaD.ptr;
+        f.bLambda = A.scratchLambdaB.ptr;
+        k_specimen_forward<<<S, kSpecimenThreads, A.ldsBytes, st>>>(f);
+        f.pos = s.pos.ptr;  // inverse masses for the reverse
+        k_specimen_reverse<<<S, kSpecimenThreads, A.ldsBytes, st>>>(f);
+        HIP_OK(hipGetLastError());
+    }
+    HIP_OK(hipStreamSynchronize(st));
+    s.time = t;
+    ++b.version;
+}
+
+void readGradients(NumiClothHipBatch& b, uint32_t specimen, const NumiClothHipSpecimenGradients& out) {
+    if (!b.adjoint.ready) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "no adjoint has run on this batch");
+    if (specimen >= b.specimens) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "specimen out of range");
+    NumiClothHipSolver& s = *b.solver;
+    AdjointBuffers& A = b.adjoint;
+    hipStream_t st = s.stream;
+    std::vector<float> gd(s.D), gb(s.B), gw(s.P), gc(s.P);
+    std::vector<double> loss(b.specimens);
+    A.gradDistance.download(gd.data(), s.D, st);
+    A.gradBend.download(gb.data(), s.B, st);
+    A.gradInverseMass.download(gw.data(), s.P, st);
+    A.gradDamping.download(gc.data(), s.P, st);
+    A.loss.download(loss.data(), b.specimens, st);
+    HIP_OK(hipStreamSynchronize(st));
+    uint32_t count = 0;
+    if (out.loss != nullptr) *out.loss = loss[specimen];
+    const uint32_t* index = mergeIndices(b, NUMI_CLOTH_HIP_MERGE_DISTANCE, specimen, &count);
+    if (out.distanceCompliance != nullptr)
+        for (uint32_t k = 0; k < count; ++k) out.distanceCompliance[k] = gd[index[k]];
+    index = mergeIndices(b, NUMI_CLOTH_HIP_MERGE_BEND, specimen, &count);
+    if (out.bendCompliance != nullptr)
+        for (uint32_t k = 0; k < count; ++k) out.bendCompliance[k] = gb[index[k]];
+    index = mergeIndices(b, NUMI_CLOTH_HIP_MERGE_PARTICLE, specimen, &count);
+    for (uint32_t k = 0; k < count; ++k) {
+        if (out.inverseMass != nullptr) out.inverseMass[k] = gw[index[k]];
+        if (out.damping != nullptr) out.damping[k] = gc[index[k]];
     }
 }
 
@@ -1685,6 +1998,21 @@ NumiClothHipSolver* numi_cloth_hip_batch_solver(NumiClothHipBatch* batch) {
 
 const NumiClothHipSceneDesc* numi_cloth_hip_batch_scene(const NumiClothHipBatch* batch) {
     return batch == nullptr ? nullptr : &batch->merged->desc;
+}
+
+int numi_cloth_hip_batch_adjoint(NumiClothHipBatch* batch, const NumiClothHipAdjointProblem* problem) {
+    return guarded([&] {
+        if (batch == nullptr || problem == nullptr) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "null argument");
+        runAdjoint(*batch, *problem);
+    });
+}
+
+int numi_cloth_hip_batch_gradients(NumiClothHipBatch* batch, uint32_t specimen,
+                                   const NumiClothHipSpecimenGradients* out) {
+    return guarded([&] {
+        if (batch == nullptr || out == nullptr) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "null argument");
+        readGradients(*batch, specimen, *out);
+    });
 }
 
 int numi_cloth_hip_batch_destroy(NumiClothHipBatch* batch) {

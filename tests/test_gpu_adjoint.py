@@ -1,0 +1,114 @@
+"""Specimen adjoint on the HIP path (needs a GPU).
+
+The forward half runs the specimen kernels (one workgroup per specimen, all
+state in LDS); it must leave the batch exactly where the plain batch step
+leaves it. The gradients (FP32, checkpoints every 16 substeps, in-segment
+recomputation) are compared with the FP64 adjoint oracle, itself pinned by
+finite differences (tests/test_oracle_adjoint.py), and the strongest one
+with central differences of the FP64 loss at h = 1e-6.
+"""
+
+import numpy as np
+import pytest
+
+import adjoint_problem
+from test_oracle_adjoint import _analytic, _damping, _fd
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu(cloth, p, checkpoint=16):
+    tracked_local = p["local"]
+    targets = p["targets"].reshape(len(p["configs"]), 2, tracked_local.shape[1], 3)
+    with cloth.Batch(p["specimens"]) as batch:
+        batch.adjoint(p["configs"], p["iterations"], p["sweeps"], tracked_local, targets, checkpoint)
+        grads = [batch.gradients(k) for k in range(2)]
+        states = [batch.read(k) for k in range(2)]
+    return grads, states
+
+
+@pytest.fixture(scope="module")
+def problem(cloth, oracle):
+    p = adjoint_problem.build(cloth)
+    p["out"] = oracle.adjoint(p["merged"], p["configs"], p["iterations"], p["sweeps"], p["tracked"], p["targets"])
+    p["gpu"], p["states"] = _gpu(cloth, p)
+    return p
+
+
+def test_specimen_forward_equals_plain_batch_step(cloth, problem):
+    p = problem
+    with cloth.Batch(p["specimens"]) as batch:
+        batch.step(p["configs"], p["iterations"], p["sweeps"])
+        plain = [batch.read(k) for k in range(2)]
+    for k in range(2):
+        assert problem["states"][k]["failure"] == 0
+        for name in ("particles", "distances", "bends"):
+            assert p["states"][k][name].tobytes() == plain[k][name].tobytes(), f"specimen {k} {name}"
+
+
+def test_loss_matches_oracle(problem):
+    p = problem
+    loss = sum(g["loss"] for g in p["gpu"])
+    print(f"loss gpu {loss:.9e} fp64 {p['out']['loss']:.9e}")
+    assert loss == pytest.approx(p["out"]["loss"], rel=1e-4)
+
+
+def _gpu_full(p, name):
+    """GPU per-record gradients scattered into merged order."""
+    key = {"distanceCompliance": "distances", "bendCompliance": "bends", "inverseMass": "particles",
+           "damping": "particles"}[name]
+    out = np.zeros_like(p["out"][name])
+    for k in range(2):
+        out[p["maps"][k][key]] = p["gpu"][k][name]
+    return out
+
+
+@pytest.mark.parametrize("name", ["distanceCompliance", "bendCompliance", "inverseMass", "damping"])
+def test_gradients_match_fp64_adjoint(problem, name):
+    p = problem
+    gpu = _gpu_full(p, name)
+    ref = p["out"][name]
+    error = np.linalg.norm(gpu - ref) / np.linalg.norm(ref)
+    print(f"{name}: relative L2 error {error:.3e}")
+    assert error < 1e-2
+    top = np.argsort(-np.abs(ref))[:5]
+    assert np.allclose(gpu[top], ref[top], rtol=1e-2)
+
+
+def test_specimen_parameters_match_fp64_adjoint(problem):
+    p = problem
+    ref = _analytic(p)
+    m = p["merged"]
+    w = m.particles["positionAndInverseMass"][:, 3].astype(np.float64)
+    damping = m.particleMotion[:, 3].astype(np.float64)
+    gw, gc = _gpu_full(p, "inverseMass"), _gpu_full(p, "damping")
+    for k in range(2):
+        idx = p["maps"][k]["particles"]
+        density = -np.sum(gw[idx] * w[idx])
+        damp = np.sum(gc[idx] * damping[idx])
+        print(f"specimen {k}: density {density:.6e} vs {ref[f'density{k}']:.6e}, damping {damp:.6e} vs "
+              f"{ref[f'damping{k}']:.6e}")
+        assert density == pytest.approx(ref[f"density{k}"], rel=1e-3)
+        assert damp == pytest.approx(ref[f"damping{k}"], rel=1e-3)
+
+
+def test_damping_gradient_matches_finite_differences(oracle, problem):
+    p = problem
+    m = p["merged"]
+    idx = p["maps"][0]["particles"]
+    damping = m.particleMotion[:, 3].astype(np.float64)
+    gpu = np.sum(_gpu_full(p, "damping")[idx] * damping[idx])
+    fd = _fd(oracle, p, 1e-6, damping=_damping(m, idx))
+    print(f"damping gradient gpu {gpu:.6e} central difference (h = 1e-6) {fd:.6e}")
+    assert gpu == pytest.approx(fd, rel=1e-3)
+
+
+def test_checkpoint_interval_does_not_change_gradients(cloth, problem):
+    p = problem
+    grads, _ = _gpu(cloth, p, checkpoint=7)
+    # Same arithmetic; only the order of the per-segment partial sums of the
+    # per-knot gradients changes.
+    for k in range(2):
+        for name in ("distanceCompliance", "bendCompliance", "inverseMass", "damping"):
+            a, b = grads[k][name], p["gpu"][k][name]
+            assert np.allclose(a, b, rtol=1e-4, atol=1e-6 * np.max(np.abs(b))), name
