@@ -34,20 +34,22 @@
 
 namespace {
 
-using Vec = V<double>;
 
+template <typename R>
 struct Entry {
     uint8_t kind;  // 0 distance, 1 bend, 2 strain
     uint32_t k, a, b;
-    Vec xa, xb;
-    double lambda;
+    V<R> xa, xb;
+    R lambda;
 };
 
 // State before a substep: the reverse pass re-runs the substep from it with
 // the tape on (one substep's tape is held at a time).
+template <typename R>
 struct SubstepRecord {
-    std::vector<Vec> x0, v0;
-    double time0 = 0, dt = 0;
+    std::vector<V<R>> x0, v0;
+    double time0 = 0;
+    R dt = 0;
 };
 
 struct Problem {
@@ -60,16 +62,19 @@ struct Problem {
     const double* damping = nullptr;
 };
 
+template <typename R>
 struct Model {
+    using Vec = V<R>;
+    static constexpr bool kTwin = sizeof(R) == sizeof(float);
     const NumiClothHipSceneDesc& d;
     uint32_t P, D, B;
     std::vector<Vec> x, v, prev, anchor;
-    std::vector<double> w, damping, dCompliance, bCompliance, maxLength, dLambda, bLambda;
-    std::vector<Entry> tape;
-    std::vector<SubstepRecord> substeps;
+    std::vector<R> w, damping, dCompliance, bCompliance, maxLength, dLambda, bLambda;
+    std::vector<Entry<R>> tape;
+    std::vector<SubstepRecord<R>> substeps;
     bool record = false;
     double time = 0;
-    double dt0 = 0;
+    float dt0 = 0;
 
     Model(const NumiClothHipSceneDesc& scene, const Problem& pr)
         : d(scene), P(scene.particleCount), D(scene.distanceCount), B(scene.bendCount) {
@@ -82,33 +87,37 @@ struct Model {
         const NumiClothHipSceneExtension& e = d.extension;
         for (uint32_t i = 0; i < P; ++i) {
             const NumiClothBagGPUParticle& s = d.particles[i];
-            x[i] = v3<double>(s.positionAndInverseMass);
-            prev[i] = v3<double>(s.previousAndMass);
-            v[i] = v3<double>(s.velocity);
-            w[i] = double(s.positionAndInverseMass.w) * (pr.inverseMassScale ? pr.inverseMassScale[i] : 1.0);
+            x[i] = v3<R>(s.positionAndInverseMass);
+            prev[i] = v3<R>(s.previousAndMass);
+            v[i] = v3<R>(s.velocity);
+            w[i] = R(double(s.positionAndInverseMass.w) * (pr.inverseMassScale ? pr.inverseMassScale[i] : 1.0));
             anchor[i] = x[i];
             const float c = e.particleMotion != nullptr ? e.particleMotion[i].w : e.damping;
-            damping[i] = pr.damping != nullptr ? pr.damping[i] : double(c);
+            damping[i] = R(pr.damping != nullptr ? pr.damping[i] : double(c));
         }
         dCompliance.resize(D);
         maxLength.resize(D);
         for (uint32_t k = 0; k < D; ++k) {
             const mr_float4 m = d.distances[k].material;
-            dCompliance[k] = double(m.y) * (pr.distanceComplianceScale ? pr.distanceComplianceScale[k] : 1.0);
-            maxLength[k] = double(m.x) * (1.0 + double(m.w));
+            dCompliance[k] = R(double(m.y) * (pr.distanceComplianceScale ? pr.distanceComplianceScale[k] : 1.0));
+            // The device precomputes rest * (1 + limit) in FP32.
+            maxLength[k] = kTwin ? R(m.x * (1.0f + m.w)) : R(m.x) * (R(1) + R(m.w));
         }
         bCompliance.resize(B);
         for (uint32_t k = 0; k < B; ++k) {
-            bCompliance[k] = double(d.bends[k].material.z) * (pr.bendComplianceScale ? pr.bendComplianceScale[k] : 1.0);
+            bCompliance[k] = R(double(d.bends[k].material.z) * (pr.bendComplianceScale ? pr.bendComplianceScale[k] : 1.0));
         }
-        dLambda.assign(D, 0.0);
-        bLambda.assign(B, 0.0);
+        dLambda.assign(D, R(0));
+        bLambda.assign(B, R(0));
         dt0 = d.config->gravityAndTimestep.w;
     }
 
-    double factor(uint32_t i) const { return damping[i] > 0.0 ? 1.0 / (1.0 + damping[i] * dt0) : 1.0; }
+    R factor(uint32_t i) const {
+        if (!(damping[i] > R(0))) return R(1);
+        return kTwin ? R(1.0f / (1.0f + float(damping[i]) * dt0)) : R(1) / (R(1) + damping[i] * R(dt0));
+    }
 
-    double driveOffset(uint32_t i, double endTime) const {
+    R driveOffset(uint32_t i, double endTime) const {
         const NumiClothHipSceneExtension& e = d.extension;
         float a = e.driveAmplitude, om = e.driveOmega, ph = e.drivePhase;
         if (e.particleMotion != nullptr) {
@@ -116,19 +125,19 @@ struct Model {
             om = e.particleMotion[i].y;
             ph = e.particleMotion[i].z;
         }
-        return double(static_cast<float>(double(a) * std::sin(double(om) * endTime + double(ph))));
+        return R(static_cast<float>(double(a) * std::sin(double(om) * endTime + double(ph))));
     }
 
-    void distancePass(const NumiClothBagGPUBatch& batch, double dt) {
+    void distancePass(const NumiClothBagGPUBatch& batch, R dt) {
         for (uint32_t k = batch.control.x; k < batch.control.x + batch.control.y; ++k) {
             const mr_uint4 e = d.distances[k].particlesAndColor;
             const Vec diff = x[e.y] - x[e.x];
-            const double len = norm(diff);
-            const double alpha = dCompliance[k] / (dt * dt);
-            const double denom = w[e.x] + w[e.y] + alpha;
-            if (!(len > 1.0e-12) || !(denom > 0.0)) continue;
+            const R len = norm(diff);
+            const R alpha = dCompliance[k] / (dt * dt);
+            const R denom = w[e.x] + w[e.y] + alpha;
+            if (!(len > R(1.0e-12)) || !(denom > R(0))) continue;
             if (record) tape.push_back({0, k, e.x, e.y, x[e.x], x[e.y], dLambda[k]});
-            const double dl = (-(len - double(d.distances[k].material.x)) - alpha * dLambda[k]) / denom;
+            const R dl = (-(len - R(d.distances[k].material.x)) - alpha * dLambda[k]) / denom;
             const Vec corr = diff * (dl / len);
             dLambda[k] += dl;
             x[e.x] = x[e.x] - corr * w[e.x];
@@ -136,20 +145,20 @@ struct Model {
         }
     }
 
-    void bendPass(const NumiClothBagGPUBatch& batch, double dt) {
+    void bendPass(const NumiClothBagGPUBatch& batch, R dt) {
         for (uint32_t k = batch.control.x; k < batch.control.x + batch.control.y; ++k) {
             const mr_uint4 e = d.bends[k].particlesAndColor;
             const uint32_t a = e.x, c = e.z;
             const Vec diff = x[c] - x[a];
-            const double chord = norm(diff);
-            if (!(chord > 1.0e-12)) continue;
+            const R chord = norm(diff);
+            if (!(chord > R(1.0e-12))) continue;
             const Vec dir = diff / chord;
             const Vec g[2] = {-dir, dir};
-            const double alpha = bCompliance[k] / (dt * dt);
-            double denom = alpha + w[a] * dot(g[0], g[0]) + w[c] * dot(g[1], g[1]);
-            if (!(denom >= 1.0e-16)) continue;
+            const R alpha = bCompliance[k] / (dt * dt);
+            R denom = alpha + w[a] * dot(g[0], g[0]) + w[c] * dot(g[1], g[1]);
+            if (!(denom >= R(1.0e-16))) continue;
             if (record) tape.push_back({1, k, a, c, x[a], x[c], bLambda[k]});
-            const double dl = (-(chord - double(d.bends[k].material.x)) - alpha * bLambda[k]) / denom;
+            const R dl = (-(chord - R(d.bends[k].material.x)) - alpha * bLambda[k]) / denom;
             bLambda[k] += dl;
             x[a] = x[a] + g[0] * (w[a] * dl);
             x[c] = x[c] + g[1] * (w[c] * dl);
@@ -160,13 +169,13 @@ struct Model {
         for (uint32_t k = batch.control.x; k < batch.control.x + batch.control.y; ++k) {
             const mr_uint4 e = d.distances[k].particlesAndColor;
             const Vec diff = x[e.y] - x[e.x];
-            const double len = norm(diff);
-            const double limit = maxLength[k];
-            if (!(len > limit) || !(len > 1.0e-12)) continue;
-            const double denom = w[e.x] + w[e.y];
-            if (!(denom > 0.0)) continue;
-            if (record) tape.push_back({2, k, e.x, e.y, x[e.x], x[e.y], 0.0});
-            const double excess = len - limit;
+            const R len = norm(diff);
+            const R limit = maxLength[k];
+            if (!(len > limit) || !(len > R(1.0e-12))) continue;
+            const R denom = w[e.x] + w[e.y];
+            if (!(denom > R(0))) continue;
+            if (record) tape.push_back({2, k, e.x, e.y, x[e.x], x[e.y], R(0)});
+            const R excess = len - limit;
             const Vec dir = diff / len;
             x[e.x] = x[e.x] + dir * (w[e.x] * excess / denom);
             x[e.y] = x[e.y] - dir * (w[e.y] * excess / denom);
@@ -178,34 +187,34 @@ struct Model {
             for (uint32_t b = 0; b < d.distanceBatchCount; ++b) strainPass(d.distanceBatches[b]);
     }
 
-    SubstepRecord snapshot(const NumiClothBagGPUConfig& c) const {
-        return {x, v, time, double(c.gravityAndTimestep.w)};
+    SubstepRecord<R> snapshot(const NumiClothBagGPUConfig& c) const {
+        return {x, v, time, R(c.gravityAndTimestep.w)};
     }
 
-    void restore(const SubstepRecord& rec) {
+    void restore(const SubstepRecord<R>& rec) {
         x = rec.x0;
         v = rec.v0;
         time = rec.time0;
     }
 
     void substep(const NumiClothBagGPUConfig& c, uint32_t iterations, uint32_t sweeps) {
-        const double dt = c.gravityAndTimestep.w;
+        const R dt = R(c.gravityAndTimestep.w);
         time += static_cast<double>(c.gravityAndTimestep.w);
-        const Vec g = v3<double>(c.gravityAndTimestep);
+        const Vec g = v3<R>(c.gravityAndTimestep);
         const bool driven = d.extension.driveAmplitude != 0.0f || d.extension.particleMotion != nullptr;
         for (uint32_t i = 0; i < P; ++i) {
             prev[i] = x[i];
-            if (w[i] > 0.0) {
-                const double f = factor(i);
-                const Vec carried = f != 1.0 ? v[i] * f : v[i];
+            if (w[i] > R(0)) {
+                const R f = factor(i);
+                const Vec carried = f != R(1) ? v[i] * f : v[i];
                 v[i] = fma3(g, dt, carried);
                 x[i] = fma3(v[i], dt, x[i]);
             } else if (driven) {
                 x[i] = {anchor[i].x, anchor[i].y, anchor[i].z + driveOffset(i, time)};
             }
         }
-        std::fill(dLambda.begin(), dLambda.end(), 0.0);
-        std::fill(bLambda.begin(), bLambda.end(), 0.0);
+        std::fill(dLambda.begin(), dLambda.end(), R(0));
+        std::fill(bLambda.begin(), bLambda.end(), R(0));
         for (uint32_t it = 0; it < iterations; ++it) {
             for (uint32_t b = 0; b < d.distanceBatchCount; ++b) distancePass(d.distanceBatches[b], dt);
             if (it % 2 == 0)
@@ -216,31 +225,33 @@ struct Model {
             for (int pass = 0; pass < 8; ++pass) strain(sweeps);
             for (int round = 0; round <= 8; ++round) strain(sweeps);
         }
-        const double inverseDt = 1.0 / dt;
+        const R inverseDt = R(1) / dt;
         for (uint32_t i = 0; i < P; ++i) v[i] = (x[i] - prev[i]) * inverseDt;
     }
 };
 
 // Adjoint of one XPBD projection x_a -= w_a n dl, x_b += w_b n dl,
 // lambda += dl with dl = (-(len - rest) - alpha lambda) / (w_a + w_b + alpha).
-void projectionAdjoint(const Entry& e, double rest, double alpha, double wa, double wb, Vec& ga, Vec& gb,
-                       double& glambda, double& galpha, double& gwa, double& gwb) {
+template <typename R>
+void projectionAdjoint(const Entry<R>& e, R rest, R alpha, R wa, R wb, V<R>& ga, V<R>& gb, R& glambda, R& galpha,
+                       R& gwa, R& gwb) {
+    using Vec = V<R>;
     const Vec d = e.xb - e.xa;
-    const double len = norm(d);
+    const R len = norm(d);
     const Vec n = d / len;
-    const double denom = wa + wb + alpha;
-    const double numerator = -(len - rest) - alpha * e.lambda;
-    const double dl = numerator / denom;
+    const R denom = wa + wb + alpha;
+    const R numerator = -(len - rest) - alpha * e.lambda;
+    const R dl = numerator / denom;
     // corr = n dl; x_a' = x_a - w_a corr; x_b' = x_b + w_b corr
     const Vec corr = n * dl;
     const Vec gcorr = gb * wb - ga * wa;
     gwa += -dot(corr, ga);
     gwb += dot(corr, gb);
-    const double gdl = glambda + dot(gcorr, n);
+    const R gdl = glambda + dot(gcorr, n);
     Vec gn = gcorr * dl;
-    const double gnum = gdl / denom;
-    const double gden = -gdl * dl / denom;
-    double glen = -gnum;
+    const R gnum = gdl / denom;
+    const R gden = -gdl * dl / denom;
+    R glen = -gnum;
     galpha += -e.lambda * gnum + gden;
     glambda += -alpha * gnum;
     gwa += gden;
@@ -253,21 +264,22 @@ void projectionAdjoint(const Entry& e, double rest, double alpha, double wa, dou
 
 // Adjoint of one strain limit x_a += w_a e/W n, x_b -= w_b e/W n,
 // e = len - maxLength, W = w_a + w_b.
-void strainAdjoint(const Entry& e, double maxLength, double wa, double wb, Vec& ga, Vec& gb, double& gwa,
-                   double& gwb) {
+template <typename R>
+void strainAdjoint(const Entry<R>& e, R maxLength, R wa, R wb, V<R>& ga, V<R>& gb, R& gwa, R& gwb) {
+    using Vec = V<R>;
     const Vec d = e.xb - e.xa;
-    const double len = norm(d);
+    const R len = norm(d);
     const Vec n = d / len;
-    const double W = wa + wb;
-    const double ex = len - maxLength;
-    const double sa = wa * ex / W, sb = wb * ex / W;
+    const R W = wa + wb;
+    const R ex = len - maxLength;
+    const R sa = wa * ex / W, sb = wb * ex / W;
     const Vec gn = ga * sa - gb * sb;
-    const double gsa = dot(ga, n), gsb = -dot(gb, n);
-    const double gex = (gsa * wa + gsb * wb) / W;
-    const double gW = -(gsa * sa + gsb * sb) / W;
+    const R gsa = dot(ga, n), gsb = -dot(gb, n);
+    const R gex = (gsa * wa + gsb * wb) / W;
+    const R gW = -(gsa * sa + gsb * sb) / W;
     gwa += gsa * ex / W + gW;
     gwb += gsb * ex / W + gW;
-    const double glen = gex;
+    const R glen = gex;
     const Vec gd = (gn - n * dot(gn, n)) / len + n * glen;
     ga = ga - gd;
     gb = gb + gd;
@@ -304,11 +316,17 @@ typedef struct NumiClothOracleGradients {
     double* positions;             /* P x 3 after the run, or NULL */
 } NumiClothOracleGradients;
 
-// Returns 0, 1 on a null argument, 5 when the scene has a family the
-// adjoint does not cover (contacts, grips, knots, fruits, air).
-int numi_cloth_oracle_adjoint(const NumiClothHipSceneDesc* scene, const NumiClothBagGPUConfig* configs,
-                              uint32_t substeps, uint32_t iterations, uint32_t sweeps,
-                              const NumiClothOracleAdjointProblem* problem, NumiClothOracleGradients* out) {
+}  // extern "C"
+
+namespace {
+
+// R = double: the FP64 adjoint. R = float: its twin, every expression in the
+// order of the device kernels (k_specimen_forward / k_specimen_reverse).
+template <typename R>
+int runAdjoint(const NumiClothHipSceneDesc* scene, const NumiClothBagGPUConfig* configs, uint32_t substeps,
+               uint32_t iterations, uint32_t sweeps, const NumiClothOracleAdjointProblem* problem,
+               NumiClothOracleGradients* out) {
+    using Vec = V<R>;
     if (scene == nullptr || configs == nullptr || problem == nullptr || out == nullptr) return 1;
     if (!supported(*scene)) return 5;
     Problem pr;
@@ -319,7 +337,7 @@ int numi_cloth_oracle_adjoint(const NumiClothHipSceneDesc* scene, const NumiClot
     pr.bendComplianceScale = problem->bendComplianceScale;
     pr.inverseMassScale = problem->inverseMassScale;
     pr.damping = problem->damping;
-    Model m(*scene, pr);
+    Model<R> m(*scene, pr);
     const bool gradients = out->distanceCompliance || out->bendCompliance || out->inverseMass || out->damping;
     const uint32_t T = pr.trackedCount;
     std::vector<std::vector<Vec>> lossGrad(gradients ? substeps : 0);
@@ -330,9 +348,9 @@ int numi_cloth_oracle_adjoint(const NumiClothHipSceneDesc* scene, const NumiClot
         if (gradients) lossGrad[s].resize(T);
         for (uint32_t t = 0; t < T; ++t) {
             const double* target = pr.targets + 3 * (size_t(s) * T + t);
-            const Vec r = m.x[pr.tracked[t]] - Vec{target[0], target[1], target[2]};
-            loss += dot(r, r);
-            if (gradients) lossGrad[s][t] = r * 2.0;
+            const Vec r = m.x[pr.tracked[t]] - Vec{R(target[0]), R(target[1]), R(target[2])};
+            loss += double(r.x) * double(r.x) + double(r.y) * double(r.y) + double(r.z) * double(r.z);
+            if (gradients) lossGrad[s][t] = Vec{R(2) * r.x, R(2) * r.y, R(2) * r.z};
         }
     }
     out->loss = loss;
@@ -346,58 +364,82 @@ int numi_cloth_oracle_adjoint(const NumiClothHipSceneDesc* scene, const NumiClot
 
     const uint32_t P = m.P;
     std::vector<Vec> gx(P), gv(P);
-    std::vector<double> gw(P, 0.0), gdamp(P, 0.0), gcd(m.D, 0.0), gcb(m.B, 0.0), gld(m.D), glb(m.B);
+    std::vector<R> gw(P, R(0)), gdamp(P, R(0)), gcd(m.D, R(0)), gcb(m.B, R(0)), gld(m.D), glb(m.B);
     m.record = true;
     for (uint32_t s = substeps; s-- > 0;) {
-        const SubstepRecord& rec = m.substeps[s];
-        const double dt = rec.dt;
+        const SubstepRecord<R>& rec = m.substeps[s];
+        const R dt = rec.dt;
         m.restore(rec);
         m.tape.clear();
         m.substep(configs[s], iterations, sweeps);
         for (uint32_t t = 0; t < T; ++t) gx[pr.tracked[t]] = gx[pr.tracked[t]] + lossGrad[s][t];
         // finalize: v = (x - prev) / dt, prev = x0
-        const double inverseDt = 1.0 / dt;
+        const R inverseDt = R(1) / dt;
         std::vector<Vec> gprev(P);
         for (uint32_t i = 0; i < P; ++i) {
             gx[i] = gx[i] + gv[i] * inverseDt;
             gprev[i] = gv[i] * -inverseDt;
         }
-        std::fill(gld.begin(), gld.end(), 0.0);
-        std::fill(glb.begin(), glb.end(), 0.0);
+        std::fill(gld.begin(), gld.end(), R(0));
+        std::fill(glb.begin(), glb.end(), R(0));
+        const R alphaScale = R(1) / (dt * dt);
         for (size_t k = m.tape.size(); k-- > 0;) {
-            const Entry& e = m.tape[k];
+            const Entry<R>& e = m.tape[k];
             if (e.kind == 2) {
                 strainAdjoint(e, m.maxLength[e.k], m.w[e.a], m.w[e.b], gx[e.a], gx[e.b], gw[e.a], gw[e.b]);
                 continue;
             }
             const bool bend = e.kind == 1;
-            const double compliance = bend ? m.bCompliance[e.k] : m.dCompliance[e.k];
-            const double rest = bend ? double(m.d.bends[e.k].material.x) : double(m.d.distances[e.k].material.x);
-            const double alpha = compliance / (dt * dt);
-            double galpha = 0.0;
-            double& gl = bend ? glb[e.k] : gld[e.k];
+            const R compliance = bend ? m.bCompliance[e.k] : m.dCompliance[e.k];
+            const R rest = bend ? R(m.d.bends[e.k].material.x) : R(m.d.distances[e.k].material.x);
+            const R alpha = compliance / (dt * dt);
+            R galpha = R(0);
+            R& gl = bend ? glb[e.k] : gld[e.k];
             projectionAdjoint(e, rest, alpha, m.w[e.a], m.w[e.b], gx[e.a], gx[e.b], gl, galpha, gw[e.a], gw[e.b]);
-            (bend ? gcb[e.k] : gcd[e.k]) += galpha / (dt * dt);
+            (bend ? gcb[e.k] : gcd[e.k]) += galpha * alphaScale;
         }
         // begin + advance: x = x0 + v' dt, v' = f v0 + g dt, prev = x0.
         for (uint32_t i = 0; i < P; ++i) {
-            if (m.w[i] > 0.0) {
-                const double f = m.factor(i);
+            if (m.w[i] > R(0)) {
+                const R f = m.factor(i);
                 const Vec gvp = gx[i] * dt;
                 gx[i] = gx[i] + gprev[i];
                 gv[i] = gvp * f;
-                if (m.damping[i] > 0.0) gdamp[i] += dot(gvp, rec.v0[i]) * (-m.dt0 * f * f);
+                if (m.damping[i] > R(0)) gdamp[i] += dot(gvp, rec.v0[i]) * (-R(m.dt0) * f * f);
             } else {
                 gx[i] = Vec{};
                 gv[i] = Vec{};
             }
         }
     }
-    if (out->distanceCompliance) std::memcpy(out->distanceCompliance, gcd.data(), m.D * sizeof(double));
-    if (out->bendCompliance) std::memcpy(out->bendCompliance, gcb.data(), m.B * sizeof(double));
-    if (out->inverseMass) std::memcpy(out->inverseMass, gw.data(), P * sizeof(double));
-    if (out->damping) std::memcpy(out->damping, gdamp.data(), P * sizeof(double));
+    auto copyOut = [](double* dst, const std::vector<R>& src) {
+        if (dst != nullptr)
+            for (size_t k = 0; k < src.size(); ++k) dst[k] = double(src[k]);
+    };
+    copyOut(out->distanceCompliance, gcd);
+    copyOut(out->bendCompliance, gcb);
+    copyOut(out->inverseMass, gw);
+    copyOut(out->damping, gdamp);
     return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Returns 0, 1 on a null argument, 5 when the scene has a family the
+// adjoint does not cover (contacts, grips, knots, fruits, air).
+int numi_cloth_oracle_adjoint(const NumiClothHipSceneDesc* scene, const NumiClothBagGPUConfig* configs,
+                              uint32_t substeps, uint32_t iterations, uint32_t sweeps,
+                              const NumiClothOracleAdjointProblem* problem, NumiClothOracleGradients* out) {
+    return runAdjoint<double>(scene, configs, substeps, iterations, sweeps, problem, out);
+}
+
+// The FP32 twin (parameter overrides must be NULL or exact in FP32).
+int numi_cloth_oracle_adjoint_fp32(const NumiClothHipSceneDesc* scene, const NumiClothBagGPUConfig* configs,
+                                   uint32_t substeps, uint32_t iterations, uint32_t sweeps,
+                                   const NumiClothOracleAdjointProblem* problem, NumiClothOracleGradients* out) {
+    return runAdjoint<float>(scene, configs, substeps, iterations, sweeps, problem, out);
 }
 
 }  // extern "C"

@@ -117,17 +117,20 @@ def _array_pointer(array):
 
 def adjoint(scene, configs: np.ndarray, iterations: int, strain_sweeps: int, tracked: np.ndarray,
             targets: np.ndarray, gradients: bool = True, distance_scale=None, bend_scale=None,
-            inverse_mass_scale=None, damping=None) -> dict:
+            inverse_mass_scale=None, damping=None, precision: str = "fp64") -> dict:
     """FP64 loss sum_s sum_k |x_k(s) - target_k(s)|^2 over the substeps and,
     with gradients=True, its gradient with respect to distance and bend
     compliance, inverse mass and damping coefficient (per record / knot).
     The *_scale / damping arrays override parameters in double."""
     lib = library()
     if not hasattr(lib, "_adjoint_ready"):
-        lib.numi_cloth_oracle_adjoint.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
-                                                  C.POINTER(AdjointProblem), C.POINTER(Gradients)]
-        lib.numi_cloth_oracle_adjoint.restype = C.c_int
+        for name in ("numi_cloth_oracle_adjoint", "numi_cloth_oracle_adjoint_fp32"):
+            fn = getattr(lib, name)
+            fn.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(AdjointProblem),
+                           C.POINTER(Gradients)]
+            fn.restype = C.c_int
         lib._adjoint_ready = True
+    entry = lib.numi_cloth_oracle_adjoint if precision == "fp64" else lib.numi_cloth_oracle_adjoint_fp32
     configs = np.ascontiguousarray(configs)
     tracked = np.ascontiguousarray(tracked, np.uint32)
     targets = np.ascontiguousarray(targets, np.float64)
@@ -142,8 +145,8 @@ def adjoint(scene, configs: np.ndarray, iterations: int, strain_sweeps: int, tra
     g = Gradients(0.0, *(out[name].ctypes.data if gradients and out[name].size else None
                          for name in ("distanceCompliance", "bendCompliance", "inverseMass", "damping")),
                   out["positions"].ctypes.data)
-    status = lib.numi_cloth_oracle_adjoint(C.addressof(scene.desc), configs.ctypes.data, len(configs), iterations,
-                                           strain_sweeps, C.byref(problem), C.byref(g))
+    status = entry(C.addressof(scene.desc), configs.ctypes.data, len(configs), iterations, strain_sweeps,
+                   C.byref(problem), C.byref(g))
     if status != 0:
         raise RuntimeError(f"oracle adjoint failed with status {status}")
     out["loss"] = g.loss

@@ -3,9 +3,15 @@
 The forward half runs the specimen kernels (one workgroup per specimen, all
 state in LDS); it must leave the batch exactly where the plain batch step
 leaves it. The gradients (FP32, checkpoints every 16 substeps, in-segment
-recomputation) are compared with the FP64 adjoint oracle, itself pinned by
-finite differences (tests/test_oracle_adjoint.py), and the strongest one
-with central differences of the FP64 loss at h = 1e-6.
+recomputation) are compared with the oracle's FP32 twin of the adjoint (same
+operation order: equal up to the order of per-segment partial sums), and
+with the FP64 adjoint (pinned by finite differences in
+tests/test_oracle_adjoint.py) as closely as FP32 allows: no farther than the
+FP32 twin is from FP64. Uniform mass scaling is a near-symmetry of XPBD
+(exact with compliance scaled alongside), so the density derivative is a
+relative sensitivity of ~1e-7, at FP32's resolution; the damping derivative
+is resolved, and is also checked against central differences of the FP64
+loss at h = 1e-6.
 """
 
 import numpy as np
@@ -30,7 +36,9 @@ def _gpu(cloth, p, checkpoint=16):
 @pytest.fixture(scope="module")
 def problem(cloth, oracle):
     p = adjoint_problem.build(cloth)
-    p["out"] = oracle.adjoint(p["merged"], p["configs"], p["iterations"], p["sweeps"], p["tracked"], p["targets"])
+    args = (p["merged"], p["configs"], p["iterations"], p["sweeps"], p["tracked"], p["targets"])
+    p["out"] = oracle.adjoint(*args)
+    p["twin"] = oracle.adjoint(*args, precision="fp32")
     p["gpu"], p["states"] = _gpu(cloth, p)
     return p
 
@@ -49,7 +57,8 @@ def test_specimen_forward_equals_plain_batch_step(cloth, problem):
 def test_loss_matches_oracle(problem):
     p = problem
     loss = sum(g["loss"] for g in p["gpu"])
-    print(f"loss gpu {loss:.9e} fp64 {p['out']['loss']:.9e}")
+    print(f"loss gpu {loss:.9e} twin {p['twin']['loss']:.9e} fp64 {p['out']['loss']:.9e}")
+    assert loss == pytest.approx(p["twin"]["loss"], rel=1e-9)
     assert loss == pytest.approx(p["out"]["loss"], rel=1e-4)
 
 
@@ -63,33 +72,45 @@ def _gpu_full(p, name):
     return out
 
 
-@pytest.mark.parametrize("name", ["distanceCompliance", "bendCompliance", "inverseMass", "damping"])
-def test_gradients_match_fp64_adjoint(problem, name):
-    p = problem
-    gpu = _gpu_full(p, name)
-    ref = p["out"][name]
-    error = np.linalg.norm(gpu - ref) / np.linalg.norm(ref)
-    print(f"{name}: relative L2 error {error:.3e}")
-    assert error < 1e-2
-    top = np.argsort(-np.abs(ref))[:5]
-    assert np.allclose(gpu[top], ref[top], rtol=1e-2)
+NAMES = ["distanceCompliance", "bendCompliance", "inverseMass", "damping"]
 
 
-def test_specimen_parameters_match_fp64_adjoint(problem):
+@pytest.mark.parametrize("name", NAMES)
+def test_gradients_match_fp32_twin(problem, name):
     p = problem
-    ref = _analytic(p)
+    gpu, twin = _gpu_full(p, name), p["twin"][name]
+    error = np.linalg.norm(gpu - twin) / np.linalg.norm(twin)
+    print(f"{name}: |gpu - twin| / |twin| = {error:.3e}")
+    assert error < 1e-4
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_gradients_as_close_to_fp64_as_fp32_allows(problem, name):
+    p = problem
+    gpu, twin, ref = _gpu_full(p, name), p["twin"][name], p["out"][name]
+    e_gpu = np.linalg.norm(gpu - ref) / np.linalg.norm(ref)
+    e_twin = np.linalg.norm(twin - ref) / np.linalg.norm(ref)
+    print(f"{name}: |gpu - fp64| {e_gpu:.3e}  |twin - fp64| {e_twin:.3e}")
+    assert e_gpu <= 1.5 * e_twin + 1e-4
+    assert e_gpu < 0.1
+
+
+def test_specimen_parameters(problem):
+    p = problem
     m = p["merged"]
     w = m.particles["positionAndInverseMass"][:, 3].astype(np.float64)
     damping = m.particleMotion[:, 3].astype(np.float64)
     gw, gc = _gpu_full(p, "inverseMass"), _gpu_full(p, "damping")
+    ref = _analytic(p)
+    twin = _analytic(dict(p, out=p["twin"]))
     for k in range(2):
         idx = p["maps"][k]["particles"]
         density = -np.sum(gw[idx] * w[idx])
         damp = np.sum(gc[idx] * damping[idx])
-        print(f"specimen {k}: density {density:.6e} vs {ref[f'density{k}']:.6e}, damping {damp:.6e} vs "
-              f"{ref[f'damping{k}']:.6e}")
-        assert density == pytest.approx(ref[f"density{k}"], rel=1e-3)
+        print(f"specimen {k}: density {density:.6e} twin {twin[f'density{k}']:.6e} fp64 {ref[f'density{k}']:.6e}; "
+              f"damping {damp:.6e} fp64 {ref[f'damping{k}']:.6e}")
         assert damp == pytest.approx(ref[f"damping{k}"], rel=1e-3)
+        assert density == pytest.approx(twin[f"density{k}"], rel=1e-3)
 
 
 def test_damping_gradient_matches_finite_differences(oracle, problem):

@@ -106,6 +106,24 @@ def test_gradients_match_finite_differences_at_1e6(cloth, oracle, problem):
     assert analytic["damping0"] == pytest.approx(_fd(oracle, p, 1e-6, damping=_damping(p["merged"], idx0)), rel=1e-3)
 
 
+def test_mass_compliance_scaling_identity(cloth, oracle, problem):
+    """Scaling every inverse mass and compliance of a specimen by s leaves
+    its trajectory unchanged (each projection is homogeneous of degree 0 in
+    (w, alpha) with lambda scaled by 1/s), so
+    sum_i w_i dL/dw_i = -sum_k c_k dL/dc_k over the specimen (Euler)."""
+    p = problem
+    m, out = p["merged"], p["out"]
+    w = m.particles["positionAndInverseMass"][:, 3].astype(np.float64)
+    cd = m.distances["material"][:, 1].astype(np.float64)
+    cb = m.bends["material"][:, 2].astype(np.float64)
+    for k in range(2):
+        idx, di, bi = p["maps"][k]["particles"], p["maps"][k]["distances"], p["maps"][k]["bends"]
+        mass = np.sum(out["inverseMass"][idx] * w[idx])
+        compliance = np.sum(out["distanceCompliance"][di] * cd[di]) + np.sum(out["bendCompliance"][bi] * cb[bi])
+        scale = np.sum(np.abs(out["inverseMass"][idx] * w[idx]))
+        assert abs(mass + compliance) <= 1e-12 * scale
+
+
 def test_unsupported_scene_is_rejected(cloth, oracle):
     scene = cloth.Scene.sheet(width=4, height=4)  # ground contact
     with pytest.raises(RuntimeError, match="status 5"):
