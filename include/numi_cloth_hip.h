@@ -144,8 +144,11 @@ typedef struct NumiClothHipDiagnostics {
     float maximumSelfPenetration;   /* over current self candidates */
     uint32_t selfCandidateCount;
     uint32_t selfCandidateOverflow;
-    uint32_t reserved0;
-    uint32_t reserved1;
+    /* Fruit/yarn solves since create that had to rerun visit by visit (the
+     * sparse schedule could not prove its skipped visits were no-ops), and
+     * the most candidate visits one solve staged. */
+    uint32_t yarnReruns;
+    uint32_t yarnCandidatesMax;
 } NumiClothHipDiagnostics;
 
 typedef struct NumiClothHipSolver NumiClothHipSolver;

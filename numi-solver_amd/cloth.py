@@ -102,7 +102,7 @@ class Diagnostics(C.Structure):
     _fields_ = [("maximumStrainViolation", C.c_float), ("maximumGroundPenetration", C.c_float),
                 ("maximumSpherePenetration", C.c_float), ("maximumSelfPenetration", C.c_float),
                 ("selfCandidateCount", C.c_uint32), ("selfCandidateOverflow", C.c_uint32),
-                ("reserved0", C.c_uint32), ("reserved1", C.c_uint32)]
+                ("yarnReruns", C.c_uint32), ("yarnCandidatesMax", C.c_uint32)]
 
 
 MATERIAL_FIELDS = ["ordinaryNodeMassKg", "hemNodeMassKg", "yarnRadiusM", "axialBodyComplianceMPerN",

@@ -31,6 +31,7 @@
 // touched knot then moves by the mean of its corrections. No float atomics.
 
 #include "device_math.hpp"
+#include "kernels_core.hpp"
 #include "kernels_scan.hpp"
 
 namespace numi_dev {
@@ -297,37 +298,132 @@ __device__ void forEntryCandidates(uint32_t k, const SelfGrid& g, const float4* 
     }
 }
 
-__global__ void k_self_pair_counts(const float4* __restrict__ boxLo, const float4* __restrict__ boxHi,
-                                   const uint2* __restrict__ ij, const SelfGrid* __restrict__ grid,
-                                   const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
-                                   const uint32_t* __restrict__ localStart, const uint32_t* __restrict__ localList,
-                                   uint32_t* __restrict__ counts) {
-    const SelfGrid g = *grid;
+// Candidate enumeration over the sorted entries with the boxes and segment
+// endpoints gathered into sorted order (the run of a cell is then read
+// sequentially) and the two-hop knot lists of both endpoints held in
+// registers (W slots each, padded with ~0u): the locality test of a pair is
+// 4W register compares instead of four binary searches in memory. Same set
+// and order of candidates as forEntryCandidates.
+template <uint32_t W, typename Emit>
+__device__ void forSortedEntryCandidates(uint32_t k, const SelfGrid& g, const float4* __restrict__ sortedLo,
+                                         const float4* __restrict__ sortedHi, const uint2* __restrict__ sortedIJ,
+                                         const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                                         const uint32_t* __restrict__ localFixed, Emit emit) {
+    const uint32_t n = g.sortCount;
+    const uint32_t id = keys[k];
+    const uint32_t s = vals[k];
+    const float4 lo = sortedLo[k], hi = sortedHi[k];
+    const uint2 es = sortedIJ[k];
+    const uint32_t x = id % g.nx;
+    const uint32_t y = (id / g.nx) % g.ny;
+    const uint32_t z = id / (g.nx * g.ny);
+    uint32_t na[W], nb[W];
+    bool loaded = false;
+    for (uint32_t j = k + 1; j < n && keys[j] == id; ++j) {
+        const float4 tLo = sortedLo[j], tHi = sortedHi[j];
+        if (!boxesOverlap(lo, hi, tLo, tHi)) continue;
+        if (cellOf(fmaxf(lo.x, tLo.x), g.origin.x, g.cell) != static_cast<int>(x) ||
+            cellOf(fmaxf(lo.y, tLo.y), g.origin.y, g.cell) != static_cast<int>(y) ||
+            cellOf(fmaxf(lo.z, tLo.z), g.origin.z, g.cell) != static_cast<int>(z)) {
+            continue;
+        }
+        if (!loaded) {
+            const uint4* ra = reinterpret_cast<const uint4*>(localFixed + static_cast<size_t>(es.x) * W);
+            const uint4* rb = reinterpret_cast<const uint4*>(localFixed + static_cast<size_t>(es.y) * W);
+#pragma unroll
+            for (uint32_t m = 0; m < W / 4; ++m) {
+                const uint4 va = ra[m], vb = rb[m];
+                na[4 * m] = va.x;
+                na[4 * m + 1] = va.y;
+                na[4 * m + 2] = va.z;
+                na[4 * m + 3] = va.w;
+                nb[4 * m] = vb.x;
+                nb[4 * m + 1] = vb.y;
+                nb[4 * m + 2] = vb.z;
+                nb[4 * m + 3] = vb.w;
+            }
+            loaded = true;
+        }
+        const uint2 et = sortedIJ[j];
+        bool local = false;
+#pragma unroll
+        for (uint32_t m = 0; m < W; ++m)
+            local |= na[m] == et.x || na[m] == et.y || nb[m] == et.x || nb[m] == et.y;
+        if (local) continue;
+        const uint32_t t = vals[j];
+        emit(min(s, t), max(s, t));
+    }
+}
+
+// Sorted-order copies of each entry's box and segment endpoints.
+__global__ void k_self_gather(const float4* __restrict__ boxLo, const float4* __restrict__ boxHi,
+                              const uint2* __restrict__ ij, const SelfGrid* __restrict__ grid,
+                              const uint32_t* __restrict__ vals, float4* __restrict__ sortedLo,
+                              float4* __restrict__ sortedHi, uint2* __restrict__ sortedIJ) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!grid->valid || k >= grid->sortCount) return;
+    const uint32_t s = vals[k];
+    sortedLo[k] = boxLo[s];
+    sortedHi[k] = boxHi[s];
+    sortedIJ[k] = ij[s];
+}
+
+struct PairArgs {
+    const float4* boxLo;
+    const float4* boxHi;
+    const uint2* ij;
+    const float4* sortedLo;
+    const float4* sortedHi;
+    const uint2* sortedIJ;
+    SelfGrid* grid;
+    const uint32_t* keys;
+    const uint32_t* vals;
+    const uint32_t* localStart;
+    const uint32_t* localList;
+    const uint32_t* localFixed;
+    uint32_t* counts;
+    const uint32_t* offsets;
+    const uint32_t* pairTotal;
+    uint32_t capacity;
+    uint2* pairs;
+    uint32_t* failure;
+};
+
+// W = 0: two-hop lists searched in memory (any mesh); W = 16 or 32: the
+// register form (meshes whose two-hop lists fit W slots).
+template <uint32_t W, typename Emit>
+__device__ __forceinline__ void entryCandidates(uint32_t k, const SelfGrid& g, const PairArgs& a, Emit emit) {
+    if constexpr (W == 0) {
+        forEntryCandidates(k, g, a.boxLo, a.boxHi, a.ij, a.keys, a.vals, a.localStart, a.localList, emit);
+    } else {
+        forSortedEntryCandidates<W>(k, g, a.sortedLo, a.sortedHi, a.sortedIJ, a.keys, a.vals, a.localFixed, emit);
+    }
+}
+
+template <uint32_t W>
+__global__ void k_self_pair_counts(PairArgs a) {
+    const SelfGrid g = *a.grid;
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (!g.valid || k >= g.sortCount) return;
     uint32_t n = 0;
-    forEntryCandidates(k, g, boxLo, boxHi, ij, keys, vals, localStart, localList, [&](uint32_t, uint32_t) { ++n; });
-    counts[k] = n;
+    entryCandidates<W>(k, g, a, [&](uint32_t, uint32_t) { ++n; });
+    a.counts[k] = n;
 }
 
-__global__ void k_self_emit_pairs(const float4* __restrict__ boxLo, const float4* __restrict__ boxHi,
-                                  const uint2* __restrict__ ij, SelfGrid* __restrict__ grid,
-                                  const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
-                                  const uint32_t* __restrict__ localStart, const uint32_t* __restrict__ localList,
-                                  const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ pairTotal,
-                                  uint32_t capacity, uint2* __restrict__ pairs, uint32_t* __restrict__ failure) {
-    const SelfGrid g = *grid;
+template <uint32_t W>
+__global__ void k_self_emit_pairs(PairArgs a) {
+    const SelfGrid g = *a.grid;
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k == 0) {
-        const uint32_t total = g.valid ? *pairTotal : 0u;
-        grid->pairTotal = total;
-        grid->pairCount = min(total, capacity);
-        if (total > capacity) recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_RANGE);
+        const uint32_t total = g.valid ? *a.pairTotal : 0u;
+        a.grid->pairTotal = total;
+        a.grid->pairCount = min(total, a.capacity);
+        if (total > a.capacity) recordFailure(a.failure, NUMI_CLOTH_BAG_GPU_FAILURE_RANGE);
     }
     if (!g.valid || k >= g.sortCount) return;
-    uint32_t out = offsets[k];
-    forEntryCandidates(k, g, boxLo, boxHi, ij, keys, vals, localStart, localList, [&](uint32_t a, uint32_t b) {
-        if (out < capacity) pairs[out] = make_uint2(a, b);
+    uint32_t out = a.offsets[k];
+    entryCandidates<W>(k, g, a, [&](uint32_t p, uint32_t q) {
+        if (out < a.capacity) a.pairs[out] = make_uint2(p, q);
         ++out;
     });
 }
@@ -513,25 +609,46 @@ __global__ void k_self_solve(const NumiClothBagGPUConfig* __restrict__ cfg, cons
 
 // Moves every touched knot by the mean of its fixed-point corrections and
 // clears its accumulator for the next call.
-__global__ void k_self_apply(const uint32_t* __restrict__ ok, float4* __restrict__ pos,
-                             FixedAccumulator* __restrict__ acc, uint32_t particles, uint32_t* __restrict__ failure) {
+// Mean of each touched knot's fixed-point corrections, then (collide = 1)
+// the ground/sphere pass that follows self-contact in an iteration: both act
+// on one knot at a time, so fusing them is exact.
+__global__ void k_self_apply(const NumiClothBagGPUConfig* __restrict__ cfg, const uint32_t* __restrict__ ok,
+                             float4* __restrict__ pos, FixedAccumulator* __restrict__ acc, uint32_t particles,
+                             const float4* __restrict__ spheres, uint32_t sphereCount, uint32_t collide,
+                             uint32_t* __restrict__ failure) {
+    extern __shared__ float4 sphereCache[];
+    if (collide != 0u) {
+        for (uint32_t k = threadIdx.x; k < sphereCount; k += blockDim.x) sphereCache[k] = spheres[k];
+        __syncthreads();
+    }
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= particles || *ok == 0u) return;
-    const uint32_t n = acc[i].count;
-    if (n == 0) return;
-    const FixedAccumulator a = acc[i];
-    const double scale = kFixedInverse / static_cast<double>(n);
     float4 p = pos[i];
-    const V3 d = v3(static_cast<float>(static_cast<double>(a.x) * scale),
-                    static_cast<float>(static_cast<double>(a.y) * scale),
-                    static_cast<float>(static_cast<double>(a.z) * scale));
-    const V3 np = xyz(p) + d;
-    if (!finite3(np)) {
-        recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
-    } else {
-        pos[i] = withXyz(p, np);
+    bool write = false;
+    const uint32_t n = acc[i].count;
+    if (n != 0) {
+        const FixedAccumulator a = acc[i];
+        const double scale = kFixedInverse / static_cast<double>(n);
+        const V3 d = v3(static_cast<float>(static_cast<double>(a.x) * scale),
+                        static_cast<float>(static_cast<double>(a.y) * scale),
+                        static_cast<float>(static_cast<double>(a.z) * scale));
+        const V3 np = xyz(p) + d;
+        if (!finite3(np)) {
+            recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
+        } else {
+            p = withXyz(p, np);
+            write = true;
+        }
+        acc[i] = FixedAccumulator{0, 0, 0, 0u, 0u};
     }
-    acc[i] = FixedAccumulator{0, 0, 0, 0u, 0u};
+    if (collide != 0u) {
+        if (!isfinite(p.z)) {
+            recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
+        } else if (collideKnot(p, cfg->constraintCounts.z != 0u, cfg->clothMaterial.x, sphereCache, sphereCount)) {
+            write = true;
+        }
+    }
+    if (write) pos[i] = p;
 }
 
 // Largest capsule overlap 2r - distance over the current candidates

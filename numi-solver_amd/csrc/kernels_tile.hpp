@@ -73,29 +73,8 @@ __global__ void k_tile_program(TileProgramArgs a) {
         for (uint32_t j = first + threadIdx.x; j < last; j += blockDim.x) {
             const uint4 item = a.items[j];
             const uint32_t la = item.x & 0xffffu, lb = item.x >> 16;
-            const uint32_t c = item.z & 0x7fffffffu;
-            const bool owner = (item.z >> 31) != 0u;
-            if (kind == kTileDistanceKind) {
-                float lam = a.distanceLambdaIn[c];
-                if (live) {
-                    float4 p = lds[la], q = lds[lb];
-                    if (projectDistance(p, q, a.distanceRestCompliance[c], lam, dt, a.failure)) {
-                        lds[la] = p;
-                        lds[lb] = q;
-                    }
-                }
-                if (owner) a.distanceLambdaOut[c] = lam;
-            } else if (kind == kTileBendKind) {
-                float lam = a.bendLambdaIn[c];
-                if (live) {
-                    float4 p = lds[la], q = lds[lb];
-                    if (projectBend(p, q, a.bendChordCompliance[c], lam, dt, ground, floorZ, a.failure)) {
-                        lds[la] = p;
-                        lds[lb] = q;
-                    }
-                }
-                if (owner) a.bendLambdaOut[c] = lam;
-            } else if (kind == kTileKnotKind) {
+            if (kind == kTileKnotKind) {
+                const uint32_t c = item.z & 0x7fffffffu;
                 float lam = a.knotLambdaIn[c];
                 if (live) {
                     const uint32_t lc = item.y & 0xffffu, ld = item.y >> 16;
@@ -107,10 +86,36 @@ __global__ void k_tile_program(TileProgramArgs a) {
                         lds[ld] = p3;
                     }
                 }
-                if (owner) a.knotLambdaOut[c] = lam;
+                if ((item.z >> 31) != 0u) a.knotLambdaOut[c] = lam;
+                continue;
+            }
+            const uint32_t c = item.y & 0x7fffffffu;
+            const bool owner = (item.y >> 31) != 0u;
+            if (kind == kTileDistanceKind) {
+                float lam = a.distanceLambdaIn[c];
+                if (live) {
+                    float4 p = lds[la], q = lds[lb];
+                    const float2 rc = make_float2(__uint_as_float(item.z), __uint_as_float(item.w));
+                    if (projectDistance(p, q, rc, lam, dt, a.failure)) {
+                        lds[la] = p;
+                        lds[lb] = q;
+                    }
+                }
+                if (owner) a.distanceLambdaOut[c] = lam;
+            } else if (kind == kTileBendKind) {
+                float lam = a.bendLambdaIn[c];
+                if (live) {
+                    float4 p = lds[la], q = lds[lb];
+                    const float2 cc = make_float2(__uint_as_float(item.z), __uint_as_float(item.w));
+                    if (projectBend(p, q, cc, lam, dt, ground, floorZ, a.failure)) {
+                        lds[la] = p;
+                        lds[lb] = q;
+                    }
+                }
+                if (owner) a.bendLambdaOut[c] = lam;
             } else if (live) {  // strain limit
                 float4 p = lds[la], q = lds[lb];
-                if (limitStrainPair(p, q, a.distanceMaxLength[c], a.failure)) {
+                if (limitStrainPair(p, q, __uint_as_float(item.z), a.failure)) {
                     lds[la] = p;
                     lds[lb] = q;
                 }

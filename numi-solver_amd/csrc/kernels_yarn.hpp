@@ -69,6 +69,7 @@ struct YarnArgs {
     NumiClothBagGPUYarnContact* contacts;
     NumiClothBagGPUFrictionStatus* status;  // friction only
     float margin;                        // present mode candidate margin
+    uint32_t* stats;                     // [reruns visit by visit, most candidates], or null
     uint32_t* failure;
 };
 
@@ -210,15 +211,19 @@ __device__ void yarnDense(const YarnArgs& a, float4* knots, float* moved) {
 }
 
 // One solve_yarn_batch call over all distance colours. One workgroup of
-// kYarnThreads; dynamic LDS: knots[P], moved[P], counts[D], slot arrays.
+// kYarnThreads; dynamic LDS: knots[P], moved[P], counts[D].
 __global__ void __launch_bounds__(kYarnThreads) k_yarn_solve(YarnArgs a) {
     extern __shared__ float4 yarnLds[];
     __shared__ uint32_t scanScratch[kYarnThreads];
-    __shared__ uint32_t slotPair[kYarnSlots];
+    __shared__ uint32_t candidates[kYarnSlots];  // pair indices, unordered
+    __shared__ uint32_t slotPair[kYarnSlots];     // pair indices in phase order
     __shared__ YarnStage slotStage[kYarnSlots];
+    __shared__ NumiClothBagGPUFruit fruitStart[32];
     __shared__ float fruitMoved[32];
     __shared__ mr_float4 fruitStaged[32];
-    __shared__ uint32_t verdict;  // 0 sparse result stands, 1 rerun visit by visit
+    __shared__ uint32_t candidateCount;
+    __shared__ uint32_t verdict;    // 0 sparse result stands, 1 rerun visit by visit
+    __shared__ uint32_t movedBits;  // largest knot travel (float bits, non-negative)
     if (*a.ok == 0u) return;
     const uint32_t tid = threadIdx.x, threads = blockDim.x;
     const uint32_t P = a.particles, D = a.segments, F = a.fruitCount;
@@ -230,25 +235,36 @@ __global__ void __launch_bounds__(kYarnThreads) k_yarn_solve(YarnArgs a) {
         moved[i] = 0.0f;
     }
     for (uint32_t i = tid; i < D; i += threads) counts[i] = 0u;
-    if (tid < 32) fruitMoved[tid] = 0.0f;
-    if (tid == 0) verdict = 0u;
+    if (tid < F) {
+        fruitStart[tid] = a.fruits[tid];
+        fruitMoved[tid] = 0.0f;
+    }
+    if (tid == 0) {
+        candidateCount = 0u;
+        verdict = 0u;
+        movedBits = 0u;
+    }
     __syncthreads();
-    // 1. count candidate visits per phase
+    // 1. candidate visits: counted per phase and listed (start state in LDS)
     const uint32_t pairs = F * D;
     for (uint32_t idx = tid; idx < pairs; idx += threads) {
         const uint32_t f = idx / D, s = idx % D;
-        if (a.segmentBatch[s].y != 0u && yarnCandidate(a, f, s, a.pos, a.fruits))
-            atomicAdd(&counts[phaseBit(f, s, a.segmentBatch[s])], 1u);
+        const uint2 batch = a.segmentBatch[s];
+        if (batch.y == 0u || !yarnCandidate(a, f, s, knots, fruitStart)) continue;
+        atomicAdd(&counts[phaseBit(f, s, batch)], 1u);
+        const uint32_t k = atomicAdd(&candidateCount, 1u);
+        if (k < kYarnSlots) candidates[k] = idx;
     }
     __syncthreads();
-    const uint32_t total = blockExclusiveScan(counts, D, scanScratch);
+    const uint32_t total = candidateCount;
+    blockExclusiveScan(counts, D, scanScratch);
     if (total > kYarnSlots) {
         if (tid == 0) verdict = 1u;
     } else {
-        // scatter: counts[b] becomes the end of phase b's group
-        for (uint32_t idx = tid; idx < pairs; idx += threads) {
+        // counts[b] becomes the end of phase b's group
+        for (uint32_t k = tid; k < total; k += threads) {
+            const uint32_t idx = candidates[k];
             const uint32_t f = idx / D, s = idx % D;
-            if (a.segmentBatch[s].y == 0u || !yarnCandidate(a, f, s, a.pos, a.fruits)) continue;
             const uint32_t slot = atomicAdd(&counts[phaseBit(f, s, a.segmentBatch[s])], 1u);
             slotPair[slot] = idx;
             const NumiClothBagGPUYarnContact& c = a.contacts[idx];
@@ -261,7 +277,7 @@ __global__ void __launch_bounds__(kYarnThreads) k_yarn_solve(YarnArgs a) {
         const uint32_t lane = tid;
         const bool live = lane < F;
         NumiClothBagGPUFruit fruit{};
-        if (live) fruit = a.fruits[lane];
+        if (live) fruit = fruitStart[lane];
         float travel = 0.0f;
         for (uint32_t batch = 0; batch < a.batchCount; ++batch) {
             const uint4 b = a.batches[batch];
@@ -293,22 +309,39 @@ __global__ void __launch_bounds__(kYarnThreads) k_yarn_solve(YarnArgs a) {
         }
     }
     __syncthreads();
-    // 3. present mode: every visit that was not run must have stayed a no-op
+    // 3. present mode: every visit that was not run must have stayed a no-op.
+    // Non-candidates started at least `margin` beyond contact; if no fruit
+    // plus no knot travelled that far in this call, none can have reached
+    // contact. Otherwise check each such visit's own start distance.
     if (verdict == 0u && a.mode == 0u) {
-        bool bad = false;
-        for (uint32_t idx = tid; idx < pairs; idx += threads) {
-            const uint32_t f = idx / D, s = idx % D;
-            if (a.segmentBatch[s].y == 0u || yarnCandidate(a, f, s, a.pos, a.fruits)) continue;
-            const uint2 e = a.ij[s];
-            const NumiClothBagGPUFruit fr = a.fruits[f];
-            const PointSample ps = samplePointSegment(xyz(fr.positionAndInverseMass), xyz(a.pos[e.x]), xyz(a.pos[e.y]));
-            const float target = fr.previousAndRadius.w + a.cfg->clothMaterial.x;
-            const float reach = fruitMoved[f] + fmaxf(moved[e.x], moved[e.y]);
-            if (!(ps.distance - reach >= target)) bad = true;
+        float local = 0.0f;
+        for (uint32_t i = tid; i < P; i += threads) local = fmaxf(local, moved[i]);
+        atomicMax(&movedBits, __float_as_uint(local));
+        __syncthreads();
+        float fruitMax = 0.0f;
+        for (uint32_t f = 0; f < F; ++f) fruitMax = fmaxf(fruitMax, fruitMoved[f]);
+        const bool cheap = fruitMax + __uint_as_float(movedBits) < a.margin;
+        if (!cheap) {
+            bool bad = false;
+            for (uint32_t idx = tid; idx < pairs; idx += threads) {
+                const uint32_t f = idx / D, s = idx % D;
+                if (a.segmentBatch[s].y == 0u || yarnCandidate(a, f, s, a.pos, fruitStart)) continue;
+                const uint2 e = a.ij[s];
+                const NumiClothBagGPUFruit fr = fruitStart[f];
+                const PointSample ps =
+                    samplePointSegment(xyz(fr.positionAndInverseMass), xyz(a.pos[e.x]), xyz(a.pos[e.y]));
+                const float target = fr.previousAndRadius.w + a.cfg->clothMaterial.x;
+                const float reach = fruitMoved[f] + fmaxf(moved[e.x], moved[e.y]);
+                if (!(ps.distance - reach >= target)) bad = true;
+            }
+            if (bad) verdict = 1u;
         }
-        if (bad) verdict = 1u;
     }
     __syncthreads();
+    if (tid == 0 && a.stats != nullptr) {
+        atomicAdd(&a.stats[0], verdict);
+        atomicMax(&a.stats[1], total);
+    }
     if (verdict == 0u) {
         for (uint32_t i = tid; i < P; i += threads) a.pos[i] = knots[i];
         for (uint32_t k = tid; k < total; k += threads) {
@@ -320,7 +353,7 @@ __global__ void __launch_bounds__(kYarnThreads) k_yarn_solve(YarnArgs a) {
             c.control.w = st.responses;
         }
         if (tid < F) {
-            NumiClothBagGPUFruit fr = a.fruits[tid];
+            NumiClothBagGPUFruit fr = fruitStart[tid];
             fr.positionAndInverseMass = fruitStaged[tid];
             if (!finite3(xyz(fr.positionAndInverseMass))) recordFailure(a.failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
             a.fruits[tid] = fr;

@@ -132,6 +132,7 @@ struct NumiClothHipSolver {
     std::vector<Range> pairColours;
     DeviceArray<uint2> pairColourRanges;
     DeviceArray<uint2> segmentBatch;  // per distance: (first, count) of its colour
+    DeviceArray<uint32_t> yarnStats;  // sparse yarn solve: reruns, most candidates
     std::vector<uint32_t> rimHost;
     DeviceArray<NumiClothBagGPUFruit> fruits;
     DeviceArray<NumiClothBagGPUFruitPair> fruitPairs;
@@ -144,6 +145,10 @@ struct NumiClothHipSolver {
     bool self = false;
     uint32_t entryCapacity = 0, pairCapacity = 0, entryTiles = 0, segmentBlocks = 0;
     DeviceArray<uint32_t> localStart, localList;
+    DeviceArray<uint32_t> localFixed;   // two-hop lists in localWidth slots per knot (0: not used)
+    uint32_t localWidth = 0;
+    DeviceArray<float4> sortedLo, sortedHi;
+    DeviceArray<uint2> sortedIJ;
     DeviceArray<float4> boxLo, boxHi, partialLo, partialHi;
     DeviceArray<SelfGrid> grid;
     DeviceArray<uint32_t> segmentCounts, entryCounts, scanSums, totals, constants;
@@ -294,6 +299,7 @@ struct NumiClothHipSolver {
         a.status = frictionStatus.ptr;
         // Present-mode candidates: within one yarn radius of contact.
         a.margin = initialConfig.clothMaterial.x;
+        a.stats = yarnStats.ptr;
         a.failure = failure.ptr;
         return a;
     }
@@ -367,22 +373,48 @@ struct NumiClothHipSolver {
             std::swap(valsIn, valsOut);
         }
         const uint32_t entryBlocks = (entryCapacity + kScanThreads - 1) / kScanThreads;
-        k_self_pair_counts<<<entryBlocks, kScanThreads, 0, stream>>>(boxLo.ptr, boxHi.ptr, dIJ.ptr, g, keysIn,
-                                                                     valsIn, localStart.ptr, localList.ptr,
-                                                                     entryCounts.ptr);
+        PairArgs pa{};
+        pa.boxLo = boxLo.ptr;
+        pa.boxHi = boxHi.ptr;
+        pa.ij = dIJ.ptr;
+        pa.sortedLo = sortedLo.ptr;
+        pa.sortedHi = sortedHi.ptr;
+        pa.sortedIJ = sortedIJ.ptr;
+        pa.grid = g;
+        pa.keys = keysIn;
+        pa.vals = valsIn;
+        pa.localStart = localStart.ptr;
+        pa.localList = localList.ptr;
+        pa.localFixed = localFixed.ptr;
+        pa.counts = entryCounts.ptr;
+        pa.offsets = entryCounts.ptr;
+        pa.pairTotal = totals.ptr + 1;
+        pa.capacity = pairCapacity;
+        pa.pairs = pairs.ptr;
+        pa.failure = failure.ptr;
+        if (localWidth != 0) {
+            k_self_gather<<<entryBlocks, kScanThreads, 0, stream>>>(boxLo.ptr, boxHi.ptr, dIJ.ptr, g, valsIn,
+                                                                   sortedLo.ptr, sortedHi.ptr, sortedIJ.ptr);
+        }
+        auto counts = localWidth == 16 ? k_self_pair_counts<16> : (localWidth == 32 ? k_self_pair_counts<32>
+                                                                                     : k_self_pair_counts<0>);
+        auto emit = localWidth == 16 ? k_self_emit_pairs<16> : (localWidth == 32 ? k_self_emit_pairs<32>
+                                                                                  : k_self_emit_pairs<0>);
+        counts<<<entryBlocks, kScanThreads, 0, stream>>>(pa);
         launchScan(entryCounts.ptr, &g->sortCount, entryCapacity, totals.ptr + 1);
-        k_self_emit_pairs<<<entryBlocks, kScanThreads, 0, stream>>>(boxLo.ptr, boxHi.ptr, dIJ.ptr, g, keysIn,
-                                                                    valsIn, localStart.ptr, localList.ptr,
-                                                                    entryCounts.ptr, totals.ptr + 1, pairCapacity,
-                                                                    pairs.ptr, failure.ptr);
+        emit<<<entryBlocks, kScanThreads, 0, stream>>>(pa);
     }
 
-    void launchSelf(uint32_t mode) {
+    // collide: also run the ground/sphere pass that follows (fused, exact).
+    void launchSelf(uint32_t mode, bool collide = false) {
         if (!self) return;
         const uint32_t blocks = std::min<uint32_t>(2048, std::max(1u, (pairCapacity + kBlock - 1) / kBlock));
         k_self_solve<<<blocks, kBlock, 0, stream>>>(cfg(), ok(), mode, grid.ptr, pairs.ptr, dIJ.ptr, posCur,
                                                      prev.ptr, accumulator.ptr, selfStatus.ptr);
-        k_self_apply<<<blocksFor(P), kBlock, 0, stream>>>(ok(), posCur, accumulator.ptr, P, failure.ptr);
+        const uint32_t n = collide ? extension.staticSphereCount : 0u;
+        k_self_apply<<<blocksFor(P), kBlock, n * sizeof(float4), stream>>>(cfg(), ok(), posCur, accumulator.ptr, P,
+                                                                         spheres.ptr, n, collide ? 1u : 0u,
+                                                                         failure.ptr);
     }
 
     void launchTileProgram(TileProgram& t, uint32_t trailingCollides, const NumiClothBagGPUConfig* config,
@@ -495,10 +527,16 @@ struct NumiClothHipSolver {
         }
         launchBuildYarnContacts();
         launchYarn(1);
+        // Nothing between the constraint passes and the ground/sphere pass
+        // of an iteration: the pass joins the tile program (or the
+        // self-contact apply when that runs in between).
+        const bool colliders = groundOrColliders();
+        const bool trailing = colliders && !self && G == 0 && F == 0;
         for (uint32_t it = 0; it < iterations; ++it) {
             TileProgram& program = it % 2 == 0 ? tileEven : tileOdd;
-            if (useTiles && program.valid) {
-                launchTileProgram(program, 0, cfg(), ok());
+            const bool tiled = useTiles && program.valid;
+            if (tiled) {
+                launchTileProgram(program, trailing ? 1u : 0u, cfg(), ok());
             } else {
                 launchDistances();
                 launchKnots();
@@ -507,8 +545,11 @@ struct NumiClothHipSolver {
             launchGrips();
             launchFruitPairs();
             launchYarn(0);
-            launchSelf(0);
-            launchCollide();
+            if (self) {
+                launchSelf(0, colliders);
+            } else if (!(tiled && trailing)) {
+                launchCollide();
+            }
             launchFruitGround();
         }
         launchStrainAndCollide(sweeps, 0);
@@ -871,6 +912,8 @@ void createSolver(const NumiClothHipSceneDesc& scene, NumiClothHipSolver& s) {
             s.distanceBatchesDevice.upload(batches.data(), batches.size(), st);
             s.segmentBatch.allocate(s.D);
             s.segmentBatch.upload(segmentBatch.data(), s.D, st);
+            s.yarnStats.allocate(2);
+            HIP_OK(hipMemsetAsync(s.yarnStats.ptr, 0, 2 * sizeof(uint32_t), st));
         }
         if (scene.mouthRimCount != 0) {
             if (scene.mouthRimParticles == nullptr) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "null mouth rim");
@@ -968,6 +1011,16 @@ void setupSelfContact(const NumiClothHipSceneDesc& scene, NumiClothHipSolver& s)
     s.localStart.upload(start.data(), start.size(), st);
     s.localList.allocate(list.size());
     s.localList.upload(list.data(), list.size(), st);
+    uint32_t longest = 0;
+    for (uint32_t i = 0; i < s.P; ++i) longest = std::max(longest, start[i + 1] - start[i]);
+    s.localWidth = longest <= 16 ? 16u : (longest <= 32 ? 32u : 0u);
+    if (s.localWidth != 0) {
+        std::vector<uint32_t> fixed(static_cast<size_t>(s.P) * s.localWidth, ~0u);
+        for (uint32_t i = 0; i < s.P; ++i)
+            for (uint32_t k = start[i]; k < start[i + 1]; ++k) fixed[static_cast<size_t>(i) * s.localWidth + k - start[i]] = list[k];
+        s.localFixed.allocate(fixed.size());
+        s.localFixed.upload(fixed.data(), fixed.size(), st);
+    }
     const uint64_t entries = static_cast<uint64_t>(s.D) * 16u;
     const uint64_t pairCapacity = scene.extension.selfCandidateCapacity != 0
                                       ? scene.extension.selfCandidateCapacity
@@ -994,6 +1047,11 @@ void setupSelfContact(const NumiClothHipSceneDesc& scene, NumiClothHipSolver& s)
     s.constants.upload(&histLength, 1, st);
     s.keysA.allocate(s.entryCapacity);
     s.valsA.allocate(s.entryCapacity);
+    if (s.localWidth != 0) {
+        s.sortedLo.allocate(s.entryCapacity);
+        s.sortedHi.allocate(s.entryCapacity);
+        s.sortedIJ.allocate(s.entryCapacity);
+    }
     s.keysB.allocate(s.entryCapacity);
     s.valsB.allocate(s.entryCapacity);
     s.histogram.allocate(histLength);
@@ -1006,8 +1064,27 @@ void setupSelfContact(const NumiClothHipSceneDesc& scene, NumiClothHipSolver& s)
 
 // ---- tile programs -------------------------------------------------------
 
+struct TileConstants {
+    std::vector<float> distanceRest, distanceCompliance, distanceMaxLength, bendRest, bendCompliance;
+};
+
 TileTopology tileTopology(const NumiClothHipSolver& s, std::vector<uint32_t>& dPairs, std::vector<uint32_t>& bPairs,
-                          std::vector<uint32_t>& kQuads) {
+                          std::vector<uint32_t>& kQuads, TileConstants& k) {
+    k.distanceRest.resize(s.D);
+    k.distanceCompliance.resize(s.D);
+    k.distanceMaxLength.resize(s.D);
+    for (uint32_t c = 0; c < s.D; ++c) {
+        const mr_float4 m = s.hostDistances[c].material;
+        k.distanceRest[c] = m.x;
+        k.distanceCompliance[c] = m.y;
+        k.distanceMaxLength[c] = m.x * (1.0f + m.w);  // the value uploaded as dMaxLength
+    }
+    k.bendRest.resize(s.B);
+    k.bendCompliance.resize(s.B);
+    for (uint32_t c = 0; c < s.B; ++c) {
+        k.bendRest[c] = s.hostBends[c].material.x;
+        k.bendCompliance[c] = s.hostBends[c].material.z;
+    }
     dPairs.resize(2 * s.D);
     for (uint32_t k = 0; k < s.D; ++k) {
         dPairs[2 * k] = s.hostDistances[k].particlesAndColor.x;
@@ -1032,6 +1109,11 @@ TileTopology tileTopology(const NumiClothHipSolver& s, std::vector<uint32_t>& dP
     t.bendPairs = bPairs.data();
     t.knotQuads = kQuads.data();
     t.positions = reinterpret_cast<const float*>(s.initialPositions.data());
+    t.distanceRest = k.distanceRest.data();
+    t.distanceCompliance = k.distanceCompliance.data();
+    t.distanceMaxLength = k.distanceMaxLength.data();
+    t.bendRest = k.bendRest.data();
+    t.bendCompliance = k.bendCompliance.data();
     return t;
 }
 
@@ -1057,7 +1139,8 @@ bool buildTileProgram(NumiClothHipSolver& s, const std::vector<TilePass>& passes
     out = NumiClothHipSolver::TileProgram{};
     if (passes.empty() || passes.size() > kMaxTilePasses) return false;
     std::vector<uint32_t> dPairs, bPairs, kQuads;
-    const TileTopology topo = tileTopology(s, dPairs, bPairs, kQuads);
+    TileConstants constants;
+    const TileTopology topo = tileTopology(s, dPairs, bPairs, kQuads, constants);
     TilePlan plan;
     if (!planTiles(topo, passes, owned, s.tileCapacity, 32, &plan)) return false;
     out.valid = true;
@@ -1317,6 +1400,13 @@ void diagnoseSolver(NumiClothHipSolver& s, NumiClothHipDiagnostics* out) {
     out->maximumGroundPenetration = s.initialConfig.constraintCounts.z != 0u ? r.y : 0.0f;
     out->maximumSpherePenetration = r.z;
     out->maximumSelfPenetration = r.w;
+    if (s.yarnStats.count != 0) {
+        uint32_t stats[2] = {0u, 0u};
+        s.yarnStats.download(stats, 2, s.stream);
+        HIP_OK(hipStreamSynchronize(s.stream));
+        out->yarnReruns = stats[0];
+        out->yarnCandidatesMax = stats[1];
+    }
     if (s.self) {
         SelfGrid g{};
         s.grid.download(&g, 1, s.stream);

@@ -25,6 +25,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <cstring>
 #include <vector>
 
 namespace numi_hip {
@@ -50,6 +51,12 @@ struct TileTopology {
     const uint32_t* bendPairs = nullptr;      // 2 per bend (first, third)
     const uint32_t* knotQuads = nullptr;      // 4 per knot
     const float* positions = nullptr;         // 4 per knot (xyz w) for ordering
+    // Constants packed into the items (the device arrays' exact values).
+    const float* distanceRest = nullptr;
+    const float* distanceCompliance = nullptr;
+    const float* distanceMaxLength = nullptr;
+    const float* bendRest = nullptr;
+    const float* bendCompliance = nullptr;
 };
 
 struct TilePlan {
@@ -61,8 +68,11 @@ struct TilePlan {
     std::vector<uint32_t> loadIds;      // owned first, then halo
     std::vector<uint32_t> ownedCount;   // tiles
     std::vector<uint32_t> itemStart;    // tiles * passes + 1
-    // One 16-byte item per computed constraint: x = local a | local b << 16,
-    // y = local c | local d << 16 (knots), z = constraint | owner << 31, w = 0.
+    // One 16-byte item per computed constraint, x = local a | local b << 16:
+    //   distance y = constraint | owner << 31, z = rest, w = compliance (bits)
+    //   bend     y = constraint | owner << 31, z = rest chord, w = compliance
+    //   strain   y = constraint, z = rest * (1 + limit), w = 0
+    //   knot     y = local c | local d << 16, z = constraint | owner << 31, w = 0
     std::vector<uint32_t> items;        // 4 per item
     uint64_t computed = 0;              // items over all tiles
     uint64_t unique = 0;                // constraint-passes in the program
@@ -240,10 +250,30 @@ inline bool planTiles(const TileTopology& topo, const std::vector<TilePass>& pro
                                            static_cast<uint32_t>(localOf[ids[1]]),
                                            n > 2 ? static_cast<uint32_t>(localOf[ids[2]]) : 0u,
                                            n > 3 ? static_cast<uint32_t>(localOf[ids[3]]) : 0u};
+                    const uint32_t tagged = c | (owner ? 0x80000000u : 0u);
+                    auto bits = [](float v) {
+                        uint32_t u;
+                        std::memcpy(&u, &v, sizeof(u));
+                        return u;
+                    };
                     tileItems.push_back(l[0] | (l[1] << 16));
-                    tileItems.push_back(l[2] | (l[3] << 16));
-                    tileItems.push_back(c | (owner ? 0x80000000u : 0u));
-                    tileItems.push_back(0u);
+                    if (pass.kind == kTileKnot) {
+                        tileItems.push_back(l[2] | (l[3] << 16));
+                        tileItems.push_back(tagged);
+                        tileItems.push_back(0u);
+                    } else if (pass.kind == kTileDistance) {
+                        tileItems.push_back(tagged);
+                        tileItems.push_back(bits(topo.distanceRest[c]));
+                        tileItems.push_back(bits(topo.distanceCompliance[c]));
+                    } else if (pass.kind == kTileBend) {
+                        tileItems.push_back(tagged);
+                        tileItems.push_back(bits(topo.bendRest[c]));
+                        tileItems.push_back(bits(topo.bendCompliance[c]));
+                    } else {
+                        tileItems.push_back(c);
+                        tileItems.push_back(bits(topo.distanceMaxLength[c]));
+                        tileItems.push_back(0u);
+                    }
                 }
             }
             bool ownedExact = ownersExact;
