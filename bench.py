@@ -217,7 +217,20 @@ def cpu_baseline_adjoint(specimens, local, targets, configs) -> dict:
 
 # -------------------------------------------------------------- roofline --
 
-def roofline(cloth, solver) -> tuple[dict, dict]:
+def measured_traffic(workload: str):
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload (two counter
+    runs, each scaled by a copy of known size); None when there is none."""
+    path = REPO / "profiles" / f"traffic_{workload}.txt"
+    if not path.exists():
+        return None
+    for text in path.read_text().splitlines():
+        if text.startswith("traffic_bytes_per_launch:"):
+            return int(text.split(":")[1])
+    return None
+
+
+def roofline(cloth, solver, workload: str) -> tuple[dict, dict]:
     """Dominant kernel, timed with HIP events on the solver's stream after the
     timed region (same state, same launch shape): the LDS tile program of an
     even iteration when the solver runs tiles, else one distance colour."""
@@ -230,7 +243,7 @@ def roofline(cloth, solver) -> tuple[dict, dict]:
         kernel = "k_distance (one colour batch)"
     achieved = timing.algorithmicBytes / (timing.averageMilliseconds * 1.0e-3) / 1.0e9
     line = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBPS, "unit": "GB/s",
-            "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": None, "kernel": kernel,
+            "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": measured_traffic(workload), "kernel": kernel,
             "bytes_per_launch": timing.algorithmicBytes,
             "avg_launch_us": round(timing.averageMilliseconds * 1000.0, 3),
             "items_per_launch": timing.itemsPerLaunch}
@@ -276,7 +289,7 @@ def run_plain(args, cloth, hip) -> dict:
     if state["failure"] != 0:
         raise RuntimeError(f"device failure bits {state['failure']}")
     diagnostics = diagnostics_of(solver, state)
-    line, tile_config = roofline(cloth, solver)
+    line, tile_config = roofline(cloth, solver, args.workload)
     released = int(state["releaseStatus"]["masks"][0, 1])
     solver.close()
     baseline = None
@@ -320,7 +333,7 @@ def run_strips(args, cloth, hip) -> dict:
     if failure != 0:
         raise RuntimeError(f"device failure bits {failure}")
     view = cloth.Solver.borrow(solver_handle, specimens[0])
-    line, tile_config = roofline(cloth, view)
+    line, tile_config = roofline(cloth, view, args.workload)
     batch.close()
     batched = len(specimens) * args.steps / seconds
 
@@ -366,7 +379,7 @@ def run_adjoint(args, cloth, hip) -> dict:
         forward.step(configs, ITERATIONS, STRAIN_SWEEPS)
     forward_seconds = timer.end() / args.steps
     view = cloth.Solver.borrow(forward.solver_handle, specimens[0])
-    line, tile_config = roofline(cloth, view)
+    line, tile_config = roofline(cloth, view, args.workload)
     forward.close()
     # Adjoint: forward with checkpoints, then the reverse; the library call
     # returns when the gradients are in device memory.
@@ -406,6 +419,10 @@ def main() -> int:
     parser.add_argument("--cpu-substeps", type=int, default=None,
                         help="substeps of the CPU FP64 baseline sample (default: sized to ~15 s)")
     parser.add_argument("--no-cpu-baseline", action="store_true")
+    parser.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                        help="after the workload, copy a MIB-MiB float32 buffer three times with an elementwise "
+                             "kernel (reads and writes MIB MiB each): the reference dispatch that scales the "
+                             "FETCH_SIZE / WRITE_SIZE counters of a profiled run to bytes")
     args = parser.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -427,8 +444,20 @@ def main() -> int:
             "cpu_baseline": result["cpu_baseline"]}
     if "diagnostics" in result:
         line["diagnostics"] = result["diagnostics"]
+    if args.copy_calibration > 0:
+        copy_calibration(args.copy_calibration)
     print(json.dumps(line))
     return 0
+
+
+def copy_calibration(mib: int) -> None:
+    import torch
+    n = mib * (1 << 20) // 4
+    source = torch.ones(n, dtype=torch.float32, device="cuda")
+    target = torch.empty_like(source)
+    for _ in range(3):
+        torch.mul(source, 1.0, out=target)
+    torch.cuda.synchronize()
 
 
 if __name__ == "__main__":

@@ -498,20 +498,168 @@ __device__ __forceinline__ bool collideKnot(float4& p, bool ground, float r, con
     return changed;
 }
 
+// ------------------------------------------------------ sphere windows --
+//
+// The spheres that can act on the knots of one workgroup. In a pass a knot
+// is tested against the spheres in their list order at a sequence of
+// positions (the start, then each push). The positions start inside the
+// workgroup's box B; while they stay inside B grown by pad, a sphere whose
+// centre lies more than reach + 2 pad outside B on some axis is farther than
+// reach + pad from every one of them along that axis (pad also covers the
+// rounding of the coordinate differences), so its test fails each time and
+// leaving it out changes nothing. The window keeps the other spheres in
+// list order; a knot that leaves B + pad is redone with the whole list.
+// The result is therefore the same bytes as collideKnot over all spheres.
+struct SphereWindow {
+    float4 lo, hi;  // B grown by pad
+    uint32_t count;
+};
+
+constexpr uint32_t kMaxBlockWaves = 16;
+
+// Every thread of the workgroup calls this with the box of its own knots
+// (+inf / -inf when it has none). culled must hold sphereCount entries;
+// scratch holds 2 * kMaxBlockWaves. Ends with a workgroup barrier.
+__device__ void buildSphereWindow(V3 lo, V3 hi, float radius, const float4* __restrict__ spheres,
+                                  uint32_t sphereCount, float4* culled, SphereWindow& w, float4* scratch) {
+    for (int offset = 32; offset > 0; offset >>= 1) {
+        lo.x = fminf(lo.x, __shfl_xor(lo.x, offset));
+        lo.y = fminf(lo.y, __shfl_xor(lo.y, offset));
+        lo.z = fminf(lo.z, __shfl_xor(lo.z, offset));
+        hi.x = fmaxf(hi.x, __shfl_xor(hi.x, offset));
+        hi.y = fmaxf(hi.y, __shfl_xor(hi.y, offset));
+        hi.z = fmaxf(hi.z, __shfl_xor(hi.z, offset));
+    }
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        scratch[wave] = make_float4(lo.x, lo.y, lo.z, 0.0f);
+        scratch[kMaxBlockWaves + wave] = make_float4(hi.x, hi.y, hi.z, 0.0f);
+    }
+    __syncthreads();
+    if (wave == 0) {
+        const uint32_t waves = blockDim.x >> 6;
+        float4 l = lane < waves ? scratch[lane] : make_float4(INFINITY, INFINITY, INFINITY, 0.0f);
+        float4 h = lane < waves ? scratch[kMaxBlockWaves + lane] : make_float4(-INFINITY, -INFINITY, -INFINITY, 0.0f);
+        float reach = 0.0f;
+        for (uint32_t k = lane; k < sphereCount; k += 64) reach = fmaxf(reach, spheres[k].w + radius);
+        for (int offset = 32; offset > 0; offset >>= 1) {
+            l.x = fminf(l.x, __shfl_xor(l.x, offset));
+            l.y = fminf(l.y, __shfl_xor(l.y, offset));
+            l.z = fminf(l.z, __shfl_xor(l.z, offset));
+            h.x = fmaxf(h.x, __shfl_xor(h.x, offset));
+            h.y = fmaxf(h.y, __shfl_xor(h.y, offset));
+            h.z = fmaxf(h.z, __shfl_xor(h.z, offset));
+            reach = fmaxf(reach, __shfl_xor(reach, offset));
+        }
+        const float extent = fmaxf(fmaxf(fmaxf(fabsf(l.x), fabsf(l.y)), fmaxf(fabsf(l.z), fabsf(h.x))),
+                                   fmaxf(fabsf(h.y), fabsf(h.z)));
+        const float pad = 0.25f * reach + 1.0e-5f * extent;
+        // An empty or non-finite box keeps every sphere and an unbounded window.
+        const bool bounded = isfinite(pad) && l.x <= h.x && l.y <= h.y && l.z <= h.z;
+        uint32_t count = 0;
+        for (uint32_t base = 0; base < sphereCount; base += 64) {
+            const uint32_t k = base + lane;
+            bool keep = false;
+            float4 c = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (k < sphereCount) {
+                c = spheres[k];
+                const float g = c.w + radius + 2.0f * pad;
+                keep = !bounded || (c.x >= l.x - g && c.x <= h.x + g && c.y >= l.y - g && c.y <= h.y + g &&
+                                    c.z >= l.z - g && c.z <= h.z + g);
+            }
+            const unsigned long long ballot = __ballot(keep);
+            const uint32_t before = static_cast<uint32_t>(__popcll(ballot & ((1ull << lane) - 1ull)));
+            if (keep) culled[count + before] = c;
+            count += static_cast<uint32_t>(__popcll(ballot));
+        }
+        if (lane == 0) {
+            w.count = count;
+            if (bounded) {
+                w.lo = make_float4(l.x - pad, l.y - pad, l.z - pad, 0.0f);
+                w.hi = make_float4(h.x + pad, h.y + pad, h.z + pad, 0.0f);
+            } else {
+                w.lo = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.0f);
+                w.hi = make_float4(INFINITY, INFINITY, INFINITY, 0.0f);
+            }
+        }
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ bool insideWindow(const float4& p, const SphereWindow& w) {
+    return p.x >= w.lo.x && p.x <= w.hi.x && p.y >= w.lo.y && p.y <= w.hi.y && p.z >= w.lo.z && p.z <= w.hi.z;
+}
+
+// collideKnot with the window's spheres; the same result as collideKnot
+// over all spheres (see SphereWindow).
+__device__ __forceinline__ bool collideKnotWindow(float4& p, bool ground, float r, const SphereWindow& w,
+                                                  const float4* culled, const float4* spheres,
+                                                  uint32_t sphereCount) {
+    const float4 start = p;
+    bool changed = false;
+    if (ground && !(p.z >= r)) {
+        p.z = fmaxf(p.z, r);
+        changed = true;
+    }
+    if (p.w > 0.0f) {
+        bool inside = insideWindow(p, w);
+        for (uint32_t k = 0; k < w.count; ++k) {
+            const float4 s = culled[k];
+            const V3 d = xyz(p) - xyz(s);
+            const float dist = length(d);
+            const float reach = s.w + r;
+            if (dist < reach && dist > 1.0e-12f) {
+                p = withXyz(p, xyz(s) + d * (reach / dist));
+                changed = true;
+                inside = inside && insideWindow(p, w);
+            }
+        }
+        if (!inside) {
+            p = start;
+            return collideKnot(p, ground, r, spheres, sphereCount);
+        }
+    }
+    return changed;
+}
+
+// Box of a thread's knot for buildSphereWindow (identity when it has none).
+__device__ __forceinline__ void knotBox(const float4& p, bool has, V3& lo, V3& hi) {
+    lo = has ? xyz(p) : v3(INFINITY, INFINITY, INFINITY);
+    hi = has ? xyz(p) : v3(-INFINITY, -INFINITY, -INFINITY);
+}
+
+// Ground and static spheres for every knot. Dynamic shared memory: the
+// spheres, then the window's list (2 * sphereCount float4).
 __global__ void k_collide(const NumiClothBagGPUConfig* __restrict__ cfg, const uint32_t* __restrict__ ok,
                           float4* __restrict__ pos, uint32_t particles, const float4* __restrict__ spheres,
                           uint32_t sphereCount, uint32_t* __restrict__ failure) {
     extern __shared__ float4 sphereCache[];
+    __shared__ SphereWindow window;
+    __shared__ float4 scratch[2 * kMaxBlockWaves];
+    float4* culled = sphereCache + sphereCount;
     for (uint32_t k = threadIdx.x; k < sphereCount; k += blockDim.x) sphereCache[k] = spheres[k];
-    __syncthreads();
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= particles || *ok == 0u) return;
-    float4 p = pos[i];
+    const bool live = *ok != 0u;
+    const bool has = live && i < particles;
+    const float4 p0 = has ? pos[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float r = cfg->clothMaterial.x;
+    if (sphereCount != 0) {
+        V3 lo, hi;
+        knotBox(p0, has && isfinite(p0.z), lo, hi);
+        buildSphereWindow(lo, hi, r, spheres, sphereCount, culled, window, scratch);
+    } else {
+        __syncthreads();
+    }
+    if (!has) return;
+    float4 p = p0;
     if (!isfinite(p.z)) {
         recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
         return;
     }
-    if (collideKnot(p, cfg->constraintCounts.z != 0u, cfg->clothMaterial.x, sphereCache, sphereCount)) pos[i] = p;
+    const bool ground = cfg->constraintCounts.z != 0u;
+    const bool moved = sphereCount != 0 ? collideKnotWindow(p, ground, r, window, culled, sphereCache, sphereCount)
+                                        : collideKnot(p, ground, r, sphereCache, 0u);
+    if (moved) pos[i] = p;
 }
 
 // ------------------------------------------------------------- finalize --

@@ -24,6 +24,12 @@
 // through count / exclusive scan / emit passes, so the lists are identical
 // on every run.
 //
+// The owner test needs no arithmetic: cell(v) = floor((v - origin) / h) is
+// non-decreasing in v, so cell(max(lo_s, lo_t)) = max(cell(lo_s), cell(lo_t))
+// per axis, and since both boxes cover the entry's cell c, that maximum is
+// c exactly when cell(lo_s) = c or cell(lo_t) = c on the axis. Each entry
+// carries those three "my box starts here" bits above the segment index.
+//
 // Narrow phase (reference solve_self_contact arithmetic, present and swept
 // modes) runs on all candidates in parallel from one position snapshot
 // (Jacobi). Each endpoint correction is converted to 2^-32 m fixed point and
@@ -37,6 +43,8 @@
 namespace numi_dev {
 
 constexpr uint32_t kMaxCellsPerSegment = 512;
+constexpr uint32_t kEntrySegmentBits = 29;               // entry value = segment | start flags << 29
+constexpr uint32_t kEntrySegmentMask = (1u << kEntrySegmentBits) - 1u;
 constexpr double kFixedScale = 4294967296.0;          // 2^32 per metre
 constexpr double kFixedInverse = 1.0 / 4294967296.0;
 
@@ -231,8 +239,10 @@ __global__ void k_self_emit_entries(const float4* __restrict__ boxLo, const floa
         for (int y = r.lo[1]; y <= r.hi[1]; ++y)
             for (int x = r.lo[0]; x <= r.hi[0]; ++x, ++out)
                 if (out < capacity) {
+                    const uint32_t starts = (x == r.lo[0] ? 1u : 0u) | (y == r.lo[1] ? 2u : 0u) |
+                                            (z == r.lo[2] ? 4u : 0u);
                     keys[out] = cellId(x, y, z, g);
-                    vals[out] = s;
+                    vals[out] = s | (starts << kEntrySegmentBits);
                 }
 }
 
@@ -278,21 +288,16 @@ __device__ void forEntryCandidates(uint32_t k, const SelfGrid& g, const float4* 
                                    Emit emit) {
     const uint32_t n = g.sortCount;
     const uint32_t id = keys[k];
-    const uint32_t s = vals[k];
+    const uint32_t vs = vals[k];
+    const uint32_t s = vs & kEntrySegmentMask;
     const float4 lo = boxLo[s], hi = boxHi[s];
     const uint2 es = ij[s];
-    const uint32_t x = id % g.nx;
-    const uint32_t y = (id / g.nx) % g.ny;
-    const uint32_t z = id / (g.nx * g.ny);
     for (uint32_t j = k + 1; j < n && keys[j] == id; ++j) {
-        const uint32_t t = vals[j];
+        const uint32_t vt = vals[j];
+        if (((vs | vt) >> kEntrySegmentBits) != 7u) continue;  // not the owner cell
+        const uint32_t t = vt & kEntrySegmentMask;
         const float4 tLo = boxLo[t], tHi = boxHi[t];
         if (!boxesOverlap(lo, hi, tLo, tHi)) continue;
-        if (cellOf(fmaxf(lo.x, tLo.x), g.origin.x, g.cell) != static_cast<int>(x) ||
-            cellOf(fmaxf(lo.y, tLo.y), g.origin.y, g.cell) != static_cast<int>(y) ||
-            cellOf(fmaxf(lo.z, tLo.z), g.origin.z, g.cell) != static_cast<int>(z)) {
-            continue;
-        }
         if (localPair(es, ij[t], localStart, localList)) continue;
         emit(min(s, t), max(s, t));
     }
@@ -311,22 +316,17 @@ __device__ void forSortedEntryCandidates(uint32_t k, const SelfGrid& g, const fl
                                          const uint32_t* __restrict__ localFixed, Emit emit) {
     const uint32_t n = g.sortCount;
     const uint32_t id = keys[k];
-    const uint32_t s = vals[k];
+    const uint32_t vs = vals[k];
+    const uint32_t s = vs & kEntrySegmentMask;
     const float4 lo = sortedLo[k], hi = sortedHi[k];
     const uint2 es = sortedIJ[k];
-    const uint32_t x = id % g.nx;
-    const uint32_t y = (id / g.nx) % g.ny;
-    const uint32_t z = id / (g.nx * g.ny);
     uint32_t na[W], nb[W];
     bool loaded = false;
     for (uint32_t j = k + 1; j < n && keys[j] == id; ++j) {
+        const uint32_t vt = vals[j];
+        if (((vs | vt) >> kEntrySegmentBits) != 7u) continue;  // not the owner cell
         const float4 tLo = sortedLo[j], tHi = sortedHi[j];
         if (!boxesOverlap(lo, hi, tLo, tHi)) continue;
-        if (cellOf(fmaxf(lo.x, tLo.x), g.origin.x, g.cell) != static_cast<int>(x) ||
-            cellOf(fmaxf(lo.y, tLo.y), g.origin.y, g.cell) != static_cast<int>(y) ||
-            cellOf(fmaxf(lo.z, tLo.z), g.origin.z, g.cell) != static_cast<int>(z)) {
-            continue;
-        }
         if (!loaded) {
             const uint4* ra = reinterpret_cast<const uint4*>(localFixed + static_cast<size_t>(es.x) * W);
             const uint4* rb = reinterpret_cast<const uint4*>(localFixed + static_cast<size_t>(es.y) * W);
@@ -350,7 +350,7 @@ __device__ void forSortedEntryCandidates(uint32_t k, const SelfGrid& g, const fl
         for (uint32_t m = 0; m < W; ++m)
             local |= na[m] == et.x || na[m] == et.y || nb[m] == et.x || nb[m] == et.y;
         if (local) continue;
-        const uint32_t t = vals[j];
+        const uint32_t t = vt & kEntrySegmentMask;
         emit(min(s, t), max(s, t));
     }
 }
@@ -362,7 +362,7 @@ __global__ void k_self_gather(const float4* __restrict__ boxLo, const float4* __
                               float4* __restrict__ sortedHi, uint2* __restrict__ sortedIJ) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (!grid->valid || k >= grid->sortCount) return;
-    const uint32_t s = vals[k];
+    const uint32_t s = vals[k] & kEntrySegmentMask;
     sortedLo[k] = boxLo[s];
     sortedHi[k] = boxHi[s];
     sortedIJ[k] = ij[s];
@@ -422,6 +422,10 @@ __global__ void k_self_emit_pairs(PairArgs a) {
     }
     if (!g.valid || k >= g.sortCount) return;
     uint32_t out = a.offsets[k];
+    // The offsets are the exclusive scan of the counts: an entry with no
+    // candidate (almost all of them) has nothing to emit.
+    const uint32_t end = k + 1 < g.sortCount ? a.offsets[k + 1] : *a.pairTotal;
+    if (end == out) return;
     entryCandidates<W>(k, g, a, [&](uint32_t p, uint32_t q) {
         if (out < a.capacity) a.pairs[out] = make_uint2(p, q);
         ++out;
@@ -479,14 +483,18 @@ __device__ V3 contactNormal(V3 separation, V3 da, V3 db, V3 previousOffset) {
     return n;
 }
 
-__device__ __forceinline__ void addFixed(FixedAccumulator* acc, V3 d) {
-    atomicAdd(reinterpret_cast<unsigned long long*>(&acc->x),
+// Adds d to knot i's fixed-point sum; the first contribution of a call also
+// sets the knot's bit in the touched mask, so the apply pass visits only the
+// knots that have corrections.
+__device__ __forceinline__ void addFixed(FixedAccumulator* acc, uint32_t* touched, uint32_t i, V3 d) {
+    FixedAccumulator* a = acc + i;
+    atomicAdd(reinterpret_cast<unsigned long long*>(&a->x),
               static_cast<unsigned long long>(__double2ll_rn(static_cast<double>(d.x) * kFixedScale)));
-    atomicAdd(reinterpret_cast<unsigned long long*>(&acc->y),
+    atomicAdd(reinterpret_cast<unsigned long long*>(&a->y),
               static_cast<unsigned long long>(__double2ll_rn(static_cast<double>(d.y) * kFixedScale)));
-    atomicAdd(reinterpret_cast<unsigned long long*>(&acc->z),
+    atomicAdd(reinterpret_cast<unsigned long long*>(&a->z),
               static_cast<unsigned long long>(__double2ll_rn(static_cast<double>(d.z) * kFixedScale)));
-    atomicAdd(&acc->count, 1u);
+    if (atomicAdd(&a->count, 1u) == 0u) atomicOr(touched + (i >> 5), 1u << (i & 31u));
 }
 
 // solve_self_contact (reference src/metal/ClothBag.metal:2125-2448) for every
@@ -496,7 +504,7 @@ __global__ void k_self_solve(const NumiClothBagGPUConfig* __restrict__ cfg, cons
                              uint32_t mode, const SelfGrid* __restrict__ grid, const uint2* __restrict__ pairs,
                              const uint2* __restrict__ ij, const float4* __restrict__ pos,
                              const float4* __restrict__ prev, FixedAccumulator* __restrict__ acc,
-                             NumiClothBagGPUSelfStatus* __restrict__ status) {
+                             uint32_t* __restrict__ touched, NumiClothBagGPUSelfStatus* __restrict__ status) {
     if (*ok == 0u) return;
     const uint32_t count = grid->valid ? grid->pairCount : 0u;
     const float r = cfg->clothMaterial.x;
@@ -589,10 +597,10 @@ __global__ void k_self_solve(const NumiClothBagGPUConfig* __restrict__ cfg, cons
                 const float denom = A0.w * ua * ua + A1.w * wa * wa + B0.w * ub * ub + B1.w * wb * wb;
                 if (denom > 0.0f && isfinite(denom)) {
                     const float lambda = correction / denom;
-                    addFixed(acc + ea.x, n * -(A0.w * ua * lambda));
-                    addFixed(acc + ea.y, n * -(A1.w * wa * lambda));
-                    addFixed(acc + eb.x, n * (B0.w * ub * lambda));
-                    addFixed(acc + eb.y, n * (B1.w * wb * lambda));
+                    addFixed(acc, touched, ea.x, n * -(A0.w * ua * lambda));
+                    addFixed(acc, touched, ea.y, n * -(A1.w * wa * lambda));
+                    addFixed(acc, touched, eb.x, n * (B0.w * ub * lambda));
+                    addFixed(acc, touched, eb.y, n * (B1.w * wb * lambda));
                     accepted = true;
                 }
             }
@@ -608,47 +616,30 @@ __global__ void k_self_solve(const NumiClothBagGPUConfig* __restrict__ cfg, cons
 }
 
 // Moves every touched knot by the mean of its fixed-point corrections and
-// clears its accumulator for the next call.
-// Mean of each touched knot's fixed-point corrections, then (collide = 1)
-// the ground/sphere pass that follows self-contact in an iteration: both act
-// on one knot at a time, so fusing them is exact.
-__global__ void k_self_apply(const NumiClothBagGPUConfig* __restrict__ cfg, const uint32_t* __restrict__ ok,
-                             float4* __restrict__ pos, FixedAccumulator* __restrict__ acc, uint32_t particles,
-                             const float4* __restrict__ spheres, uint32_t sphereCount, uint32_t collide,
+// clears its accumulator and mask bit for the next call. One thread per
+// 32-knot word of the touched mask.
+__global__ void k_self_apply(const uint32_t* __restrict__ ok, float4* __restrict__ pos,
+                             FixedAccumulator* __restrict__ acc, uint32_t* __restrict__ touched, uint32_t words,
                              uint32_t* __restrict__ failure) {
-    extern __shared__ float4 sphereCache[];
-    if (collide != 0u) {
-        for (uint32_t k = threadIdx.x; k < sphereCount; k += blockDim.x) sphereCache[k] = spheres[k];
-        __syncthreads();
-    }
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= particles || *ok == 0u) return;
-    float4 p = pos[i];
-    bool write = false;
-    const uint32_t n = acc[i].count;
-    if (n != 0) {
+    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= words || *ok == 0u) return;
+    uint32_t bits = touched[w];
+    if (bits == 0u) return;
+    touched[w] = 0u;
+    while (bits != 0u) {
+        const uint32_t i = (w << 5) + static_cast<uint32_t>(__ffs(bits) - 1);
+        bits &= bits - 1u;
         const FixedAccumulator a = acc[i];
-        const double scale = kFixedInverse / static_cast<double>(n);
+        const double scale = kFixedInverse / static_cast<double>(a.count);
         const V3 d = v3(static_cast<float>(static_cast<double>(a.x) * scale),
                         static_cast<float>(static_cast<double>(a.y) * scale),
                         static_cast<float>(static_cast<double>(a.z) * scale));
+        const float4 p = pos[i];
         const V3 np = xyz(p) + d;
-        if (!finite3(np)) {
-            recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
-        } else {
-            p = withXyz(p, np);
-            write = true;
-        }
+        if (!finite3(np)) recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
+        else pos[i] = withXyz(p, np);
         acc[i] = FixedAccumulator{0, 0, 0, 0u, 0u};
     }
-    if (collide != 0u) {
-        if (!isfinite(p.z)) {
-            recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
-        } else if (collideKnot(p, cfg->constraintCounts.z != 0u, cfg->clothMaterial.x, sphereCache, sphereCount)) {
-            write = true;
-        }
-    }
-    if (write) pos[i] = p;
 }
 
 // Largest capsule overlap 2r - distance over the current candidates

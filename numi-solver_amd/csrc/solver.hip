@@ -155,6 +155,7 @@ struct NumiClothHipSolver {
     DeviceArray<uint32_t> keysA, valsA, keysB, valsB, histogram;
     DeviceArray<uint2> pairs;
     DeviceArray<FixedAccumulator> accumulator;
+    DeviceArray<uint32_t> touched;            // one bit per knot with a pending self correction
     DeviceArray<SubstepSlot> debugSlot;
 
     // LDS tile programs (NUMI_CLOTH_HIP_TILES=0 keeps the per-colour launches).
@@ -237,6 +238,7 @@ struct NumiClothHipSolver {
     }
 
     void launchDistances() {
+        flushCollide();
         for (const Range& r : distanceColours) {
             k_distance<<<blocksFor(r.count), kBlock, 0, stream>>>(cfg(), ok(), r.first, r.count, posCur, dIJ.ptr,
                                                                   dRestCompliance.ptr, dLamCur, failure.ptr);
@@ -244,6 +246,7 @@ struct NumiClothHipSolver {
     }
 
     void launchKnots() {
+        flushCollide();
         for (const Range& r : knotColours) {
             k_knot<<<blocksFor(r.count), kBlock, 0, stream>>>(cfg(), ok(), r.first, r.count, posCur, kQuad.ptr,
                                                               kCosineCompliance.ptr, kLamCur, failure.ptr);
@@ -251,6 +254,7 @@ struct NumiClothHipSolver {
     }
 
     void launchBends() {
+        flushCollide();
         for (const Range& r : bendColours) {
             k_bend<<<blocksFor(r.count), kBlock, 0, stream>>>(cfg(), ok(), r.first, r.count, posCur, bEnds.ptr,
                                                               bChordCompliance.ptr, bLamCur, failure.ptr);
@@ -258,6 +262,7 @@ struct NumiClothHipSolver {
     }
 
     void launchStrain(uint32_t sweeps) {
+        flushCollide();
         for (uint32_t sweep = 0; sweep < sweeps; ++sweep) {
             for (const Range& r : distanceColours) {
                 k_strain<<<blocksFor(r.count), kBlock, 0, stream>>>(ok(), r.first, r.count, posCur, dIJ.ptr,
@@ -275,6 +280,7 @@ struct NumiClothHipSolver {
 
     void launchFruitPairs() {
         if (pairColours.empty()) return;
+        flushCollide();
         k_fruit_pairs<<<1, 64, 0, stream>>>(cfg(), ok(), pairColourRanges.ptr,
                                             static_cast<uint32_t>(pairColours.size()), fruits.ptr, F,
                                             fruitPairs.ptr, failure.ptr);
@@ -306,18 +312,21 @@ struct NumiClothHipSolver {
 
     void launchYarn(uint32_t mode) {
         if (Y == 0) return;
+        flushCollide();
         const size_t lds = static_cast<size_t>(P) * (sizeof(float4) + sizeof(float)) + static_cast<size_t>(D) * 4;
         k_yarn_solve<<<1, kYarnThreads, lds, stream>>>(yarnArgs(mode, posCur));
     }
 
     void launchBuildYarnContacts() {
         if (Y == 0) return;
+        flushCollide();
         k_yarn_contacts_build<<<blocksFor(Y), kBlock, 0, stream>>>(cfg(), ok(), posCur, prev.ptr, dIJ.ptr, D,
                                                                    fruits.ptr, F, yarnContacts.ptr, failure.ptr);
     }
 
     void launchFruitGround() {
         if (F == 0 || initialConfig.constraintCounts.z == 0u) return;
+        flushCollide();
         k_fruit_ground<<<blocksFor(F), kBlock, 0, stream>>>(cfg(), ok(), fruits.ptr, F, failure.ptr);
     }
 
@@ -330,10 +339,27 @@ struct NumiClothHipSolver {
     }
 
     void launchCollide() {
+        flushCollide();
+        collideNow();
+    }
+
+    void collideNow() {
         if (!groundOrColliders()) return;
         const uint32_t n = extension.staticSphereCount;
-        k_collide<<<blocksFor(P), kBlock, n * sizeof(float4), stream>>>(cfg(), ok(), posCur, P, spheres.ptr, n,
-                                                                        failure.ptr);
+        k_collide<<<blocksFor(P), kBlock, 2 * n * sizeof(float4), stream>>>(cfg(), ok(), posCur, P, spheres.ptr, n,
+                                                                            failure.ptr);
+    }
+
+    // A ground/sphere pass that follows self-contact can run at the start of
+    // the next tile program instead of as a pass of its own (it acts on one
+    // knot at a time, so loaded halo knots stay exact). Every other launch
+    // that reads or writes positions first runs it here.
+    bool pendingCollide = false;
+
+    void flushCollide() {
+        if (!pendingCollide) return;
+        pendingCollide = false;
+        collideNow();
     }
 
     // Exclusive scan of `capacity`-sized data whose live length is *count.
@@ -405,16 +431,20 @@ struct NumiClothHipSolver {
         emit<<<entryBlocks, kScanThreads, 0, stream>>>(pa);
     }
 
-    // collide: also run the ground/sphere pass that follows (fused, exact).
-    void launchSelf(uint32_t mode, bool collide = false) {
+    // collide: run the ground/sphere pass that follows; defer: leave it to the
+    // next launch (the next tile program takes it as its leading pass).
+    void launchSelf(uint32_t mode, bool collide = false, bool defer = false) {
+        flushCollide();
         if (!self) return;
         const uint32_t blocks = std::min<uint32_t>(2048, std::max(1u, (pairCapacity + kBlock - 1) / kBlock));
         k_self_solve<<<blocks, kBlock, 0, stream>>>(cfg(), ok(), mode, grid.ptr, pairs.ptr, dIJ.ptr, posCur,
-                                                     prev.ptr, accumulator.ptr, selfStatus.ptr);
-        const uint32_t n = collide ? extension.staticSphereCount : 0u;
-        k_self_apply<<<blocksFor(P), kBlock, n * sizeof(float4), stream>>>(cfg(), ok(), posCur, accumulator.ptr, P,
-                                                                         spheres.ptr, n, collide ? 1u : 0u,
-                                                                         failure.ptr);
+                                                     prev.ptr, accumulator.ptr, touched.ptr, selfStatus.ptr);
+        const uint32_t words = (P + 31) / 32;
+        k_self_apply<<<blocksFor(words), kBlock, 0, stream>>>(ok(), posCur, accumulator.ptr, touched.ptr, words,
+                                                              failure.ptr);
+        if (!collide || !groundOrColliders()) return;
+        if (defer) pendingCollide = true;
+        else collideNow();
     }
 
     void launchTileProgram(TileProgram& t, uint32_t trailingCollides, const NumiClothBagGPUConfig* config,
@@ -442,10 +472,13 @@ struct NumiClothHipSolver {
         a.spheres = spheres.ptr;
         a.sphereCount = extension.staticSphereCount;
         a.passes = t.passes;
+        a.leadingCollide = !scratch && pendingCollide ? 1u : 0u;
+        if (!scratch) pendingCollide = false;
         a.trailingCollides = groundOrColliders() ? trailingCollides : 0u;
         a.failure = failure.ptr;
         for (uint32_t k = 0; k < t.passes; ++k) a.kinds[k] = t.kinds[k];
-        const size_t shared = (t.maxLoaded + (a.trailingCollides ? a.sphereCount : 0u)) * sizeof(float4);
+        const bool collides = a.leadingCollide != 0 || a.trailingCollides != 0;
+        const size_t shared = (t.maxLoaded + (collides ? 2 * a.sphereCount : 0u)) * sizeof(float4);
         k_tile_program<<<t.tiles, t.block, shared, stream>>>(a);
         if (scratch) return;
         posCur = a.posOut;
@@ -456,6 +489,7 @@ struct NumiClothHipSolver {
 
     // Moves the live state back to the primary buffers at the end of a substep.
     void restorePrimaryBuffers() {
+        flushCollide();
         if (posCur != pos.ptr) k_copy_float4<<<blocksFor(P), kBlock, 0, stream>>>(posCur, pos.ptr, P);
         if (D != 0 && dLamCur != dLambda.ptr) k_copy_float<<<blocksFor(D), kBlock, 0, stream>>>(dLamCur, dLambda.ptr, D);
         if (B != 0 && bLamCur != bLambda.ptr) k_copy_float<<<blocksFor(B), kBlock, 0, stream>>>(bLamCur, bLambda.ptr, B);
@@ -468,7 +502,7 @@ struct NumiClothHipSolver {
 
     // Strain sweeps followed by `collides` ground/sphere passes.
     void launchStrainAndCollide(uint32_t sweeps, uint32_t collides) {
-        if (useTiles && sweeps != 0 && tileStrain.valid) {
+        if (useTiles && sweeps != 0 && tileStrain.valid) {  // consumes a deferred collide
             launchTileProgram(tileStrain, collides, cfg(), ok());
             return;
         }
@@ -478,6 +512,7 @@ struct NumiClothHipSolver {
 
     void launchGrips() {
         if (G == 0) return;
+        flushCollide();
         k_grip<<<blocksFor(G), kBlock, 0, stream>>>(cfg(), ok(), posCur, grips.ptr, G, P, failure.ptr);
     }
 
@@ -532,6 +567,10 @@ struct NumiClothHipSolver {
         // self-contact apply when that runs in between).
         const bool colliders = groundOrColliders();
         const bool trailing = colliders && !self && G == 0 && F == 0;
+        // With self-contact, the iteration's ground/sphere pass moves into the
+        // next tile program (nothing else runs in between without grips or
+        // fruits).
+        const bool defer = colliders && self && G == 0 && F == 0 && Y == 0 && useTiles;
         for (uint32_t it = 0; it < iterations; ++it) {
             TileProgram& program = it % 2 == 0 ? tileEven : tileOdd;
             const bool tiled = useTiles && program.valid;
@@ -546,7 +585,7 @@ struct NumiClothHipSolver {
             launchFruitPairs();
             launchYarn(0);
             if (self) {
-                launchSelf(0, colliders);
+                launchSelf(0, colliders, defer);
             } else if (!(tiled && trailing)) {
                 launchCollide();
             }
@@ -1006,6 +1045,8 @@ void setupSelfContact(const NumiClothHipSceneDesc& scene, NumiClothHipSolver& s)
         list.insert(list.end(), near.begin(), near.end());
     }
     hipStream_t st = s.stream;
+    if (s.D > kEntrySegmentMask)
+        fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "hash self-contact supports at most 2^29 - 1 segments");
     s.self = true;
     s.localStart.allocate(start.size());
     s.localStart.upload(start.data(), start.size(), st);
@@ -1058,6 +1099,8 @@ void setupSelfContact(const NumiClothHipSceneDesc& scene, NumiClothHipSolver& s)
     s.pairs.allocate(s.pairCapacity);
     s.accumulator.allocate(s.P);
     HIP_OK(hipMemsetAsync(s.accumulator.ptr, 0, s.P * sizeof(FixedAccumulator), st));
+    s.touched.allocate((s.P + 31) / 32);
+    HIP_OK(hipMemsetAsync(s.touched.ptr, 0, (s.P + 31) / 32 * sizeof(uint32_t), st));
     HIP_OK(hipMemsetAsync(s.grid.ptr, 0, sizeof(SelfGrid), st));
     HIP_OK(hipStreamSynchronize(st));
 }
@@ -1212,8 +1255,11 @@ void setupTiles(NumiClothHipSolver& s) {
     int maxShared = 0;
     HIP_OK(hipDeviceGetAttribute(&maxShared, hipDeviceAttributeMaxSharedMemoryPerBlock, s.device));
     const uint32_t spheres = s.extension.staticSphereCount;
-    if (static_cast<uint64_t>(spheres + 64) * sizeof(float4) >= static_cast<uint64_t>(maxShared)) return;
-    s.tileCapacity = static_cast<uint32_t>(maxShared / sizeof(float4)) - spheres;
+    // Dynamic LDS: the loaded knots, the spheres and a sphere window's list;
+    // 1 KiB stays free for the kernel's static window and reduction scratch.
+    const uint64_t reserved = 1024 + 2ull * spheres * sizeof(float4);
+    if (reserved + 64 * sizeof(float4) >= static_cast<uint64_t>(maxShared)) return;
+    s.tileCapacity = static_cast<uint32_t>((static_cast<uint64_t>(maxShared) - reserved) / sizeof(float4));
     s.posAlt.allocate(s.P);
     s.dLambdaAlt.allocate(s.D);
     s.bLambdaAlt.allocate(s.B);
@@ -1469,7 +1515,7 @@ void timePass(NumiClothHipSolver& s, uint32_t pass, uint32_t colour, uint32_t re
                                                                       s.dIJ.ptr, s.dMaxLength.ptr, s.failure.ptr);
         } else {
             const uint32_t n = s.extension.staticSphereCount;
-            k_collide<<<blocksFor(s.P), kBlock, n * sizeof(float4), s.stream>>>(s.cfg(), s.ok(), s.pos.ptr, s.P,
+            k_collide<<<blocksFor(s.P), kBlock, 2 * n * sizeof(float4), s.stream>>>(s.cfg(), s.ok(), s.pos.ptr, s.P,
                                                                                s.spheres.ptr, n, s.failure.ptr);
         }
     }
@@ -1515,6 +1561,8 @@ void selfBroadPhase(NumiClothHipSolver& s, NumiClothHipSelfBroadPhase* io) {
     if (io->cellIds != nullptr) s.keysA.download(io->cellIds, std::min(io->entryCount, io->entryCapacity), s.stream);
     if (io->segments != nullptr) s.valsA.download(io->segments, std::min(io->entryCount, io->entryCapacity), s.stream);
     HIP_OK(hipStreamSynchronize(s.stream));
+    if (io->segments != nullptr)  // drop the owner-cell flags kept above the segment index
+        for (uint32_t k = 0; k < std::min(io->entryCount, io->entryCapacity); ++k) io->segments[k] &= kEntrySegmentMask;
 }
 
 }  // namespace
