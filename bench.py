@@ -15,8 +15,8 @@ Workloads (synthetic, seeded with SplitMix64 / numpy PCG64 from SEED):
              per 1/60 s frame.
   bag        the produce bag (1,465 knots, 12 fruits of radius 2-3 cm),
              stretch compliance 1e-7, bend 1e-4, strain limit 0.10, ground
-             on, cuff grip lifted 0.4 m over 2 s (smoothstep) then held,
-             48 substeps per 1/120 s frame.
+             on, yarn/yarn self-contact and friction, cuff grip lifted 0.4 m
+             over 2 s (smoothstep) then held, 48 substeps per 1/120 s frame.
   strips512  512 hanging strips, width U[8,64] x height U[64,256] knots,
              compliances log-U[1e-8,1e-5], density U[0.1,0.4] kg/m^2, top row
              driven by a 0.5-2 Hz vertical sinusoid, one batch (merged scene,
@@ -114,7 +114,8 @@ def bag_scene(cloth, frames: int):
     """Produce bag with the stated materials; frame f's 48 configs lift the
     cuff grip by 0.4 m * smoothstep(t / 2 s)."""
     scene = cloth.Scene.bag(stretchComplianceOverride=1.0e-7, bendComplianceOverride=1.0e-4,
-                            strainLimitOverride=0.10, fruitRadiusMin=0.02, fruitRadiusMax=0.03, seed=SEED)
+                            strainLimitOverride=0.10, fruitRadiusMin=0.02, fruitRadiusMax=0.03, seed=SEED,
+                            selfContact=1)
     scene.config["constraintCounts"][0, 2] = 1
     scene.refresh()
     substeps, frame = 48, 1.0 / 120.0

@@ -113,10 +113,15 @@ __global__ void k_scan_add(uint32_t* __restrict__ data, const uint32_t* __restri
 // Per-tile digit histogram, stored digit-major: hist[digit * tiles + tile],
 // so one exclusive scan of `hist` yields every (digit, tile) output base in
 // stable order.
+// The histogram is digit-major over the live tiles, ceil(*count / kScanTile),
+// so its exclusive scan (of kRadixBins * live tiles entries) gives every
+// tile's first output slot per digit; blocks past the live tiles exit.
 __global__ void k_radix_histogram(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ count,
-                                  uint32_t shift, uint32_t tiles, uint32_t* __restrict__ hist) {
+                                  uint32_t shift, uint32_t* __restrict__ hist) {
     __shared__ uint32_t bins[kRadixBins];
     const uint32_t n = *count;
+    const uint32_t tiles = (n + kScanTile - 1) / kScanTile;
+    if (blockIdx.x >= tiles) return;
     bins[threadIdx.x] = 0;
     __syncthreads();
     const uint32_t base = blockIdx.x * kScanTile;
@@ -133,12 +138,13 @@ __global__ void k_radix_histogram(const uint32_t* __restrict__ keys, const uint3
 // (8 digit bits -> same-digit lane mask), waves are ordered through LDS, and
 // per-digit running counters carry across rounds.
 __global__ void k_radix_scatter(const uint32_t* __restrict__ keysIn, const uint32_t* __restrict__ valsIn,
-                                const uint32_t* __restrict__ count, uint32_t shift, uint32_t tiles,
+                                const uint32_t* __restrict__ count, uint32_t shift,
                                 const uint32_t* __restrict__ histScanned, uint32_t* __restrict__ keysOut,
                                 uint32_t* __restrict__ valsOut) {
     __shared__ uint32_t running[kRadixBins];
     __shared__ uint32_t waveCount[kScanThreads / 64][kRadixBins];
     const uint32_t n = *count;
+    const uint32_t tiles = (n + kScanTile - 1) / kScanTile;
     const uint32_t base = blockIdx.x * kScanTile;
     if (base >= n) return;
     const uint32_t tid = threadIdx.x;

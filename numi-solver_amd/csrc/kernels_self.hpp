@@ -58,7 +58,8 @@ struct SelfGrid {
     uint32_t pairTotal;      // candidate pairs found
     uint32_t pairCount;      // min(pairTotal, capacity)
     uint32_t valid;          // 0 when the grid would exceed 2^32 cells
-    uint32_t pad[2];
+    uint32_t histLength;     // radix histogram length: kRadixBins * ceil(sortCount / kScanTile)
+    uint32_t pad;
 };
 
 struct FixedAccumulator {
@@ -227,8 +228,10 @@ __global__ void k_self_emit_entries(const float4* __restrict__ boxLo, const floa
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     const SelfGrid g = *grid;
     if (s == 0) {
+        const uint32_t sorted = min(*entryTotal, capacity);
         grid->entryCount = *entryTotal;
-        grid->sortCount = min(*entryTotal, capacity);
+        grid->sortCount = sorted;
+        grid->histLength = kRadixBins * ((sorted + kScanTile - 1) / kScanTile);
         if (*entryTotal > capacity) recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_RANGE);
     }
     if (s >= g.segments || !g.valid) return;
@@ -360,12 +363,14 @@ __global__ void k_self_gather(const float4* __restrict__ boxLo, const float4* __
                               const uint2* __restrict__ ij, const SelfGrid* __restrict__ grid,
                               const uint32_t* __restrict__ vals, float4* __restrict__ sortedLo,
                               float4* __restrict__ sortedHi, uint2* __restrict__ sortedIJ) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (!grid->valid || k >= grid->sortCount) return;
-    const uint32_t s = vals[k] & kEntrySegmentMask;
-    sortedLo[k] = boxLo[s];
-    sortedHi[k] = boxHi[s];
-    sortedIJ[k] = ij[s];
+    if (!grid->valid) return;
+    const uint32_t n = grid->sortCount;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const uint32_t s = vals[k] & kEntrySegmentMask;
+        sortedLo[k] = boxLo[s];
+        sortedHi[k] = boxHi[s];
+        sortedIJ[k] = ij[s];
+    }
 }
 
 struct PairArgs {
@@ -386,6 +391,7 @@ struct PairArgs {
     const uint32_t* pairTotal;
     uint32_t capacity;
     uint2* pairs;
+    NumiClothBagGPUSelfImpulse* impulses;  // per-candidate impulse aggregates, cleared here
     uint32_t* failure;
 };
 
@@ -400,14 +406,16 @@ __device__ __forceinline__ void entryCandidates(uint32_t k, const SelfGrid& g, c
     }
 }
 
+// Grid-stride over the live entries (the grid is sized for the capacity).
 template <uint32_t W>
 __global__ void k_self_pair_counts(PairArgs a) {
     const SelfGrid g = *a.grid;
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (!g.valid || k >= g.sortCount) return;
-    uint32_t n = 0;
-    entryCandidates<W>(k, g, a, [&](uint32_t, uint32_t) { ++n; });
-    a.counts[k] = n;
+    if (!g.valid) return;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < g.sortCount; k += gridDim.x * blockDim.x) {
+        uint32_t n = 0;
+        entryCandidates<W>(k, g, a, [&](uint32_t, uint32_t) { ++n; });
+        a.counts[k] = n;
+    }
 }
 
 template <uint32_t W>
@@ -420,16 +428,23 @@ __global__ void k_self_emit_pairs(PairArgs a) {
         a.grid->pairCount = min(total, a.capacity);
         if (total > a.capacity) recordFailure(a.failure, NUMI_CLOTH_BAG_GPU_FAILURE_RANGE);
     }
-    if (!g.valid || k >= g.sortCount) return;
-    uint32_t out = a.offsets[k];
-    // The offsets are the exclusive scan of the counts: an entry with no
-    // candidate (almost all of them) has nothing to emit.
-    const uint32_t end = k + 1 < g.sortCount ? a.offsets[k + 1] : *a.pairTotal;
-    if (end == out) return;
-    entryCandidates<W>(k, g, a, [&](uint32_t p, uint32_t q) {
-        if (out < a.capacity) a.pairs[out] = make_uint2(p, q);
-        ++out;
-    });
+    if (!g.valid) return;
+    for (uint32_t e = k; e < g.sortCount; e += gridDim.x * blockDim.x) {
+        uint32_t out = a.offsets[e];
+        // The offsets are the exclusive scan of the counts: an entry with no
+        // candidate (almost all of them) has nothing to emit.
+        const uint32_t end = e + 1 < g.sortCount ? a.offsets[e + 1] : *a.pairTotal;
+        if (end == out) continue;
+        entryCandidates<W>(e, g, a, [&](uint32_t p, uint32_t q) {
+            if (out < a.capacity) {
+                a.pairs[out] = make_uint2(p, q);
+                NumiClothBagGPUSelfImpulse empty{};
+                empty.identity = mr_uint4{out, p, q, 0u};
+                a.impulses[out] = empty;
+            }
+            ++out;
+        });
+    }
 }
 
 // ---------------------------------------------------------- narrow phase --
@@ -504,7 +519,8 @@ __global__ void k_self_solve(const NumiClothBagGPUConfig* __restrict__ cfg, cons
                              uint32_t mode, const SelfGrid* __restrict__ grid, const uint2* __restrict__ pairs,
                              const uint2* __restrict__ ij, const float4* __restrict__ pos,
                              const float4* __restrict__ prev, FixedAccumulator* __restrict__ acc,
-                             uint32_t* __restrict__ touched, NumiClothBagGPUSelfStatus* __restrict__ status) {
+                             uint32_t* __restrict__ touched, NumiClothBagGPUSelfImpulse* __restrict__ impulses,
+                             NumiClothBagGPUSelfStatus* __restrict__ status) {
     if (*ok == 0u) return;
     const uint32_t count = grid->valid ? grid->pairCount : 0u;
     const float r = cfg->clothMaterial.x;
@@ -602,6 +618,22 @@ __global__ void k_self_solve(const NumiClothBagGPUConfig* __restrict__ cfg, cons
                     addFixed(acc, touched, eb.x, n * (B0.w * ub * lambda));
                     addFixed(acc, touched, eb.y, n * (B1.w * wb * lambda));
                     accepted = true;
+                    // The contact's impulse record (reference :2391-2407),
+                    // summed into the candidate's aggregate in call order
+                    // (reference aggregate_self_impulses :2503-2553). Only
+                    // this thread touches candidate k in a call.
+                    const float impulse = lambda / cfg->gravityAndTimestep.w;
+                    NumiClothBagGPUSelfImpulse a = impulses[k];
+                    a.identity.w += 1u;
+                    a.normalAndImpulse.x = a.normalAndImpulse.x + -n.x * impulse;
+                    a.normalAndImpulse.y = a.normalAndImpulse.y + -n.y * impulse;
+                    a.normalAndImpulse.z = a.normalAndImpulse.z + -n.z * impulse;
+                    a.normalAndImpulse.w = a.normalAndImpulse.w + impulse;
+                    a.endpointImpulses.x = a.endpointImpulses.x + ua * impulse;
+                    a.endpointImpulses.y = a.endpointImpulses.y + wa * impulse;
+                    a.endpointImpulses.z = a.endpointImpulses.z + ub * impulse;
+                    a.endpointImpulses.w = a.endpointImpulses.w + wb * impulse;
+                    impulses[k] = a;
                 }
             }
         }
@@ -639,6 +671,79 @@ __global__ void k_self_apply(const uint32_t* __restrict__ ok, float4* __restrict
         if (!finite3(np)) recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
         else pos[i] = withXyz(p, np);
         acc[i] = FixedAccumulator{0, 0, 0, 0u, 0u};
+    }
+}
+
+// Self-contact friction (reference apply_self_friction,
+// src/metal/ClothBag.metal:2580-2731) on the candidates that made contact in
+// the substep: the aggregate's normal, normal impulse and endpoint weights
+// give a maximum-dissipation tangential impulse limited by mu times the
+// normal impulse. The reference walks a static pair colouring in order; the
+// hash candidates have none, so every pair acts on one velocity snapshot
+// (Jacobi) and each touched knot takes the mean of its velocity changes,
+// summed in 2^-32 m/s fixed point (order independent), as the normal
+// response does with positions. Contact count -> metrics.w, largest
+// impulse / limit -> metrics.x (integer max on float bits).
+__global__ void k_self_friction(const NumiClothBagGPUConfig* __restrict__ cfg, const uint32_t* __restrict__ ok,
+                                const SelfGrid* __restrict__ grid, const uint2* __restrict__ pairs,
+                                const uint2* __restrict__ ij, const float4* __restrict__ pos,
+                                const float4* __restrict__ vel, const NumiClothBagGPUSelfImpulse* __restrict__ impulses,
+                                FixedAccumulator* __restrict__ acc, uint32_t* __restrict__ touched,
+                                NumiClothBagGPUFrictionStatus* __restrict__ friction) {
+    if (*ok == 0u) return;
+    const uint32_t count = grid->valid ? grid->pairCount : 0u;
+    const float mu = cfg->clothMaterial.w;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t rounds = (count + stride - 1) / stride;
+    for (uint32_t round = 0; round < rounds; ++round) {
+        const uint32_t k = round * stride + blockIdx.x * blockDim.x + threadIdx.x;
+        bool applied = false;
+        float ratio = 0.0f;
+        if (k < count && impulses[k].identity.w != 0u) {
+            const NumiClothBagGPUSelfImpulse c = impulses[k];
+            const uint2 pr = pairs[k];
+            const uint2 ea = ij[pr.x];
+            const uint2 eb = ij[pr.y];
+            const float normalImpulse = c.normalAndImpulse.w;
+            const V3 nsum = v3(c.normalAndImpulse.x, c.normalAndImpulse.y, c.normalAndImpulse.z);
+            const float normalLength = length(nsum);
+            const float firstSum = c.endpointImpulses.x + c.endpointImpulses.y;
+            const float secondSum = c.endpointImpulses.z + c.endpointImpulses.w;
+            if (normalImpulse > 0.0f && normalLength > 1.0e-10f && firstSum > 1.0e-12f && secondSum > 1.0e-12f) {
+                const V3 normal = nsum / normalLength;
+                const float f0 = c.endpointImpulses.x / firstSum, f1 = c.endpointImpulses.y / firstSum;
+                const float s0 = c.endpointImpulses.z / secondSum, s1 = c.endpointImpulses.w / secondSum;
+                const float4 A0 = pos[ea.x], A1 = pos[ea.y], B0 = pos[eb.x], B1 = pos[eb.y];
+                const V3 firstVelocity = xyz(vel[ea.x]) * f0 + xyz(vel[ea.y]) * f1;
+                const V3 secondVelocity = xyz(vel[eb.x]) * s0 + xyz(vel[eb.y]) * s1;
+                const V3 relative = firstVelocity - secondVelocity;
+                const V3 tangentVelocity = relative - normal * dot(relative, normal);
+                const float slip = length(tangentVelocity);
+                if (slip > 1.0e-10f) {
+                    const V3 tangent = tangentVelocity / slip;
+                    const float denominator = A0.w * f0 * f0 + A1.w * f1 * f1 + B0.w * s0 * s0 + B1.w * s1 * s1;
+                    if (denominator > 0.0f) {
+                        const float limit = mu * normalImpulse;
+                        const float tangential = fminf(slip / denominator, limit);
+                        if (tangential > 0.0f) {
+                            const V3 onFirst = tangent * -tangential;
+                            addFixed(acc, touched, ea.x, onFirst * (A0.w * f0));
+                            addFixed(acc, touched, ea.y, onFirst * (A1.w * f1));
+                            addFixed(acc, touched, eb.x, -(onFirst * (B0.w * s0)));
+                            addFixed(acc, touched, eb.y, -(onFirst * (B1.w * s1)));
+                            applied = true;
+                            if (limit > 0.0f) ratio = tangential / limit;
+                        }
+                    }
+                }
+            }
+        }
+        const unsigned long long ballot = __ballot(applied);
+        for (int offset = 32; offset > 0; offset >>= 1) ratio = fmaxf(ratio, __shfl_xor(ratio, offset));
+        if ((threadIdx.x & 63u) == 0 && ballot != 0ull) {
+            atomicAdd(&friction->metrics.w, static_cast<uint32_t>(__popcll(ballot)));
+            atomicMax(&friction->metrics.x, __float_as_uint(ratio));
+        }
     }
 }
 

@@ -151,11 +151,12 @@ struct NumiClothHipSolver {
     DeviceArray<uint2> sortedIJ;
     DeviceArray<float4> boxLo, boxHi, partialLo, partialHi;
     DeviceArray<SelfGrid> grid;
-    DeviceArray<uint32_t> segmentCounts, entryCounts, scanSums, totals, constants;
+    DeviceArray<uint32_t> segmentCounts, entryCounts, scanSums, totals;
     DeviceArray<uint32_t> keysA, valsA, keysB, valsB, histogram;
     DeviceArray<uint2> pairs;
     DeviceArray<FixedAccumulator> accumulator;
     DeviceArray<uint32_t> touched;            // one bit per knot with a pending self correction
+    DeviceArray<NumiClothBagGPUSelfImpulse> selfImpulses;  // per-candidate impulse aggregate of the substep
     DeviceArray<SubstepSlot> debugSlot;
 
     // LDS tile programs (NUMI_CLOTH_HIP_TILES=0 keeps the per-colour launches).
@@ -390,15 +391,16 @@ struct NumiClothHipSolver {
         uint32_t* keysOut = keysB.ptr;
         uint32_t* valsOut = valsB.ptr;
         for (uint32_t shift = 0; shift < 32; shift += kRadixBits) {
-            k_radix_histogram<<<entryTiles, kScanThreads, 0, stream>>>(keysIn, &g->sortCount, shift, entryTiles,
-                                                                        histogram.ptr);
-            launchScan(histogram.ptr, constants.ptr, kRadixBins * entryTiles, totals.ptr + 2);
+            k_radix_histogram<<<entryTiles, kScanThreads, 0, stream>>>(keysIn, &g->sortCount, shift, histogram.ptr);
+            launchScan(histogram.ptr, &g->histLength, kRadixBins * entryTiles, totals.ptr + 2);
             k_radix_scatter<<<entryTiles, kScanThreads, 0, stream>>>(keysIn, valsIn, &g->sortCount, shift,
-                                                                      entryTiles, histogram.ptr, keysOut, valsOut);
+                                                                      histogram.ptr, keysOut, valsOut);
             std::swap(keysIn, keysOut);
             std::swap(valsIn, valsOut);
         }
-        const uint32_t entryBlocks = (entryCapacity + kScanThreads - 1) / kScanThreads;
+        // Grid-stride kernels over the live entries: enough blocks to fill the
+        // chip, not one per capacity slot.
+        const uint32_t entryBlocks = std::min<uint32_t>(8192, (entryCapacity + kScanThreads - 1) / kScanThreads);
         PairArgs pa{};
         pa.boxLo = boxLo.ptr;
         pa.boxHi = boxHi.ptr;
@@ -417,6 +419,7 @@ struct NumiClothHipSolver {
         pa.pairTotal = totals.ptr + 1;
         pa.capacity = pairCapacity;
         pa.pairs = pairs.ptr;
+        pa.impulses = selfImpulses.ptr;
         pa.failure = failure.ptr;
         if (localWidth != 0) {
             k_self_gather<<<entryBlocks, kScanThreads, 0, stream>>>(boxLo.ptr, boxHi.ptr, dIJ.ptr, g, valsIn,
@@ -438,13 +441,27 @@ struct NumiClothHipSolver {
         if (!self) return;
         const uint32_t blocks = std::min<uint32_t>(2048, std::max(1u, (pairCapacity + kBlock - 1) / kBlock));
         k_self_solve<<<blocks, kBlock, 0, stream>>>(cfg(), ok(), mode, grid.ptr, pairs.ptr, dIJ.ptr, posCur,
-                                                     prev.ptr, accumulator.ptr, touched.ptr, selfStatus.ptr);
+                                                     prev.ptr, accumulator.ptr, touched.ptr, selfImpulses.ptr,
+                                                     selfStatus.ptr);
         const uint32_t words = (P + 31) / 32;
         k_self_apply<<<blocksFor(words), kBlock, 0, stream>>>(ok(), posCur, accumulator.ptr, touched.ptr, words,
                                                               failure.ptr);
         if (!collide || !groundOrColliders()) return;
         if (defer) pendingCollide = true;
         else collideNow();
+    }
+
+    // Self-contact friction after finalize (velocities) and the cloth/ground
+    // friction fused into it, before the fruit/yarn frictions.
+    void launchSelfFriction() {
+        if (!self) return;
+        const uint32_t blocks = std::min<uint32_t>(2048, std::max(1u, (pairCapacity + kBlock - 1) / kBlock));
+        k_self_friction<<<blocks, kBlock, 0, stream>>>(cfg(), ok(), grid.ptr, pairs.ptr, dIJ.ptr, pos.ptr, vel.ptr,
+                                                        selfImpulses.ptr, accumulator.ptr, touched.ptr,
+                                                        frictionStatus.ptr);
+        const uint32_t words = (P + 31) / 32;
+        k_self_apply<<<blocksFor(words), kBlock, 0, stream>>>(ok(), vel.ptr, accumulator.ptr, touched.ptr, words,
+                                                              failure.ptr);
     }
 
     void launchTileProgram(TileProgram& t, uint32_t trailingCollides, const NumiClothBagGPUConfig* config,
@@ -617,6 +634,7 @@ struct NumiClothHipSolver {
         launchBuildYarnContacts();
         k_finalize<<<blocksFor(P), kBlock, 0, stream>>>(cfg(), ok(), pos.ptr, prev.ptr, vel.ptr, P,
                                                         frictionStatus.ptr, failure.ptr);
+        launchSelfFriction();
         if (F != 0) {
             k_fruit_finalize<<<blocksFor(F), kBlock, 0, stream>>>(cfg(), ok(), fruits.ptr, F, failure.ptr);
             if (Y != 0) {
@@ -1062,7 +1080,10 @@ void setupSelfContact(const NumiClothHipSceneDesc& scene, NumiClothHipSolver& s)
         s.localFixed.allocate(fixed.size());
         s.localFixed.upload(fixed.data(), fixed.size(), st);
     }
-    const uint64_t entries = static_cast<uint64_t>(s.D) * 16u;
+    // 32 cell entries per segment: a segment whose swept box (start and
+    // predicted ends, grown by radius + margin) spans up to 4 x 4 x 2 cells.
+    // More is a RANGE failure, as is a candidate list beyond its capacity.
+    const uint64_t entries = static_cast<uint64_t>(s.D) * 32u;
     const uint64_t pairCapacity = scene.extension.selfCandidateCapacity != 0
                                       ? scene.extension.selfCandidateCapacity
                                       : static_cast<uint64_t>(s.D) * 16u;
@@ -1083,9 +1104,7 @@ void setupSelfContact(const NumiClothHipSceneDesc& scene, NumiClothHipSolver& s)
     const uint32_t maxScanLength = std::max({s.D, kRadixBins * s.entryTiles, s.entryCapacity, 1u});
     s.scanSums.allocate((maxScanLength + kScanTile - 1) / kScanTile + 1);
     s.totals.allocate(4);
-    s.constants.allocate(1);
     const uint32_t histLength = kRadixBins * s.entryTiles;
-    s.constants.upload(&histLength, 1, st);
     s.keysA.allocate(s.entryCapacity);
     s.valsA.allocate(s.entryCapacity);
     if (s.localWidth != 0) {
@@ -1097,6 +1116,7 @@ void setupSelfContact(const NumiClothHipSceneDesc& scene, NumiClothHipSolver& s)
     s.valsB.allocate(s.entryCapacity);
     s.histogram.allocate(histLength);
     s.pairs.allocate(s.pairCapacity);
+    s.selfImpulses.allocate(s.pairCapacity);  // written by the emit pass before any use
     s.accumulator.allocate(s.P);
     HIP_OK(hipMemsetAsync(s.accumulator.ptr, 0, s.P * sizeof(FixedAccumulator), st));
     s.touched.allocate((s.P + 31) / 32);

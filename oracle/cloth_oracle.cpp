@@ -71,7 +71,18 @@ struct Oracle {
     uint32_t failure = 0;
     uint32_t frictionContacts = 0;
     uint32_t presentContacts = 0, sweptContacts = 0;
+    uint32_t selfFrictionContacts = 0;
     double maximumConeRatio = 0;
+    // Impulse aggregate of each candidate over the substep's self-contact
+    // calls (reference aggregate_self_impulses, src/metal/ClothBag.metal:
+    // 2503-2553: sum of -normal * impulse, impulse, endpoint shares, count).
+    struct SelfImpulse {
+        V<R> n{};
+        R impulse = 0;
+        R e[4] = {0, 0, 0, 0};
+        uint32_t count = 0;
+    };
+    std::vector<SelfImpulse> selfImpulses;
     double time = 0;
 
     explicit Oracle(const NumiClothHipSceneDesc& scene)
@@ -423,6 +434,7 @@ struct Oracle {
             origin = {std::min(origin.x, b.lo.x), std::min(origin.y, b.lo.y), std::min(origin.z, b.lo.z)};
         }
         candidates.clear();
+        selfImpulses.clear();
         if (!(cell > R(0))) return;
         std::unordered_map<uint64_t, std::vector<uint32_t>> cells;
         auto key = [](long x, long y, long z) {
@@ -530,7 +542,9 @@ struct Oracle {
         fixed.assign(P, Fixed{});
         deltaSum.assign(P, V<R>{});
         deltaCount.assign(P, 0u);
-        for (const auto& pr : candidates) {
+        selfImpulses.resize(candidates.size());
+        for (size_t k = 0; k < candidates.size(); ++k) {
+            const auto& pr = candidates[k];
             const mr_uint4 ea = d.distances[pr.first].particlesAndColor;
             const mr_uint4 eb = d.distances[pr.second].particlesAndColor;
             const Particle<R>& A0 = p[ea.x];
@@ -613,6 +627,16 @@ struct Oracle {
             accumulate(eb.x, n * (B0.w * ub * lambda));
             accumulate(eb.y, n * (B1.w * wb * lambda));
             (mode == 0 ? presentContacts : sweptContacts) += 1;
+            // Impulse record (reference :2391-2407) into the aggregate.
+            const R impulse = lambda / R(c.gravityAndTimestep.w);
+            SelfImpulse& a = selfImpulses[k];
+            a.count += 1;
+            a.n = {a.n.x + -n.x * impulse, a.n.y + -n.y * impulse, a.n.z + -n.z * impulse};
+            a.impulse = a.impulse + impulse;
+            a.e[0] = a.e[0] + ua * impulse;
+            a.e[1] = a.e[1] + wa * impulse;
+            a.e[2] = a.e[2] + ub * impulse;
+            a.e[3] = a.e[3] + wb * impulse;
         }
         for (uint32_t i = 0; i < P; ++i) {
             if (sizeof(R) == sizeof(float)) {
@@ -624,6 +648,67 @@ struct Oracle {
             } else {
                 if (deltaCount[i] == 0) continue;
                 p[i].x = p[i].x + deltaSum[i] / R(deltaCount[i]);
+            }
+        }
+    }
+
+    // Self-contact friction (reference apply_self_friction,
+    // src/metal/ClothBag.metal:2580-2731) on every candidate that made contact
+    // in the substep, Jacobi from one velocity snapshot, each touched knot
+    // taking the mean of its velocity changes (the hash redesign's rule, the
+    // same as the normal response).
+    void selfFriction(const NumiClothBagGPUConfig& c) {
+        if (!d.extension.selfContact) return;
+        const R mu = R(c.clothMaterial.w);
+        fixed.assign(P, Fixed{});
+        deltaSum.assign(P, V<R>{});
+        deltaCount.assign(P, 0u);
+        for (size_t k = 0; k < selfImpulses.size(); ++k) {
+            const SelfImpulse& a = selfImpulses[k];
+            if (a.count == 0) continue;
+            const mr_uint4 ea = d.distances[candidates[k].first].particlesAndColor;
+            const mr_uint4 eb = d.distances[candidates[k].second].particlesAndColor;
+            const R normalImpulse = a.impulse;
+            const R normalLength = norm(a.n);
+            const R firstSum = a.e[0] + a.e[1];
+            const R secondSum = a.e[2] + a.e[3];
+            if (!(normalImpulse > R(0) && normalLength > R(1.0e-10) && firstSum > R(1.0e-12) &&
+                  secondSum > R(1.0e-12)))
+                continue;
+            const V<R> normal = a.n / normalLength;
+            const R f0 = a.e[0] / firstSum, f1 = a.e[1] / firstSum;
+            const R s0 = a.e[2] / secondSum, s1 = a.e[3] / secondSum;
+            const Particle<R>&A0 = p[ea.x], &A1 = p[ea.y], &B0 = p[eb.x], &B1 = p[eb.y];
+            const V<R> firstVelocity = A0.v * f0 + A1.v * f1;
+            const V<R> secondVelocity = B0.v * s0 + B1.v * s1;
+            const V<R> relative = firstVelocity - secondVelocity;
+            const V<R> tangentVelocity = relative - normal * dot(relative, normal);
+            const R slip = norm(tangentVelocity);
+            if (!(slip > R(1.0e-10))) continue;
+            const V<R> tangent = tangentVelocity / slip;
+            const R denominator = A0.w * f0 * f0 + A1.w * f1 * f1 + B0.w * s0 * s0 + B1.w * s1 * s1;
+            if (!(denominator > R(0))) continue;
+            const R limit = mu * normalImpulse;
+            const R tangential = std::min(slip / denominator, limit);
+            if (!(tangential > R(0))) continue;
+            const V<R> onFirst = tangent * -tangential;
+            accumulate(ea.x, onFirst * (A0.w * f0));
+            accumulate(ea.y, onFirst * (A1.w * f1));
+            accumulate(eb.x, -(onFirst * (B0.w * s0)));
+            accumulate(eb.y, -(onFirst * (B1.w * s1)));
+            selfFrictionContacts += 1;
+            if (limit > R(0)) maximumConeRatio = std::max(maximumConeRatio, static_cast<double>(tangential / limit));
+        }
+        for (uint32_t i = 0; i < P; ++i) {
+            if (sizeof(R) == sizeof(float)) {
+                const Fixed& f = fixed[i];
+                if (f.count == 0) continue;
+                const double scale = (1.0 / 4294967296.0) / static_cast<double>(f.count);
+                p[i].v = p[i].v + V<R>{R(static_cast<double>(f.x) * scale), R(static_cast<double>(f.y) * scale),
+                                       R(static_cast<double>(f.z) * scale)};
+            } else {
+                if (deltaCount[i] == 0) continue;
+                p[i].v = p[i].v + deltaSum[i] / R(deltaCount[i]);
             }
         }
     }
@@ -1311,6 +1396,7 @@ struct Oracle {
         }
         buildYarnContacts(c);
         finalize(c);
+        selfFriction(c);
         finalizeFruit(c);
         yarnFriction(c);
         pairFriction(c);
@@ -1356,6 +1442,8 @@ typedef struct NumiClothOracleOutput {
     uint32_t fruitGroundContacts;
     uint32_t rollingContacts;
     double maximumRollingRatio;
+    uint32_t selfFrictionContacts;
+    uint32_t reserved2;
 } NumiClothOracleOutput;
 
 }  // extern "C"
@@ -1436,6 +1524,7 @@ static int runOracle(const NumiClothHipSceneDesc* scene, const NumiClothBagGPUCo
     out->fruitGroundContacts = o.fruitGroundContacts;
     out->rollingContacts = o.rollingContacts;
     out->maximumRollingRatio = o.maximumRollingRatio;
+    out->selfFrictionContacts = o.selfFrictionContacts;
     return 0;
 }
 

@@ -27,7 +27,8 @@ class OracleOutput(C.Structure):
                 ("candidateMask", C.c_uint32),
                 ("releasedMask", C.c_uint32), ("pairFrictionContacts", C.c_uint32),
                 ("yarnFrictionContacts", C.c_uint32), ("fruitGroundContacts", C.c_uint32),
-                ("rollingContacts", C.c_uint32), ("maximumRollingRatio", C.c_double)]
+                ("rollingContacts", C.c_uint32), ("maximumRollingRatio", C.c_double),
+                ("selfFrictionContacts", C.c_uint32), ("reserved2", C.c_uint32)]
 
 
 _lib = None
@@ -81,7 +82,7 @@ def run(scene, configs: np.ndarray, iterations: int, strain_sweeps: int, precisi
     out["sweptContacts"] = int(result.sweptContacts)
     out["candidateCount"] = int(result.candidateCount)
     for name in ["candidateMask", "releasedMask", "pairFrictionContacts", "yarnFrictionContacts",
-                 "fruitGroundContacts", "rollingContacts"]:
+                 "fruitGroundContacts", "rollingContacts", "selfFrictionContacts"]:
         out[name] = int(getattr(result, name))
     for name in ["maximumRollingRatio", "maximumYarnAeroForce", "maximumFruitAeroForce", "maximumFruitAeroTorque"]:
         out[name] = float(getattr(result, name))

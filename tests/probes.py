@@ -236,3 +236,17 @@ MOUTH_CASES = {
     "edge_exit": (((1.30, 0.0, 1.02), False, 1), 1),
     "rotated": (((1.130, 0.0, 0.0), True, 0), 1),
 }
+
+
+def crossing_segments_probe(cloth, mu, slip=0.2, overlap=0.001, dt=1.0 / 960.0, radius=0.004):
+    """Two free 2 cm segments crossing at right angles, B (along y) lying
+    `overlap` deeper than contact on A (along x) and sliding along x at
+    `slip`; no gravity, no ground; self-contact with cloth/self friction mu."""
+    h = 2.0 * radius - overlap
+    p = particles(cloth, [(-0.01, 0, 0), (0.01, 0, 0), (0, -0.01, h), (0, 0.01, h)], [1.0, 1.0, 1.0, 1.0],
+                  velocities=[(0, 0, 0), (0, 0, 0), (slip, 0, 0), (slip, 0, 0)])
+    d = np.concatenate([distance(cloth, 0, 1, 0.02), distance(cloth, 2, 3, 0.02)])
+    c = config(cloth, particles=4, distances=2, dt=dt, radius=radius)
+    c["clothMaterial"] = (radius, 0.02, 0.45, mu)
+    return cloth.Scene(c, particles=p, distances=d, distanceBatches=batch(cloth, 0, 2, 0), selfContact=1,
+                       selfContactMargin=0.25 * radius)

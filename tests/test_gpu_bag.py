@@ -173,3 +173,37 @@ def test_bag_runs_are_bit_identical(cloth):
     second = _run(cloth, scene, 24, 32, 3)
     for name in ("particles", "fruits", "fruitPairs", "yarnContacts", "frictionStatus", "aerodynamicsStatus"):
         assert first[name].tobytes() == second[name].tobytes(), name
+
+
+def test_bag_with_self_contact_frame_parity(cloth, oracle):
+    """The bag with yarn/yarn self-contact (hash broad phase, normal response,
+    impulse aggregation and self friction) over 12 substeps: knots and fruits
+    as the FP32 twin computes them, the same self-contact and friction
+    counts, and the FP64 gate."""
+    scene = cloth.Scene.bag(selfContact=1)
+    substeps, iterations, sweeps = 12, 32, 3
+    configs = scene.configs(substeps)
+    state = _run(cloth, scene, substeps, iterations, sweeps)
+    assert state["failure"] == 0
+    twin = oracle.run(scene, configs, iterations, sweeps, precision="fp32")
+    fp64 = oracle.run(scene, configs, iterations, sweeps)
+    gpu = state["particles"]["positionAndInverseMass"][:, :3].astype(np.float64)
+    x = scene.particles["positionAndInverseMass"][:, :3]
+    extent = float(np.max(x.max(axis=0) - x.min(axis=0)))
+    e_twin = np.max(np.abs(gpu - twin["positions"]))
+    e_fp64 = np.max(np.abs(gpu - fp64["positions"]))
+    e_ref = np.max(np.abs(twin["positions"] - fp64["positions"]))
+    counters = state["selfStatus"]["counters"][0]
+    metrics = state["frictionStatus"]["metrics"][0]
+    print(f"bag+self: |gpu-twin| {e_twin:.3e} |gpu-fp64| {e_fp64:.3e} |twin-fp64| {e_ref:.3e}; contacts gpu "
+          f"{counters[:2]} twin {twin['presentContacts']}/{twin['sweptContacts']}; self friction gpu {metrics[3]} "
+          f"twin {twin['selfFrictionContacts']}")
+    assert twin["presentContacts"] + twin["sweptContacts"] > 0, "the bag made no yarn/yarn contact"
+    assert e_twin <= 1e-6 * extent
+    assert e_fp64 <= max(1e-5 * extent, 1.5 * e_ref)
+    # Contact activation is a threshold; the reference's own gate on the self
+    # friction count is a difference of at most 4 (reference
+    # tools/cloth_bag_metal.mm:6769-6772, :7699).
+    assert abs(int(counters[0]) - twin["presentContacts"]) <= max(4, twin["presentContacts"] // 100)
+    assert abs(int(counters[1]) - twin["sweptContacts"]) <= 4
+    assert abs(int(metrics[3]) - twin["selfFrictionContacts"]) <= 4

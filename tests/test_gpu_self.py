@@ -216,3 +216,29 @@ def test_self_contact_runs_are_bit_identical(cloth):
             results.append(solver.read())
     assert results[0]["particles"].tobytes() == results[1]["particles"].tobytes()
     assert results[0]["selfStatus"].tobytes() == results[1]["selfStatus"].tobytes()
+
+
+@pytest.mark.parametrize("mu", [0.01, 0.34, 100.0])
+def test_self_friction_matches_fp32_twin(cloth, oracle, mu):
+    """Impulse aggregation and self friction (crossing segments, one slides
+    over the other): positions and velocities as the FP32 twin computes
+    them, the same friction contact count and cone ratio."""
+    from probes import crossing_segments_probe
+    scene = crossing_segments_probe(cloth, mu)
+    configs = scene.configs(1)
+    with cloth.Solver(scene) as solver:
+        solver.step(configs, 4, 0)
+        state = solver.read()
+    twin = oracle.run(scene, configs, 4, 0, precision="fp32")
+    p = state["particles"]
+    dx = np.max(np.abs(p["positionAndInverseMass"][:, :3] - twin["positions"]))
+    dv = np.max(np.abs(p["velocity"][:, :3] - twin["velocities"]))
+    metrics = state["frictionStatus"]["metrics"][0]
+    ratio = float(np.array([metrics[0]], np.uint32).view(np.float32)[0])
+    print(f"mu {mu}: |dx| {dx:.2e}, |dv| {dv:.2e}, friction contacts gpu {metrics[3]} twin "
+          f"{twin['selfFrictionContacts']}, cone ratio gpu {ratio:.7f} twin {twin['maximumConeRatio']:.7f}")
+    assert state["failure"] == 0
+    assert metrics[3] == twin["selfFrictionContacts"] == 1
+    assert dx <= 1e-7
+    assert dv <= 1e-6
+    assert abs(ratio - twin["maximumConeRatio"]) <= 1e-6
