@@ -498,6 +498,13 @@ __device__ V3 contactNormal(V3 separation, V3 da, V3 db, V3 previousOffset) {
     return n;
 }
 
+// Mean correction of a touched knot's fixed-point sum.
+__device__ __forceinline__ V3 fixedMean(const FixedAccumulator& a) {
+    const double scale = kFixedInverse / static_cast<double>(a.count);
+    return v3(static_cast<float>(static_cast<double>(a.x) * scale), static_cast<float>(static_cast<double>(a.y) * scale),
+              static_cast<float>(static_cast<double>(a.z) * scale));
+}
+
 // Adds d to knot i's fixed-point sum; the first contribution of a call also
 // sets the knot's bit in the touched mask, so the apply pass visits only the
 // knots that have corrections.
@@ -650,22 +657,24 @@ __global__ void k_self_solve(const NumiClothBagGPUConfig* __restrict__ cfg, cons
 // Moves every touched knot by the mean of its fixed-point corrections and
 // clears its accumulator and mask bit for the next call. One thread per
 // 32-knot word of the touched mask.
+// pos = nullptr: clear only (a buffer whose corrections were applied by the
+// tile programs that loaded the knots).
 __global__ void k_self_apply(const uint32_t* __restrict__ ok, float4* __restrict__ pos,
                              FixedAccumulator* __restrict__ acc, uint32_t* __restrict__ touched, uint32_t words,
                              uint32_t* __restrict__ failure) {
     const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= words || *ok == 0u) return;
+    if (w >= words) return;
     uint32_t bits = touched[w];
     if (bits == 0u) return;
     touched[w] = 0u;
     while (bits != 0u) {
         const uint32_t i = (w << 5) + static_cast<uint32_t>(__ffs(bits) - 1);
         bits &= bits - 1u;
-        const FixedAccumulator a = acc[i];
-        const double scale = kFixedInverse / static_cast<double>(a.count);
-        const V3 d = v3(static_cast<float>(static_cast<double>(a.x) * scale),
-                        static_cast<float>(static_cast<double>(a.y) * scale),
-                        static_cast<float>(static_cast<double>(a.z) * scale));
+        if (pos == nullptr || *ok == 0u) {
+            acc[i] = FixedAccumulator{0, 0, 0, 0u, 0u};
+            continue;
+        }
+        const V3 d = fixedMean(acc[i]);
         const float4 p = pos[i];
         const V3 np = xyz(p) + d;
         if (!finite3(np)) recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);

@@ -3,6 +3,9 @@
 // LDS-resident tile programs (see tile_plan.hpp for the plan and its
 // exactness argument). One workgroup per tile:
 //   1. gather the tile's loaded knots (owned + halo) from posIn into LDS,
+//   1b. optionally apply the self-contact corrections of the previous
+//      self-contact solve and the ground/sphere pass that followed it
+//      (both act on one knot at a time, so halo copies stay exact),
 //   2. run every pass of the program on LDS, one item per thread slot, with a
 //      workgroup barrier between colours; multipliers are read from the
 //      input buffer and written to the output buffer by the owning tile only,
@@ -14,6 +17,7 @@
 // kernels call, so results are bit-identical to the per-colour schedule.
 
 #include "kernels_core.hpp"
+#include "kernels_self.hpp"
 
 namespace numi_dev {
 
@@ -46,6 +50,14 @@ struct TileProgramArgs {
     const float4* spheres;
     uint32_t sphereCount;
     uint32_t passes;
+    uint32_t sweepPasses;       // strain programs: passes per sweep (0 otherwise)
+    // Self-contact corrections to apply first (null: none), and the buffer of
+    // an earlier solve to clear for reuse (null: none). Other tiles read the
+    // applied buffer in this launch, so it is cleared by a later one.
+    const FixedAccumulator* selfAcc;
+    const uint32_t* selfTouched;
+    FixedAccumulator* clearAcc;
+    uint32_t* clearTouched;
     uint32_t leadingCollide;    // 1: the ground/sphere pass before the first colour pass
     uint32_t trailingCollides;
     uint32_t* failure;
@@ -55,7 +67,7 @@ struct TileProgramArgs {
 // Ground/sphere pass over every loaded knot (owned and halo: the pass acts on
 // one knot at a time, so halo knots stay equal to their owners' values).
 // Non-finite knots are skipped and reported by their owning tile only.
-__device__ void tileCollide(float4* lds, uint32_t loaded, uint32_t owned, const TileProgramArgs& a, bool ground,
+__device__ __forceinline__ void tileCollide(float4* lds, uint32_t loaded, uint32_t owned, const TileProgramArgs& a, bool ground,
                             float radius, const float4* sphereCache, float4* culled, SphereWindow& window,
                             float4* scratch) {
     if (a.sphereCount != 0) {
@@ -92,59 +104,83 @@ __device__ __forceinline__ bool tileItemOwner(uint32_t kind, const uint4& item) 
     return ((kind == kTileKnotKind ? item.z : item.y) >> 31) != 0u && kind != kTileStrainKind;
 }
 
-__device__ __forceinline__ float tileItemLambda(const TileProgramArgs& a, uint32_t kind, const uint4& item) {
-    const uint32_t c = tileItemConstraint(kind, item);
-    if (kind == kTileDistanceKind) return a.distanceLambdaIn[c];
-    if (kind == kTileBendKind) return a.bendLambdaIn[c];
-    if (kind == kTileKnotKind) return a.knotLambdaIn[c];
-    return 0.0f;
+// Item slots m = 0..kTilePrefetch-1 of thread t in pass k (zero past the end).
+__device__ __forceinline__ void tileLoadItems(const uint4* __restrict__ items, const uint32_t* __restrict__ itemStart,
+                                              uint32_t passes, uint32_t k, uint32_t t, uint32_t stride, uint4* dst) {
+    if (k >= passes) return;
+    const uint32_t first = itemStart[k], last = itemStart[k + 1];
+#pragma unroll
+    for (uint32_t m = 0; m < kTilePrefetch; ++m) {
+        const uint32_t j = first + t + m * stride;
+        dst[m] = j < last ? items[j] : make_uint4(0u, 0u, 0u, 0u);
+    }
+}
+
+// The input multipliers of those slots (in: the pass kind's input buffer).
+__device__ __forceinline__ void tileLoadLambdas(const float* __restrict__ in, const uint32_t* __restrict__ itemStart,
+                                                uint32_t passes, uint32_t k, uint32_t kind, uint32_t t,
+                                                uint32_t stride, const uint4* src, float* dst) {
+    if (k >= passes) return;
+    const uint32_t first = itemStart[k], last = itemStart[k + 1];
+#pragma unroll
+    for (uint32_t m = 0; m < kTilePrefetch; ++m)
+        dst[m] = first + t + m * stride < last ? in[tileItemConstraint(kind, src[m])] : 0.0f;
 }
 
 // One constraint of a pass on the LDS knots; lam is its multiplier from the
 // input buffer, written to the output buffer when this tile owns it.
-__device__ __forceinline__ void tileItem(float4* lds, const TileProgramArgs& a, uint32_t kind, const uint4& item,
-                                         float lam, bool live, float dt, bool ground, float floorZ) {
+// Strain = true handles strain-limit items only and returns whether the item
+// moved its knots; Strain = false handles distance, knot and bend items.
+// out is the pass kind's output multiplier buffer.
+template <bool Strain>
+__device__ __forceinline__ bool tileItem(float4* lds, uint32_t kind, const uint4& item, float lam, float* out,
+                                         const float2* __restrict__ knotCC, uint32_t* failure, bool live, float dt,
+                                         bool ground, float floorZ) {
+    if constexpr (Strain) {
+        const uint32_t la = item.x & 0xffffu, lb = item.x >> 16;
+        if (!live) return false;
+        float4 p = lds[la], q = lds[lb];
+        if (!limitStrainPair(p, q, __uint_as_float(item.z), failure)) return false;
+        lds[la] = p;
+        lds[lb] = q;
+        return true;
+    }
     const uint32_t la = item.x & 0xffffu, lb = item.x >> 16;
     const uint32_t c = tileItemConstraint(kind, item);
     if (kind == kTileKnotKind) {
         if (live) {
             const uint32_t lc = item.y & 0xffffu, ld = item.y >> 16;
             float4 p0 = lds[la], p1 = lds[lb], p2 = lds[lc], p3 = lds[ld];
-            if (projectKnot(p0, p1, p2, p3, a.knotCosineCompliance[c], lam, dt, a.failure)) {
+            if (projectKnot(p0, p1, p2, p3, knotCC[c], lam, dt, failure)) {
                 lds[la] = p0;
                 lds[lb] = p1;
                 lds[lc] = p2;
                 lds[ld] = p3;
             }
         }
-        if (tileItemOwner(kind, item)) a.knotLambdaOut[c] = lam;
+        if (tileItemOwner(kind, item)) out[c] = lam;
     } else if (kind == kTileDistanceKind) {
         if (live) {
             float4 p = lds[la], q = lds[lb];
             const float2 rc = make_float2(__uint_as_float(item.z), __uint_as_float(item.w));
-            if (projectDistance(p, q, rc, lam, dt, a.failure)) {
+            if (projectDistance(p, q, rc, lam, dt, failure)) {
                 lds[la] = p;
                 lds[lb] = q;
             }
         }
-        if (tileItemOwner(kind, item)) a.distanceLambdaOut[c] = lam;
+        if (tileItemOwner(kind, item)) out[c] = lam;
     } else if (kind == kTileBendKind) {
         if (live) {
             float4 p = lds[la], q = lds[lb];
             const float2 cc = make_float2(__uint_as_float(item.z), __uint_as_float(item.w));
-            if (projectBend(p, q, cc, lam, dt, ground, floorZ, a.failure)) {
+            if (projectBend(p, q, cc, lam, dt, ground, floorZ, failure)) {
                 lds[la] = p;
                 lds[lb] = q;
             }
         }
-        if (tileItemOwner(kind, item)) a.bendLambdaOut[c] = lam;
-    } else if (live) {  // strain limit
-        float4 p = lds[la], q = lds[lb];
-        if (limitStrainPair(p, q, __uint_as_float(item.z), a.failure)) {
-            lds[la] = p;
-            lds[lb] = q;
-        }
+        if (tileItemOwner(kind, item)) out[c] = lam;
     }
+    return false;
 }
 
 // The first kTilePrefetch item slots of a thread in every pass are software
@@ -152,11 +188,35 @@ __device__ __forceinline__ void tileItem(float4* lds, const TileProgramArgs& a, 
 // requested before pass k runs, so their latency overlaps the LDS work and
 // the barrier instead of following each other. Slots beyond those (tiles
 // with more than kTilePrefetch x workgroup items in one pass) load in place.
+//
+// Strain programs stop after the first sweep (all distance colours) that
+// moves no knot of the tile: strain passes keep no state besides the knots,
+// so every later sweep would repeat the same checks on the same positions
+// and move nothing either. The owned knots are the same bytes.
+// Strain = true: a strain-limit program (strain passes only, early exit);
+// false: an iteration program (distance, knot and bend passes).
+template <bool Strain>
 __global__ void k_tile_program(TileProgramArgs a) {
     extern __shared__ float4 lds[];
     __shared__ SphereWindow window;
     __shared__ float4 scratch[2 * kMaxBlockWaves];
     __shared__ uint32_t starts[kMaxTilePasses + 1];
+    __shared__ uint32_t kinds[kMaxTilePasses];
+    // Multiplier buffers by pass kind (distance, knot, bend), read from LDS:
+    // choosing among fields of the kernel argument by a run-time kind would
+    // copy the whole argument block to scratch.
+    __shared__ const float* lambdaIn[3];
+    __shared__ float* lambdaOut[3];
+    if (threadIdx.x == 0) {
+        lambdaIn[kTileDistanceKind] = a.distanceLambdaIn;
+        lambdaIn[kTileKnotKind] = a.knotLambdaIn;
+        lambdaIn[kTileBendKind] = a.bendLambdaIn;
+        lambdaOut[kTileDistanceKind] = a.distanceLambdaOut;
+        lambdaOut[kTileKnotKind] = a.knotLambdaOut;
+        lambdaOut[kTileBendKind] = a.bendLambdaOut;
+    }
+    const float2* __restrict__ knotCC = a.knotCosineCompliance;
+    uint32_t* const failure = a.failure;
     const uint32_t tile = blockIdx.x;
     const uint32_t base = a.loadStart[tile];
     const uint32_t loaded = a.loadStart[tile + 1] - base;
@@ -165,32 +225,25 @@ __global__ void k_tile_program(TileProgramArgs a) {
     float4* sphereCache = lds + loaded;
     float4* culled = sphereCache + a.sphereCount;
     const bool collides = a.leadingCollide != 0 || a.trailingCollides != 0;
-    for (uint32_t k = threadIdx.x; k <= passes; k += blockDim.x) starts[k] = a.itemStart[tile * passes + k];
+    const uint32_t* __restrict__ itemStart = a.itemStart + tile * passes;
+    const uint4* __restrict__ items = a.items;
+    // The pass kinds go to LDS: indexing the kernel argument's array with a
+    // loop counter would copy the whole argument block to scratch.
+    for (uint32_t k = threadIdx.x; k < passes; k += blockDim.x) kinds[k] = a.kinds[k];
+    for (uint32_t k = threadIdx.x; k <= passes; k += blockDim.x) starts[k] = itemStart[k];
     // Items of passes 0 and 1, multipliers of pass 0 (issued before the knot
     // gather so the two overlap).
     const uint32_t t = threadIdx.x, stride = blockDim.x;
     uint4 cur[kTilePrefetch], next[kTilePrefetch], ahead[kTilePrefetch];
     float lam[kTilePrefetch], nextLam[kTilePrefetch];
-    auto loadItems = [&](uint32_t k, uint4* dst) {
-        if (k >= passes) return;
-        const uint32_t first = a.itemStart[tile * passes + k], last = a.itemStart[tile * passes + k + 1];
-#pragma unroll
-        for (uint32_t m = 0; m < kTilePrefetch; ++m) {
-            const uint32_t j = first + t + m * stride;
-            dst[m] = j < last ? a.items[j] : make_uint4(0u, 0u, 0u, 0u);
-        }
-    };
-    auto loadLambdas = [&](uint32_t k, const uint4* src, float* dst) {
-        if (k >= passes) return;
-        const uint32_t kind = a.kinds[k];
-        const uint32_t first = a.itemStart[tile * passes + k], last = a.itemStart[tile * passes + k + 1];
-#pragma unroll
-        for (uint32_t m = 0; m < kTilePrefetch; ++m)
-            dst[m] = first + t + m * stride < last ? tileItemLambda(a, kind, src[m]) : 0.0f;
-    };
-    loadItems(0, cur);
-    loadItems(1, next);
-    loadLambdas(0, cur, lam);
+    tileLoadItems(items, itemStart, passes, 0, t, stride, cur);
+    tileLoadItems(items, itemStart, passes, 1, t, stride, next);
+    if constexpr (!Strain) {
+        const uint32_t kind0 = a.kinds[0];
+        const float* in0 = kind0 == kTileDistanceKind ? a.distanceLambdaIn
+                                                      : (kind0 == kTileKnotKind ? a.knotLambdaIn : a.bendLambdaIn);
+        tileLoadLambdas(in0, itemStart, passes, 0, kind0, t, stride, cur, lam);
+    }
     for (uint32_t i = t; i < loaded; i += stride) lds[i] = a.posIn[a.loadIds[base + i]];
     if (collides)
         for (uint32_t k = t; k < a.sphereCount; k += stride) sphereCache[k] = a.spheres[k];
@@ -199,21 +252,62 @@ __global__ void k_tile_program(TileProgramArgs a) {
     const float dt = a.cfg->gravityAndTimestep.w;
     const bool ground = a.cfg->constraintCounts.z != 0u;
     const float floorZ = a.cfg->clothMaterial.x;
-    if (live && a.leadingCollide != 0)
-        tileCollide(lds, loaded, owned, a, ground, floorZ, sphereCache, culled, window, scratch);
-    for (uint32_t k = 0; k < passes; ++k) {
-        loadLambdas(k + 1, next, nextLam);
-        loadItems(k + 2, ahead);
-        const uint32_t kind = a.kinds[k];
-        const uint32_t first = starts[k], last = starts[k + 1];
-#pragma unroll
-        for (uint32_t m = 0; m < kTilePrefetch; ++m)
-            if (first + t + m * stride < last) tileItem(lds, a, kind, cur[m], lam[m], live, dt, ground, floorZ);
-        for (uint32_t j = first + t + kTilePrefetch * stride; j < last; j += stride) {
-            const uint4 item = a.items[j];
-            tileItem(lds, a, kind, item, tileItemLambda(a, kind, item), live, dt, ground, floorZ);
+    if (a.clearAcc != nullptr) {
+        // Owned knots only, one mask bit at a time: other tiles clear the
+        // other bits of a shared word.
+        for (uint32_t i = t; i < owned; i += stride) {
+            const uint32_t id = a.loadIds[base + i];
+            const uint32_t bit = 1u << (id & 31u);
+            if ((a.clearTouched[id >> 5] & bit) != 0u) {
+                a.clearAcc[id] = FixedAccumulator{0, 0, 0, 0u, 0u};
+                atomicAnd(a.clearTouched + (id >> 5), ~bit);
+            }
+        }
+    }
+    if (live && a.selfAcc != nullptr) {
+        for (uint32_t i = t; i < loaded; i += stride) {
+            const uint32_t id = a.loadIds[base + i];
+            if (((a.selfTouched[id >> 5] >> (id & 31u)) & 1u) == 0u) continue;
+            const float4 p = lds[i];
+            const V3 np = xyz(p) + fixedMean(a.selfAcc[id]);
+            if (!finite3(np)) recordFailure(a.failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
+            else lds[i] = withXyz(p, np);
         }
         __syncthreads();
+    }
+    if (live && a.leadingCollide != 0)
+        tileCollide(lds, loaded, owned, a, ground, floorZ, sphereCache, culled, window, scratch);
+    bool sweepMoved = false;
+    for (uint32_t k = 0; k < passes; ++k) {
+        if constexpr (!Strain)
+            if (k + 1 < passes)
+                tileLoadLambdas(lambdaIn[kinds[k + 1]], itemStart, passes, k + 1, kinds[k + 1], t, stride, next,
+                                nextLam);
+        tileLoadItems(items, itemStart, passes, k + 2, t, stride, ahead);
+        const uint32_t kind = kinds[k];
+        const float* in = Strain ? nullptr : lambdaIn[kind];
+        float* out = Strain ? nullptr : lambdaOut[kind];
+        const uint32_t first = starts[k], last = starts[k + 1];
+        bool moved = false;
+#pragma unroll
+        for (uint32_t m = 0; m < kTilePrefetch; ++m)
+            if (first + t + m * stride < last)
+                moved |= tileItem<Strain>(lds, kind, cur[m], lam[m], out, knotCC, failure, live, dt, ground, floorZ);
+        for (uint32_t j = first + t + kTilePrefetch * stride; j < last; j += stride) {
+            const uint4 item = items[j];
+            const float lj = Strain ? 0.0f : in[tileItemConstraint(kind, item)];
+            moved |= tileItem<Strain>(lds, kind, item, lj, out, knotCC, failure, live, dt, ground, floorZ);
+        }
+        if constexpr (Strain) {
+            sweepMoved |= __syncthreads_or(moved ? 1 : 0) != 0;
+            if ((k + 1) % a.sweepPasses == 0) {
+                if (!sweepMoved) break;
+                sweepMoved = false;
+            }
+        } else {
+            (void)moved;
+            __syncthreads();
+        }
 #pragma unroll
         for (uint32_t m = 0; m < kTilePrefetch; ++m) {
             cur[m] = next[m];

@@ -154,8 +154,15 @@ struct NumiClothHipSolver {
     DeviceArray<uint32_t> segmentCounts, entryCounts, scanSums, totals;
     DeviceArray<uint32_t> keysA, valsA, keysB, valsB, histogram;
     DeviceArray<uint2> pairs;
-    DeviceArray<FixedAccumulator> accumulator;
-    DeviceArray<uint32_t> touched;            // one bit per knot with a pending self correction
+    // Two fixed-point accumulators and touched masks (one bit per knot with a
+    // pending correction), used by alternate self-contact solves: a tile
+    // program that applies one buffer's corrections cannot clear it (other
+    // tiles read it in the same launch), so the next solve uses the other.
+    DeviceArray<FixedAccumulator> accumulator[2];
+    DeviceArray<uint32_t> touched[2];
+    uint32_t selfBuffer = 0;   // buffer of the next self-contact solve (record time)
+    int pendingSelfApply = -1; // buffer whose corrections the next launch applies
+    int dirtySelf = -1;        // buffer applied by tile programs, still to clear
     DeviceArray<NumiClothBagGPUSelfImpulse> selfImpulses;  // per-candidate impulse aggregate of the substep
     DeviceArray<SubstepSlot> debugSlot;
 
@@ -357,7 +364,13 @@ struct NumiClothHipSolver {
     // that reads or writes positions first runs it here.
     bool pendingCollide = false;
 
+    // Runs what was left to the next tile program: the self-contact
+    // corrections, then the ground/sphere pass.
     void flushCollide() {
+        if (pendingSelfApply >= 0) {
+            applySelf(static_cast<uint32_t>(pendingSelfApply), posCur);
+            pendingSelfApply = -1;
+        }
         if (!pendingCollide) return;
         pendingCollide = false;
         collideNow();
@@ -439,16 +452,40 @@ struct NumiClothHipSolver {
     void launchSelf(uint32_t mode, bool collide = false, bool defer = false) {
         flushCollide();
         if (!self) return;
+        const uint32_t b = selfBuffer;
+        if (dirtySelf == static_cast<int>(b)) {
+            clearSelf(b);
+            dirtySelf = -1;
+        }
         const uint32_t blocks = std::min<uint32_t>(2048, std::max(1u, (pairCapacity + kBlock - 1) / kBlock));
         k_self_solve<<<blocks, kBlock, 0, stream>>>(cfg(), ok(), mode, grid.ptr, pairs.ptr, dIJ.ptr, posCur,
-                                                     prev.ptr, accumulator.ptr, touched.ptr, selfImpulses.ptr,
+                                                     prev.ptr, accumulator[b].ptr, touched[b].ptr, selfImpulses.ptr,
                                                      selfStatus.ptr);
+        selfBuffer ^= 1u;
+        const bool colliding = collide && groundOrColliders();
+        if (defer) {
+            pendingSelfApply = static_cast<int>(b);
+            pendingCollide = colliding;
+            return;
+        }
+        applySelf(b, posCur);
+        if (colliding) collideNow();
+    }
+
+    void applySelf(uint32_t b, float4* target) {
         const uint32_t words = (P + 31) / 32;
-        k_self_apply<<<blocksFor(words), kBlock, 0, stream>>>(ok(), posCur, accumulator.ptr, touched.ptr, words,
+        k_self_apply<<<blocksFor(words), kBlock, 0, stream>>>(ok(), target, accumulator[b].ptr, touched[b].ptr, words,
                                                               failure.ptr);
-        if (!collide || !groundOrColliders()) return;
-        if (defer) pendingCollide = true;
-        else collideNow();
+    }
+
+    void clearSelf(uint32_t b) { applySelf(b, nullptr); }
+
+    // Ends the substep's self-contact buffers: nothing pending, both clean.
+    void settleSelf() {
+        flushCollide();
+        if (dirtySelf >= 0) clearSelf(static_cast<uint32_t>(dirtySelf));
+        dirtySelf = -1;
+        selfBuffer = 0;
     }
 
     // Self-contact friction after finalize (velocities) and the cloth/ground
@@ -457,11 +494,9 @@ struct NumiClothHipSolver {
         if (!self) return;
         const uint32_t blocks = std::min<uint32_t>(2048, std::max(1u, (pairCapacity + kBlock - 1) / kBlock));
         k_self_friction<<<blocks, kBlock, 0, stream>>>(cfg(), ok(), grid.ptr, pairs.ptr, dIJ.ptr, pos.ptr, vel.ptr,
-                                                        selfImpulses.ptr, accumulator.ptr, touched.ptr,
+                                                        selfImpulses.ptr, accumulator[0].ptr, touched[0].ptr,
                                                         frictionStatus.ptr);
-        const uint32_t words = (P + 31) / 32;
-        k_self_apply<<<blocksFor(words), kBlock, 0, stream>>>(ok(), vel.ptr, accumulator.ptr, touched.ptr, words,
-                                                              failure.ptr);
+        applySelf(0, vel.ptr);
     }
 
     void launchTileProgram(TileProgram& t, uint32_t trailingCollides, const NumiClothBagGPUConfig* config,
@@ -489,14 +524,28 @@ struct NumiClothHipSolver {
         a.spheres = spheres.ptr;
         a.sphereCount = extension.staticSphereCount;
         a.passes = t.passes;
+        a.sweepPasses = &t == &tileStrain ? static_cast<uint32_t>(distanceColours.size()) : 0u;
         a.leadingCollide = !scratch && pendingCollide ? 1u : 0u;
-        if (!scratch) pendingCollide = false;
+        if (!scratch) {
+            pendingCollide = false;
+            if (pendingSelfApply >= 0) {
+                a.selfAcc = accumulator[pendingSelfApply].ptr;
+                a.selfTouched = touched[pendingSelfApply].ptr;
+            }
+            if (dirtySelf >= 0) {
+                a.clearAcc = accumulator[dirtySelf].ptr;
+                a.clearTouched = touched[dirtySelf].ptr;
+            }
+            dirtySelf = pendingSelfApply;
+            pendingSelfApply = -1;
+        }
         a.trailingCollides = groundOrColliders() ? trailingCollides : 0u;
         a.failure = failure.ptr;
         for (uint32_t k = 0; k < t.passes; ++k) a.kinds[k] = t.kinds[k];
         const bool collides = a.leadingCollide != 0 || a.trailingCollides != 0;
         const size_t shared = (t.maxLoaded + (collides ? 2 * a.sphereCount : 0u)) * sizeof(float4);
-        k_tile_program<<<t.tiles, t.block, shared, stream>>>(a);
+        if (a.sweepPasses != 0) k_tile_program<true><<<t.tiles, t.block, shared, stream>>>(a);
+        else k_tile_program<false><<<t.tiles, t.block, shared, stream>>>(a);
         if (scratch) return;
         posCur = a.posOut;
         if (t.hasDistance) dLamCur = a.distanceLambdaOut;
@@ -506,7 +555,7 @@ struct NumiClothHipSolver {
 
     // Moves the live state back to the primary buffers at the end of a substep.
     void restorePrimaryBuffers() {
-        flushCollide();
+        settleSelf();
         if (posCur != pos.ptr) k_copy_float4<<<blocksFor(P), kBlock, 0, stream>>>(posCur, pos.ptr, P);
         if (D != 0 && dLamCur != dLambda.ptr) k_copy_float<<<blocksFor(D), kBlock, 0, stream>>>(dLamCur, dLambda.ptr, D);
         if (B != 0 && bLamCur != bLambda.ptr) k_copy_float<<<blocksFor(B), kBlock, 0, stream>>>(bLamCur, bLambda.ptr, B);
@@ -536,6 +585,10 @@ struct NumiClothHipSolver {
     // The substep in the reference's dispatch order
     // (tools/cloth_bag_metal.mm:4211-4709), families without records omitted.
     void recordSubstep(uint32_t iterations, uint32_t sweeps) {
+        selfBuffer = 0;
+        pendingSelfApply = -1;
+        dirtySelf = -1;
+        pendingCollide = false;
         posCur = pos.ptr;
         dLamCur = dLambda.ptr;
         bLamCur = bLambda.ptr;
@@ -573,9 +626,13 @@ struct NumiClothHipSolver {
             k_advance<<<blocksFor(P), kBlock, 0, stream>>>(cfg(), ok(), posCur, vel.ptr, P);
             if (F != 0) k_fruit_advance<<<blocksFor(F), kBlock, 0, stream>>>(cfg(), ok(), fruits.ptr, F);
         }
+        // Without grips or fruits nothing runs between a self-contact solve
+        // and the next tile program: that program applies the corrections
+        // (and the ground/sphere pass that follows in an iteration).
+        const bool defer = self && G == 0 && F == 0 && Y == 0 && useTiles;
         if (self) {
             launchBroadPhase(cfg(), ok());
-            launchSelf(1);
+            launchSelf(1, false, defer);
         }
         launchBuildYarnContacts();
         launchYarn(1);
@@ -584,10 +641,6 @@ struct NumiClothHipSolver {
         // self-contact apply when that runs in between).
         const bool colliders = groundOrColliders();
         const bool trailing = colliders && !self && G == 0 && F == 0;
-        // With self-contact, the iteration's ground/sphere pass moves into the
-        // next tile program (nothing else runs in between without grips or
-        // fruits).
-        const bool defer = colliders && self && G == 0 && F == 0 && Y == 0 && useTiles;
         for (uint32_t it = 0; it < iterations; ++it) {
             TileProgram& program = it % 2 == 0 ? tileEven : tileOdd;
             const bool tiled = useTiles && program.valid;
@@ -619,7 +672,7 @@ struct NumiClothHipSolver {
                 launchFruitGround();
             }
             for (uint32_t round = 0; round <= kCertificatePasses; ++round) {
-                launchSelf(0);
+                launchSelf(0, false, defer);
                 if (F == 0) {
                     // Without fruits the final contact passes are ground
                     // passes only and fuse into the strain program.
@@ -1117,10 +1170,12 @@ void setupSelfContact(const NumiClothHipSceneDesc& scene, NumiClothHipSolver& s)
     s.histogram.allocate(histLength);
     s.pairs.allocate(s.pairCapacity);
     s.selfImpulses.allocate(s.pairCapacity);  // written by the emit pass before any use
-    s.accumulator.allocate(s.P);
-    HIP_OK(hipMemsetAsync(s.accumulator.ptr, 0, s.P * sizeof(FixedAccumulator), st));
-    s.touched.allocate((s.P + 31) / 32);
-    HIP_OK(hipMemsetAsync(s.touched.ptr, 0, (s.P + 31) / 32 * sizeof(uint32_t), st));
+    for (int b = 0; b < 2; ++b) {
+        s.accumulator[b].allocate(s.P);
+        HIP_OK(hipMemsetAsync(s.accumulator[b].ptr, 0, s.P * sizeof(FixedAccumulator), st));
+        s.touched[b].allocate((s.P + 31) / 32);
+        HIP_OK(hipMemsetAsync(s.touched[b].ptr, 0, (s.P + 31) / 32 * sizeof(uint32_t), st));
+    }
     HIP_OK(hipMemsetAsync(s.grid.ptr, 0, sizeof(SelfGrid), st));
     HIP_OK(hipStreamSynchronize(st));
 }
