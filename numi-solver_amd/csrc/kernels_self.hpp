@@ -62,6 +62,14 @@ struct SelfGrid {
     uint32_t pad;
 };
 
+// A sorted cell entry with its segment's box, gathered once per substep so
+// that the candidate passes read one 32-byte record per visited entry:
+// a = (cell id, segment | start flags << 29, lo.x, lo.y) as bits / floats,
+// b = (lo.z, hi.x, hi.y, hi.z).
+struct SortedEntry {
+    float4 a, b;
+};
+
 struct FixedAccumulator {
     long long x, y, z;
     uint32_t count;
@@ -306,69 +314,92 @@ __device__ void forEntryCandidates(uint32_t k, const SelfGrid& g, const float4* 
     }
 }
 
-// Candidate enumeration over the sorted entries with the boxes and segment
-// endpoints gathered into sorted order (the run of a cell is then read
-// sequentially) and the two-hop knot lists of both endpoints held in
-// registers (W slots each, padded with ~0u): the locality test of a pair is
-// 4W register compares instead of four binary searches in memory. Same set
-// and order of candidates as forEntryCandidates.
+// Candidate enumeration over the gathered sorted entries (one 32-byte record
+// per entry, read four at a time so their loads overlap) with the two-hop
+// knot lists of both endpoints held in registers (W slots each, padded with
+// ~0u): the locality test of a pair is 4W register compares instead of four
+// binary searches in memory. Same set and order of candidates as
+// forEntryCandidates.
 template <uint32_t W, typename Emit>
-__device__ void forSortedEntryCandidates(uint32_t k, const SelfGrid& g, const float4* __restrict__ sortedLo,
-                                         const float4* __restrict__ sortedHi, const uint2* __restrict__ sortedIJ,
-                                         const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
-                                         const uint32_t* __restrict__ localFixed, Emit emit) {
+__device__ void forSortedEntryCandidates(uint32_t k, const SelfGrid& g, const SortedEntry* __restrict__ entries,
+                                         const uint2* __restrict__ sortedIJ, const uint32_t* __restrict__ localFixed,
+                                         Emit emit) {
     const uint32_t n = g.sortCount;
-    const uint32_t id = keys[k];
-    const uint32_t vs = vals[k];
+    const SortedEntry self = entries[k];
+    const uint32_t id = __float_as_uint(self.a.x);
+    const uint32_t vs = __float_as_uint(self.a.y);
     const uint32_t s = vs & kEntrySegmentMask;
-    const float4 lo = sortedLo[k], hi = sortedHi[k];
-    const uint2 es = sortedIJ[k];
+    const float4 lo = make_float4(self.a.z, self.a.w, self.b.x, 0.0f);
+    const float4 hi = make_float4(self.b.y, self.b.z, self.b.w, 0.0f);
     uint32_t na[W], nb[W];
     bool loaded = false;
-    for (uint32_t j = k + 1; j < n && keys[j] == id; ++j) {
-        const uint32_t vt = vals[j];
-        if (((vs | vt) >> kEntrySegmentBits) != 7u) continue;  // not the owner cell
-        const float4 tLo = sortedLo[j], tHi = sortedHi[j];
-        if (!boxesOverlap(lo, hi, tLo, tHi)) continue;
-        if (!loaded) {
-            const uint4* ra = reinterpret_cast<const uint4*>(localFixed + static_cast<size_t>(es.x) * W);
-            const uint4* rb = reinterpret_cast<const uint4*>(localFixed + static_cast<size_t>(es.y) * W);
+    constexpr uint32_t kUnroll = 4;
+    for (uint32_t j0 = k + 1; j0 < n; j0 += kUnroll) {
+        SortedEntry e[kUnroll];
 #pragma unroll
-            for (uint32_t m = 0; m < W / 4; ++m) {
-                const uint4 va = ra[m], vb = rb[m];
-                na[4 * m] = va.x;
-                na[4 * m + 1] = va.y;
-                na[4 * m + 2] = va.z;
-                na[4 * m + 3] = va.w;
-                nb[4 * m] = vb.x;
-                nb[4 * m + 1] = vb.y;
-                nb[4 * m + 2] = vb.z;
-                nb[4 * m + 3] = vb.w;
+        for (uint32_t u = 0; u < kUnroll; ++u)
+            if (j0 + u < n) e[u] = entries[j0 + u];
+        bool stop = false;
+#pragma unroll
+        for (uint32_t u = 0; u < kUnroll; ++u) {
+            if (stop) continue;
+            const uint32_t j = j0 + u;
+            if (j >= n || __float_as_uint(e[u].a.x) != id) {
+                stop = true;
+                continue;
             }
-            loaded = true;
-        }
-        const uint2 et = sortedIJ[j];
-        bool local = false;
+            const uint32_t vt = __float_as_uint(e[u].a.y);
+            if (((vs | vt) >> kEntrySegmentBits) != 7u) continue;  // not the owner cell
+            const float4 tLo = make_float4(e[u].a.z, e[u].a.w, e[u].b.x, 0.0f);
+            const float4 tHi = make_float4(e[u].b.y, e[u].b.z, e[u].b.w, 0.0f);
+            if (!boxesOverlap(lo, hi, tLo, tHi)) continue;
+            if (!loaded) {
+                const uint2 es = sortedIJ[k];
+                const uint4* ra = reinterpret_cast<const uint4*>(localFixed + static_cast<size_t>(es.x) * W);
+                const uint4* rb = reinterpret_cast<const uint4*>(localFixed + static_cast<size_t>(es.y) * W);
 #pragma unroll
-        for (uint32_t m = 0; m < W; ++m)
-            local |= na[m] == et.x || na[m] == et.y || nb[m] == et.x || nb[m] == et.y;
-        if (local) continue;
-        const uint32_t t = vt & kEntrySegmentMask;
-        emit(min(s, t), max(s, t));
+                for (uint32_t m = 0; m < W / 4; ++m) {
+                    const uint4 va = ra[m], vb = rb[m];
+                    na[4 * m] = va.x;
+                    na[4 * m + 1] = va.y;
+                    na[4 * m + 2] = va.z;
+                    na[4 * m + 3] = va.w;
+                    nb[4 * m] = vb.x;
+                    nb[4 * m + 1] = vb.y;
+                    nb[4 * m + 2] = vb.z;
+                    nb[4 * m + 3] = vb.w;
+                }
+                loaded = true;
+            }
+            const uint2 et = sortedIJ[j];
+            bool local = false;
+#pragma unroll
+            for (uint32_t m = 0; m < W; ++m)
+                local |= na[m] == et.x || na[m] == et.y || nb[m] == et.x || nb[m] == et.y;
+            if (local) continue;
+            const uint32_t t = vt & kEntrySegmentMask;
+            emit(min(s, t), max(s, t));
+        }
+        if (stop) break;
     }
 }
 
-// Sorted-order copies of each entry's box and segment endpoints.
+// Sorted-order records of each entry (cell, value, box) and its segment's
+// endpoints.
 __global__ void k_self_gather(const float4* __restrict__ boxLo, const float4* __restrict__ boxHi,
                               const uint2* __restrict__ ij, const SelfGrid* __restrict__ grid,
-                              const uint32_t* __restrict__ vals, float4* __restrict__ sortedLo,
-                              float4* __restrict__ sortedHi, uint2* __restrict__ sortedIJ) {
+                              const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                              SortedEntry* __restrict__ entries, uint2* __restrict__ sortedIJ) {
     if (!grid->valid) return;
     const uint32_t n = grid->sortCount;
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-        const uint32_t s = vals[k] & kEntrySegmentMask;
-        sortedLo[k] = boxLo[s];
-        sortedHi[k] = boxHi[s];
+        const uint32_t v = vals[k];
+        const uint32_t s = v & kEntrySegmentMask;
+        const float4 lo = boxLo[s], hi = boxHi[s];
+        SortedEntry e;
+        e.a = make_float4(__uint_as_float(keys[k]), __uint_as_float(v), lo.x, lo.y);
+        e.b = make_float4(lo.z, hi.x, hi.y, hi.z);
+        entries[k] = e;
         sortedIJ[k] = ij[s];
     }
 }
@@ -377,8 +408,7 @@ struct PairArgs {
     const float4* boxLo;
     const float4* boxHi;
     const uint2* ij;
-    const float4* sortedLo;
-    const float4* sortedHi;
+    const SortedEntry* entries;
     const uint2* sortedIJ;
     SelfGrid* grid;
     const uint32_t* keys;
@@ -402,7 +432,7 @@ __device__ __forceinline__ void entryCandidates(uint32_t k, const SelfGrid& g, c
     if constexpr (W == 0) {
         forEntryCandidates(k, g, a.boxLo, a.boxHi, a.ij, a.keys, a.vals, a.localStart, a.localList, emit);
     } else {
-        forSortedEntryCandidates<W>(k, g, a.sortedLo, a.sortedHi, a.sortedIJ, a.keys, a.vals, a.localFixed, emit);
+        forSortedEntryCandidates<W>(k, g, a.entries, a.sortedIJ, a.localFixed, emit);
     }
 }
 

@@ -147,7 +147,7 @@ struct NumiClothHipSolver {
     DeviceArray<uint32_t> localStart, localList;
     DeviceArray<uint32_t> localFixed;   // two-hop lists in localWidth slots per knot (0: not used)
     uint32_t localWidth = 0;
-    DeviceArray<float4> sortedLo, sortedHi;
+    DeviceArray<SortedEntry> sortedEntries;
     DeviceArray<uint2> sortedIJ;
     DeviceArray<float4> boxLo, boxHi, partialLo, partialHi;
     DeviceArray<SelfGrid> grid;
@@ -418,8 +418,7 @@ struct NumiClothHipSolver {
         pa.boxLo = boxLo.ptr;
         pa.boxHi = boxHi.ptr;
         pa.ij = dIJ.ptr;
-        pa.sortedLo = sortedLo.ptr;
-        pa.sortedHi = sortedHi.ptr;
+        pa.entries = sortedEntries.ptr;
         pa.sortedIJ = sortedIJ.ptr;
         pa.grid = g;
         pa.keys = keysIn;
@@ -435,8 +434,8 @@ struct NumiClothHipSolver {
         pa.impulses = selfImpulses.ptr;
         pa.failure = failure.ptr;
         if (localWidth != 0) {
-            k_self_gather<<<entryBlocks, kScanThreads, 0, stream>>>(boxLo.ptr, boxHi.ptr, dIJ.ptr, g, valsIn,
-                                                                   sortedLo.ptr, sortedHi.ptr, sortedIJ.ptr);
+            k_self_gather<<<entryBlocks, kScanThreads, 0, stream>>>(boxLo.ptr, boxHi.ptr, dIJ.ptr, g, keysIn, valsIn,
+                                                                   sortedEntries.ptr, sortedIJ.ptr);
         }
         auto counts = localWidth == 16 ? k_self_pair_counts<16> : (localWidth == 32 ? k_self_pair_counts<32>
                                                                                      : k_self_pair_counts<0>);
@@ -1161,8 +1160,7 @@ void setupSelfContact(const NumiClothHipSceneDesc& scene, NumiClothHipSolver& s)
     s.keysA.allocate(s.entryCapacity);
     s.valsA.allocate(s.entryCapacity);
     if (s.localWidth != 0) {
-        s.sortedLo.allocate(s.entryCapacity);
-        s.sortedHi.allocate(s.entryCapacity);
+        s.sortedEntries.allocate(s.entryCapacity);
         s.sortedIJ.allocate(s.entryCapacity);
     }
     s.keysB.allocate(s.entryCapacity);
