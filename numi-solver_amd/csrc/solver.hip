@@ -1666,16 +1666,33 @@ struct AdjointBuffers {
     DeviceArray<double> times;
     DeviceArray<uint32_t> tracked;
     DeviceArray<float> targets;
-    DeviceArray<float4> checkpointPos, checkpointVel, scratchPos, scratchPrev, scratchVel;
-    DeviceArray<float> scratchLambdaD, scratchLambdaB;
-    DeviceArray<float4> tapeX, tapeV0;
-    DeviceArray<float> tapeLambdaD, tapeLambdaB, lossGrad;
-    DeviceArray<uint32_t> tapeActive;
+    DeviceArray<float4> checkpointPos, checkpointVel;
+    // One segment's recomputation: scratch state and the tape it records.
+    // Two sets, so that the recomputation of segment k - 1 (side stream) runs
+    // while segment k is reversed (solver stream).
+    struct TapeSet {
+        DeviceArray<float4> scratchPos, scratchPrev, scratchVel;
+        DeviceArray<float> scratchLambdaD, scratchLambdaB;
+        DeviceArray<float4> tapeX, tapeV0;
+        DeviceArray<float> tapeLambdaD, tapeLambdaB, lossGrad;
+        DeviceArray<uint32_t> tapeActive;
+    };
+    TapeSet sets[2];
+    hipStream_t side = nullptr;                 // default priority, like the solver stream
+    hipEvent_t recorded[2] = {nullptr, nullptr};  // set k's tape is written
+    hipEvent_t released[2] = {nullptr, nullptr};  // set k's tape has been reversed
     DeviceArray<float4> gxState, gvState;
     DeviceArray<float> gradInverseMass, gradDamping, gradDistance, gradBend;
     DeviceArray<double> loss;
     std::vector<uint32_t> trackedFirst, trackedCount;
     bool ready = false;
+    ~AdjointBuffers() {
+        for (int k = 0; k < 2; ++k) {
+            if (recorded[k] != nullptr) (void)hipEventDestroy(recorded[k]);
+            if (released[k] != nullptr) (void)hipEventDestroy(released[k]);
+        }
+        if (side != nullptr) (void)hipStreamDestroy(side);
+    }
 };
 
 struct NumiClothHipBatch {
@@ -1873,11 +1890,19 @@ void setupAdjoint(NumiClothHipBatch& b) {
     A.gradDistance.allocate(std::max(s.D, 1u));
     A.gradBend.allocate(std::max(s.B, 1u));
     A.loss.allocate(S);
-    A.scratchPos.allocate(s.P);
-    A.scratchPrev.allocate(s.P);
-    A.scratchVel.allocate(s.P);
-    A.scratchLambdaD.allocate(std::max(s.D, 1u));
-    A.scratchLambdaB.allocate(std::max(s.B, 1u));
+    for (AdjointBuffers::TapeSet& set : A.sets) {
+        set.scratchPos.allocate(s.P);
+        set.scratchPrev.allocate(s.P);
+        set.scratchVel.allocate(s.P);
+        set.scratchLambdaD.allocate(std::max(s.D, 1u));
+        set.scratchLambdaB.allocate(std::max(s.B, 1u));
+    }
+    HIP_OK(hipSetDevice(s.device));
+    HIP_OK(hipStreamCreateWithFlags(&A.side, hipStreamNonBlocking));
+    for (int k = 0; k < 2; ++k) {
+        HIP_OK(hipEventCreateWithFlags(&A.recorded[k], hipEventDisableTiming));
+        HIP_OK(hipEventCreateWithFlags(&A.released[k], hipEventDisableTiming));
+    }
     HIP_OK(hipStreamSynchronize(st));
     A.ready = true;
 }
@@ -1947,12 +1972,16 @@ void runAdjoint(NumiClothHipBatch& b, const NumiClothHipAdjointProblem& pr) {
     const uint32_t passes = adjointPasses(A, pr.iterations, pr.strainSweeps);
     A.checkpointPos.allocate(static_cast<size_t>(segments) * s.P);
     A.checkpointVel.allocate(static_cast<size_t>(segments) * s.P);
-    A.tapeX.allocate(static_cast<size_t>(L) * passes * s.P);
-    A.tapeV0.allocate(static_cast<size_t>(L) * s.P);
-    A.tapeLambdaD.allocate(std::max<size_t>(static_cast<size_t>(L) * pr.iterations * s.D, 1));
-    A.tapeLambdaB.allocate(std::max<size_t>(static_cast<size_t>(L) * ((pr.iterations + 1) / 2) * s.B, 1));
-    A.tapeActive.allocate(std::max<size_t>(static_cast<size_t>(L) * passes * S, 1));
-    A.lossGrad.allocate(std::max<size_t>(static_cast<size_t>(L) * S * Tk * 3, 1));
+    const uint32_t setsUsed = segments > 1 ? 2u : 1u;
+    for (uint32_t k = 0; k < setsUsed; ++k) {
+        AdjointBuffers::TapeSet& set = A.sets[k];
+        set.tapeX.allocate(static_cast<size_t>(L) * passes * s.P);
+        set.tapeV0.allocate(static_cast<size_t>(L) * s.P);
+        set.tapeLambdaD.allocate(std::max<size_t>(static_cast<size_t>(L) * pr.iterations * s.D, 1));
+        set.tapeLambdaB.allocate(std::max<size_t>(static_cast<size_t>(L) * ((pr.iterations + 1) / 2) * s.B, 1));
+        set.tapeActive.allocate(std::max<size_t>(static_cast<size_t>(L) * passes * S, 1));
+        set.lossGrad.allocate(std::max<size_t>(static_cast<size_t>(L) * S * Tk * 3, 1));
+    }
     HIP_OK(hipMemsetAsync(A.loss.ptr, 0, S * sizeof(double), st));
     HIP_OK(hipMemsetAsync(A.gxState.ptr, 0, s.P * sizeof(float4), st));
     HIP_OK(hipMemsetAsync(A.gvState.ptr, 0, s.P * sizeof(float4), st));
@@ -1990,12 +2019,6 @@ void runAdjoint(NumiClothHipBatch& b, const NumiClothHipAdjointProblem& pr) {
     a.particles = s.P;
     a.distances = s.D;
     a.bends = s.B;
-    a.tapeX = A.tapeX.ptr;
-    a.tapeLambdaD = A.tapeLambdaD.ptr;
-    a.tapeLambdaB = A.tapeLambdaB.ptr;
-    a.tapeV0 = A.tapeV0.ptr;
-    a.tapeActive = A.tapeActive.ptr;
-    a.lossGrad = A.lossGrad.ptr;
     a.specimenCount = S;
     a.failure = s.failure.ptr;
     a.gxState = A.gxState.ptr;
@@ -2025,28 +2048,50 @@ void runAdjoint(NumiClothHipBatch& b, const NumiClothHipAdjointProblem& pr) {
         HIP_OK(hipGetLastError());
     }
     // Reverse, segment by segment: recompute the segment from its checkpoint
-    // with the tape on (scratch state), then run it backwards.
+    // with the tape on (scratch state), then run it backwards. The
+    // recomputations run on the side stream into alternating tape sets, one
+    // segment ahead of the reverse on the solver stream: recomputing segment
+    // k waits for the checkpoints and for the reverse that last read its set
+    // (segment k + 2); reversing segment k waits for its recomputation.
+    HIP_OK(hipEventRecord(A.released[0], st));  // checkpoints written
+    HIP_OK(hipEventRecord(A.released[1], st));
     for (uint32_t k = segments; k-- > 0;) {
-        HIP_OK(hipMemcpyAsync(A.scratchPos.ptr, A.checkpointPos.ptr + static_cast<size_t>(k) * s.P,
-                              s.P * sizeof(float4), hipMemcpyDeviceToDevice, st));
-        HIP_OK(hipMemcpyAsync(A.scratchVel.ptr, A.checkpointVel.ptr + static_cast<size_t>(k) * s.P,
-                              s.P * sizeof(float4), hipMemcpyDeviceToDevice, st));
-        HIP_OK(hipMemcpyAsync(A.scratchPrev.ptr, s.prev.ptr, s.P * sizeof(float4), hipMemcpyDeviceToDevice, st));
+        const uint32_t which = (segments - 1 - k) % setsUsed;
+        AdjointBuffers::TapeSet& set = A.sets[which];
+        hipStream_t rs = setsUsed > 1 ? A.side : st;
+        HIP_OK(hipStreamWaitEvent(rs, A.released[which], 0));
+        HIP_OK(hipMemcpyAsync(set.scratchPos.ptr, A.checkpointPos.ptr + static_cast<size_t>(k) * s.P,
+                              s.P * sizeof(float4), hipMemcpyDeviceToDevice, rs));
+        HIP_OK(hipMemcpyAsync(set.scratchVel.ptr, A.checkpointVel.ptr + static_cast<size_t>(k) * s.P,
+                              s.P * sizeof(float4), hipMemcpyDeviceToDevice, rs));
+        // prev is written before it is read; its w lane (the mass) is kept.
+        HIP_OK(hipMemcpyAsync(set.scratchPrev.ptr, s.prev.ptr, s.P * sizeof(float4), hipMemcpyDeviceToDevice, rs));
         SpecimenArgs f = a;
         f.firstSubstep = k * L;
         f.substeps = std::min(L, T - k * L);
         f.record = 1u;
-        f.pos = A.scratchPos.ptr;
-        f.prev = A.scratchPrev.ptr;
-        f.vel = A.scratchVel.ptr;
-        f.dLambda = A.scratchLambdaD.ptr;
-        f.bLambda = A.scratchLambdaB.ptr;
-        k_specimen_forward<<<S, kSpecimenThreads, A.ldsBytes, st>>>(f);
+        f.pos = set.scratchPos.ptr;
+        f.prev = set.scratchPrev.ptr;
+        f.vel = set.scratchVel.ptr;
+        f.dLambda = set.scratchLambdaD.ptr;
+        f.bLambda = set.scratchLambdaB.ptr;
+        f.tapeX = set.tapeX.ptr;
+        f.tapeLambdaD = set.tapeLambdaD.ptr;
+        f.tapeLambdaB = set.tapeLambdaB.ptr;
+        f.tapeV0 = set.tapeV0.ptr;
+        f.tapeActive = set.tapeActive.ptr;
+        f.lossGrad = set.lossGrad.ptr;
+        k_specimen_forward<<<S, kSpecimenThreads, A.ldsBytes, rs>>>(f);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipEventRecord(A.recorded[which], rs));
+        HIP_OK(hipStreamWaitEvent(st, A.recorded[which], 0));
         f.pos = s.pos.ptr;  // inverse masses for the reverse
         k_specimen_reverse<<<S, kSpecimenThreads, A.ldsBytes, st>>>(f);
         HIP_OK(hipGetLastError());
+        HIP_OK(hipEventRecord(A.released[which], st));
     }
     HIP_OK(hipStreamSynchronize(st));
+    HIP_OK(hipStreamSynchronize(A.side));
     s.time = t;
     ++b.version;
 }
