@@ -17,6 +17,12 @@ Workloads (synthetic, seeded with SplitMix64 / numpy PCG64 from SEED):
              stretch compliance 1e-7, bend 1e-4, strain limit 0.10, ground
              on, yarn/yarn self-contact and friction, cuff grip lifted 0.4 m
              over 2 s (smoothstep) then held, 48 substeps per 1/120 s frame.
+  pickup     the reference's pickup replay of the same bag (default
+             materials, self-contact): grip lifted 1.25 m over 0.8 s, snapped
+             at 1.0 s, recovered at 1.45 s, 48 substeps of 1/5760 s per frame,
+             the state read back and FNV-hashed after every frame as the
+             reference's frame loop does; vs_baseline against the reference's
+             Metal pickup time.
   strips512  512 hanging strips, width U[8,64] x height U[64,256] knots,
              compliances log-U[1e-8,1e-5], density U[0.1,0.4] kg/m^2, top row
              driven by a 0.5-2 Hz vertical sinusoid, one batch (merged scene,
@@ -64,7 +70,10 @@ SHEETS = {
     "sheet1024": dict(width=1024, height=1024, substeps=16, frame=1.0 / 60.0, sphereMode=2, sphereCount=64,
                       sphereRadiusMin=0.02, sphereRadiusMax=0.10),
 }
-WORKLOADS = sorted([*SHEETS, "bag", "strips512", "adjoint64"])
+WORKLOADS = sorted([*SHEETS, "bag", "pickup", "strips512", "adjoint64"])
+# Reference Metal pickup replay: 480 frames x 48 substeps of the 1,465-knot
+# bag in 7086.04 s of GPU command time on an Apple M4 Pro.
+PICKUP_REFERENCE = 1465 * 480 * 48 / 7086.04
 
 
 def _hip():
@@ -313,6 +322,44 @@ def run_plain(args, cloth, hip) -> dict:
             "data": "synthetic (seeded scene, SplitMix64 seed %d)" % SEED}
 
 
+def run_pickup(args, cloth, hip) -> dict:
+    c = cloth.cloth
+    scene = cloth.Scene.bag_replay(c.REPLAY_PICKUP, selfContact=1)
+    particles = len(scene.particles)
+    solver = cloth.Solver(scene)
+    if args.warmup:
+        solver.replay(c.REPLAY_PICKUP, args.warmup, ITERATIONS, STRAIN_SWEEPS)
+    # The timed frames continue the replay's clock from a fresh solver so the
+    # trajectory is the reference's from t = 0.
+    solver.close()
+    solver = cloth.Solver(scene)
+    solver.synchronize()
+    assert hip.hipDeviceSynchronize() == 0
+    timer = Timer(hip, solver.stream)
+    timer.begin()
+    result = solver.replay(c.REPLAY_PICKUP, args.steps, ITERATIONS, STRAIN_SWEEPS)
+    seconds = timer.end()
+    if not result["failureFree"]:
+        raise RuntimeError("pickup replay raised a failure bit")
+    line, tile_config = roofline(cloth, solver, args.workload)
+    solver.close()
+    baseline = None
+    if not args.no_cpu_baseline:
+        baseline = cpu_baseline(scene, c.replay_frame_configs(scene.config, c.REPLAY_PICKUP, 0), 48)
+    value = particles * 48 * args.steps / seconds
+    config = {"workload": "pickup", "particles": particles, "substeps_per_frame": 48, "iterations": ITERATIONS,
+              "strain_sweeps": STRAIN_SWEEPS, "frames": args.steps, "self_contact": True,
+              "frame_seconds": 48 * c.REPLAY_TIMESTEP, "wall_seconds_timed": round(timer.wall, 3),
+              "frame_hashes": [hex(h) for h in result["hashes"]], "maximum_handle_lag_m": result["maximumHandleLag"],
+              "released_mask": result["releasedMask"], "tile_programs": tile_config,
+              "reference": "Metal pickup replay, 480 frames x 48 substeps, 7086.04 s GPU command time on an Apple "
+                           "M4 Pro -> %.1f particle-substeps/s" % PICKUP_REFERENCE}
+    return {"metric": "particle-substeps/s", "value": value, "unit": "particle-substeps/s",
+            "ms_per_step": seconds * 1000.0 / args.steps, "dtype": "f32", "config": config, "roofline": line,
+            "cpu_baseline": baseline, "vs_baseline": value / PICKUP_REFERENCE,
+            "data": "reference pickup trajectory (deterministic), default bag materials"}
+
+
 def run_strips(args, cloth, hip) -> dict:
     specimens = strip_specimens(cloth)
     configs = specimens[0].configs(48)
@@ -436,11 +483,13 @@ def main() -> int:
         result = run_strips(args, cloth, hip)
     elif args.workload == "adjoint64":
         result = run_adjoint(args, cloth, hip)
+    elif args.workload == "pickup":
+        result = run_pickup(args, cloth, hip)
     else:
         result = run_plain(args, cloth, hip)
     line = {"metric": result["metric"], "value": result["value"], "unit": result["unit"], "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": result["ms_per_step"],
-            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": result["dtype"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": result.get("vs_baseline"), "dtype": result["dtype"],
             "data": result["data"], "config": result["config"], "roofline": result["roofline"],
             "cpu_baseline": result["cpu_baseline"]}
     if "diagnostics" in result:

@@ -37,7 +37,8 @@ enum NumiClothHipStatus {
     NUMI_CLOTH_HIP_ERROR_DEVICE = 3,
     NUMI_CLOTH_HIP_ERROR_ALLOCATION = 4,
     NUMI_CLOTH_HIP_ERROR_UNSUPPORTED = 5,
-    NUMI_CLOTH_HIP_ERROR_PARSE = 6
+    NUMI_CLOTH_HIP_ERROR_PARSE = 6,
+    NUMI_CLOTH_HIP_ERROR_IO = 7      /* a file could not be opened or written */
 };
 
 /*

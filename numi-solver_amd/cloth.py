@@ -26,7 +26,7 @@ FAILURE_NONFINITE = 8
 FAILURE_GRIP_CAPTURE = 16
 
 STATUS_NAMES = {0: "ok", 1: "invalid argument", 2: "topology", 3: "device", 4: "allocation",
-                5: "unsupported", 6: "parse"}
+                5: "unsupported", 6: "parse", 7: "io"}
 
 _u4 = ("<u4", (4,))
 _f4 = ("<f4", (4,))
@@ -184,6 +184,23 @@ class SpecimenGradients(C.Structure):
                 ("inverseMass", C.c_void_p), ("damping", C.c_void_p)]
 
 
+class ReplayPose(C.Structure):
+    _fields_ = [("target", C.c_double * 3), ("orientation", C.c_double * 4), ("active", C.c_uint32),
+                ("attachmentGeneration", C.c_uint32)]
+
+
+class ReplayResult(C.Structure):
+    _fields_ = [("frameHashes", C.c_void_p), ("framesRun", C.c_uint32), ("failureFree", C.c_uint32),
+                ("releasedMask", C.c_uint32), ("reserved", C.c_uint32), ("maximumHandleLag", C.c_double),
+                ("stepSeconds", C.c_double)]
+
+
+# Trajectory replay scenarios (include/numi_cloth_replay.h).
+REPLAY_GROUNDED, REPLAY_SPIN, REPLAY_PICKUP, REPLAY_RECORDED = 0, 1, 2, 3
+REPLAY_SUBSTEPS_PER_FRAME = 48
+REPLAY_TIMESTEP = float(np.float32(1.0) / np.float32(5760.0))
+
+
 class BagSpec(C.Structure):
     _fields_ = [("material", MaterialValues), ("stretchComplianceOverride", C.c_float),
                 ("bendComplianceOverride", C.c_float), ("strainLimitOverride", C.c_float),
@@ -247,6 +264,13 @@ def library() -> C.CDLL:
         "numi_cloth_scene_destroy": ([vp], None),
         "numi_cloth_splitmix64": ([C.POINTER(C.c_uint64)], C.c_uint64),
         "numi_cloth_fnv1a64": ([vp, C.c_size_t, C.c_uint64], C.c_uint64),
+        "numi_cloth_replay_pose": ([u32, C.c_double, vp, C.POINTER(ReplayPose)], i32),
+        "numi_cloth_replay_prepare": ([vp, u32, vp], i32),
+        "numi_cloth_replay_frame_configs": ([vp, u32, vp, u32, u32, vp], i32),
+        "numi_cloth_state_hash": ([C.POINTER(SceneDesc), C.POINTER(State), C.POINTER(C.c_uint64)], i32),
+        "numi_cloth_obj_write": ([C.c_char_p, vp, u32, vp, u32, vp, u32, vp], i32),
+        "numi_cloth_replay_run": ([vp, C.POINTER(SceneDesc), u32, vp, u32, u32, u32, u32, C.c_char_p,
+                                   C.POINTER(ReplayResult)], i32),
     }
     for name, (args, result) in signatures.items():
         function = getattr(lib, name)
@@ -431,6 +455,24 @@ class Scene:
             setattr(spec, key, value)
         handle = C.c_void_p()
         _check(library().numi_cloth_scene_bag(C.byref(spec), C.byref(handle)))
+        return cls._from_builder(handle)
+
+    @classmethod
+    def bag_replay(cls, scenario: int, trajectory: "GripTrajectory | None" = None,
+                   material: MaterialArtifact | None = None, **fields) -> "Scene":
+        """The bag scene prepared as a trajectory replay's initial state
+        (numi_cloth_replay_prepare: spin lift, ground flag, grip pose at t = 0)."""
+        spec = BagSpec()
+        spec.material = (material or material_defaults()).values
+        for key, value in fields.items():
+            setattr(spec, key, value)
+        handle = C.c_void_p()
+        lib = library()
+        _check(lib.numi_cloth_scene_bag(C.byref(spec), C.byref(handle)))
+        status = lib.numi_cloth_replay_prepare(handle, scenario, trajectory._handle if trajectory else None)
+        if status != 0:
+            lib.numi_cloth_scene_destroy(handle)
+            _check(status)
         return cls._from_builder(handle)
 
     @classmethod
@@ -638,6 +680,22 @@ class Solver:
         out["failure"] = int(out["failure"][0])
         return out
 
+    def replay(self, scenario: int, frames: int, iterations: int = 32, strain_sweeps: int = 3,
+               trajectory: "GripTrajectory | None" = None, dump_every: int = 0,
+               dump_prefix: str | None = None) -> dict:
+        """Frames of a trajectory replay on this solver (numi_cloth_replay_run);
+        the solver must have been created from Scene.bag_replay(scenario)."""
+        hashes = np.zeros(max(frames, 1), np.uint64)
+        out = ReplayResult()
+        out.frameHashes = hashes.ctypes.data
+        _check(library().numi_cloth_replay_run(self._handle, C.byref(self.scene.desc), scenario,
+                                               trajectory._handle if trajectory else None, frames, iterations,
+                                               strain_sweeps, dump_every,
+                                               dump_prefix.encode() if dump_prefix else None, C.byref(out)))
+        return {"hashes": [int(h) for h in hashes[:out.framesRun]], "framesRun": out.framesRun,
+                "failureFree": bool(out.failureFree), "releasedMask": out.releasedMask,
+                "maximumHandleLag": out.maximumHandleLag, "stepSeconds": out.stepSeconds}
+
     def time_pass(self, kind: int, colour: int = 0, repeats: int = 50) -> PassTiming:
         out = PassTiming()
         _check(library().numi_cloth_hip_time_pass(self._handle, kind, colour, repeats, C.byref(out)))
@@ -667,6 +725,47 @@ class Solver:
         out = Diagnostics()
         _check(library().numi_cloth_hip_diagnostics(self._handle, C.byref(out)))
         return out
+
+
+def replay_pose(scenario: int, time_seconds: float, trajectory: "GripTrajectory | None" = None) -> ReplayPose:
+    out = ReplayPose()
+    _check(library().numi_cloth_replay_pose(scenario, time_seconds, trajectory._handle if trajectory else None,
+                                            C.byref(out)))
+    return out
+
+
+def replay_frame_configs(config: np.ndarray, scenario: int, frame: int,
+                         trajectory: "GripTrajectory | None" = None,
+                         substeps: int = REPLAY_SUBSTEPS_PER_FRAME) -> np.ndarray:
+    """The substep configs of one replay frame (numi_cloth_replay_frame_configs)."""
+    base = np.ascontiguousarray(config, dtype=CONFIG).reshape(1)
+    out = np.zeros(substeps, CONFIG)
+    _check(library().numi_cloth_replay_frame_configs(base.ctypes.data, scenario,
+                                                     trajectory._handle if trajectory else None, frame, substeps,
+                                                     out.ctypes.data))
+    return out
+
+
+def frame_hash(scene: "Scene", state: dict) -> int:
+    """FNV-1a 64 of a read state in the reference's frame-hash order
+    (numi_cloth_state_hash)."""
+    s = State()
+    arrays = dict(state)
+    arrays["failure"] = np.array([state["failure"]], np.uint32)
+    for name in ["particles", "distances", "grips", "knots", "bends", "fruits", "fruitPairs", "yarnContacts",
+                 "selfStatus", "frictionStatus", "aerodynamicsStatus", "releaseStatus", "failure",
+                 "mouthClearance"]:
+        setattr(s, name, _ptr(arrays[name]))
+    out = C.c_uint64()
+    _check(library().numi_cloth_state_hash(C.byref(scene.desc), C.byref(s), C.byref(out)))
+    return int(out.value)
+
+
+def write_obj(path, particles: np.ndarray, fruits: np.ndarray, grips: np.ndarray, config: np.ndarray) -> None:
+    """Renderer OBJ of a bag state (numi_cloth_obj_write)."""
+    base = np.ascontiguousarray(config, dtype=CONFIG).reshape(1)
+    _check(library().numi_cloth_obj_write(str(path).encode(), _ptr(particles), len(particles), _ptr(fruits),
+                                          len(fruits), _ptr(grips), len(grips), base.ctypes.data))
 
 
 def state_hash(arrays: dict) -> int:

@@ -745,6 +745,20 @@ void buildBag(const NumiClothHipBagSpec& spec, NumiClothHipScene& s) {
 }
 
 }  // namespace
+
+void bagAuthoredPosition(uint32_t level, uint32_t ring, double out[3]) {
+    const D3 p = wallPosition(level, ring);
+    out[0] = p.x;
+    out[1] = p.y;
+    out[2] = p.z;
+}
+
+uint32_t bagWallIndex(uint32_t level, uint32_t ring) { return wallIndex(level, ring); }
+
+uint32_t bagGridIndex(uint32_t row, uint32_t column) { return gridIndex(row, column); }
+
+static_assert(kBagAround == kAround && kBagLevels == kLevels && kBagGrid == kGrid, "bag geometry");
+
 }  // namespace numi_hip
 
 using namespace numi_hip;

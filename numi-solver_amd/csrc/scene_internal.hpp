@@ -68,3 +68,21 @@ struct NumiClothHipScene {
         desc.extension.particleMotion = motion.empty() ? nullptr : motion.data();
     }
 };
+
+namespace numi_hip {
+
+// Produce-bag geometry shared by the scene builder and the frame driver
+// (reference tools/cloth_bag_metal.mm:33-38, :302-440): 48 knots around,
+// 28 wall levels, a 13 x 13 bottom grid whose boundary is ring 0 of level 0.
+constexpr uint32_t kBagAround = 48;
+constexpr uint32_t kBagLevels = 28;
+constexpr uint32_t kBagGrid = 13;
+constexpr uint32_t kBagKnots = kBagAround * kBagLevels + (kBagGrid - 2) * (kBagGrid - 2);
+
+// Authored (rest) position of wall knot (level, ring), metres.
+void bagAuthoredPosition(uint32_t level, uint32_t ring, double out[3]);
+// Knot index of wall knot (level, ring) and of bottom-grid node (row, column).
+uint32_t bagWallIndex(uint32_t level, uint32_t ring);
+uint32_t bagGridIndex(uint32_t row, uint32_t column);
+
+}  // namespace numi_hip
