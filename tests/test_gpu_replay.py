@@ -88,8 +88,8 @@ def test_recorded_replay_and_obj_dumps(cloth, golden, tmp_path):
         result = solver.replay(c.REPLAY_RECORDED, 1, ITERATIONS, SWEEPS, trajectory=trajectory, dump_every=1,
                                dump_prefix=prefix)
         state = solver.read()
-        with pytest.raises(cloth.ClothError):
-            solver.replay(c.REPLAY_RECORDED, 2, ITERATIONS, SWEEPS, trajectory=trajectory)
+    with cloth.Solver(scene) as solver, pytest.raises(cloth.ClothError):
+        solver.replay(c.REPLAY_RECORDED, 2, ITERATIONS, SWEEPS, trajectory=trajectory)
     assert result["failureFree"] and result["framesRun"] == 1
     names = sorted(p.name for p in tmp_path.iterdir())
     assert names == ["recorded-0.obj", "recorded-1.obj"]
