@@ -140,6 +140,7 @@ __global__ void __launch_bounds__(kSpecimenThreads) k_specimen_forward(SpecimenA
         const uint32_t s = a.firstSubstep + j;
         const NumiClothBagGPUConfig& cfg = a.configs[s];
         const float dt = cfg.gravityAndTimestep.w;
+        const float invDt2 = 1.0f / (dt * dt);
         const V3 g = xyz(cfg.gravityAndTimestep);
         const double endTime = a.times[s];
 #pragma unroll
@@ -180,7 +181,7 @@ __global__ void __launch_bounds__(kSpecimenThreads) k_specimen_forward(SpecimenA
                     const uint2 e = a.ij[k];
                     float lam = lamD[r.z + t];
                     float4 pa = x[e.x - k0], pb = x[e.y - k0];
-                    if (projectDistance(pa, pb, a.restCompliance[k], lam, dt, a.failure)) {
+                    if (projectDistance(pa, pb, a.restCompliance[k], lam, invDt2, a.failure)) {
                         x[e.x - k0] = pa;
                         x[e.y - k0] = pb;
                         lamD[r.z + t] = lam;
@@ -207,7 +208,7 @@ __global__ void __launch_bounds__(kSpecimenThreads) k_specimen_forward(SpecimenA
                         const uint2 e = a.bEnds[k];
                         float lam = lamB[r.z + t];
                         float4 pa = x[e.x - k0], pb = x[e.y - k0];
-                        if (projectBend(pa, pb, a.chordCompliance[k], lam, dt, false, 0.0f, a.failure)) {
+                        if (projectBend(pa, pb, a.chordCompliance[k], lam, invDt2, false, 0.0f, a.failure)) {
                             x[e.x - k0] = pa;
                             x[e.y - k0] = pb;
                             lamB[r.z + t] = lam;
@@ -456,7 +457,7 @@ __global__ void __launch_bounds__(kSpecimenThreads) k_specimen_reverse(SpecimenA
                         const uint2 e = a.bEnds[k];
                         const float4 pa = xs[e.x], pb = xs[e.y];
                         const float2 cc = a.chordCompliance[k];
-                        const float alpha = cc.y / (dt * dt);
+                        const float alpha = cc.y * alphaScale;  // as the forward computes it
                         const float chord = length(xyz(pb) - xyz(pa));
                         if (!(chord > 1.0e-12f) || !(alpha + pa.w + pb.w >= 1.0e-16f)) continue;
                         float4 ga = gx[e.x - k0], gb = gx[e.y - k0];
@@ -481,7 +482,7 @@ __global__ void __launch_bounds__(kSpecimenThreads) k_specimen_reverse(SpecimenA
                     const uint2 e = a.ij[k];
                     const float4 pa = xs[e.x], pb = xs[e.y];
                     const float2 rc = a.restCompliance[k];
-                    const float alpha = rc.y / (dt * dt);
+                    const float alpha = rc.y * alphaScale;
                     const float len = length(xyz(pb) - xyz(pa));
                     if (!(len > 1.0e-12f) || !(pa.w + pb.w + alpha > 0.0f)) continue;
                     float4 ga = gx[e.x - k0], gb = gx[e.y - k0];

@@ -125,11 +125,12 @@ __global__ void k_advance(const NumiClothBagGPUConfig* __restrict__ cfg, const u
 // One XPBD distance projection (reference solve_distance,
 // src/metal/ClothBag.metal:1148-1213) on values already loaded: updates a, b
 // and lambda in place; returns false when nothing is written.
-__device__ __forceinline__ bool projectDistance(float4& a, float4& b, float2 rc, float& lam, float dt,
+// invDt2 = 1 / (dt * dt) of the substep: alpha = compliance * invDt2.
+__device__ __forceinline__ bool projectDistance(float4& a, float4& b, float2 rc, float& lam, float invDt2,
                                                 uint32_t* failure) {
     const V3 d = xyz(b) - xyz(a);
     const float len = length(d);
-    const float alpha = rc.y / (dt * dt);
+    const float alpha = rc.y * invDt2;
     const float denom = a.w + b.w + alpha;
     if (!isfinite(len) || !isfinite(alpha) || !isfinite(denom) || !isfinite(rc.x) || !isfinite(lam)) {
         recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
@@ -164,7 +165,8 @@ __global__ void k_distance(const NumiClothBagGPUConfig* __restrict__ cfg, const 
     float lam = lambda[c];
     float4 a = pos[e.x];
     float4 b = pos[e.y];
-    if (!projectDistance(a, b, restCompliance[c], lam, cfg->gravityAndTimestep.w, failure)) return;
+    const float dt = cfg->gravityAndTimestep.w;
+    if (!projectDistance(a, b, restCompliance[c], lam, 1.0f / (dt * dt), failure)) return;
     pos[e.x] = a;
     pos[e.y] = b;
     lambda[c] = lam;
@@ -183,9 +185,10 @@ __device__ __forceinline__ bool limitStrainPair(float4& a, float4& b, float limi
     const float denom = a.w + b.w;
     if (!(denom > 0.0f) || !isfinite(denom)) return false;
     const float excess = len - limit;
-    const V3 dir = d / len;
-    a = withXyz(a, xyz(a) + dir * (a.w * excess / denom));
-    b = withXyz(b, xyz(b) - dir * (b.w * excess / denom));
+    const V3 dir = d * (1.0f / len);
+    const float share = excess / denom;
+    a = withXyz(a, xyz(a) + dir * (a.w * share));
+    b = withXyz(b, xyz(b) - dir * (b.w * share));
     return true;
 }
 
@@ -272,7 +275,8 @@ __global__ void k_knot(const NumiClothBagGPUConfig* __restrict__ cfg, const uint
 // chord-length XPBD between the outer knots with a ground active set (a
 // grounded endpoint that would be pushed down keeps its height) and a step
 // fraction that stops free endpoints at the ground.
-__device__ __forceinline__ bool projectBend(float4& a, float4& b, float2 cc, float& lam, float dt, bool ground,
+// invDt2 = 1 / (dt * dt) of the substep: alpha = compliance * invDt2.
+__device__ __forceinline__ bool projectBend(float4& a, float4& b, float2 cc, float& lam, float invDt2, bool ground,
                                             float floorZ, uint32_t* failure) {
     const V3 d = xyz(b) - xyz(a);
     const float chord = length(d);
@@ -282,10 +286,10 @@ __device__ __forceinline__ bool projectBend(float4& a, float4& b, float2 cc, flo
     }
     if (!(chord > 1.0e-12f)) return false;
     const float value = chord - cc.x;
-    const V3 dir = d / chord;
+    const V3 dir = d * (1.0f / chord);
     const V3 g[2] = {-dir, dir};
     const float w[2] = {a.w, b.w};
-    const float alpha = cc.y / (dt * dt);
+    const float alpha = cc.y * invDt2;
     float freeDenom = alpha;
     for (int k = 0; k < 2; ++k) freeDenom += w[k] * dot(g[k], g[k]);
     if (!(freeDenom >= 1.0e-16f) || !isfinite(freeDenom)) return false;
@@ -299,7 +303,9 @@ __device__ __forceinline__ bool projectBend(float4& a, float4& b, float2 cc, flo
         denom += w[k] * (dot(g[k], g[k]) - (active[k] ? g[k].z * g[k].z : 0.0f));
     }
     if (!(denom >= 1.0e-16f) || !isfinite(denom)) return false;
-    const float dl = numerator / denom;
+    // Without an active ground knot denom is freeDenom to the bit (the same
+    // sums with zero subtracted), so the free multiplier is reused.
+    const float dl = active[0] || active[1] ? numerator / denom : freeDl;
     float fraction = 1.0f;
     if (ground) {
         for (int k = 0; k < 2; ++k) {
@@ -341,7 +347,8 @@ __global__ void k_bend(const NumiClothBagGPUConfig* __restrict__ cfg, const uint
     float lam = lambda[c];
     float4 a = pos[e.x];
     float4 b = pos[e.y];
-    if (!projectBend(a, b, chordCompliance[c], lam, cfg->gravityAndTimestep.w, cfg->constraintCounts.z != 0u,
+    const float dt = cfg->gravityAndTimestep.w;
+    if (!projectBend(a, b, chordCompliance[c], lam, 1.0f / (dt * dt), cfg->constraintCounts.z != 0u,
                      cfg->clothMaterial.x, failure)) {
         return;
     }

@@ -135,7 +135,7 @@ __device__ __forceinline__ void tileLoadLambdas(const float* __restrict__ in, co
 template <bool Strain>
 __device__ __forceinline__ bool tileItem(float4* lds, uint32_t kind, const uint4& item, float lam, float* out,
                                          const float2* __restrict__ knotCC, uint32_t* failure, bool live, float dt,
-                                         bool ground, float floorZ) {
+                                         float invDt2, bool ground, float floorZ) {
     if constexpr (Strain) {
         const uint32_t la = item.x & 0xffffu, lb = item.x >> 16;
         if (!live) return false;
@@ -163,7 +163,7 @@ __device__ __forceinline__ bool tileItem(float4* lds, uint32_t kind, const uint4
         if (live) {
             float4 p = lds[la], q = lds[lb];
             const float2 rc = make_float2(__uint_as_float(item.z), __uint_as_float(item.w));
-            if (projectDistance(p, q, rc, lam, dt, failure)) {
+            if (projectDistance(p, q, rc, lam, invDt2, failure)) {
                 lds[la] = p;
                 lds[lb] = q;
             }
@@ -173,7 +173,7 @@ __device__ __forceinline__ bool tileItem(float4* lds, uint32_t kind, const uint4
         if (live) {
             float4 p = lds[la], q = lds[lb];
             const float2 cc = make_float2(__uint_as_float(item.z), __uint_as_float(item.w));
-            if (projectBend(p, q, cc, lam, dt, ground, floorZ, failure)) {
+            if (projectBend(p, q, cc, lam, invDt2, ground, floorZ, failure)) {
                 lds[la] = p;
                 lds[lb] = q;
             }
@@ -250,6 +250,7 @@ __global__ void k_tile_program(TileProgramArgs a) {
     __syncthreads();
     const bool live = *a.ok != 0u;
     const float dt = a.cfg->gravityAndTimestep.w;
+    const float invDt2 = 1.0f / (dt * dt);
     const bool ground = a.cfg->constraintCounts.z != 0u;
     const float floorZ = a.cfg->clothMaterial.x;
     if (a.clearAcc != nullptr) {
@@ -292,11 +293,12 @@ __global__ void k_tile_program(TileProgramArgs a) {
 #pragma unroll
         for (uint32_t m = 0; m < kTilePrefetch; ++m)
             if (first + t + m * stride < last)
-                moved |= tileItem<Strain>(lds, kind, cur[m], lam[m], out, knotCC, failure, live, dt, ground, floorZ);
+                moved |= tileItem<Strain>(lds, kind, cur[m], lam[m], out, knotCC, failure, live, dt, invDt2, ground,
+                                          floorZ);
         for (uint32_t j = first + t + kTilePrefetch * stride; j < last; j += stride) {
             const uint4 item = items[j];
             const float lj = Strain ? 0.0f : in[tileItemConstraint(kind, item)];
-            moved |= tileItem<Strain>(lds, kind, item, lj, out, knotCC, failure, live, dt, ground, floorZ);
+            moved |= tileItem<Strain>(lds, kind, item, lj, out, knotCC, failure, live, dt, invDt2, ground, floorZ);
         }
         if constexpr (Strain) {
             sweepMoved |= __syncthreads_or(moved ? 1 : 0) != 0;

@@ -133,7 +133,7 @@ struct Model {
             const mr_uint4 e = d.distances[k].particlesAndColor;
             const Vec diff = x[e.y] - x[e.x];
             const R len = norm(diff);
-            const R alpha = dCompliance[k] / (dt * dt);
+            const R alpha = dCompliance[k] * (R(1) / (dt * dt));
             const R denom = w[e.x] + w[e.y] + alpha;
             if (!(len > R(1.0e-12)) || !(denom > R(0))) continue;
             if (record) tape.push_back({0, k, e.x, e.y, x[e.x], x[e.y], dLambda[k]});
@@ -152,9 +152,9 @@ struct Model {
             const Vec diff = x[c] - x[a];
             const R chord = norm(diff);
             if (!(chord > R(1.0e-12))) continue;
-            const Vec dir = diff / chord;
+            const Vec dir = diff * (R(1) / chord);
             const Vec g[2] = {-dir, dir};
-            const R alpha = bCompliance[k] / (dt * dt);
+            const R alpha = bCompliance[k] * (R(1) / (dt * dt));
             R denom = alpha + w[a] * dot(g[0], g[0]) + w[c] * dot(g[1], g[1]);
             if (!(denom >= R(1.0e-16))) continue;
             if (record) tape.push_back({1, k, a, c, x[a], x[c], bLambda[k]});
@@ -176,9 +176,10 @@ struct Model {
             if (!(denom > R(0))) continue;
             if (record) tape.push_back({2, k, e.x, e.y, x[e.x], x[e.y], R(0)});
             const R excess = len - limit;
-            const Vec dir = diff / len;
-            x[e.x] = x[e.x] + dir * (w[e.x] * excess / denom);
-            x[e.y] = x[e.y] - dir * (w[e.y] * excess / denom);
+            const Vec dir = diff * (R(1) / len);
+            const R share = excess / denom;
+            x[e.x] = x[e.x] + dir * (w[e.x] * share);
+            x[e.y] = x[e.y] - dir * (w[e.y] * share);
         }
     }
 
@@ -392,7 +393,7 @@ int runAdjoint(const NumiClothHipSceneDesc* scene, const NumiClothBagGPUConfig* 
             const bool bend = e.kind == 1;
             const R compliance = bend ? m.bCompliance[e.k] : m.dCompliance[e.k];
             const R rest = bend ? R(m.d.bends[e.k].material.x) : R(m.d.distances[e.k].material.x);
-            const R alpha = compliance / (dt * dt);
+            const R alpha = compliance * alphaScale;  // as the forward computes it
             R galpha = R(0);
             R& gl = bend ? glb[e.k] : gld[e.k];
             projectionAdjoint(e, rest, alpha, m.w[e.a], m.w[e.b], gx[e.a], gx[e.b], gl, galpha, gw[e.a], gw[e.b]);

@@ -260,7 +260,9 @@ struct Oracle {
                 Particle<R>& c = p[r.particlesAndColor.y];
                 const V<R> diff = c.x - a.x;
                 const R len = norm(diff);
-                const R alpha = R(r.material.y) / (dt * dt);
+                // alpha = compliance * (1 / dt^2), the device's form (one
+                // reciprocal per substep instead of a division per constraint).
+                const R alpha = R(r.material.y) * (R(1) / (dt * dt));
                 const R denom = a.w + c.w + alpha;
                 if (!(len > R(1.0e-12)) || !(denom > R(0))) continue;
                 const R dl = (-(len - R(r.material.x)) - alpha * dLambda[k]) / denom;
@@ -311,9 +313,9 @@ struct Oracle {
                 const V<R> diff = q[1]->x - q[0]->x;
                 const R chord = norm(diff);
                 if (!(chord > R(1.0e-12))) continue;
-                const V<R> dir = diff / chord;
+                const V<R> dir = diff * (R(1) / chord);
                 const V<R> g[2] = {-dir, dir};
-                const R alpha = R(r.material.z) / (dt * dt);
+                const R alpha = R(r.material.z) * (R(1) / (dt * dt));
                 R freeDenom = alpha;
                 for (int m = 0; m < 2; ++m) freeDenom += q[m]->w * dot(g[m], g[m]);
                 if (!(freeDenom >= R(1.0e-16))) continue;
@@ -396,10 +398,11 @@ struct Oracle {
                     if (!(len > limit) || !(len > R(1.0e-12))) continue;
                     const R denom = a.w + c.w;
                     if (!(denom > R(0))) continue;
-                    const V<R> dir = diff / len;
+                    const V<R> dir = diff * (R(1) / len);
                     const R excess = len - limit;
-                    a.x = a.x + dir * (a.w * excess / denom);
-                    c.x = c.x - dir * (c.w * excess / denom);
+                    const R share = excess / denom;
+                    a.x = a.x + dir * (a.w * share);
+                    c.x = c.x - dir * (c.w * share);
                 }
             }
         }
