@@ -269,6 +269,7 @@ def diagnostics_of(solver, state) -> dict:
     return {"max_strain_violation": d.maximumStrainViolation, "max_ground_penetration": d.maximumGroundPenetration,
             "max_sphere_penetration": d.maximumSpherePenetration, "max_self_penetration": d.maximumSelfPenetration,
             "self_candidates_last_substep": d.selfCandidateCount, "self_candidate_overflow": bool(d.selfCandidateOverflow),
+            "self_entries_last_substep": d.selfEntryCount, "self_grid_cells": list(d.selfGridCells),
             "self_contacts_last_step": {"present": int(self_status[0]), "swept": int(self_status[1])}}
 
 

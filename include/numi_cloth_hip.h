@@ -150,6 +150,10 @@ typedef struct NumiClothHipDiagnostics {
      * the most candidate visits one solve staged. */
     uint32_t yarnReruns;
     uint32_t yarnCandidatesMax;
+    /* Self-contact broad phase of the last substep: (cell, segment) entries
+     * the boxes requested and the grid's cell counts per axis. */
+    uint32_t selfEntryCount;
+    uint32_t selfGridCells[3];
 } NumiClothHipDiagnostics;
 
 typedef struct NumiClothHipSolver NumiClothHipSolver;

@@ -102,7 +102,8 @@ class Diagnostics(C.Structure):
     _fields_ = [("maximumStrainViolation", C.c_float), ("maximumGroundPenetration", C.c_float),
                 ("maximumSpherePenetration", C.c_float), ("maximumSelfPenetration", C.c_float),
                 ("selfCandidateCount", C.c_uint32), ("selfCandidateOverflow", C.c_uint32),
-                ("yarnReruns", C.c_uint32), ("yarnCandidatesMax", C.c_uint32)]
+                ("yarnReruns", C.c_uint32), ("yarnCandidatesMax", C.c_uint32),
+                ("selfEntryCount", C.c_uint32), ("selfGridCells", C.c_uint32 * 3)]
 
 
 MATERIAL_FIELDS = ["ordinaryNodeMassKg", "hemNodeMassKg", "yarnRadiusM", "axialBodyComplianceMPerN",

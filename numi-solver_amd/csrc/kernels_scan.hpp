@@ -3,8 +3,10 @@
 // Device exclusive scan of uint32 arrays and a stable LSD radix sort of
 // (uint32 key, uint32 value) pairs, both deterministic (integer arithmetic
 // only, fixed reduction order). Element counts live in device memory so the
-// launches can sit inside a captured graph; grids are sized for capacity and
-// tiles past the live count exit early.
+// launches can sit inside a captured graph; each workgroup walks tiles
+// blockIdx.x, blockIdx.x + gridDim.x, ... up to the live count, so a grid of
+// at most kScanBlocks workgroups serves any capacity and a launch over few
+// live elements costs few workgroups.
 
 #include "device_math.hpp"
 
@@ -15,6 +17,7 @@ constexpr uint32_t kScanItems = 8;
 constexpr uint32_t kScanTile = kScanThreads * kScanItems;  // 2048
 constexpr uint32_t kRadixBits = 8;
 constexpr uint32_t kRadixBins = 1u << kRadixBits;
+constexpr uint32_t kScanBlocks = 2048;
 
 // Exclusive scan of the 256 thread totals held in LDS; returns the block total.
 __device__ uint32_t blockExclusiveScan256(uint32_t value, uint32_t* scratch) {
@@ -56,25 +59,26 @@ __global__ void k_scan_tiles(uint32_t* __restrict__ data, const uint32_t* __rest
                              uint32_t* __restrict__ sums) {
     __shared__ uint32_t scratch[kScanThreads / 64];
     const uint32_t n = *count;
-    const uint32_t base = blockIdx.x * kScanTile;
-    if (base >= n) return;
-    uint32_t items[kScanItems];
-    uint32_t local = 0;
-    const uint32_t first = base + threadIdx.x * kScanItems;
-    for (uint32_t k = 0; k < kScanItems; ++k) {
-        const uint32_t i = first + k;
-        items[k] = i < n ? data[i] : 0u;
-        local += items[k];
+    const uint32_t tiles = (n + kScanTile - 1) / kScanTile;
+    for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        uint32_t items[kScanItems];
+        uint32_t local = 0;
+        const uint32_t first = tile * kScanTile + threadIdx.x * kScanItems;
+        for (uint32_t k = 0; k < kScanItems; ++k) {
+            const uint32_t i = first + k;
+            items[k] = i < n ? data[i] : 0u;
+            local += items[k];
+        }
+        const uint32_t offset = blockExclusiveScan256(local, scratch);
+        const uint32_t total = blockTotal256(local, scratch);
+        uint32_t running = offset;
+        for (uint32_t k = 0; k < kScanItems; ++k) {
+            const uint32_t i = first + k;
+            if (i < n) data[i] = running;
+            running += items[k];
+        }
+        if (threadIdx.x == 0) sums[tile] = total;
     }
-    const uint32_t offset = blockExclusiveScan256(local, scratch);
-    const uint32_t total = blockTotal256(local, scratch);
-    uint32_t running = offset;
-    for (uint32_t k = 0; k < kScanItems; ++k) {
-        const uint32_t i = first + k;
-        if (i < n) data[i] = running;
-        running += items[k];
-    }
-    if (threadIdx.x == 0) sums[blockIdx.x] = total;
 }
 
 // Phase 2: one workgroup scans the tile totals sequentially in chunks of 256
@@ -99,38 +103,40 @@ __global__ void k_scan_sums(uint32_t* __restrict__ sums, const uint32_t* __restr
 __global__ void k_scan_add(uint32_t* __restrict__ data, const uint32_t* __restrict__ count,
                            const uint32_t* __restrict__ sums) {
     const uint32_t n = *count;
-    const uint32_t base = blockIdx.x * kScanTile;
-    if (base >= n) return;
-    const uint32_t add = sums[blockIdx.x];
-    for (uint32_t k = threadIdx.x; k < kScanTile; k += kScanThreads) {
-        const uint32_t i = base + k;
-        if (i < n) data[i] += add;
+    const uint32_t tiles = (n + kScanTile - 1) / kScanTile;
+    for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        const uint32_t add = sums[tile];
+        const uint32_t base = tile * kScanTile;
+        for (uint32_t k = threadIdx.x; k < kScanTile; k += kScanThreads) {
+            const uint32_t i = base + k;
+            if (i < n) data[i] += add;
+        }
     }
 }
 
 // ---------------------------------------------------------------- radix --
 
-// Per-tile digit histogram, stored digit-major: hist[digit * tiles + tile],
-// so one exclusive scan of `hist` yields every (digit, tile) output base in
-// stable order.
-// The histogram is digit-major over the live tiles, ceil(*count / kScanTile),
-// so its exclusive scan (of kRadixBins * live tiles entries) gives every
-// tile's first output slot per digit; blocks past the live tiles exit.
+// Per-tile digit histogram, stored digit-major over the live tiles
+// (ceil(*count / kScanTile)): hist[digit * tiles + tile], so one exclusive
+// scan of kRadixBins * tiles entries yields every (digit, tile) output base
+// in stable order.
 __global__ void k_radix_histogram(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ count,
                                   uint32_t shift, uint32_t* __restrict__ hist) {
     __shared__ uint32_t bins[kRadixBins];
     const uint32_t n = *count;
     const uint32_t tiles = (n + kScanTile - 1) / kScanTile;
-    if (blockIdx.x >= tiles) return;
-    bins[threadIdx.x] = 0;
-    __syncthreads();
-    const uint32_t base = blockIdx.x * kScanTile;
-    for (uint32_t k = threadIdx.x; k < kScanTile; k += kScanThreads) {
-        const uint32_t i = base + k;
-        if (i < n) atomicAdd(&bins[(keys[i] >> shift) & (kRadixBins - 1)], 1u);
+    for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        bins[threadIdx.x] = 0;
+        __syncthreads();
+        const uint32_t base = tile * kScanTile;
+        for (uint32_t k = threadIdx.x; k < kScanTile; k += kScanThreads) {
+            const uint32_t i = base + k;
+            if (i < n) atomicAdd(&bins[(keys[i] >> shift) & (kRadixBins - 1)], 1u);
+        }
+        __syncthreads();
+        hist[threadIdx.x * tiles + tile] = bins[threadIdx.x];
+        __syncthreads();
     }
-    __syncthreads();
-    hist[threadIdx.x * tiles + blockIdx.x] = bins[threadIdx.x];
 }
 
 // Stable scatter. The tile is consumed in 8 rounds of 256 consecutive items;
@@ -145,43 +151,44 @@ __global__ void k_radix_scatter(const uint32_t* __restrict__ keysIn, const uint3
     __shared__ uint32_t waveCount[kScanThreads / 64][kRadixBins];
     const uint32_t n = *count;
     const uint32_t tiles = (n + kScanTile - 1) / kScanTile;
-    const uint32_t base = blockIdx.x * kScanTile;
-    if (base >= n) return;
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u;
     const uint32_t wave = tid >> 6;
-    running[tid] = histScanned[tid * tiles + blockIdx.x];
-    for (uint32_t w = 0; w < kScanThreads / 64; ++w) waveCount[w][tid] = 0;
-    __syncthreads();
     const unsigned long long below = (1ull << lane) - 1ull;
-    for (uint32_t round = 0; round < kScanItems; ++round) {
-        const uint32_t i = base + round * kScanThreads + tid;
-        const bool live = i < n;
-        const uint32_t key = live ? keysIn[i] : 0u;
-        const uint32_t value = live ? valsIn[i] : 0u;
-        const uint32_t digit = (key >> shift) & (kRadixBins - 1);
-        unsigned long long same = __ballot(live);
-        for (uint32_t b = 0; b < kRadixBits; ++b) {
-            const unsigned long long ones = __ballot(live && ((digit >> b) & 1u));
-            same &= ((digit >> b) & 1u) ? ones : ~ones;
-        }
-        const uint32_t rank = static_cast<uint32_t>(__popcll(same & below));
-        if (live && rank == 0) waveCount[wave][digit] = static_cast<uint32_t>(__popcll(same));
+    for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        const uint32_t base = tile * kScanTile;
+        running[tid] = histScanned[tid * tiles + tile];
+        for (uint32_t w = 0; w < kScanThreads / 64; ++w) waveCount[w][tid] = 0;
         __syncthreads();
-        if (live) {
-            uint32_t position = running[digit] + rank;
-            for (uint32_t w = 0; w < wave; ++w) position += waveCount[w][digit];
-            keysOut[position] = key;
-            valsOut[position] = value;
+        for (uint32_t round = 0; round < kScanItems; ++round) {
+            const uint32_t i = base + round * kScanThreads + tid;
+            const bool live = i < n;
+            const uint32_t key = live ? keysIn[i] : 0u;
+            const uint32_t value = live ? valsIn[i] : 0u;
+            const uint32_t digit = (key >> shift) & (kRadixBins - 1);
+            unsigned long long same = __ballot(live);
+            for (uint32_t b = 0; b < kRadixBits; ++b) {
+                const unsigned long long ones = __ballot(live && ((digit >> b) & 1u));
+                same &= ((digit >> b) & 1u) ? ones : ~ones;
+            }
+            const uint32_t rank = static_cast<uint32_t>(__popcll(same & below));
+            if (live && rank == 0) waveCount[wave][digit] = static_cast<uint32_t>(__popcll(same));
+            __syncthreads();
+            if (live) {
+                uint32_t position = running[digit] + rank;
+                for (uint32_t w = 0; w < wave; ++w) position += waveCount[w][digit];
+                keysOut[position] = key;
+                valsOut[position] = value;
+            }
+            __syncthreads();
+            uint32_t add = 0;
+            for (uint32_t w = 0; w < kScanThreads / 64; ++w) {
+                add += waveCount[w][tid];
+                waveCount[w][tid] = 0;
+            }
+            running[tid] += add;
+            __syncthreads();
         }
-        __syncthreads();
-        uint32_t add = 0;
-        for (uint32_t w = 0; w < kScanThreads / 64; ++w) {
-            add += waveCount[w][tid];
-            waveCount[w][tid] = 0;
-        }
-        running[tid] += add;
-        __syncthreads();
     }
 }
 

@@ -58,8 +58,11 @@ struct SelfGrid {
     uint32_t pairTotal;      // candidate pairs found
     uint32_t pairCount;      // min(pairTotal, capacity)
     uint32_t valid;          // 0 when the grid would exceed 2^32 cells
-    uint32_t histLength;     // radix histogram length: kRadixBins * ceil(sortCount / kScanTile)
-    uint32_t pad;
+    uint32_t histLength;     // radix histogram length: kRadixBins * ceil(radixCount / kScanTile)
+    uint32_t denseCells;     // nx * ny * nz when it fits the per-cell counters, else 0
+    uint32_t cellScanLength; // denseCells when this substep sorts by counting, else 0
+    uint32_t radixCount;     // sortCount when this substep sorts by radix, else 0
+    uint32_t pad[3];
 };
 
 // A sorted cell entry with its segment's box, gathered once per substep so
@@ -137,8 +140,8 @@ __device__ __forceinline__ int cellOf(float v, float origin, float cell) {
 // (min/max are exact, so the reduction order does not matter).
 __global__ void k_self_grid(const float4* __restrict__ partialLo, const float4* __restrict__ partialHi,
                             uint32_t blocks, const NumiClothBagGPUConfig* __restrict__ cfg,
-                            const uint32_t* __restrict__ ok, uint32_t segments, SelfGrid* __restrict__ grid,
-                            uint32_t* __restrict__ failure) {
+                            const uint32_t* __restrict__ ok, uint32_t segments, uint32_t cellCapacity,
+                            SelfGrid* __restrict__ grid, uint32_t* __restrict__ failure) {
     __shared__ float4 waveLo[kScanThreads / 64];
     __shared__ float4 waveHi[kScanThreads / 64];
     float l[3] = {INFINITY, INFINITY, INFINITY};
@@ -183,6 +186,7 @@ __global__ void k_self_grid(const float4* __restrict__ partialLo, const float4* 
             g.nx = static_cast<uint32_t>(nx);
             g.ny = static_cast<uint32_t>(ny);
             g.nz = static_cast<uint32_t>(nz);
+            if (nx * ny * nz <= static_cast<double>(cellCapacity)) g.denseCells = g.nx * g.ny * g.nz;
         }
     }
     if (!valid && *ok != 0u) recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_RANGE);
@@ -213,11 +217,14 @@ __device__ __forceinline__ uint32_t cellId(int x, int y, int z, const SelfGrid& 
     return (static_cast<uint32_t>(z) * g.ny + static_cast<uint32_t>(y)) * g.nx + static_cast<uint32_t>(x);
 }
 
+// Per-segment entry counts; on a dense grid also clears the per-cell
+// counters of the counting sort.
 __global__ void k_self_cell_counts(const float4* __restrict__ boxLo, const float4* __restrict__ boxHi,
                                    const SelfGrid* __restrict__ grid, uint32_t* __restrict__ counts,
-                                   uint32_t* __restrict__ failure) {
+                                   uint32_t* __restrict__ cellCounters, uint32_t* __restrict__ failure) {
     const SelfGrid g = *grid;
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint32_t c = s; c < g.denseCells; c += gridDim.x * blockDim.x) cellCounters[c] = 0u;
     if (s >= g.segments) return;
     uint32_t n = 0;
     if (g.valid) {
@@ -228,33 +235,109 @@ __global__ void k_self_cell_counts(const float4* __restrict__ boxLo, const float
     counts[s] = n;
 }
 
+// The entries of segment s in emission order: its covered cells, x fastest,
+// with the "box starts in this cell" bits of the owner test.
+template <typename Visit>
+__device__ __forceinline__ void forSegmentCells(uint32_t s, const CellRange& r, const SelfGrid& g, Visit visit) {
+    for (int z = r.lo[2]; z <= r.hi[2]; ++z)
+        for (int y = r.lo[1]; y <= r.hi[1]; ++y)
+            for (int x = r.lo[0]; x <= r.hi[0]; ++x) {
+                const uint32_t starts = (x == r.lo[0] ? 1u : 0u) | (y == r.lo[1] ? 2u : 0u) |
+                                        (z == r.lo[2] ? 4u : 0u);
+                visit(cellId(x, y, z, g), s | (starts << kEntrySegmentBits));
+            }
+}
+
+// Entries are sorted by (cell, emission order) = (cell, segment), either by
+// a counting sort over the grid's cells (dense grid, every entry fits) or by
+// the LSD radix sort of the (cell, value) pairs written here in emission
+// order. Every thread takes the same decision from the grid and the total.
 __global__ void k_self_emit_entries(const float4* __restrict__ boxLo, const float4* __restrict__ boxHi,
                                     SelfGrid* __restrict__ grid, const uint32_t* __restrict__ offsets,
                                     const uint32_t* __restrict__ entryTotal, uint32_t capacity,
                                     uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
-                                    uint32_t* __restrict__ failure) {
+                                    uint32_t* __restrict__ cellCounters, uint32_t* __restrict__ failure) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     const SelfGrid g = *grid;
+    const uint32_t total = *entryTotal;
+    const bool dense = g.valid && g.denseCells != 0u && total <= capacity;
     if (s == 0) {
-        const uint32_t sorted = min(*entryTotal, capacity);
-        grid->entryCount = *entryTotal;
+        const uint32_t sorted = min(total, capacity);
+        grid->entryCount = total;
         grid->sortCount = sorted;
-        grid->histLength = kRadixBins * ((sorted + kScanTile - 1) / kScanTile);
-        if (*entryTotal > capacity) recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_RANGE);
+        grid->radixCount = dense ? 0u : sorted;
+        grid->cellScanLength = dense ? g.denseCells : 0u;
+        grid->histLength = dense ? 0u : kRadixBins * ((sorted + kScanTile - 1) / kScanTile);
+        if (total > capacity) recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_RANGE);
     }
     if (s >= g.segments || !g.valid) return;
     const CellRange r = cellRange(boxLo[s], boxHi[s], g);
-    uint32_t out = offsets[s];
     if (r.count == 0) return;
-    for (int z = r.lo[2]; z <= r.hi[2]; ++z)
-        for (int y = r.lo[1]; y <= r.hi[1]; ++y)
-            for (int x = r.lo[0]; x <= r.hi[0]; ++x, ++out)
-                if (out < capacity) {
-                    const uint32_t starts = (x == r.lo[0] ? 1u : 0u) | (y == r.lo[1] ? 2u : 0u) |
-                                            (z == r.lo[2] ? 4u : 0u);
-                    keys[out] = cellId(x, y, z, g);
-                    vals[out] = s | (starts << kEntrySegmentBits);
-                }
+    if (dense) {
+        forSegmentCells(s, r, g, [&](uint32_t cell, uint32_t) { atomicAdd(&cellCounters[cell], 1u); });
+        return;
+    }
+    uint32_t out = offsets[s];
+    forSegmentCells(s, r, g, [&](uint32_t cell, uint32_t value) {
+        if (out < capacity) {
+            keys[out] = cell;
+            vals[out] = value;
+        }
+        ++out;
+    });
+}
+
+// Counting sort, after the exclusive scan of the per-cell counts: each entry
+// takes the next slot of its cell's run (the order inside a run depends on
+// the atomics' order; the rank pass below restores it). Afterwards
+// cellSlots[c] holds the end of cell c's run, which is the start of c + 1's.
+__global__ void k_self_place_entries(const float4* __restrict__ boxLo, const float4* __restrict__ boxHi,
+                                     const SelfGrid* __restrict__ grid, uint32_t* __restrict__ cellSlots,
+                                     uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    const SelfGrid g = *grid;
+    if (g.cellScanLength == 0u || s >= g.segments) return;
+    const CellRange r = cellRange(boxLo[s], boxHi[s], g);
+    if (r.count == 0) return;
+    forSegmentCells(s, r, g, [&](uint32_t cell, uint32_t value) {
+        const uint32_t at = atomicAdd(&cellSlots[cell], 1u);
+        keys[at] = cell;
+        vals[at] = value;
+    });
+}
+
+// Final position of each placed entry: its cell's run start plus the number
+// of entries of the run with a smaller segment (a segment enters a cell at
+// most once, so ranks are distinct). This is the order a stable sort by cell
+// of the emission-ordered entries gives. Also writes the gathered records of
+// the candidate walk (entries, sortedIJ: may be null).
+__global__ void k_self_rank_entries(const float4* __restrict__ boxLo, const float4* __restrict__ boxHi,
+                                    const uint2* __restrict__ ij, const SelfGrid* __restrict__ grid,
+                                    const uint32_t* __restrict__ cellEnds, const uint32_t* __restrict__ keysIn,
+                                    const uint32_t* __restrict__ valsIn, uint32_t* __restrict__ keysOut,
+                                    uint32_t* __restrict__ valsOut, SortedEntry* __restrict__ entries,
+                                    uint2* __restrict__ sortedIJ) {
+    const SelfGrid g = *grid;
+    if (g.cellScanLength == 0u) return;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < g.sortCount; k += gridDim.x * blockDim.x) {
+        const uint32_t cell = keysIn[k];
+        const uint32_t v = valsIn[k];
+        const uint32_t s = v & kEntrySegmentMask;
+        const uint32_t begin = cell == 0u ? 0u : cellEnds[cell - 1u];
+        const uint32_t end = cellEnds[cell];
+        uint32_t at = begin;
+        for (uint32_t j = begin; j < end; ++j) at += (valsIn[j] & kEntrySegmentMask) < s ? 1u : 0u;
+        keysOut[at] = cell;
+        valsOut[at] = v;
+        if (entries != nullptr) {
+            const float4 lo = boxLo[s], hi = boxHi[s];
+            SortedEntry e;
+            e.a = make_float4(__uint_as_float(cell), __uint_as_float(v), lo.x, lo.y);
+            e.b = make_float4(lo.z, hi.x, hi.y, hi.z);
+            entries[at] = e;
+            sortedIJ[at] = ij[s];
+        }
+    }
 }
 
 // Two knots are local when one lies in the other's two-hop neighbourhood
@@ -331,6 +414,7 @@ __device__ void forSortedEntryCandidates(uint32_t k, const SelfGrid& g, const So
     const uint32_t s = vs & kEntrySegmentMask;
     const float4 lo = make_float4(self.a.z, self.a.w, self.b.x, 0.0f);
     const float4 hi = make_float4(self.b.y, self.b.z, self.b.w, 0.0f);
+    const uint2 es = sortedIJ[k];
     uint32_t na[W], nb[W];
     bool loaded = false;
     constexpr uint32_t kUnroll = 4;
@@ -353,8 +437,11 @@ __device__ void forSortedEntryCandidates(uint32_t k, const SelfGrid& g, const So
             const float4 tLo = make_float4(e[u].a.z, e[u].a.w, e[u].b.x, 0.0f);
             const float4 tHi = make_float4(e[u].b.y, e[u].b.z, e[u].b.w, 0.0f);
             if (!boxesOverlap(lo, hi, tLo, tHi)) continue;
+            const uint2 et = sortedIJ[j];
+            // A shared knot is in both two-hop lists (each list holds its own
+            // knot): local without reading them.
+            if (et.x == es.x || et.x == es.y || et.y == es.x || et.y == es.y) continue;
             if (!loaded) {
-                const uint2 es = sortedIJ[k];
                 const uint4* ra = reinterpret_cast<const uint4*>(localFixed + static_cast<size_t>(es.x) * W);
                 const uint4* rb = reinterpret_cast<const uint4*>(localFixed + static_cast<size_t>(es.y) * W);
 #pragma unroll
@@ -371,7 +458,6 @@ __device__ void forSortedEntryCandidates(uint32_t k, const SelfGrid& g, const So
                 }
                 loaded = true;
             }
-            const uint2 et = sortedIJ[j];
             bool local = false;
 #pragma unroll
             for (uint32_t m = 0; m < W; ++m)
@@ -391,7 +477,7 @@ __global__ void k_self_gather(const float4* __restrict__ boxLo, const float4* __
                               const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
                               SortedEntry* __restrict__ entries, uint2* __restrict__ sortedIJ) {
     if (!grid->valid) return;
-    const uint32_t n = grid->sortCount;
+    const uint32_t n = grid->radixCount;  // 0 when the counting sort wrote the records
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
         const uint32_t v = vals[k];
         const uint32_t s = v & kEntrySegmentMask;

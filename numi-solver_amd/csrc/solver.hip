@@ -144,6 +144,8 @@ struct NumiClothHipSolver {
     // Hash self-contact (present when extension.selfContact != 0).
     bool self = false;
     uint32_t entryCapacity = 0, pairCapacity = 0, entryTiles = 0, segmentBlocks = 0;
+    uint32_t cellCapacity = 0;          // per-cell counters of the counting sort (0: radix sort only)
+    DeviceArray<uint32_t> cellSlots;
     DeviceArray<uint32_t> localStart, localList;
     DeviceArray<uint32_t> localFixed;   // two-hop lists in localWidth slots per knot (0: not used)
     uint32_t localWidth = 0;
@@ -378,7 +380,7 @@ struct NumiClothHipSolver {
 
     // Exclusive scan of `capacity`-sized data whose live length is *count.
     void launchScan(uint32_t* data, const uint32_t* count, uint32_t capacity, uint32_t* total) {
-        const uint32_t tiles = std::max(1u, (capacity + kScanTile - 1) / kScanTile);
+        const uint32_t tiles = std::min(kScanBlocks, std::max(1u, (capacity + kScanTile - 1) / kScanTile));
         k_scan_tiles<<<tiles, kScanThreads, 0, stream>>>(data, count, scanSums.ptr);
         k_scan_sums<<<1, kScanThreads, 0, stream>>>(scanSums.ptr, count, total);
         k_scan_add<<<tiles, kScanThreads, 0, stream>>>(data, count, scanSums.ptr);
@@ -391,29 +393,42 @@ struct NumiClothHipSolver {
         k_self_boxes<<<segmentBlocks, kScanThreads, 0, stream>>>(config, margin, posCur, prev.ptr, dIJ.ptr, D,
                                                                  boxLo.ptr, boxHi.ptr, partialLo.ptr,
                                                                  partialHi.ptr);
-        k_self_grid<<<1, kScanThreads, 0, stream>>>(partialLo.ptr, partialHi.ptr, segmentBlocks, config, valid, D, g,
-                                          failure.ptr);
+        k_self_grid<<<1, kScanThreads, 0, stream>>>(partialLo.ptr, partialHi.ptr, segmentBlocks, config, valid, D,
+                                                    cellCapacity, g, failure.ptr);
         k_self_cell_counts<<<segmentBlocks, kScanThreads, 0, stream>>>(boxLo.ptr, boxHi.ptr, g, segmentCounts.ptr,
-                                                                       failure.ptr);
+                                                                       cellSlots.ptr, failure.ptr);
         launchScan(segmentCounts.ptr, &g->segments, D, totals.ptr);
         k_self_emit_entries<<<segmentBlocks, kScanThreads, 0, stream>>>(boxLo.ptr, boxHi.ptr, g, segmentCounts.ptr,
                                                                         totals.ptr, entryCapacity, keysA.ptr,
-                                                                        valsA.ptr, failure.ptr);
+                                                                        valsA.ptr, cellSlots.ptr, failure.ptr);
+        // Grid-stride kernels over the live entries: enough blocks to fill the
+        // chip, not one per capacity slot.
+        const uint32_t entryBlocks = std::min<uint32_t>(8192, (entryCapacity + kScanThreads - 1) / kScanThreads);
+        // Dense grid: counting sort (per-cell counts from the emit pass, one
+        // scan, placement, rank within each cell run) straight into A and the
+        // gathered records. Otherwise these three exit at once and the radix
+        // passes below sort; each side's kernels exit when the other sorts.
+        if (cellCapacity != 0) {
+            launchScan(cellSlots.ptr, &g->cellScanLength, cellCapacity, totals.ptr + 3);
+            k_self_place_entries<<<segmentBlocks, kScanThreads, 0, stream>>>(boxLo.ptr, boxHi.ptr, g, cellSlots.ptr,
+                                                                             keysB.ptr, valsB.ptr);
+            k_self_rank_entries<<<entryBlocks, kScanThreads, 0, stream>>>(
+                boxLo.ptr, boxHi.ptr, dIJ.ptr, g, cellSlots.ptr, keysB.ptr, valsB.ptr, keysA.ptr, valsA.ptr,
+                localWidth != 0 ? sortedEntries.ptr : nullptr, localWidth != 0 ? sortedIJ.ptr : nullptr);
+        }
         uint32_t* keysIn = keysA.ptr;
         uint32_t* valsIn = valsA.ptr;
         uint32_t* keysOut = keysB.ptr;
         uint32_t* valsOut = valsB.ptr;
+        const uint32_t radixBlocks = std::min(kScanBlocks, entryTiles);
         for (uint32_t shift = 0; shift < 32; shift += kRadixBits) {
-            k_radix_histogram<<<entryTiles, kScanThreads, 0, stream>>>(keysIn, &g->sortCount, shift, histogram.ptr);
+            k_radix_histogram<<<radixBlocks, kScanThreads, 0, stream>>>(keysIn, &g->radixCount, shift, histogram.ptr);
             launchScan(histogram.ptr, &g->histLength, kRadixBins * entryTiles, totals.ptr + 2);
-            k_radix_scatter<<<entryTiles, kScanThreads, 0, stream>>>(keysIn, valsIn, &g->sortCount, shift,
-                                                                      histogram.ptr, keysOut, valsOut);
+            k_radix_scatter<<<radixBlocks, kScanThreads, 0, stream>>>(keysIn, valsIn, &g->radixCount, shift,
+                                                                       histogram.ptr, keysOut, valsOut);
             std::swap(keysIn, keysOut);
             std::swap(valsIn, valsOut);
         }
-        // Grid-stride kernels over the live entries: enough blocks to fill the
-        // chip, not one per capacity slot.
-        const uint32_t entryBlocks = std::min<uint32_t>(8192, (entryCapacity + kScanThreads - 1) / kScanThreads);
         PairArgs pa{};
         pa.boxLo = boxLo.ptr;
         pa.boxHi = boxHi.ptr;
@@ -1146,6 +1161,13 @@ void setupSelfContact(const NumiClothHipSceneDesc& scene, NumiClothHipSolver& s)
     s.pairCapacity = static_cast<uint32_t>(pairCapacity);
     s.entryTiles = (s.entryCapacity + kScanTile - 1) / kScanTile;
     s.segmentBlocks = (s.D + kScanThreads - 1) / kScanThreads;
+    // Grids of up to max(entry capacity, 2^22) cells sort their entries by
+    // counting; larger (sparse) grids by radix. NUMI_CLOTH_HIP_SELF_SORT=radix
+    // sorts every grid by radix.
+    const char* sortMode = std::getenv("NUMI_CLOTH_HIP_SELF_SORT");
+    const bool radixOnly = sortMode != nullptr && std::strcmp(sortMode, "radix") == 0;
+    s.cellCapacity = radixOnly ? 0u : std::max(s.entryCapacity, 1u << 22);
+    if (s.cellCapacity != 0) s.cellSlots.allocate(s.cellCapacity);
     s.boxLo.allocate(s.D);
     s.boxHi.allocate(s.D);
     s.partialLo.allocate(s.segmentBlocks);
@@ -1153,7 +1175,7 @@ void setupSelfContact(const NumiClothHipSceneDesc& scene, NumiClothHipSolver& s)
     s.grid.allocate(1);
     s.segmentCounts.allocate(s.D);
     s.entryCounts.allocate(s.entryCapacity);
-    const uint32_t maxScanLength = std::max({s.D, kRadixBins * s.entryTiles, s.entryCapacity, 1u});
+    const uint32_t maxScanLength = std::max({s.D, kRadixBins * s.entryTiles, s.entryCapacity, s.cellCapacity, 1u});
     s.scanSums.allocate((maxScanLength + kScanTile - 1) / kScanTile + 1);
     s.totals.allocate(4);
     const uint32_t histLength = kRadixBins * s.entryTiles;
@@ -1532,6 +1554,10 @@ void diagnoseSolver(NumiClothHipSolver& s, NumiClothHipDiagnostics* out) {
         HIP_OK(hipStreamSynchronize(s.stream));
         out->selfCandidateCount = g.valid ? g.pairCount : 0u;
         out->selfCandidateOverflow = g.pairTotal > g.pairCount || g.entryCount > g.sortCount ? 1u : 0u;
+        out->selfEntryCount = g.valid ? g.entryCount : 0u;
+        out->selfGridCells[0] = g.nx;
+        out->selfGridCells[1] = g.ny;
+        out->selfGridCells[2] = g.nz;
     }
 }
 
@@ -1630,7 +1656,8 @@ void selfBroadPhase(NumiClothHipSolver& s, NumiClothHipSelfBroadPhase* io) {
         const uint32_t n = std::min(io->pairCount, io->pairCapacity);
         s.pairs.download(reinterpret_cast<uint2*>(io->pairs), n, s.stream);
     }
-    // After the four ping-pong radix passes the sorted entries are in A.
+    // Both sorts leave the sorted entries in A (the counting sort's rank pass
+    // writes A; the radix sort's four ping-pong passes end there).
     if (io->cellIds != nullptr) s.keysA.download(io->cellIds, std::min(io->entryCount, io->entryCapacity), s.stream);
     if (io->segments != nullptr) s.valsA.download(io->segments, std::min(io->entryCount, io->entryCapacity), s.stream);
     HIP_OK(hipStreamSynchronize(s.stream));

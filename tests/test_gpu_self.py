@@ -130,8 +130,21 @@ def _sorted(pairs):
     return pairs[np.lexsort((pairs[:, 1], pairs[:, 0]))]
 
 
+def use_sort(monkeypatch, sort):
+    """The solver sorts cell entries by counting on grids that fit its
+    per-cell counters (every grid here) and by radix otherwise;
+    NUMI_CLOTH_HIP_SELF_SORT=radix, read when a solver is created, forces
+    the radix sort."""
+    if sort == "radix":
+        monkeypatch.setenv("NUMI_CLOTH_HIP_SELF_SORT", "radix")
+    else:
+        monkeypatch.delenv("NUMI_CLOTH_HIP_SELF_SORT", raising=False)
+
+
+@pytest.mark.parametrize("sort", ["counting", "radix"])
 @pytest.mark.parametrize("make", [folded_sheet, two_layer_impact])
-def test_cell_entries_and_candidates_are_bit_exact(cloth, oracle, make):
+def test_cell_entries_and_candidates_are_bit_exact(cloth, oracle, make, sort, monkeypatch):
+    use_sort(monkeypatch, sort)
     scene = make(cloth)
     # The folded layers separate within a few substeps; probe them early.
     substeps = 1 if make is folded_sheet else 6
@@ -153,13 +166,31 @@ def test_cell_entries_and_candidates_are_bit_exact(cloth, oracle, make):
     assert np.all(broad["pairs"][:, 0] < broad["pairs"][:, 1])
 
 
-def test_candidates_equal_brute_force(cloth):
+@pytest.mark.parametrize("sort", ["counting", "radix"])
+def test_candidates_equal_brute_force(cloth, sort, monkeypatch):
+    use_sort(monkeypatch, sort)
     scene = folded_sheet(cloth, width=14, height=20)
     with cloth.Solver(scene) as solver:
         solver.step(scene.configs(3), 4, 0)
         state = solver.read()
         broad = solver.self_broad_phase()
     assert np.array_equal(_sorted(broad["pairs"]), _sorted(brute_force_pairs(scene, state)))
+
+
+def test_counting_and_radix_sorts_step_identically(cloth, monkeypatch):
+    """Both sorts give the same entry order, so whole substeps (broad phase,
+    narrow phase, friction) give the same bytes."""
+    scene = two_layer_impact(cloth)
+    states = []
+    for sort in ["counting", "radix"]:
+        use_sort(monkeypatch, sort)
+        with cloth.Solver(scene) as solver:
+            solver.step(scene.configs(8), 8, 2)
+            states.append(solver.read())
+            assert solver.diagnostics().selfEntryCount > 0
+    assert states[0]["particles"].tobytes() == states[1]["particles"].tobytes()
+    assert states[0]["selfStatus"].tobytes() == states[1]["selfStatus"].tobytes()
+    assert states[0]["frictionStatus"].tobytes() == states[1]["frictionStatus"].tobytes()
 
 
 def test_folded_layers_are_pushed_apart(cloth):
