@@ -290,9 +290,16 @@ const NumiClothHipSceneDesc* numi_cloth_hip_batch_scene(const NumiClothHipBatch*
  * LDS), storing (x, v) every `checkpointInterval` substeps; then, segment by
  * segment from the last, recomputes the segment from its checkpoint with a
  * tape of every pass and runs it backwards. The loss of specimen s is
- *   L_s = sum over substeps t and tracked knots k of |x_k(t) - target_k(t)|^2
+ *   NUMI_CLOTH_ADJOINT_TRACKING:
+ *     L_s = sum over substeps t and tracked knots k of |x_k(t) - target_k(t)|^2
+ *   NUMI_CLOTH_ADJOINT_LINEAR (an observable for the calibration fitter,
+ *   e.g. a knot's height after the last substep):
+ *     L_s = sum over substeps t and tracked knots k of target_k(t) . x_k(t)
+ *     (`targets` holds the weights)
  * with x_k(t) the position after substep t.
  */
+#define NUMI_CLOTH_ADJOINT_TRACKING 0u
+#define NUMI_CLOTH_ADJOINT_LINEAR 1u
 typedef struct NumiClothHipAdjointProblem {
     const NumiClothBagGPUConfig* configs;  /* one per substep; counts replaced */
     uint32_t substeps;
@@ -301,7 +308,7 @@ typedef struct NumiClothHipAdjointProblem {
     uint32_t checkpointInterval;           /* substeps per checkpoint segment */
     const uint32_t* tracked;               /* specimens x trackedPerSpecimen, specimen-local knots */
     uint32_t trackedPerSpecimen;
-    uint32_t reserved;
+    uint32_t observable;                   /* NUMI_CLOTH_ADJOINT_TRACKING or _LINEAR */
     const float* targets;                  /* substeps x specimens x trackedPerSpecimen x 3 */
 } NumiClothHipAdjointProblem;
 

@@ -21,7 +21,7 @@ BUILD = PACKAGE / "_build"
 LIBRARY = PACKAGE / "libnumi_cloth_hip.so"
 INCLUDE = REPO / "include"
 
-HOST_SOURCES = ["artifacts.cpp", "scene.cpp", "replay.cpp"]
+HOST_SOURCES = ["artifacts.cpp", "scene.cpp", "replay.cpp", "calibration.cpp"]
 DEVICE_SOURCES = ["solver.hip"]
 
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wextra",

@@ -177,7 +177,21 @@ class StripSpec(C.Structure):
 class AdjointProblem(C.Structure):
     _fields_ = [("configs", C.c_void_p), ("substeps", C.c_uint32), ("iterations", C.c_uint32),
                 ("strainSweeps", C.c_uint32), ("checkpointInterval", C.c_uint32), ("tracked", C.c_void_p),
-                ("trackedPerSpecimen", C.c_uint32), ("reserved", C.c_uint32), ("targets", C.c_void_p)]
+                ("trackedPerSpecimen", C.c_uint32), ("observable", C.c_uint32), ("targets", C.c_void_p)]
+
+
+class CalibrationColumns(C.Structure):
+    _fields_ = [("columnCount", C.c_uint32), ("reserved", C.c_uint32), ("distanceColumn", C.c_void_p),
+                ("bendColumn", C.c_void_p), ("massColumn", C.c_void_p), ("dampingColumn", C.c_void_p)]
+
+
+class CalibrationRow(C.Structure):
+    _fields_ = [("trialId", C.c_char_p), ("heldout", C.c_uint32), ("reserved", C.c_uint32),
+                ("observable", C.c_char_p), ("measured", C.c_double), ("sigma", C.c_double),
+                ("baseline", C.c_double), ("dlog", C.c_void_p)]
+
+
+NO_COLUMN = 0xFFFFFFFF
 
 
 class SpecimenGradients(C.Structure):
@@ -272,6 +286,8 @@ def library() -> C.CDLL:
         "numi_cloth_obj_write": ([C.c_char_p, vp, u32, vp, u32, vp, u32, vp], i32),
         "numi_cloth_replay_run": ([vp, C.POINTER(SceneDesc), u32, vp, u32, u32, u32, u32, C.c_char_p,
                                    C.POINTER(ReplayResult)], i32),
+        "numi_cloth_calibration_dlog": ([vp, u32, C.POINTER(SceneDesc), C.POINTER(CalibrationColumns), vp, vp], i32),
+        "numi_cloth_calibration_write": ([C.c_char_p, u32, vp, C.POINTER(CalibrationRow), u32], i32),
     }
     for name, (args, result) in signatures.items():
         function = getattr(lib, name)
@@ -586,17 +602,18 @@ class Batch:
         return info
 
     def adjoint(self, configs: np.ndarray, iterations: int, strain_sweeps: int, tracked: np.ndarray,
-                targets: np.ndarray, checkpoint_interval: int = 16) -> None:
+                targets: np.ndarray, checkpoint_interval: int = 16, linear: bool = False) -> None:
         """Run len(configs) substeps with the specimen kernels and the reverse
         pass; `tracked` is specimens x T specimen-local knots, `targets`
-        substeps x specimens x T x 3."""
+        substeps x specimens x T x 3 (the weights of a linear observable when
+        linear=True)."""
         configs = np.ascontiguousarray(configs, dtype=CONFIG)
         tracked = np.ascontiguousarray(tracked, np.uint32)
         targets = np.ascontiguousarray(targets, np.float32)
         assert tracked.shape[0] == len(self.specimens)
         assert targets.shape == (len(configs), len(self.specimens), tracked.shape[1], 3)
         problem = AdjointProblem(configs.ctypes.data, len(configs), iterations, strain_sweeps, checkpoint_interval,
-                                 tracked.ctypes.data, tracked.shape[1], 0, targets.ctypes.data)
+                                 tracked.ctypes.data, tracked.shape[1], 1 if linear else 0, targets.ctypes.data)
         _check(library().numi_cloth_hip_batch_adjoint(self._handle, C.byref(problem)))
 
     def gradients(self, specimen: int) -> dict:
@@ -610,6 +627,20 @@ class Batch:
         _check(library().numi_cloth_hip_batch_gradients(self._handle, specimen, C.byref(g)))
         out["loss"] = float(out["loss"][0])
         return out
+
+    def dlog(self, specimen: int, column_count: int, distance=None, bend=None, mass=None,
+             damping=None) -> tuple[np.ndarray, float]:
+        """Calibration sensitivity columns of the last adjoint's observable for
+        one specimen (numi_cloth_calibration_dlog): `distance`, `bend` (per
+        record) and `mass`, `damping` (per knot) give each parameter's column
+        or NO_COLUMN; None leaves the family out. Returns (dlog, baseline)."""
+        keep = [None if a is None else np.ascontiguousarray(a, np.uint32) for a in (distance, bend, mass, damping)]
+        columns = CalibrationColumns(column_count, 0, *[None if a is None else a.ctypes.data for a in keep])
+        out = np.zeros(max(column_count, 1))
+        baseline = C.c_double()
+        _check(library().numi_cloth_calibration_dlog(self._handle, specimen, C.byref(self.specimens[specimen].desc),
+                                                     C.byref(columns), out.ctypes.data, C.byref(baseline)))
+        return out[:column_count], baseline.value
 
     def read(self, specimen: int) -> dict:
         s = self.specimens[specimen]
@@ -767,6 +798,26 @@ def write_obj(path, particles: np.ndarray, fruits: np.ndarray, grips: np.ndarray
     base = np.ascontiguousarray(config, dtype=CONFIG).reshape(1)
     _check(library().numi_cloth_obj_write(str(path).encode(), _ptr(particles), len(particles), _ptr(fruits),
                                           len(fruits), _ptr(grips), len(grips), base.ctypes.data))
+
+
+def write_calibration(path, names: list[str], rows: list[dict]) -> None:
+    """Observation CSV of the reference's calibration fitter
+    (numi_cloth_calibration_write). Each row: trial, heldout (bool),
+    observable, measured, sigma, baseline, dlog (one value per name)."""
+    encoded = [n.encode() for n in names]
+    name_array = (C.c_char_p * max(len(names), 1))(*encoded)
+    dlogs = [np.ascontiguousarray(r["dlog"], np.float64) for r in rows]
+    for d in dlogs:
+        assert d.shape == (len(names),)
+    records = (CalibrationRow * max(len(rows), 1))()
+    texts = []
+    for k, r in enumerate(rows):
+        trial, observable = r["trial"].encode(), r["observable"].encode()
+        texts += [trial, observable]
+        records[k] = CalibrationRow(trial, 1 if r.get("heldout") else 0, 0, observable, r["measured"], r["sigma"],
+                                    r["baseline"], dlogs[k].ctypes.data)
+    _check(library().numi_cloth_calibration_write(str(path).encode(), len(names), C.cast(name_array, C.c_void_p),
+                                                  records, len(rows)))
 
 
 def state_hash(arrays: dict) -> int:

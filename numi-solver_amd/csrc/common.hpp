@@ -26,6 +26,9 @@ struct StatusError : std::runtime_error {
     throw StatusError(status, message);
 }
 
+// Particle, distance and bend counts of one specimen of a batch.
+void batchSpecimenCounts(const NumiClothHipBatch& batch, uint32_t specimen, uint32_t counts[3]);
+
 template <typename Body>
 int guarded(Body&& body) {
     try {

@@ -62,8 +62,9 @@ struct SpecimenArgs {
     const float2* chordCompliance;
     float* bLambda;
     const uint32_t* tracked;       // merged knot indices, grouped by specimen
-    const float* targets;          // substep x trackedTotal x 3
+    const float* targets;          // substep x trackedTotal x 3 (weights when linear)
     uint32_t trackedTotal;
+    uint32_t linear;               // 1: loss = sum w . x, 0: sum |x - target|^2
     double* loss;                  // per specimen
     uint32_t record;
     uint32_t passes;               // passes per substep (tape stride)
@@ -272,13 +273,25 @@ __global__ void __launch_bounds__(kSpecimenThreads) k_specimen_forward(SpecimenA
             const uint32_t q = info.trackedFirst + t;
             const uint32_t i = a.tracked[q] - k0;
             const float* target = a.targets + (static_cast<size_t>(s) * a.trackedTotal + q) * 3;
-            const float rx = x[i].x - target[0], ry = x[i].y - target[1], rz = x[i].z - target[2];
-            part += static_cast<double>(rx) * rx + static_cast<double>(ry) * ry + static_cast<double>(rz) * rz;
+            float gx0, gx1, gx2;
+            if (a.linear) {
+                gx0 = target[0];
+                gx1 = target[1];
+                gx2 = target[2];
+                part += static_cast<double>(x[i].x) * gx0 + static_cast<double>(x[i].y) * gx1 +
+                        static_cast<double>(x[i].z) * gx2;
+            } else {
+                const float rx = x[i].x - target[0], ry = x[i].y - target[1], rz = x[i].z - target[2];
+                part += static_cast<double>(rx) * rx + static_cast<double>(ry) * ry + static_cast<double>(rz) * rz;
+                gx0 = 2.0f * rx;
+                gx1 = 2.0f * ry;
+                gx2 = 2.0f * rz;
+            }
             if (a.record) {
                 float* out = a.lossGrad + (static_cast<size_t>(j) * a.trackedTotal + q) * 3;
-                out[0] = 2.0f * rx;
-                out[1] = 2.0f * ry;
-                out[2] = 2.0f * rz;
+                out[0] = gx0;
+                out[1] = gx1;
+                out[2] = gx2;
             }
         }
         if (!a.record) {

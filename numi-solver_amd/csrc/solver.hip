@@ -1949,6 +1949,8 @@ void runAdjoint(NumiClothHipBatch& b, const NumiClothHipAdjointProblem& pr) {
         fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "adjoint problem needs substeps, a checkpoint interval, configs, "
                                                     "tracked knots and targets");
     }
+    if (pr.observable != NUMI_CLOTH_ADJOINT_TRACKING && pr.observable != NUMI_CLOTH_ADJOINT_LINEAR)
+        fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "adjoint observable must be TRACKING or LINEAR");
     hipStream_t st = s.stream;
     HIP_OK(hipStreamSynchronize(st));
     // configs with the batch totals, times, tracked knots (merged indices)
@@ -2041,6 +2043,7 @@ void runAdjoint(NumiClothHipBatch& b, const NumiClothHipAdjointProblem& pr) {
     a.tracked = A.tracked.ptr;
     a.targets = A.targets.ptr;
     a.trackedTotal = S * Tk;
+    a.linear = pr.observable == NUMI_CLOTH_ADJOINT_LINEAR ? 1u : 0u;
     a.loss = A.loss.ptr;
     a.passes = passes;
     a.particles = s.P;
@@ -2153,6 +2156,13 @@ void readGradients(NumiClothHipBatch& b, uint32_t specimen, const NumiClothHipSp
 }
 
 }  // namespace
+
+void numi_hip::batchSpecimenCounts(const NumiClothHipBatch& b, uint32_t specimen, uint32_t counts[3]) {
+    if (specimen >= b.specimens) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "specimen out of range");
+    mergeIndices(b, NUMI_CLOTH_HIP_MERGE_PARTICLE, specimen, &counts[0]);
+    mergeIndices(b, NUMI_CLOTH_HIP_MERGE_DISTANCE, specimen, &counts[1]);
+    mergeIndices(b, NUMI_CLOTH_HIP_MERGE_BEND, specimen, &counts[2]);
+}
 
 extern "C" {
 

@@ -300,7 +300,7 @@ extern "C" {
 typedef struct NumiClothOracleAdjointProblem {
     const uint32_t* tracked;       /* knot indices in the loss */
     uint32_t trackedCount;
-    uint32_t reserved;
+    uint32_t observable;           /* 0: sum |x - target|^2, 1: sum target . x (weights) */
     const double* targets;         /* substeps x trackedCount x 3 */
     const double* distanceComplianceScale;  /* D, or NULL */
     const double* bendComplianceScale;      /* B, or NULL */
@@ -341,6 +341,8 @@ int runAdjoint(const NumiClothHipSceneDesc* scene, const NumiClothBagGPUConfig* 
     Model<R> m(*scene, pr);
     const bool gradients = out->distanceCompliance || out->bendCompliance || out->inverseMass || out->damping;
     const uint32_t T = pr.trackedCount;
+    if (problem->observable > 1u) return 1;
+    const bool linear = problem->observable == 1u;
     std::vector<std::vector<Vec>> lossGrad(gradients ? substeps : 0);
     double loss = 0.0;
     for (uint32_t s = 0; s < substeps; ++s) {
@@ -349,9 +351,16 @@ int runAdjoint(const NumiClothHipSceneDesc* scene, const NumiClothBagGPUConfig* 
         if (gradients) lossGrad[s].resize(T);
         for (uint32_t t = 0; t < T; ++t) {
             const double* target = pr.targets + 3 * (size_t(s) * T + t);
-            const Vec r = m.x[pr.tracked[t]] - Vec{R(target[0]), R(target[1]), R(target[2])};
-            loss += double(r.x) * double(r.x) + double(r.y) * double(r.y) + double(r.z) * double(r.z);
-            if (gradients) lossGrad[s][t] = Vec{R(2) * r.x, R(2) * r.y, R(2) * r.z};
+            const Vec w{R(target[0]), R(target[1]), R(target[2])};
+            const Vec& x = m.x[pr.tracked[t]];
+            if (linear) {
+                loss += double(x.x) * double(w.x) + double(x.y) * double(w.y) + double(x.z) * double(w.z);
+                if (gradients) lossGrad[s][t] = w;
+            } else {
+                const Vec r = x - w;
+                loss += double(r.x) * double(r.x) + double(r.y) * double(r.y) + double(r.z) * double(r.z);
+                if (gradients) lossGrad[s][t] = Vec{R(2) * r.x, R(2) * r.y, R(2) * r.z};
+            }
         }
     }
     out->loss = loss;

@@ -102,7 +102,7 @@ def self_candidates_fp32(scene, particles: np.ndarray, config: np.ndarray) -> np
 
 
 class AdjointProblem(C.Structure):
-    _fields_ = [("tracked", C.c_void_p), ("trackedCount", C.c_uint32), ("reserved", C.c_uint32),
+    _fields_ = [("tracked", C.c_void_p), ("trackedCount", C.c_uint32), ("observable", C.c_uint32),
                 ("targets", C.c_void_p), ("distanceComplianceScale", C.c_void_p),
                 ("bendComplianceScale", C.c_void_p), ("inverseMassScale", C.c_void_p), ("damping", C.c_void_p)]
 
@@ -118,8 +118,9 @@ def _array_pointer(array):
 
 def adjoint(scene, configs: np.ndarray, iterations: int, strain_sweeps: int, tracked: np.ndarray,
             targets: np.ndarray, gradients: bool = True, distance_scale=None, bend_scale=None,
-            inverse_mass_scale=None, damping=None, precision: str = "fp64") -> dict:
-    """FP64 loss sum_s sum_k |x_k(s) - target_k(s)|^2 over the substeps and,
+            inverse_mass_scale=None, damping=None, precision: str = "fp64", linear: bool = False) -> dict:
+    """FP64 loss sum_s sum_k |x_k(s) - target_k(s)|^2 over the substeps
+    (linear=True: the observable sum_s sum_k target_k(s) . x_k(s)) and,
     with gradients=True, its gradient with respect to distance and bend
     compliance, inverse mass and damping coefficient (per record / knot).
     The *_scale / damping arrays override parameters in double."""
@@ -138,7 +139,7 @@ def adjoint(scene, configs: np.ndarray, iterations: int, strain_sweeps: int, tra
     assert targets.shape == (len(configs), len(tracked), 3)
     keep = [np.ascontiguousarray(a, np.float64) if a is not None else None
             for a in (distance_scale, bend_scale, inverse_mass_scale, damping)]
-    problem = AdjointProblem(tracked.ctypes.data, len(tracked), 0, targets.ctypes.data,
+    problem = AdjointProblem(tracked.ctypes.data, len(tracked), 1 if linear else 0, targets.ctypes.data,
                              *[_array_pointer(a) for a in keep])
     P, D, B = len(scene.particles), len(scene.distances), len(scene.bends)
     out = {"distanceCompliance": np.zeros(D), "bendCompliance": np.zeros(B), "inverseMass": np.zeros(P),
