@@ -259,7 +259,8 @@ def roofline(cloth, solver, workload: str) -> tuple[dict, dict]:
             "items_per_launch": timing.itemsPerLaunch}
     tile_config = {"enabled": bool(tiles.enabled), "tiles": tiles.tiles, "owned_per_tile": tiles.ownedPerTile,
                    "max_loaded": tiles.maxLoaded, "halo_hops": tiles.halo, "workgroup": tiles.workgroupSize,
-                   "passes_fused": tiles.passes, "redundancy": round(tiles.redundancy, 3)}
+                   "passes_fused": tiles.passes, "staged": bool(tiles.staged),
+                   "redundancy": round(tiles.redundancy, 3)}
     return line, tile_config
 
 

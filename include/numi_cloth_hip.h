@@ -205,7 +205,7 @@ typedef struct NumiClothHipTileInfo {
     uint32_t halo;             /* halo depth in constraint hops */
     uint32_t workgroupSize;
     uint32_t passes;           /* colour passes fused per even-iteration launch */
-    uint32_t reserved;
+    uint32_t staged;           /* 1 when each tile's whole program is staged in LDS first */
     double redundancy;         /* constraint solves per launch / unique ones */
 } NumiClothHipTileInfo;
 

@@ -131,7 +131,7 @@ PASS_DISTANCE, PASS_BEND, PASS_STRAIN, PASS_COLLIDE, PASS_TILE_ITERATION = 0, 1,
 class TileInfo(C.Structure):
     _fields_ = [("enabled", C.c_uint32), ("tiles", C.c_uint32), ("ownedPerTile", C.c_uint32),
                 ("maxLoaded", C.c_uint32), ("halo", C.c_uint32), ("workgroupSize", C.c_uint32),
-                ("passes", C.c_uint32), ("reserved", C.c_uint32), ("redundancy", C.c_double)]
+                ("passes", C.c_uint32), ("staged", C.c_uint32), ("redundancy", C.c_double)]
 
 
 class MaterialValues(C.Structure):

@@ -291,6 +291,9 @@ __global__ void k_self_emit_entries(const float4* __restrict__ boxLo, const floa
 // takes the next slot of its cell's run (the order inside a run depends on
 // the atomics' order; the rank pass below restores it). Afterwards
 // cellSlots[c] holds the end of cell c's run, which is the start of c + 1's.
+// The cells are taken eight at a time (linear index -> x fastest, as in
+// forSegmentCells): all eight slot requests are in flight before the first
+// result is used.
 __global__ void k_self_place_entries(const float4* __restrict__ boxLo, const float4* __restrict__ boxHi,
                                      const SelfGrid* __restrict__ grid, uint32_t* __restrict__ cellSlots,
                                      uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
@@ -299,11 +302,29 @@ __global__ void k_self_place_entries(const float4* __restrict__ boxLo, const flo
     if (g.cellScanLength == 0u || s >= g.segments) return;
     const CellRange r = cellRange(boxLo[s], boxHi[s], g);
     if (r.count == 0) return;
-    forSegmentCells(s, r, g, [&](uint32_t cell, uint32_t value) {
-        const uint32_t at = atomicAdd(&cellSlots[cell], 1u);
-        keys[at] = cell;
-        vals[at] = value;
-    });
+    const uint32_t rx = static_cast<uint32_t>(r.hi[0] - r.lo[0] + 1);
+    const uint32_t ry = static_cast<uint32_t>(r.hi[1] - r.lo[1] + 1);
+    constexpr uint32_t kSlots = 8;
+    for (uint32_t base = 0; base < r.count; base += kSlots) {
+        uint32_t cell[kSlots], value[kSlots], at[kSlots];
+#pragma unroll
+        for (uint32_t u = 0; u < kSlots; ++u) {
+            const uint32_t q = min(base + u, r.count - 1u);
+            const uint32_t x = q % rx, y = (q / rx) % ry, z = q / (rx * ry);
+            const uint32_t starts = (x == 0u ? 1u : 0u) | (y == 0u ? 2u : 0u) | (z == 0u ? 4u : 0u);
+            cell[u] = cellId(r.lo[0] + static_cast<int>(x), r.lo[1] + static_cast<int>(y),
+                             r.lo[2] + static_cast<int>(z), g);
+            value[u] = s | (starts << kEntrySegmentBits);
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kSlots; ++u) at[u] = base + u < r.count ? atomicAdd(&cellSlots[cell[u]], 1u) : 0u;
+#pragma unroll
+        for (uint32_t u = 0; u < kSlots; ++u)
+            if (base + u < r.count) {
+                keys[at[u]] = cell[u];
+                vals[at[u]] = value[u];
+            }
+    }
 }
 
 // Final position of each placed entry: its cell's run start plus the number
@@ -402,19 +423,19 @@ __device__ void forEntryCandidates(uint32_t k, const SelfGrid& g, const float4* 
 // knot lists of both endpoints held in registers (W slots each, padded with
 // ~0u): the locality test of a pair is 4W register compares instead of four
 // binary searches in memory. Same set and order of candidates as
-// forEntryCandidates.
-template <uint32_t W, typename Emit>
-__device__ void forSortedEntryCandidates(uint32_t k, const SelfGrid& g, const SortedEntry* __restrict__ entries,
-                                         const uint2* __restrict__ sortedIJ, const uint32_t* __restrict__ localFixed,
-                                         Emit emit) {
+// forEntryCandidates. entryAt(j) / ijAt(j) return sorted entry j's record
+// and its segment's endpoints (from global memory, or from an LDS copy).
+template <uint32_t W, typename EntryAt, typename IJAt, typename Emit>
+__device__ void forSortedEntryCandidates(uint32_t k, const SelfGrid& g, EntryAt entryAt, IJAt ijAt,
+                                         const uint32_t* __restrict__ localFixed, Emit emit) {
     const uint32_t n = g.sortCount;
-    const SortedEntry self = entries[k];
+    const SortedEntry self = entryAt(k);
     const uint32_t id = __float_as_uint(self.a.x);
     const uint32_t vs = __float_as_uint(self.a.y);
     const uint32_t s = vs & kEntrySegmentMask;
     const float4 lo = make_float4(self.a.z, self.a.w, self.b.x, 0.0f);
     const float4 hi = make_float4(self.b.y, self.b.z, self.b.w, 0.0f);
-    const uint2 es = sortedIJ[k];
+    const uint2 es = ijAt(k);
     uint32_t na[W], nb[W];
     bool loaded = false;
     constexpr uint32_t kUnroll = 4;
@@ -422,7 +443,7 @@ __device__ void forSortedEntryCandidates(uint32_t k, const SelfGrid& g, const So
         SortedEntry e[kUnroll];
 #pragma unroll
         for (uint32_t u = 0; u < kUnroll; ++u)
-            if (j0 + u < n) e[u] = entries[j0 + u];
+            if (j0 + u < n) e[u] = entryAt(j0 + u);
         bool stop = false;
 #pragma unroll
         for (uint32_t u = 0; u < kUnroll; ++u) {
@@ -437,7 +458,7 @@ __device__ void forSortedEntryCandidates(uint32_t k, const SelfGrid& g, const So
             const float4 tLo = make_float4(e[u].a.z, e[u].a.w, e[u].b.x, 0.0f);
             const float4 tHi = make_float4(e[u].b.y, e[u].b.z, e[u].b.w, 0.0f);
             if (!boxesOverlap(lo, hi, tLo, tHi)) continue;
-            const uint2 et = sortedIJ[j];
+            const uint2 et = ijAt(j);
             // A shared knot is in both two-hop lists (each list holds its own
             // knot): local without reading them.
             if (et.x == es.x || et.x == es.y || et.y == es.x || et.y == es.y) continue;
@@ -518,19 +539,55 @@ __device__ __forceinline__ void entryCandidates(uint32_t k, const SelfGrid& g, c
     if constexpr (W == 0) {
         forEntryCandidates(k, g, a.boxLo, a.boxHi, a.ij, a.keys, a.vals, a.localStart, a.localList, emit);
     } else {
-        forSortedEntryCandidates<W>(k, g, a.entries, a.sortedIJ, a.localFixed, emit);
+        const SortedEntry* __restrict__ entries = a.entries;
+        const uint2* __restrict__ sortedIJ = a.sortedIJ;
+        forSortedEntryCandidates<W>(
+            k, g, [&](uint32_t j) { return entries[j]; }, [&](uint32_t j) { return sortedIJ[j]; }, a.localFixed,
+            emit);
     }
 }
 
-// Grid-stride over the live entries (the grid is sized for the capacity).
+constexpr uint32_t kPairChunk = 1024;   // sorted entries per LDS chunk of the candidate count
+constexpr uint32_t kPairThreads = 256;  // workgroup size of k_self_pair_counts
+
+// Candidate count per sorted entry. W > 0: each workgroup copies chunks of
+// kPairChunk consecutive sorted records (and their endpoints) to LDS once
+// and walks them there, so a record is read from memory once instead of
+// once per earlier entry of its cell run; runs that cross the chunk's end
+// continue in global memory. W = 0: grid-stride over the entries.
 template <uint32_t W>
-__global__ void k_self_pair_counts(PairArgs a) {
+__global__ void __launch_bounds__(kPairThreads) k_self_pair_counts(PairArgs a) {
     const SelfGrid g = *a.grid;
     if (!g.valid) return;
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < g.sortCount; k += gridDim.x * blockDim.x) {
-        uint32_t n = 0;
-        entryCandidates<W>(k, g, a, [&](uint32_t, uint32_t) { ++n; });
-        a.counts[k] = n;
+    if constexpr (W == 0) {
+        for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < g.sortCount; k += gridDim.x * blockDim.x) {
+            uint32_t n = 0;
+            entryCandidates<W>(k, g, a, [&](uint32_t, uint32_t) { ++n; });
+            a.counts[k] = n;
+        }
+    } else {
+        __shared__ SortedEntry chunk[kPairChunk];
+        __shared__ uint2 chunkIJ[kPairChunk];
+        const SortedEntry* __restrict__ entries = a.entries;
+        const uint2* __restrict__ sortedIJ = a.sortedIJ;
+        const uint32_t n = g.sortCount;
+        for (uint32_t c0 = blockIdx.x * kPairChunk; c0 < n; c0 += gridDim.x * kPairChunk) {
+            const uint32_t cn = min(kPairChunk, n - c0);
+            for (uint32_t i = threadIdx.x; i < cn; i += blockDim.x) {
+                chunk[i] = entries[c0 + i];
+                chunkIJ[i] = sortedIJ[c0 + i];
+            }
+            __syncthreads();
+            const auto entryAt = [&](uint32_t j) { return j - c0 < cn ? chunk[j - c0] : entries[j]; };
+            const auto ijAt = [&](uint32_t j) { return j - c0 < cn ? chunkIJ[j - c0] : sortedIJ[j]; };
+            for (uint32_t i = threadIdx.x; i < cn; i += blockDim.x) {
+                uint32_t count = 0;
+                forSortedEntryCandidates<W>(c0 + i, g, entryAt, ijAt, a.localFixed,
+                                            [&](uint32_t, uint32_t) { ++count; });
+                a.counts[c0 + i] = count;
+            }
+            __syncthreads();
+        }
     }
 }
 

@@ -22,6 +22,7 @@
 namespace numi_dev {
 
 constexpr uint32_t kMaxTilePasses = 64;
+constexpr uint32_t kTileStaticLds = 2048;  // bound on k_tile_program's static LDS (checked below)
 constexpr uint32_t kTileDistanceKind = 0;  // values of numi_hip::TilePassKind
 constexpr uint32_t kTileKnotKind = 1;
 constexpr uint32_t kTileBendKind = 2;
@@ -60,6 +61,10 @@ struct TileProgramArgs {
     uint32_t* clearTouched;
     uint32_t leadingCollide;    // 1: the ground/sphere pass before the first colour pass
     uint32_t trailingCollides;
+    // Staged programs: LDS offset (float4 units) of the staged items and the
+    // item capacity; the staged input multipliers follow the items.
+    uint32_t stageOffset;
+    uint32_t stageItems;
     uint32_t* failure;
     uint32_t kinds[kMaxTilePasses];
 };
@@ -189,13 +194,21 @@ __device__ __forceinline__ bool tileItem(float4* lds, uint32_t kind, const uint4
 // the barrier instead of following each other. Slots beyond those (tiles
 // with more than kTilePrefetch x workgroup items in one pass) load in place.
 //
+// Staged = true: the tile's items of every pass, and the input multiplier of
+// each, are copied to LDS before the first pass, all requests in flight at
+// once (items first, then the multipliers they index, alongside the knot
+// gather), so the passes wait on no global load at all. Used when the
+// tile's whole program fits in LDS and the autotuner finds it faster: tiles
+// with one workgroup per CU, whose passes would otherwise each wait out a
+// global load latency.
+//
 // Strain programs stop after the first sweep (all distance colours) that
 // moves no knot of the tile: strain passes keep no state besides the knots,
 // so every later sweep would repeat the same checks on the same positions
 // and move nothing either. The owned knots are the same bytes.
 // Strain = true: a strain-limit program (strain passes only, early exit);
 // false: an iteration program (distance, knot and bend passes).
-template <bool Strain>
+template <bool Strain, bool Staged>
 __global__ void k_tile_program(TileProgramArgs a) {
     extern __shared__ float4 lds[];
     __shared__ SphereWindow window;
@@ -207,6 +220,9 @@ __global__ void k_tile_program(TileProgramArgs a) {
     // copy the whole argument block to scratch.
     __shared__ const float* lambdaIn[3];
     __shared__ float* lambdaOut[3];
+    static_assert(sizeof(window) + sizeof(scratch) + sizeof(starts) + sizeof(kinds) + sizeof(lambdaIn) +
+                      sizeof(lambdaOut) <= kTileStaticLds,
+                  "static LDS beyond the host's reservation");
     if (threadIdx.x == 0) {
         lambdaIn[kTileDistanceKind] = a.distanceLambdaIn;
         lambdaIn[kTileKnotKind] = a.knotLambdaIn;
@@ -236,18 +252,72 @@ __global__ void k_tile_program(TileProgramArgs a) {
     const uint32_t t = threadIdx.x, stride = blockDim.x;
     uint4 cur[kTilePrefetch], next[kTilePrefetch], ahead[kTilePrefetch];
     float lam[kTilePrefetch], nextLam[kTilePrefetch];
-    tileLoadItems(items, itemStart, passes, 0, t, stride, cur);
-    tileLoadItems(items, itemStart, passes, 1, t, stride, next);
-    if constexpr (!Strain) {
-        const uint32_t kind0 = a.kinds[0];
-        const float* in0 = kind0 == kTileDistanceKind ? a.distanceLambdaIn
-                                                      : (kind0 == kTileKnotKind ? a.knotLambdaIn : a.bendLambdaIn);
-        tileLoadLambdas(in0, itemStart, passes, 0, kind0, t, stride, cur, lam);
+    uint4* const stagedItems = reinterpret_cast<uint4*>(lds + a.stageOffset);
+    float* const stagedLambdas = reinterpret_cast<float*>(stagedItems + a.stageItems);
+    const uint32_t itemBase = itemStart[0];
+    if constexpr (Staged) {
+        // Two phases, each with every request of the workgroup in flight at
+        // once: (1) the knot ids into registers and the items of every pass
+        // into LDS; (2) after a barrier, the knot positions by id and the
+        // input multiplier of every item. Slot arrays are filled for every
+        // slot (indices clamped to the last element) so they stay in
+        // registers; only live slots are stored.
+        const uint32_t count = itemStart[passes] - itemBase;
+        constexpr uint32_t kKnotSlots = 4, kItemSlots = 8;
+        uint32_t ids[kKnotSlots];
+        const uint32_t lastKnot = loaded > 0u ? loaded - 1u : 0u;
+#pragma unroll
+        for (uint32_t u = 0; u < kKnotSlots; ++u)
+            ids[u] = loaded > 0u ? a.loadIds[base + min(t + u * stride, lastKnot)] : 0u;
+        for (uint32_t j0 = t; j0 < count; j0 += kItemSlots * stride) {
+            uint4 v[kItemSlots];
+#pragma unroll
+            for (uint32_t u = 0; u < kItemSlots; ++u) v[u] = items[itemBase + min(j0 + u * stride, count - 1u)];
+#pragma unroll
+            for (uint32_t u = 0; u < kItemSlots; ++u)
+                if (j0 + u * stride < count) stagedItems[j0 + u * stride] = v[u];
+        }
+        if (collides)
+            for (uint32_t k = t; k < a.sphereCount; k += stride) sphereCache[k] = a.spheres[k];
+        __syncthreads();
+        float4 p[kKnotSlots];
+#pragma unroll
+        for (uint32_t u = 0; u < kKnotSlots; ++u) p[u] = a.posIn[ids[u]];
+        if constexpr (!Strain) {
+            for (uint32_t j0 = t; j0 < count; j0 += kItemSlots * stride) {
+                float v[kItemSlots];
+#pragma unroll
+                for (uint32_t u = 0; u < kItemSlots; ++u) {
+                    const uint32_t j = min(j0 + u * stride, count - 1u);
+                    uint32_t k = 0;
+                    while (starts[k + 1] - itemBase <= j) ++k;
+                    const uint32_t kind = kinds[k];
+                    v[u] = lambdaIn[kind][tileItemConstraint(kind, stagedItems[j])];
+                }
+#pragma unroll
+                for (uint32_t u = 0; u < kItemSlots; ++u)
+                    if (j0 + u * stride < count) stagedLambdas[j0 + u * stride] = v[u];
+            }
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kKnotSlots; ++u)
+            if (t + u * stride < loaded) lds[t + u * stride] = p[u];
+        for (uint32_t i = t + kKnotSlots * stride; i < loaded; i += stride) lds[i] = a.posIn[a.loadIds[base + i]];
+        __syncthreads();
+    } else {
+        tileLoadItems(items, itemStart, passes, 0, t, stride, cur);
+        tileLoadItems(items, itemStart, passes, 1, t, stride, next);
+        if constexpr (!Strain) {
+            const uint32_t kind0 = a.kinds[0];
+            const float* in0 = kind0 == kTileDistanceKind ? a.distanceLambdaIn
+                                                          : (kind0 == kTileKnotKind ? a.knotLambdaIn : a.bendLambdaIn);
+            tileLoadLambdas(in0, itemStart, passes, 0, kind0, t, stride, cur, lam);
+        }
+        for (uint32_t i = t; i < loaded; i += stride) lds[i] = a.posIn[a.loadIds[base + i]];
+        if (collides)
+            for (uint32_t k = t; k < a.sphereCount; k += stride) sphereCache[k] = a.spheres[k];
+        __syncthreads();
     }
-    for (uint32_t i = t; i < loaded; i += stride) lds[i] = a.posIn[a.loadIds[base + i]];
-    if (collides)
-        for (uint32_t k = t; k < a.sphereCount; k += stride) sphereCache[k] = a.spheres[k];
-    __syncthreads();
     const bool live = *a.ok != 0u;
     const float dt = a.cfg->gravityAndTimestep.w;
     const float invDt2 = 1.0f / (dt * dt);
@@ -280,25 +350,34 @@ __global__ void k_tile_program(TileProgramArgs a) {
         tileCollide(lds, loaded, owned, a, ground, floorZ, sphereCache, culled, window, scratch);
     bool sweepMoved = false;
     for (uint32_t k = 0; k < passes; ++k) {
-        if constexpr (!Strain)
-            if (k + 1 < passes)
-                tileLoadLambdas(lambdaIn[kinds[k + 1]], itemStart, passes, k + 1, kinds[k + 1], t, stride, next,
-                                nextLam);
-        tileLoadItems(items, itemStart, passes, k + 2, t, stride, ahead);
         const uint32_t kind = kinds[k];
-        const float* in = Strain ? nullptr : lambdaIn[kind];
         float* out = Strain ? nullptr : lambdaOut[kind];
         const uint32_t first = starts[k], last = starts[k + 1];
         bool moved = false;
+        if constexpr (Staged) {
+            for (uint32_t j = first - itemBase + t; j < last - itemBase; j += stride) {
+                const float lj = Strain ? 0.0f : stagedLambdas[j];
+                moved |= tileItem<Strain>(lds, kind, stagedItems[j], lj, out, knotCC, failure, live, dt, invDt2,
+                                          ground, floorZ);
+            }
+        } else {
+            if constexpr (!Strain)
+                if (k + 1 < passes)
+                    tileLoadLambdas(lambdaIn[kinds[k + 1]], itemStart, passes, k + 1, kinds[k + 1], t, stride, next,
+                                    nextLam);
+            tileLoadItems(items, itemStart, passes, k + 2, t, stride, ahead);
+            const float* in = Strain ? nullptr : lambdaIn[kind];
 #pragma unroll
-        for (uint32_t m = 0; m < kTilePrefetch; ++m)
-            if (first + t + m * stride < last)
-                moved |= tileItem<Strain>(lds, kind, cur[m], lam[m], out, knotCC, failure, live, dt, invDt2, ground,
+            for (uint32_t m = 0; m < kTilePrefetch; ++m)
+                if (first + t + m * stride < last)
+                    moved |= tileItem<Strain>(lds, kind, cur[m], lam[m], out, knotCC, failure, live, dt, invDt2,
+                                              ground, floorZ);
+            for (uint32_t j = first + t + kTilePrefetch * stride; j < last; j += stride) {
+                const uint4 item = items[j];
+                const float lj = Strain ? 0.0f : in[tileItemConstraint(kind, item)];
+                moved |= tileItem<Strain>(lds, kind, item, lj, out, knotCC, failure, live, dt, invDt2, ground,
                                           floorZ);
-        for (uint32_t j = first + t + kTilePrefetch * stride; j < last; j += stride) {
-            const uint4 item = items[j];
-            const float lj = Strain ? 0.0f : in[tileItemConstraint(kind, item)];
-            moved |= tileItem<Strain>(lds, kind, item, lj, out, knotCC, failure, live, dt, invDt2, ground, floorZ);
+            }
         }
         if constexpr (Strain) {
             sweepMoved |= __syncthreads_or(moved ? 1 : 0) != 0;
@@ -310,11 +389,13 @@ __global__ void k_tile_program(TileProgramArgs a) {
             (void)moved;
             __syncthreads();
         }
+        if constexpr (!Staged) {
 #pragma unroll
-        for (uint32_t m = 0; m < kTilePrefetch; ++m) {
-            cur[m] = next[m];
-            next[m] = ahead[m];
-            lam[m] = nextLam[m];
+            for (uint32_t m = 0; m < kTilePrefetch; ++m) {
+                cur[m] = next[m];
+                next[m] = ahead[m];
+                lam[m] = nextLam[m];
+            }
         }
     }
     if (live)

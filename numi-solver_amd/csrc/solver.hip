@@ -172,6 +172,9 @@ struct NumiClothHipSolver {
     struct TileProgram {
         bool valid = false;
         uint32_t tiles = 0, passes = 0, maxLoaded = 0, block = 256, owned = 0, halo = 0;
+        uint32_t maxItems = 0;     // most items of one tile, all passes
+        bool stageable = false;    // the whole program of every tile fits in LDS
+        bool staged = false;       // run the staged kernel
         bool hasDistance = false, hasBend = false, hasKnot = false;
         std::vector<uint32_t> kinds;
         DeviceArray<uint32_t> loadStart, loadIds, ownedCount, itemStart;
@@ -182,6 +185,8 @@ struct NumiClothHipSolver {
     bool useTiles = false;
     TileProgram tileEven, tileOdd, tileStrain;
     uint32_t tileOwned = 0, tileBlock = 256, tileCapacity = 0, tileStrainSweeps = ~0u;
+    bool tileStaged = false;            // the autotuner's choice for every program that can stage
+    uint64_t ldsBytes = 0;              // dynamic LDS available to a tile program
     std::vector<float4> initialPositions;  // create-time layout used to order tiles
     DeviceArray<float4> posAlt;
     DeviceArray<float> dLambdaAlt, bLambdaAlt, kLambdaAlt;
@@ -456,7 +461,7 @@ struct NumiClothHipSolver {
                                                                                      : k_self_pair_counts<0>);
         auto emit = localWidth == 16 ? k_self_emit_pairs<16> : (localWidth == 32 ? k_self_emit_pairs<32>
                                                                                   : k_self_emit_pairs<0>);
-        counts<<<entryBlocks, kScanThreads, 0, stream>>>(pa);
+        counts<<<entryBlocks, kPairThreads, 0, stream>>>(pa);
         launchScan(entryCounts.ptr, &g->sortCount, entryCapacity, totals.ptr + 1);
         emit<<<entryBlocks, kScanThreads, 0, stream>>>(pa);
     }
@@ -557,14 +562,28 @@ struct NumiClothHipSolver {
         a.failure = failure.ptr;
         for (uint32_t k = 0; k < t.passes; ++k) a.kinds[k] = t.kinds[k];
         const bool collides = a.leadingCollide != 0 || a.trailingCollides != 0;
-        const size_t shared = (t.maxLoaded + (collides ? 2 * a.sphereCount : 0u)) * sizeof(float4);
-        if (a.sweepPasses != 0) k_tile_program<true><<<t.tiles, t.block, shared, stream>>>(a);
-        else k_tile_program<false><<<t.tiles, t.block, shared, stream>>>(a);
+        a.stageOffset = t.maxLoaded + (collides ? 2 * a.sphereCount : 0u);
+        a.stageItems = t.staged ? t.maxItems : 0u;
+        const size_t shared = a.stageOffset * sizeof(float4) + (t.staged ? stagedBytes(t.maxItems) : 0u);
+        const bool strain = a.sweepPasses != 0;
+        if (t.staged) {
+            if (strain) k_tile_program<true, true><<<t.tiles, t.block, shared, stream>>>(a);
+            else k_tile_program<false, true><<<t.tiles, t.block, shared, stream>>>(a);
+        } else {
+            if (strain) k_tile_program<true, false><<<t.tiles, t.block, shared, stream>>>(a);
+            else k_tile_program<false, false><<<t.tiles, t.block, shared, stream>>>(a);
+        }
         if (scratch) return;
         posCur = a.posOut;
         if (t.hasDistance) dLamCur = a.distanceLambdaOut;
         if (t.hasBend) bLamCur = a.bendLambdaOut;
         if (t.hasKnot) kLamCur = a.knotLambdaOut;
+    }
+
+    // LDS of a staged program beyond the knots and spheres: 16-byte items and
+    // their input multipliers, rounded to whole float4.
+    static size_t stagedBytes(uint32_t items) {
+        return static_cast<size_t>(items) * sizeof(uint4) + (static_cast<size_t>(items) * sizeof(float) + 15) / 16 * 16;
     }
 
     // Moves the live state back to the primary buffers at the end of a substep.
@@ -1315,6 +1334,13 @@ bool buildTileProgram(NumiClothHipSolver& s, const std::vector<TilePass>& passes
     out.ownedCount.upload(plan.ownedCount.data(), plan.ownedCount.size(), st);
     out.itemStart.allocate(plan.itemStart.size());
     out.itemStart.upload(plan.itemStart.data(), plan.itemStart.size(), st);
+    for (uint32_t tile = 0; tile < plan.tiles; ++tile)
+        out.maxItems = std::max(out.maxItems, plan.itemStart[(tile + 1) * out.passes] - plan.itemStart[tile * out.passes]);
+    // Staged: knots, both sphere lists and the whole program in LDS.
+    const uint64_t stagedLds = (static_cast<uint64_t>(out.maxLoaded) + 2ull * s.extension.staticSphereCount) * 16u +
+                               NumiClothHipSolver::stagedBytes(out.maxItems);
+    out.stageable = stagedLds <= s.ldsBytes;
+    out.staged = out.stageable && s.tileStaged;
     out.items.allocate(plan.items.size() / 4);
     out.items.upload(reinterpret_cast<const uint4*>(plan.items.data()), plan.items.size() / 4, st);
     HIP_OK(hipStreamSynchronize(st));
@@ -1351,10 +1377,12 @@ void setupTiles(NumiClothHipSolver& s) {
     HIP_OK(hipDeviceGetAttribute(&maxShared, hipDeviceAttributeMaxSharedMemoryPerBlock, s.device));
     const uint32_t spheres = s.extension.staticSphereCount;
     // Dynamic LDS: the loaded knots, the spheres and a sphere window's list;
-    // 1 KiB stays free for the kernel's static window and reduction scratch.
-    const uint64_t reserved = 1024 + 2ull * spheres * sizeof(float4);
+    // 2 KiB stay free for the kernel's static LDS (sphere window, reduction
+    // scratch, pass starts and kinds: 1,124 bytes).
+    const uint64_t reserved = kTileStaticLds + 2ull * spheres * sizeof(float4);
     if (reserved + 64 * sizeof(float4) >= static_cast<uint64_t>(maxShared)) return;
     s.tileCapacity = static_cast<uint32_t>((static_cast<uint64_t>(maxShared) - reserved) / sizeof(float4));
+    s.ldsBytes = static_cast<uint64_t>(maxShared) - kTileStaticLds;
     s.posAlt.allocate(s.P);
     s.dLambdaAlt.allocate(s.D);
     s.bLambdaAlt.allocate(s.B);
@@ -1366,29 +1394,39 @@ void setupTiles(NumiClothHipSolver& s) {
     const std::vector<TilePass> even = tilePasses(s, true, 0);
     double best = 1e30;
     uint32_t bestOwned = 0, bestBlock = 256;
+    bool bestStaged = false;
     std::vector<uint32_t> sizes;
     if (s.P <= 4096) sizes.push_back(s.P);
     // Smaller tiles only add halo work once the tiles of 4096 knots already
-    // give every CU a workgroup, so large scenes try fewer sizes.
-    const uint32_t smallest = s.P >= 1024u * 4096u ? 2048u : (s.P >= 256u * 4096u ? 1024u : 256u);
-    for (uint32_t owned : {4096u, 2048u, 1024u, 512u, 256u})
+    // give every CU a workgroup, so large scenes try fewer sizes; scenes of
+    // at most 256 x 1024 knots also try tiles of 128 and 64 knots, which put
+    // several workgroups on each CU to hide the projections' latency.
+    const uint32_t smallest = s.P >= 1024u * 4096u ? 2048u
+                              : (s.P >= 256u * 4096u ? 1024u : (s.P > 256u * 1024u ? 256u : 64u));
+    for (uint32_t owned : {4096u, 2048u, 1024u, 512u, 256u, 128u, 64u})
         if (owned < s.P && owned >= smallest) sizes.push_back(owned);
     for (uint32_t owned : sizes) {
         NumiClothHipSolver::TileProgram candidate;
         if (!buildTileProgram(s, even, owned, candidate)) continue;
-        for (uint32_t block : {256u, 512u, 1024u}) {
-            candidate.block = block;
-            const double ms = timeTileProgram(s, candidate, 5);
-            if (ms < best) {
-                best = ms;
-                bestOwned = owned;
-                bestBlock = block;
+        for (uint32_t block : {256u, 512u, 1024u})
+            for (bool staged : {false, true}) {
+                if (staged && !candidate.stageable) continue;
+                candidate.block = block;
+                candidate.staged = staged;
+                const double ms = timeTileProgram(s, candidate, 5);
+                if (ms < best) {
+                    best = ms;
+                    bestOwned = owned;
+                    bestBlock = block;
+                    bestStaged = staged;
+                }
             }
-        }
     }
     if (bestOwned == 0) return;
     s.tileOwned = bestOwned;
     s.tileBlock = bestBlock;
+    s.tileStaged = bestStaged;
+    if (const char* env = std::getenv("NUMI_CLOTH_HIP_TILE_STAGED")) s.tileStaged = std::strcmp(env, "0") != 0;
     if (!buildTileProgram(s, even, bestOwned, s.tileEven)) return;
     buildTileProgram(s, tilePasses(s, false, 0), bestOwned, s.tileOdd);
     s.useTiles = true;
@@ -2186,6 +2224,7 @@ int numi_cloth_hip_tile_info(NumiClothHipSolver* solver, NumiClothHipTileInfo* o
         out->halo = t.halo;
         out->workgroupSize = t.block;
         out->passes = t.passes;
+        out->staged = t.valid && t.staged ? 1u : 0u;
         out->redundancy = t.unique != 0 ? static_cast<double>(t.computed) / static_cast<double>(t.unique) : 0.0;
     });
 }

@@ -16,19 +16,24 @@ from test_gpu_self import two_layer_impact
 pytestmark = pytest.mark.gpu
 
 
-def _run(cloth, scene, substeps, iterations, sweeps, tiles):
-    previous = os.environ.get("NUMI_CLOTH_HIP_TILES")
+def _run(cloth, scene, substeps, iterations, sweeps, tiles, staged=None):
+    """staged: None lets the autotuner choose; True / False force the staged
+    or the streaming tile kernel (NUMI_CLOTH_HIP_TILE_STAGED, read at create)."""
+    saved = {k: os.environ.get(k) for k in ("NUMI_CLOTH_HIP_TILES", "NUMI_CLOTH_HIP_TILE_STAGED")}
     os.environ["NUMI_CLOTH_HIP_TILES"] = "1" if tiles else "0"
+    if staged is not None:
+        os.environ["NUMI_CLOTH_HIP_TILE_STAGED"] = "1" if staged else "0"
     try:
         with cloth.Solver(scene) as solver:
             info = solver.tile_info()
             solver.step(scene.configs(substeps), iterations, sweeps)
             return solver.read(), info
     finally:
-        if previous is None:
-            del os.environ["NUMI_CLOTH_HIP_TILES"]
-        else:
-            os.environ["NUMI_CLOTH_HIP_TILES"] = previous
+        for key, value in saved.items():
+            if value is None:
+                os.environ.pop(key, None)
+            else:
+                os.environ[key] = value
 
 
 SCENES = {
@@ -43,15 +48,20 @@ SCENES = {
 }
 
 
+@pytest.mark.parametrize("staged", [False, True])
 @pytest.mark.parametrize("name", sorted(SCENES))
-def test_tiles_equal_per_colour_schedule(cloth, name):
+def test_tiles_equal_per_colour_schedule(cloth, name, staged):
+    """Both tile kernels (whole program staged in LDS first, or items and
+    multipliers streamed pass by pass) against the per-colour launches."""
     scene = SCENES[name](cloth)
     substeps, iterations, sweeps = (1, 0, 1) if name == "strain_probe" else (12, 9, 3)
-    tiled, info = _run(cloth, scene, substeps, iterations, sweeps, True)
+    tiled, info = _run(cloth, scene, substeps, iterations, sweeps, True, staged)
     plain, off = _run(cloth, scene, substeps, iterations, sweeps, False)
     print(f"{name}: tiles={info.tiles} owned={info.ownedPerTile} loaded<= {info.maxLoaded} halo={info.halo} "
-          f"block={info.workgroupSize} redundancy={info.redundancy:.2f}")
+          f"block={info.workgroupSize} staged={info.staged} redundancy={info.redundancy:.2f}")
     assert info.enabled == 1 and off.enabled == 0
+    if not staged:
+        assert info.staged == 0
     for key in ["particles", "distances", "bends", "knots", "grips", "selfStatus", "frictionStatus"]:
         assert tiled[key].tobytes() == plain[key].tobytes(), key
     assert tiled["failure"] == plain["failure"] == 0
