@@ -547,32 +547,34 @@ __device__ __forceinline__ void entryCandidates(uint32_t k, const SelfGrid& g, c
     }
 }
 
-constexpr uint32_t kPairChunk = 1024;   // sorted entries per LDS chunk of the candidate count
 constexpr uint32_t kPairThreads = 256;  // workgroup size of k_self_pair_counts
 
-// Candidate count per sorted entry. W > 0: each workgroup copies chunks of
-// kPairChunk consecutive sorted records (and their endpoints) to LDS once
-// and walks them there, so a record is read from memory once instead of
-// once per earlier entry of its cell run; runs that cross the chunk's end
-// continue in global memory. W = 0: grid-stride over the entries.
-template <uint32_t W>
+// Candidate count per sorted entry. Chunk > 0 (W > 0): each workgroup
+// copies chunks of Chunk consecutive sorted records (and their endpoints)
+// to LDS once and walks them there, so a record is read from memory once
+// instead of once per earlier entry of its cell run; runs that cross the
+// chunk's end continue in global memory. Used with 4 entries per thread on
+// scenes of tens of millions of entries, where it is faster; smaller scenes
+// (Chunk = 0) keep one thread per entry walking global memory, which keeps
+// more walks in flight.
+template <uint32_t W, uint32_t Chunk>
 __global__ void __launch_bounds__(kPairThreads) k_self_pair_counts(PairArgs a) {
     const SelfGrid g = *a.grid;
     if (!g.valid) return;
-    if constexpr (W == 0) {
+    if constexpr (W == 0 || Chunk == 0) {
         for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < g.sortCount; k += gridDim.x * blockDim.x) {
             uint32_t n = 0;
             entryCandidates<W>(k, g, a, [&](uint32_t, uint32_t) { ++n; });
             a.counts[k] = n;
         }
     } else {
-        __shared__ SortedEntry chunk[kPairChunk];
-        __shared__ uint2 chunkIJ[kPairChunk];
+        __shared__ SortedEntry chunk[Chunk];
+        __shared__ uint2 chunkIJ[Chunk];
         const SortedEntry* __restrict__ entries = a.entries;
         const uint2* __restrict__ sortedIJ = a.sortedIJ;
         const uint32_t n = g.sortCount;
-        for (uint32_t c0 = blockIdx.x * kPairChunk; c0 < n; c0 += gridDim.x * kPairChunk) {
-            const uint32_t cn = min(kPairChunk, n - c0);
+        for (uint32_t c0 = blockIdx.x * Chunk; c0 < n; c0 += gridDim.x * Chunk) {
+            const uint32_t cn = min(Chunk, n - c0);
             for (uint32_t i = threadIdx.x; i < cn; i += blockDim.x) {
                 chunk[i] = entries[c0 + i];
                 chunkIJ[i] = sortedIJ[c0 + i];

@@ -457,8 +457,12 @@ struct NumiClothHipSolver {
             k_self_gather<<<entryBlocks, kScanThreads, 0, stream>>>(boxLo.ptr, boxHi.ptr, dIJ.ptr, g, keysIn, valsIn,
                                                                    sortedEntries.ptr, sortedIJ.ptr);
         }
-        auto counts = localWidth == 16 ? k_self_pair_counts<16> : (localWidth == 32 ? k_self_pair_counts<32>
-                                                                                     : k_self_pair_counts<0>);
+        // LDS chunks of 4 entries per thread once the scene has 2^20
+        // segments (tens of millions of entries); one thread per entry below.
+        const bool wide = D >= (1u << 20);
+        auto counts = localWidth == 16 ? (wide ? k_self_pair_counts<16, 4 * kPairThreads> : k_self_pair_counts<16, 0>)
+                      : localWidth == 32 ? (wide ? k_self_pair_counts<32, 4 * kPairThreads> : k_self_pair_counts<32, 0>)
+                                         : k_self_pair_counts<0, 0>;
         auto emit = localWidth == 16 ? k_self_emit_pairs<16> : (localWidth == 32 ? k_self_emit_pairs<32>
                                                                                   : k_self_emit_pairs<0>);
         counts<<<entryBlocks, kPairThreads, 0, stream>>>(pa);
