@@ -217,6 +217,7 @@ __global__ void __launch_bounds__(kYarnThreads) k_yarn_solve(YarnArgs a) {
     __shared__ uint32_t scanScratch[kYarnThreads];
     __shared__ uint32_t candidates[kYarnSlots];  // pair indices, unordered
     __shared__ uint32_t slotPair[kYarnSlots];     // pair indices in phase order
+    __shared__ uint32_t slotPhase[kYarnSlots];    // the phase of each slot
     __shared__ YarnStage slotStage[kYarnSlots];
     __shared__ NumiClothBagGPUFruit fruitStart[32];
     __shared__ float fruitMoved[32];
@@ -265,43 +266,46 @@ __global__ void __launch_bounds__(kYarnThreads) k_yarn_solve(YarnArgs a) {
         for (uint32_t k = tid; k < total; k += threads) {
             const uint32_t idx = candidates[k];
             const uint32_t f = idx / D, s = idx % D;
-            const uint32_t slot = atomicAdd(&counts[phaseBit(f, s, a.segmentBatch[s])], 1u);
+            const uint32_t phase = phaseBit(f, s, a.segmentBatch[s]);
+            const uint32_t slot = atomicAdd(&counts[phase], 1u);
             slotPair[slot] = idx;
+            slotPhase[slot] = phase;
             const NumiClothBagGPUYarnContact& c = a.contacts[idx];
             slotStage[slot] = YarnStage{c.fruitNormalAndImpulse, c.segmentImpulse, c.control.w, 0u};
         }
     }
     __syncthreads();
-    // 2. candidate visits in phase order on wave 0
+    // 2. candidate visits in phase order on wave 0: the slots are grouped by
+    // phase in increasing phase order (the scan above), so the wave walks
+    // the non-empty groups only; a group's fruits are distinct, and fruits
+    // f and f + n of a colour of n segments take turns (rounds).
     if (verdict == 0u && tid < 64) {
         const uint32_t lane = tid;
         const bool live = lane < F;
         NumiClothBagGPUFruit fruit{};
         if (live) fruit = fruitStart[lane];
         float travel = 0.0f;
-        for (uint32_t batch = 0; batch < a.batchCount; ++batch) {
-            const uint4 b = a.batches[batch];
-            const uint32_t rounds = (F + b.y - 1) / b.y;
-            for (uint32_t bit = b.x; bit < b.x + b.y; ++bit) {
-                const uint32_t first = bit == 0 ? 0u : counts[bit - 1];
-                const uint32_t last = counts[bit];
-                if (first == last) continue;
-                for (uint32_t round = 0; round < rounds; ++round) {
-                    if (live && lane / b.y == round) {
-                        for (uint32_t k = first; k < last; ++k) {
-                            const uint32_t idx = slotPair[k];
-                            if (idx / D != lane) continue;
-                            const uint32_t s = idx % D;
-                            YarnStage stage = slotStage[k];
-                            bool acted = false;
-                            travel += yarnVisit(a, fruit, a.contacts[idx], stage, knots, a.ij[s], moved, acted);
-                            stage.acted = acted ? 1u : 0u;
-                            slotStage[k] = stage;
-                        }
+        for (uint32_t first = 0; first < total;) {
+            const uint32_t phase = slotPhase[first];
+            const uint32_t last = counts[phase];  // end of the phase's group
+            const uint32_t n = a.segmentBatch[phase].y;
+            const uint32_t rounds = (F + n - 1) / n;
+            for (uint32_t round = 0; round < rounds; ++round) {
+                if (live && lane / n == round) {
+                    for (uint32_t k = first; k < last; ++k) {
+                        const uint32_t idx = slotPair[k];
+                        if (idx / D != lane) continue;
+                        const uint32_t s = idx % D;
+                        YarnStage stage = slotStage[k];
+                        bool acted = false;
+                        travel += yarnVisit(a, fruit, a.contacts[idx], stage, knots, a.ij[s], moved, acted);
+                        stage.acted = acted ? 1u : 0u;
+                        slotStage[k] = stage;
                     }
-                    waveSync();
                 }
+                waveSync();
             }
+            first = last;
         }
         if (live) {
             fruitMoved[lane] = travel;
@@ -421,6 +425,7 @@ __global__ void __launch_bounds__(kYarnThreads) k_yarn_friction_solve(YarnArgs a
     extern __shared__ float4 yarnLds[];
     __shared__ uint32_t scanScratch[kYarnThreads];
     __shared__ uint32_t slotPair[kYarnSlots];
+    __shared__ uint32_t slotPhase[kYarnSlots];
     __shared__ uint32_t overflow;
     if (*a.ok == 0u) return;
     const float mu = a.cfg->fruitMaterial.w;
@@ -451,7 +456,11 @@ __global__ void __launch_bounds__(kYarnThreads) k_yarn_friction_solve(YarnArgs a
     } else {
         for (uint32_t idx = tid; idx < pairs; idx += threads) {
             const uint32_t f = idx / D, s = idx % D;
-            if (acts(idx)) slotPair[atomicAdd(&counts[phaseBit(f, s, a.segmentBatch[s])], 1u)] = idx;
+            if (!acts(idx)) continue;
+            const uint32_t phase = phaseBit(f, s, a.segmentBatch[s]);
+            const uint32_t slot = atomicAdd(&counts[phase], 1u);
+            slotPair[slot] = idx;
+            slotPhase[slot] = phase;
         }
     }
     __syncthreads();
@@ -460,11 +469,11 @@ __global__ void __launch_bounds__(kYarnThreads) k_yarn_friction_solve(YarnArgs a
         const bool live = lane < F;
         NumiClothBagGPUFruit fruit{};
         if (live) fruit = a.fruits[lane];
-        for (uint32_t batch = 0; batch < a.batchCount; ++batch) {
-            const uint4 b = a.batches[batch];
-            const uint32_t rounds = (F + b.y - 1) / b.y;
-            if (overflow != 0u) {
-                // every visit, in the reference order
+        if (overflow != 0u) {
+            // every visit, in the reference order
+            for (uint32_t batch = 0; batch < a.batchCount; ++batch) {
+                const uint4 b = a.batches[batch];
+                const uint32_t rounds = (F + b.y - 1) / b.y;
                 for (uint32_t step = 0; step < b.y * rounds; ++step) {
                     const uint32_t phase = step / rounds, round = step % rounds;
                     if (live && lane / b.y == round) {
@@ -473,14 +482,16 @@ __global__ void __launch_bounds__(kYarnThreads) k_yarn_friction_solve(YarnArgs a
                     }
                     waveSync();
                 }
-                continue;
             }
-            for (uint32_t bit = b.x; bit < b.x + b.y; ++bit) {
-                const uint32_t first = bit == 0 ? 0u : counts[bit - 1];
-                const uint32_t last = counts[bit];
-                if (first == last) continue;
+        } else {
+            // the non-empty phase groups only, in phase order
+            for (uint32_t first = 0; first < total;) {
+                const uint32_t phase = slotPhase[first];
+                const uint32_t last = counts[phase];
+                const uint32_t n = a.segmentBatch[phase].y;
+                const uint32_t rounds = (F + n - 1) / n;
                 for (uint32_t round = 0; round < rounds; ++round) {
-                    if (live && lane / b.y == round) {
+                    if (live && lane / n == round) {
                         for (uint32_t k = first; k < last; ++k) {
                             const uint32_t idx = slotPair[k];
                             if (idx / D != lane) continue;
@@ -489,6 +500,7 @@ __global__ void __launch_bounds__(kYarnThreads) k_yarn_friction_solve(YarnArgs a
                     }
                     waveSync();
                 }
+                first = last;
             }
         }
         if (live) {
