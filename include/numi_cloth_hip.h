@@ -74,6 +74,24 @@ typedef struct NumiClothHipSceneExtension {
      * every particle, so independent specimens merged into one scene keep
      * their own drive and damping. NULL uses the scalars. */
     const mr_float4* particleMotion;
+    /* Passes of the independent CPU reference's schedule (tools/cloth_bag.cpp),
+     * absent from the Metal path; 0 (the default) leaves them out.
+     * sweptGround: after the advance, with the ground on, a knot or fruit
+     *   that started the substep at or above contact height and is
+     *   predicted below it is put back at contact height; a fruit's removed
+     *   advance / (w dt) is added to its ground impulse (reference
+     *   sweepGroundPrediction, :2692-2721).
+     * pointSelfCollision: in every iteration, after the segment self-contact
+     *   pass, knots closer than two yarn radii that are not within two
+     *   distance hops of each other are pushed apart by the reference's
+     *   correction d * (2r - |d|) / (|d| (w_i + w_j)), shared by inverse mass
+     *   (solveSelfCollision, :4249-4315). The reference applies the pairs one
+     *   after another; here every pair acts on one position snapshot and each
+     *   knot moves by the mean of its corrections (2^-32 m fixed-point sums,
+     *   order independent), which is the reference's result for a knot in
+     *   one contact. */
+    uint32_t sweptGround;
+    uint32_t pointSelfCollision;
 } NumiClothHipSceneExtension;
 
 /*
@@ -154,6 +172,10 @@ typedef struct NumiClothHipDiagnostics {
      * the boxes requested and the grid's cell counts per axis. */
     uint32_t selfEntryCount;
     uint32_t selfGridCells[3];
+    /* Knot pairs of the last point self-collision pass (0 when the scene
+     * has none). */
+    uint32_t pointSelfContacts;
+    uint32_t reserved;
 } NumiClothHipDiagnostics;
 
 typedef struct NumiClothHipSolver NumiClothHipSolver;

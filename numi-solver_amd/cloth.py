@@ -74,7 +74,8 @@ class SceneExtension(C.Structure):
     _fields_ = [("staticSpheres", C.c_void_p), ("staticSphereCount", C.c_uint32), ("selfContact", C.c_uint32),
                 ("selfContactMargin", C.c_float), ("selfCandidateCapacity", C.c_uint32),
                 ("driveAmplitude", C.c_float), ("driveOmega", C.c_float), ("drivePhase", C.c_float),
-                ("damping", C.c_float), ("particleMotion", C.c_void_p)]
+                ("damping", C.c_float), ("particleMotion", C.c_void_p), ("sweptGround", C.c_uint32),
+                ("pointSelfCollision", C.c_uint32)]
 
 
 _DESC_POINTERS = ["config", "particles", "distances", "distanceBatches", "grips", "knots", "knotBatches", "bends",
@@ -103,7 +104,8 @@ class Diagnostics(C.Structure):
                 ("maximumSpherePenetration", C.c_float), ("maximumSelfPenetration", C.c_float),
                 ("selfCandidateCount", C.c_uint32), ("selfCandidateOverflow", C.c_uint32),
                 ("yarnReruns", C.c_uint32), ("yarnCandidatesMax", C.c_uint32),
-                ("selfEntryCount", C.c_uint32), ("selfGridCells", C.c_uint32 * 3)]
+                ("selfEntryCount", C.c_uint32), ("selfGridCells", C.c_uint32 * 3),
+                ("pointSelfContacts", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 MATERIAL_FIELDS = ["ordinaryNodeMassKg", "hemNodeMassKg", "yarnRadiusM", "axialBodyComplianceMPerN",
@@ -383,6 +385,8 @@ class Scene:
         self.damping = float(arrays.pop("damping", 0.0))
         motion = arrays.pop("particleMotion", None)
         self.particleMotion = None if motion is None else np.ascontiguousarray(motion, np.float32).reshape(-1, 4)
+        self.sweptGround = int(arrays.pop("sweptGround", 0))
+        self.pointSelfCollision = int(arrays.pop("pointSelfCollision", 0))
         self.initialCandidateMask = int(arrays.pop("initialCandidateMask", 0))
         self.initialReleasedMask = int(arrays.pop("initialReleasedMask", 0))
         if arrays:
@@ -409,6 +413,8 @@ class Scene:
         e.drivePhase = self.drivePhase
         e.damping = self.damping
         e.particleMotion = _ptr(self.particleMotion)
+        e.sweptGround = self.sweptGround
+        e.pointSelfCollision = self.pointSelfCollision
         return desc
 
     def refresh(self) -> None:
@@ -441,6 +447,7 @@ class Scene:
         scene = cls(config, staticSpheres=spheres, selfContact=e.selfContact, selfContactMargin=e.selfContactMargin,
                     selfCandidateCapacity=e.selfCandidateCapacity, driveAmplitude=e.driveAmplitude,
                     driveOmega=e.driveOmega, drivePhase=e.drivePhase, damping=e.damping, particleMotion=motion,
+                    sweptGround=e.sweptGround, pointSelfCollision=e.pointSelfCollision,
                     initialCandidateMask=d.initialCandidateMask, initialReleasedMask=d.initialReleasedMask, **arrays)
         if keep:
             return scene, handle

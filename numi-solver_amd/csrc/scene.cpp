@@ -372,9 +372,11 @@ void mergeScenes(const std::vector<const NumiClothHipSceneDesc*>& specimens, Num
         if (d == nullptr || d->config == nullptr) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "null specimen");
         const NumiClothBagGPUConfig& c = *d->config;
         if (d->gripCount || d->fruitCount || d->fruitPairCount || d->yarnContactCount || d->mouthRimCount ||
-            d->cuffCount || d->extension.staticSphereCount || d->extension.selfContact) {
+            d->cuffCount || d->extension.staticSphereCount || d->extension.selfContact ||
+            d->extension.pointSelfCollision || d->extension.sweptGround) {
             fail(NUMI_CLOTH_HIP_ERROR_UNSUPPORTED,
-                 "merged specimens may not have grips, fruits, mouth, cuff, static spheres or self-contact");
+                 "merged specimens may not have grips, fruits, mouth, cuff, static spheres, self-contact, point "
+                 "self-collision or the swept ground pass");
         }
         if (c.airVelocityAndDensity.w > 0.0f) {
             fail(NUMI_CLOTH_HIP_ERROR_UNSUPPORTED, "merged specimens may not have air loads (their reduction is global)");

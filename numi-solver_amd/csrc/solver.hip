@@ -28,6 +28,7 @@
 #include "kernels_bag.hpp"
 #include "kernels_yarn.hpp"
 #include "kernels_adjoint.hpp"
+#include "kernels_point.hpp"
 #include "tile_plan.hpp"
 #include "numi_cloth_hip.h"
 #include "numi_cloth_scene.h"
@@ -166,6 +167,14 @@ struct NumiClothHipSolver {
     int pendingSelfApply = -1; // buffer whose corrections the next launch applies
     int dirtySelf = -1;        // buffer applied by tile programs, still to clear
     DeviceArray<NumiClothBagGPUSelfImpulse> selfImpulses;  // per-candidate impulse aggregate of the substep
+    // Point self-collision (present when extension.pointSelfCollision != 0):
+    // hashed knot grid (bucket counts -> ends, knot ids by bucket), its own
+    // fixed-point accumulators, the contact count of the last pass.
+    bool pointSelf = false;
+    uint32_t pointBuckets = 0;
+    DeviceArray<uint32_t> pointEnds, pointIds, pointTouched, pointContacts, pointBucketCount, pointScanSums,
+        pointTotal;
+    DeviceArray<FixedAccumulator> pointAcc;
     DeviceArray<SubstepSlot> debugSlot;
 
     // LDS tile programs (NUMI_CLOTH_HIP_TILES=0 keeps the per-colour launches).
@@ -384,11 +393,31 @@ struct NumiClothHipSolver {
     }
 
     // Exclusive scan of `capacity`-sized data whose live length is *count.
-    void launchScan(uint32_t* data, const uint32_t* count, uint32_t capacity, uint32_t* total) {
+    void launchScan(uint32_t* data, const uint32_t* count, uint32_t capacity, uint32_t* total,
+                    uint32_t* sums = nullptr) {
+        if (sums == nullptr) sums = scanSums.ptr;
         const uint32_t tiles = std::min(kScanBlocks, std::max(1u, (capacity + kScanTile - 1) / kScanTile));
-        k_scan_tiles<<<tiles, kScanThreads, 0, stream>>>(data, count, scanSums.ptr);
-        k_scan_sums<<<1, kScanThreads, 0, stream>>>(scanSums.ptr, count, total);
-        k_scan_add<<<tiles, kScanThreads, 0, stream>>>(data, count, scanSums.ptr);
+        k_scan_tiles<<<tiles, kScanThreads, 0, stream>>>(data, count, sums);
+        k_scan_sums<<<1, kScanThreads, 0, stream>>>(sums, count, total);
+        k_scan_add<<<tiles, kScanThreads, 0, stream>>>(data, count, sums);
+    }
+
+    // Point self-collision pass: hash the knots, solve every pair on the
+    // snapshot into the point accumulators, apply the means (and clear).
+    void launchPointSelf() {
+        if (!pointSelf) return;
+        flushCollide();
+        const uint32_t mask = pointBuckets - 1u;
+        k_point_clear<<<blocksFor(pointBuckets), kBlock, 0, stream>>>(pointEnds.ptr, pointBuckets, pointContacts.ptr);
+        k_point_count<<<blocksFor(P), kBlock, 0, stream>>>(cfg(), ok(), posCur, P, mask, pointEnds.ptr);
+        launchScan(pointEnds.ptr, pointBucketCount.ptr, pointBuckets, pointTotal.ptr, pointScanSums.ptr);
+        k_point_place<<<blocksFor(P), kBlock, 0, stream>>>(cfg(), ok(), posCur, P, mask, pointEnds.ptr, pointIds.ptr);
+        k_point_solve<<<blocksFor(P), kBlock, 0, stream>>>(cfg(), ok(), posCur, P, mask, pointEnds.ptr, pointIds.ptr,
+                                                            localStart.ptr, localList.ptr, pointAcc.ptr,
+                                                            pointTouched.ptr, pointContacts.ptr);
+        const uint32_t words = (P + 31) / 32;
+        k_self_apply<<<blocksFor(words), kBlock, 0, stream>>>(ok(), posCur, pointAcc.ptr, pointTouched.ptr, words,
+                                                              failure.ptr);
     }
 
     // Broad phase: boxes, grid, cell entries, radix sort, candidate pairs.
@@ -663,10 +692,13 @@ struct NumiClothHipSolver {
             k_advance<<<blocksFor(P), kBlock, 0, stream>>>(cfg(), ok(), posCur, vel.ptr, P);
             if (F != 0) k_fruit_advance<<<blocksFor(F), kBlock, 0, stream>>>(cfg(), ok(), fruits.ptr, F);
         }
+        if (extension.sweptGround != 0u)
+            k_swept_ground<<<blocksFor(std::max(P, F)), kBlock, 0, stream>>>(cfg(), ok(), posCur, prev.ptr, P,
+                                                                              fruits.ptr, F);
         // Without grips or fruits nothing runs between a self-contact solve
         // and the next tile program: that program applies the corrections
         // (and the ground/sphere pass that follows in an iteration).
-        const bool defer = self && G == 0 && F == 0 && Y == 0 && useTiles;
+        const bool defer = self && !pointSelf && G == 0 && F == 0 && Y == 0 && useTiles;
         if (self) {
             launchBroadPhase(cfg(), ok());
             launchSelf(1, false, defer);
@@ -677,7 +709,7 @@ struct NumiClothHipSolver {
         // of an iteration: the pass joins the tile program (or the
         // self-contact apply when that runs in between).
         const bool colliders = groundOrColliders();
-        const bool trailing = colliders && !self && G == 0 && F == 0;
+        const bool trailing = colliders && !self && !pointSelf && G == 0 && F == 0;
         for (uint32_t it = 0; it < iterations; ++it) {
             TileProgram& program = it % 2 == 0 ? tileEven : tileOdd;
             const bool tiled = useTiles && program.valid;
@@ -691,7 +723,12 @@ struct NumiClothHipSolver {
             launchGrips();
             launchFruitPairs();
             launchYarn(0);
-            if (self) {
+            if (pointSelf) {
+                // segment self-contact, point self-collision, then ground
+                if (self) launchSelf(0, false, false);
+                launchPointSelf();
+                if (colliders) launchCollide();
+            } else if (self) {
                 launchSelf(0, colliders, defer);
             } else if (!(tiled && trailing)) {
                 launchCollide();
@@ -847,6 +884,7 @@ void checkColouring(const std::vector<Range>& ranges, uint32_t particles, Partic
 }
 
 void setupSelfContact(const NumiClothHipSceneDesc& scene, NumiClothHipSolver& s);
+void setupPointSelf(NumiClothHipSolver& s);
 void setupTiles(NumiClothHipSolver& s);
 double timeTileProgram(NumiClothHipSolver& s, NumiClothHipSolver::TileProgram& t, uint32_t repeats);
 
@@ -1119,6 +1157,7 @@ void createSolver(const NumiClothHipSceneDesc& scene, NumiClothHipSolver& s) {
     s.debugSlot.allocate(1);
     s.initialPositions = hp;
     if (scene.extension.selfContact != 0) setupSelfContact(scene, s);
+    if (scene.extension.pointSelfCollision != 0) setupPointSelf(s);
     bool tiles = true;
     if (const char* env = std::getenv("NUMI_CLOTH_HIP_TILES")) tiles = std::strcmp(env, "0") != 0;
     if (tiles) setupTiles(s);
@@ -1130,17 +1169,14 @@ void createSolver(const NumiClothHipSceneDesc& scene, NumiClothHipSolver& s) {
 // Two-hop neighbourhoods over the distance graph (sorted CSR, each list
 // includes its own knot): the locality rule that keeps adjacent yarn out of
 // self-contact (reference tools/cloth_bag_metal.mm:1107-1152).
-void setupSelfContact(const NumiClothHipSceneDesc& scene, NumiClothHipSolver& s) {
-    if (s.D == 0) return;
-    if (!(scene.config->clothMaterial.y > 0.0f)) {
-        fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "self-contact needs a positive cell size (clothMaterial.y)");
-    }
+void twoHopLists(const NumiClothHipSolver& s, std::vector<uint32_t>& start, std::vector<uint32_t>& list) {
     std::vector<std::vector<uint32_t>> direct(s.P);
     for (const NumiClothBagGPUDistance& d : s.hostDistances) {
         direct[d.particlesAndColor.x].push_back(d.particlesAndColor.y);
         direct[d.particlesAndColor.y].push_back(d.particlesAndColor.x);
     }
-    std::vector<uint32_t> start(s.P + 1, 0), list;
+    start.assign(s.P + 1, 0);
+    list.clear();
     for (uint32_t i = 0; i < s.P; ++i) {
         std::vector<uint32_t> near{i};
         for (uint32_t a : direct[i]) {
@@ -1152,6 +1188,46 @@ void setupSelfContact(const NumiClothHipSceneDesc& scene, NumiClothHipSolver& s)
         start[i + 1] = start[i] + static_cast<uint32_t>(near.size());
         list.insert(list.end(), near.begin(), near.end());
     }
+}
+
+// Point self-collision buffers: a power-of-two bucket table of at least
+// twice the knots, the two-hop lists (shared with segment self-contact).
+void setupPointSelf(NumiClothHipSolver& s) {
+    if (s.P == 0) return;
+    hipStream_t st = s.stream;
+    if (s.localStart.count == 0) {
+        std::vector<uint32_t> start, list;
+        twoHopLists(s, start, list);
+        s.localStart.allocate(start.size());
+        s.localStart.upload(start.data(), start.size(), st);
+        s.localList.allocate(std::max<size_t>(list.size(), 1));
+        s.localList.upload(list.data(), list.size(), st);
+    }
+    uint32_t buckets = 1024;
+    while (buckets < 2u * s.P) buckets <<= 1;
+    s.pointBuckets = buckets;
+    s.pointEnds.allocate(buckets);
+    s.pointIds.allocate(s.P);
+    s.pointAcc.allocate(s.P);
+    HIP_OK(hipMemsetAsync(s.pointAcc.ptr, 0, s.P * sizeof(FixedAccumulator), st));
+    s.pointTouched.allocate((s.P + 31) / 32);
+    HIP_OK(hipMemsetAsync(s.pointTouched.ptr, 0, (s.P + 31) / 32 * sizeof(uint32_t), st));
+    s.pointContacts.allocate(1);
+    HIP_OK(hipMemsetAsync(s.pointContacts.ptr, 0, sizeof(uint32_t), st));
+    s.pointBucketCount.allocate(1);
+    s.pointBucketCount.upload(&buckets, 1, st);
+    s.pointScanSums.allocate((buckets + kScanTile - 1) / kScanTile + 1);
+    s.pointTotal.allocate(1);
+    s.pointSelf = true;
+}
+
+void setupSelfContact(const NumiClothHipSceneDesc& scene, NumiClothHipSolver& s) {
+    if (s.D == 0) return;
+    if (!(scene.config->clothMaterial.y > 0.0f)) {
+        fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "self-contact needs a positive cell size (clothMaterial.y)");
+    }
+    std::vector<uint32_t> start, list;
+    twoHopLists(s, start, list);
     hipStream_t st = s.stream;
     if (s.D > kEntrySegmentMask)
         fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "hash self-contact supports at most 2^29 - 1 segments");
@@ -1600,6 +1676,10 @@ void diagnoseSolver(NumiClothHipSolver& s, NumiClothHipDiagnostics* out) {
         out->selfGridCells[0] = g.nx;
         out->selfGridCells[1] = g.ny;
         out->selfGridCells[2] = g.nz;
+    }
+    if (s.pointSelf) {
+        s.pointContacts.download(&out->pointSelfContacts, 1, s.stream);
+        HIP_OK(hipStreamSynchronize(s.stream));
     }
 }
 

@@ -30,9 +30,11 @@
 //                  rounding.
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <map>
 #include <unordered_map>
 #include <vector>
 
@@ -72,6 +74,7 @@ struct Oracle {
     uint32_t frictionContacts = 0;
     uint32_t presentContacts = 0, sweptContacts = 0;
     uint32_t selfFrictionContacts = 0;
+    uint32_t pointContacts = 0;  // pairs of the last point self-collision pass
     double maximumConeRatio = 0;
     // Impulse aggregate of each candidate over the substep's self-contact
     // calls (reference aggregate_self_impulses, src/metal/ClothBag.metal:
@@ -121,7 +124,7 @@ struct Oracle {
             gripGeneration[i] = g.particle.y;
             gripCentre[i] = g.particle.w;
         }
-        if (d.extension.selfContact) buildLocality();
+        if (d.extension.selfContact || d.extension.pointSelfCollision) buildLocality();
         initBag();
     }
 
@@ -248,6 +251,80 @@ struct Oracle {
     void advance(R dt) {
         for (Particle<R>& q : p)
             if (q.w > R(0)) q.x = fma3(q.v, dt, q.x);
+    }
+
+    // Swept ground pass after the advance (reference sweepGroundPrediction,
+    // tools/cloth_bag.cpp:2692-2721): knots and fruits that crossed contact
+    // height go back to it; a fruit's removed advance becomes ground impulse.
+    void sweptGround(const NumiClothBagGPUConfig& c) {
+        if (c.constraintCounts.z == 0u) return;
+        const R r = R(c.clothMaterial.x);
+        for (Particle<R>& q : p)
+            if (q.prev.z >= r && q.x.z < r) q.x.z = r;
+        for (Fruit& f : fruit)
+            if (f.prev.z >= f.radius && f.x.z < f.radius) {
+                const R removed = f.radius - f.x.z;
+                f.groundImpulse += removed / (f.invMass * R(c.gravityAndTimestep.w));
+                f.x.z = f.radius;
+            }
+    }
+
+    // Point self-collision (reference solveSelfCollision, tools/cloth_bag.cpp:
+    // 4249-4315) with every pair on one snapshot and each knot moved by the
+    // mean of its corrections (FP64: exact sums; FP32 twin: the device's
+    // 2^-32 m fixed point). The pair set is every i < j within 2r (and at
+    // least 1e-12), outside i's two-hop list, with w_i + w_j > 0; the grid
+    // here (cells of 2r, neighbours only) just finds them.
+    void pointSelf(const NumiClothBagGPUConfig& c) {
+        const R target = R(2) * R(c.clothMaterial.x);
+        const double cellSize = 2.0 * static_cast<double>(c.clothMaterial.x) * 1.0009765625;
+        std::map<std::array<long long, 3>, std::vector<uint32_t>> cells;
+        auto cellOf = [&](uint32_t i) {
+            return std::array<long long, 3>{static_cast<long long>(std::floor(static_cast<double>(p[i].x.x) / cellSize)),
+                                            static_cast<long long>(std::floor(static_cast<double>(p[i].x.y) / cellSize)),
+                                            static_cast<long long>(std::floor(static_cast<double>(p[i].x.z) / cellSize))};
+        };
+        for (uint32_t i = 0; i < P; ++i)
+            if (std::isfinite(static_cast<double>(p[i].x.x)) && std::isfinite(static_cast<double>(p[i].x.y)) &&
+                std::isfinite(static_cast<double>(p[i].x.z)))
+                cells[cellOf(i)].push_back(i);
+        fixed.assign(P, Fixed{});
+        deltaSum.assign(P, V<R>{});
+        deltaCount.assign(P, 0u);
+        pointContacts = 0;
+        for (const auto& [cell, members] : cells)
+            for (uint32_t i : members)
+                for (long long dz = -1; dz <= 1; ++dz)
+                    for (long long dy = -1; dy <= 1; ++dy)
+                        for (long long dx = -1; dx <= 1; ++dx) {
+                            const auto found = cells.find({cell[0] + dx, cell[1] + dy, cell[2] + dz});
+                            if (found == cells.end()) continue;
+                            for (uint32_t j : found->second) {
+                                if (j <= i) continue;
+                                const V<R> dd = p[j].x - p[i].x;
+                                const R len = norm(dd);
+                                if (!(len < target) || len < R(1.0e-12)) continue;
+                                const R denom = p[i].w + p[j].w;
+                                if (!(denom > R(0))) continue;
+                                if (near(i, j)) continue;
+                                const V<R> correction = dd * ((target - len) / (len * denom));
+                                accumulate(i, correction * -p[i].w);
+                                accumulate(j, correction * p[j].w);
+                                ++pointContacts;
+                            }
+                        }
+        for (uint32_t i = 0; i < P; ++i) {
+            if (sizeof(R) == sizeof(float)) {
+                const Fixed& f = fixed[i];
+                if (f.count == 0) continue;
+                const double scale = (1.0 / 4294967296.0) / static_cast<double>(f.count);
+                p[i].x = p[i].x + V<R>{R(static_cast<double>(f.x) * scale), R(static_cast<double>(f.y) * scale),
+                                       R(static_cast<double>(f.z) * scale)};
+            } else {
+                if (deltaCount[i] == 0) continue;
+                p[i].x = p[i].x + deltaSum[i] / R(deltaCount[i]);
+            }
+        }
     }
 
     // ------------------------------------------------------- constraints --
@@ -1365,6 +1442,7 @@ struct Oracle {
         fruitAirLoads(c);
         advance(dt);
         fruitAdvance(dt);
+        if (e.sweptGround) sweptGround(c);
         const bool self = e.selfContact != 0;
         if (self) {
             broadPhase(c);
@@ -1381,6 +1459,7 @@ struct Oracle {
             fruitPairs(c);
             yarnBatches(c, 0);
             if (self) selfContact(c, 0);
+            if (e.pointSelfCollision) pointSelf(c);
             ground(c, colliders);
         }
         strain(sweeps);
@@ -1446,7 +1525,7 @@ typedef struct NumiClothOracleOutput {
     uint32_t rollingContacts;
     double maximumRollingRatio;
     uint32_t selfFrictionContacts;
-    uint32_t reserved2;
+    uint32_t pointSelfContacts;  /* pairs of the last point self-collision pass */
 } NumiClothOracleOutput;
 
 }  // extern "C"
@@ -1528,6 +1607,7 @@ static int runOracle(const NumiClothHipSceneDesc* scene, const NumiClothBagGPUCo
     out->rollingContacts = o.rollingContacts;
     out->maximumRollingRatio = o.maximumRollingRatio;
     out->selfFrictionContacts = o.selfFrictionContacts;
+    out->pointSelfContacts = o.pointContacts;
     return 0;
 }
 

@@ -28,7 +28,7 @@ class OracleOutput(C.Structure):
                 ("releasedMask", C.c_uint32), ("pairFrictionContacts", C.c_uint32),
                 ("yarnFrictionContacts", C.c_uint32), ("fruitGroundContacts", C.c_uint32),
                 ("rollingContacts", C.c_uint32), ("maximumRollingRatio", C.c_double),
-                ("selfFrictionContacts", C.c_uint32), ("reserved2", C.c_uint32)]
+                ("selfFrictionContacts", C.c_uint32), ("pointSelfContacts", C.c_uint32)]
 
 
 _lib = None
@@ -82,7 +82,7 @@ def run(scene, configs: np.ndarray, iterations: int, strain_sweeps: int, precisi
     out["sweptContacts"] = int(result.sweptContacts)
     out["candidateCount"] = int(result.candidateCount)
     for name in ["candidateMask", "releasedMask", "pairFrictionContacts", "yarnFrictionContacts",
-                 "fruitGroundContacts", "rollingContacts", "selfFrictionContacts"]:
+                 "fruitGroundContacts", "rollingContacts", "selfFrictionContacts", "pointSelfContacts"]:
         out[name] = int(getattr(result, name))
     for name in ["maximumRollingRatio", "maximumYarnAeroForce", "maximumFruitAeroForce", "maximumFruitAeroTorque"]:
         out[name] = float(getattr(result, name))
