@@ -418,7 +418,7 @@ __global__ void k_yarn_contacts_build(const NumiClothBagGPUConfig* __restrict__ 
 // applyYarnCorrection (reference :352-462): normal response of a fruit and a
 // yarn segment with a ground active set of at most three steps. Knots are in
 // LDS; returns the accumulated multiplier.
-__device__ float yarnCorrection(float4& first, float4& second, NumiClothBagGPUFruit& fruit, float wa, float wb, V3 n,
+__device__ __forceinline__ float yarnCorrection(float4& first, float4& second, NumiClothBagGPUFruit& fruit, float wa, float wb, V3 n,
                                 float distance, bool groundEnabled, float floorZ, uint32_t* failure, bool& wrote) {
     wrote = false;
     float4 a = first, b = second;

@@ -218,6 +218,8 @@ __global__ void __launch_bounds__(kYarnThreads) k_yarn_solve(YarnArgs a) {
     __shared__ uint32_t candidates[kYarnSlots];  // pair indices, unordered
     __shared__ uint32_t slotPair[kYarnSlots];     // pair indices in phase order
     __shared__ uint32_t slotPhase[kYarnSlots];    // the phase of each slot
+    __shared__ uint32_t slotColour[kYarnSlots];   // segments in the slot's colour
+    __shared__ uint2 slotEnds[kYarnSlots];        // the slot's segment knots
     __shared__ YarnStage slotStage[kYarnSlots];
     __shared__ NumiClothBagGPUFruit fruitStart[32];
     __shared__ float fruitMoved[32];
@@ -266,10 +268,13 @@ __global__ void __launch_bounds__(kYarnThreads) k_yarn_solve(YarnArgs a) {
         for (uint32_t k = tid; k < total; k += threads) {
             const uint32_t idx = candidates[k];
             const uint32_t f = idx / D, s = idx % D;
-            const uint32_t phase = phaseBit(f, s, a.segmentBatch[s]);
+            const uint2 batch = a.segmentBatch[s];
+            const uint32_t phase = phaseBit(f, s, batch);
             const uint32_t slot = atomicAdd(&counts[phase], 1u);
             slotPair[slot] = idx;
             slotPhase[slot] = phase;
+            slotColour[slot] = batch.y;
+            slotEnds[slot] = a.ij[s];
             const NumiClothBagGPUYarnContact& c = a.contacts[idx];
             slotStage[slot] = YarnStage{c.fruitNormalAndImpulse, c.segmentImpulse, c.control.w, 0u};
         }
@@ -278,7 +283,8 @@ __global__ void __launch_bounds__(kYarnThreads) k_yarn_solve(YarnArgs a) {
     // 2. candidate visits in phase order on wave 0: the slots are grouped by
     // phase in increasing phase order (the scan above), so the wave walks
     // the non-empty groups only; a group's fruits are distinct, and fruits
-    // f and f + n of a colour of n segments take turns (rounds).
+    // f and f + n of a colour of n segments take turns (rounds). Everything
+    // a step reads is in LDS (slot, record fields, knot indices, knots).
     if (verdict == 0u && tid < 64) {
         const uint32_t lane = tid;
         const bool live = lane < F;
@@ -288,17 +294,16 @@ __global__ void __launch_bounds__(kYarnThreads) k_yarn_solve(YarnArgs a) {
         for (uint32_t first = 0; first < total;) {
             const uint32_t phase = slotPhase[first];
             const uint32_t last = counts[phase];  // end of the phase's group
-            const uint32_t n = a.segmentBatch[phase].y;
+            const uint32_t n = slotColour[first];
             const uint32_t rounds = (F + n - 1) / n;
             for (uint32_t round = 0; round < rounds; ++round) {
                 if (live && lane / n == round) {
                     for (uint32_t k = first; k < last; ++k) {
                         const uint32_t idx = slotPair[k];
                         if (idx / D != lane) continue;
-                        const uint32_t s = idx % D;
                         YarnStage stage = slotStage[k];
                         bool acted = false;
-                        travel += yarnVisit(a, fruit, a.contacts[idx], stage, knots, a.ij[s], moved, acted);
+                        travel += yarnVisit(a, fruit, a.contacts[idx], stage, knots, slotEnds[k], moved, acted);
                         stage.acted = acted ? 1u : 0u;
                         slotStage[k] = stage;
                     }

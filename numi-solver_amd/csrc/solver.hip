@@ -1080,9 +1080,11 @@ void createSolver(const NumiClothHipSceneDesc& scene, NumiClothHipSolver& s) {
             if (s.F > 32) fail(NUMI_CLOTH_HIP_ERROR_UNSUPPORTED, "fruit/yarn solve runs one fruit per lane of one wave");
             int maxShared = 0;
             HIP_OK(hipDeviceGetAttribute(&maxShared, hipDeviceAttributeMaxSharedMemoryPerBlock, s.device));
-            // Static LDS of the yarn kernels (scan scratch, staged visits,
-            // fruits) plus the knots, their travel and the phase counts.
-            const uint64_t fixed = 4ull * kYarnThreads + (4ull + sizeof(YarnStage)) * kYarnSlots + 1024;
+            // Static LDS of the yarn kernels (scan scratch, candidate list,
+            // staged visits with their phases, fruits) plus the knots, their
+            // travel and the phase counts.
+            const uint64_t fixed = 4ull * kYarnThreads + (24ull + sizeof(YarnStage)) * kYarnSlots +
+                                   32ull * (sizeof(NumiClothBagGPUFruit) + 20) + 256;
             const uint64_t dynamic = static_cast<uint64_t>(s.P) * 20 + static_cast<uint64_t>(s.D) * 4;
             if (fixed + dynamic > static_cast<uint64_t>(maxShared)) {
                 fail(NUMI_CLOTH_HIP_ERROR_UNSUPPORTED, "fruit/yarn contact stages every knot in LDS");
