@@ -211,14 +211,17 @@ __device__ void yarnDense(const YarnArgs& a, float4* knots, float* moved) {
 }
 
 // One solve_yarn_batch call over all distance colours. One workgroup of
-// kYarnThreads; dynamic LDS: knots[P], moved[P], counts[D].
+// kYarnThreads; dynamic LDS: knots[P], moved[P], counts[D], visitors[D],
+// visited[D] (per segment: fruits with a candidate visit on it, fruits whose
+// visit has run).
 __global__ void __launch_bounds__(kYarnThreads) k_yarn_solve(YarnArgs a) {
     extern __shared__ float4 yarnLds[];
     __shared__ uint32_t scanScratch[kYarnThreads];
     __shared__ uint32_t candidates[kYarnSlots];  // pair indices, unordered
     __shared__ uint32_t slotPair[kYarnSlots];     // pair indices in phase order
     __shared__ uint32_t slotPhase[kYarnSlots];    // the phase of each slot
-    __shared__ uint32_t slotColour[kYarnSlots];   // segments in the slot's colour
+    __shared__ uint32_t slotColour[kYarnSlots];   // first segment of the slot's colour
+    __shared__ uint32_t slotEarlier[kYarnSlots];  // fruits visiting the slot's segment before it
     __shared__ uint2 slotEnds[kYarnSlots];        // the slot's segment knots
     __shared__ YarnStage slotStage[kYarnSlots];
     __shared__ NumiClothBagGPUFruit fruitStart[32];
@@ -233,11 +236,17 @@ __global__ void __launch_bounds__(kYarnThreads) k_yarn_solve(YarnArgs a) {
     float4* knots = yarnLds;
     float* moved = reinterpret_cast<float*>(knots + P);
     uint32_t* counts = reinterpret_cast<uint32_t*>(moved + P);
+    uint32_t* visitors = counts + D;
+    uint32_t* visited = visitors + D;
     for (uint32_t i = tid; i < P; i += threads) {
         knots[i] = a.pos[i];
         moved[i] = 0.0f;
     }
-    for (uint32_t i = tid; i < D; i += threads) counts[i] = 0u;
+    for (uint32_t i = tid; i < D; i += threads) {
+        counts[i] = 0u;
+        visitors[i] = 0u;
+        visited[i] = 0u;
+    }
     if (tid < F) {
         fruitStart[tid] = a.fruits[tid];
         fruitMoved[tid] = 0.0f;
@@ -255,6 +264,7 @@ __global__ void __launch_bounds__(kYarnThreads) k_yarn_solve(YarnArgs a) {
         const uint2 batch = a.segmentBatch[s];
         if (batch.y == 0u || !yarnCandidate(a, f, s, knots, fruitStart)) continue;
         atomicAdd(&counts[phaseBit(f, s, batch)], 1u);
+        atomicOr(&visitors[s], 1u << f);
         const uint32_t k = atomicAdd(&candidateCount, 1u);
         if (k < kYarnSlots) candidates[k] = idx;
     }
@@ -273,44 +283,66 @@ __global__ void __launch_bounds__(kYarnThreads) k_yarn_solve(YarnArgs a) {
             const uint32_t slot = atomicAdd(&counts[phase], 1u);
             slotPair[slot] = idx;
             slotPhase[slot] = phase;
-            slotColour[slot] = batch.y;
+            slotColour[slot] = batch.x;
             slotEnds[slot] = a.ij[s];
+            // The other fruits whose visit of segment s comes first in the
+            // reference order: an earlier phase, or the same phase and a
+            // lower fruit (fruits f and f + n of a colour of n segments).
+            uint32_t earlier = 0u;
+            for (uint32_t others = visitors[s] & ~(1u << f); others != 0u; others &= others - 1u) {
+                const uint32_t g = static_cast<uint32_t>(__ffs(static_cast<int>(others))) - 1u;
+                const uint32_t pg = phaseBit(g, s, batch);
+                if (pg < phase || (pg == phase && g < f)) earlier |= 1u << g;
+            }
+            slotEarlier[slot] = earlier;
             const NumiClothBagGPUYarnContact& c = a.contacts[idx];
             slotStage[slot] = YarnStage{c.fruitNormalAndImpulse, c.segmentImpulse, c.control.w, 0u};
         }
     }
     __syncthreads();
-    // 2. candidate visits in phase order on wave 0: the slots are grouped by
-    // phase in increasing phase order (the scan above), so the wave walks
-    // the non-empty groups only; a group's fruits are distinct, and fruits
-    // f and f + n of a colour of n segments take turns (rounds). Everything
-    // a step reads is in LDS (slot, record fields, knot indices, knots).
+    // 2. candidate visits on wave 0, one lane per fruit, colour by colour
+    // (the slots of a colour are contiguous and in phase order). A visit
+    // reads and writes only its fruit, the two knots of its segment and its
+    // own staged record, and the segments of one colour share no knot; so
+    // within a colour each lane runs its own visits in phase order and a
+    // visit waits only for the visits of its segment that the reference
+    // runs before it (slotEarlier, against visited[s]). The earliest
+    // pending visit is always ready, and the result is the reference's
+    // sequential one. Everything a round reads is in LDS.
     if (verdict == 0u && tid < 64) {
         const uint32_t lane = tid;
         const bool live = lane < F;
         NumiClothBagGPUFruit fruit{};
         if (live) fruit = fruitStart[lane];
         float travel = 0.0f;
-        for (uint32_t first = 0; first < total;) {
-            const uint32_t phase = slotPhase[first];
-            const uint32_t last = counts[phase];  // end of the phase's group
-            const uint32_t n = slotColour[first];
-            const uint32_t rounds = (F + n - 1) / n;
-            for (uint32_t round = 0; round < rounds; ++round) {
-                if (live && lane / n == round) {
-                    for (uint32_t k = first; k < last; ++k) {
-                        const uint32_t idx = slotPair[k];
-                        if (idx / D != lane) continue;
+        for (uint32_t colourStart = 0; colourStart < total;) {
+            const uint32_t colour = slotColour[colourStart];
+            const uint32_t colourEnd = counts[colour + a.segmentBatch[colour].y - 1u];  // end of its last phase
+            uint32_t cursor = colourStart;
+            bool finished = !live;
+            while (true) {
+                if (!finished) {
+                    while (cursor < colourEnd && slotPair[cursor] / D != lane) ++cursor;
+                    finished = cursor >= colourEnd;
+                }
+                if (__all(finished)) break;
+                if (!finished) {
+                    const uint32_t k = cursor;
+                    const uint32_t idx = slotPair[k];
+                    const uint32_t s = idx % D;
+                    if ((slotEarlier[k] & ~visited[s]) == 0u) {
                         YarnStage stage = slotStage[k];
                         bool acted = false;
                         travel += yarnVisit(a, fruit, a.contacts[idx], stage, knots, slotEnds[k], moved, acted);
                         stage.acted = acted ? 1u : 0u;
                         slotStage[k] = stage;
+                        visited[s] |= 1u << lane;
+                        ++cursor;
                     }
                 }
                 waveSync();
             }
-            first = last;
+            colourStart = colourEnd;
         }
         if (live) {
             fruitMoved[lane] = travel;

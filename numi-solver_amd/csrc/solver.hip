@@ -337,7 +337,7 @@ struct NumiClothHipSolver {
     void launchYarn(uint32_t mode) {
         if (Y == 0) return;
         flushCollide();
-        const size_t lds = static_cast<size_t>(P) * (sizeof(float4) + sizeof(float)) + static_cast<size_t>(D) * 4;
+        const size_t lds = static_cast<size_t>(P) * (sizeof(float4) + sizeof(float)) + static_cast<size_t>(D) * 12;
         k_yarn_solve<<<1, kYarnThreads, lds, stream>>>(yarnArgs(mode, posCur));
     }
 
@@ -1083,9 +1083,9 @@ void createSolver(const NumiClothHipSceneDesc& scene, NumiClothHipSolver& s) {
             // Static LDS of the yarn kernels (scan scratch, candidate list,
             // staged visits with their phases, fruits) plus the knots, their
             // travel and the phase counts.
-            const uint64_t fixed = 4ull * kYarnThreads + (24ull + sizeof(YarnStage)) * kYarnSlots +
+            const uint64_t fixed = 4ull * kYarnThreads + (28ull + sizeof(YarnStage)) * kYarnSlots +
                                    32ull * (sizeof(NumiClothBagGPUFruit) + 20) + 256;
-            const uint64_t dynamic = static_cast<uint64_t>(s.P) * 20 + static_cast<uint64_t>(s.D) * 4;
+            const uint64_t dynamic = static_cast<uint64_t>(s.P) * 20 + static_cast<uint64_t>(s.D) * 12;
             if (fixed + dynamic > static_cast<uint64_t>(maxShared)) {
                 fail(NUMI_CLOTH_HIP_ERROR_UNSUPPORTED, "fruit/yarn contact stages every knot in LDS");
             }
