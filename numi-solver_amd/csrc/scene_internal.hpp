@@ -32,8 +32,13 @@ struct NumiClothHipScene {
     std::vector<uint32_t> mergeIndex[4];
     NumiClothHipSceneDesc desc{};
 
+    // Points the descriptor at the arrays (after they changed); the scalar
+    // extension fields the builder set (self-contact, margin, drive,
+    // damping, extra passes) are kept.
     void publish() {
+        const NumiClothHipSceneExtension extension = desc.extension;
         desc = NumiClothHipSceneDesc{};
+        desc.extension = extension;
         desc.config = &config;
         desc.particles = particles.data();
         desc.distances = distances.data();

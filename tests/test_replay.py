@@ -99,6 +99,17 @@ def test_frame_configs(cloth, scenario):
             assert np.array_equal(cfg[name], scene.config[0][name]), name
 
 
+def test_prepare_keeps_the_scene_extension(cloth):
+    """Preparing a replay re-points the scene's arrays without dropping the
+    builder's extension fields (self-contact and its margin)."""
+    c = cloth.cloth
+    plain = cloth.Scene.bag(selfContact=1)
+    for scenario in (c.REPLAY_GROUNDED, c.REPLAY_SPIN, c.REPLAY_PICKUP):
+        scene = cloth.Scene.bag_replay(scenario, selfContact=1)
+        assert scene.selfContact == 1
+        assert scene.selfContactMargin == plain.selfContactMargin > 0.0
+
+
 def test_prepare_sets_ground_lift_and_initial_pose(cloth, golden):
     c = cloth.cloth
     plain = cloth.Scene.bag()
