@@ -100,6 +100,75 @@ __device__ __forceinline__ void tileCollide(float4* lds, uint32_t loaded, uint32
 }
 
 constexpr uint32_t kTilePrefetch = 4;  // item slots per thread and pass held in registers
+constexpr uint32_t kTileKnotSlots = 4; // loaded knots per thread whose ids stay in registers
+
+// Ids of a thread's first kTileKnotSlots loaded knots (load list entries
+// t + u * stride; slots past the end repeat the last entry so every slot
+// holds a valid knot id).
+__device__ __forceinline__ void tileLoadKnotIds(const TileProgramArgs& a, uint32_t base, uint32_t loaded, uint32_t t,
+                                                uint32_t stride, uint32_t* ids) {
+    const uint32_t last = loaded > 0u ? loaded - 1u : 0u;
+#pragma unroll
+    for (uint32_t u = 0; u < kTileKnotSlots; ++u)
+        ids[u] = loaded > 0u ? a.loadIds[base + min(t + u * stride, last)] : 0u;
+}
+
+// The tile's knots into LDS (no barrier), each with the self-contact
+// correction of the previous solve applied (mean of its fixed-point sum,
+// live runs only), and the owned knots' entries of the accumulator buffer
+// being recycled cleared, one mask bit at a time (other tiles clear the
+// other bits of a shared word). For the register slots every request is in
+// flight before any is used: the position and both mask words together,
+// then the accumulators of touched knots; inFlight() runs in between
+// (further independent loads of the caller).
+template <typename InFlight>
+__device__ __forceinline__ void tileGatherKnots(float4* lds, const TileProgramArgs& a, uint32_t base, uint32_t loaded,
+                                                uint32_t owned, uint32_t t, uint32_t stride, bool live,
+                                                const uint32_t* ids, InFlight&& inFlight) {
+    const bool applySelf = live && a.selfAcc != nullptr;
+    const bool clearing = a.clearAcc != nullptr;
+    float4 p[kTileKnotSlots];
+    uint32_t selfWord[kTileKnotSlots], clearWord[kTileKnotSlots];
+#pragma unroll
+    for (uint32_t u = 0; u < kTileKnotSlots; ++u) {
+        p[u] = a.posIn[ids[u]];
+        selfWord[u] = applySelf ? a.selfTouched[ids[u] >> 5] : 0u;
+        clearWord[u] = clearing ? a.clearTouched[ids[u] >> 5] : 0u;
+    }
+    inFlight();
+#pragma unroll
+    for (uint32_t u = 0; u < kTileKnotSlots; ++u) {
+        const uint32_t i = t + u * stride;
+        if (i >= loaded) break;
+        const uint32_t id = ids[u];
+        const uint32_t bit = 1u << (id & 31u);
+        if ((selfWord[u] & bit) != 0u) {
+            const V3 np = xyz(p[u]) + fixedMean(a.selfAcc[id]);
+            if (!finite3(np)) recordFailure(a.failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
+            else p[u] = withXyz(p[u], np);
+        }
+        lds[i] = p[u];
+        if (i < owned && (clearWord[u] & bit) != 0u) {
+            a.clearAcc[id] = FixedAccumulator{0, 0, 0, 0u, 0u};
+            atomicAnd(a.clearTouched + (id >> 5), ~bit);
+        }
+    }
+    for (uint32_t i = t + kTileKnotSlots * stride; i < loaded; i += stride) {
+        const uint32_t id = a.loadIds[base + i];
+        const uint32_t bit = 1u << (id & 31u);
+        float4 q = a.posIn[id];
+        if (applySelf && (a.selfTouched[id >> 5] & bit) != 0u) {
+            const V3 np = xyz(q) + fixedMean(a.selfAcc[id]);
+            if (!finite3(np)) recordFailure(a.failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
+            else q = withXyz(q, np);
+        }
+        lds[i] = q;
+        if (clearing && i < owned && (a.clearTouched[id >> 5] & bit) != 0u) {
+            a.clearAcc[id] = FixedAccumulator{0, 0, 0, 0u, 0u};
+            atomicAnd(a.clearTouched + (id >> 5), ~bit);
+        }
+    }
+}
 
 __device__ __forceinline__ uint32_t tileItemConstraint(uint32_t kind, const uint4& item) {
     return (kind == kTileKnotKind ? item.z : item.y) & 0x7fffffffu;
@@ -247,6 +316,12 @@ __global__ void k_tile_program(TileProgramArgs a) {
     // loop counter would copy the whole argument block to scratch.
     for (uint32_t k = threadIdx.x; k < passes; k += blockDim.x) kinds[k] = a.kinds[k];
     for (uint32_t k = threadIdx.x; k <= passes; k += blockDim.x) starts[k] = itemStart[k];
+    // Scalars every pass reads, requested before the gather.
+    const bool live = *a.ok != 0u;
+    const float dt = a.cfg->gravityAndTimestep.w;
+    const float invDt2 = 1.0f / (dt * dt);
+    const bool ground = a.cfg->constraintCounts.z != 0u;
+    const float floorZ = a.cfg->clothMaterial.x;
     // Items of passes 0 and 1, multipliers of pass 0 (issued before the knot
     // gather so the two overlap).
     const uint32_t t = threadIdx.x, stride = blockDim.x;
@@ -255,20 +330,19 @@ __global__ void k_tile_program(TileProgramArgs a) {
     uint4* const stagedItems = reinterpret_cast<uint4*>(lds + a.stageOffset);
     float* const stagedLambdas = reinterpret_cast<float*>(stagedItems + a.stageItems);
     const uint32_t itemBase = itemStart[0];
+    // Ids of the thread's first loaded knots (the first kept for the write-back).
+    uint32_t ids[kTileKnotSlots];
     if constexpr (Staged) {
         // Two phases, each with every request of the workgroup in flight at
         // once: (1) the knot ids into registers and the items of every pass
-        // into LDS; (2) after a barrier, the knot positions by id and the
-        // input multiplier of every item. Slot arrays are filled for every
-        // slot (indices clamped to the last element) so they stay in
-        // registers; only live slots are stored.
+        // into LDS; (2) after a barrier, the knot positions by id (with the
+        // touched-mask words of the self-contact corrections) and the input
+        // multiplier of every item. Slot arrays are filled for every slot
+        // (indices clamped to the last element) so they stay in registers;
+        // only live slots are stored.
         const uint32_t count = itemStart[passes] - itemBase;
-        constexpr uint32_t kKnotSlots = 4, kItemSlots = 8;
-        uint32_t ids[kKnotSlots];
-        const uint32_t lastKnot = loaded > 0u ? loaded - 1u : 0u;
-#pragma unroll
-        for (uint32_t u = 0; u < kKnotSlots; ++u)
-            ids[u] = loaded > 0u ? a.loadIds[base + min(t + u * stride, lastKnot)] : 0u;
+        constexpr uint32_t kItemSlots = 8;
+        tileLoadKnotIds(a, base, loaded, t, stride, ids);
         for (uint32_t j0 = t; j0 < count; j0 += kItemSlots * stride) {
             uint4 v[kItemSlots];
 #pragma unroll
@@ -280,29 +354,24 @@ __global__ void k_tile_program(TileProgramArgs a) {
         if (collides)
             for (uint32_t k = t; k < a.sphereCount; k += stride) sphereCache[k] = a.spheres[k];
         __syncthreads();
-        float4 p[kKnotSlots];
+        tileGatherKnots(lds, a, base, loaded, owned, t, stride, live, ids, [&]() {
+            if constexpr (!Strain) {
+                for (uint32_t j0 = t; j0 < count; j0 += kItemSlots * stride) {
+                    float v[kItemSlots];
 #pragma unroll
-        for (uint32_t u = 0; u < kKnotSlots; ++u) p[u] = a.posIn[ids[u]];
-        if constexpr (!Strain) {
-            for (uint32_t j0 = t; j0 < count; j0 += kItemSlots * stride) {
-                float v[kItemSlots];
+                    for (uint32_t u = 0; u < kItemSlots; ++u) {
+                        const uint32_t j = min(j0 + u * stride, count - 1u);
+                        uint32_t k = 0;
+                        while (starts[k + 1] - itemBase <= j) ++k;
+                        const uint32_t kind = kinds[k];
+                        v[u] = lambdaIn[kind][tileItemConstraint(kind, stagedItems[j])];
+                    }
 #pragma unroll
-                for (uint32_t u = 0; u < kItemSlots; ++u) {
-                    const uint32_t j = min(j0 + u * stride, count - 1u);
-                    uint32_t k = 0;
-                    while (starts[k + 1] - itemBase <= j) ++k;
-                    const uint32_t kind = kinds[k];
-                    v[u] = lambdaIn[kind][tileItemConstraint(kind, stagedItems[j])];
+                    for (uint32_t u = 0; u < kItemSlots; ++u)
+                        if (j0 + u * stride < count) stagedLambdas[j0 + u * stride] = v[u];
                 }
-#pragma unroll
-                for (uint32_t u = 0; u < kItemSlots; ++u)
-                    if (j0 + u * stride < count) stagedLambdas[j0 + u * stride] = v[u];
             }
-        }
-#pragma unroll
-        for (uint32_t u = 0; u < kKnotSlots; ++u)
-            if (t + u * stride < loaded) lds[t + u * stride] = p[u];
-        for (uint32_t i = t + kKnotSlots * stride; i < loaded; i += stride) lds[i] = a.posIn[a.loadIds[base + i]];
+        });
         __syncthreads();
     } else {
         tileLoadItems(items, itemStart, passes, 0, t, stride, cur);
@@ -313,37 +382,11 @@ __global__ void k_tile_program(TileProgramArgs a) {
                                                           : (kind0 == kTileKnotKind ? a.knotLambdaIn : a.bendLambdaIn);
             tileLoadLambdas(in0, itemStart, passes, 0, kind0, t, stride, cur, lam);
         }
-        for (uint32_t i = t; i < loaded; i += stride) lds[i] = a.posIn[a.loadIds[base + i]];
-        if (collides)
-            for (uint32_t k = t; k < a.sphereCount; k += stride) sphereCache[k] = a.spheres[k];
-        __syncthreads();
-    }
-    const bool live = *a.ok != 0u;
-    const float dt = a.cfg->gravityAndTimestep.w;
-    const float invDt2 = 1.0f / (dt * dt);
-    const bool ground = a.cfg->constraintCounts.z != 0u;
-    const float floorZ = a.cfg->clothMaterial.x;
-    if (a.clearAcc != nullptr) {
-        // Owned knots only, one mask bit at a time: other tiles clear the
-        // other bits of a shared word.
-        for (uint32_t i = t; i < owned; i += stride) {
-            const uint32_t id = a.loadIds[base + i];
-            const uint32_t bit = 1u << (id & 31u);
-            if ((a.clearTouched[id >> 5] & bit) != 0u) {
-                a.clearAcc[id] = FixedAccumulator{0, 0, 0, 0u, 0u};
-                atomicAnd(a.clearTouched + (id >> 5), ~bit);
-            }
-        }
-    }
-    if (live && a.selfAcc != nullptr) {
-        for (uint32_t i = t; i < loaded; i += stride) {
-            const uint32_t id = a.loadIds[base + i];
-            if (((a.selfTouched[id >> 5] >> (id & 31u)) & 1u) == 0u) continue;
-            const float4 p = lds[i];
-            const V3 np = xyz(p) + fixedMean(a.selfAcc[id]);
-            if (!finite3(np)) recordFailure(a.failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
-            else lds[i] = withXyz(p, np);
-        }
+        tileLoadKnotIds(a, base, loaded, t, stride, ids);
+        tileGatherKnots(lds, a, base, loaded, owned, t, stride, live, ids, [&]() {
+            if (collides)
+                for (uint32_t k = t; k < a.sphereCount; k += stride) sphereCache[k] = a.spheres[k];
+        });
         __syncthreads();
     }
     if (live && a.leadingCollide != 0)
@@ -401,7 +444,11 @@ __global__ void k_tile_program(TileProgramArgs a) {
     if (live)
         for (uint32_t pass = 0; pass < a.trailingCollides; ++pass)
             tileCollide(lds, loaded, owned, a, ground, floorZ, sphereCache, culled, window, scratch);
-    for (uint32_t i = t; i < owned; i += stride) a.posOut[a.loadIds[base + i]] = lds[i];
+    // The first slot's id is still in a register (the others would cost the
+    // passes registers): most tiles own fewer knots than the workgroup has
+    // threads.
+    if (t < owned) a.posOut[ids[0]] = lds[t];
+    for (uint32_t i = t + stride; i < owned; i += stride) a.posOut[a.loadIds[base + i]] = lds[i];
 }
 
 __global__ void k_copy_float4(const float4* __restrict__ from, float4* __restrict__ to, uint32_t n) {

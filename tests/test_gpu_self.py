@@ -193,6 +193,31 @@ def test_counting_and_radix_sorts_step_identically(cloth, monkeypatch):
     assert states[0]["frictionStatus"].tobytes() == states[1]["frictionStatus"].tobytes()
 
 
+def test_cell_size_changes_no_byte_of_the_step(cloth):
+    """The grid only accelerates the candidate search: cells of 1, 2 and 3
+    segment lengths give the same candidate set and the same substep bytes
+    (corrections and friction are summed per knot in fixed point, so the
+    candidates' order does not matter), with fewer entries in larger cells."""
+    scene = two_layer_impact(cloth)
+    states, pairs, entries = [], [], []
+    for cell in [0.01, 0.02, 0.03]:
+        scene.config["clothMaterial"][0, 1] = cell
+        scene.refresh()
+        with cloth.Solver(scene) as solver:
+            solver.step(scene.configs(8), 8, 2)
+            states.append(solver.read())
+            pairs.append(_sorted(solver.self_broad_phase()["pairs"]))
+            entries.append(solver.diagnostics().selfEntryCount)
+    print(f"entries per cell size: {entries}, candidates {len(pairs[0])}")
+    assert len(pairs[0]) > 0 and entries[0] > entries[1] > entries[2]
+    for k in (1, 2):
+        assert np.array_equal(pairs[0], pairs[k])
+        assert states[0]["failure"] == states[k]["failure"] == 0
+        assert states[0]["particles"].tobytes() == states[k]["particles"].tobytes()
+        assert states[0]["selfStatus"].tobytes() == states[k]["selfStatus"].tobytes()
+        assert states[0]["frictionStatus"].tobytes() == states[k]["frictionStatus"].tobytes()
+
+
 def test_folded_layers_are_pushed_apart(cloth):
     scene = folded_sheet(cloth)
     radius = float(scene.config["clothMaterial"][0, 0])
