@@ -75,7 +75,8 @@ struct TileProgramArgs {
 __device__ __forceinline__ void tileCollide(float4* lds, uint32_t loaded, uint32_t owned, const TileProgramArgs& a, bool ground,
                             float radius, const float4* sphereCache, float4* culled, SphereWindow& window,
                             float4* scratch) {
-    if (a.sphereCount != 0) {
+    const bool windowed = a.sphereCount >= kSphereWindowMin;
+    if (windowed) {
         V3 lo = v3(INFINITY, INFINITY, INFINITY), hi = v3(-INFINITY, -INFINITY, -INFINITY);
         for (uint32_t i = threadIdx.x; i < loaded; i += blockDim.x) {
             const float4 p = lds[i];
@@ -83,7 +84,7 @@ __device__ __forceinline__ void tileCollide(float4* lds, uint32_t loaded, uint32
             lo = v3(fminf(lo.x, p.x), fminf(lo.y, p.y), fminf(lo.z, p.z));
             hi = v3(fmaxf(hi.x, p.x), fmaxf(hi.y, p.y), fmaxf(hi.z, p.z));
         }
-        buildSphereWindow(lo, hi, radius, a.spheres, a.sphereCount, culled, window, scratch);
+        buildSphereWindow(lo, hi, radius, sphereCache, a.sphereCount, culled, window, scratch);
     }
     for (uint32_t i = threadIdx.x; i < loaded; i += blockDim.x) {
         float4 p = lds[i];
@@ -91,9 +92,9 @@ __device__ __forceinline__ void tileCollide(float4* lds, uint32_t loaded, uint32
             if (i < owned) recordFailure(a.failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
             continue;
         }
-        const bool moved = a.sphereCount != 0
+        const bool moved = windowed
                                ? collideKnotWindow(p, ground, radius, window, culled, sphereCache, a.sphereCount)
-                               : collideKnot(p, ground, radius, sphereCache, 0u);
+                               : collideKnot(p, ground, radius, sphereCache, a.sphereCount);
         if (moved) lds[i] = p;
     }
     __syncthreads();
