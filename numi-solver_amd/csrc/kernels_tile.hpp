@@ -37,7 +37,7 @@ struct TileProgramArgs {
     const uint32_t* loadIds;
     const uint32_t* ownedCount;
     const uint32_t* itemStart;
-    const uint4* items;
+    const uint4* items;         // 16-byte items; strain programs: two 8-byte items per uint4
     const float2* distanceRestCompliance;
     const float* distanceMaxLength;
     const float* distanceLambdaIn;
@@ -179,7 +179,23 @@ __device__ __forceinline__ bool tileItemOwner(uint32_t kind, const uint4& item) 
     return ((kind == kTileKnotKind ? item.z : item.y) >> 31) != 0u && kind != kTileStrainKind;
 }
 
+// Item j of a program. Strain programs store 8-byte items, two per uint4
+// (x = local a | local b << 16, y = rest * (1 + limit) as bits): the strain
+// pass reads nothing else, and its passes are bound by these loads. The
+// item is returned in x and y of a uint4 either way, so every kernel keeps
+// the same register arrays.
+template <bool Strain>
+__device__ __forceinline__ uint4 tileItemAt(const uint4* __restrict__ items, uint32_t j) {
+    if constexpr (Strain) {
+        const uint2 v = reinterpret_cast<const uint2*>(items)[j];
+        return make_uint4(v.x, v.y, 0u, 0u);
+    } else {
+        return items[j];
+    }
+}
+
 // Item slots m = 0..kTilePrefetch-1 of thread t in pass k (zero past the end).
+template <bool Strain>
 __device__ __forceinline__ void tileLoadItems(const uint4* __restrict__ items, const uint32_t* __restrict__ itemStart,
                                               uint32_t passes, uint32_t k, uint32_t t, uint32_t stride, uint4* dst) {
     if (k >= passes) return;
@@ -187,7 +203,7 @@ __device__ __forceinline__ void tileLoadItems(const uint4* __restrict__ items, c
 #pragma unroll
     for (uint32_t m = 0; m < kTilePrefetch; ++m) {
         const uint32_t j = first + t + m * stride;
-        dst[m] = j < last ? items[j] : make_uint4(0u, 0u, 0u, 0u);
+        dst[m] = j < last ? tileItemAt<Strain>(items, j) : make_uint4(0u, 0u, 0u, 0u);
     }
 }
 
@@ -215,7 +231,7 @@ __device__ __forceinline__ bool tileItem(float4* lds, uint32_t kind, const uint4
         const uint32_t la = item.x & 0xffffu, lb = item.x >> 16;
         if (!live) return false;
         float4 p = lds[la], q = lds[lb];
-        if (!limitStrainPair(p, q, __uint_as_float(item.z), failure)) return false;
+        if (!limitStrainPair(p, q, __uint_as_float(item.y), failure)) return false;
         lds[la] = p;
         lds[lb] = q;
         return true;
@@ -347,7 +363,8 @@ __global__ void k_tile_program(TileProgramArgs a) {
         for (uint32_t j0 = t; j0 < count; j0 += kItemSlots * stride) {
             uint4 v[kItemSlots];
 #pragma unroll
-            for (uint32_t u = 0; u < kItemSlots; ++u) v[u] = items[itemBase + min(j0 + u * stride, count - 1u)];
+            for (uint32_t u = 0; u < kItemSlots; ++u)
+                v[u] = tileItemAt<Strain>(items, itemBase + min(j0 + u * stride, count - 1u));
 #pragma unroll
             for (uint32_t u = 0; u < kItemSlots; ++u)
                 if (j0 + u * stride < count) stagedItems[j0 + u * stride] = v[u];
@@ -375,8 +392,8 @@ __global__ void k_tile_program(TileProgramArgs a) {
         });
         __syncthreads();
     } else {
-        tileLoadItems(items, itemStart, passes, 0, t, stride, cur);
-        tileLoadItems(items, itemStart, passes, 1, t, stride, next);
+        tileLoadItems<Strain>(items, itemStart, passes, 0, t, stride, cur);
+        tileLoadItems<Strain>(items, itemStart, passes, 1, t, stride, next);
         if constexpr (!Strain) {
             const uint32_t kind0 = a.kinds[0];
             const float* in0 = kind0 == kTileDistanceKind ? a.distanceLambdaIn
@@ -409,7 +426,7 @@ __global__ void k_tile_program(TileProgramArgs a) {
                 if (k + 1 < passes)
                     tileLoadLambdas(lambdaIn[kinds[k + 1]], itemStart, passes, k + 1, kinds[k + 1], t, stride, next,
                                     nextLam);
-            tileLoadItems(items, itemStart, passes, k + 2, t, stride, ahead);
+            tileLoadItems<Strain>(items, itemStart, passes, k + 2, t, stride, ahead);
             const float* in = Strain ? nullptr : lambdaIn[kind];
 #pragma unroll
             for (uint32_t m = 0; m < kTilePrefetch; ++m)
@@ -417,7 +434,7 @@ __global__ void k_tile_program(TileProgramArgs a) {
                     moved |= tileItem<Strain>(lds, kind, cur[m], lam[m], out, knotCC, failure, live, dt, invDt2,
                                               ground, floorZ);
             for (uint32_t j = first + t + kTilePrefetch * stride; j < last; j += stride) {
-                const uint4 item = items[j];
+                const uint4 item = tileItemAt<Strain>(items, j);
                 const float lj = Strain ? 0.0f : in[tileItemConstraint(kind, item)];
                 moved |= tileItem<Strain>(lds, kind, item, lj, out, knotCC, failure, live, dt, invDt2, ground,
                                           floorZ);

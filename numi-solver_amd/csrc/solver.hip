@@ -185,6 +185,7 @@ struct NumiClothHipSolver {
         bool stageable = false;    // the whole program of every tile fits in LDS
         bool staged = false;       // run the staged kernel
         bool hasDistance = false, hasBend = false, hasKnot = false;
+        bool packedStrain = false;  // 8-byte strain items (k_tile_program<true, *>)
         std::vector<uint32_t> kinds;
         DeviceArray<uint32_t> loadStart, loadIds, ownedCount, itemStart;
         DeviceArray<uint4> items;
@@ -599,6 +600,7 @@ struct NumiClothHipSolver {
         a.stageItems = t.staged ? t.maxItems : 0u;
         const size_t shared = a.stageOffset * sizeof(float4) + (t.staged ? stagedBytes(t.maxItems) : 0u);
         const bool strain = a.sweepPasses != 0;
+        if (strain != t.packedStrain) fail(NUMI_CLOTH_HIP_ERROR_UNSUPPORTED, "tile program item layout mismatch");
         if (t.staged) {
             if (strain) k_tile_program<true, true><<<t.tiles, t.block, shared, stream>>>(a);
             else k_tile_program<false, true><<<t.tiles, t.block, shared, stream>>>(a);
@@ -1423,8 +1425,25 @@ bool buildTileProgram(NumiClothHipSolver& s, const std::vector<TilePass>& passes
                                NumiClothHipSolver::stagedBytes(out.maxItems);
     out.stageable = stagedLds <= s.ldsBytes;
     out.staged = out.stageable && s.tileStaged;
-    out.items.allocate(plan.items.size() / 4);
-    out.items.upload(reinterpret_cast<const uint4*>(plan.items.data()), plan.items.size() / 4, st);
+    if (strain) {
+        // A strain program reads 8-byte items (local endpoints, rest * (1 +
+        // limit)), two per uint4; its passes are all strain passes.
+        for (const TilePass& pass : passes)
+            if (pass.kind != kTileStrain && pass.kind != kTileCollide)
+                fail(NUMI_CLOTH_HIP_ERROR_UNSUPPORTED, "a strain program mixes pass kinds");
+        const size_t n = plan.items.size() / 4;
+        std::vector<uint32_t> packed((n + 1) / 2 * 4, 0u);
+        for (size_t j = 0; j < n; ++j) {
+            packed[2 * j] = plan.items[4 * j];
+            packed[2 * j + 1] = plan.items[4 * j + 2];
+        }
+        out.packedStrain = true;
+        out.items.allocate(packed.size() / 4);
+        out.items.upload(reinterpret_cast<const uint4*>(packed.data()), packed.size() / 4, st);
+    } else {
+        out.items.allocate(plan.items.size() / 4);
+        out.items.upload(reinterpret_cast<const uint4*>(plan.items.data()), plan.items.size() / 4, st);
+    }
     HIP_OK(hipStreamSynchronize(st));
     return true;
 }
