@@ -38,7 +38,8 @@ Workloads (synthetic, seeded with SplitMix64 / numpy PCG64 from SEED):
              bend compliance, every knot's inverse mass (density) and damping,
              checkpoints every 16 substeps; reports gradients/s (one gradient
              = one specimen's full parameter gradient) and the cost relative
-             to the plain forward of the same 200 substeps.
+             to the plain forward of the same 200 substeps; every sample runs
+             the window from the seeded state on a fresh batch.
 All use 32 XPBD iterations and 3 strain-limit sweeps per substep.
 """
 
@@ -426,30 +427,44 @@ def run_adjoint(args, cloth, hip) -> dict:
     specimens, local, targets, configs = adjoint_problem(cloth)
     substeps = len(configs)
     particles = sum(len(s.particles) for s in specimens)
-    # Plain forward of the same trajectory (the library's step path).
-    forward = cloth.Batch(specimens)
-    forward.step(configs[:16], ITERATIONS, STRAIN_SWEEPS)
-    forward.synchronize()
-    stream = cloth.cloth.library().numi_cloth_hip_stream(forward.solver_handle)
-    timer = Timer(hip, stream)
-    timer.begin()
-    for _ in range(args.steps):
+    # Each sample runs the workload's 200-substep window from the seeded
+    # state on a fresh batch (a call continues the batch's trajectory, and
+    # later windows of a longer run, where strain limits act, tape more
+    # passes). Plain forward: the library's step path, timed right after a
+    # 16-substep step that captures its graph.
+    forward_times, line, tile_config = [], None, None
+    for k in range(args.steps):
+        forward = cloth.Batch(specimens)
+        forward.step(configs[:16], ITERATIONS, STRAIN_SWEEPS)
+        forward.synchronize()
+        timer = Timer(hip, cloth.cloth.library().numi_cloth_hip_stream(forward.solver_handle))
+        timer.begin()
         forward.step(configs, ITERATIONS, STRAIN_SWEEPS)
-    forward_seconds = timer.end() / args.steps
-    view = cloth.Solver.borrow(forward.solver_handle, specimens[0])
-    line, tile_config = roofline(cloth, view, args.workload)
-    forward.close()
+        forward_times.append(timer.end())
+        if k == args.steps - 1:
+            view = cloth.Solver.borrow(forward.solver_handle, specimens[0])
+            line, tile_config = roofline(cloth, view, args.workload)
+        forward.close()
+    forward_seconds = sum(forward_times) / len(forward_times)
     # Adjoint: forward with checkpoints, then the reverse; the library call
-    # returns when the gradients are in device memory.
-    batch = cloth.Batch(specimens)
-    for _ in range(args.warmup):
+    # returns when the gradients are in device memory. Warm-up calls run on a
+    # batch of their own.
+    if args.warmup:
+        batch = cloth.Batch(specimens)
+        for _ in range(args.warmup):
+            batch.adjoint(configs, ITERATIONS, STRAIN_SWEEPS, local, targets, 16)
+        batch.close()
+    adjoint_times = []
+    for k in range(args.steps):
+        batch = cloth.Batch(specimens)
+        batch.synchronize()
+        timer = Timer(hip, cloth.cloth.library().numi_cloth_hip_stream(batch.solver_handle))
+        timer.begin()
         batch.adjoint(configs, ITERATIONS, STRAIN_SWEEPS, local, targets, 16)
-    stream = cloth.cloth.library().numi_cloth_hip_stream(batch.solver_handle)
-    timer = Timer(hip, stream)
-    timer.begin()
-    for _ in range(args.steps):
-        batch.adjoint(configs, ITERATIONS, STRAIN_SWEEPS, local, targets, 16)
-    adjoint_seconds = timer.end() / args.steps
+        adjoint_times.append(timer.end())
+        if k != args.steps - 1:
+            batch.close()
+    adjoint_seconds = sum(adjoint_times) / len(adjoint_times)
     grads = batch.gradients(0)
     failure = batch.read(0)["failure"]
     batch.close()
@@ -459,6 +474,8 @@ def run_adjoint(args, cloth, hip) -> dict:
               "checkpoint_interval": 16, "tracked_per_specimen": int(local.shape[1]), "iterations": ITERATIONS,
               "strain_sweeps": STRAIN_SWEEPS, "forward_seconds": forward_seconds,
               "adjoint_seconds": adjoint_seconds, "adjoint_over_forward": adjoint_seconds / forward_seconds,
+              "window": "substeps 0-200 from the seeded state (forward: 16-216, after its graph-capture step), "
+                        "a fresh batch per sample",
               "gradient_entries_per_specimen": int(grads["distanceCompliance"].size + grads["bendCompliance"].size
                                                    + grads["inverseMass"].size + grads["damping"].size),
               "loss_specimen0": grads["loss"], "tile_programs": tile_config}
