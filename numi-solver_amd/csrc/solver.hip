@@ -510,7 +510,10 @@ struct NumiClothHipSolver {
             clearSelf(b);
             dirtySelf = -1;
         }
-        const uint32_t blocks = std::min<uint32_t>(2048, std::max(1u, (pairCapacity + kBlock - 1) / kBlock));
+        // Two workgroups per CU: 131,072 candidates per round of the kernel's
+        // grid-stride loop (the workloads here have at most a few hundred),
+        // and a call with no candidate costs little more than its launch.
+        const uint32_t blocks = std::min<uint32_t>(512, std::max(1u, (pairCapacity + kBlock - 1) / kBlock));
         k_self_solve<<<blocks, kBlock, 0, stream>>>(cfg(), ok(), mode, grid.ptr, pairs.ptr, dIJ.ptr, posCur,
                                                      prev.ptr, accumulator[b].ptr, touched[b].ptr, selfImpulses.ptr,
                                                      selfStatus.ptr);
