@@ -1526,6 +1526,19 @@ typedef struct NumiClothOracleOutput {
     double maximumRollingRatio;
     uint32_t selfFrictionContacts;
     uint32_t pointSelfContacts;  /* pairs of the last point self-collision pass */
+    /* Self-contact impulse aggregates of the last substep, one row of 11 per
+     * candidate (s, t, records, sum of -n * impulse xyz, sum of impulses,
+     * endpoint shares e0..e3), up to selfImpulseCapacity rows; the count of
+     * candidates is written to selfImpulseCount. NULL skips them. */
+    double* selfImpulses;
+    uint32_t selfImpulseCapacity;
+    uint32_t selfImpulseCount;
+    /* Per grip, as the ABI grip record holds them: particle, attachment
+     * generation, largest capture distance (float bits in the FP32 twin,
+     * rounded to float in FP64) and the selected cuff-patch centre ring;
+     * then the largest capture error. NULL skips them. */
+    uint32_t* gripRecords;      /* G x 4 */
+    double* gripCaptureError;   /* G */
 } NumiClothOracleOutput;
 
 }  // extern "C"
@@ -1608,6 +1621,23 @@ static int runOracle(const NumiClothHipSceneDesc* scene, const NumiClothBagGPUCo
     out->maximumRollingRatio = o.maximumRollingRatio;
     out->selfFrictionContacts = o.selfFrictionContacts;
     out->pointSelfContacts = o.pointContacts;
+    for (uint32_t i = 0; i < o.G; ++i) {
+        if (out->gripRecords) {
+            const float capture = static_cast<float>(o.gripCapture[i]);
+            uint32_t bits;
+            std::memcpy(&bits, &capture, sizeof bits);
+            const uint32_t r[4] = {o.gripParticle[i], o.gripGeneration[i], bits, o.gripCentre[i]};
+            std::copy(r, r + 4, out->gripRecords + 4 * i);
+        }
+        if (out->gripCaptureError) out->gripCaptureError[i] = o.gripError[i];
+    }
+    out->selfImpulseCount = static_cast<uint32_t>(o.selfImpulses.size());
+    for (uint32_t k = 0; k < o.selfImpulses.size() && k < out->selfImpulseCapacity && out->selfImpulses; ++k) {
+        const auto& a = o.selfImpulses[k];
+        const double r[11] = {double(o.candidates[k].first), double(o.candidates[k].second), double(a.count),
+                              a.n.x, a.n.y, a.n.z, a.impulse, a.e[0], a.e[1], a.e[2], a.e[3]};
+        std::copy(r, r + 11, out->selfImpulses + 11 * k);
+    }
     return 0;
 }
 

@@ -28,7 +28,9 @@ class OracleOutput(C.Structure):
                 ("releasedMask", C.c_uint32), ("pairFrictionContacts", C.c_uint32),
                 ("yarnFrictionContacts", C.c_uint32), ("fruitGroundContacts", C.c_uint32),
                 ("rollingContacts", C.c_uint32), ("maximumRollingRatio", C.c_double),
-                ("selfFrictionContacts", C.c_uint32), ("pointSelfContacts", C.c_uint32)]
+                ("selfFrictionContacts", C.c_uint32), ("pointSelfContacts", C.c_uint32),
+                ("selfImpulses", C.c_void_p), ("selfImpulseCapacity", C.c_uint32),
+                ("selfImpulseCount", C.c_uint32), ("gripRecords", C.c_void_p), ("gripCaptureError", C.c_void_p)]
 
 
 _lib = None
@@ -66,10 +68,14 @@ def run(scene, configs: np.ndarray, iterations: int, strain_sweeps: int, precisi
         "gripLambda": np.zeros((len(scene.grips), 3)), "gripParticle": np.zeros(len(scene.grips), np.uint32),
         "fruits": np.zeros((len(scene.fruits), 20)), "fruitPairs": np.zeros((len(scene.fruitPairs), 4)),
         "yarnContacts": np.zeros((len(scene.yarnContacts), 24)), "mouthClearance": np.zeros(len(scene.fruits)),
+        "gripRecords": np.zeros((len(scene.grips), 4), np.uint32), "gripCaptureError": np.zeros(len(scene.grips)),
     }
     result = OracleOutput()
     for name, array in out.items():
         setattr(result, name, array.ctypes.data if array.size else None)
+    impulses = np.zeros((max(1, min(len(scene.distances) * 16, 4096)), 11))
+    result.selfImpulses = impulses.ctypes.data
+    result.selfImpulseCapacity = len(impulses)
     entry = library().numi_cloth_oracle_run if precision == "fp64" else library().numi_cloth_oracle_run_fp32
     status = entry(C.addressof(scene.desc), configs.ctypes.data, len(configs), iterations, strain_sweeps,
                    C.byref(result))
@@ -86,6 +92,9 @@ def run(scene, configs: np.ndarray, iterations: int, strain_sweeps: int, precisi
         out[name] = int(getattr(result, name))
     for name in ["maximumRollingRatio", "maximumYarnAeroForce", "maximumFruitAeroForce", "maximumFruitAeroTorque"]:
         out[name] = float(getattr(result, name))
+    # Per candidate of the last substep: s, t, records, sum -n*impulse (3),
+    # sum impulse, endpoint shares (4).
+    out["selfImpulses"] = impulses[:min(int(result.selfImpulseCount), len(impulses))].copy()
     return out
 
 

@@ -250,3 +250,22 @@ def crossing_segments_probe(cloth, mu, slip=0.2, overlap=0.001, dt=1.0 / 960.0, 
     c["clothMaterial"] = (radius, 0.02, 0.45, mu)
     return cloth.Scene(c, particles=p, distances=d, distanceBatches=batch(cloth, 0, 2, 0), selfContact=1,
                        selfContactMargin=0.25 * radius)
+
+
+def self_ccd_probe(cloth):
+    """The reference's yarn/yarn swept-contact probe (tools/cloth_bag_metal.mm:
+    1497-1554): a fixed-length 2 m segment at rest on the x axis and a second
+    2 m segment along y, 8 cm above it, moving at (2, 0, -16) m/s, so that in
+    one 0.01 s substep it would pass 8 cm through the first. Unit inverse
+    masses, no gravity, no ground, 0 iterations and 0 strain sweeps: only the
+    swept self-contact pass acts. The hash cell is the reference's
+    2 + 2r + 1e-6 m."""
+    radius = 0.004
+    p = particles(cloth, [(-1.0, 0.0, 0.0), (1.0, 0.0, 0.0), (0.0, -1.0, 0.08), (0.0, 1.0, 0.08)],
+                  [1.0, 1.0, 1.0, 1.0],
+                  velocities=[(0, 0, 0), (0, 0, 0), (2.0, 0.0, -16.0), (2.0, 0.0, -16.0)])
+    d = np.concatenate([distance(cloth, 0, 1, 0.0, limit=0.0), distance(cloth, 2, 3, 0.0, limit=0.0)])
+    c = config(cloth, particles=4, distances=2, dt=0.01, radius=radius)
+    c["clothMaterial"] = (radius, np.float32(2.0) + np.float32(2.0) * np.float32(radius) + np.float32(1.0e-6),
+                          0.45, 0.34)
+    return cloth.Scene(c, particles=p, distances=d, distanceBatches=batch(cloth, 0, 2, 0), selfContact=1)
