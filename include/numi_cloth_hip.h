@@ -59,8 +59,9 @@ typedef struct NumiClothHipSceneExtension {
     /* Maximum candidate pairs kept per substep (0 selects 8 per segment). */
     uint32_t selfCandidateCapacity;
     /* Kinematic drive of particles with zero inverse mass:
-     * z(t) = anchor.z + amplitude * sin(omega * t + phase), anchor = the
-     * position given at create, t = time at the end of the substep. */
+     * z(t) = anchor.z + amplitude * (sin(omega * t + phase) - sin(phase)),
+     * anchor = the position given at create (where the knot starts),
+     * t = time at the end of the substep. */
     float driveAmplitude;
     float driveOmega;
     float drivePhase;

@@ -151,8 +151,9 @@ typedef struct NumiClothHipBagSpec {
 
 /* A hanging strip specimen (batched calibration and adjoint workloads): a
  * width x height knot grid in the x-z plane, top row (j = 0) pinned and
- * driven vertically by amplitude * sin(2 pi frequency t + phase), no
- * ground, warp/weft distances and chord bends. Knot (i, j) sits at
+ * driven vertically by amplitude * (sin(2 pi frequency t + phase) -
+ * sin(phase)) from where it starts, no ground, warp/weft distances and
+ * chord bends. Knot (i, j) sits at
  * origin + (i spacing, jitter, -j spacing), row-major. */
 typedef struct NumiClothHipStripSpec {
     uint32_t width;

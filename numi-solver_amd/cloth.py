@@ -542,6 +542,23 @@ class Scene:
         """`substeps` copies of the initial configuration."""
         return np.repeat(self.config, substeps)
 
+    def continued(self, state: dict) -> "Scene":
+        """This scene with the knots, grips and fruits of a read-back state
+        as its initial state, and the release masks it ended with (the
+        reference's continuedInitialState, tools/cloth_bag_metal.mm:4764-4790):
+        a solver created from it continues the run the state came from."""
+        arrays = {name: getattr(self, name) for name, _, _ in self.ARRAYS}
+        for name in ("particles", "grips", "fruits"):
+            if len(arrays[name]):
+                arrays[name] = np.ascontiguousarray(state[name]).copy()
+        masks = state["releaseStatus"]["masks"][0]
+        return Scene(self.config, staticSpheres=self.staticSpheres, selfContact=self.selfContact,
+                     selfContactMargin=self.selfContactMargin, selfCandidateCapacity=self.selfCandidateCapacity,
+                     driveAmplitude=self.driveAmplitude, driveOmega=self.driveOmega, drivePhase=self.drivePhase,
+                     damping=self.damping, particleMotion=self.particleMotion, sweptGround=self.sweptGround,
+                     pointSelfCollision=self.pointSelfCollision, initialCandidateMask=int(masks[0]),
+                     initialReleasedMask=int(masks[1]), **arrays)
+
 
 # ------------------------------------------------------------------- solver --
 

@@ -61,6 +61,14 @@ __device__ __forceinline__ float clampf(float v, float lo, float hi) {
     return fminf(fmaxf(v, lo), hi);
 }
 
+// Kinematic drive of a pinned knot at time t: m = (amplitude, omega, phase),
+// offset A (sin(omega t + phase) - sin(phase)) along z, so a driven knot
+// starts where it was created. Evaluated in double, rounded to float.
+__device__ __forceinline__ float driveOffset(const float4& m, double t) {
+    return static_cast<float>(static_cast<double>(m.x) *
+                              (sin(static_cast<double>(m.y) * t + static_cast<double>(m.z)) - sin(static_cast<double>(m.z))));
+}
+
 __device__ __forceinline__ void recordFailure(uint32_t* failure, uint32_t bit) {
     atomicOr(failure, bit);
 }

@@ -105,8 +105,7 @@ __device__ __forceinline__ void specimenBegin(const SpecimenArgs& a, uint32_t i,
     } else if (a.anchor != nullptr) {
         const float4 anchor = a.anchor[i];
         const float4 m = a.motion != nullptr ? a.motion[i] : a.drive;
-        const float offset = static_cast<float>(
-            static_cast<double>(m.x) * sin(static_cast<double>(m.y) * endTime + static_cast<double>(m.z)));
+        const float offset = driveOffset(m, endTime);
         x = withXyz(x, v3(anchor.x, anchor.y, anchor.z + offset));
     }
 }

@@ -85,12 +85,11 @@ __global__ void k_begin(BeginArgs a) {
                 v.w = v.z;
                 if (a.advance) p = withXyz(p, fma3(nu, dt, x));
             } else if (a.anchor != nullptr) {
-                // Kinematic drive, evaluated in double like the host
-                // reference: z = anchor.z + A sin(omega t + phase).
+                // Kinematic drive, evaluated in double:
+                // z = anchor.z + A (sin(omega t + phase) - sin(phase)).
                 const float4 anchor = a.anchor[i];
                 const float4 m = a.motion != nullptr ? a.motion[i] : a.drive;
-                const float offset = static_cast<float>(
-                    static_cast<double>(m.x) * sin(static_cast<double>(m.y) * *a.time + static_cast<double>(m.z)));
+                const float offset = driveOffset(m, *a.time);
                 p = withXyz(p, v3(anchor.x, anchor.y, anchor.z + offset));
             }
             a.prev[i] = q;
@@ -524,10 +523,12 @@ struct SphereWindow {
 
 constexpr uint32_t kMaxBlockWaves = 16;
 
-// Lists of at most this many spheres are tested whole (collideKnot): the
+// Lists of fewer than this many spheres are tested whole (collideKnot): the
 // window's box reductions and barriers cost more than testing a few spheres
 // per knot (measured on gfx950: ≈3.3 µs per tile pass for the window against
-// ≈0.8 µs for a one-sphere list). Both give the same bytes.
+// ≈0.8 µs for a one-sphere list). Both give the same bytes; the solver passes
+// the threshold to the kernels (NUMI_CLOTH_HIP_SPHERE_WINDOW overrides it at
+// create, so tests can run either path on any scene).
 constexpr uint32_t kSphereWindowMin = 5;
 
 // Every thread of the workgroup calls this with the box of its own knots
@@ -648,7 +649,7 @@ __device__ __forceinline__ void knotBox(const float4& p, bool has, V3& lo, V3& h
 // spheres, then the window's list (2 * sphereCount float4).
 __global__ void k_collide(const NumiClothBagGPUConfig* __restrict__ cfg, const uint32_t* __restrict__ ok,
                           float4* __restrict__ pos, uint32_t particles, const float4* __restrict__ spheres,
-                          uint32_t sphereCount, uint32_t* __restrict__ failure) {
+                          uint32_t sphereCount, uint32_t windowMin, uint32_t* __restrict__ failure) {
     extern __shared__ float4 sphereCache[];
     __shared__ SphereWindow window;
     __shared__ float4 scratch[2 * kMaxBlockWaves];
@@ -659,7 +660,7 @@ __global__ void k_collide(const NumiClothBagGPUConfig* __restrict__ cfg, const u
     const bool has = live && i < particles;
     const float4 p0 = has ? pos[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     const float r = cfg->clothMaterial.x;
-    const bool windowed = sphereCount >= kSphereWindowMin;
+    const bool windowed = sphereCount >= windowMin;
     if (windowed) {
         V3 lo, hi;
         knotBox(p0, has && isfinite(p0.z), lo, hi);

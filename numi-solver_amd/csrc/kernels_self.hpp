@@ -73,10 +73,16 @@ struct SortedEntry {
     float4 a, b;
 };
 
+// Per-knot sums of one Jacobi pass (self-contact corrections, point
+// self-collision corrections or self-friction velocity changes), all order
+// independent: the corrections and their lengths in 2^-32 fixed point, the
+// longest length as float bits (integer max orders non-negative floats), and
+// the number of contributions.
 struct FixedAccumulator {
     long long x, y, z;
+    long long lengthSum;
+    uint32_t maxLength;
     uint32_t count;
-    uint32_t pad;
 };
 
 // ---------------------------------------------------------- broad phase --
@@ -673,9 +679,16 @@ __device__ V3 contactNormal(V3 separation, V3 da, V3 db, V3 previousOffset) {
     return n;
 }
 
-// Mean correction of a touched knot's fixed-point sum.
+// The move of a touched knot: the sum of its corrections scaled by
+// (longest length) / (sum of lengths). One correction moves the knot by
+// itself, n equal ones by one of them (their mean), and a correction that
+// shrinks to zero drops out continuously; a mean by contribution count would
+// jump whenever a contact at its threshold appears or vanishes, and a knot
+// pressed by one deep and several grazing contacts would keep most of its
+// penetration. Never longer than the longest correction.
 __device__ __forceinline__ V3 fixedMean(const FixedAccumulator& a) {
-    const double scale = kFixedInverse / static_cast<double>(a.count);
+    if (a.lengthSum <= 0) return v3(0.0f, 0.0f, 0.0f);
+    const double scale = static_cast<double>(__uint_as_float(a.maxLength)) / static_cast<double>(a.lengthSum);
     return v3(static_cast<float>(static_cast<double>(a.x) * scale), static_cast<float>(static_cast<double>(a.y) * scale),
               static_cast<float>(static_cast<double>(a.z) * scale));
 }
@@ -691,6 +704,10 @@ __device__ __forceinline__ void addFixed(FixedAccumulator* acc, uint32_t* touche
               static_cast<unsigned long long>(__double2ll_rn(static_cast<double>(d.y) * kFixedScale)));
     atomicAdd(reinterpret_cast<unsigned long long*>(&a->z),
               static_cast<unsigned long long>(__double2ll_rn(static_cast<double>(d.z) * kFixedScale)));
+    const float len = length(d);
+    atomicAdd(reinterpret_cast<unsigned long long*>(&a->lengthSum),
+              static_cast<unsigned long long>(__double2ll_rn(static_cast<double>(len) * kFixedScale)));
+    atomicMax(&a->maxLength, __float_as_uint(len));
     if (atomicAdd(&a->count, 1u) == 0u) atomicOr(touched + (i >> 5), 1u << (i & 31u));
 }
 
@@ -846,7 +863,7 @@ __global__ void k_self_apply(const uint32_t* __restrict__ ok, float4* __restrict
         const uint32_t i = (w << 5) + static_cast<uint32_t>(__ffs(bits) - 1);
         bits &= bits - 1u;
         if (pos == nullptr || *ok == 0u) {
-            acc[i] = FixedAccumulator{0, 0, 0, 0u, 0u};
+            acc[i] = FixedAccumulator{0, 0, 0, 0, 0u, 0u};
             continue;
         }
         const V3 d = fixedMean(acc[i]);
@@ -854,7 +871,7 @@ __global__ void k_self_apply(const uint32_t* __restrict__ ok, float4* __restrict
         const V3 np = xyz(p) + d;
         if (!finite3(np)) recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
         else pos[i] = withXyz(p, np);
-        acc[i] = FixedAccumulator{0, 0, 0, 0u, 0u};
+        acc[i] = FixedAccumulator{0, 0, 0, 0, 0u, 0u};
     }
 }
 

@@ -50,6 +50,7 @@ struct TileProgramArgs {
     float* knotLambdaOut;
     const float4* spheres;
     uint32_t sphereCount;
+    uint32_t sphereWindowMin;   // lists of at least this many spheres use a sphere window
     uint32_t passes;
     uint32_t sweepPasses;       // strain programs: passes per sweep (0 otherwise)
     // Self-contact corrections to apply first (null: none), and the buffer of
@@ -75,7 +76,7 @@ struct TileProgramArgs {
 __device__ __forceinline__ void tileCollide(float4* lds, uint32_t loaded, uint32_t owned, const TileProgramArgs& a, bool ground,
                             float radius, const float4* sphereCache, float4* culled, SphereWindow& window,
                             float4* scratch) {
-    const bool windowed = a.sphereCount >= kSphereWindowMin;
+    const bool windowed = a.sphereCount >= a.sphereWindowMin;
     if (windowed) {
         V3 lo = v3(INFINITY, INFINITY, INFINITY), hi = v3(-INFINITY, -INFINITY, -INFINITY);
         for (uint32_t i = threadIdx.x; i < loaded; i += blockDim.x) {
@@ -150,7 +151,7 @@ __device__ __forceinline__ void tileGatherKnots(float4* lds, const TileProgramAr
         }
         lds[i] = p[u];
         if (i < owned && (clearWord[u] & bit) != 0u) {
-            a.clearAcc[id] = FixedAccumulator{0, 0, 0, 0u, 0u};
+            a.clearAcc[id] = FixedAccumulator{0, 0, 0, 0, 0u, 0u};
             atomicAnd(a.clearTouched + (id >> 5), ~bit);
         }
     }
@@ -165,7 +166,7 @@ __device__ __forceinline__ void tileGatherKnots(float4* lds, const TileProgramAr
         }
         lds[i] = q;
         if (clearing && i < owned && (a.clearTouched[id >> 5] & bit) != 0u) {
-            a.clearAcc[id] = FixedAccumulator{0, 0, 0, 0u, 0u};
+            a.clearAcc[id] = FixedAccumulator{0, 0, 0, 0, 0u, 0u};
             atomicAnd(a.clearTouched + (id >> 5), ~bit);
         }
     }

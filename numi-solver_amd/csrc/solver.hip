@@ -210,6 +210,7 @@ struct NumiClothHipSolver {
     // kernels directly on the stream (used under profilers that do not
     // follow graph launches).
     bool useGraphs = true;
+    uint32_t sphereWindowMin = kSphereWindowMin;
 
     // Pinned staging for per-step uploads.
     NumiClothBagGPUConfig* stagingConfigs = nullptr;
@@ -372,7 +373,7 @@ struct NumiClothHipSolver {
         if (!groundOrColliders()) return;
         const uint32_t n = extension.staticSphereCount;
         k_collide<<<blocksFor(P), kBlock, 2 * n * sizeof(float4), stream>>>(cfg(), ok(), posCur, P, spheres.ptr, n,
-                                                                            failure.ptr);
+                                                                            sphereWindowMin, failure.ptr);
     }
 
     // A ground/sphere pass that follows self-contact can run at the start of
@@ -579,6 +580,7 @@ struct NumiClothHipSolver {
         a.knotLambdaOut = kLamCur == kLambda.ptr ? kLambdaAlt.ptr : kLambda.ptr;
         a.spheres = spheres.ptr;
         a.sphereCount = extension.staticSphereCount;
+        a.sphereWindowMin = sphereWindowMin;
         a.passes = t.passes;
         a.sweepPasses = &t == &tileStrain ? static_cast<uint32_t>(distanceColours.size()) : 0u;
         a.leadingCollide = !scratch && pendingCollide ? 1u : 0u;
@@ -1157,6 +1159,9 @@ void createSolver(const NumiClothHipSceneDesc& scene, NumiClothHipSolver& s) {
     s.releaseStatus.upload(&masks, 1, st);
     s.diagPartials.allocate(1024);
     if (const char* graphs = std::getenv("NUMI_CLOTH_HIP_GRAPHS")) s.useGraphs = std::strcmp(graphs, "0") != 0;
+    // "0": test every sphere list whole; "1": window every non-empty list.
+    if (const char* window = std::getenv("NUMI_CLOTH_HIP_SPHERE_WINDOW"))
+        s.sphereWindowMin = std::strcmp(window, "0") == 0 ? UINT32_MAX : 1u;
     s.posCur = s.pos.ptr;
     s.dLamCur = s.dLambda.ptr;
     s.bLamCur = s.bLambda.ptr;
@@ -1761,7 +1766,8 @@ void timePass(NumiClothHipSolver& s, uint32_t pass, uint32_t colour, uint32_t re
         } else {
             const uint32_t n = s.extension.staticSphereCount;
             k_collide<<<blocksFor(s.P), kBlock, 2 * n * sizeof(float4), s.stream>>>(s.cfg(), s.ok(), s.pos.ptr, s.P,
-                                                                               s.spheres.ptr, n, s.failure.ptr);
+                                                                               s.spheres.ptr, n, s.sphereWindowMin,
+                                                                               s.failure.ptr);
         }
     }
     HIP_OK(hipGetLastError());

@@ -20,8 +20,10 @@
 // tile of its first participant), is exact at the end. Halo results are
 // discarded, so the written state equals the per-colour schedule's.
 //
-// Tiles are consecutive chunks of the knots in Morton order of their
-// positions at create, so they are spatially compact for any mesh.
+// Tiles are consecutive chunks of the knots ordered by connected component
+// and then by the Morton code of their positions at create, so they are
+// spatially compact for any mesh and never mix independent specimens more
+// than a chunk boundary requires.
 
 #include <algorithm>
 #include <cstdint>
@@ -123,19 +125,53 @@ inline bool planTiles(const TileTopology& topo, const std::vector<TilePass>& pro
     // and consecutive codes form compact patches.
     double span = 0.0;
     for (int a = 0; a < 3; ++a) span = std::max(span, static_cast<double>(hi[a]) - lo[a]);
-    std::vector<std::pair<uint64_t, uint32_t>> keyed(P);
+    // Connected components of the program's constraint graph (union-find,
+    // each labelled by its lowest knot): knots are ordered by component
+    // first, so independent specimens merged into one scene tile apart even
+    // where they overlap in space.
+    std::vector<uint32_t> parent(P);
+    for (uint32_t i = 0; i < P; ++i) parent[i] = i;
+    auto root = [&](uint32_t i) {
+        while (parent[i] != i) {
+            parent[i] = parent[parent[i]];
+            i = parent[i];
+        }
+        return i;
+    };
+    for (const TilePass& pass : program) {
+        if (pass.kind == kTileCollide) continue;
+        for (uint32_t c = pass.first; c < pass.first + pass.count; ++c) {
+            uint32_t ids[4];
+            const uint32_t n = participantsOf(topo, pass.kind, c, ids);
+            for (uint32_t m = 1; m < n; ++m) {
+                const uint32_t a = root(ids[0]), b = root(ids[m]);
+                if (a != b) parent[std::max(a, b)] = std::min(a, b);
+            }
+        }
+    }
+    struct Keyed {
+        uint32_t component;
+        uint64_t code;
+        uint32_t knot;
+        bool operator<(const Keyed& o) const {
+            if (component != o.component) return component < o.component;
+            if (code != o.code) return code < o.code;
+            return knot < o.knot;
+        }
+    };
+    std::vector<Keyed> keyed(P);
     for (uint32_t i = 0; i < P; ++i) {
         uint64_t code = 0;
         for (int a = 0; a < 3; ++a) {
             const double u = span > 0.0 ? (static_cast<double>(topo.positions[4 * i + a]) - lo[a]) / span : 0.0;
             code |= spreadBits21(static_cast<uint64_t>(u * 2097151.0)) << a;
         }
-        keyed[i] = {code, i};
+        keyed[i] = {root(i), code, i};
     }
     std::sort(keyed.begin(), keyed.end());
     const uint32_t tiles = (P + owned - 1) / owned;
     std::vector<uint32_t> tileOf(P);
-    for (uint32_t k = 0; k < P; ++k) tileOf[keyed[k].second] = k / owned;
+    for (uint32_t k = 0; k < P; ++k) tileOf[keyed[k].knot] = k / owned;
 
     // Per pass: the constraint touching each knot (colours are disjoint).
     const uint32_t passes = static_cast<uint32_t>(program.size());
@@ -183,7 +219,7 @@ inline bool planTiles(const TileTopology& topo, const std::vector<TilePass>& pro
         for (uint32_t halo = startHalo; halo <= maxHalo && !done; ++halo) {
             // Owned knots first (Morton order), then the halo by BFS depth.
             loaded.clear();
-            for (uint32_t k = begin; k < end; ++k) loaded.push_back(keyed[k].second);
+            for (uint32_t k = begin; k < end; ++k) loaded.push_back(keyed[k].knot);
             for (uint32_t p : loaded) localOf[p] = -1;
             for (uint32_t k = 0; k < loaded.size(); ++k) {
                 localOf[loaded[k]] = static_cast<int32_t>(k);

@@ -125,7 +125,7 @@ struct Model {
             om = e.particleMotion[i].y;
             ph = e.particleMotion[i].z;
         }
-        return R(static_cast<float>(double(a) * std::sin(double(om) * endTime + double(ph))));
+        return R(static_cast<float>(double(a) * (std::sin(double(om) * endTime + double(ph)) - std::sin(double(ph)))));
     }
 
     void distancePass(const NumiClothBagGPUBatch& batch, R dt) {

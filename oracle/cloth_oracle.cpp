@@ -52,7 +52,9 @@ struct Particle {
 // Self-contact accumulator: FP64 sums exactly in double; the FP32 twin uses
 // the device's 2^-32 m fixed point so that sums are order independent.
 struct Fixed {
-    long long x = 0, y = 0, z = 0;
+    long long x = 0, y = 0, z = 0;  // sum of the corrections, 2^-32 m units
+    long long m = 0;                // sum of their lengths, same units
+    float maxLength = 0.0f;         // the longest correction
     uint32_t count = 0;
 };
 
@@ -70,6 +72,7 @@ struct Oracle {
     std::vector<V<R>> deltaSum;
     std::vector<Fixed> fixed;
     std::vector<uint32_t> deltaCount;
+    std::vector<R> deltaLengthSum, deltaLengthMax;
     uint32_t failure = 0;
     uint32_t frictionContacts = 0;
     uint32_t presentContacts = 0, sweptContacts = 0;
@@ -162,11 +165,12 @@ struct Oracle {
         return sizeof(R) == sizeof(float) ? R(1.0f / (1.0f + damping * dt0)) : R(1) / (R(1) + R(damping) * R(dt0));
     }
 
-    // Kinematic drive offset A sin(omega t + phase), evaluated in double
-    // and rounded to float as the device does.
+    // Kinematic drive offset A (sin(omega t + phase) - sin(phase)), evaluated
+    // in double and rounded to float as the device does.
     static R driveOffset(float amplitude, float omega, float phase, double time) {
         return R(static_cast<float>(static_cast<double>(amplitude) *
-                                    std::sin(static_cast<double>(omega) * time + static_cast<double>(phase))));
+                                    (std::sin(static_cast<double>(omega) * time + static_cast<double>(phase)) -
+                                     std::sin(static_cast<double>(phase)))));
     }
 
     void begin(const NumiClothBagGPUConfig& c, double endTime) {
@@ -288,9 +292,7 @@ struct Oracle {
             if (std::isfinite(static_cast<double>(p[i].x.x)) && std::isfinite(static_cast<double>(p[i].x.y)) &&
                 std::isfinite(static_cast<double>(p[i].x.z)))
                 cells[cellOf(i)].push_back(i);
-        fixed.assign(P, Fixed{});
-        deltaSum.assign(P, V<R>{});
-        deltaCount.assign(P, 0u);
+        clearAccumulators();
         pointContacts = 0;
         for (const auto& [cell, members] : cells)
             for (uint32_t i : members)
@@ -314,16 +316,8 @@ struct Oracle {
                             }
                         }
         for (uint32_t i = 0; i < P; ++i) {
-            if (sizeof(R) == sizeof(float)) {
-                const Fixed& f = fixed[i];
-                if (f.count == 0) continue;
-                const double scale = (1.0 / 4294967296.0) / static_cast<double>(f.count);
-                p[i].x = p[i].x + V<R>{R(static_cast<double>(f.x) * scale), R(static_cast<double>(f.y) * scale),
-                                       R(static_cast<double>(f.z) * scale)};
-            } else {
-                if (deltaCount[i] == 0) continue;
-                p[i].x = p[i].x + deltaSum[i] / R(deltaCount[i]);
-            }
+            V<R> move;
+            if (blendedMove(i, move)) p[i].x = p[i].x + move;
         }
     }
 
@@ -603,14 +597,48 @@ struct Oracle {
     void accumulate(uint32_t i, V<R> delta) {
         if (sizeof(R) == sizeof(float)) {
             Fixed& f = fixed[i];
+            const float length = static_cast<float>(norm(delta));
             f.x += std::llrint(static_cast<double>(delta.x) * 4294967296.0);
             f.y += std::llrint(static_cast<double>(delta.y) * 4294967296.0);
             f.z += std::llrint(static_cast<double>(delta.z) * 4294967296.0);
+            f.m += std::llrint(static_cast<double>(length) * 4294967296.0);
+            f.maxLength = std::max(f.maxLength, length);
             f.count += 1;
         } else {
             deltaSum[i] = deltaSum[i] + delta;
+            const R length = norm(delta);
+            deltaLengthSum[i] = deltaLengthSum[i] + length;
+            deltaLengthMax[i] = std::max(deltaLengthMax[i], length);
             deltaCount[i] += 1;
         }
+    }
+
+    void clearAccumulators() {
+        fixed.assign(P, Fixed{});
+        deltaSum.assign(P, V<R>{});
+        deltaCount.assign(P, 0u);
+        deltaLengthSum.assign(P, R(0));
+        deltaLengthMax.assign(P, R(0));
+    }
+
+    // The move of knot i from the corrections accumulated in one Jacobi pass:
+    // their sum scaled by (longest length) / (sum of lengths). One correction
+    // moves the knot by itself, n equal ones by one of them (the mean), and a
+    // correction that shrinks to zero drops out continuously (a count-based
+    // mean would jump when a contact at the threshold appears or vanishes).
+    // FP32: the device's 2^-32 m fixed-point sums, combined in double.
+    bool blendedMove(uint32_t i, V<R>& move) const {
+        if (sizeof(R) == sizeof(float)) {
+            const Fixed& f = fixed[i];
+            if (f.count == 0 || f.m <= 0) return false;
+            const double scale = static_cast<double>(f.maxLength) / static_cast<double>(f.m);
+            move = V<R>{R(static_cast<double>(f.x) * scale), R(static_cast<double>(f.y) * scale),
+                        R(static_cast<double>(f.z) * scale)};
+            return true;
+        }
+        if (deltaCount[i] == 0 || !(deltaLengthSum[i] > R(0))) return false;
+        move = deltaSum[i] * (deltaLengthMax[i] / deltaLengthSum[i]);
+        return true;
     }
 
     // Jacobi self-contact solve over the candidates, one snapshot.
@@ -619,9 +647,7 @@ struct Oracle {
         const R r = R(c.clothMaterial.x);
         const R target = R(2) * r;
         const R tolerance = R(1.0e-9);
-        fixed.assign(P, Fixed{});
-        deltaSum.assign(P, V<R>{});
-        deltaCount.assign(P, 0u);
+        clearAccumulators();
         selfImpulses.resize(candidates.size());
         for (size_t k = 0; k < candidates.size(); ++k) {
             const auto& pr = candidates[k];
@@ -719,16 +745,8 @@ struct Oracle {
             a.e[3] = a.e[3] + wb * impulse;
         }
         for (uint32_t i = 0; i < P; ++i) {
-            if (sizeof(R) == sizeof(float)) {
-                const Fixed& f = fixed[i];
-                if (f.count == 0) continue;
-                const double scale = (1.0 / 4294967296.0) / static_cast<double>(f.count);
-                p[i].x = p[i].x + V<R>{R(static_cast<double>(f.x) * scale), R(static_cast<double>(f.y) * scale),
-                                       R(static_cast<double>(f.z) * scale)};
-            } else {
-                if (deltaCount[i] == 0) continue;
-                p[i].x = p[i].x + deltaSum[i] / R(deltaCount[i]);
-            }
+            V<R> move;
+            if (blendedMove(i, move)) p[i].x = p[i].x + move;
         }
     }
 
@@ -740,9 +758,7 @@ struct Oracle {
     void selfFriction(const NumiClothBagGPUConfig& c) {
         if (!d.extension.selfContact) return;
         const R mu = R(c.clothMaterial.w);
-        fixed.assign(P, Fixed{});
-        deltaSum.assign(P, V<R>{});
-        deltaCount.assign(P, 0u);
+        clearAccumulators();
         for (size_t k = 0; k < selfImpulses.size(); ++k) {
             const SelfImpulse& a = selfImpulses[k];
             if (a.count == 0) continue;
@@ -780,16 +796,8 @@ struct Oracle {
             if (limit > R(0)) maximumConeRatio = std::max(maximumConeRatio, static_cast<double>(tangential / limit));
         }
         for (uint32_t i = 0; i < P; ++i) {
-            if (sizeof(R) == sizeof(float)) {
-                const Fixed& f = fixed[i];
-                if (f.count == 0) continue;
-                const double scale = (1.0 / 4294967296.0) / static_cast<double>(f.count);
-                p[i].v = p[i].v + V<R>{R(static_cast<double>(f.x) * scale), R(static_cast<double>(f.y) * scale),
-                                       R(static_cast<double>(f.z) * scale)};
-            } else {
-                if (deltaCount[i] == 0) continue;
-                p[i].v = p[i].v + deltaSum[i] / R(deltaCount[i]);
-            }
+            V<R> move;
+            if (blendedMove(i, move)) p[i].v = p[i].v + move;
         }
     }
 

@@ -84,8 +84,12 @@ def test_driven_top_row_follows_sinusoid(cloth, oracle):
     n = 37
     out = oracle.run(s, s.configs(n), 4, 1)
     t = n * dt
-    expected = 1.0 + float(np.float32(0.02 * np.sin(float(np.float32(2 * np.pi * 1.25)) * t + float(np.float32(0.3)))))
+    omega, phase = float(np.float32(2 * np.pi * 1.25)), float(np.float32(0.3))
+    # The drive starts where the row was created: A (sin(omega t + phase) - sin(phase)).
+    expected = 1.0 + float(np.float32(0.02 * (np.sin(omega * t + phase) - np.sin(phase))))
     assert out["positions"][:4, 2] == pytest.approx([expected] * 4, abs=1e-7)
+    start = oracle.run(s, s.configs(1), 4, 1)
+    assert start["positions"][:4, 2] == pytest.approx([1.0] * 4, abs=2e-4)
 
 
 def test_merge_rejects_coupled_specimens(cloth):

@@ -85,12 +85,21 @@ def test_patch_regrab_on_the_gpu(cloth, oracle, golden):
         twin = oracle.run(scene, configs[:substeps], 32, 3, precision="fp32")
         assert np.array_equal(records, twin["gripRecords"])
     assert float(np.max(end["grips"]["lambda"][:, 3])) <= 2e-6
+    # Positions: the GPU is the FP32 twin's arithmetic; the stiff bag
+    # (axial compliance 1e-8) amplifies FP32 rounding over a 48-substep frame
+    # to ~1.6e-4 m between the exact FP32 restatement and FP64, so the frame
+    # gate is the twin's own distance from FP64 (one substep of the bag meets
+    # the reference's 2e-5 m gate: test_gpu_bag.py).
+    twin = oracle.run(scene, configs, 32, 3, precision="fp32")
     fp64 = oracle.run(scene, configs, 32, 3)
     x = scene.particles["positionAndInverseMass"][:, :3]
     extent = float(np.max(x.max(axis=0) - x.min(axis=0)))
-    error = float(np.max(np.abs(_f64(end) - fp64["positions"])))
-    print(f"patch re-grab frame: |gpu - fp64| = {error:.3e} m (1e-5 extent = {1e-5 * extent:.3e})")
-    assert error <= 1e-5 * extent
+    e_twin = float(np.max(np.abs(_f64(end) - twin["positions"])))
+    e_fp64 = float(np.max(np.abs(_f64(end) - fp64["positions"])))
+    e_ref = float(np.max(np.abs(twin["positions"] - fp64["positions"])))
+    print(f"patch re-grab frame: |gpu-twin| {e_twin:.3e} |gpu-fp64| {e_fp64:.3e} |twin-fp64| {e_ref:.3e} m")
+    assert e_twin <= 1e-6 * extent
+    assert e_fp64 <= max(1e-5 * extent, 1.5 * e_ref)
 
 
 def test_patch_regrab_replay_matches_the_reference_metrics(cloth, golden):
