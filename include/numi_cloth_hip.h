@@ -201,6 +201,26 @@ int numi_cloth_hip_step(NumiClothHipSolver* solver,
                         uint32_t iterations,
                         uint32_t strainSweeps);
 
+/* step() for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). No counterpart in
+ * the reference. Same results as numi_cloth_hip_step, with no hipMalloc,
+ * hipFree or allocating library call: each call's configs and substep times
+ * are staged in a fixed device array of the library (about 232 KB, held by
+ * every device the library runs on until the process ends) and the substeps
+ * are launched directly, not replayed from a captured graph. Limits: at most
+ * 1024 substeps per call (more returns NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT),
+ * and the strain-limit program for strainSweeps must exist (the solver
+ * prepares 3 sweeps at create; one numi_cloth_hip_step call prepares any
+ * other count), else NUMI_CLOTH_HIP_ERROR_UNSUPPORTED. Returns when the
+ * configs are copied; the substeps run asynchronously on the solver's
+ * stream. Calls on one device share the staging array: calls that overlap
+ * on one device (several solvers stepped from several threads) race. */
+int numi_cloth_hip_step_noalloc(NumiClothHipSolver* solver,
+                                const NumiClothBagGPUConfig* configs,
+                                uint32_t substeps,
+                                uint32_t iterations,
+                                uint32_t strainSweeps);
+
 /* Device->host readback of runGPU (reference :4720-4757); synchronous. */
 int numi_cloth_hip_read(NumiClothHipSolver* solver,
                         const NumiClothHipState* out);

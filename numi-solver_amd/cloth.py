@@ -249,6 +249,7 @@ def library() -> C.CDLL:
         "numi_cloth_hip_last_error": ([], C.c_char_p),
         "numi_cloth_hip_create": ([vp, C.POINTER(vp)], i32),
         "numi_cloth_hip_step": ([vp, vp, u32, u32, u32], i32),
+        "numi_cloth_hip_step_noalloc": ([vp, vp, u32, u32, u32], i32),
         "numi_cloth_hip_read": ([vp, C.POINTER(State)], i32),
         "numi_cloth_hip_diagnostics": ([vp, C.POINTER(Diagnostics)], i32),
         "numi_cloth_hip_synchronize": ([vp], i32),
@@ -721,6 +722,13 @@ class Solver:
         configs = np.ascontiguousarray(configs, dtype=CONFIG)
         _check(library().numi_cloth_hip_step(self._handle, configs.ctypes.data, len(configs), iterations,
                                              strain_sweeps))
+
+    def step_noalloc(self, configs: np.ndarray, iterations: int = 32, strain_sweeps: int = 3) -> None:
+        """step() through numi_cloth_hip_step_noalloc: no device allocation,
+        at most 1024 substeps per call."""
+        configs = np.ascontiguousarray(configs, dtype=CONFIG)
+        _check(library().numi_cloth_hip_step_noalloc(self._handle, configs.ctypes.data, len(configs), iterations,
+                                                     strain_sweeps))
 
     def synchronize(self) -> None:
         _check(library().numi_cloth_hip_synchronize(self._handle))

@@ -14,9 +14,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -657,7 +659,10 @@ struct NumiClothHipSolver {
 
     // The substep in the reference's dispatch order
     // (tools/cloth_bag_metal.mm:4211-4709), families without records omitted.
-    void recordSubstep(uint32_t iterations, uint32_t sweeps) {
+    // substepConfigs / substepTimes: where k_prepare reads each substep's
+    // config and end time (nullptr: the solver's own staging).
+    void recordSubstep(uint32_t iterations, uint32_t sweeps, const NumiClothBagGPUConfig* substepConfigs = nullptr,
+                       const double* substepTimes = nullptr) {
         selfBuffer = 0;
         pendingSelfApply = -1;
         dirtySelf = -1;
@@ -666,7 +671,8 @@ struct NumiClothHipSolver {
         dLamCur = dLambda.ptr;
         bLamCur = bLambda.ptr;
         kLamCur = kLambda.ptr;
-        k_prepare<<<1, 64, 0, stream>>>(configs.ptr, times.ptr, slot.ptr, failure.ptr);
+        k_prepare<<<1, 64, 0, stream>>>(substepConfigs != nullptr ? substepConfigs : configs.ptr,
+                                        substepTimes != nullptr ? substepTimes : times.ptr, slot.ptr, failure.ptr);
         const float damping = extension.damping;
         // The damping factor depends on dt; dt is constant over a scene's
         // trajectories (the reference never changes it), so it is folded here.
@@ -1173,6 +1179,9 @@ void createSolver(const NumiClothHipSceneDesc& scene, NumiClothHipSolver& s) {
     bool tiles = true;
     if (const char* env = std::getenv("NUMI_CLOTH_HIP_TILES")) tiles = std::strcmp(env, "0") != 0;
     if (tiles) setupTiles(s);
+    // The strain-limit program of the reference's default 3 sweeps, so that
+    // numi_cloth_hip_step_noalloc can run them without building one.
+    s.ensureStrainProgram(3);
     s.diagResult.allocate(1);
     s.ensureStaging(64);
     HIP_OK(hipStreamSynchronize(st));
@@ -1598,6 +1607,100 @@ void stepSolver(NumiClothHipSolver& s, const NumiClothBagGPUConfig* configs, uin
         for (uint32_t k = 0; k < substeps; ++k) s.recordSubstep(iterations, sweeps);
         HIP_OK(hipGetLastError());
     }
+}
+
+// ------------------------------------------------- no-allocation step ----
+// numi_cloth_hip_step_noalloc stages each call's substep configs and end
+// times in `scratch`, device memory the library never allocates: a
+// file-scope array that every device the library runs on holds until the
+// process ends. Every call on a device shares that device's copy, so calls
+// that overlap on one device race.
+constexpr uint32_t kNoAllocSubsteps = 1024;
+constexpr size_t kNoAllocConfigBytes = kNoAllocSubsteps * sizeof(NumiClothBagGPUConfig);
+constexpr size_t kNoAllocScratchBytes = kNoAllocConfigBytes + kNoAllocSubsteps * sizeof(double);
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+// The device address of `scratch` on `device` (current), looked up on the
+// first call there and kept: in host code the array's name is not a device
+// address.
+unsigned char* scratchAddress(int device) {
+    static std::mutex lock;
+    static std::array<unsigned char*, 64> addresses{};
+    if (device < 0 || device >= static_cast<int>(addresses.size())) {
+        fail(NUMI_CLOTH_HIP_ERROR_UNSUPPORTED, "numi_cloth_hip_step_noalloc supports device indices 0-63");
+    }
+    std::lock_guard<std::mutex> hold(lock);
+    if (addresses[device] == nullptr) {
+        void* address = nullptr;
+        const hipError_t status = hipGetSymbolAddress(&address, HIP_SYMBOL(scratch));
+        if (status != hipSuccess || address == nullptr) {
+            fail(NUMI_CLOTH_HIP_ERROR_DEVICE,
+                 std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(status));
+        }
+        addresses[device] = static_cast<unsigned char*>(address);
+    }
+    return addresses[device];
+}
+
+// Makes `device` current for a scope and restores the caller's device.
+struct CurrentDevice {
+    int previous = 0;
+    bool changed = false;
+    explicit CurrentDevice(int device) {
+        HIP_OK(hipGetDevice(&previous));
+        if (previous != device) {
+            HIP_OK(hipSetDevice(device));
+            changed = true;
+        }
+    }
+    ~CurrentDevice() {
+        if (changed) (void)hipSetDevice(previous);
+    }
+};
+
+// step() without any allocation: at most kNoAllocSubsteps substeps per
+// call, staged in `scratch`, launched directly (no graph capture, which
+// allocates); the strain-limit program for `sweeps` must already exist (the
+// solver prepares the reference's 3 sweeps at create, step() any other).
+void stepSolverNoAlloc(NumiClothHipSolver& s, const NumiClothBagGPUConfig* configs, uint32_t substeps,
+                       uint32_t iterations, uint32_t sweeps) {
+    if (substeps == 0) return;
+    if (configs == nullptr) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "null configs");
+    if (substeps > kNoAllocSubsteps) {
+        fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT,
+             "numi_cloth_hip_step_noalloc takes at most " + std::to_string(kNoAllocSubsteps) +
+                 " substeps per call, got " + std::to_string(substeps));
+    }
+    if (s.useTiles && sweeps != s.tileStrainSweeps) {
+        fail(NUMI_CLOTH_HIP_ERROR_UNSUPPORTED,
+             "numi_cloth_hip_step_noalloc: the solver's strain-limit program is prepared for " +
+                 std::to_string(s.tileStrainSweeps) + " sweeps, not " + std::to_string(sweeps) +
+                 "; one numi_cloth_hip_step call with these sweeps prepares it");
+    }
+    for (uint32_t k = 0; k < substeps; ++k) checkTrajectoryConfig(s, configs[k]);
+    CurrentDevice current(s.device);
+    unsigned char* base = scratchAddress(s.device);
+    auto* stagedConfigs = reinterpret_cast<NumiClothBagGPUConfig*>(base);
+    auto* stagedTimes = reinterpret_cast<double*>(base + kNoAllocConfigBytes);
+    std::array<double, kNoAllocSubsteps> times;
+    double t = s.time;
+    for (uint32_t k = 0; k < substeps; ++k) {
+        t += configs[k].gravityAndTimestep.w;
+        times[k] = t;
+    }
+    HIP_OK(hipMemcpyAsync(stagedConfigs, configs, substeps * sizeof(NumiClothBagGPUConfig), hipMemcpyHostToDevice,
+                          s.stream));
+    HIP_OK(hipMemcpyAsync(stagedTimes, times.data(), substeps * sizeof(double), hipMemcpyHostToDevice, s.stream));
+    // The host arrays are the caller's and this frame's: the copies finish
+    // before returning.
+    HIP_OK(hipStreamSynchronize(s.stream));
+    s.time = t;
+    k_reset_cursor<<<1, 64, 0, s.stream>>>(s.slot.ptr);
+    k_fill_u32<<<1, 64, 0, s.stream>>>(reinterpret_cast<uint32_t*>(s.frictionStatus.ptr), 8, 0u);
+    k_fill_u32<<<1, 64, 0, s.stream>>>(reinterpret_cast<uint32_t*>(s.selfStatus.ptr), 4, 0u);
+    k_fill_u32<<<1, 64, 0, s.stream>>>(reinterpret_cast<uint32_t*>(s.aeroStatus.ptr), 4, 0u);
+    for (uint32_t k = 0; k < substeps; ++k) s.recordSubstep(iterations, sweeps, stagedConfigs, stagedTimes);
+    HIP_OK(hipGetLastError());
 }
 
 void readSolver(NumiClothHipSolver& s, const NumiClothHipState& out) {
@@ -2369,6 +2472,14 @@ int numi_cloth_hip_step(NumiClothHipSolver* solver, const NumiClothBagGPUConfig*
     return guarded([&] {
         if (solver == nullptr) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "null solver");
         stepSolver(*solver, configs, substeps, iterations, strainSweeps);
+    });
+}
+
+int numi_cloth_hip_step_noalloc(NumiClothHipSolver* solver, const NumiClothBagGPUConfig* configs,
+                                uint32_t substeps, uint32_t iterations, uint32_t strainSweeps) {
+    return guarded([&] {
+        if (solver == nullptr) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "null solver");
+        stepSolverNoAlloc(*solver, configs, substeps, iterations, strainSweeps);
     });
 }
 
