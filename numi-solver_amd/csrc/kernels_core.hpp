@@ -684,31 +684,43 @@ __global__ void k_collide(const NumiClothBagGPUConfig* __restrict__ cfg, const u
 
 // finalize_substep (reference src/metal/ClothBag.metal:2890-2921) fused with
 // apply_cloth_ground_friction (:2976-3022); nothing runs between them on the
-// cloth knots. Friction contact counts and the maximum cone ratio are reduced
-// per wavefront before one integer atomic per wave.
+// cloth knots. A grid-stride loop over a bounded grid: the friction contact
+// count and the largest cone ratio are reduced per wavefront, then per
+// workgroup in LDS, and each workgroup adds one integer atomic pair (a
+// draped sheet has most knots in ground contact; one atomic pair per wave on
+// the same two words serialised ~16k waves).
+constexpr uint32_t kFinalizeMaxBlocks = 1024;
+
 __global__ void k_finalize(const NumiClothBagGPUConfig* __restrict__ cfg, const uint32_t* __restrict__ ok,
                            float4* __restrict__ pos, const float4* __restrict__ prev, float4* __restrict__ vel,
                            uint32_t particles, NumiClothBagGPUFrictionStatus* __restrict__ friction,
                            uint32_t* __restrict__ failure) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ uint32_t blockContacts;
+    __shared__ uint32_t blockRatio;
+    if (threadIdx.x == 0) {
+        blockContacts = 0u;
+        blockRatio = 0u;
+    }
+    __syncthreads();
     if (*ok == 0u) return;
-    bool contact = false;
+    const float inverseDt = 1.0f / cfg->gravityAndTimestep.w;
+    const bool ground = cfg->constraintCounts.z != 0u;
+    const float mu = cfg->clothMaterial.z;
+    const float contactHeight = cfg->clothMaterial.x + 1.0e-6f;
+    uint32_t contacts = 0u;
     float ratio = 0.0f;
-    if (i < particles) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < particles; i += gridDim.x * blockDim.x) {
         const float4 p = pos[i];
-        float4 v = vel[i];
+        const float4 v = vel[i];
         const float4 q = prev[i];
-        const float inverseDt = 1.0f / cfg->gravityAndTimestep.w;
         const float predicted = v.w;
         V3 u = (xyz(p) - xyz(q)) * inverseDt;
         float jn = 0.0f;
-        const bool ground = cfg->constraintCounts.z != 0u;
-        if (ground && p.w > 0.0f && p.z <= cfg->clothMaterial.x + 1.0e-6f) {
+        if (ground && p.w > 0.0f && p.z <= contactHeight) {
             jn = fmaxf(0.0f, (u.z - predicted) / p.w);
         }
         bool good = finite3(u);
         if (good && ground) {
-            const float mu = cfg->clothMaterial.z;
             if (!(mu >= 0.0f) || !isfinite(mu)) {
                 good = false;
             } else if (jn > 0.0f && p.w > 0.0f) {
@@ -719,8 +731,8 @@ __global__ void k_finalize(const NumiClothBagGPUConfig* __restrict__ cfg, const 
                     const float jt = fminf(slip / p.w, limit);
                     if (jt > 0.0f) {
                         u = u - tangent * (jt * p.w / slip);
-                        contact = true;
-                        ratio = limit > 0.0f ? jt / limit : 0.0f;
+                        contacts += 1u;
+                        ratio = fmaxf(ratio, limit > 0.0f ? jt / limit : 0.0f);
                         good = finite3(u) && isfinite(jn);
                     }
                 }
@@ -732,12 +744,18 @@ __global__ void k_finalize(const NumiClothBagGPUConfig* __restrict__ cfg, const 
             vel[i] = make_float4(u.x, u.y, u.z, jn);
         }
     }
-    const unsigned long long ballot = __ballot(contact);
-    float waveMax = ratio;
-    for (int offset = 32; offset > 0; offset >>= 1) waveMax = fmaxf(waveMax, __shfl_xor(waveMax, offset));
-    if ((threadIdx.x & 63u) == 0u && ballot != 0ull) {
-        atomicAdd(&friction->counters.z, static_cast<uint32_t>(__popcll(ballot)));
-        atomicMax(&friction->metrics.x, __float_as_uint(waveMax));
+    for (int offset = 32; offset > 0; offset >>= 1) {
+        contacts += __shfl_xor(contacts, offset);
+        ratio = fmaxf(ratio, __shfl_xor(ratio, offset));
+    }
+    if ((threadIdx.x & 63u) == 0u && contacts != 0u) {
+        atomicAdd(&blockContacts, contacts);
+        atomicMax(&blockRatio, __float_as_uint(ratio));
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && blockContacts != 0u) {
+        atomicAdd(&friction->counters.z, blockContacts);
+        atomicMax(&friction->metrics.x, blockRatio);
     }
 }
 

@@ -772,8 +772,8 @@ struct NumiClothHipSolver {
         }
         restorePrimaryBuffers();
         launchBuildYarnContacts();
-        k_finalize<<<blocksFor(P), kBlock, 0, stream>>>(cfg(), ok(), pos.ptr, prev.ptr, vel.ptr, P,
-                                                        frictionStatus.ptr, failure.ptr);
+        k_finalize<<<std::min(blocksFor(P), kFinalizeMaxBlocks), kBlock, 0, stream>>>(
+            cfg(), ok(), pos.ptr, prev.ptr, vel.ptr, P, frictionStatus.ptr, failure.ptr);
         launchSelfFriction();
         if (F != 0) {
             k_fruit_finalize<<<blocksFor(F), kBlock, 0, stream>>>(cfg(), ok(), fruits.ptr, F, failure.ptr);

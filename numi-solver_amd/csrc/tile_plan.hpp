@@ -105,6 +105,68 @@ inline uint32_t participantsOf(const TileTopology& t, uint32_t kind, uint32_t c,
     return 4;
 }
 
+// Drops the items of one tile whose results reach neither an owned knot nor
+// an owned multiplier. Walking the passes backwards from the end, where the
+// owned knots are needed: an item of pass k is kept when it touches a knot
+// needed after pass k, and then all its participants are needed before
+// pass k (within a colour no two items share a knot, so the order inside a
+// pass does not matter). Every kept item reads exactly the values it read
+// in the full program, so the owned knots and multipliers are the same
+// bytes; only halo work whose results were discarded goes. Owner items touch
+// their owned first participant, so they are always kept.
+inline void pruneUnneededItems(const std::vector<TilePass>& program, uint32_t owned, uint32_t loaded,
+                               std::vector<uint32_t>& items, std::vector<uint32_t>& passItemStart) {
+    const uint32_t passes = static_cast<uint32_t>(program.size());
+    const uint32_t total = static_cast<uint32_t>(items.size() / 4);
+    std::vector<uint8_t> needed(loaded, 0), keep(total, 0);
+    for (uint32_t k = 0; k < owned; ++k) needed[k] = 1;
+    for (uint32_t k = passes; k-- > 0;) {
+        const uint32_t first = passItemStart[k];
+        const uint32_t last = k + 1 < passes ? passItemStart[k + 1] : total;
+        const bool quad = program[k].kind == kTileKnot;
+        for (uint32_t j = first; j < last; ++j) {
+            const uint32_t* it = &items[4 * j];
+            const uint32_t l[4] = {it[0] & 0xffffu, it[0] >> 16, quad ? (it[1] & 0xffffu) : 0u, quad ? it[1] >> 16 : 0u};
+            const uint32_t n = quad ? 4u : 2u;
+            bool any = false;
+            for (uint32_t m = 0; m < n; ++m) any = any || needed[l[m]] != 0;
+            if (!any) continue;
+            keep[j] = 1;
+        }
+        for (uint32_t j = first; j < last; ++j) {
+            if (!keep[j]) continue;
+            const uint32_t* it = &items[4 * j];
+            needed[it[0] & 0xffffu] = 1;
+            needed[it[0] >> 16] = 1;
+            if (quad) {
+                needed[it[1] & 0xffffu] = 1;
+                needed[it[1] >> 16] = 1;
+            }
+        }
+    }
+    // Within a pass (one colour: no two items share a knot, so their order
+    // does not change any result) the kept items go in ascending LDS slot of
+    // their first knot, so the lanes of a wave read neighbouring LDS slots
+    // rather than the scattered slots of constraint order.
+    std::vector<uint32_t> kept;
+    kept.reserve(items.size());
+    std::vector<uint32_t> starts(passes), order;
+    for (uint32_t k = 0; k < passes; ++k) {
+        starts[k] = static_cast<uint32_t>(kept.size() / 4);
+        const uint32_t first = passItemStart[k];
+        const uint32_t last = k + 1 < passes ? passItemStart[k + 1] : total;
+        order.clear();
+        for (uint32_t j = first; j < last; ++j)
+            if (keep[j]) order.push_back(j);
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+            return (items[4 * x] & 0xffffu) < (items[4 * y] & 0xffffu);
+        });
+        for (uint32_t j : order) kept.insert(kept.end(), items.begin() + 4 * j, items.begin() + 4 * j + 4);
+    }
+    items.swap(kept);
+    passItemStart.swap(starts);
+}
+
 // Builds a plan for `program`; returns false when some tile would need more
 // than `capacity` loaded knots or more than `maxHalo` hops of halo.
 inline bool planTiles(const TileTopology& topo, const std::vector<TilePass>& program, uint32_t owned,
@@ -319,6 +381,7 @@ inline bool planTiles(const TileTopology& topo, const std::vector<TilePass>& pro
                 continue;
             }
             while (passItemStart.size() < passes) passItemStart.push_back(static_cast<uint32_t>(tileItems.size() / 4));
+            pruneUnneededItems(program, end - begin, static_cast<uint32_t>(loaded.size()), tileItems, passItemStart);
             const uint32_t base = static_cast<uint32_t>(plan->items.size() / 4);
             for (uint32_t k = 1; k < passes; ++k) plan->itemStart.push_back(base + passItemStart[k]);
             plan->items.insert(plan->items.end(), tileItems.begin(), tileItems.end());
