@@ -391,21 +391,35 @@ def run_pickup(args, cloth, hip) -> dict:
     c = cloth.cloth
     scene = cloth.Scene.bag_replay(c.REPLAY_PICKUP, selfContact=1)
     particles = len(scene.particles)
+
+    def replay(solver, frames, report):
+        """The reference's frame loop: a frame's 48 substep configs, the step,
+        the state read back and FNV-hashed (the library's frame driver does
+        the same in numi_cloth_replay_run; this loop reports its progress)."""
+        hashes = []
+        for f in range(frames):
+            solver.step(c.replay_frame_configs(scene.config, c.REPLAY_PICKUP, f), ITERATIONS, STRAIN_SWEEPS)
+            state = solver.read()
+            if state["failure"] != 0:
+                raise RuntimeError(f"pickup replay raised failure bits {state['failure']} in frame {f + 1}")
+            hashes.append(c.frame_hash(scene, state))
+            if report and ((f + 1) % 24 == 0 or f + 1 == frames):
+                print(f"pickup frame {f + 1}/{frames}: released mask {int(state['releaseStatus']['masks'][0, 1])}",
+                      file=sys.stderr, flush=True)
+        return hashes
+
     solver = cloth.Solver(scene)
     if args.warmup:
-        solver.replay(c.REPLAY_PICKUP, args.warmup, ITERATIONS, STRAIN_SWEEPS)
-    # The timed frames continue the replay's clock from a fresh solver so the
-    # trajectory is the reference's from t = 0.
+        replay(solver, args.warmup, False)
+    # The timed frames start the replay again from t = 0 on a fresh solver.
     solver.close()
     solver = cloth.Solver(scene)
     solver.synchronize()
     assert hip.hipDeviceSynchronize() == 0
     timer = Timer(hip, solver.stream)
     timer.begin()
-    result = solver.replay(c.REPLAY_PICKUP, args.steps, ITERATIONS, STRAIN_SWEEPS)
+    hashes = replay(solver, args.steps, True)
     seconds = timer.end()
-    if not result["failureFree"]:
-        raise RuntimeError("pickup replay raised a failure bit")
     state = solver.read()
     d = solver.diagnostics()
     # The reference's pickup qualification (tools/cloth_bag_metal.mm:6214-6240):
@@ -431,7 +445,7 @@ def run_pickup(args, cloth, hip) -> dict:
     config = {"workload": "pickup", "particles": particles, "substeps_per_frame": 48, "iterations": ITERATIONS,
               "strain_sweeps": STRAIN_SWEEPS, "frames": args.steps, "self_contact": True,
               "frame_seconds": 48 * c.REPLAY_TIMESTEP, "wall_seconds_timed": round(timer.wall, 3),
-              "frame_hashes": [hex(h) for h in result["hashes"][-3:]], "maximum_handle_lag_m": result["maximumHandleLag"],
+              "last_frame_hashes": [hex(h) for h in hashes[-3:]],
               "qualification_at_last_frame": qualification, "tile_programs": tile_config,
               "reference": "Metal pickup replay, 480 frames x 48 substeps, 7086.04 s GPU command time on an Apple "
                            "M4 Pro -> %.1f particle-substeps/s" % PICKUP_REFERENCE}
@@ -503,15 +517,18 @@ def run_adjoint(args, cloth, hip) -> dict:
     specimens, local, targets, configs = adjoint_problem(cloth)
     substeps = len(configs)
     particles = sum(len(s.particles) for s in specimens)
-    # Each sample runs the workload's 200-substep window from the seeded
-    # state on a fresh batch (a call continues the batch's trajectory, and
-    # later windows of a longer run, where strain limits act, tape more
-    # passes). Plain forward: the library's step path, timed right after a
-    # 16-substep step that captures its graph.
+    start = args.adjoint_start
+    lead = specimens[0].configs(max(start, 16))
+    # Each sample runs a 200-substep window on a fresh batch: substeps
+    # start..start+200 of the strips' trajectory from the seeded state (a call
+    # continues the batch's trajectory; later windows, where the strips swing
+    # far enough for strain limits to act, tape more passes). Plain forward:
+    # the library's step path over the same window, after the untimed
+    # substeps before it (at least 16, which capture its graph).
     forward_times, line, tile_config = [], None, None
     for k in range(args.steps):
         forward = cloth.Batch(specimens)
-        forward.step(configs[:16], ITERATIONS, STRAIN_SWEEPS)
+        forward.step(lead, ITERATIONS, STRAIN_SWEEPS)
         forward.synchronize()
         timer = Timer(hip, cloth.cloth.library().numi_cloth_hip_stream(forward.solver_handle))
         timer.begin()
@@ -533,6 +550,8 @@ def run_adjoint(args, cloth, hip) -> dict:
     adjoint_times = []
     for k in range(args.steps):
         batch = cloth.Batch(specimens)
+        if start:
+            batch.step(specimens[0].configs(start), ITERATIONS, STRAIN_SWEEPS)
         batch.synchronize()
         timer = Timer(hip, cloth.cloth.library().numi_cloth_hip_stream(batch.solver_handle))
         timer.begin()
@@ -550,8 +569,9 @@ def run_adjoint(args, cloth, hip) -> dict:
               "checkpoint_interval": 16, "tracked_per_specimen": int(local.shape[1]), "iterations": ITERATIONS,
               "strain_sweeps": STRAIN_SWEEPS, "forward_seconds": forward_seconds,
               "adjoint_seconds": adjoint_seconds, "adjoint_over_forward": adjoint_seconds / forward_seconds,
-              "window": "substeps 0-200 from the seeded state (forward: 16-216, after its graph-capture step), "
-                        "a fresh batch per sample",
+              "window": f"substeps {start}-{start + substeps} of the trajectory from the seeded state, a fresh "
+                        f"batch per sample (forward: substeps {max(start, 16)}-{max(start, 16) + substeps}, after "
+                        "untimed steps that also capture its graph)",
               "gradient_entries_per_specimen": int(grads["distanceCompliance"].size + grads["bendCompliance"].size
                                                    + grads["inverseMass"].size + grads["damping"].size),
               "loss_specimen0": grads["loss"], "tile_programs": tile_config}
@@ -570,6 +590,8 @@ def main() -> int:
     parser.add_argument("--cpu-substeps", type=int, default=None,
                         help="substeps of the CPU FP64 baseline sample (default: sized to ~15 s)")
     parser.add_argument("--no-cpu-baseline", action="store_true")
+    parser.add_argument("--adjoint-start", type=int, default=0, metavar="SUBSTEPS",
+                        help="adjoint64: substeps stepped (untimed) before the timed 200-substep window")
     parser.add_argument("--cell-size", type=float, default=None,
                         help="self-contact hash cell of the sheet workloads in metres (default: the workload's)")
     parser.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
