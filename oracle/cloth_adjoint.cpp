@@ -316,6 +316,8 @@ typedef struct NumiClothOracleGradients {
     double* inverseMass;           /* P: dL/dw */
     double* damping;               /* P: dL/dc_damping */
     double* positions;             /* P x 3 after the run, or NULL */
+    double* trajectory;            /* substeps x trackedCount x 3 tracked positions, or NULL */
+    double* trajectoryLow;         /* the same, position - trajectory (nonzero only in extended precision) */
 } NumiClothOracleGradients;
 
 }  // extern "C"
@@ -354,6 +356,14 @@ int runAdjoint(const NumiClothHipSceneDesc* scene, const NumiClothBagGPUConfig* 
             const double* target = pr.targets + 3 * (size_t(s) * T + t);
             const Vec w{R(target[0]), R(target[1]), R(target[2])};
             const Vec& x = m.x[pr.tracked[t]];
+            if (out->trajectory != nullptr) {
+                double* row = out->trajectory + 3 * (size_t(s) * T + t);
+                row[0] = double(x.x), row[1] = double(x.y), row[2] = double(x.z);
+                if (out->trajectoryLow != nullptr) {
+                    double* low = out->trajectoryLow + 3 * (size_t(s) * T + t);
+                    low[0] = double(x.x - R(row[0])), low[1] = double(x.y - R(row[1])), low[2] = double(x.z - R(row[2]));
+                }
+            }
             if (linear) {
                 loss += double(x.x) * double(w.x) + double(x.y) * double(w.y) + double(x.z) * double(w.z);
                 if (gradients) lossGrad[s][t] = w;
@@ -444,6 +454,15 @@ int numi_cloth_oracle_adjoint(const NumiClothHipSceneDesc* scene, const NumiClot
                               uint32_t substeps, uint32_t iterations, uint32_t sweeps,
                               const NumiClothOracleAdjointProblem* problem, NumiClothOracleGradients* out) {
     return runAdjoint<double>(scene, configs, substeps, iterations, sweeps, problem, out);
+}
+
+// The same algorithm in x87 extended precision (64-bit significand): finite
+// differences of its loss resolve parameters whose effect on the loss lies
+// below the FP64 trajectory's rounding noise at small steps.
+int numi_cloth_oracle_adjoint_extended(const NumiClothHipSceneDesc* scene, const NumiClothBagGPUConfig* configs,
+                                       uint32_t substeps, uint32_t iterations, uint32_t sweeps,
+                                       const NumiClothOracleAdjointProblem* problem, NumiClothOracleGradients* out) {
+    return runAdjoint<long double>(scene, configs, substeps, iterations, sweeps, problem, out);
 }
 
 // The FP32 twin (parameter overrides must be NULL or exact in FP32).

@@ -118,7 +118,8 @@ class AdjointProblem(C.Structure):
 
 class Gradients(C.Structure):
     _fields_ = [("loss", C.c_double), ("distanceCompliance", C.c_void_p), ("bendCompliance", C.c_void_p),
-                ("inverseMass", C.c_void_p), ("damping", C.c_void_p), ("positions", C.c_void_p)]
+                ("inverseMass", C.c_void_p), ("damping", C.c_void_p), ("positions", C.c_void_p),
+                ("trajectory", C.c_void_p), ("trajectoryLow", C.c_void_p)]
 
 
 def _array_pointer(array):
@@ -132,16 +133,24 @@ def adjoint(scene, configs: np.ndarray, iterations: int, strain_sweeps: int, tra
     (linear=True: the observable sum_s sum_k target_k(s) . x_k(s)) and,
     with gradients=True, its gradient with respect to distance and bend
     compliance, inverse mass and damping coefficient (per record / knot).
-    The *_scale / damping arrays override parameters in double."""
+    The *_scale / damping arrays override parameters in double. The result
+    also holds "trajectory": the tracked knots' positions after every
+    substep (substeps x tracked x 3), from which differences of two losses
+    can be formed term by term, and "trajectoryLow", the part of each
+    position below FP64 (precision="extended": the same algorithm with x87
+    80-bit arithmetic; zero in the other precisions)."""
     lib = library()
     if not hasattr(lib, "_adjoint_ready"):
-        for name in ("numi_cloth_oracle_adjoint", "numi_cloth_oracle_adjoint_fp32"):
+        for name in ("numi_cloth_oracle_adjoint", "numi_cloth_oracle_adjoint_fp32",
+                     "numi_cloth_oracle_adjoint_extended"):
             fn = getattr(lib, name)
             fn.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(AdjointProblem),
                            C.POINTER(Gradients)]
             fn.restype = C.c_int
         lib._adjoint_ready = True
-    entry = lib.numi_cloth_oracle_adjoint if precision == "fp64" else lib.numi_cloth_oracle_adjoint_fp32
+    entries = {"fp64": lib.numi_cloth_oracle_adjoint, "fp32": lib.numi_cloth_oracle_adjoint_fp32,
+               "extended": lib.numi_cloth_oracle_adjoint_extended}
+    entry = entries[precision]
     configs = np.ascontiguousarray(configs)
     tracked = np.ascontiguousarray(tracked, np.uint32)
     targets = np.ascontiguousarray(targets, np.float64)
@@ -152,10 +161,12 @@ def adjoint(scene, configs: np.ndarray, iterations: int, strain_sweeps: int, tra
                              *[_array_pointer(a) for a in keep])
     P, D, B = len(scene.particles), len(scene.distances), len(scene.bends)
     out = {"distanceCompliance": np.zeros(D), "bendCompliance": np.zeros(B), "inverseMass": np.zeros(P),
-           "damping": np.zeros(P), "positions": np.zeros((P, 3))}
+           "damping": np.zeros(P), "positions": np.zeros((P, 3)),
+           "trajectory": np.zeros((len(configs), len(tracked), 3)),
+           "trajectoryLow": np.zeros((len(configs), len(tracked), 3))}
     g = Gradients(0.0, *(out[name].ctypes.data if gradients and out[name].size else None
                          for name in ("distanceCompliance", "bendCompliance", "inverseMass", "damping")),
-                  out["positions"].ctypes.data)
+                  out["positions"].ctypes.data, out["trajectory"].ctypes.data, out["trajectoryLow"].ctypes.data)
     status = entry(C.addressof(scene.desc), configs.ctypes.data, len(configs), iterations, strain_sweeps,
                    C.byref(problem), C.byref(g))
     if status != 0:
