@@ -501,16 +501,87 @@ def run_strips(args, cloth, hip) -> dict:
                 solver.step(own, ITERATIONS, STRAIN_SWEEPS)
             total += t.end()
     single = 2 * len(sample) / total
+    resident = resident_comparison(cloth, hip, specimens, configs, max(args.steps, 1))
+    split = split_job(cloth, specimens, configs, max(args.steps, 1))
     baseline = None if args.no_cpu_baseline else cpu_baseline_strips(specimens, 16)
     config = {"workload": "strips512", "specimens": len(specimens), "particles": particles,
               "substeps_per_frame": 48, "iterations": ITERATIONS, "strain_sweeps": STRAIN_SWEEPS,
               "frame_seconds": 1.0 / 60.0, "wall_seconds_timed": round(timer.wall, 3),
               "one_per_launch_specimen_frames_per_s": single, "speedup_vs_one_per_launch": batched / single,
               "one_per_launch_sample": "16 seeded specimens, 2 frames each, own solver each",
+              "resident_subset": resident, "split_job": split,
+              "split_over_merged": split["specimen_frames_per_s"] / batched,
               "particle_substeps_per_s": particles * 48 * args.steps / seconds, "tile_programs": tile_config}
     return {"metric": "batched specimen-frames/s", "value": batched, "unit": "specimen-frames/s",
             "ms_per_step": seconds * 1000.0 / args.steps, "dtype": "f32", "config": config, "roofline": line,
             "cpu_baseline": baseline, "data": "synthetic (seeded strips, PCG64 seed %d)" % SEED}
+
+
+def fits_one_workgroup(scene) -> bool:
+    """The specimen's state fits one workgroup's LDS (160 KiB on gfx950):
+    16 B per knot, 4 B per distance and bend multiplier, at most 5120 knots."""
+    lds = 16 * len(scene.particles) + 4 * (len(scene.distances) + len(scene.bends)) + 1024
+    return len(scene.particles) <= 5120 and lds <= 160 * 1024
+
+
+def resident_comparison(cloth, hip, specimens, configs, frames: int) -> dict:
+    """The merged tile path (Batch.step) against one persistent launch per
+    call (Batch.step_resident: a workgroup per specimen, its state resident
+    in LDS for every substep), each on a fresh batch of the specimens that
+    fit one workgroup, one untimed frame then `frames` timed frames; both
+    must end in the same bytes."""
+    subset = [s for s in specimens if fits_one_workgroup(s)]
+    rates, ends = {}, {}
+    for mode in ("tiles", "resident"):
+        with cloth.Batch(subset) as batch:
+            step = batch.step if mode == "tiles" else batch.step_resident
+            step(configs, ITERATIONS, STRAIN_SWEEPS)
+            batch.synchronize()
+            stream = cloth.cloth.library().numi_cloth_hip_stream(batch.solver_handle)
+            timer = Timer(hip, stream)
+            timer.begin()
+            for _ in range(frames):
+                step(configs, ITERATIONS, STRAIN_SWEEPS)
+            seconds = timer.end()
+            batch.synchronize()
+            ends[mode] = [batch.read(k)["particles"].tobytes() for k in range(len(subset))]
+            rates[mode] = len(subset) * frames / seconds
+            print(f"strips512 {mode}: {rates[mode]:.1f} specimen-frames/s over {len(subset)} specimens",
+                  file=sys.stderr, flush=True)
+    return {"specimens": len(subset), "particles": sum(len(s.particles) for s in subset),
+            "largest": max(len(s.particles) for s in subset), "frames": frames,
+            "merged_tiles_specimen_frames_per_s": rates["tiles"],
+            "resident_specimen_frames_per_s": rates["resident"],
+            "resident_over_tiles": rates["resident"] / rates["tiles"],
+            "equal_bytes": ends["tiles"] == ends["resident"]}
+
+
+def split_job(cloth, specimens, configs, frames: int) -> dict:
+    """All specimens as two batches stepped side by side on their own
+    streams: those that fit one workgroup by the persistent resident launch,
+    the rest by the merged tile path; whole-job wall time from the host clock
+    (both streams drained before and after)."""
+    fit = [s for s in specimens if fits_one_workgroup(s)]
+    rest = [s for s in specimens if not fits_one_workgroup(s)]
+    with cloth.Batch(fit) as resident, cloth.Batch(rest) as merged:
+        resident.step_resident(configs, ITERATIONS, STRAIN_SWEEPS)
+        merged.step(configs, ITERATIONS, STRAIN_SWEEPS)
+        resident.synchronize()
+        merged.synchronize()
+        start = time.perf_counter()
+        for _ in range(frames):
+            resident.step_resident(configs, ITERATIONS, STRAIN_SWEEPS)
+            merged.step(configs, ITERATIONS, STRAIN_SWEEPS)
+        resident.synchronize()
+        merged.synchronize()
+        seconds = time.perf_counter() - start
+        if resident.read(0)["failure"] != 0 or merged.read(0)["failure"] != 0:
+            raise RuntimeError("device failure in the split job")
+    rate = len(specimens) * frames / seconds
+    print(f"strips512 split: {rate:.1f} specimen-frames/s ({len(fit)} resident, {len(rest)} merged)",
+          file=sys.stderr, flush=True)
+    return {"resident_specimens": len(fit), "merged_specimens": len(rest), "frames": frames,
+            "specimen_frames_per_s": rate, "ms_per_frame": seconds * 1000.0 / frames}
 
 
 def run_adjoint(args, cloth, hip) -> dict:

@@ -312,6 +312,23 @@ int numi_cloth_hip_batch_step(NumiClothHipBatch* batch,
                               uint32_t substeps, uint32_t iterations,
                               uint32_t strainSweeps);
 
+/*
+ * The same substeps as numi_cloth_hip_batch_step in one persistent launch
+ * (no counterpart in the reference): one workgroup per specimen runs every
+ * substep of the call with the specimen's knots and multipliers resident in
+ * LDS, so HBM is touched only to load the state, read the constraint records
+ * and store the state at the end. The batch ends bit for bit where
+ * numi_cloth_hip_batch_step leaves it. For contact-free specimens
+ * (distances, bends, strain limits, drive, damping) of at most 5120 knots
+ * whose knots (16 B each) and multipliers (4 B per distance and bend) fit in
+ * one workgroup's LDS; other batches fail with
+ * NUMI_CLOTH_HIP_ERROR_UNSUPPORTED.
+ */
+int numi_cloth_hip_batch_step_resident(NumiClothHipBatch* batch,
+                                       const NumiClothBagGPUConfig* configs,
+                                       uint32_t substeps, uint32_t iterations,
+                                       uint32_t strainSweeps);
+
 /* Specimen `specimen`'s particles and constraint records in its own order and
  * indexing; failure and friction status are those of the whole batch. */
 int numi_cloth_hip_batch_read(NumiClothHipBatch* batch, uint32_t specimen,

@@ -272,6 +272,7 @@ def library() -> C.CDLL:
         "numi_cloth_scene_merge_indices": ([vp, u32, u32, C.POINTER(C.POINTER(u32)), C.POINTER(u32)], i32),
         "numi_cloth_hip_batch_create": ([C.POINTER(vp), u32, C.POINTER(vp)], i32),
         "numi_cloth_hip_batch_step": ([vp, vp, u32, u32, u32], i32),
+        "numi_cloth_hip_batch_step_resident": ([vp, vp, u32, u32, u32], i32),
         "numi_cloth_hip_batch_read": ([vp, u32, C.POINTER(State)], i32),
         "numi_cloth_hip_batch_solver": ([vp], vp),
         "numi_cloth_hip_batch_scene": ([vp], C.POINTER(SceneDesc)),
@@ -617,6 +618,14 @@ class Batch:
         configs = np.ascontiguousarray(configs, dtype=CONFIG)
         _check(library().numi_cloth_hip_batch_step(self._handle, configs.ctypes.data, len(configs), iterations,
                                                    strain_sweeps))
+
+    def step_resident(self, configs: np.ndarray, iterations: int = 32, strain_sweeps: int = 3) -> None:
+        """The same substeps in one persistent launch, one workgroup per
+        specimen with its state resident in LDS (contact-free specimens that
+        fit one workgroup's LDS); ends where step() ends, bit for bit."""
+        configs = np.ascontiguousarray(configs, dtype=CONFIG)
+        _check(library().numi_cloth_hip_batch_step_resident(self._handle, configs.ctypes.data, len(configs),
+                                                            iterations, strain_sweeps))
 
     def synchronize(self) -> None:
         _check(library().numi_cloth_hip_synchronize(self.solver_handle))

@@ -1523,11 +1523,22 @@ void setupTiles(NumiClothHipSolver& s) {
                               : (s.P >= 256u * 4096u ? 1024u : (s.P > 256u * 1024u ? 256u : 64u));
     for (uint32_t owned : {4096u, 2048u, 1024u, 512u, 256u, 128u, 64u})
         if (owned < s.P && owned >= smallest) sizes.push_back(owned);
+    // Pins of the autotuner's axes, so that separate profiler runs measure one
+    // configuration: NUMI_CLOTH_HIP_TILE_OWNED / _BLOCK name a candidate (the
+    // tiles are off when the pinned one cannot be built);
+    // NUMI_CLOTH_HIP_TILE_STAGED=0/1 overrides staging after the choice.
+    auto pin = [](const char* name) -> long {
+        const char* env = std::getenv(name);
+        return env != nullptr ? std::strtol(env, nullptr, 10) : -1;
+    };
+    const long pinOwned = pin("NUMI_CLOTH_HIP_TILE_OWNED"), pinBlock = pin("NUMI_CLOTH_HIP_TILE_BLOCK");
     for (uint32_t owned : sizes) {
+        if (pinOwned >= 0 && owned != static_cast<uint32_t>(pinOwned)) continue;
         NumiClothHipSolver::TileProgram candidate;
         if (!buildTileProgram(s, even, owned, candidate)) continue;
         for (uint32_t block : {256u, 512u, 1024u})
             for (bool staged : {false, true}) {
+                if (pinBlock >= 0 && block != static_cast<uint32_t>(pinBlock)) continue;
                 if (staged && !candidate.stageable) continue;
                 candidate.block = block;
                 candidate.staged = staged;
@@ -1967,7 +1978,11 @@ struct AdjointBuffers {
     DeviceArray<float> gradInverseMass, gradDamping, gradDistance, gradBend;
     DeviceArray<double> loss;
     std::vector<uint32_t> trackedFirst, trackedCount;
-    bool ready = false;
+    std::vector<SpecimenInfo> baseInfo;         // per specimen, no tracked knots
+    std::vector<NumiClothBagGPUConfig> hostConfigs;
+    std::vector<double> hostTimes;
+    bool specimensReady = false;                // layout for the specimen kernels
+    bool ready = false;                         // adjoint buffers
     ~AdjointBuffers() {
         for (int k = 0; k < 2; ++k) {
             if (recorded[k] != nullptr) (void)hipEventDestroy(recorded[k]);
@@ -2093,17 +2108,18 @@ void readBatch(NumiClothHipBatch& b, uint32_t specimen, const NumiClothHipState&
 // ---------------------------------------------------------------- adjoint --
 
 // Per-specimen colour ranges of the merged scene (colour c of specimen s is
-// one contiguous range of merged colour batch c) and the LDS footprint.
-void setupAdjoint(NumiClothHipBatch& b) {
+// one contiguous range of merged colour batch c) and the LDS footprint of the
+// specimen kernels (one workgroup per specimen, state resident in LDS).
+void setupSpecimens(NumiClothHipBatch& b) {
     AdjointBuffers& A = b.adjoint;
-    if (A.ready) return;
+    if (A.specimensReady) return;
     NumiClothHipSolver& s = *b.solver;
     const NumiClothHipSceneDesc& d = b.merged->desc;
     const NumiClothBagGPUConfig& c = *d.config;
     if (d.knotCount || d.gripCount || d.fruitCount || d.extension.staticSphereCount || d.extension.selfContact ||
         c.constraintCounts.z != 0u || c.airVelocityAndDensity.w > 0.0f) {
         fail(NUMI_CLOTH_HIP_ERROR_UNSUPPORTED,
-             "the adjoint covers distances, bends, strain limits, drive and damping (no contacts, grips, "
+             "the specimen kernels cover distances, bends, strain limits, drive and damping (no contacts, grips, "
              "crossing knots, fruits or air)");
     }
     const uint32_t S = b.specimens;
@@ -2115,7 +2131,7 @@ void setupAdjoint(NumiClothHipBatch& b) {
         info[k] = SpecimenInfo{count ? idx[0] : 0u, count, 0u, 0u, 0u, 0u, {0u, 0u}};
         for (uint32_t i = 0; i < count; ++i) owner[idx[i]] = k;
         if (count > kSpecimenOwn * kSpecimenThreads) {
-            fail(NUMI_CLOTH_HIP_ERROR_UNSUPPORTED, "adjoint specimens hold at most 5120 knots");
+            fail(NUMI_CLOTH_HIP_ERROR_UNSUPPORTED, "specimens hold at most 5120 knots");
         }
     }
     auto ranges = [&](const NumiClothBagGPUBatch* batches, uint32_t batchCount, auto endsOf, uint32_t* colours,
@@ -2156,22 +2172,36 @@ void setupAdjoint(NumiClothHipBatch& b) {
     HIP_OK(hipDeviceGetAttribute(&maxShared, hipDeviceAttributeMaxSharedMemoryPerBlock, s.device));
     A.ldsBytes = static_cast<size_t>(A.maxKnots) * sizeof(float4) + static_cast<size_t>(A.maxConstraints) * 4;
     if (A.ldsBytes + 1024 > static_cast<size_t>(maxShared)) {
-        fail(NUMI_CLOTH_HIP_ERROR_UNSUPPORTED, "adjoint specimen does not fit in LDS");
+        fail(NUMI_CLOTH_HIP_ERROR_UNSUPPORTED, "specimen does not fit in one workgroup's LDS");
     }
     hipStream_t st = s.stream;
+    A.baseInfo = info;
     A.info.allocate(S);
-    A.info.upload(info.data(), S, st);
+    A.info.upload(A.baseInfo.data(), S, st);
     A.distanceRanges.allocate(dr.size());
     A.distanceRanges.upload(dr.data(), dr.size(), st);
     A.bendRanges.allocate(std::max<size_t>(br.size(), 1));
     A.bendRanges.upload(br.data(), br.size(), st);
+    A.loss.allocate(S);
+    A.tracked.allocate(1);
+    A.targets.allocate(1);
+    HIP_OK(hipStreamSynchronize(st));
+    A.specimensReady = true;
+}
+
+// Gradient accumulators, tape sets and the side stream of the adjoint.
+void setupAdjoint(NumiClothHipBatch& b) {
+    setupSpecimens(b);
+    AdjointBuffers& A = b.adjoint;
+    if (A.ready) return;
+    NumiClothHipSolver& s = *b.solver;
+    hipStream_t st = s.stream;
     A.gxState.allocate(s.P);
     A.gvState.allocate(s.P);
     A.gradInverseMass.allocate(s.P);
     A.gradDamping.allocate(s.P);
     A.gradDistance.allocate(std::max(s.D, 1u));
     A.gradBend.allocate(std::max(s.B, 1u));
-    A.loss.allocate(S);
     for (AdjointBuffers::TapeSet& set : A.sets) {
         set.scratchPos.allocate(s.P);
         set.scratchPrev.allocate(s.P);
@@ -2194,6 +2224,104 @@ uint32_t adjointPasses(const AdjointBuffers& A, uint32_t iterations, uint32_t sw
            (sweeps == 0 ? 0u : 18u * sweeps * A.distanceColours);
 }
 
+// The trajectory's configs with the batch totals and the time at the end of
+// every substep, uploaded for the specimen kernels; returns the end time.
+double uploadSpecimenConfigs(NumiClothHipBatch& b, const NumiClothBagGPUConfig* in, uint32_t T) {
+    NumiClothHipSolver& s = *b.solver;
+    AdjointBuffers& A = b.adjoint;
+    A.hostConfigs.assign(in, in + T);
+    A.hostTimes.resize(T);
+    double t = s.time;
+    const NumiClothBagGPUConfig& totals = b.merged->config;
+    for (uint32_t k = 0; k < T; ++k) {
+        NumiClothBagGPUConfig& c = A.hostConfigs[k];
+        c.control.y = totals.control.y;
+        c.control.z = totals.control.z;
+        c.control.w = totals.control.w;
+        c.constraintCounts = totals.constraintCounts;
+        c.contactCounts = totals.contactCounts;
+        checkTrajectoryConfig(s, c);
+        if (c.control.x != NUMI_CLOTH_BAG_GPU_ABI_VERSION || !(c.gravityAndTimestep.w > 0.0f) ||
+            !std::isfinite(c.gravityAndTimestep.w) || !std::isfinite(c.gravityAndTimestep.z)) {
+            fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "specimen configs must be valid (ABI version, finite dt > 0)");
+        }
+        t += c.gravityAndTimestep.w;
+        A.hostTimes[k] = t;
+    }
+    if (A.configs.count < T) A.configs.allocate(T);
+    A.configs.upload(A.hostConfigs.data(), T, s.stream);
+    if (A.times.count < T) A.times.allocate(T);
+    A.times.upload(A.hostTimes.data(), T, s.stream);
+    return t;
+}
+
+// Fields of the specimen kernels' arguments that every launch shares; the
+// state pointers are the solver's own.
+SpecimenArgs specimenArgs(NumiClothHipBatch& b, uint32_t iterations, uint32_t sweeps) {
+    NumiClothHipSolver& s = *b.solver;
+    AdjointBuffers& A = b.adjoint;
+    SpecimenArgs a{};
+    a.specimens = A.info.ptr;
+    a.distanceRanges = A.distanceRanges.ptr;
+    a.bendRanges = A.bendRanges.ptr;
+    a.distanceColours = A.distanceColours;
+    a.bendColours = A.bendColours;
+    a.configs = A.configs.ptr;
+    a.times = A.times.ptr;
+    a.iterations = iterations;
+    a.sweeps = sweeps;
+    a.anchor = s.driven ? s.anchor.ptr : nullptr;
+    a.motion = s.motion.ptr;
+    a.drive = make_float4(s.extension.driveAmplitude, s.extension.driveOmega, s.extension.drivePhase, 0.0f);
+    const float dt0 = s.initialConfig.gravityAndTimestep.w;
+    a.dampingFactor = s.extension.damping > 0.0f ? 1.0f / (1.0f + s.extension.damping * dt0) : 1.0f;
+    a.dampingDt = dt0;
+    a.ij = s.dIJ.ptr;
+    a.restCompliance = s.dRestCompliance.ptr;
+    a.maxLength = s.dMaxLength.ptr;
+    a.bEnds = s.bEnds.ptr;
+    a.chordCompliance = s.bChordCompliance.ptr;
+    a.tracked = A.tracked.ptr;
+    a.targets = A.targets.ptr;
+    a.loss = A.loss.ptr;
+    a.passes = adjointPasses(A, iterations, sweeps);
+    a.particles = s.P;
+    a.distances = s.D;
+    a.bends = s.B;
+    a.specimenCount = b.specimens;
+    a.failure = s.failure.ptr;
+    a.pos = s.pos.ptr;
+    a.prev = s.prev.ptr;
+    a.vel = s.vel.ptr;
+    a.dLambda = s.dLambda.ptr;
+    a.bLambda = s.bLambda.ptr;
+    return a;
+}
+
+// One persistent launch for all substeps of the call: one workgroup per
+// specimen, its knots and multipliers resident in LDS throughout (the
+// adjoint's forward kernel with neither tape nor tracked knots). Leaves the
+// batch where stepBatch would, bit for bit.
+void stepResident(NumiClothHipBatch& b, const NumiClothBagGPUConfig* configs, uint32_t T, uint32_t iterations,
+                  uint32_t sweeps) {
+    if (T == 0) return;
+    if (configs == nullptr) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "null configs");
+    NumiClothHipSolver& s = *b.solver;
+    setupSpecimens(b);
+    AdjointBuffers& A = b.adjoint;
+    const double end = uploadSpecimenConfigs(b, configs, T);
+    A.info.upload(A.baseInfo.data(), b.specimens, s.stream);  // no tracked knots
+    SpecimenArgs f = specimenArgs(b, iterations, sweeps);
+    f.firstSubstep = 0;
+    f.substeps = T;
+    f.record = 0u;
+    f.trackedTotal = 0;
+    k_specimen_forward<<<b.specimens, kSpecimenThreads, A.ldsBytes, s.stream>>>(f);
+    HIP_OK(hipGetLastError());
+    s.time = end;
+    ++b.version;
+}
+
 void runAdjoint(NumiClothHipBatch& b, const NumiClothHipAdjointProblem& pr) {
     NumiClothHipSolver& s = *b.solver;
     setupAdjoint(b);
@@ -2208,31 +2336,10 @@ void runAdjoint(NumiClothHipBatch& b, const NumiClothHipAdjointProblem& pr) {
         fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "adjoint observable must be TRACKING or LINEAR");
     hipStream_t st = s.stream;
     HIP_OK(hipStreamSynchronize(st));
-    // configs with the batch totals, times, tracked knots (merged indices)
-    std::vector<NumiClothBagGPUConfig> configs(pr.configs, pr.configs + T);
-    std::vector<double> times(T);
-    double t = s.time;
-    const NumiClothBagGPUConfig& totals = b.merged->config;
-    for (uint32_t k = 0; k < T; ++k) {
-        NumiClothBagGPUConfig& c = configs[k];
-        c.control.y = totals.control.y;
-        c.control.z = totals.control.z;
-        c.control.w = totals.control.w;
-        c.constraintCounts = totals.constraintCounts;
-        c.contactCounts = totals.contactCounts;
-        checkTrajectoryConfig(s, c);
-        if (c.control.x != NUMI_CLOTH_BAG_GPU_ABI_VERSION || !(c.gravityAndTimestep.w > 0.0f) ||
-            !std::isfinite(c.gravityAndTimestep.w) || !std::isfinite(c.gravityAndTimestep.z)) {
-            fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "adjoint configs must be valid (ABI version, finite dt > 0)");
-        }
-        t += c.gravityAndTimestep.w;
-        times[k] = t;
-    }
+    const double t = uploadSpecimenConfigs(b, pr.configs, T);
     const uint32_t Tk = pr.trackedPerSpecimen;
     std::vector<uint32_t> tracked(static_cast<size_t>(S) * Tk);
-    std::vector<SpecimenInfo> info(S);
-    A.info.download(info.data(), S, st);
-    HIP_OK(hipStreamSynchronize(st));
+    std::vector<SpecimenInfo> info = A.baseInfo;
     for (uint32_t k = 0; k < S; ++k) {
         for (uint32_t q = 0; q < Tk; ++q) {
             const uint32_t local = pr.tracked[static_cast<size_t>(k) * Tk + q];
@@ -2243,10 +2350,6 @@ void runAdjoint(NumiClothHipBatch& b, const NumiClothHipAdjointProblem& pr) {
         info[k].trackedCount = Tk;
     }
     A.info.upload(info.data(), S, st);
-    A.configs.allocate(T);
-    A.configs.upload(configs.data(), T, st);
-    A.times.allocate(T);
-    A.times.upload(times.data(), T, st);
     A.tracked.allocate(std::max<size_t>(tracked.size(), 1));
     A.tracked.upload(tracked.data(), tracked.size(), st);
     const size_t targetCount = static_cast<size_t>(T) * S * Tk * 3;
@@ -2274,38 +2377,9 @@ void runAdjoint(NumiClothHipBatch& b, const NumiClothHipAdjointProblem& pr) {
     HIP_OK(hipMemsetAsync(A.gradDistance.ptr, 0, std::max(s.D, 1u) * sizeof(float), st));
     HIP_OK(hipMemsetAsync(A.gradBend.ptr, 0, std::max(s.B, 1u) * sizeof(float), st));
 
-    SpecimenArgs a{};
-    a.specimens = A.info.ptr;
-    a.distanceRanges = A.distanceRanges.ptr;
-    a.bendRanges = A.bendRanges.ptr;
-    a.distanceColours = A.distanceColours;
-    a.bendColours = A.bendColours;
-    a.configs = A.configs.ptr;
-    a.times = A.times.ptr;
-    a.iterations = pr.iterations;
-    a.sweeps = pr.strainSweeps;
-    a.anchor = s.driven ? s.anchor.ptr : nullptr;
-    a.motion = s.motion.ptr;
-    a.drive = make_float4(s.extension.driveAmplitude, s.extension.driveOmega, s.extension.drivePhase, 0.0f);
-    const float dt0 = s.initialConfig.gravityAndTimestep.w;
-    a.dampingFactor = s.extension.damping > 0.0f ? 1.0f / (1.0f + s.extension.damping * dt0) : 1.0f;
-    a.dampingDt = dt0;
-    a.ij = s.dIJ.ptr;
-    a.restCompliance = s.dRestCompliance.ptr;
-    a.maxLength = s.dMaxLength.ptr;
-    a.bEnds = s.bEnds.ptr;
-    a.chordCompliance = s.bChordCompliance.ptr;
-    a.tracked = A.tracked.ptr;
-    a.targets = A.targets.ptr;
+    SpecimenArgs a = specimenArgs(b, pr.iterations, pr.strainSweeps);
     a.trackedTotal = S * Tk;
     a.linear = pr.observable == NUMI_CLOTH_ADJOINT_LINEAR ? 1u : 0u;
-    a.loss = A.loss.ptr;
-    a.passes = passes;
-    a.particles = s.P;
-    a.distances = s.D;
-    a.bends = s.B;
-    a.specimenCount = S;
-    a.failure = s.failure.ptr;
     a.gxState = A.gxState.ptr;
     a.gvState = A.gvState.ptr;
     a.gradInverseMass = A.gradInverseMass.ptr;
@@ -2539,6 +2613,14 @@ int numi_cloth_hip_batch_step(NumiClothHipBatch* batch, const NumiClothBagGPUCon
     return guarded([&] {
         if (batch == nullptr) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "null batch");
         stepBatch(*batch, configs, substeps, iterations, strainSweeps);
+    });
+}
+
+int numi_cloth_hip_batch_step_resident(NumiClothHipBatch* batch, const NumiClothBagGPUConfig* configs,
+                                       uint32_t substeps, uint32_t iterations, uint32_t strainSweeps) {
+    return guarded([&] {
+        if (batch == nullptr) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "null batch");
+        stepResident(*batch, configs, substeps, iterations, strainSweeps);
     });
 }
 
