@@ -306,6 +306,15 @@ def roofline(cloth, solver, workload: str) -> tuple[dict, dict]:
     return line, tile_config
 
 
+def launch_of(solver) -> dict:
+    """How the solver launches substeps (graph replay or direct), as the
+    first step()'s probe decided, with the probe's timings."""
+    m = solver.launch_mode()
+    return {"mode": "graph" if m.graphs else "direct", "probed": bool(m.decided) and m.probeGraphMilliseconds > 0,
+            "probe_ms_two_substeps": {"graph": round(m.probeGraphMilliseconds, 3),
+                                      "direct": round(m.probeDirectMilliseconds, 3)}}
+
+
 def diagnostics_of(solver, state) -> dict:
     d = solver.diagnostics()
     self_status = state["selfStatus"]["counters"][0]
@@ -362,6 +371,7 @@ def run_plain(args, cloth, hip) -> dict:
     diagnostics["window_start"] = start_contacts
     diagnostics["window_end"] = contact_counts(scene, state)
     line, tile_config = roofline(cloth, solver, args.workload)
+    launch = launch_of(solver)
     released = int(state["releaseStatus"]["masks"][0, 1])
     solver.close()
     baseline = None
@@ -378,7 +388,7 @@ def run_plain(args, cloth, hip) -> dict:
               "self_cell_size": float(scene.config["clothMaterial"][0, 1]),
               "frame_seconds": frame_seconds, "wall_seconds_timed": round(timer.wall, 3),
               "timed_frames": f"{untimed + 1}-{untimed + args.steps} (1-based; {untimed} untimed before)",
-              "tile_programs": tile_config}
+              "tile_programs": tile_config, "launch": launch}
     if args.workload == "bag":
         config["released_mask"] = released
     return {"metric": "particle-substeps/s", "value": value, "unit": "particle-substeps/s",
@@ -632,6 +642,7 @@ def run_adjoint(args, cloth, hip) -> dict:
             batch.close()
     adjoint_seconds = sum(adjoint_times) / len(adjoint_times)
     grads = batch.gradients(0)
+    stats = batch.adjoint_stats()
     failure = batch.read(0)["failure"]
     batch.close()
     if failure != 0:
@@ -645,7 +656,9 @@ def run_adjoint(args, cloth, hip) -> dict:
                         "untimed steps that also capture its graph)",
               "gradient_entries_per_specimen": int(grads["distanceCompliance"].size + grads["bendCompliance"].size
                                                    + grads["inverseMass"].size + grads["damping"].size),
-              "loss_specimen0": grads["loss"], "tile_programs": tile_config}
+              "loss_specimen0": grads["loss"], "tile_programs": tile_config,
+              "strain_passes": int(stats.strainPasses), "strain_passes_acting": int(stats.strainPassesTaped),
+              "strain_acting_fraction": stats.strainPassesTaped / max(1, stats.strainPasses)}
     return {"metric": "adjoint gradients/s", "value": len(specimens) / adjoint_seconds, "unit": "gradients/s",
             "ms_per_step": adjoint_seconds * 1000.0, "dtype": "f32", "config": config, "roofline": line,
             "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline_adjoint(specimens, local, targets, configs),

@@ -93,6 +93,19 @@ typedef struct NumiClothHipSceneExtension {
      *   one contact. */
     uint32_t sweptGround;
     uint32_t pointSelfCollision;
+    /* certificatePasses: at most this many extra certificate passes at the
+     *   last substep of every step() call with strain sweeps, stopping after
+     *   the first pass whose residuals all fall below the independent CPU
+     *   reference's thresholds (its data-dependent loop at the last substep
+     *   of a frame, up to 128 passes, tools/cloth_bag.cpp:5143-5185). A pass
+     *   is one of the schedule's certificate rounds: segment self-contact,
+     *   the strain sweeps, then the final contact passes (fruit pairs,
+     *   fruit/yarn, ground). Residuals, each max(0, ...): strain-limit
+     *   violation < 1e-6 m, ground penetration of knots and fruits < 1e-9 m,
+     *   fruit/yarn penetration < 1e-6 m, segment self-penetration over the
+     *   self-contact candidates < 1e-6 m. The passes run are reported by
+     *   numi_cloth_hip_diagnostics. 0 (the default) runs none. */
+    uint32_t certificatePasses;
 } NumiClothHipSceneExtension;
 
 /*
@@ -176,7 +189,9 @@ typedef struct NumiClothHipDiagnostics {
     /* Knot pairs of the last point self-collision pass (0 when the scene
      * has none). */
     uint32_t pointSelfContacts;
-    uint32_t reserved;
+    /* Certificate passes the last step() call ran (0 without
+     * extension.certificatePasses). */
+    uint32_t certificatePasses;
 } NumiClothHipDiagnostics;
 
 typedef struct NumiClothHipSolver NumiClothHipSolver;
@@ -288,6 +303,23 @@ typedef struct NumiClothHipSelfBroadPhase {
 
 int numi_cloth_hip_self_broad_phase(NumiClothHipSolver* solver, NumiClothHipSelfBroadPhase* io);
 
+/*
+ * How the solver launches its substeps (no counterpart in the reference,
+ * whose substeps are Metal command buffers): graphs = 1 replays a captured
+ * hipGraph per substep, 0 launches the kernels one by one; the bytes are
+ * the same. Unless NUMI_CLOTH_HIP_GRAPHS fixes it, the first step() of at
+ * least five substeps times two substeps each way (after one replay that
+ * uploads the graph) and keeps the faster; decided = 0 until then.
+ */
+typedef struct NumiClothHipLaunchMode {
+    uint32_t graphs;
+    uint32_t decided;
+    float probeGraphMilliseconds;   /* two substeps by graph replay */
+    float probeDirectMilliseconds;  /* two substeps by direct launches */
+} NumiClothHipLaunchMode;
+
+int numi_cloth_hip_launch_mode(NumiClothHipSolver* solver, NumiClothHipLaunchMode* out);
+
 /* The hipStream_t all of the solver's work is queued on. */
 void* numi_cloth_hip_stream(NumiClothHipSolver* solver);
 
@@ -385,6 +417,18 @@ typedef struct NumiClothHipSpecimenGradients {
 } NumiClothHipSpecimenGradients;
 
 int numi_cloth_hip_batch_adjoint(NumiClothHipBatch* batch, const NumiClothHipAdjointProblem* problem);
+/* What the last adjoint call taped (no counterpart in the reference): of
+ * the strain-limit passes its recomputations ran (substeps x strain passes
+ * per substep x specimens), those in which a constraint acted and whose
+ * positions were therefore taped and reversed. */
+typedef struct NumiClothHipAdjointStats {
+    uint64_t strainPasses;
+    uint32_t strainPassesTaped;
+    uint32_t reserved;
+} NumiClothHipAdjointStats;
+
+int numi_cloth_hip_batch_adjoint_stats(NumiClothHipBatch* batch, NumiClothHipAdjointStats* out);
+
 int numi_cloth_hip_batch_gradients(NumiClothHipBatch* batch, uint32_t specimen,
                                    const NumiClothHipSpecimenGradients* out);
 

@@ -75,7 +75,7 @@ class SceneExtension(C.Structure):
                 ("selfContactMargin", C.c_float), ("selfCandidateCapacity", C.c_uint32),
                 ("driveAmplitude", C.c_float), ("driveOmega", C.c_float), ("drivePhase", C.c_float),
                 ("damping", C.c_float), ("particleMotion", C.c_void_p), ("sweptGround", C.c_uint32),
-                ("pointSelfCollision", C.c_uint32)]
+                ("pointSelfCollision", C.c_uint32), ("certificatePasses", C.c_uint32)]
 
 
 _DESC_POINTERS = ["config", "particles", "distances", "distanceBatches", "grips", "knots", "knotBatches", "bends",
@@ -99,13 +99,22 @@ class State(C.Structure):
                  "mouthClearance"]]
 
 
+class AdjointStats(C.Structure):
+    _fields_ = [("strainPasses", C.c_uint64), ("strainPassesTaped", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class LaunchMode(C.Structure):
+    _fields_ = [("graphs", C.c_uint32), ("decided", C.c_uint32), ("probeGraphMilliseconds", C.c_float),
+                ("probeDirectMilliseconds", C.c_float)]
+
+
 class Diagnostics(C.Structure):
     _fields_ = [("maximumStrainViolation", C.c_float), ("maximumGroundPenetration", C.c_float),
                 ("maximumSpherePenetration", C.c_float), ("maximumSelfPenetration", C.c_float),
                 ("selfCandidateCount", C.c_uint32), ("selfCandidateOverflow", C.c_uint32),
                 ("yarnReruns", C.c_uint32), ("yarnCandidatesMax", C.c_uint32),
                 ("selfEntryCount", C.c_uint32), ("selfGridCells", C.c_uint32 * 3),
-                ("pointSelfContacts", C.c_uint32), ("reserved", C.c_uint32)]
+                ("pointSelfContacts", C.c_uint32), ("certificatePasses", C.c_uint32)]
 
 
 MATERIAL_FIELDS = ["ordinaryNodeMassKg", "hemNodeMassKg", "yarnRadiusM", "axialBodyComplianceMPerN",
@@ -252,6 +261,8 @@ def library() -> C.CDLL:
         "numi_cloth_hip_step_noalloc": ([vp, vp, u32, u32, u32], i32),
         "numi_cloth_hip_read": ([vp, C.POINTER(State)], i32),
         "numi_cloth_hip_diagnostics": ([vp, C.POINTER(Diagnostics)], i32),
+        "numi_cloth_hip_launch_mode": ([vp, C.POINTER(LaunchMode)], i32),
+        "numi_cloth_hip_batch_adjoint_stats": ([vp, C.POINTER(AdjointStats)], i32),
         "numi_cloth_hip_synchronize": ([vp], i32),
         "numi_cloth_hip_stream": ([vp], vp),
         "numi_cloth_hip_destroy": ([vp], i32),
@@ -389,6 +400,7 @@ class Scene:
         self.particleMotion = None if motion is None else np.ascontiguousarray(motion, np.float32).reshape(-1, 4)
         self.sweptGround = int(arrays.pop("sweptGround", 0))
         self.pointSelfCollision = int(arrays.pop("pointSelfCollision", 0))
+        self.certificatePasses = int(arrays.pop("certificatePasses", 0))
         self.initialCandidateMask = int(arrays.pop("initialCandidateMask", 0))
         self.initialReleasedMask = int(arrays.pop("initialReleasedMask", 0))
         if arrays:
@@ -417,6 +429,7 @@ class Scene:
         e.particleMotion = _ptr(self.particleMotion)
         e.sweptGround = self.sweptGround
         e.pointSelfCollision = self.pointSelfCollision
+        e.certificatePasses = self.certificatePasses
         return desc
 
     def refresh(self) -> None:
@@ -450,7 +463,7 @@ class Scene:
                     selfCandidateCapacity=e.selfCandidateCapacity, driveAmplitude=e.driveAmplitude,
                     driveOmega=e.driveOmega, drivePhase=e.drivePhase, damping=e.damping, particleMotion=motion,
                     sweptGround=e.sweptGround, pointSelfCollision=e.pointSelfCollision,
-                    initialCandidateMask=d.initialCandidateMask, initialReleasedMask=d.initialReleasedMask, **arrays)
+                    certificatePasses=e.certificatePasses, initialCandidateMask=d.initialCandidateMask, initialReleasedMask=d.initialReleasedMask, **arrays)
         if keep:
             return scene, handle
         lib.numi_cloth_scene_destroy(handle)
@@ -558,8 +571,8 @@ class Scene:
                      selfContactMargin=self.selfContactMargin, selfCandidateCapacity=self.selfCandidateCapacity,
                      driveAmplitude=self.driveAmplitude, driveOmega=self.driveOmega, drivePhase=self.drivePhase,
                      damping=self.damping, particleMotion=self.particleMotion, sweptGround=self.sweptGround,
-                     pointSelfCollision=self.pointSelfCollision, initialCandidateMask=int(masks[0]),
-                     initialReleasedMask=int(masks[1]), **arrays)
+                     pointSelfCollision=self.pointSelfCollision, certificatePasses=self.certificatePasses,
+                     initialCandidateMask=int(masks[0]), initialReleasedMask=int(masks[1]), **arrays)
 
 
 # ------------------------------------------------------------------- solver --
@@ -649,6 +662,13 @@ class Batch:
         problem = AdjointProblem(configs.ctypes.data, len(configs), iterations, strain_sweeps, checkpoint_interval,
                                  tracked.ctypes.data, tracked.shape[1], 1 if linear else 0, targets.ctypes.data)
         _check(library().numi_cloth_hip_batch_adjoint(self._handle, C.byref(problem)))
+
+    def adjoint_stats(self) -> AdjointStats:
+        """Strain-limit passes of the last adjoint call's recomputations, and
+        how many of them acted (and were taped)."""
+        out = AdjointStats()
+        _check(library().numi_cloth_hip_batch_adjoint_stats(self._handle, C.byref(out)))
+        return out
 
     def gradients(self, specimen: int) -> dict:
         s = self.specimens[specimen]
@@ -797,6 +817,11 @@ class Solver:
     def diagnostics(self) -> Diagnostics:
         out = Diagnostics()
         _check(library().numi_cloth_hip_diagnostics(self._handle, C.byref(out)))
+        return out
+
+    def launch_mode(self) -> LaunchMode:
+        out = LaunchMode()
+        _check(library().numi_cloth_hip_launch_mode(self._handle, C.byref(out)))
         return out
 
 

@@ -74,6 +74,7 @@ struct SpecimenArgs {
     float* tapeLambdaB;            // [substep in launch][even iteration][bends]
     float4* tapeV0;                // [substep in launch][particles]
     uint32_t* tapeActive;          // [substep in launch][pass][specimen]
+    uint32_t* strainTaped;         // record mode: strain passes taped (acting), summed over specimens
     float* lossGrad;               // [substep in launch][trackedTotal][3]
     uint32_t specimenCount;
     uint32_t* failure;
@@ -236,8 +237,10 @@ __global__ void __launch_bounds__(kSpecimenThreads) k_specimen_forward(SpecimenA
                         if (any) acting = 1u;
                         __syncthreads();
                         const bool act = acting != 0u;
-                        if (tid == 0)
+                        if (tid == 0) {
                             a.tapeActive[(static_cast<size_t>(j) * a.passes + pass) * a.specimenCount + spec] = act;
+                            if (act) atomicAdd(a.strainTaped, 1u);
+                        }
                         if (!act) continue;  // uniform: no constraint moves, nothing to tape
                         snapshot(pass);
                         __syncthreads();

@@ -373,10 +373,10 @@ void mergeScenes(const std::vector<const NumiClothHipSceneDesc*>& specimens, Num
         const NumiClothBagGPUConfig& c = *d->config;
         if (d->gripCount || d->fruitCount || d->fruitPairCount || d->yarnContactCount || d->mouthRimCount ||
             d->cuffCount || d->extension.staticSphereCount || d->extension.selfContact ||
-            d->extension.pointSelfCollision || d->extension.sweptGround) {
+            d->extension.pointSelfCollision || d->extension.sweptGround || d->extension.certificatePasses) {
             fail(NUMI_CLOTH_HIP_ERROR_UNSUPPORTED,
                  "merged specimens may not have grips, fruits, mouth, cuff, static spheres, self-contact, point "
-                 "self-collision or the swept ground pass");
+                 "self-collision, the swept ground pass or certificate passes");
         }
         if (c.airVelocityAndDensity.w > 0.0f) {
             fail(NUMI_CLOTH_HIP_ERROR_UNSUPPORTED, "merged specimens may not have air loads (their reduction is global)");

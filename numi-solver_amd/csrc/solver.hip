@@ -31,6 +31,7 @@
 #include "kernels_yarn.hpp"
 #include "kernels_adjoint.hpp"
 #include "kernels_point.hpp"
+#include "kernels_certificate.hpp"
 #include "tile_plan.hpp"
 #include "numi_cloth_hip.h"
 #include "numi_cloth_scene.h"
@@ -55,6 +56,10 @@ namespace {
 constexpr uint32_t kReconciliationPasses = 8;
 constexpr uint32_t kFinalContactPasses = 2;
 constexpr uint32_t kCertificatePasses = 8;
+
+// Substeps of the first step() call that time graph replays against direct
+// launches: one replay that uploads the graph, then two of each.
+constexpr uint32_t kLaunchProbeSubsteps = 5;
 
 template <typename T>
 struct DeviceArray {
@@ -128,6 +133,12 @@ struct NumiClothHipSolver {
     DeviceArray<NumiClothBagGPUAerodynamicsStatus> aeroStatus;
     DeviceArray<NumiClothBagGPUReleaseStatus> releaseStatus;
     DeviceArray<float4> diagPartials, diagResult;
+    // Certificate loop (extension.certificatePasses != 0): per-workgroup
+    // residual maxima, the reduced residuals and their pinned host copy.
+    DeviceArray<float4> certificatePartials, certificateResult;
+    float4* certificateHost = nullptr;
+    uint32_t certificateBlocks = 0;
+    uint32_t certificateLast = 0;  // passes of the last step() call
 
     // Produce-bag families (fruits, fruit pairs, fruit/yarn records, air).
     uint32_t F = 0, FP = 0, Y = 0;
@@ -208,10 +219,16 @@ struct NumiClothHipSolver {
     float* bLamCur = nullptr;
     float* kLamCur = nullptr;
 
-    // Graphs are the default; NUMI_CLOTH_HIP_GRAPHS=0 launches the same
-    // kernels directly on the stream (used under profilers that do not
-    // follow graph launches).
+    // A substep is launched either as a replay of its captured hipGraph or
+    // kernel by kernel on the stream; the bytes are the same. The first
+    // step() of at least kLaunchProbeSubsteps substeps times both ways on the
+    // live scene and keeps the faster (on an MI355X with an idle host the two
+    // are within 2 % for the 1M and 256^2 sheets and the bag; a busy host
+    // slows the direct launches first). NUMI_CLOTH_HIP_GRAPHS=0 / 1 fixes the
+    // choice (=0 also under profilers that do not follow graph launches).
     bool useGraphs = true;
+    bool launchDecided = false;
+    float launchProbeGraphMs = 0.0f, launchProbeDirectMs = 0.0f;
     uint32_t sphereWindowMin = kSphereWindowMin;
 
     // Pinned staging for per-step uploads.
@@ -229,6 +246,7 @@ struct NumiClothHipSolver {
         if (graph != nullptr) (void)hipGraphExecDestroy(graph);
         if (stagingConfigs != nullptr) (void)hipHostFree(stagingConfigs);
         if (stagingTimes != nullptr) (void)hipHostFree(stagingTimes);
+        if (certificateHost != nullptr) (void)hipHostFree(certificateHost);
         if (stream != nullptr) (void)hipStreamDestroy(stream);
     }
 
@@ -661,8 +679,10 @@ struct NumiClothHipSolver {
     // (tools/cloth_bag_metal.mm:4211-4709), families without records omitted.
     // substepConfigs / substepTimes: where k_prepare reads each substep's
     // config and end time (nullptr: the solver's own staging).
+    // certify: run the certificate loop (host-synchronised, so never while
+    // capturing) after the schedule's certificate rounds.
     void recordSubstep(uint32_t iterations, uint32_t sweeps, const NumiClothBagGPUConfig* substepConfigs = nullptr,
-                       const double* substepTimes = nullptr) {
+                       const double* substepTimes = nullptr, bool certify = false) {
         selfBuffer = 0;
         pendingSelfApply = -1;
         dirtySelf = -1;
@@ -758,17 +778,8 @@ struct NumiClothHipSolver {
                 launchStrainAndCollide(sweeps, 1);
                 launchFruitGround();
             }
-            for (uint32_t round = 0; round <= kCertificatePasses; ++round) {
-                launchSelf(0, false, defer);
-                if (F == 0) {
-                    // Without fruits the final contact passes are ground
-                    // passes only and fuse into the strain program.
-                    launchStrainAndCollide(sweeps, kFinalContactPasses);
-                } else {
-                    launchStrainAndCollide(sweeps, 0);
-                    for (uint32_t pass = 0; pass < kFinalContactPasses; ++pass) launchFinalContacts();
-                }
-            }
+            for (uint32_t round = 0; round <= kCertificatePasses; ++round) certificateRound(sweeps, defer);
+            if (certify) runCertificates(sweeps, defer);
         }
         restorePrimaryBuffers();
         launchBuildYarnContacts();
@@ -792,6 +803,46 @@ struct NumiClothHipSolver {
             if (!rimHost.empty()) {
                 k_release<<<1, fruitLanes(), 0, stream>>>(cfg(), ok(), pos.ptr, rim.ptr, fruits.ptr, F,
                                                           releaseStatus.ptr, mouthClearance.ptr, failure.ptr);
+            }
+        }
+    }
+
+    // One certificate round of the schedule: segment self-contact, the
+    // strain sweeps, the final contact passes.
+    void certificateRound(uint32_t sweeps, bool defer) {
+        launchSelf(0, false, defer);
+        if (F == 0) {
+            // Without fruits the final contact passes are ground passes only
+            // and fuse into the strain program.
+            launchStrainAndCollide(sweeps, kFinalContactPasses);
+        } else {
+            launchStrainAndCollide(sweeps, 0);
+            for (uint32_t pass = 0; pass < kFinalContactPasses; ++pass) launchFinalContacts();
+        }
+    }
+
+    // The independent CPU reference's data-dependent certificate loop
+    // (tools/cloth_bag.cpp:5143-5185): certificate rounds until the residuals
+    // of one all fall below its thresholds, at most
+    // extension.certificatePasses rounds. Each round's residuals are read
+    // back (the stream is synchronised once per round).
+    void runCertificates(uint32_t sweeps, bool defer) {
+        certificateLast = 0;
+        for (uint32_t pass = 0; pass < extension.certificatePasses; ++pass) {
+            certificateRound(sweeps, defer);
+            flushCollide();
+            k_certificate_partial<<<certificateBlocks, kBlock, 0, stream>>>(
+                cfg(), posCur, P, dIJ.ptr, dMaxLength.ptr, D, fruits.ptr, F, self ? grid.ptr : nullptr, pairs.ptr,
+                certificatePartials.ptr);
+            k_diag_final<<<1, kBlock, 0, stream>>>(certificatePartials.ptr, certificateBlocks, certificateResult.ptr);
+            HIP_OK(hipMemcpyAsync(certificateHost, certificateResult.ptr, sizeof(float4), hipMemcpyDeviceToHost,
+                                  stream));
+            HIP_OK(hipStreamSynchronize(stream));
+            certificateLast = pass + 1;
+            const float4 r = *certificateHost;
+            if (static_cast<double>(r.w) < 1.0e-6 && static_cast<double>(r.z) < 1.0e-6 &&
+                static_cast<double>(r.y) < 1.0e-9 && static_cast<double>(r.x) < 1.0e-6) {
+                break;
             }
         }
     }
@@ -1164,7 +1215,10 @@ void createSolver(const NumiClothHipSceneDesc& scene, NumiClothHipSolver& s) {
     const NumiClothBagGPUReleaseStatus masks{{scene.initialCandidateMask, scene.initialReleasedMask, 0u, 0u}};
     s.releaseStatus.upload(&masks, 1, st);
     s.diagPartials.allocate(1024);
-    if (const char* graphs = std::getenv("NUMI_CLOTH_HIP_GRAPHS")) s.useGraphs = std::strcmp(graphs, "0") != 0;
+    if (const char* graphs = std::getenv("NUMI_CLOTH_HIP_GRAPHS")) {
+        s.useGraphs = std::strcmp(graphs, "0") != 0;
+        s.launchDecided = true;
+    }
     // "0": test every sphere list whole; "1": window every non-empty list.
     if (const char* window = std::getenv("NUMI_CLOTH_HIP_SPHERE_WINDOW"))
         s.sphereWindowMin = std::strcmp(window, "0") == 0 ? UINT32_MAX : 1u;
@@ -1183,6 +1237,13 @@ void createSolver(const NumiClothHipSceneDesc& scene, NumiClothHipSolver& s) {
     // numi_cloth_hip_step_noalloc can run them without building one.
     s.ensureStrainProgram(3);
     s.diagResult.allocate(1);
+    if (scene.extension.certificatePasses != 0) {
+        const uint64_t work = std::max<uint64_t>({s.P, s.D, static_cast<uint64_t>(s.F) * s.D, s.pairCapacity});
+        s.certificateBlocks = static_cast<uint32_t>(std::min<uint64_t>(1024, std::max<uint64_t>(1, (work + kBlock - 1) / kBlock)));
+        s.certificatePartials.allocate(s.certificateBlocks);
+        s.certificateResult.allocate(1);
+        HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&s.certificateHost), sizeof(float4)));
+    }
     s.ensureStaging(64);
     HIP_OK(hipStreamSynchronize(st));
 }
@@ -1612,12 +1673,37 @@ void stepSolver(NumiClothHipSolver& s, const NumiClothBagGPUConfig* configs, uin
     k_fill_u32<<<1, 64, 0, s.stream>>>(reinterpret_cast<uint32_t*>(s.selfStatus.ptr), 4, 0u);
     k_fill_u32<<<1, 64, 0, s.stream>>>(reinterpret_cast<uint32_t*>(s.aeroStatus.ptr), 4, 0u);
     HIP_OK(hipGetLastError());
-    if (s.useGraphs) {
-        for (uint32_t k = 0; k < substeps; ++k) HIP_OK(hipGraphLaunch(s.graph, s.stream));
-    } else {
-        for (uint32_t k = 0; k < substeps; ++k) s.recordSubstep(iterations, sweeps);
-        HIP_OK(hipGetLastError());
+    // With certificate passes the call's last substep runs outside the graph:
+    // its certificate loop reads residuals back between rounds.
+    const bool certify = s.extension.certificatePasses != 0;
+    const uint32_t captured = certify ? substeps - 1 : substeps;
+    s.certificateLast = 0;
+    uint32_t done = 0;
+    if (s.useGraphs && !s.launchDecided && captured >= kLaunchProbeSubsteps) {
+        // One replay uploads the graph; then two substeps each way, timed.
+        hipEvent_t marks[3];
+        for (hipEvent_t& e : marks) HIP_OK(hipEventCreate(&e));
+        HIP_OK(hipGraphLaunch(s.graph, s.stream));
+        HIP_OK(hipEventRecord(marks[0], s.stream));
+        for (int k = 0; k < 2; ++k) HIP_OK(hipGraphLaunch(s.graph, s.stream));
+        HIP_OK(hipEventRecord(marks[1], s.stream));
+        for (int k = 0; k < 2; ++k) s.recordSubstep(iterations, sweeps);
+        HIP_OK(hipEventRecord(marks[2], s.stream));
+        HIP_OK(hipEventSynchronize(marks[2]));
+        HIP_OK(hipEventElapsedTime(&s.launchProbeGraphMs, marks[0], marks[1]));
+        HIP_OK(hipEventElapsedTime(&s.launchProbeDirectMs, marks[1], marks[2]));
+        for (hipEvent_t& e : marks) (void)hipEventDestroy(e);
+        s.useGraphs = s.launchProbeGraphMs <= s.launchProbeDirectMs;
+        s.launchDecided = true;
+        done = kLaunchProbeSubsteps;
     }
+    if (s.useGraphs) {
+        for (uint32_t k = done; k < captured; ++k) HIP_OK(hipGraphLaunch(s.graph, s.stream));
+    } else {
+        for (uint32_t k = done; k < captured; ++k) s.recordSubstep(iterations, sweeps);
+    }
+    if (certify) s.recordSubstep(iterations, sweeps, nullptr, nullptr, true);
+    HIP_OK(hipGetLastError());
 }
 
 // ------------------------------------------------- no-allocation step ----
@@ -1710,7 +1796,10 @@ void stepSolverNoAlloc(NumiClothHipSolver& s, const NumiClothBagGPUConfig* confi
     k_fill_u32<<<1, 64, 0, s.stream>>>(reinterpret_cast<uint32_t*>(s.frictionStatus.ptr), 8, 0u);
     k_fill_u32<<<1, 64, 0, s.stream>>>(reinterpret_cast<uint32_t*>(s.selfStatus.ptr), 4, 0u);
     k_fill_u32<<<1, 64, 0, s.stream>>>(reinterpret_cast<uint32_t*>(s.aeroStatus.ptr), 4, 0u);
-    for (uint32_t k = 0; k < substeps; ++k) s.recordSubstep(iterations, sweeps, stagedConfigs, stagedTimes);
+    s.certificateLast = 0;
+    for (uint32_t k = 0; k < substeps; ++k)
+        s.recordSubstep(iterations, sweeps, stagedConfigs, stagedTimes,
+                        k + 1 == substeps && s.extension.certificatePasses != 0);
     HIP_OK(hipGetLastError());
 }
 
@@ -1802,6 +1891,7 @@ void diagnoseSolver(NumiClothHipSolver& s, NumiClothHipDiagnostics* out) {
     out->maximumGroundPenetration = s.initialConfig.constraintCounts.z != 0u ? r.y : 0.0f;
     out->maximumSpherePenetration = r.z;
     out->maximumSelfPenetration = r.w;
+    out->certificatePasses = s.certificateLast;
     if (s.yarnStats.count != 0) {
         uint32_t stats[2] = {0u, 0u};
         s.yarnStats.download(stats, 2, s.stream);
@@ -1977,6 +2067,8 @@ struct AdjointBuffers {
     DeviceArray<float4> gxState, gvState;
     DeviceArray<float> gradInverseMass, gradDamping, gradDistance, gradBend;
     DeviceArray<double> loss;
+    DeviceArray<uint32_t> strainTaped;          // strain passes taped by the last adjoint call
+    uint32_t lastSubsteps = 0, lastPasses = 0;  // of the last adjoint call
     std::vector<uint32_t> trackedFirst, trackedCount;
     std::vector<SpecimenInfo> baseInfo;         // per specimen, no tracked knots
     std::vector<NumiClothBagGPUConfig> hostConfigs;
@@ -2202,6 +2294,7 @@ void setupAdjoint(NumiClothHipBatch& b) {
     A.gradDamping.allocate(s.P);
     A.gradDistance.allocate(std::max(s.D, 1u));
     A.gradBend.allocate(std::max(s.B, 1u));
+    A.strainTaped.allocate(1);
     for (AdjointBuffers::TapeSet& set : A.sets) {
         set.scratchPos.allocate(s.P);
         set.scratchPrev.allocate(s.P);
@@ -2376,6 +2469,9 @@ void runAdjoint(NumiClothHipBatch& b, const NumiClothHipAdjointProblem& pr) {
     HIP_OK(hipMemsetAsync(A.gradDamping.ptr, 0, s.P * sizeof(float), st));
     HIP_OK(hipMemsetAsync(A.gradDistance.ptr, 0, std::max(s.D, 1u) * sizeof(float), st));
     HIP_OK(hipMemsetAsync(A.gradBend.ptr, 0, std::max(s.B, 1u) * sizeof(float), st));
+    HIP_OK(hipMemsetAsync(A.strainTaped.ptr, 0, sizeof(uint32_t), st));
+    A.lastSubsteps = T;
+    A.lastPasses = pr.strainSweeps == 0 ? 0u : 18u * pr.strainSweeps * A.distanceColours;
 
     SpecimenArgs a = specimenArgs(b, pr.iterations, pr.strainSweeps);
     a.trackedTotal = S * Tk;
@@ -2386,6 +2482,7 @@ void runAdjoint(NumiClothHipBatch& b, const NumiClothHipAdjointProblem& pr) {
     a.gradDamping = A.gradDamping.ptr;
     a.gradDistance = A.gradDistance.ptr;
     a.gradBend = A.gradBend.ptr;
+    a.strainTaped = A.strainTaped.ptr;
 
     // Forward over the trajectory on the solver's own state, a checkpoint of
     // (x, v) at the start of every segment.
@@ -2578,6 +2675,16 @@ int numi_cloth_hip_synchronize(NumiClothHipSolver* solver) {
     });
 }
 
+int numi_cloth_hip_launch_mode(NumiClothHipSolver* solver, NumiClothHipLaunchMode* out) {
+    return guarded([&] {
+        if (solver == nullptr || out == nullptr) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "null argument");
+        out->graphs = solver->useGraphs ? 1u : 0u;
+        out->decided = solver->launchDecided ? 1u : 0u;
+        out->probeGraphMilliseconds = solver->launchProbeGraphMs;
+        out->probeDirectMilliseconds = solver->launchProbeDirectMs;
+    });
+}
+
 void* numi_cloth_hip_stream(NumiClothHipSolver* solver) {
     return solver == nullptr ? nullptr : static_cast<void*>(solver->stream);
 }
@@ -2643,6 +2750,19 @@ int numi_cloth_hip_batch_adjoint(NumiClothHipBatch* batch, const NumiClothHipAdj
     return guarded([&] {
         if (batch == nullptr || problem == nullptr) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "null argument");
         runAdjoint(*batch, *problem);
+    });
+}
+
+int numi_cloth_hip_batch_adjoint_stats(NumiClothHipBatch* batch, NumiClothHipAdjointStats* out) {
+    return guarded([&] {
+        if (batch == nullptr || out == nullptr) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "null argument");
+        if (!batch->adjoint.ready) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "no adjoint has run on this batch");
+        AdjointBuffers& A = batch->adjoint;
+        uint32_t taped = 0;
+        A.strainTaped.download(&taped, 1, batch->solver->stream);
+        HIP_OK(hipStreamSynchronize(batch->solver->stream));
+        out->strainPassesTaped = taped;
+        out->strainPasses = static_cast<uint64_t>(A.lastSubsteps) * A.lastPasses * batch->specimens;
     });
 }
 

@@ -290,7 +290,7 @@ bool supported(const NumiClothHipSceneDesc& d) {
     const NumiClothBagGPUConfig& c = *d.config;
     return d.gripCount == 0 && d.knotCount == 0 && d.fruitCount == 0 && d.fruitPairCount == 0 &&
            d.yarnContactCount == 0 && d.extension.staticSphereCount == 0 && d.extension.selfContact == 0 &&
-           d.extension.pointSelfCollision == 0 && d.extension.sweptGround == 0 &&
+           d.extension.pointSelfCollision == 0 && d.extension.sweptGround == 0 && d.extension.certificatePasses == 0 &&
            c.constraintCounts.z == 0u && !(c.airVelocityAndDensity.w > 0.0f) && d.mouthRimCount == 0;
 }
 

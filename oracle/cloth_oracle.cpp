@@ -76,6 +76,7 @@ struct Oracle {
     uint32_t failure = 0;
     uint32_t frictionContacts = 0;
     uint32_t presentContacts = 0, sweptContacts = 0;
+    uint32_t certificatePasses = 0;  // passes of the last certificate loop
     uint32_t selfFrictionContacts = 0;
     uint32_t pointContacts = 0;  // pairs of the last point self-collision pass
     double maximumConeRatio = 0;
@@ -1439,7 +1440,52 @@ struct Oracle {
         ground(c, colliders);
     }
 
-    void substep(const NumiClothBagGPUConfig& c, uint32_t iterations, uint32_t sweeps) {
+    // Residuals of the independent CPU reference's certificate loop, each
+    // max(0, ...) in the device's arithmetic: strain-limit violation
+    // (measureStrainLimitViolation, tools/cloth_bag.cpp:1765-1779), ground
+    // penetration of knots and fruits (measureGroundPenetration, :1880-1894),
+    // fruit/yarn penetration (measureBallYarnPenetration, :3111-3137), segment
+    // self-penetration over the self candidates (measurePrimitiveSelfPenetration,
+    // :4182-4245); true when all four fall below the loop's thresholds
+    // (:5171-5174). residuals() returns them in that order.
+    std::array<R, 4> residuals(const NumiClothBagGPUConfig& c) const {
+        const R r = R(c.clothMaterial.x);
+        R strainMax = R(0), groundMax = R(0), yarnMax = R(0), selfMax = R(0);
+        for (uint32_t k = 0; k < D; ++k) {
+            const mr_uint4 e = d.distances[k].particlesAndColor;
+            strainMax = std::max(strainMax, norm(p[e.y].x - p[e.x].x) - maxLength[k]);
+        }
+        if (c.constraintCounts.z != 0u) {
+            for (const Particle<R>& q : p) groundMax = std::max(groundMax, r - q.x.z);
+            for (const Fruit& f : fruit) groundMax = std::max(groundMax, f.radius - f.x.z);
+        }
+        for (const Fruit& f : fruit) {
+            const R target = f.radius + r;
+            for (uint32_t k = 0; k < D; ++k) {
+                const mr_uint4 e = d.distances[k].particlesAndColor;
+                yarnMax = std::max(yarnMax, target - pointSegment(f.x, p[e.x].x, p[e.y].x).distance);
+            }
+        }
+        if (d.extension.selfContact) {
+            for (const auto& pr : candidates) {
+                const mr_uint4 ea = d.distances[pr.first].particlesAndColor;
+                const mr_uint4 eb = d.distances[pr.second].particlesAndColor;
+                selfMax = std::max(selfMax, R(2) * r - segments(p[ea.x].x, p[ea.y].x, p[eb.x].x, p[eb.y].x).distance);
+            }
+        }
+        return {strainMax, groundMax, yarnMax, selfMax};
+    }
+
+    bool certified(const NumiClothBagGPUConfig& c) const {
+        const std::array<R, 4> v = residuals(c);
+        return static_cast<double>(v[3]) < 1.0e-6 && static_cast<double>(v[2]) < 1.0e-6 &&
+               static_cast<double>(v[1]) < 1.0e-9 && static_cast<double>(v[0]) < 1.0e-6;
+    }
+
+    // certify: the last substep of a call, with extension.certificatePasses
+    // set, runs the data-dependent certificate loop after the schedule's
+    // certificate rounds (tools/cloth_bag.cpp:5143-5185).
+    void substep(const NumiClothBagGPUConfig& c, uint32_t iterations, uint32_t sweeps, bool certify = false) {
         const R dt = R(c.gravityAndTimestep.w);
         time += static_cast<double>(c.gravityAndTimestep.w);
         const NumiClothHipSceneExtension& e = d.extension;
@@ -1483,6 +1529,15 @@ struct Oracle {
                 strain(sweeps);
                 for (int pass = 0; pass < 2; ++pass) finalContacts(c, colliders);
             }
+            if (certify) {
+                for (uint32_t pass = 0; pass < e.certificatePasses; ++pass) {
+                    if (self) selfContact(c, 0);
+                    strain(sweeps);
+                    for (int final = 0; final < 2; ++final) finalContacts(c, colliders);
+                    certificatePasses = pass + 1;
+                    if (certified(c)) break;
+                }
+            }
         }
         buildYarnContacts(c);
         finalize(c);
@@ -1517,7 +1572,7 @@ typedef struct NumiClothOracleOutput {
     uint32_t presentContacts;
     uint32_t sweptContacts;
     uint32_t candidateCount;  /* candidates of the last broad phase */
-    uint32_t reserved;
+    uint32_t certificatePasses; /* passes of the certificate loop at the run's last substep */
     double* fruits;           /* F x 20: the five float4 fields of a fruit record */
     double* fruitPairs;       /* FP x 4: contact (normal impulse vector, impulse) */
     double* yarnContacts;     /* Y x 24: the six float4/uint4 fields after identity */
@@ -1556,7 +1611,8 @@ static int runOracle(const NumiClothHipSceneDesc* scene, const NumiClothBagGPUCo
                      uint32_t iterations, uint32_t sweeps, NumiClothOracleOutput* out) {
     if (scene == nullptr || configs == nullptr || out == nullptr) return 1;
     Oracle<R> o(*scene);
-    for (uint32_t k = 0; k < substeps; ++k) o.substep(configs[k], iterations, sweeps);
+    for (uint32_t k = 0; k < substeps; ++k)
+        o.substep(configs[k], iterations, sweeps, k + 1 == substeps && scene->extension.certificatePasses != 0);
     for (uint32_t i = 0; i < o.P; ++i) {
         const Particle<R>& q = o.p[i];
         if (out->positions) {
@@ -1593,6 +1649,7 @@ static int runOracle(const NumiClothHipSceneDesc* scene, const NumiClothBagGPUCo
     out->presentContacts = o.presentContacts;
     out->sweptContacts = o.sweptContacts;
     out->candidateCount = static_cast<uint32_t>(o.candidates.size());
+    out->certificatePasses = o.certificatePasses;
     for (uint32_t i = 0; i < o.fruit.size() && out->fruits; ++i) {
         const auto& f = o.fruit[i];
         const double r[20] = {f.x.x,    f.x.y,    f.x.z,    f.invMass, f.prev.x, f.prev.y, f.prev.z,
@@ -1679,6 +1736,23 @@ int64_t numi_cloth_oracle_self_candidates_fp32(const NumiClothHipSceneDesc* scen
         pairs[2 * k + 1] = o.candidates[k].second;
     }
     return static_cast<int64_t>(o.candidates.size());
+}
+
+// The certificate loop's four residuals (strain-limit violation, ground,
+// fruit/yarn and segment self-penetration; each max(0, ...)) of the scene's
+// own state under `config`, the self candidates from a broad phase on that
+// state; precision 0 = FP64, 1 = FP32 twin. Returns 0, 1 on a null argument.
+int numi_cloth_oracle_certificate_residuals(const NumiClothHipSceneDesc* scene, const NumiClothBagGPUConfig* config,
+                                            uint32_t precision, double* out) {
+    if (scene == nullptr || config == nullptr || out == nullptr) return 1;
+    auto measure = [&](auto oracle) {
+        if (scene->extension.selfContact) oracle.broadPhase(*config);
+        const auto v = oracle.residuals(*config);
+        for (int k = 0; k < 4; ++k) out[k] = static_cast<double>(v[k]);
+    };
+    if (precision == 1) measure(Oracle<float>(*scene));
+    else measure(Oracle<double>(*scene));
+    return 0;
 }
 
 }  // extern "C"

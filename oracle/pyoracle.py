@@ -21,7 +21,7 @@ class OracleOutput(C.Structure):
                 ("knotLambda", C.c_void_p), ("gripLambda", C.c_void_p), ("gripParticle", C.c_void_p),
                 ("failure", C.c_uint32), ("frictionContacts", C.c_uint32), ("maximumConeRatio", C.c_double),
                 ("presentContacts", C.c_uint32), ("sweptContacts", C.c_uint32), ("candidateCount", C.c_uint32),
-                ("reserved", C.c_uint32), ("fruits", C.c_void_p), ("fruitPairs", C.c_void_p),
+                ("certificatePasses", C.c_uint32), ("fruits", C.c_void_p), ("fruitPairs", C.c_void_p),
                 ("yarnContacts", C.c_void_p), ("mouthClearance", C.c_void_p), ("maximumYarnAeroForce", C.c_double),
                 ("maximumFruitAeroForce", C.c_double), ("maximumFruitAeroTorque", C.c_double),
                 ("candidateMask", C.c_uint32),
@@ -87,6 +87,7 @@ def run(scene, configs: np.ndarray, iterations: int, strain_sweeps: int, precisi
     out["presentContacts"] = int(result.presentContacts)
     out["sweptContacts"] = int(result.sweptContacts)
     out["candidateCount"] = int(result.candidateCount)
+    out["certificatePasses"] = int(result.certificatePasses)
     for name in ["candidateMask", "releasedMask", "pairFrictionContacts", "yarnFrictionContacts",
                  "fruitGroundContacts", "rollingContacts", "selfFrictionContacts", "pointSelfContacts"]:
         out[name] = int(getattr(result, name))
@@ -95,6 +96,25 @@ def run(scene, configs: np.ndarray, iterations: int, strain_sweeps: int, precisi
     # Per candidate of the last substep: s, t, records, sum -n*impulse (3),
     # sum impulse, endpoint shares (4).
     out["selfImpulses"] = impulses[:min(int(result.selfImpulseCount), len(impulses))].copy()
+    return out
+
+
+def certificate_residuals(scene, config=None, precision: str = "fp64") -> np.ndarray:
+    """The certificate loop's residuals of the scene's own state: strain-limit
+    violation, ground penetration, fruit/yarn penetration, segment
+    self-penetration (each max(0, ...); self candidates from a broad phase
+    on that state)."""
+    lib = library()
+    if not hasattr(lib, "_residuals_ready"):
+        lib.numi_cloth_oracle_certificate_residuals.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]
+        lib.numi_cloth_oracle_certificate_residuals.restype = C.c_int
+        lib._residuals_ready = True
+    config = np.ascontiguousarray(scene.config if config is None else config)
+    out = np.zeros(4)
+    status = lib.numi_cloth_oracle_certificate_residuals(C.addressof(scene.desc), config.ctypes.data,
+                                                         1 if precision == "fp32" else 0, out.ctypes.data)
+    if status != 0:
+        raise RuntimeError(f"oracle certificate residuals failed with status {status}")
     return out
 
 

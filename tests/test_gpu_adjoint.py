@@ -139,3 +139,23 @@ def test_checkpoint_interval_does_not_change_gradients(cloth, problem):
         for name in ("distanceCompliance", "bendCompliance", "inverseMass", "damping"):
             a, b = grads[k][name], p["gpu"][k][name]
             assert np.allclose(a, b, rtol=1e-4, atol=1e-6 * np.max(np.abs(b))), name
+
+
+def test_adjoint_reports_the_strain_passes_it_tapes(cloth):
+    """Strips pulled 20 % beyond rest (limit 10 %): the strain limit acts in
+    the first substeps, so some strain passes are taped, and never more than
+    the recomputations ran (substeps x 18 invocations x sweeps x colours x
+    specimens)."""
+    from test_oracle_certificate import _stretched_strip
+    specimens = [_stretched_strip(cloth, 1.2, 6, 0), cloth.Scene.strip(width=5, height=8, seed=4)]
+    merged, _ = cloth.Scene.merge(specimens)
+    colours = int(merged.distanceBatches["control"][:, 2].max()) + 1
+    substeps, sweeps = 32, 2
+    tracked = np.array([[10, 14], [20, 30]], np.uint32)
+    targets = np.zeros((substeps, 2, 2, 3), np.float32)
+    with cloth.Batch(specimens) as batch:
+        batch.adjoint(specimens[0].configs(substeps), 8, sweeps, tracked, targets, 16)
+        stats = batch.adjoint_stats()
+    print(f"strain passes {stats.strainPasses}, taped {stats.strainPassesTaped}")
+    assert stats.strainPasses == substeps * 18 * sweeps * colours * 2
+    assert 0 < stats.strainPassesTaped < stats.strainPasses
