@@ -252,13 +252,13 @@ def cpu_baseline_strips(specimens, count: int) -> dict:
                       f"{seconds:.2f} s on {host['cpu_model']}", **host}
 
 
-def cpu_baseline_adjoint(specimens, local, targets, configs) -> dict:
+def cpu_baseline_adjoint(specimens, local, targets, configs, iterations: int = ITERATIONS) -> dict:
     """FP64 adjoint oracle, one specimen, all 200 substeps, one core."""
     from oracle import pyoracle
     pyoracle.library()
     scene = specimens[0]
     start = time.perf_counter()
-    pyoracle.adjoint(scene, configs if len(configs) else scene.configs(1), ITERATIONS, STRAIN_SWEEPS,
+    pyoracle.adjoint(scene, configs if len(configs) else scene.configs(1), iterations, STRAIN_SWEEPS,
                      local[0], targets[:, 0].astype(np.float64))
     seconds = time.perf_counter() - start
     host = host_cpu()
@@ -599,47 +599,43 @@ def run_adjoint(args, cloth, hip) -> dict:
     substeps = len(configs)
     particles = sum(len(s.particles) for s in specimens)
     start = args.adjoint_start
-    lead = specimens[0].configs(max(start, 16))
-    # Each sample runs a 200-substep window on a fresh batch: substeps
-    # start..start+200 of the strips' trajectory from the seeded state (a call
-    # continues the batch's trajectory; later windows, where the strips swing
-    # far enough for strain limits to act, tape more passes). Plain forward:
-    # the library's step path over the same window, after the untimed
-    # substeps before it (at least 16, which capture its graph).
-    forward_times, line, tile_config = [], None, None
-    for k in range(args.steps):
-        forward = cloth.Batch(specimens)
-        forward.step(lead, ITERATIONS, STRAIN_SWEEPS)
-        forward.synchronize()
-        timer = Timer(hip, cloth.cloth.library().numi_cloth_hip_stream(forward.solver_handle))
-        timer.begin()
-        forward.step(configs, ITERATIONS, STRAIN_SWEEPS)
-        forward_times.append(timer.end())
-        if k == args.steps - 1:
-            view = cloth.Solver.borrow(forward.solver_handle, specimens[0])
-            line, tile_config = roofline(cloth, view, args.workload)
-        forward.close()
+    iterations = ITERATIONS if args.adjoint_iterations is None else args.adjoint_iterations
+    # One trajectory per path, as a calibration loop runs it: the untimed
+    # substeps before `start`, `warm` untimed 200-substep calls (they capture
+    # the forward's graph and size the adjoint's checkpoints and tape), then
+    # `steps` timed calls over the next consecutive 200-substep windows. The
+    # forward is the library's step path over the same windows.
+    warm = max(1, args.warmup)
+    forward = cloth.Batch(specimens)
+    if start:
+        forward.step(specimens[0].configs(start), iterations, STRAIN_SWEEPS)
+    for _ in range(warm):
+        forward.step(configs, iterations, STRAIN_SWEEPS)
+    forward.synchronize()
+    forward_timer = Timer(hip, cloth.cloth.library().numi_cloth_hip_stream(forward.solver_handle))
+    forward_times = []
+    for _ in range(args.steps):
+        forward_timer.begin()
+        forward.step(configs, iterations, STRAIN_SWEEPS)
+        forward_times.append(forward_timer.end())
+    view = cloth.Solver.borrow(forward.solver_handle, specimens[0])
+    line, tile_config = roofline(cloth, view, args.workload)
+    forward.close()
     forward_seconds = sum(forward_times) / len(forward_times)
     # Adjoint: forward with checkpoints, then the reverse; the library call
-    # returns when the gradients are in device memory. Warm-up calls run on a
-    # batch of their own.
-    if args.warmup:
-        batch = cloth.Batch(specimens)
-        for _ in range(args.warmup):
-            batch.adjoint(configs, ITERATIONS, STRAIN_SWEEPS, local, targets, 16)
-        batch.close()
+    # returns when the gradients are in device memory.
+    batch = cloth.Batch(specimens)
+    if start:
+        batch.step(specimens[0].configs(start), iterations, STRAIN_SWEEPS)
+    for _ in range(warm):
+        batch.adjoint(configs, iterations, STRAIN_SWEEPS, local, targets, 16)
+    batch.synchronize()
+    timer = Timer(hip, cloth.cloth.library().numi_cloth_hip_stream(batch.solver_handle))
     adjoint_times = []
-    for k in range(args.steps):
-        batch = cloth.Batch(specimens)
-        if start:
-            batch.step(specimens[0].configs(start), ITERATIONS, STRAIN_SWEEPS)
-        batch.synchronize()
-        timer = Timer(hip, cloth.cloth.library().numi_cloth_hip_stream(batch.solver_handle))
+    for _ in range(args.steps):
         timer.begin()
-        batch.adjoint(configs, ITERATIONS, STRAIN_SWEEPS, local, targets, 16)
+        batch.adjoint(configs, iterations, STRAIN_SWEEPS, local, targets, 16)
         adjoint_times.append(timer.end())
-        if k != args.steps - 1:
-            batch.close()
     adjoint_seconds = sum(adjoint_times) / len(adjoint_times)
     grads = batch.gradients(0)
     stats = batch.adjoint_stats()
@@ -648,12 +644,12 @@ def run_adjoint(args, cloth, hip) -> dict:
     if failure != 0:
         raise RuntimeError(f"device failure bits {failure}")
     config = {"workload": "adjoint64", "specimens": len(specimens), "particles": particles, "substeps": substeps,
-              "checkpoint_interval": 16, "tracked_per_specimen": int(local.shape[1]), "iterations": ITERATIONS,
+              "checkpoint_interval": 16, "tracked_per_specimen": int(local.shape[1]), "iterations": iterations,
               "strain_sweeps": STRAIN_SWEEPS, "forward_seconds": forward_seconds,
               "adjoint_seconds": adjoint_seconds, "adjoint_over_forward": adjoint_seconds / forward_seconds,
-              "window": f"substeps {start}-{start + substeps} of the trajectory from the seeded state, a fresh "
-                        f"batch per sample (forward: substeps {max(start, 16)}-{max(start, 16) + substeps}, after "
-                        "untimed steps that also capture its graph)",
+              "window": f"substeps {start + warm * substeps}-{start + (warm + args.steps) * substeps} of the "
+                        f"trajectory from the seeded state, {args.steps} consecutive calls of {substeps} substeps "
+                        f"after {start + warm * substeps} untimed; the forward over the same substeps",
               "gradient_entries_per_specimen": int(grads["distanceCompliance"].size + grads["bendCompliance"].size
                                                    + grads["inverseMass"].size + grads["damping"].size),
               "loss_specimen0": grads["loss"], "tile_programs": tile_config,
@@ -661,7 +657,7 @@ def run_adjoint(args, cloth, hip) -> dict:
               "strain_acting_fraction": stats.strainPassesTaped / max(1, stats.strainPasses)}
     return {"metric": "adjoint gradients/s", "value": len(specimens) / adjoint_seconds, "unit": "gradients/s",
             "ms_per_step": adjoint_seconds * 1000.0, "dtype": "f32", "config": config, "roofline": line,
-            "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline_adjoint(specimens, local, targets, configs),
+            "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline_adjoint(specimens, local, targets, configs, iterations),
             "data": "synthetic (seeded strips and targets, PCG64 seed %d)" % (SEED + 1)}
 
 
@@ -676,6 +672,10 @@ def main() -> int:
     parser.add_argument("--no-cpu-baseline", action="store_true")
     parser.add_argument("--adjoint-start", type=int, default=0, metavar="SUBSTEPS",
                         help="adjoint64: substeps stepped (untimed) before the timed 200-substep window")
+    parser.add_argument("--adjoint-iterations", type=int, default=None, metavar="N",
+                        help="adjoint64: solver iterations per substep of both the forward and the adjoint "
+                             "(default: the library's 32; few iterations leave the distance constraints "
+                             "unconverged, so the strain limits act and their passes are taped)")
     parser.add_argument("--cell-size", type=float, default=None,
                         help="self-contact hash cell of the sheet workloads in metres (default: the workload's)")
     parser.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",

@@ -12,9 +12,13 @@
 // per-constraint arithmetic is the device functions the per-colour kernels
 // use (projectDistance, projectBend without ground, limitStrainPair), in the
 // same colour order, so the state equals the plain schedule's.
-// In record mode every pass first writes the positions it reads (the tape;
-// a strain pass only when one of its constraints will act) and every
-// iteration the multipliers it starts from.
+// In record mode every distance and bend pass first writes the positions it
+// reads (the tape) and every iteration the multipliers it starts from. A
+// strain pass in which a constraint acts tapes only what it moves: a flag
+// per constraint and, for each constraint that moves, its two endpoints as
+// the pass found them (a colour's constraints share no knot, so the knots'
+// slots of the position tape hold them); the reverse reads the flags and the
+// endpoints of flagged constraints only.
 //
 // Reverse (k_specimen_reverse): replays a segment of substeps backwards from
 // the tape with the hand-derived adjoint of each projection (same formulas as
@@ -74,6 +78,7 @@ struct SpecimenArgs {
     float* tapeLambdaB;            // [substep in launch][even iteration][bends]
     float4* tapeV0;                // [substep in launch][particles]
     uint32_t* tapeActive;          // [substep in launch][pass][specimen]
+    uint8_t* tapeStrainActs;       // [substep in launch][strain pass][distances]: the constraint moved
     uint32_t* strainTaped;         // record mode: strain passes taped (acting), summed over specimens
     float* lossGrad;               // [substep in launch][trackedTotal][3]
     uint32_t specimenCount;
@@ -115,7 +120,6 @@ __device__ __forceinline__ uint32_t strainInvocations(uint32_t sweeps) { return 
 
 __global__ void __launch_bounds__(kSpecimenThreads) k_specimen_forward(SpecimenArgs a) {
     extern __shared__ float4 specimenLds[];
-    __shared__ uint32_t acting;
     __shared__ double lossParts[kSpecimenThreads / 64];
     const uint32_t spec = blockIdx.x;
     const SpecimenInfo info = a.specimens[spec];
@@ -220,41 +224,60 @@ __global__ void __launch_bounds__(kSpecimenThreads) k_specimen_forward(SpecimenA
             }
         }
         const uint32_t invocations = strainInvocations(a.sweeps);
+        const uint32_t strainPasses = invocations * a.sweeps * a.distanceColours;
         for (uint32_t inv = 0; inv < invocations; ++inv) {
             for (uint32_t sweep = 0; sweep < a.sweeps; ++sweep) {
+                bool moved = false;
                 for (uint32_t c = 0; c < a.distanceColours; ++c, ++pass) {
                     const uint4 r = dRanges[c];
-                    if (a.record) {
-                        // Tape only the passes in which a constraint acts.
-                        if (tid == 0) acting = 0u;
-                        __syncthreads();
-                        bool any = false;
-                        for (uint32_t t = tid; t < r.y && !any; t += threads) {
-                            const uint2 e = a.ij[r.x + t];
-                            const float len = length(xyz(x[e.y - k0]) - xyz(x[e.x - k0]));
-                            any = len > a.maxLength[r.x + t] && len > 1.0e-12f;
-                        }
-                        if (any) acting = 1u;
-                        __syncthreads();
-                        const bool act = acting != 0u;
-                        if (tid == 0) {
-                            a.tapeActive[(static_cast<size_t>(j) * a.passes + pass) * a.specimenCount + spec] = act;
-                            if (act) atomicAdd(a.strainTaped, 1u);
-                        }
-                        if (!act) continue;  // uniform: no constraint moves, nothing to tape
-                        snapshot(pass);
-                        __syncthreads();
-                    }
+                    uint8_t* acts = a.record ? a.tapeStrainActs + (static_cast<size_t>(j) * strainPasses +
+                                                                   (pass - (a.passes - strainPasses))) * a.distances
+                                             : nullptr;
+                    float4* xs = a.record ? a.tapeX + (static_cast<size_t>(j) * a.passes + pass) * a.particles : nullptr;
+                    bool taped = false;
                     for (uint32_t t = tid; t < r.y; t += threads) {
                         const uint32_t k = r.x + t;
                         const uint2 e = a.ij[k];
                         float4 pa = x[e.x - k0], pb = x[e.y - k0];
+                        if (a.record) {
+                            // The reverse's test of whether the projection moves the pair.
+                            const float len = length(xyz(pb) - xyz(pa));
+                            const bool moves = len > a.maxLength[k] && len > 1.0e-12f && pa.w + pb.w > 0.0f;
+                            acts[k] = moves ? 1u : 0u;
+                            if (moves) {
+                                xs[e.x] = pa;
+                                xs[e.y] = pb;
+                                taped = true;
+                            }
+                        }
                         if (limitStrainPair(pa, pb, a.maxLength[k], a.failure)) {
                             x[e.x - k0] = pa;
                             x[e.y - k0] = pb;
+                            moved = true;
                         }
                     }
-                    __syncthreads();
+                    if (a.record) {
+                        // The reverse visits only the passes in which a pair moved.
+                        const bool act = __syncthreads_or(taped ? 1 : 0) != 0;
+                        if (tid == 0) {
+                            a.tapeActive[(static_cast<size_t>(j) * a.passes + pass) * a.specimenCount + spec] = act;
+                            if (act) atomicAdd(a.strainTaped, 1u);
+                        }
+                    } else {
+                        __syncthreads();
+                    }
+                }
+                // A sweep that moved no knot leaves the positions the next
+                // sweep of this invocation would check: the remaining sweeps
+                // are no-ops (the strain tile program stops the same way), and
+                // their passes are marked untaped for the reverse.
+                if (__syncthreads_or(moved ? 1 : 0) == 0) {
+                    const uint32_t skipped = (a.sweeps - 1 - sweep) * a.distanceColours;
+                    if (a.record && tid == 0)
+                        for (uint32_t q = 0; q < skipped; ++q)
+                            a.tapeActive[(static_cast<size_t>(j) * a.passes + pass + q) * a.specimenCount + spec] = 0u;
+                    pass += skipped;
+                    break;
                 }
             }
         }
@@ -441,8 +464,10 @@ __global__ void __launch_bounds__(kSpecimenThreads) k_specimen_reverse(SpecimenA
             if (a.tapeActive[(static_cast<size_t>(j) * a.passes + pass) * a.specimenCount + spec] == 0u) continue;
             const uint4 r = dRanges[p % a.distanceColours];
             const float4* xs = tapeAt(pass);
+            const uint8_t* acts = a.tapeStrainActs + (static_cast<size_t>(j) * strainPasses + p) * a.distances;
             for (uint32_t t = tid; t < r.y; t += threads) {
                 const uint32_t k = r.x + t;
+                if (acts[k] == 0u) continue;  // the projection left the pair as it was
                 const uint2 e = a.ij[k];
                 const float4 pa = xs[e.x], pb = xs[e.y];
                 const V3 d = xyz(pb) - xyz(pa);

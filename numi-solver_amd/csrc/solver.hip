@@ -71,6 +71,11 @@ struct DeviceArray {
         if (n == 0) return;
         HIP_OK(hipMalloc(&ptr, n * sizeof(T)));
     }
+    // Allocates when the array holds fewer than n elements (contents are not
+    // kept); a larger array is reused as is.
+    void ensure(std::size_t n) {
+        if (count < n) allocate(n);
+    }
     void upload(const T* host, std::size_t n, hipStream_t stream) {
         if (n == 0) return;
         HIP_OK(hipMemcpyAsync(ptr, host, n * sizeof(T), hipMemcpyHostToDevice, stream));
@@ -2059,6 +2064,7 @@ struct AdjointBuffers {
         DeviceArray<float4> tapeX, tapeV0;
         DeviceArray<float> tapeLambdaD, tapeLambdaB, lossGrad;
         DeviceArray<uint32_t> tapeActive;
+        DeviceArray<uint8_t> tapeStrainActs;
     };
     TapeSet sets[2];
     hipStream_t side = nullptr;                 // default priority, like the solver stream
@@ -2443,24 +2449,28 @@ void runAdjoint(NumiClothHipBatch& b, const NumiClothHipAdjointProblem& pr) {
         info[k].trackedCount = Tk;
     }
     A.info.upload(info.data(), S, st);
-    A.tracked.allocate(std::max<size_t>(tracked.size(), 1));
+    // Buffers grow to the largest call and are reused by later calls.
+    A.tracked.ensure(std::max<size_t>(tracked.size(), 1));
     A.tracked.upload(tracked.data(), tracked.size(), st);
     const size_t targetCount = static_cast<size_t>(T) * S * Tk * 3;
-    A.targets.allocate(std::max<size_t>(targetCount, 1));
+    A.targets.ensure(std::max<size_t>(targetCount, 1));
     A.targets.upload(pr.targets, targetCount, st);
     const uint32_t segments = (T + L - 1) / L;
     const uint32_t passes = adjointPasses(A, pr.iterations, pr.strainSweeps);
-    A.checkpointPos.allocate(static_cast<size_t>(segments) * s.P);
-    A.checkpointVel.allocate(static_cast<size_t>(segments) * s.P);
+    A.checkpointPos.ensure(static_cast<size_t>(segments) * s.P);
+    A.checkpointVel.ensure(static_cast<size_t>(segments) * s.P);
+    A.lastSubsteps = T;
+    A.lastPasses = pr.strainSweeps == 0 ? 0u : 18u * pr.strainSweeps * A.distanceColours;
     const uint32_t setsUsed = segments > 1 ? 2u : 1u;
     for (uint32_t k = 0; k < setsUsed; ++k) {
         AdjointBuffers::TapeSet& set = A.sets[k];
-        set.tapeX.allocate(static_cast<size_t>(L) * passes * s.P);
-        set.tapeV0.allocate(static_cast<size_t>(L) * s.P);
-        set.tapeLambdaD.allocate(std::max<size_t>(static_cast<size_t>(L) * pr.iterations * s.D, 1));
-        set.tapeLambdaB.allocate(std::max<size_t>(static_cast<size_t>(L) * ((pr.iterations + 1) / 2) * s.B, 1));
-        set.tapeActive.allocate(std::max<size_t>(static_cast<size_t>(L) * passes * S, 1));
-        set.lossGrad.allocate(std::max<size_t>(static_cast<size_t>(L) * S * Tk * 3, 1));
+        set.tapeX.ensure(static_cast<size_t>(L) * passes * s.P);
+        set.tapeV0.ensure(static_cast<size_t>(L) * s.P);
+        set.tapeLambdaD.ensure(std::max<size_t>(static_cast<size_t>(L) * pr.iterations * s.D, 1));
+        set.tapeLambdaB.ensure(std::max<size_t>(static_cast<size_t>(L) * ((pr.iterations + 1) / 2) * s.B, 1));
+        set.tapeActive.ensure(std::max<size_t>(static_cast<size_t>(L) * passes * S, 1));
+        set.tapeStrainActs.ensure(std::max<size_t>(static_cast<size_t>(L) * A.lastPasses * s.D, 1));
+        set.lossGrad.ensure(std::max<size_t>(static_cast<size_t>(L) * S * Tk * 3, 1));
     }
     HIP_OK(hipMemsetAsync(A.loss.ptr, 0, S * sizeof(double), st));
     HIP_OK(hipMemsetAsync(A.gxState.ptr, 0, s.P * sizeof(float4), st));
@@ -2470,8 +2480,6 @@ void runAdjoint(NumiClothHipBatch& b, const NumiClothHipAdjointProblem& pr) {
     HIP_OK(hipMemsetAsync(A.gradDistance.ptr, 0, std::max(s.D, 1u) * sizeof(float), st));
     HIP_OK(hipMemsetAsync(A.gradBend.ptr, 0, std::max(s.B, 1u) * sizeof(float), st));
     HIP_OK(hipMemsetAsync(A.strainTaped.ptr, 0, sizeof(uint32_t), st));
-    A.lastSubsteps = T;
-    A.lastPasses = pr.strainSweeps == 0 ? 0u : 18u * pr.strainSweeps * A.distanceColours;
 
     SpecimenArgs a = specimenArgs(b, pr.iterations, pr.strainSweeps);
     a.trackedTotal = S * Tk;
@@ -2536,6 +2544,7 @@ void runAdjoint(NumiClothHipBatch& b, const NumiClothHipAdjointProblem& pr) {
         f.tapeLambdaB = set.tapeLambdaB.ptr;
         f.tapeV0 = set.tapeV0.ptr;
         f.tapeActive = set.tapeActive.ptr;
+        f.tapeStrainActs = set.tapeStrainActs.ptr;
         f.lossGrad = set.lossGrad.ptr;
         k_specimen_forward<<<S, kSpecimenThreads, A.ldsBytes, rs>>>(f);
         HIP_OK(hipGetLastError());

@@ -142,10 +142,12 @@ def test_checkpoint_interval_does_not_change_gradients(cloth, problem):
 
 
 def test_adjoint_reports_the_strain_passes_it_tapes(cloth):
-    """Strips pulled 20 % beyond rest (limit 10 %): the strain limit acts in
-    the first substeps, so some strain passes are taped, and never more than
-    the recomputations ran (substeps x 18 invocations x sweeps x colours x
-    specimens)."""
+    """Strips pulled 20 % beyond rest (limit 10 %) and stepped without
+    distance iterations, so the strain limit acts in the first substeps and
+    some strain passes are taped, never more than the recomputations ran
+    (substeps x 18 invocations x sweeps x colours x specimens). With eight
+    iterations the stiff distance constraints pull the strips back within
+    the limit before any strain pass."""
     from test_oracle_certificate import _stretched_strip
     specimens = [_stretched_strip(cloth, 1.2, 6, 0), cloth.Scene.strip(width=5, height=8, seed=4)]
     merged, _ = cloth.Scene.merge(specimens)
@@ -153,9 +155,39 @@ def test_adjoint_reports_the_strain_passes_it_tapes(cloth):
     substeps, sweeps = 32, 2
     tracked = np.array([[10, 14], [20, 30]], np.uint32)
     targets = np.zeros((substeps, 2, 2, 3), np.float32)
-    with cloth.Batch(specimens) as batch:
-        batch.adjoint(specimens[0].configs(substeps), 8, sweeps, tracked, targets, 16)
+    taped = {}
+    for iterations in (0, 8):
+        with cloth.Batch(specimens) as batch:
+            batch.adjoint(specimens[0].configs(substeps), iterations, sweeps, tracked, targets, 16)
+            stats = batch.adjoint_stats()
+        print(f"{iterations} iterations: strain passes {stats.strainPasses}, taped {stats.strainPassesTaped}")
+        assert stats.strainPasses == substeps * 18 * sweeps * colours * 2
+        taped[iterations] = stats.strainPassesTaped
+    assert 0 < taped[0] < substeps * 18 * sweeps * colours * 2
+    assert taped[8] == 0
+
+
+def test_gradients_when_strain_limits_act(cloth, oracle):
+    """The strained problem (strain limits act in most substeps): GPU
+    gradients equal the FP32 twin's and are as close to FP64 as the twin is;
+    the adjoint reports the strain passes it taped."""
+    p = adjoint_problem.build_strained(cloth)
+    args = (p["merged"], p["configs"], p["iterations"], p["sweeps"], p["tracked"], p["targets"])
+    ref = oracle.adjoint(*args)
+    twin = oracle.adjoint(*args, precision="fp32")
+    targets = p["targets"].reshape(len(p["configs"]), 2, p["local"].shape[1], 3)
+    with cloth.Batch(p["specimens"]) as batch:
+        batch.adjoint(p["configs"], p["iterations"], p["sweeps"], p["local"], targets, 16)
+        gpu = [batch.gradients(k) for k in range(2)]
         stats = batch.adjoint_stats()
-    print(f"strain passes {stats.strainPasses}, taped {stats.strainPassesTaped}")
-    assert stats.strainPasses == substeps * 18 * sweeps * colours * 2
-    assert 0 < stats.strainPassesTaped < stats.strainPasses
+    print(f"strained problem: {stats.strainPassesTaped} of {stats.strainPasses} strain passes taped")
+    assert stats.strainPassesTaped > 0
+    q = dict(p, gpu=gpu, out=ref)
+    for name in NAMES:
+        g = _gpu_full(q, name)
+        e_twin = np.linalg.norm(g - twin[name]) / np.linalg.norm(twin[name])
+        e_gpu = np.linalg.norm(g - ref[name]) / np.linalg.norm(ref[name])
+        e_ref = np.linalg.norm(twin[name] - ref[name]) / np.linalg.norm(ref[name])
+        print(f"{name}: |gpu - twin| {e_twin:.3e}  |gpu - fp64| {e_gpu:.3e}  |twin - fp64| {e_ref:.3e}")
+        assert e_twin < 1e-4
+        assert e_gpu <= 1.5 * e_ref + 1e-4

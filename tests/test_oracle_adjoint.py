@@ -199,6 +199,21 @@ def test_extended_trajectory_is_the_fp64_trajectory(cloth, oracle, problem):
     assert math.fsum(((reference - p["targets"]) ** 2).ravel()) == pytest.approx(loss, rel=1e-13)
 
 
+@pytest.mark.parametrize("specimen", [0, 1])
+def test_gradients_when_strain_limits_act(cloth, oracle, specimen):
+    """Every family of both specimens of a problem whose strain limits act
+    (a strip pulled 50 % beyond rest, one iteration per substep), against
+    extended-precision central differences at h = 1e-6."""
+    p = adjoint_problem.build_strained(cloth)
+    p["out"] = oracle.adjoint(p["merged"], p["configs"], p["iterations"], p["sweeps"], p["tracked"], p["targets"])
+    for family in ("distance", "bend", "density", "damping"):
+        name, n, idx, gradient = _family(p, specimen, family)
+        u = _direction("every", n, idx, gradient, seed=0)
+        analytic = float(np.sum(u * gradient))
+        fd = _fd_terms(oracle, p, 1e-6, "extended", name, u)
+        assert analytic == pytest.approx(fd, rel=1e-4), (family, specimen, analytic, fd)
+
+
 def test_mass_compliance_scaling_identity(cloth, oracle, problem):
     """Scaling every inverse mass and compliance of a specimen by s leaves
     its trajectory unchanged (each projection is homogeneous of degree 0 in
