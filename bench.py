@@ -87,7 +87,8 @@ SHEETS = {
 SHEET_WINDOW_START = 24
 WORKLOADS = sorted([*SHEETS, "bag", "pickup", "strips512", "adjoint64"])
 # Reference Metal pickup replay: 480 frames x 48 substeps of the 1,465-knot
-# bag in 7086.04 s of GPU command time on an Apple M4 Pro.
+# bag in 7086.04 s of GPU command time on an Apple M4 Pro (reference
+# docs/METAL_CLOTH.md:419, first_gpu_seconds).
 PICKUP_REFERENCE = 1465 * 480 * 48 / 7086.04
 
 
