@@ -114,7 +114,8 @@ class Diagnostics(C.Structure):
                 ("selfCandidateCount", C.c_uint32), ("selfCandidateOverflow", C.c_uint32),
                 ("yarnReruns", C.c_uint32), ("yarnCandidatesMax", C.c_uint32),
                 ("selfEntryCount", C.c_uint32), ("selfGridCells", C.c_uint32 * 3),
-                ("pointSelfContacts", C.c_uint32), ("certificatePasses", C.c_uint32)]
+                ("pointSelfContacts", C.c_uint32), ("certificatePasses", C.c_uint32),
+                ("tileRoundsSkipped", C.c_uint64), ("tileRoundsRun", C.c_uint64)]
 
 
 MATERIAL_FIELDS = ["ordinaryNodeMassKg", "hemNodeMassKg", "yarnRadiusM", "axialBodyComplianceMPerN",

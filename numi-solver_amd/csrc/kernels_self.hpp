@@ -62,11 +62,12 @@ struct SelfGrid {
     uint32_t denseCells;     // nx * ny * nz when it fits the per-cell counters, else 0
     uint32_t cellScanLength; // denseCells when this substep sorts by counting, else 0
     uint32_t radixCount;     // sortCount when this substep sorts by radix, else 0
-    uint32_t pad[3];
+    uint32_t activeEntries;  // entries with candidates (run walks), counted by the count pass
+    uint32_t pad[2];
 };
 
-// A sorted cell entry with its segment's box, gathered once per substep so
-// that the candidate passes read one 32-byte record per visited entry:
+// A sorted cell entry with its segment's box, the 32-byte record the
+// candidate walk reads per visited entry (staged in LDS per run window):
 // a = (cell id, segment | start flags << 29, lo.x, lo.y) as bits / floats,
 // b = (lo.z, hi.x, hi.y, hi.z).
 struct SortedEntry {
@@ -258,11 +259,16 @@ __device__ __forceinline__ void forSegmentCells(uint32_t s, const CellRange& r, 
 // a counting sort over the grid's cells (dense grid, every entry fits) or by
 // the LSD radix sort of the (cell, value) pairs written here in emission
 // order. Every thread takes the same decision from the grid and the total.
+// Both write the entries in emission order; the counting sort also keeps,
+// per entry, the slot its cell counter's increment returned (its place
+// among the cell's entries in an arbitrary order; the candidate pass ranks
+// each cell's entries by segment).
 __global__ void k_self_emit_entries(const float4* __restrict__ boxLo, const float4* __restrict__ boxHi,
                                     SelfGrid* __restrict__ grid, const uint32_t* __restrict__ offsets,
                                     const uint32_t* __restrict__ entryTotal, uint32_t capacity,
                                     uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
-                                    uint32_t* __restrict__ cellCounters, uint32_t* __restrict__ failure) {
+                                    uint32_t* __restrict__ cellCounters, uint32_t* __restrict__ cellSlot,
+                                    uint32_t* __restrict__ failure) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     const SelfGrid g = *grid;
     const uint32_t total = *entryTotal;
@@ -279,91 +285,55 @@ __global__ void k_self_emit_entries(const float4* __restrict__ boxLo, const floa
     if (s >= g.segments || !g.valid) return;
     const CellRange r = cellRange(boxLo[s], boxHi[s], g);
     if (r.count == 0) return;
-    if (dense) {
-        forSegmentCells(s, r, g, [&](uint32_t cell, uint32_t) { atomicAdd(&cellCounters[cell], 1u); });
-        return;
-    }
     uint32_t out = offsets[s];
     forSegmentCells(s, r, g, [&](uint32_t cell, uint32_t value) {
         if (out < capacity) {
             keys[out] = cell;
             vals[out] = value;
+            if (dense) cellSlot[out] = atomicAdd(&cellCounters[cell], 1u);
         }
         ++out;
     });
 }
 
-// Counting sort, after the exclusive scan of the per-cell counts: each entry
-// takes the next slot of its cell's run (the order inside a run depends on
-// the atomics' order; the rank pass below restores it). Afterwards
-// cellSlots[c] holds the end of cell c's run, which is the start of c + 1's.
-// The cells are taken eight at a time (linear index -> x fastest, as in
-// forSegmentCells): all eight slot requests are in flight before the first
-// result is used.
-__global__ void k_self_place_entries(const float4* __restrict__ boxLo, const float4* __restrict__ boxHi,
-                                     const SelfGrid* __restrict__ grid, uint32_t* __restrict__ cellSlots,
-                                     uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+// Counting sort placement: every entry to its cell's start (the exclusive
+// scan of the cell counts) plus the slot it drew, so each cell's entries
+// are contiguous and cells ascend (their order within a cell is the
+// arbitrary slot order; the candidate pass ranks them by segment).
+__global__ void k_self_place_entries(const SelfGrid* __restrict__ grid, const uint32_t* __restrict__ cellStarts,
+                                     const uint32_t* __restrict__ keysIn, const uint32_t* __restrict__ valsIn,
+                                     const uint32_t* __restrict__ cellSlot, uint32_t* __restrict__ keysOut,
+                                     uint32_t* __restrict__ valsOut) {
     const SelfGrid g = *grid;
-    if (g.cellScanLength == 0u || s >= g.segments) return;
-    const CellRange r = cellRange(boxLo[s], boxHi[s], g);
-    if (r.count == 0) return;
-    const uint32_t rx = static_cast<uint32_t>(r.hi[0] - r.lo[0] + 1);
-    const uint32_t ry = static_cast<uint32_t>(r.hi[1] - r.lo[1] + 1);
-    constexpr uint32_t kSlots = 8;
-    for (uint32_t base = 0; base < r.count; base += kSlots) {
-        uint32_t cell[kSlots], value[kSlots], at[kSlots];
-#pragma unroll
-        for (uint32_t u = 0; u < kSlots; ++u) {
-            const uint32_t q = min(base + u, r.count - 1u);
-            const uint32_t x = q % rx, y = (q / rx) % ry, z = q / (rx * ry);
-            const uint32_t starts = (x == 0u ? 1u : 0u) | (y == 0u ? 2u : 0u) | (z == 0u ? 4u : 0u);
-            cell[u] = cellId(r.lo[0] + static_cast<int>(x), r.lo[1] + static_cast<int>(y),
-                             r.lo[2] + static_cast<int>(z), g);
-            value[u] = s | (starts << kEntrySegmentBits);
-        }
-#pragma unroll
-        for (uint32_t u = 0; u < kSlots; ++u) at[u] = base + u < r.count ? atomicAdd(&cellSlots[cell[u]], 1u) : 0u;
-#pragma unroll
-        for (uint32_t u = 0; u < kSlots; ++u)
-            if (base + u < r.count) {
-                keys[at[u]] = cell[u];
-                vals[at[u]] = value[u];
-            }
+    if (g.cellScanLength == 0u) return;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < g.sortCount; k += gridDim.x * blockDim.x) {
+        const uint32_t cell = keysIn[k];
+        const uint32_t at = cellStarts[cell] + cellSlot[k];
+        keysOut[at] = cell;
+        valsOut[at] = valsIn[k];
     }
 }
 
-// Final position of each placed entry: its cell's run start plus the number
-// of entries of the run with a smaller segment (a segment enters a cell at
-// most once, so ranks are distinct). This is the order a stable sort by cell
-// of the emission-ordered entries gives. Also writes the gathered records of
-// the candidate walk (entries, sortedIJ: may be null).
-__global__ void k_self_rank_entries(const float4* __restrict__ boxLo, const float4* __restrict__ boxHi,
-                                    const uint2* __restrict__ ij, const SelfGrid* __restrict__ grid,
-                                    const uint32_t* __restrict__ cellEnds, const uint32_t* __restrict__ keysIn,
-                                    const uint32_t* __restrict__ valsIn, uint32_t* __restrict__ keysOut,
-                                    uint32_t* __restrict__ valsOut, SortedEntry* __restrict__ entries,
-                                    uint2* __restrict__ sortedIJ) {
+// Final position of each placed entry in the sorted order (two-hop lists
+// searched in memory): its cell's start plus the number of entries of the
+// cell with a smaller segment (a segment enters a cell at most once, so
+// ranks are distinct). This is the order a stable sort by cell of the
+// emission-ordered entries gives.
+__global__ void k_self_rank_entries(const SelfGrid* __restrict__ grid, const uint32_t* __restrict__ cellStarts,
+                                    const uint32_t* __restrict__ keysIn, const uint32_t* __restrict__ valsIn,
+                                    uint32_t* __restrict__ keysOut, uint32_t* __restrict__ valsOut) {
     const SelfGrid g = *grid;
     if (g.cellScanLength == 0u) return;
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < g.sortCount; k += gridDim.x * blockDim.x) {
         const uint32_t cell = keysIn[k];
         const uint32_t v = valsIn[k];
         const uint32_t s = v & kEntrySegmentMask;
-        const uint32_t begin = cell == 0u ? 0u : cellEnds[cell - 1u];
-        const uint32_t end = cellEnds[cell];
+        const uint32_t begin = cellStarts[cell];
+        const uint32_t end = cell + 1u < g.cellScanLength ? cellStarts[cell + 1u] : g.sortCount;
         uint32_t at = begin;
         for (uint32_t j = begin; j < end; ++j) at += (valsIn[j] & kEntrySegmentMask) < s ? 1u : 0u;
         keysOut[at] = cell;
         valsOut[at] = v;
-        if (entries != nullptr) {
-            const float4 lo = boxLo[s], hi = boxHi[s];
-            SortedEntry e;
-            e.a = make_float4(__uint_as_float(cell), __uint_as_float(v), lo.x, lo.y);
-            e.b = make_float4(lo.z, hi.x, hi.y, hi.z);
-            entries[at] = e;
-            sortedIJ[at] = ij[s];
-        }
     }
 }
 
@@ -424,96 +394,115 @@ __device__ void forEntryCandidates(uint32_t k, const SelfGrid& g, const float4* 
     }
 }
 
-// Candidate enumeration over the gathered sorted entries (one 32-byte record
-// per entry, read four at a time so their loads overlap) with the two-hop
-// knot lists of both endpoints held in registers (W slots each, padded with
-// ~0u): the locality test of a pair is 4W register compares instead of four
-// binary searches in memory. Same set and order of candidates as
-// forEntryCandidates. entryAt(j) / ijAt(j) return sorted entry j's record
-// and its segment's endpoints (from global memory, or from an LDS copy).
+// Whether knot x or y is in either two-hop list: one compare per slot and
+// knot, the results OR-ed as wave masks (scalar instructions) and read back
+// at this lane's bit.
+template <uint32_t W>
+__device__ __forceinline__ bool inLists(const uint32_t (&na)[W], const uint32_t (&nb)[W], uint32_t x, uint32_t y) {
+    unsigned long long hits = 0;
+#pragma unroll
+    for (uint32_t m = 0; m < W; ++m)
+        hits |= __ballot(na[m] == x) | __ballot(na[m] == y) | __ballot(nb[m] == x) | __ballot(nb[m] == y);
+    return ((hits >> __lane_id()) & 1ull) != 0ull;
+}
+
+// The cyclic half of a cell run that sorted entry i walks: with the run at
+// [r0, r1) (L entries) and i at offset q, the h(q) entries after i, wrapping
+// from r1 - 1 to r0, with h = (L - 1) / 2, plus one for the first L / 2
+// entries when L is even. Every unordered pair of the run lies in exactly
+// one of the two entries' halves (their offsets differ by d and L - d, and
+// exactly one of those is below L / 2, or both equal L / 2 and only the
+// earlier entry takes that step), and every entry of a run walks the same
+// number of records up to one, so the lanes of a wave, which hold
+// consecutive entries, stay together (walking the later entries of the run
+// instead leaves half of each wave idle on average).
+__device__ __forceinline__ uint32_t runHalf(uint32_t q, uint32_t L) {
+    return (L - 1u) / 2u + ((L & 1u) == 0u && q < L / 2u ? 1u : 0u);
+}
+
+// Candidates of sorted entry i over its cyclic half run (the pairs of a cell
+// run are the same set as over the later entries of the run, so every
+// candidate is still found exactly once, from its owner cell, as (min, max)).
+// The two-hop knot lists of both endpoints (W slots each, padded with ~0u)
+// are requested before the walk and tested only afterwards: the walk over
+// the run's records (owner cell, box overlap, shared knot) marks the
+// partners left for the locality test in a 64-bit mask, so the lists'
+// latency overlaps it and every lane requests them once, together.
+// entryAt(j) / ijAt(j) return sorted entry j's record and its segment's
+// endpoints (from an LDS copy or global memory). Candidates come out in
+// walk order (increasing step), so the list is identical on every run.
 template <uint32_t W, typename EntryAt, typename IJAt, typename Emit>
-__device__ void forSortedEntryCandidates(uint32_t k, const SelfGrid& g, EntryAt entryAt, IJAt ijAt,
-                                         const uint32_t* __restrict__ localFixed, Emit emit) {
-    const uint32_t n = g.sortCount;
-    const SortedEntry self = entryAt(k);
-    const uint32_t id = __float_as_uint(self.a.x);
+__device__ __forceinline__ void forRunCandidates(uint32_t i, uint32_t r0, uint32_t r1, EntryAt entryAt, IJAt ijAt,
+                                                 const uint32_t* __restrict__ localFixed, Emit emit) {
+    const SortedEntry self = entryAt(i);
     const uint32_t vs = __float_as_uint(self.a.y);
     const uint32_t s = vs & kEntrySegmentMask;
     const float4 lo = make_float4(self.a.z, self.a.w, self.b.x, 0.0f);
     const float4 hi = make_float4(self.b.y, self.b.z, self.b.w, 0.0f);
-    const uint2 es = ijAt(k);
+    const uint2 es = ijAt(i);
+    const uint32_t L = r1 - r0;
+    const uint32_t h = runHalf(i - r0, L);
+    if (h == 0u) return;
     uint32_t na[W], nb[W];
-    bool loaded = false;
+    {
+        const uint4* ra = reinterpret_cast<const uint4*>(localFixed + static_cast<size_t>(es.x) * W);
+        const uint4* rb = reinterpret_cast<const uint4*>(localFixed + static_cast<size_t>(es.y) * W);
+#pragma unroll
+        for (uint32_t m = 0; m < W / 4; ++m) {
+            const uint4 va = ra[m], vb = rb[m];
+            na[4 * m] = va.x;
+            na[4 * m + 1] = va.y;
+            na[4 * m + 2] = va.z;
+            na[4 * m + 3] = va.w;
+            nb[4 * m] = vb.x;
+            nb[4 * m + 1] = vb.y;
+            nb[4 * m + 2] = vb.z;
+            nb[4 * m + 3] = vb.w;
+        }
+    }
     constexpr uint32_t kUnroll = 4;
-    for (uint32_t j0 = k + 1; j0 < n; j0 += kUnroll) {
-        SortedEntry e[kUnroll];
+    for (uint32_t d0 = 0; d0 < h; d0 += 64u) {
+        const uint32_t dn = min(64u, h - d0);
+        uint64_t pending = 0;
+        // j = the partner at step d0 + d + 1 (i + d0 + 1 < r1 + L, so one
+        // wrap suffices).
+        uint32_t j = i + d0 + 1u;
+        if (j >= r1) j -= L;
+        for (uint32_t d = 0; d < dn; d += kUnroll) {
+            SortedEntry e[kUnroll];
+            uint2 et[kUnroll];
 #pragma unroll
-        for (uint32_t u = 0; u < kUnroll; ++u)
-            if (j0 + u < n) e[u] = entryAt(j0 + u);
-        bool stop = false;
-#pragma unroll
-        for (uint32_t u = 0; u < kUnroll; ++u) {
-            if (stop) continue;
-            const uint32_t j = j0 + u;
-            if (j >= n || __float_as_uint(e[u].a.x) != id) {
-                stop = true;
-                continue;
-            }
-            const uint32_t vt = __float_as_uint(e[u].a.y);
-            if (((vs | vt) >> kEntrySegmentBits) != 7u) continue;  // not the owner cell
-            const float4 tLo = make_float4(e[u].a.z, e[u].a.w, e[u].b.x, 0.0f);
-            const float4 tHi = make_float4(e[u].b.y, e[u].b.z, e[u].b.w, 0.0f);
-            if (!boxesOverlap(lo, hi, tLo, tHi)) continue;
-            const uint2 et = ijAt(j);
-            // A shared knot is in both two-hop lists (each list holds its own
-            // knot): local without reading them.
-            if (et.x == es.x || et.x == es.y || et.y == es.x || et.y == es.y) continue;
-            if (!loaded) {
-                const uint4* ra = reinterpret_cast<const uint4*>(localFixed + static_cast<size_t>(es.x) * W);
-                const uint4* rb = reinterpret_cast<const uint4*>(localFixed + static_cast<size_t>(es.y) * W);
-#pragma unroll
-                for (uint32_t m = 0; m < W / 4; ++m) {
-                    const uint4 va = ra[m], vb = rb[m];
-                    na[4 * m] = va.x;
-                    na[4 * m + 1] = va.y;
-                    na[4 * m + 2] = va.z;
-                    na[4 * m + 3] = va.w;
-                    nb[4 * m] = vb.x;
-                    nb[4 * m + 1] = vb.y;
-                    nb[4 * m + 2] = vb.z;
-                    nb[4 * m + 3] = vb.w;
+            for (uint32_t u = 0; u < kUnroll; ++u) {
+                if (d + u < dn) {
+                    e[u] = entryAt(j);
+                    et[u] = ijAt(j);
                 }
-                loaded = true;
+                if (++j == r1) j = r0;
             }
-            bool local = false;
 #pragma unroll
-            for (uint32_t m = 0; m < W; ++m)
-                local |= na[m] == et.x || na[m] == et.y || nb[m] == et.x || nb[m] == et.y;
-            if (local) continue;
-            const uint32_t t = vt & kEntrySegmentMask;
+            for (uint32_t u = 0; u < kUnroll; ++u) {
+                if (d + u >= dn) continue;
+                const uint32_t vt = __float_as_uint(e[u].a.y);
+                if (((vs | vt) >> kEntrySegmentBits) != 7u) continue;  // not the owner cell
+                const float4 tLo = make_float4(e[u].a.z, e[u].a.w, e[u].b.x, 0.0f);
+                const float4 tHi = make_float4(e[u].b.y, e[u].b.z, e[u].b.w, 0.0f);
+                if (!boxesOverlap(lo, hi, tLo, tHi)) continue;
+                // A shared knot is in both two-hop lists (each list holds its
+                // own knot): local without reading them.
+                if (et[u].x == es.x || et[u].x == es.y || et[u].y == es.x || et[u].y == es.y) continue;
+                pending |= 1ull << (d + u);
+            }
+        }
+        while (pending != 0ull) {
+            const uint32_t d = static_cast<uint32_t>(__builtin_ctzll(pending));
+            pending &= pending - 1ull;
+            uint32_t jt = i + d0 + d + 1u;
+            if (jt >= r1) jt -= L;
+            const uint2 et = ijAt(jt);
+            if (inLists<W>(na, nb, et.x, et.y)) continue;
+            const uint32_t t = __float_as_uint(entryAt(jt).a.y) & kEntrySegmentMask;
             emit(min(s, t), max(s, t));
         }
-        if (stop) break;
-    }
-}
-
-// Sorted-order records of each entry (cell, value, box) and its segment's
-// endpoints.
-__global__ void k_self_gather(const float4* __restrict__ boxLo, const float4* __restrict__ boxHi,
-                              const uint2* __restrict__ ij, const SelfGrid* __restrict__ grid,
-                              const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
-                              SortedEntry* __restrict__ entries, uint2* __restrict__ sortedIJ) {
-    if (!grid->valid) return;
-    const uint32_t n = grid->radixCount;  // 0 when the counting sort wrote the records
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-        const uint32_t v = vals[k];
-        const uint32_t s = v & kEntrySegmentMask;
-        const float4 lo = boxLo[s], hi = boxHi[s];
-        SortedEntry e;
-        e.a = make_float4(__uint_as_float(keys[k]), __uint_as_float(v), lo.x, lo.y);
-        e.b = make_float4(lo.z, hi.x, hi.y, hi.z);
-        entries[k] = e;
-        sortedIJ[k] = ij[s];
     }
 }
 
@@ -521,11 +510,14 @@ struct PairArgs {
     const float4* boxLo;
     const float4* boxHi;
     const uint2* ij;
-    const SortedEntry* entries;
-    const uint2* sortedIJ;
     SelfGrid* grid;
-    const uint32_t* keys;
-    const uint32_t* vals;
+    // The sorted entries (cell id, segment | start flags): the radix sort
+    // leaves them here; the counting sort's candidate pass writes them here
+    // from the placed entries, whose cells are contiguous but unranked.
+    uint32_t* keys;
+    uint32_t* vals;
+    const uint32_t* placedKeys;
+    const uint32_t* placedVals;
     const uint32_t* localStart;
     const uint32_t* localList;
     const uint32_t* localFixed;
@@ -536,79 +528,63 @@ struct PairArgs {
     uint2* pairs;
     NumiClothBagGPUSelfImpulse* impulses;  // per-candidate impulse aggregates, cleared here
     uint32_t* failure;
+    // Run walks (W > 0): entries with candidates, appended by the count pass
+    // as (entry, run start, run end) in any order (each entry's candidates
+    // go to its own scanned offsets, so the order does not matter), and the
+    // list's capacity.
+    uint4* active;
+    uint32_t activeCapacity;
 };
 
-// W = 0: two-hop lists searched in memory (any mesh); W = 16 or 32: the
-// register form (meshes whose two-hop lists fit W slots).
-template <uint32_t W, typename Emit>
-__device__ __forceinline__ void entryCandidates(uint32_t k, const SelfGrid& g, const PairArgs& a, Emit emit) {
-    if constexpr (W == 0) {
-        forEntryCandidates(k, g, a.boxLo, a.boxHi, a.ij, a.keys, a.vals, a.localStart, a.localList, emit);
-    } else {
-        const SortedEntry* __restrict__ entries = a.entries;
-        const uint2* __restrict__ sortedIJ = a.sortedIJ;
-        forSortedEntryCandidates<W>(
-            k, g, [&](uint32_t j) { return entries[j]; }, [&](uint32_t j) { return sortedIJ[j]; }, a.localFixed,
-            emit);
-    }
+// Sorted entry j's record, read from the sorted entries and its segment's box.
+__device__ __forceinline__ SortedEntry sortedRecord(const PairArgs& a, uint32_t j) {
+    const uint32_t v = a.vals[j];
+    const uint32_t s = v & kEntrySegmentMask;
+    const float4 lo = a.boxLo[s], hi = a.boxHi[s];
+    SortedEntry e;
+    e.a = make_float4(__uint_as_float(a.keys[j]), __uint_as_float(v), lo.x, lo.y);
+    e.b = make_float4(lo.z, hi.x, hi.y, hi.z);
+    return e;
 }
 
-constexpr uint32_t kPairThreads = 256;  // workgroup size of k_self_pair_counts
+constexpr uint32_t kPairThreads = 256;  // workgroup size of the candidate count passes
 
-// Candidate count per sorted entry. Chunk > 0 (W > 0): each workgroup
-// copies chunks of Chunk consecutive sorted records (and their endpoints)
-// to LDS once and walks them there, so a record is read from memory once
-// instead of once per earlier entry of its cell run; runs that cross the
-// chunk's end continue in global memory. Used with 4 entries per thread on
-// scenes of tens of millions of entries, where it is faster; smaller scenes
-// (Chunk = 0) keep one thread per entry walking global memory, which keeps
-// more walks in flight.
-template <uint32_t W, uint32_t Chunk>
+// Emits a candidate at out (the entry's next scanned offset) and clears its
+// impulse aggregate; slots past the capacity are dropped (a RANGE failure).
+__device__ __forceinline__ void emitCandidate(const PairArgs& a, uint32_t& out, uint32_t p, uint32_t q) {
+    if (out < a.capacity) {
+        a.pairs[out] = make_uint2(p, q);
+        NumiClothBagGPUSelfImpulse empty{};
+        empty.identity = mr_uint4{out, p, q, 0u};
+        a.impulses[out] = empty;
+    }
+    ++out;
+}
+
+__device__ __forceinline__ void publishPairTotal(const PairArgs& a, const SelfGrid& g) {
+    const uint32_t total = g.valid ? *a.pairTotal : 0u;
+    a.grid->pairTotal = total;
+    a.grid->pairCount = min(total, a.capacity);
+    if (total > a.capacity) recordFailure(a.failure, NUMI_CLOTH_BAG_GPU_FAILURE_RANGE);
+}
+
+// Two-hop lists searched in memory (meshes whose lists exceed 32 knots):
+// one thread per entry, walking the later entries of its run.
 __global__ void __launch_bounds__(kPairThreads) k_self_pair_counts(PairArgs a) {
     const SelfGrid g = *a.grid;
     if (!g.valid) return;
-    if constexpr (W == 0 || Chunk == 0) {
-        for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < g.sortCount; k += gridDim.x * blockDim.x) {
-            uint32_t n = 0;
-            entryCandidates<W>(k, g, a, [&](uint32_t, uint32_t) { ++n; });
-            a.counts[k] = n;
-        }
-    } else {
-        __shared__ SortedEntry chunk[Chunk];
-        __shared__ uint2 chunkIJ[Chunk];
-        const SortedEntry* __restrict__ entries = a.entries;
-        const uint2* __restrict__ sortedIJ = a.sortedIJ;
-        const uint32_t n = g.sortCount;
-        for (uint32_t c0 = blockIdx.x * Chunk; c0 < n; c0 += gridDim.x * Chunk) {
-            const uint32_t cn = min(Chunk, n - c0);
-            for (uint32_t i = threadIdx.x; i < cn; i += blockDim.x) {
-                chunk[i] = entries[c0 + i];
-                chunkIJ[i] = sortedIJ[c0 + i];
-            }
-            __syncthreads();
-            const auto entryAt = [&](uint32_t j) { return j - c0 < cn ? chunk[j - c0] : entries[j]; };
-            const auto ijAt = [&](uint32_t j) { return j - c0 < cn ? chunkIJ[j - c0] : sortedIJ[j]; };
-            for (uint32_t i = threadIdx.x; i < cn; i += blockDim.x) {
-                uint32_t count = 0;
-                forSortedEntryCandidates<W>(c0 + i, g, entryAt, ijAt, a.localFixed,
-                                            [&](uint32_t, uint32_t) { ++count; });
-                a.counts[c0 + i] = count;
-            }
-            __syncthreads();
-        }
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < g.sortCount; k += gridDim.x * blockDim.x) {
+        uint32_t n = 0;
+        forEntryCandidates(k, g, a.boxLo, a.boxHi, a.ij, a.keys, a.vals, a.localStart, a.localList,
+                           [&](uint32_t, uint32_t) { ++n; });
+        a.counts[k] = n;
     }
 }
 
-template <uint32_t W>
 __global__ void k_self_emit_pairs(PairArgs a) {
     const SelfGrid g = *a.grid;
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k == 0) {
-        const uint32_t total = g.valid ? *a.pairTotal : 0u;
-        a.grid->pairTotal = total;
-        a.grid->pairCount = min(total, a.capacity);
-        if (total > a.capacity) recordFailure(a.failure, NUMI_CLOTH_BAG_GPU_FAILURE_RANGE);
-    }
+    if (k == 0) publishPairTotal(a, g);
     if (!g.valid) return;
     for (uint32_t e = k; e < g.sortCount; e += gridDim.x * blockDim.x) {
         uint32_t out = a.offsets[e];
@@ -616,15 +592,235 @@ __global__ void k_self_emit_pairs(PairArgs a) {
         // candidate (almost all of them) has nothing to emit.
         const uint32_t end = e + 1 < g.sortCount ? a.offsets[e + 1] : *a.pairTotal;
         if (end == out) continue;
-        entryCandidates<W>(e, g, a, [&](uint32_t p, uint32_t q) {
-            if (out < a.capacity) {
-                a.pairs[out] = make_uint2(p, q);
-                NumiClothBagGPUSelfImpulse empty{};
-                empty.identity = mr_uint4{out, p, q, 0u};
-                a.impulses[out] = empty;
+        forEntryCandidates(e, g, a.boxLo, a.boxHi, a.ij, a.keys, a.vals, a.localStart, a.localList,
+                           [&](uint32_t p, uint32_t q) { emitCandidate(a, out, p, q); });
+    }
+}
+
+// Run walks with the two-hop lists in registers (W = 16 or 32 slots).
+// A workgroup owns the cell runs that start in a chunk of kRunChunk sorted
+// entries and stages the window of kRunWindow records from the chunk's start
+// (entries and their segments' endpoints) in LDS, which holds every owned
+// run unless a run is longer than the window's tail; records past the
+// window are read from global memory.
+constexpr uint32_t kRunWindow = 768;
+constexpr uint32_t kRunChunk = 640;
+constexpr uint32_t kRunSlots = kRunWindow / kPairThreads;  // window slots per thread, contiguous
+static_assert(kRunSlots * kPairThreads == kRunWindow && kRunChunk < kRunWindow, "run window layout");
+
+// The end of the run of key `key` that continues past `from` (the keys of
+// the sorted or placed entries), found by the whole workgroup 256 entries at
+// a time.
+__device__ uint32_t runEndFrom(const uint32_t* __restrict__ keys, uint32_t n, uint32_t from, uint32_t key,
+                               uint32_t* found) {
+    for (uint32_t base = from;; base += kPairThreads) {
+        if (threadIdx.x == 0) *found = ~0u;
+        __syncthreads();
+        const uint32_t j = base + threadIdx.x;
+        if (j >= n || keys[j] != key) atomicMin(found, j);
+        __syncthreads();
+        const uint32_t end = *found;
+        __syncthreads();
+        if (end != ~0u) return min(end, n);
+    }
+}
+
+// Candidate counts of every entry of the owned runs; entries with candidates
+// are appended to the active list with their run bounds. The input is the
+// placed entries of the counting sort (each cell's entries contiguous, in
+// slot order) or the radix sort's sorted entries: each entry of an owned run
+// is ranked by segment within its run (a segment enters a cell once, so
+// ranks are distinct; for sorted input the rank is the entry's own place),
+// its record (cell, value, box, endpoints) goes to LDS at that place, and
+// the counting sort's sorted entries are written out, so the walk sees the
+// order of a stable sort by cell whichever sort ran.
+template <uint32_t W>
+__global__ void __launch_bounds__(kPairThreads) k_self_run_counts(PairArgs a) {
+    // First the window's keys and values, then the ranked records.
+    __shared__ SortedEntry rec[kRunWindow];
+    __shared__ uint2 recIJ[kRunWindow];
+    __shared__ uint32_t runStart[kRunWindow];  // absolute start of the window's k-th run
+    __shared__ uint16_t runOf[kRunWindow];     // run index of a window slot (0xffff: run began before the chunk)
+    __shared__ uint32_t scratch[kPairThreads / 64];
+    __shared__ uint32_t info[3];  // runs in the window, (unused), run end search
+    static_assert(2 * sizeof(uint32_t) * kRunWindow <= sizeof(rec), "window keys and values alias the records");
+    uint32_t* const winKey = reinterpret_cast<uint32_t*>(rec);
+    uint32_t* const winVal = winKey + kRunWindow;
+    const SelfGrid g = *a.grid;
+    if (!g.valid) return;
+    const uint32_t n = g.sortCount;
+    const bool counting = g.cellScanLength != 0u;
+    const uint32_t* __restrict__ keysIn = counting ? a.placedKeys : a.keys;
+    const uint32_t* __restrict__ valsIn = counting ? a.placedVals : a.vals;
+    const uint32_t t = threadIdx.x;
+    for (uint32_t c0 = blockIdx.x * kRunChunk; c0 < n; c0 += gridDim.x * kRunChunk) {
+        const uint32_t wn = min(kRunWindow, n - c0);
+        const uint32_t k0 = t * kRunSlots;
+        uint32_t keys[kRunSlots], vals[kRunSlots];
+#pragma unroll
+        for (uint32_t u = 0; u < kRunSlots; ++u) {
+            const uint32_t k = k0 + u;
+            keys[u] = k < wn ? keysIn[c0 + k] : 0u;
+            vals[u] = k < wn ? valsIn[c0 + k] : 0u;
+            if (k < wn) {
+                winKey[k] = keys[u];
+                winVal[k] = vals[u];
             }
-            ++out;
-        });
+        }
+        const uint32_t keyBefore = c0 == 0u ? ~0u : keysIn[c0 - 1u];
+        __syncthreads();
+        // Run starts of the window: a block-wide scan of the start flags
+        // numbers the runs and gives every slot its run.
+        uint32_t flags = 0, count = 0;
+#pragma unroll
+        for (uint32_t u = 0; u < kRunSlots; ++u) {
+            const uint32_t k = k0 + u;
+            if (k >= wn) continue;
+            const uint32_t before = k == 0u ? keyBefore : (u == 0u ? winKey[k - 1u] : keys[u - 1u]);
+            if (keys[u] != before) {
+                flags |= 1u << u;
+                ++count;
+            }
+        }
+        uint32_t r = blockExclusiveScan256(count, scratch);
+#pragma unroll
+        for (uint32_t u = 0; u < kRunSlots; ++u) {
+            const uint32_t k = k0 + u;
+            if (k >= wn) continue;
+            if ((flags >> u) & 1u) runStart[r++] = c0 + k;
+            runOf[k] = r == 0u ? static_cast<uint16_t>(0xffffu) : static_cast<uint16_t>(r - 1u);
+        }
+        if (t == kPairThreads - 1u) info[0] = r;
+        __syncthreads();
+        const uint32_t runs = info[0];
+        // Owned runs start in the chunk's first kRunChunk slots.
+        uint32_t owned = runs;
+        if (wn > kRunChunk) owned = runOf[kRunChunk - 1u] == 0xffffu ? 0u : runOf[kRunChunk - 1u] + 1u;
+        if (owned == 0u) {
+            __syncthreads();
+            continue;
+        }
+        const uint32_t first = runStart[0];
+        uint32_t ownedEnd;
+        if (owned < runs) {
+            ownedEnd = runStart[owned];
+        } else if (c0 + wn == n) {
+            ownedEnd = n;
+        } else {
+            // The last owned run reaches past the window (a run longer than
+            // the window's tail): find its end in global memory.
+            ownedEnd = runEndFrom(keysIn, n, c0 + wn, winKey[runStart[owned - 1u] - c0], &info[2]);
+        }
+        // Entries from `giant` on belong to a run that reaches past the
+        // window; it is ranked and walked in global memory.
+        const uint32_t giant = ownedEnd > c0 + wn ? runStart[owned - 1u] : ownedEnd;
+        uint32_t place[kRunSlots];
+#pragma unroll
+        for (uint32_t u = 0; u < kRunSlots; ++u) {
+            const uint32_t k = k0 + u;
+            const uint32_t i = c0 + k;
+            place[u] = ~0u;
+            if (k >= wn || i < first || i >= giant) continue;
+            const uint32_t run = runOf[k];
+            const uint32_t r0 = runStart[run];
+            const uint32_t r1 = run + 1u < runs ? runStart[run + 1u] : ownedEnd;
+            const uint32_t seg = vals[u] & kEntrySegmentMask;
+            uint32_t rank = 0;
+            for (uint32_t j = r0 - c0; j < r1 - c0; ++j) rank += (winVal[j] & kEntrySegmentMask) < seg ? 1u : 0u;
+            place[u] = r0 + rank;
+        }
+        __syncthreads();  // the window's keys and values are read no more
+#pragma unroll
+        for (uint32_t u = 0; u < kRunSlots; ++u) {
+            if (place[u] == ~0u) continue;
+            const uint32_t seg = vals[u] & kEntrySegmentMask;
+            const float4 lo = a.boxLo[seg], hi = a.boxHi[seg];
+            SortedEntry e;
+            e.a = make_float4(__uint_as_float(keys[u]), __uint_as_float(vals[u]), lo.x, lo.y);
+            e.b = make_float4(lo.z, hi.x, hi.y, hi.z);
+            rec[place[u] - c0] = e;
+            recIJ[place[u] - c0] = a.ij[seg];
+            if (counting) {
+                a.keys[place[u]] = keys[u];
+                a.vals[place[u]] = vals[u];
+            }
+        }
+        if (giant < ownedEnd && counting) {
+            for (uint32_t i = giant + t; i < ownedEnd; i += kPairThreads) {
+                const uint32_t v = valsIn[i];
+                const uint32_t seg = v & kEntrySegmentMask;
+                uint32_t rank = 0;
+                for (uint32_t j = giant; j < ownedEnd; ++j) rank += (valsIn[j] & kEntrySegmentMask) < seg ? 1u : 0u;
+                a.keys[giant + rank] = keysIn[i];
+                a.vals[giant + rank] = v;
+            }
+        }
+        __syncthreads();
+        const auto ldsEntry = [&](uint32_t j) { return rec[j - c0]; };
+        const auto ldsIJ = [&](uint32_t j) { return recIJ[j - c0]; };
+        const auto globalEntry = [&](uint32_t j) { return sortedRecord(a, j); };
+        const auto globalIJ = [&](uint32_t j) { return a.ij[a.vals[j] & kEntrySegmentMask]; };
+        for (uint32_t i = first + t; i < ownedEnd; i += kPairThreads) {
+            uint32_t r0, r1;
+            if (i >= giant) {
+                r0 = giant;
+                r1 = ownedEnd;
+            } else {
+                const uint32_t run = runOf[i - c0];
+                r0 = runStart[run];
+                r1 = run + 1u < runs ? runStart[run + 1u] : ownedEnd;
+            }
+            uint32_t found = 0;
+            const auto tally = [&](uint32_t, uint32_t) { ++found; };
+            if (i < giant) forRunCandidates<W>(i, r0, r1, ldsEntry, ldsIJ, a.localFixed, tally);
+            else forRunCandidates<W>(i, r0, r1, globalEntry, globalIJ, a.localFixed, tally);
+            a.counts[i] = found;
+            if (found != 0u) {
+                const uint32_t slot = atomicAdd(&a.grid->activeEntries, 1u);
+                if (slot < a.activeCapacity) a.active[slot] = make_uint4(i, r0, r1, 0u);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Candidates of the active entries at their scanned offsets, from the
+// sorted entries. Should the active list have overflowed (more entries with
+// candidates than the list holds, which implies more candidates than their
+// capacity, a RANGE failure), every entry with candidates is walked, its
+// run bounds read from the keys, so the emitted list is the same.
+template <uint32_t W>
+__global__ void __launch_bounds__(kScanThreads) k_self_emit_run_pairs(PairArgs a) {
+    const SelfGrid g = *a.grid;
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k == 0) publishPairTotal(a, g);
+    if (!g.valid) return;
+    const auto entryAt = [&](uint32_t j) { return sortedRecord(a, j); };
+    const auto ijAt = [&](uint32_t j) { return a.ij[a.vals[j] & kEntrySegmentMask]; };
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const bool listed = g.activeEntries <= a.activeCapacity;
+    const uint32_t n = g.sortCount;
+    const uint32_t count = listed ? g.activeEntries : n;
+    for (uint32_t u = k; u < count; u += stride) {
+        uint32_t e, r0, r1;
+        if (listed) {
+            const uint4 w = a.active[u];
+            e = w.x;
+            r0 = w.y;
+            r1 = w.z;
+        } else {
+            e = u;
+            const uint32_t end = e + 1 < n ? a.offsets[e + 1] : *a.pairTotal;
+            if (end == a.offsets[e]) continue;
+            const uint32_t key = a.keys[e];
+            r0 = e;
+            r1 = e + 1u;
+            while (r0 > 0u && a.keys[r0 - 1u] == key) --r0;
+            while (r1 < n && a.keys[r1] == key) ++r1;
+        }
+        uint32_t out = a.offsets[e];
+        forRunCandidates<W>(e, r0, r1, entryAt, ijAt, a.localFixed,
+                            [&](uint32_t p, uint32_t q) { emitCandidate(a, out, p, q); });
     }
 }
 
@@ -736,7 +932,15 @@ __global__ void k_self_solve(const NumiClothBagGPUConfig* __restrict__ cfg, cons
             const uint2 ea = ij[pr.x];
             const uint2 eb = ij[pr.y];
             const float4 A0 = pos[ea.x], A1 = pos[ea.y], B0 = pos[eb.x], B1 = pos[eb.y];
-            const float4 Q0 = prev[ea.x], Q1 = prev[ea.y], R0 = prev[eb.x], R1 = prev[eb.y];
+            // The start positions: the swept mode reads them always, the
+            // present mode only for a contact normal of coincident points.
+            float4 Q0{}, Q1{}, R0{}, R1{};
+            if (mode == 1u) {
+                Q0 = prev[ea.x];
+                Q1 = prev[ea.y];
+                R0 = prev[eb.x];
+                R1 = prev[eb.y];
+            }
             // edgeBoundsOverlap (reference :213-274) with expansion r.
             float aLo[3] = {fminf(A0.x, A1.x), fminf(A0.y, A1.y), fminf(A0.z, A1.z)};
             float aHi[3] = {fmaxf(A0.x, A1.x), fmaxf(A0.y, A1.y), fmaxf(A0.z, A1.z)};
@@ -762,9 +966,14 @@ __global__ void k_self_solve(const NumiClothBagGPUConfig* __restrict__ cfg, cons
                     if (c.distance < target) {
                         wa = c.firstWeight;
                         wb = c.secondWeight;
-                        const V3 pa = mix3(xyz(Q0), xyz(Q1), wa);
-                        const V3 pb = mix3(xyz(R0), xyz(R1), wb);
-                        n = contactNormal(c.second - c.first, xyz(A1) - xyz(A0), xyz(B1) - xyz(B0), pa - pb);
+                        const V3 separation = c.second - c.first;
+                        V3 previousOffset = v3(0.0f, 0.0f, 0.0f);  // read by contactNormal only at zero separation
+                        if (!(length(separation) > 1.0e-12f)) {
+                            const V3 pa = mix3(xyz(prev[ea.x]), xyz(prev[ea.y]), wa);
+                            const V3 pb = mix3(xyz(prev[eb.x]), xyz(prev[eb.y]), wb);
+                            previousOffset = pa - pb;
+                        }
+                        n = contactNormal(separation, xyz(A1) - xyz(A0), xyz(B1) - xyz(B0), previousOffset);
                         correction = target - c.distance;
                     }
                 } else {

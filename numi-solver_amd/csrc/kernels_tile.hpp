@@ -67,6 +67,22 @@ struct TileProgramArgs {
     uint32_t stageOffset;
     uint32_t stageItems;
     uint32_t* failure;
+    // Strain programs of the reconciliation rounds: tileChanged (null: not
+    // tracked) receives, per tile, whether the tile's owned knots leave the
+    // launch different from how they entered it. With prevChanged (the
+    // previous round's flags) a tile returns at once when no tile owning one
+    // of its loaded knots (tileNbrs[tileNbrStart[tile]..], itself included)
+    // changed them in the previous round. The host sets prevChanged only on a
+    // program that repeats the previous round's with nothing in between and
+    // no leading pass: the tile's loaded knots are then the previous round's
+    // output, equal to that round's input for these tiles, so its owned
+    // knots would come out as that round produced them, which is what the
+    // output buffer (that round's input) already holds.
+    uint32_t* tileChanged;
+    const uint32_t* prevChanged;
+    const uint32_t* tileNbrStart;
+    const uint32_t* tileNbrs;
+    unsigned long long* roundStats;  // tracked tile-rounds: [0] skipped, [1] run
     uint32_t kinds[kMaxTilePasses];
 };
 
@@ -123,10 +139,12 @@ __device__ __forceinline__ void tileLoadKnotIds(const TileProgramArgs& a, uint32
 // flight before any is used: the position and both mask words together,
 // then the accumulators of touched knots; inFlight() runs in between
 // (further independent loads of the caller).
+// orig receives the register slots' positions as read.
 template <typename InFlight>
 __device__ __forceinline__ void tileGatherKnots(float4* lds, const TileProgramArgs& a, uint32_t base, uint32_t loaded,
                                                 uint32_t owned, uint32_t t, uint32_t stride, bool live,
-                                                const uint32_t* ids, InFlight&& inFlight) {
+                                                const uint32_t* ids, float4 (&orig)[kTileKnotSlots],
+                                                InFlight&& inFlight) {
     const bool applySelf = live && a.selfAcc != nullptr;
     const bool clearing = a.clearAcc != nullptr;
     float4 p[kTileKnotSlots];
@@ -138,6 +156,8 @@ __device__ __forceinline__ void tileGatherKnots(float4* lds, const TileProgramAr
         clearWord[u] = clearing ? a.clearTouched[ids[u] >> 5] : 0u;
     }
     inFlight();
+#pragma unroll
+    for (uint32_t u = 0; u < kTileKnotSlots; ++u) orig[u] = p[u];
 #pragma unroll
     for (uint32_t u = 0; u < kTileKnotSlots; ++u) {
         const uint32_t i = t + u * stride;
@@ -321,6 +341,20 @@ __global__ void k_tile_program(TileProgramArgs a) {
     const float2* __restrict__ knotCC = a.knotCosineCompliance;
     uint32_t* const failure = a.failure;
     const uint32_t tile = blockIdx.x;
+    if constexpr (Strain) {
+        if (a.prevChanged != nullptr) {
+            const uint32_t n0 = a.tileNbrStart[tile], n1 = a.tileNbrStart[tile + 1];
+            bool moved = false;
+            for (uint32_t k = n0 + threadIdx.x; k < n1; k += blockDim.x) moved |= a.prevChanged[a.tileNbrs[k]] != 0u;
+            if (__syncthreads_or(moved ? 1 : 0) == 0) {
+                if (threadIdx.x == 0) {
+                    a.tileChanged[tile] = 0u;
+                    atomicAdd(a.roundStats, 1ull);
+                }
+                return;
+            }
+        }
+    }
     const uint32_t base = a.loadStart[tile];
     const uint32_t loaded = a.loadStart[tile + 1] - base;
     const uint32_t owned = a.ownedCount[tile];
@@ -350,6 +384,9 @@ __global__ void k_tile_program(TileProgramArgs a) {
     const uint32_t itemBase = itemStart[0];
     // Ids of the thread's first loaded knots (the first kept for the write-back).
     uint32_t ids[kTileKnotSlots];
+    // The register slots' knots as read (strain programs compare the owned
+    // knots with them when tracking changes).
+    float4 orig[kTileKnotSlots];
     if constexpr (Staged) {
         // Two phases, each with every request of the workgroup in flight at
         // once: (1) the knot ids into registers and the items of every pass
@@ -373,7 +410,7 @@ __global__ void k_tile_program(TileProgramArgs a) {
         if (collides)
             for (uint32_t k = t; k < a.sphereCount; k += stride) sphereCache[k] = a.spheres[k];
         __syncthreads();
-        tileGatherKnots(lds, a, base, loaded, owned, t, stride, live, ids, [&]() {
+        tileGatherKnots(lds, a, base, loaded, owned, t, stride, live, ids, orig, [&]() {
             if constexpr (!Strain) {
                 for (uint32_t j0 = t; j0 < count; j0 += kItemSlots * stride) {
                     float v[kItemSlots];
@@ -402,7 +439,7 @@ __global__ void k_tile_program(TileProgramArgs a) {
             tileLoadLambdas(in0, itemStart, passes, 0, kind0, t, stride, cur, lam);
         }
         tileLoadKnotIds(a, base, loaded, t, stride, ids);
-        tileGatherKnots(lds, a, base, loaded, owned, t, stride, live, ids, [&]() {
+        tileGatherKnots(lds, a, base, loaded, owned, t, stride, live, ids, orig, [&]() {
             if (collides)
                 for (uint32_t k = t; k < a.sphereCount; k += stride) sphereCache[k] = a.spheres[k];
         });
@@ -463,6 +500,36 @@ __global__ void k_tile_program(TileProgramArgs a) {
     if (live)
         for (uint32_t pass = 0; pass < a.trailingCollides; ++pass)
             tileCollide(lds, loaded, owned, a, ground, floorZ, sphereCache, culled, window, scratch);
+    if constexpr (Strain) {
+        if (a.tileChanged != nullptr) {
+            // Owned knots as written against as read, bit for bit (the
+            // register slots' reads, or the input again past them).
+            bool differs = false;
+#pragma unroll
+            for (uint32_t u = 0; u < kTileKnotSlots; ++u) {
+                const uint32_t i = t + u * stride;
+                if (i < owned) {
+                    const float4 now = lds[i];
+                    differs |= __float_as_uint(now.x) != __float_as_uint(orig[u].x) ||
+                               __float_as_uint(now.y) != __float_as_uint(orig[u].y) ||
+                               __float_as_uint(now.z) != __float_as_uint(orig[u].z) ||
+                               __float_as_uint(now.w) != __float_as_uint(orig[u].w);
+                }
+            }
+            for (uint32_t i = t + kTileKnotSlots * stride; i < owned; i += stride) {
+                const float4 now = lds[i], was = a.posIn[a.loadIds[base + i]];
+                differs |= __float_as_uint(now.x) != __float_as_uint(was.x) ||
+                           __float_as_uint(now.y) != __float_as_uint(was.y) ||
+                           __float_as_uint(now.z) != __float_as_uint(was.z) ||
+                           __float_as_uint(now.w) != __float_as_uint(was.w);
+            }
+            const bool changed = __syncthreads_or(differs ? 1 : 0) != 0;
+            if (t == 0) {
+                a.tileChanged[tile] = changed ? 1u : 0u;
+                atomicAdd(a.roundStats + 1, 1ull);
+            }
+        }
+    }
     // The first slot's id is still in a register (the others would cost the
     // passes registers): most tiles own fewer knots than the workgroup has
     // threads.

@@ -133,6 +133,8 @@ struct NumiClothHipSolver {
     DeviceArray<double> times;      // per substep: time at the substep end
     DeviceArray<SubstepSlot> slot;
     DeviceArray<uint32_t> failure;
+    DeviceArray<uint32_t> roundFlags;  // per-tile change flags of the reconciliation rounds' strain programs
+    DeviceArray<unsigned long long> roundStats;  // reconciliation tile-rounds since create: skipped, run
     DeviceArray<NumiClothBagGPUSelfStatus> selfStatus;
     DeviceArray<NumiClothBagGPUFrictionStatus> frictionStatus;
     DeviceArray<NumiClothBagGPUAerodynamicsStatus> aeroStatus;
@@ -168,13 +170,12 @@ struct NumiClothHipSolver {
     DeviceArray<uint32_t> localStart, localList;
     DeviceArray<uint32_t> localFixed;   // two-hop lists in localWidth slots per knot (0: not used)
     uint32_t localWidth = 0;
-    DeviceArray<SortedEntry> sortedEntries;
-    DeviceArray<uint2> sortedIJ;
     DeviceArray<float4> boxLo, boxHi, partialLo, partialHi;
     DeviceArray<SelfGrid> grid;
     DeviceArray<uint32_t> segmentCounts, entryCounts, scanSums, totals;
     DeviceArray<uint32_t> keysA, valsA, keysB, valsB, histogram;
     DeviceArray<uint2> pairs;
+    DeviceArray<uint4> activeEntries;  // run walks: (entry, run start, run end) of entries with candidates
     // Two fixed-point accumulators and touched masks (one bit per knot with a
     // pending correction), used by alternate self-contact solves: a tile
     // program that applies one buffer's corrections cannot clear it (other
@@ -207,6 +208,8 @@ struct NumiClothHipSolver {
         std::vector<uint32_t> kinds;
         DeviceArray<uint32_t> loadStart, loadIds, ownedCount, itemStart;
         DeviceArray<uint4> items;
+        // Per tile, the tiles owning its loaded knots (itself included).
+        DeviceArray<uint32_t> nbrStart, nbrs;
         double algorithmicBytes = 0.0;
         uint64_t computed = 0, unique = 0;
     };
@@ -459,23 +462,29 @@ struct NumiClothHipSolver {
         k_self_cell_counts<<<segmentBlocks, kScanThreads, 0, stream>>>(boxLo.ptr, boxHi.ptr, g, segmentCounts.ptr,
                                                                        cellSlots.ptr, failure.ptr);
         launchScan(segmentCounts.ptr, &g->segments, D, totals.ptr);
+        // The entries in emission order into A; on a dense grid each also
+        // draws its slot among its cell's entries (into entryCounts, free
+        // until the candidate pass).
         k_self_emit_entries<<<segmentBlocks, kScanThreads, 0, stream>>>(boxLo.ptr, boxHi.ptr, g, segmentCounts.ptr,
                                                                         totals.ptr, entryCapacity, keysA.ptr,
-                                                                        valsA.ptr, cellSlots.ptr, failure.ptr);
+                                                                        valsA.ptr, cellSlots.ptr, entryCounts.ptr,
+                                                                        failure.ptr);
         // Grid-stride kernels over the live entries: enough blocks to fill the
         // chip, not one per capacity slot.
         const uint32_t entryBlocks = std::min<uint32_t>(8192, (entryCapacity + kScanThreads - 1) / kScanThreads);
         // Dense grid: counting sort (per-cell counts from the emit pass, one
-        // scan, placement, rank within each cell run) straight into A and the
-        // gathered records. Otherwise these three exit at once and the radix
-        // passes below sort; each side's kernels exit when the other sorts.
+        // scan, placement into B by cell start + slot; the ranking by segment
+        // within each cell happens in the candidate pass, or in the rank pass
+        // for lists searched in memory). Otherwise these exit at once and the
+        // radix passes below sort A; each side's kernels exit when the other
+        // sorts.
         if (cellCapacity != 0) {
             launchScan(cellSlots.ptr, &g->cellScanLength, cellCapacity, totals.ptr + 3);
-            k_self_place_entries<<<segmentBlocks, kScanThreads, 0, stream>>>(boxLo.ptr, boxHi.ptr, g, cellSlots.ptr,
-                                                                             keysB.ptr, valsB.ptr);
-            k_self_rank_entries<<<entryBlocks, kScanThreads, 0, stream>>>(
-                boxLo.ptr, boxHi.ptr, dIJ.ptr, g, cellSlots.ptr, keysB.ptr, valsB.ptr, keysA.ptr, valsA.ptr,
-                localWidth != 0 ? sortedEntries.ptr : nullptr, localWidth != 0 ? sortedIJ.ptr : nullptr);
+            k_self_place_entries<<<entryBlocks, kScanThreads, 0, stream>>>(g, cellSlots.ptr, keysA.ptr, valsA.ptr,
+                                                                           entryCounts.ptr, keysB.ptr, valsB.ptr);
+            if (localWidth == 0)
+                k_self_rank_entries<<<entryBlocks, kScanThreads, 0, stream>>>(g, cellSlots.ptr, keysB.ptr, valsB.ptr,
+                                                                              keysA.ptr, valsA.ptr);
         }
         uint32_t* keysIn = keysA.ptr;
         uint32_t* valsIn = valsA.ptr;
@@ -490,15 +499,18 @@ struct NumiClothHipSolver {
             std::swap(keysIn, keysOut);
             std::swap(valsIn, valsOut);
         }
+        // Four passes: the radix sort ends in A, where the candidate pass (or
+        // the rank pass) also writes the counting sort's order.
+        static_assert(32 / kRadixBits % 2 == 0, "the radix sort must end in A");
         PairArgs pa{};
         pa.boxLo = boxLo.ptr;
         pa.boxHi = boxHi.ptr;
         pa.ij = dIJ.ptr;
-        pa.entries = sortedEntries.ptr;
-        pa.sortedIJ = sortedIJ.ptr;
         pa.grid = g;
-        pa.keys = keysIn;
-        pa.vals = valsIn;
+        pa.keys = keysA.ptr;
+        pa.vals = valsA.ptr;
+        pa.placedKeys = keysB.ptr;
+        pa.placedVals = valsB.ptr;
         pa.localStart = localStart.ptr;
         pa.localList = localList.ptr;
         pa.localFixed = localFixed.ptr;
@@ -509,21 +521,27 @@ struct NumiClothHipSolver {
         pa.pairs = pairs.ptr;
         pa.impulses = selfImpulses.ptr;
         pa.failure = failure.ptr;
+        pa.active = activeEntries.ptr;
+        pa.activeCapacity = pairCapacity;
+        // Lists that fit 16 or 32 register slots: run walks over LDS windows
+        // (one workgroup per 640 entries' runs), then the emit pass over the
+        // entries that found candidates. Longer lists: one thread per entry
+        // with the lists searched in memory.
         if (localWidth != 0) {
-            k_self_gather<<<entryBlocks, kScanThreads, 0, stream>>>(boxLo.ptr, boxHi.ptr, dIJ.ptr, g, keysIn, valsIn,
-                                                                   sortedEntries.ptr, sortedIJ.ptr);
+            const uint32_t chunks = std::min<uint32_t>(4096, (entryCapacity + kRunChunk - 1) / kRunChunk);
+            if (localWidth == 16) k_self_run_counts<16><<<chunks, kPairThreads, 0, stream>>>(pa);
+            else k_self_run_counts<32><<<chunks, kPairThreads, 0, stream>>>(pa);
+        } else {
+            k_self_pair_counts<<<entryBlocks, kPairThreads, 0, stream>>>(pa);
         }
-        // LDS chunks of 4 entries per thread once the scene has 2^20
-        // segments (tens of millions of entries); one thread per entry below.
-        const bool wide = D >= (1u << 20);
-        auto counts = localWidth == 16 ? (wide ? k_self_pair_counts<16, 4 * kPairThreads> : k_self_pair_counts<16, 0>)
-                      : localWidth == 32 ? (wide ? k_self_pair_counts<32, 4 * kPairThreads> : k_self_pair_counts<32, 0>)
-                                         : k_self_pair_counts<0, 0>;
-        auto emit = localWidth == 16 ? k_self_emit_pairs<16> : (localWidth == 32 ? k_self_emit_pairs<32>
-                                                                                  : k_self_emit_pairs<0>);
-        counts<<<entryBlocks, kPairThreads, 0, stream>>>(pa);
         launchScan(entryCounts.ptr, &g->sortCount, entryCapacity, totals.ptr + 1);
-        emit<<<entryBlocks, kScanThreads, 0, stream>>>(pa);
+        if (localWidth != 0) {
+            const uint32_t blocks = std::min<uint32_t>(2048, (pairCapacity + kScanThreads - 1) / kScanThreads);
+            if (localWidth == 16) k_self_emit_run_pairs<16><<<blocks, kScanThreads, 0, stream>>>(pa);
+            else k_self_emit_run_pairs<32><<<blocks, kScanThreads, 0, stream>>>(pa);
+        } else {
+            k_self_emit_pairs<<<entryBlocks, kScanThreads, 0, stream>>>(pa);
+        }
     }
 
     // collide: run the ground/sphere pass that follows; defer: leave it to the
@@ -536,10 +554,12 @@ struct NumiClothHipSolver {
             clearSelf(b);
             dirtySelf = -1;
         }
-        // Two workgroups per CU: 131,072 candidates per round of the kernel's
-        // grid-stride loop (the workloads here have at most a few hundred),
-        // and a call with no candidate costs little more than its launch.
-        const uint32_t blocks = std::min<uint32_t>(512, std::max(1u, (pairCapacity + kBlock - 1) / kBlock));
+        // Four workgroups per CU: 262,144 candidates per round of the
+        // kernel's grid-stride loop, so a draped 1M-knot sheet (~230,000
+        // candidates) waits out its chain of dependent loads (pair,
+        // endpoints, positions) once, and a call with no candidate still
+        // costs little more than its launch.
+        const uint32_t blocks = std::min<uint32_t>(1024, std::max(1u, (pairCapacity + kBlock - 1) / kBlock));
         k_self_solve<<<blocks, kBlock, 0, stream>>>(cfg(), ok(), mode, grid.ptr, pairs.ptr, dIJ.ptr, posCur,
                                                      prev.ptr, accumulator[b].ptr, touched[b].ptr, selfImpulses.ptr,
                                                      selfStatus.ptr);
@@ -581,8 +601,11 @@ struct NumiClothHipSolver {
         applySelf(0, vel.ptr);
     }
 
+    // tileChanged / prevChanged: the strain programs' per-tile change flags
+    // (see TileProgramArgs); a program with a leading pass skips no tile.
     void launchTileProgram(TileProgram& t, uint32_t trailingCollides, const NumiClothBagGPUConfig* config,
-                           const uint32_t* valid, bool scratch = false) {
+                           const uint32_t* valid, bool scratch = false, uint32_t* tileChanged = nullptr,
+                           const uint32_t* prevChanged = nullptr) {
         TileProgramArgs a{};
         a.cfg = config;
         a.ok = valid;
@@ -624,6 +647,12 @@ struct NumiClothHipSolver {
         }
         a.trailingCollides = groundOrColliders() ? trailingCollides : 0u;
         a.failure = failure.ptr;
+        a.tileChanged = tileChanged;
+        const bool leading = a.leadingCollide != 0 || a.selfAcc != nullptr || a.clearAcc != nullptr;
+        a.prevChanged = leading || tileChanged == nullptr ? nullptr : prevChanged;
+        a.tileNbrStart = t.nbrStart.ptr;
+        a.tileNbrs = t.nbrs.ptr;
+        a.roundStats = roundStats.ptr;
         for (uint32_t k = 0; k < t.passes; ++k) a.kinds[k] = t.kinds[k];
         const bool collides = a.leadingCollide != 0 || a.trailingCollides != 0;
         a.stageOffset = t.maxLoaded + (collides ? 2 * a.sphereCount : 0u);
@@ -664,10 +693,17 @@ struct NumiClothHipSolver {
         kLamCur = kLambda.ptr;
     }
 
-    // Strain sweeps followed by `collides` ground/sphere passes.
-    void launchStrainAndCollide(uint32_t sweeps, uint32_t collides) {
+    // Strain sweeps followed by `collides` ground/sphere passes. round >= 0:
+    // the reconciliation round of that index, whose strain program records
+    // which tiles changed their knots and, when it repeats the previous
+    // round's program, skips the tiles whose loaded knots that round left
+    // unchanged.
+    void launchStrainAndCollide(uint32_t sweeps, uint32_t collides, int round = -1) {
         if (useTiles && sweeps != 0 && tileStrain.valid) {  // consumes a deferred collide
-            launchTileProgram(tileStrain, collides, cfg(), ok());
+            const size_t tiles = tileStrain.tiles;
+            uint32_t* changed = round >= 0 ? roundFlags.ptr + static_cast<size_t>(round) * tiles : nullptr;
+            const uint32_t* prev = round > 0 ? roundFlags.ptr + static_cast<size_t>(round - 1) * tiles : nullptr;
+            launchTileProgram(tileStrain, collides, cfg(), ok(), false, changed, prev);
             return;
         }
         launchStrain(sweeps);
@@ -777,10 +813,14 @@ struct NumiClothHipSolver {
         if (sweeps != 0) {
             // Reconciliation, endpoint and certificate passes
             // (reference tools/cloth_bag_metal.mm:4557-4590).
+            // Without fruits nothing runs between the rounds' strain programs,
+            // so a round after one that changed nothing is skipped (its
+            // input and program are the unchanged round's).
+            const bool repeating = F == 0 && FP == 0 && Y == 0;
             for (uint32_t pass = 0; pass < kReconciliationPasses; ++pass) {
                 launchFruitPairs();
                 launchYarn(0);
-                launchStrainAndCollide(sweeps, 1);
+                launchStrainAndCollide(sweeps, 1, repeating ? static_cast<int>(pass) : -1);
                 launchFruitGround();
             }
             for (uint32_t round = 0; round <= kCertificatePasses; ++round) certificateRound(sweeps, defer);
@@ -1206,6 +1246,8 @@ void createSolver(const NumiClothHipSceneDesc& scene, NumiClothHipSolver& s) {
 
     s.slot.allocate(1);
     s.failure.allocate(1);
+    s.roundStats.allocate(2);
+    HIP_OK(hipMemsetAsync(s.roundStats.ptr, 0, 2 * sizeof(unsigned long long), st));
     s.selfStatus.allocate(1);
     s.frictionStatus.allocate(1);
     s.aeroStatus.allocate(1);
@@ -1367,14 +1409,11 @@ void setupSelfContact(const NumiClothHipSceneDesc& scene, NumiClothHipSolver& s)
     const uint32_t histLength = kRadixBins * s.entryTiles;
     s.keysA.allocate(s.entryCapacity);
     s.valsA.allocate(s.entryCapacity);
-    if (s.localWidth != 0) {
-        s.sortedEntries.allocate(s.entryCapacity);
-        s.sortedIJ.allocate(s.entryCapacity);
-    }
     s.keysB.allocate(s.entryCapacity);
     s.valsB.allocate(s.entryCapacity);
     s.histogram.allocate(histLength);
     s.pairs.allocate(s.pairCapacity);
+    if (s.localWidth != 0) s.activeEntries.allocate(s.pairCapacity);
     s.selfImpulses.allocate(s.pairCapacity);  // written by the emit pass before any use
     for (int b = 0; b < 2; ++b) {
         s.accumulator[b].allocate(s.P);
@@ -1503,6 +1542,24 @@ bool buildTileProgram(NumiClothHipSolver& s, const std::vector<TilePass>& passes
     out.itemStart.upload(plan.itemStart.data(), plan.itemStart.size(), st);
     for (uint32_t tile = 0; tile < plan.tiles; ++tile)
         out.maxItems = std::max(out.maxItems, plan.itemStart[(tile + 1) * out.passes] - plan.itemStart[tile * out.passes]);
+    {
+        std::vector<uint32_t> ownerOf(s.P, 0u);
+        for (uint32_t tile = 0; tile < plan.tiles; ++tile)
+            for (uint32_t k = 0; k < plan.ownedCount[tile]; ++k) ownerOf[plan.loadIds[plan.loadStart[tile] + k]] = tile;
+        std::vector<uint32_t> start{0u}, list, seen;
+        for (uint32_t tile = 0; tile < plan.tiles; ++tile) {
+            seen.clear();
+            for (uint32_t k = plan.loadStart[tile]; k < plan.loadStart[tile + 1]; ++k) seen.push_back(ownerOf[plan.loadIds[k]]);
+            std::sort(seen.begin(), seen.end());
+            seen.erase(std::unique(seen.begin(), seen.end()), seen.end());
+            list.insert(list.end(), seen.begin(), seen.end());
+            start.push_back(static_cast<uint32_t>(list.size()));
+        }
+        out.nbrStart.allocate(start.size());
+        out.nbrStart.upload(start.data(), start.size(), st);
+        out.nbrs.allocate(list.size());
+        out.nbrs.upload(list.data(), list.size(), st);
+    }
     // Staged: knots, both sphere lists and the whole program in LDS.
     const uint64_t stagedLds = (static_cast<uint64_t>(out.maxLoaded) + 2ull * s.extension.staticSphereCount) * 16u +
                                NumiClothHipSolver::stagedBytes(out.maxItems);
@@ -1897,6 +1954,13 @@ void diagnoseSolver(NumiClothHipSolver& s, NumiClothHipDiagnostics* out) {
     out->maximumSpherePenetration = r.z;
     out->maximumSelfPenetration = r.w;
     out->certificatePasses = s.certificateLast;
+    {
+        unsigned long long stats[2] = {0ull, 0ull};
+        s.roundStats.download(stats, 2, s.stream);
+        HIP_OK(hipStreamSynchronize(s.stream));
+        out->tileRoundsSkipped = stats[0];
+        out->tileRoundsRun = stats[1];
+    }
     if (s.yarnStats.count != 0) {
         uint32_t stats[2] = {0u, 0u};
         s.yarnStats.download(stats, 2, s.stream);
@@ -2017,8 +2081,9 @@ void selfBroadPhase(NumiClothHipSolver& s, NumiClothHipSelfBroadPhase* io) {
         const uint32_t n = std::min(io->pairCount, io->pairCapacity);
         s.pairs.download(reinterpret_cast<uint2*>(io->pairs), n, s.stream);
     }
-    // Both sorts leave the sorted entries in A (the counting sort's rank pass
-    // writes A; the radix sort's four ping-pong passes end there).
+    // Both sorts leave the sorted entries in A (the counting sort's candidate
+    // pass, or its rank pass, writes A; the radix sort's four ping-pong passes
+    // end there).
     if (io->cellIds != nullptr) s.keysA.download(io->cellIds, std::min(io->entryCount, io->entryCapacity), s.stream);
     if (io->segments != nullptr) s.valsA.download(io->segments, std::min(io->entryCount, io->entryCapacity), s.stream);
     HIP_OK(hipStreamSynchronize(s.stream));
@@ -2033,6 +2098,9 @@ void NumiClothHipSolver::ensureStrainProgram(uint32_t sweeps) {
     tileStrainSweeps = sweeps;
     if (sweeps == 0 || !buildTileProgram(*this, tilePasses(*this, false, sweeps), tileOwned, tileStrain)) {
         tileStrain.valid = false;
+    } else {
+        // Every tile of every round writes its flag, so they need no clearing.
+        roundFlags.allocate(static_cast<size_t>(kReconciliationPasses) * tileStrain.tiles);
     }
     if (graph != nullptr) {
         (void)hipGraphExecDestroy(graph);

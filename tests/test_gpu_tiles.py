@@ -75,3 +75,32 @@ def test_large_sheet_uses_several_tiles_with_halo(cloth):
     assert info.enabled and info.tiles > 1 and info.halo > 0
     assert info.maxLoaded > info.ownedPerTile
     assert timing.averageMilliseconds > 0.0 and timing.algorithmicBytes > 0.0
+
+
+def test_reconciliation_rounds_skip_unchanged_tiles(cloth):
+    """A reconciliation round's strain program skips the tiles whose loaded
+    knots the previous round left unchanged; the state equals the
+    per-colour schedule's, which runs every round in full."""
+    scene = SCENES["large"](cloth)
+    saved = os.environ.get("NUMI_CLOTH_HIP_TILES")
+    os.environ["NUMI_CLOTH_HIP_TILES"] = "1"
+    try:
+        with cloth.Solver(scene) as solver:
+            info = solver.tile_info()
+            solver.step(scene.configs(24), 9, 3)
+            tiled = solver.read()
+            d = solver.diagnostics()
+    finally:
+        if saved is None:
+            os.environ.pop("NUMI_CLOTH_HIP_TILES", None)
+        else:
+            os.environ["NUMI_CLOTH_HIP_TILES"] = saved
+    plain, _ = _run(cloth, scene, 24, 9, 3, False)
+    print(f"tiles={info.tiles}: reconciliation tile-rounds skipped {d.tileRoundsSkipped}, run {d.tileRoundsRun}")
+    assert info.tiles > 1
+    # 24 substeps x 8 rounds, every tile counted once per round.
+    assert d.tileRoundsSkipped + d.tileRoundsRun == 24 * 8 * info.tiles
+    assert d.tileRoundsSkipped > 0
+    for key in ["particles", "distances", "bends", "selfStatus", "frictionStatus"]:
+        assert tiled[key].tobytes() == plain[key].tobytes(), key
+    assert tiled["failure"] == plain["failure"] == 0
