@@ -22,6 +22,7 @@
 namespace numi_dev {
 
 constexpr uint32_t kMaxTilePasses = 64;
+constexpr uint32_t kXcds = 8;           // accelerator complex dies of an MI355X
 constexpr uint32_t kTileStaticLds = 2048;  // bound on k_tile_program's static LDS (checked below)
 constexpr uint32_t kTileDistanceKind = 0;  // values of numi_hip::TilePassKind
 constexpr uint32_t kTileKnotKind = 1;
@@ -340,7 +341,12 @@ __global__ void k_tile_program(TileProgramArgs a) {
     }
     const float2* __restrict__ knotCC = a.knotCosineCompliance;
     uint32_t* const failure = a.failure;
-    const uint32_t tile = blockIdx.x;
+    // Workgroups are dealt to the eight XCDs in turn (workgroup b runs on XCD
+    // b mod 8); each XCD takes a run of consecutive tiles instead, so that
+    // most of a tile's halo knots were written by tiles on its own XCD and
+    // come from its L2. Any bijection gives the same bytes.
+    const uint32_t xcd = blockIdx.x % kXcds, turn = blockIdx.x / kXcds;
+    const uint32_t tile = xcd * (gridDim.x / kXcds) + min(xcd, gridDim.x % kXcds) + turn;
     if constexpr (Strain) {
         if (a.prevChanged != nullptr) {
             const uint32_t n0 = a.tileNbrStart[tile], n1 = a.tileNbrStart[tile + 1];
