@@ -380,8 +380,8 @@ __global__ void k_tile_program(TileProgramArgs a) {
     const float invDt2 = 1.0f / (dt * dt);
     const bool ground = a.cfg->constraintCounts.z != 0u;
     const float floorZ = a.cfg->clothMaterial.x;
-    // Items of passes 0 and 1, multipliers of pass 0 (issued before the knot
-    // gather so the two overlap).
+    // Items of passes 0 and 1 and multipliers of pass 0, in flight together
+    // with the knot gather.
     const uint32_t t = threadIdx.x, stride = blockDim.x;
     uint4 cur[kTilePrefetch], next[kTilePrefetch], ahead[kTilePrefetch];
     float lam[kTilePrefetch], nextLam[kTilePrefetch];
@@ -436,16 +436,19 @@ __global__ void k_tile_program(TileProgramArgs a) {
         });
         __syncthreads();
     } else {
+        // Knot ids first: the gather waits on them, while the first items
+        // arrive meanwhile; the multipliers they index are requested with
+        // the knot positions.
+        tileLoadKnotIds(a, base, loaded, t, stride, ids);
         tileLoadItems<Strain>(items, itemStart, passes, 0, t, stride, cur);
         tileLoadItems<Strain>(items, itemStart, passes, 1, t, stride, next);
-        if constexpr (!Strain) {
-            const uint32_t kind0 = a.kinds[0];
-            const float* in0 = kind0 == kTileDistanceKind ? a.distanceLambdaIn
-                                                          : (kind0 == kTileKnotKind ? a.knotLambdaIn : a.bendLambdaIn);
-            tileLoadLambdas(in0, itemStart, passes, 0, kind0, t, stride, cur, lam);
-        }
-        tileLoadKnotIds(a, base, loaded, t, stride, ids);
         tileGatherKnots(lds, a, base, loaded, owned, t, stride, live, ids, orig, [&]() {
+            if constexpr (!Strain) {
+                const uint32_t kind0 = a.kinds[0];
+                const float* in0 = kind0 == kTileDistanceKind ? a.distanceLambdaIn
+                                                              : (kind0 == kTileKnotKind ? a.knotLambdaIn : a.bendLambdaIn);
+                tileLoadLambdas(in0, itemStart, passes, 0, kind0, t, stride, cur, lam);
+            }
             if (collides)
                 for (uint32_t k = t; k < a.sphereCount; k += stride) sphereCache[k] = a.spheres[k];
         });
