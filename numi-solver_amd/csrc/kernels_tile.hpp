@@ -84,17 +84,35 @@ struct TileProgramArgs {
     const uint32_t* tileNbrStart;
     const uint32_t* tileNbrs;
     unsigned long long* roundStats;  // tracked tile-rounds: [0] skipped, [1] run
+    // Sphere windows built once per substep by k_tile_windows (null: each
+    // ground/sphere pass builds its own): per tile the window and its
+    // sphereCount-slot list. A window holds for any knot inside its box
+    // (knots that leave it are redone with the whole list), so one built
+    // from the loaded knots at the substep's start serves every pass.
+    const SphereWindow* tileWindows;
+    const float4* tileWindowSpheres;
     uint32_t kinds[kMaxTilePasses];
 };
 
 // Ground/sphere pass over every loaded knot (owned and halo: the pass acts on
 // one knot at a time, so halo knots stay equal to their owners' values).
 // Non-finite knots are skipped and reported by their owning tile only.
-__device__ __forceinline__ void tileCollide(float4* lds, uint32_t loaded, uint32_t owned, const TileProgramArgs& a, bool ground,
-                            float radius, const float4* sphereCache, float4* culled, SphereWindow& window,
-                            float4* scratch) {
+// windowReady: the window in LDS is the tile's substep window, loaded by an
+// earlier pass of the launch.
+__device__ __forceinline__ void tileCollide(float4* lds, uint32_t tile, uint32_t loaded, uint32_t owned,
+                                            const TileProgramArgs& a, bool ground, float radius,
+                                            const float4* sphereCache, float4* culled, SphereWindow& window,
+                                            float4* scratch, bool& windowReady) {
     const bool windowed = a.sphereCount >= a.sphereWindowMin;
-    if (windowed) {
+    if (windowed && a.tileWindows != nullptr) {
+        if (!windowReady) {
+            if (threadIdx.x == 0) window = a.tileWindows[tile];
+            for (uint32_t k = threadIdx.x; k < a.sphereCount; k += blockDim.x)
+                culled[k] = a.tileWindowSpheres[static_cast<size_t>(tile) * a.sphereCount + k];
+            __syncthreads();
+            windowReady = true;
+        }
+    } else if (windowed) {
         V3 lo = v3(INFINITY, INFINITY, INFINITY), hi = v3(-INFINITY, -INFINITY, -INFINITY);
         for (uint32_t i = threadIdx.x; i < loaded; i += blockDim.x) {
             const float4 p = lds[i];
@@ -454,8 +472,9 @@ __global__ void k_tile_program(TileProgramArgs a) {
         });
         __syncthreads();
     }
+    bool windowReady = false;
     if (live && a.leadingCollide != 0)
-        tileCollide(lds, loaded, owned, a, ground, floorZ, sphereCache, culled, window, scratch);
+        tileCollide(lds, tile, loaded, owned, a, ground, floorZ, sphereCache, culled, window, scratch, windowReady);
     bool sweepMoved = false;
     for (uint32_t k = 0; k < passes; ++k) {
         const uint32_t kind = kinds[k];
@@ -508,7 +527,8 @@ __global__ void k_tile_program(TileProgramArgs a) {
     }
     if (live)
         for (uint32_t pass = 0; pass < a.trailingCollides; ++pass)
-            tileCollide(lds, loaded, owned, a, ground, floorZ, sphereCache, culled, window, scratch);
+            tileCollide(lds, tile, loaded, owned, a, ground, floorZ, sphereCache, culled, window, scratch,
+                        windowReady);
     if constexpr (Strain) {
         if (a.tileChanged != nullptr) {
             // Owned knots as written against as read, bit for bit (the
@@ -544,6 +564,34 @@ __global__ void k_tile_program(TileProgramArgs a) {
     // threads.
     if (t < owned) a.posOut[ids[0]] = lds[t];
     for (uint32_t i = t + stride; i < owned; i += stride) a.posOut[a.loadIds[base + i]] = lds[i];
+}
+
+// The substep's sphere window of every tile of a program: the box of the
+// tile's loaded knots at the substep's start and the spheres that can act
+// inside it grown by the pad (buildSphereWindow). One workgroup per tile;
+// dynamic LDS: the spheres and the culled list (2 * sphereCount float4).
+__global__ void k_tile_windows(const NumiClothBagGPUConfig* __restrict__ cfg, const float4* __restrict__ pos,
+                               const uint32_t* __restrict__ loadStart, const uint32_t* __restrict__ loadIds,
+                               const float4* __restrict__ spheres, uint32_t sphereCount,
+                               SphereWindow* __restrict__ windows, float4* __restrict__ windowSpheres) {
+    extern __shared__ float4 sphereCache[];
+    __shared__ SphereWindow window;
+    __shared__ float4 scratch[2 * kMaxBlockWaves];
+    float4* culled = sphereCache + sphereCount;
+    const uint32_t tile = blockIdx.x;
+    for (uint32_t k = threadIdx.x; k < sphereCount; k += blockDim.x) sphereCache[k] = spheres[k];
+    V3 lo = v3(INFINITY, INFINITY, INFINITY), hi = v3(-INFINITY, -INFINITY, -INFINITY);
+    for (uint32_t i = loadStart[tile] + threadIdx.x; i < loadStart[tile + 1]; i += blockDim.x) {
+        const float4 p = pos[loadIds[i]];
+        if (!isfinite(p.z)) continue;
+        lo = v3(fminf(lo.x, p.x), fminf(lo.y, p.y), fminf(lo.z, p.z));
+        hi = v3(fmaxf(hi.x, p.x), fmaxf(hi.y, p.y), fmaxf(hi.z, p.z));
+    }
+    __syncthreads();
+    buildSphereWindow(lo, hi, cfg->clothMaterial.x, sphereCache, sphereCount, culled, window, scratch);
+    if (threadIdx.x == 0) windows[tile] = window;
+    for (uint32_t k = threadIdx.x; k < window.count; k += blockDim.x)
+        windowSpheres[static_cast<size_t>(tile) * sphereCount + k] = culled[k];
 }
 
 __global__ void k_copy_float4(const float4* __restrict__ from, float4* __restrict__ to, uint32_t n) {

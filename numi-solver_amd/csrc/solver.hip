@@ -210,6 +210,9 @@ struct NumiClothHipSolver {
         DeviceArray<uint4> items;
         // Per tile, the tiles owning its loaded knots (itself included).
         DeviceArray<uint32_t> nbrStart, nbrs;
+        // Per tile, the substep's sphere window and its list (windowed scenes).
+        DeviceArray<SphereWindow> windows;
+        DeviceArray<float4> windowSpheres;
         double algorithmicBytes = 0.0;
         uint64_t computed = 0, unique = 0;
     };
@@ -653,6 +656,11 @@ struct NumiClothHipSolver {
         a.tileNbrStart = t.nbrStart.ptr;
         a.tileNbrs = t.nbrs.ptr;
         a.roundStats = roundStats.ptr;
+        // The odd program's tiles own the even program's knots, with a
+        // smaller halo: the even program's windows serve both.
+        const TileProgram& w = &t == &tileOdd ? tileEven : t;
+        a.tileWindows = !scratch && tileWindowsBuilt && w.windows.count != 0 ? w.windows.ptr : nullptr;
+        a.tileWindowSpheres = w.windowSpheres.ptr;
         for (uint32_t k = 0; k < t.passes; ++k) a.kinds[k] = t.kinds[k];
         const bool collides = a.leadingCollide != 0 || a.trailingCollides != 0;
         a.stageOffset = t.maxLoaded + (collides ? 2 * a.sphereCount : 0u);
@@ -691,6 +699,26 @@ struct NumiClothHipSolver {
         dLamCur = dLambda.ptr;
         bLamCur = bLambda.ptr;
         kLamCur = kLambda.ptr;
+    }
+
+    // The substep's sphere windows of the even and strain programs' tiles,
+    // from the positions before the first constraint pass (windowed scenes
+    // with tile programs); the programs' ground/sphere passes then load them
+    // instead of building their own.
+    bool tileWindowsBuilt = false;
+    void launchTileWindows() {
+        tileWindowsBuilt = false;
+        if (!useTiles || extension.staticSphereCount < sphereWindowMin || extension.staticSphereCount == 0) return;
+        // Pending corrections may still move knots: a window needs no exact
+        // box, only one that most knots stay inside.
+        const uint32_t n = extension.staticSphereCount;
+        for (TileProgram* t : {&tileEven, &tileStrain}) {
+            if (!t->valid || t->windows.count == 0) continue;
+            k_tile_windows<<<t->tiles, kBlock, 2 * n * sizeof(float4), stream>>>(
+                cfg(), posCur, t->loadStart.ptr, t->loadIds.ptr, spheres.ptr, n, t->windows.ptr, t->windowSpheres.ptr);
+        }
+        tileWindowsBuilt = tileEven.valid && tileEven.windows.count != 0 &&
+                           (!tileStrain.valid || tileStrain.windows.count != 0);
     }
 
     // Strain sweeps followed by `collides` ground/sphere passes. round >= 0:
@@ -779,6 +807,7 @@ struct NumiClothHipSolver {
         }
         launchBuildYarnContacts();
         launchYarn(1);
+        launchTileWindows();
         // Nothing between the constraint passes and the ground/sphere pass
         // of an iteration: the pass joins the tile program (or the
         // self-contact apply when that runs in between).
@@ -1559,6 +1588,11 @@ bool buildTileProgram(NumiClothHipSolver& s, const std::vector<TilePass>& passes
         out.nbrStart.upload(start.data(), start.size(), st);
         out.nbrs.allocate(list.size());
         out.nbrs.upload(list.data(), list.size(), st);
+    }
+    const uint32_t spheres = s.extension.staticSphereCount;
+    if (spheres != 0 && spheres >= s.sphereWindowMin) {
+        out.windows.allocate(out.tiles);
+        out.windowSpheres.allocate(static_cast<size_t>(out.tiles) * spheres);
     }
     // Staged: knots, both sphere lists and the whole program in LDS.
     const uint64_t stagedLds = (static_cast<uint64_t>(out.maxLoaded) + 2ull * s.extension.staticSphereCount) * 16u +
