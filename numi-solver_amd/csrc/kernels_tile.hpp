@@ -568,8 +568,9 @@ __global__ void k_tile_program(TileProgramArgs a) {
 
 // The substep's sphere window of every tile of a program: the box of the
 // tile's loaded knots at the substep's start and the spheres that can act
-// inside it grown by the pad (buildSphereWindow). One workgroup per tile;
-// dynamic LDS: the spheres and the culled list (2 * sphereCount float4).
+// inside it grown by the pad (buildSphereWindow). One workgroup of 1024
+// threads per tile; dynamic LDS: the spheres and the culled list
+// (2 * sphereCount float4).
 __global__ void k_tile_windows(const NumiClothBagGPUConfig* __restrict__ cfg, const float4* __restrict__ pos,
                                const uint32_t* __restrict__ loadStart, const uint32_t* __restrict__ loadIds,
                                const float4* __restrict__ spheres, uint32_t sphereCount,
@@ -581,11 +582,24 @@ __global__ void k_tile_windows(const NumiClothBagGPUConfig* __restrict__ cfg, co
     const uint32_t tile = blockIdx.x;
     for (uint32_t k = threadIdx.x; k < sphereCount; k += blockDim.x) sphereCache[k] = spheres[k];
     V3 lo = v3(INFINITY, INFINITY, INFINITY), hi = v3(-INFINITY, -INFINITY, -INFINITY);
-    for (uint32_t i = loadStart[tile] + threadIdx.x; i < loadStart[tile + 1]; i += blockDim.x) {
-        const float4 p = pos[loadIds[i]];
-        if (!isfinite(p.z)) continue;
-        lo = v3(fminf(lo.x, p.x), fminf(lo.y, p.y), fminf(lo.z, p.z));
-        hi = v3(fmaxf(hi.x, p.x), fmaxf(hi.y, p.y), fmaxf(hi.z, p.z));
+    // Eight knots per thread and round, every id then every position in
+    // flight together (the loads, not the reduction, bound this kernel).
+    constexpr uint32_t kSlots = 8;
+    const uint32_t begin = loadStart[tile], end = loadStart[tile + 1];
+    for (uint32_t i0 = begin + threadIdx.x; i0 < end; i0 += kSlots * blockDim.x) {
+        uint32_t ids[kSlots];
+#pragma unroll
+        for (uint32_t u = 0; u < kSlots; ++u) ids[u] = loadIds[min(i0 + u * blockDim.x, end - 1u)];
+        float4 q[kSlots];
+#pragma unroll
+        for (uint32_t u = 0; u < kSlots; ++u) q[u] = pos[ids[u]];
+#pragma unroll
+        for (uint32_t u = 0; u < kSlots; ++u) {
+            const float4 p = q[u];
+            if (!isfinite(p.z)) continue;  // slots past the end repeat the last knot
+            lo = v3(fminf(lo.x, p.x), fminf(lo.y, p.y), fminf(lo.z, p.z));
+            hi = v3(fmaxf(hi.x, p.x), fmaxf(hi.y, p.y), fmaxf(hi.z, p.z));
+        }
     }
     __syncthreads();
     buildSphereWindow(lo, hi, cfg->clothMaterial.x, sphereCache, sphereCount, culled, window, scratch);

@@ -714,7 +714,7 @@ struct NumiClothHipSolver {
         const uint32_t n = extension.staticSphereCount;
         for (TileProgram* t : {&tileEven, &tileStrain}) {
             if (!t->valid || t->windows.count == 0) continue;
-            k_tile_windows<<<t->tiles, kBlock, 2 * n * sizeof(float4), stream>>>(
+            k_tile_windows<<<t->tiles, 1024, 2 * n * sizeof(float4), stream>>>(
                 cfg(), posCur, t->loadStart.ptr, t->loadIds.ptr, spheres.ptr, n, t->windows.ptr, t->windowSpheres.ptr);
         }
         tileWindowsBuilt = tileEven.valid && tileEven.windows.count != 0 &&
