@@ -62,7 +62,7 @@ struct SelfGrid {
     uint32_t denseCells;     // nx * ny * nz when it fits the per-cell counters, else 0
     uint32_t cellScanLength; // denseCells when this substep sorts by counting, else 0
     uint32_t radixCount;     // sortCount when this substep sorts by radix, else 0
-    uint32_t activeEntries;  // entries with candidates (run walks), counted by the count pass
+    uint32_t foundCount;     // candidates the run walks appended (may exceed the capacity)
     uint32_t pad[2];
 };
 
@@ -429,11 +429,14 @@ __device__ __forceinline__ uint32_t runHalf(uint32_t q, uint32_t L) {
 // partners left for the locality test in a 64-bit mask, so the lists'
 // latency overlaps it and every lane requests them once, together.
 // entryAt(j) / ijAt(j) return sorted entry j's record and its segment's
-// endpoints (from an LDS copy or global memory). Candidates come out in
-// walk order (increasing step), so the list is identical on every run.
-template <uint32_t W, typename EntryAt, typename IJAt, typename Emit>
+// endpoints (from an LDS copy or global memory). For every 64 steps with
+// candidates, reserve(n) is called with their number, then emit(p, q) for
+// each in walk order (increasing step), so the list is identical on every
+// run.
+template <uint32_t W, typename EntryAt, typename IJAt, typename Reserve, typename Emit>
 __device__ __forceinline__ void forRunCandidates(uint32_t i, uint32_t r0, uint32_t r1, EntryAt entryAt, IJAt ijAt,
-                                                 const uint32_t* __restrict__ localFixed, Emit emit) {
+                                                 const uint32_t* __restrict__ localFixed, Reserve reserve,
+                                                 Emit emit) {
     const SortedEntry self = entryAt(i);
     const uint32_t vs = __float_as_uint(self.a.y);
     const uint32_t s = vs & kEntrySegmentMask;
@@ -493,13 +496,20 @@ __device__ __forceinline__ void forRunCandidates(uint32_t i, uint32_t r0, uint32
                 pending |= 1ull << (d + u);
             }
         }
-        while (pending != 0ull) {
-            const uint32_t d = static_cast<uint32_t>(__builtin_ctzll(pending));
-            pending &= pending - 1ull;
+        uint64_t found = 0;
+        for (uint64_t left = pending; left != 0ull; left &= left - 1ull) {
+            const uint32_t d = static_cast<uint32_t>(__builtin_ctzll(left));
             uint32_t jt = i + d0 + d + 1u;
             if (jt >= r1) jt -= L;
             const uint2 et = ijAt(jt);
-            if (inLists<W>(na, nb, et.x, et.y)) continue;
+            if (!inLists<W>(na, nb, et.x, et.y)) found |= 1ull << d;
+        }
+        if (found == 0ull) continue;
+        reserve(static_cast<uint32_t>(__popcll(found)));
+        for (; found != 0ull; found &= found - 1ull) {
+            const uint32_t d = static_cast<uint32_t>(__builtin_ctzll(found));
+            uint32_t jt = i + d0 + d + 1u;
+            if (jt >= r1) jt -= L;
             const uint32_t t = __float_as_uint(entryAt(jt).a.y) & kEntrySegmentMask;
             emit(min(s, t), max(s, t));
         }
@@ -528,12 +538,11 @@ struct PairArgs {
     uint2* pairs;
     NumiClothBagGPUSelfImpulse* impulses;  // per-candidate impulse aggregates, cleared here
     uint32_t* failure;
-    // Run walks (W > 0): entries with candidates, appended by the count pass
-    // as (entry, run start, run end) in any order (each entry's candidates
-    // go to its own scanned offsets, so the order does not matter), and the
-    // list's capacity.
-    uint4* active;
-    uint32_t activeCapacity;
+    // Run walks (W > 0): the candidates the count pass found, appended as
+    // (entry, rank among the entry's candidates, s, t) in any order (the
+    // emission pass puts each at its entry's scanned offset plus its rank),
+    // and the list's capacity (the candidates' capacity).
+    uint4* found;
 };
 
 // Sorted entry j's record, read from the sorted entries and its segment's box.
@@ -598,15 +607,15 @@ __global__ void k_self_emit_pairs(PairArgs a) {
 }
 
 // Run walks with the two-hop lists in registers (W = 16 or 32 slots).
-// A workgroup owns the cell runs that start in a chunk of kRunChunk sorted
-// entries and stages the window of kRunWindow records from the chunk's start
-// (entries and their segments' endpoints) in LDS, which holds every owned
-// run unless a run is longer than the window's tail; records past the
-// window are read from global memory.
-constexpr uint32_t kRunWindow = 768;
+// A workgroup owns the cell runs that start in a chunk of Chunk sorted
+// entries and stages the window of Window entries from the chunk's start
+// in LDS, which holds every owned run unless a run is longer than the
+// window's tail; such a run is ranked and walked in global memory. Chunks
+// of 640 in windows of 768 keep four workgroups on a CU (a window of 2048
+// for the bag's runs of up to ~130 entries in 10 cm cells measured slower:
+// one workgroup per CU).
 constexpr uint32_t kRunChunk = 640;
-constexpr uint32_t kRunSlots = kRunWindow / kPairThreads;  // window slots per thread, contiguous
-static_assert(kRunSlots * kPairThreads == kRunWindow && kRunChunk < kRunWindow, "run window layout");
+constexpr uint32_t kRunWindow = 768;
 
 // The end of the run of key `key` that continues past `from` (the keys of
 // the sorted or placed entries), found by the whole workgroup 256 entries at
@@ -625,27 +634,29 @@ __device__ uint32_t runEndFrom(const uint32_t* __restrict__ keys, uint32_t n, ui
     }
 }
 
-// Candidate counts of every entry of the owned runs; entries with candidates
-// are appended to the active list with their run bounds. The input is the
-// placed entries of the counting sort (each cell's entries contiguous, in
-// slot order) or the radix sort's sorted entries: each entry of an owned run
-// is ranked by segment within its run (a segment enters a cell once, so
-// ranks are distinct; for sorted input the rank is the entry's own place),
-// its record (cell, value, box, endpoints) goes to LDS at that place, and
-// the counting sort's sorted entries are written out, so the walk sees the
-// order of a stable sort by cell whichever sort ran.
-template <uint32_t W>
+// Candidate counts of every entry of the owned runs, and the candidates
+// themselves appended to the found list. The input is the placed entries
+// of the counting sort (each cell's entries contiguous, in slot order) or
+// the radix sort's sorted entries: each entry of an owned run is ranked by
+// segment within its run (a segment enters a cell once, so ranks are
+// distinct; for sorted input the rank is the entry's own place), its record
+// (cell, value, box, endpoints) goes to LDS at that place, and the counting
+// sort's sorted entries are written out, so the walk sees the order of a
+// stable sort by cell whichever sort ran.
+template <uint32_t W, uint32_t Chunk, uint32_t Window>
 __global__ void __launch_bounds__(kPairThreads) k_self_run_counts(PairArgs a) {
+    constexpr uint32_t kSlots = Window / kPairThreads;  // window slots per thread, contiguous
+    static_assert(kSlots * kPairThreads == Window && Chunk < Window && Window <= 65535u, "run window layout");
     // First the window's keys and values, then the ranked records.
-    __shared__ SortedEntry rec[kRunWindow];
-    __shared__ uint2 recIJ[kRunWindow];
-    __shared__ uint32_t runStart[kRunWindow];  // absolute start of the window's k-th run
-    __shared__ uint16_t runOf[kRunWindow];     // run index of a window slot (0xffff: run began before the chunk)
+    __shared__ SortedEntry rec[Window];
+    __shared__ uint2 recIJ[Window];
+    __shared__ uint32_t runStart[Window];  // absolute start of the window's k-th run
+    __shared__ uint16_t runOf[Window];     // run index of a window slot (0xffff: run began before the chunk)
     __shared__ uint32_t scratch[kPairThreads / 64];
     __shared__ uint32_t info[3];  // runs in the window, (unused), run end search
-    static_assert(2 * sizeof(uint32_t) * kRunWindow <= sizeof(rec), "window keys and values alias the records");
+    static_assert(2 * sizeof(uint32_t) * Window <= sizeof(rec), "window keys and values alias the records");
     uint32_t* const winKey = reinterpret_cast<uint32_t*>(rec);
-    uint32_t* const winVal = winKey + kRunWindow;
+    uint32_t* const winVal = winKey + Window;
     const SelfGrid g = *a.grid;
     if (!g.valid) return;
     const uint32_t n = g.sortCount;
@@ -653,12 +664,12 @@ __global__ void __launch_bounds__(kPairThreads) k_self_run_counts(PairArgs a) {
     const uint32_t* __restrict__ keysIn = counting ? a.placedKeys : a.keys;
     const uint32_t* __restrict__ valsIn = counting ? a.placedVals : a.vals;
     const uint32_t t = threadIdx.x;
-    for (uint32_t c0 = blockIdx.x * kRunChunk; c0 < n; c0 += gridDim.x * kRunChunk) {
-        const uint32_t wn = min(kRunWindow, n - c0);
-        const uint32_t k0 = t * kRunSlots;
-        uint32_t keys[kRunSlots], vals[kRunSlots];
+    for (uint32_t c0 = blockIdx.x * Chunk; c0 < n; c0 += gridDim.x * Chunk) {
+        const uint32_t wn = min(Window, n - c0);
+        const uint32_t k0 = t * kSlots;
+        uint32_t keys[kSlots], vals[kSlots];
 #pragma unroll
-        for (uint32_t u = 0; u < kRunSlots; ++u) {
+        for (uint32_t u = 0; u < kSlots; ++u) {
             const uint32_t k = k0 + u;
             keys[u] = k < wn ? keysIn[c0 + k] : 0u;
             vals[u] = k < wn ? valsIn[c0 + k] : 0u;
@@ -673,7 +684,7 @@ __global__ void __launch_bounds__(kPairThreads) k_self_run_counts(PairArgs a) {
         // numbers the runs and gives every slot its run.
         uint32_t flags = 0, count = 0;
 #pragma unroll
-        for (uint32_t u = 0; u < kRunSlots; ++u) {
+        for (uint32_t u = 0; u < kSlots; ++u) {
             const uint32_t k = k0 + u;
             if (k >= wn) continue;
             const uint32_t before = k == 0u ? keyBefore : (u == 0u ? winKey[k - 1u] : keys[u - 1u]);
@@ -684,7 +695,7 @@ __global__ void __launch_bounds__(kPairThreads) k_self_run_counts(PairArgs a) {
         }
         uint32_t r = blockExclusiveScan256(count, scratch);
 #pragma unroll
-        for (uint32_t u = 0; u < kRunSlots; ++u) {
+        for (uint32_t u = 0; u < kSlots; ++u) {
             const uint32_t k = k0 + u;
             if (k >= wn) continue;
             if ((flags >> u) & 1u) runStart[r++] = c0 + k;
@@ -693,9 +704,9 @@ __global__ void __launch_bounds__(kPairThreads) k_self_run_counts(PairArgs a) {
         if (t == kPairThreads - 1u) info[0] = r;
         __syncthreads();
         const uint32_t runs = info[0];
-        // Owned runs start in the chunk's first kRunChunk slots.
+        // Owned runs start in the chunk's first Chunk slots.
         uint32_t owned = runs;
-        if (wn > kRunChunk) owned = runOf[kRunChunk - 1u] == 0xffffu ? 0u : runOf[kRunChunk - 1u] + 1u;
+        if (wn > Chunk) owned = runOf[Chunk - 1u] == 0xffffu ? 0u : runOf[Chunk - 1u] + 1u;
         if (owned == 0u) {
             __syncthreads();
             continue;
@@ -714,9 +725,9 @@ __global__ void __launch_bounds__(kPairThreads) k_self_run_counts(PairArgs a) {
         // Entries from `giant` on belong to a run that reaches past the
         // window; it is ranked and walked in global memory.
         const uint32_t giant = ownedEnd > c0 + wn ? runStart[owned - 1u] : ownedEnd;
-        uint32_t place[kRunSlots];
+        uint32_t place[kSlots];
 #pragma unroll
-        for (uint32_t u = 0; u < kRunSlots; ++u) {
+        for (uint32_t u = 0; u < kSlots; ++u) {
             const uint32_t k = k0 + u;
             const uint32_t i = c0 + k;
             place[u] = ~0u;
@@ -731,7 +742,7 @@ __global__ void __launch_bounds__(kPairThreads) k_self_run_counts(PairArgs a) {
         }
         __syncthreads();  // the window's keys and values are read no more
 #pragma unroll
-        for (uint32_t u = 0; u < kRunSlots; ++u) {
+        for (uint32_t u = 0; u < kSlots; ++u) {
             if (place[u] == ~0u) continue;
             const uint32_t seg = vals[u] & kEntrySegmentMask;
             const float4 lo = a.boxLo[seg], hi = a.boxHi[seg];
@@ -770,56 +781,58 @@ __global__ void __launch_bounds__(kPairThreads) k_self_run_counts(PairArgs a) {
                 r0 = runStart[run];
                 r1 = run + 1u < runs ? runStart[run + 1u] : ownedEnd;
             }
-            uint32_t found = 0;
-            const auto tally = [&](uint32_t, uint32_t) { ++found; };
-            if (i < giant) forRunCandidates<W>(i, r0, r1, ldsEntry, ldsIJ, a.localFixed, tally);
-            else forRunCandidates<W>(i, r0, r1, globalEntry, globalIJ, a.localFixed, tally);
-            a.counts[i] = found;
-            if (found != 0u) {
-                const uint32_t slot = atomicAdd(&a.grid->activeEntries, 1u);
-                if (slot < a.activeCapacity) a.active[slot] = make_uint4(i, r0, r1, 0u);
-            }
+            // Each group of candidates takes slots of the found list; every
+            // record carries its rank among the entry's candidates.
+            uint32_t total = 0, slot = 0;
+            const auto reserve = [&](uint32_t m) {
+                slot = atomicAdd(&a.grid->foundCount, m);
+            };
+            const auto emit = [&](uint32_t p, uint32_t q) {
+                if (slot < a.capacity) a.found[slot] = make_uint4(i, total, p, q);
+                ++slot;
+                ++total;
+            };
+            if (i < giant) forRunCandidates<W>(i, r0, r1, ldsEntry, ldsIJ, a.localFixed, reserve, emit);
+            else forRunCandidates<W>(i, r0, r1, globalEntry, globalIJ, a.localFixed, reserve, emit);
+            a.counts[i] = total;
         }
         __syncthreads();
     }
 }
 
-// Candidates of the active entries at their scanned offsets, from the
-// sorted entries. Should the active list have overflowed (more entries with
-// candidates than the list holds, which implies more candidates than their
-// capacity, a RANGE failure), every entry with candidates is walked, its
-// run bounds read from the keys, so the emitted list is the same.
+// Candidates at their entries' scanned offsets plus their ranks: a scatter
+// of the found list. Should the list have overflowed (more candidates than
+// their capacity, a RANGE failure, and then which ones the list kept
+// depends on timing), every entry with candidates is walked again, its run
+// bounds read from the keys, and the first `capacity` candidates of the
+// scanned order are written, the same list on every run.
 template <uint32_t W>
 __global__ void __launch_bounds__(kScanThreads) k_self_emit_run_pairs(PairArgs a) {
     const SelfGrid g = *a.grid;
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k == 0) publishPairTotal(a, g);
     if (!g.valid) return;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    if (g.foundCount <= a.capacity) {
+        for (uint32_t u = k; u < g.foundCount; u += stride) {
+            const uint4 c = a.found[u];
+            uint32_t out = a.offsets[c.x] + c.y;
+            emitCandidate(a, out, c.z, c.w);
+        }
+        return;
+    }
     const auto entryAt = [&](uint32_t j) { return sortedRecord(a, j); };
     const auto ijAt = [&](uint32_t j) { return a.ij[a.vals[j] & kEntrySegmentMask]; };
-    const uint32_t stride = gridDim.x * blockDim.x;
-    const bool listed = g.activeEntries <= a.activeCapacity;
     const uint32_t n = g.sortCount;
-    const uint32_t count = listed ? g.activeEntries : n;
-    for (uint32_t u = k; u < count; u += stride) {
-        uint32_t e, r0, r1;
-        if (listed) {
-            const uint4 w = a.active[u];
-            e = w.x;
-            r0 = w.y;
-            r1 = w.z;
-        } else {
-            e = u;
-            const uint32_t end = e + 1 < n ? a.offsets[e + 1] : *a.pairTotal;
-            if (end == a.offsets[e]) continue;
-            const uint32_t key = a.keys[e];
-            r0 = e;
-            r1 = e + 1u;
-            while (r0 > 0u && a.keys[r0 - 1u] == key) --r0;
-            while (r1 < n && a.keys[r1] == key) ++r1;
-        }
+    for (uint32_t e = k; e < n; e += stride) {
         uint32_t out = a.offsets[e];
-        forRunCandidates<W>(e, r0, r1, entryAt, ijAt, a.localFixed,
+        const uint32_t end = e + 1 < n ? a.offsets[e + 1] : *a.pairTotal;
+        if (end == out) continue;
+        const uint32_t key = a.keys[e];
+        uint32_t r0 = e, r1 = e + 1u;
+        while (r0 > 0u && a.keys[r0 - 1u] == key) --r0;
+        while (r1 < n && a.keys[r1] == key) ++r1;
+        forRunCandidates<W>(e, r0, r1, entryAt, ijAt, a.localFixed, [](uint32_t) {},
                             [&](uint32_t p, uint32_t q) { emitCandidate(a, out, p, q); });
     }
 }

@@ -175,7 +175,7 @@ struct NumiClothHipSolver {
     DeviceArray<uint32_t> segmentCounts, entryCounts, scanSums, totals;
     DeviceArray<uint32_t> keysA, valsA, keysB, valsB, histogram;
     DeviceArray<uint2> pairs;
-    DeviceArray<uint4> activeEntries;  // run walks: (entry, run start, run end) of entries with candidates
+    DeviceArray<uint4> foundPairs;  // run walks: (entry, rank, s, t) of every candidate, in any order
     // Two fixed-point accumulators and touched masks (one bit per knot with a
     // pending correction), used by alternate self-contact solves: a tile
     // program that applies one buffer's corrections cannot clear it (other
@@ -524,16 +524,15 @@ struct NumiClothHipSolver {
         pa.pairs = pairs.ptr;
         pa.impulses = selfImpulses.ptr;
         pa.failure = failure.ptr;
-        pa.active = activeEntries.ptr;
-        pa.activeCapacity = pairCapacity;
+        pa.found = foundPairs.ptr;
         // Lists that fit 16 or 32 register slots: run walks over LDS windows
-        // (one workgroup per 640 entries' runs), then the emit pass over the
-        // entries that found candidates. Longer lists: one thread per entry
-        // with the lists searched in memory.
+        // (one workgroup per chunk's runs) that also append the candidates,
+        // then the emission pass puts them in scanned order. Longer lists:
+        // one thread per entry with the lists searched in memory.
         if (localWidth != 0) {
             const uint32_t chunks = std::min<uint32_t>(4096, (entryCapacity + kRunChunk - 1) / kRunChunk);
-            if (localWidth == 16) k_self_run_counts<16><<<chunks, kPairThreads, 0, stream>>>(pa);
-            else k_self_run_counts<32><<<chunks, kPairThreads, 0, stream>>>(pa);
+            if (localWidth == 16) k_self_run_counts<16, kRunChunk, kRunWindow><<<chunks, kPairThreads, 0, stream>>>(pa);
+            else k_self_run_counts<32, kRunChunk, kRunWindow><<<chunks, kPairThreads, 0, stream>>>(pa);
         } else {
             k_self_pair_counts<<<entryBlocks, kPairThreads, 0, stream>>>(pa);
         }
@@ -1442,7 +1441,7 @@ void setupSelfContact(const NumiClothHipSceneDesc& scene, NumiClothHipSolver& s)
     s.valsB.allocate(s.entryCapacity);
     s.histogram.allocate(histLength);
     s.pairs.allocate(s.pairCapacity);
-    if (s.localWidth != 0) s.activeEntries.allocate(s.pairCapacity);
+    if (s.localWidth != 0) s.foundPairs.allocate(s.pairCapacity);
     s.selfImpulses.allocate(s.pairCapacity);  // written by the emit pass before any use
     for (int b = 0; b < 2; ++b) {
         s.accumulator[b].allocate(s.P);
