@@ -137,17 +137,6 @@ __device__ __forceinline__ float yarnVisit(const YarnArgs& a, NumiClothBagGPUFru
     return travel;
 }
 
-// Can visit (f, s) act? Swept mode: its impact flag. Present mode: the fruit
-// is within margin of contact at the start of the call.
-__device__ __forceinline__ bool yarnCandidate(const YarnArgs& a, uint32_t f, uint32_t s, const float4* knotsStart,
-                                              const NumiClothBagGPUFruit* fruitsStart) {
-    if (a.mode == 1u) return a.contacts[f * a.segments + s].control.y != 0u;
-    const uint2 e = a.ij[s];
-    const NumiClothBagGPUFruit fr = fruitsStart[f];
-    const PointSample ps = samplePointSegment(xyz(fr.positionAndInverseMass), xyz(knotsStart[e.x]), xyz(knotsStart[e.y]));
-    return ps.distance < fr.previousAndRadius.w + a.cfg->clothMaterial.x + a.margin;
-}
-
 // Exclusive scan of counts[0..n) in place by the whole workgroup; returns
 // the total (same value in every thread).
 __device__ uint32_t blockExclusiveScan(uint32_t* counts, uint32_t n, uint32_t* scratch) {
@@ -257,16 +246,34 @@ __global__ void __launch_bounds__(kYarnThreads) k_yarn_solve(YarnArgs a) {
         movedBits = 0u;
     }
     __syncthreads();
-    // 1. candidate visits: counted per phase and listed (start state in LDS)
-    const uint32_t pairs = F * D;
-    for (uint32_t idx = tid; idx < pairs; idx += threads) {
-        const uint32_t f = idx / D, s = idx % D;
+    // 1. candidate visits: counted per phase and listed (start state in LDS).
+    // A visit can act in the swept mode when its record's impact flag is
+    // set, in the present mode when the fruit is within margin of contact
+    // at the start of the call. A thread takes segments and tests every
+    // fruit against each, so a segment's colour and endpoints are read once
+    // (nothing depends on the order of the listing).
+    const float yarnRadius = a.cfg->clothMaterial.x;
+    for (uint32_t s = tid; s < D; s += threads) {
         const uint2 batch = a.segmentBatch[s];
-        if (batch.y == 0u || !yarnCandidate(a, f, s, knots, fruitStart)) continue;
-        atomicAdd(&counts[phaseBit(f, s, batch)], 1u);
-        atomicOr(&visitors[s], 1u << f);
-        const uint32_t k = atomicAdd(&candidateCount, 1u);
-        if (k < kYarnSlots) candidates[k] = idx;
+        if (batch.y == 0u) continue;
+        const uint2 e = a.ij[s];
+        const V3 k0 = xyz(knots[e.x]), k1 = xyz(knots[e.y]);
+        for (uint32_t f = 0; f < F; ++f) {
+            const uint32_t idx = f * D + s;
+            bool candidate;
+            if (a.mode == 1u) {
+                candidate = a.contacts[idx].control.y != 0u;
+            } else {
+                const NumiClothBagGPUFruit& fr = fruitStart[f];
+                const PointSample ps = samplePointSegment(xyz(fr.positionAndInverseMass), k0, k1);
+                candidate = ps.distance < fr.previousAndRadius.w + yarnRadius + a.margin;
+            }
+            if (!candidate) continue;
+            atomicAdd(&counts[phaseBit(f, s, batch)], 1u);
+            atomicOr(&visitors[s], 1u << f);
+            const uint32_t k = atomicAdd(&candidateCount, 1u);
+            if (k < kYarnSlots) candidates[k] = idx;
+        }
     }
     __syncthreads();
     const uint32_t total = candidateCount;
@@ -363,17 +370,22 @@ __global__ void __launch_bounds__(kYarnThreads) k_yarn_solve(YarnArgs a) {
         for (uint32_t f = 0; f < F; ++f) fruitMax = fmaxf(fruitMax, fruitMoved[f]);
         const bool cheap = fruitMax + __uint_as_float(movedBits) < a.margin;
         if (!cheap) {
+            // Segment-major like the listing: a segment's colour, endpoints
+            // and start positions are read once for all fruits.
             bool bad = false;
-            for (uint32_t idx = tid; idx < pairs; idx += threads) {
-                const uint32_t f = idx / D, s = idx % D;
-                if (a.segmentBatch[s].y == 0u || yarnCandidate(a, f, s, a.pos, fruitStart)) continue;
+            for (uint32_t s = tid; s < D; s += threads) {
+                if (a.segmentBatch[s].y == 0u) continue;
                 const uint2 e = a.ij[s];
-                const NumiClothBagGPUFruit fr = fruitStart[f];
-                const PointSample ps =
-                    samplePointSegment(xyz(fr.positionAndInverseMass), xyz(a.pos[e.x]), xyz(a.pos[e.y]));
-                const float target = fr.previousAndRadius.w + a.cfg->clothMaterial.x;
-                const float reach = fruitMoved[f] + fmaxf(moved[e.x], moved[e.y]);
-                if (!(ps.distance - reach >= target)) bad = true;
+                const V3 k0 = xyz(a.pos[e.x]), k1 = xyz(a.pos[e.y]);
+                const float knotReach = fmaxf(moved[e.x], moved[e.y]);
+                for (uint32_t f = 0; f < F; ++f) {
+                    const NumiClothBagGPUFruit& fr = fruitStart[f];
+                    const PointSample ps = samplePointSegment(xyz(fr.positionAndInverseMass), k0, k1);
+                    if (ps.distance < fr.previousAndRadius.w + yarnRadius + a.margin) continue;  // listed above
+                    const float target = fr.previousAndRadius.w + yarnRadius;
+                    const float reach = fruitMoved[f] + knotReach;
+                    if (!(ps.distance - reach >= target)) bad = true;
+                }
             }
             if (bad) verdict = 1u;
         }
