@@ -137,6 +137,21 @@ __device__ __forceinline__ float yarnVisit(const YarnArgs& a, NumiClothBagGPUFru
     return travel;
 }
 
+// Whether point c is certainly farther than `limit` from the segment of
+// midpoint m and half length h: |c - m| - h > limit plus a slack that covers
+// the rounding of the distances (point-segment distances computed in float
+// are within a few ulps of the scene's coordinates of the true ones). The
+// exact test (samplePointSegment) is then sure to fail, so the candidate
+// listing and the check of the unlisted visits skip it; both ways give the
+// same candidates. Non-finite input is never "beyond".
+__device__ __forceinline__ bool beyondSegment(V3 c, V3 m, float h, float limit) {
+    const V3 q = c - m;
+    const float scale = fmaxf(fmaxf(fmaxf(fabsf(c.x), fabsf(c.y)), fmaxf(fabsf(c.z), fabsf(m.x))),
+                              fmaxf(fabsf(m.y), fabsf(m.z)));
+    const float r = limit + h + 1.0e-4f + 1.0e-5f * scale;
+    return dot(q, q) > r * r;
+}
+
 // Exclusive scan of counts[0..n) in place by the whole workgroup; returns
 // the total (same value in every thread).
 __device__ uint32_t blockExclusiveScan(uint32_t* counts, uint32_t n, uint32_t* scratch) {
@@ -258,6 +273,8 @@ __global__ void __launch_bounds__(kYarnThreads) k_yarn_solve(YarnArgs a) {
         if (batch.y == 0u) continue;
         const uint2 e = a.ij[s];
         const V3 k0 = xyz(knots[e.x]), k1 = xyz(knots[e.y]);
+        const V3 mid = (k0 + k1) * 0.5f;
+        const float half = 0.5f * length(k1 - k0);
         for (uint32_t f = 0; f < F; ++f) {
             const uint32_t idx = f * D + s;
             bool candidate;
@@ -265,8 +282,9 @@ __global__ void __launch_bounds__(kYarnThreads) k_yarn_solve(YarnArgs a) {
                 candidate = a.contacts[idx].control.y != 0u;
             } else {
                 const NumiClothBagGPUFruit& fr = fruitStart[f];
-                const PointSample ps = samplePointSegment(xyz(fr.positionAndInverseMass), k0, k1);
-                candidate = ps.distance < fr.previousAndRadius.w + yarnRadius + a.margin;
+                const V3 c = xyz(fr.positionAndInverseMass);
+                const float limit = fr.previousAndRadius.w + yarnRadius + a.margin;
+                candidate = !beyondSegment(c, mid, half, limit) && samplePointSegment(c, k0, k1).distance < limit;
             }
             if (!candidate) continue;
             atomicAdd(&counts[phaseBit(f, s, batch)], 1u);
@@ -377,13 +395,19 @@ __global__ void __launch_bounds__(kYarnThreads) k_yarn_solve(YarnArgs a) {
                 if (a.segmentBatch[s].y == 0u) continue;
                 const uint2 e = a.ij[s];
                 const V3 k0 = xyz(a.pos[e.x]), k1 = xyz(a.pos[e.y]);
+                const V3 mid = (k0 + k1) * 0.5f;
+                const float half = 0.5f * length(k1 - k0);
                 const float knotReach = fmaxf(moved[e.x], moved[e.y]);
                 for (uint32_t f = 0; f < F; ++f) {
                     const NumiClothBagGPUFruit& fr = fruitStart[f];
-                    const PointSample ps = samplePointSegment(xyz(fr.positionAndInverseMass), k0, k1);
-                    if (ps.distance < fr.previousAndRadius.w + yarnRadius + a.margin) continue;  // listed above
+                    const V3 c = xyz(fr.positionAndInverseMass);
                     const float target = fr.previousAndRadius.w + yarnRadius;
                     const float reach = fruitMoved[f] + knotReach;
+                    // Far beyond contact plus reach: an unlisted visit that
+                    // stayed out of contact, without the exact distance.
+                    if (beyondSegment(c, mid, half, target + reach)) continue;
+                    const PointSample ps = samplePointSegment(c, k0, k1);
+                    if (ps.distance < fr.previousAndRadius.w + yarnRadius + a.margin) continue;  // listed above
                     if (!(ps.distance - reach >= target)) bad = true;
                 }
             }
