@@ -280,12 +280,14 @@ __device__ __forceinline__ void projectFruitPair(NumiClothBagGPUFruit& a, NumiCl
 // All fruit-pair colours of one solve pass in one workgroup: fruits staged
 // in LDS, a barrier between colours (the reference launches one dispatch per
 // colour, :4519-4528).
-__global__ void k_fruit_pairs(const NumiClothBagGPUConfig* __restrict__ cfg, const uint32_t* __restrict__ ok,
-                              const uint2* __restrict__ colours, uint32_t colourCount,
-                              NumiClothBagGPUFruit* __restrict__ fruits, uint32_t fruitCount,
-                              NumiClothBagGPUFruitPair* __restrict__ pairs, uint32_t* __restrict__ failure) {
-    __shared__ NumiClothBagGPUFruit local[32];
-    if (*ok == 0u) return;
+// All fruit-pair colours in one workgroup (every thread calls it): the
+// fruits in LDS (`local`, at least fruitCount records), one pair per thread
+// within a colour (a colour's pairs share no fruit), a barrier between
+// colours; the fruits are written back and the workgroup synchronised.
+__device__ void fruitPairColours(const NumiClothBagGPUConfig* __restrict__ cfg, const uint2* __restrict__ colours,
+                                 uint32_t colourCount, NumiClothBagGPUFruit* __restrict__ fruits, uint32_t fruitCount,
+                                 NumiClothBagGPUFruitPair* __restrict__ pairs, uint32_t* __restrict__ failure,
+                                 NumiClothBagGPUFruit* local) {
     for (uint32_t i = threadIdx.x; i < fruitCount; i += blockDim.x) local[i] = fruits[i];
     __syncthreads();
     const float dt = cfg->gravityAndTimestep.w;
@@ -306,6 +308,16 @@ __global__ void k_fruit_pairs(const NumiClothBagGPUConfig* __restrict__ cfg, con
         __syncthreads();
     }
     for (uint32_t i = threadIdx.x; i < fruitCount; i += blockDim.x) fruits[i] = local[i];
+    __syncthreads();
+}
+
+__global__ void k_fruit_pairs(const NumiClothBagGPUConfig* __restrict__ cfg, const uint32_t* __restrict__ ok,
+                              const uint2* __restrict__ colours, uint32_t colourCount,
+                              NumiClothBagGPUFruit* __restrict__ fruits, uint32_t fruitCount,
+                              NumiClothBagGPUFruitPair* __restrict__ pairs, uint32_t* __restrict__ failure) {
+    __shared__ NumiClothBagGPUFruit local[32];
+    if (*ok == 0u) return;
+    fruitPairColours(cfg, colours, colourCount, fruits, fruitCount, pairs, failure, local);
 }
 
 // Fruit part of solve_ground: z >= radius with the impulse that removes the
@@ -315,17 +327,7 @@ __global__ void k_fruit_ground(const NumiClothBagGPUConfig* __restrict__ cfg, co
                                uint32_t* __restrict__ failure) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= fruitCount || *ok == 0u || cfg->constraintCounts.z == 0u) return;
-    NumiClothBagGPUFruit f = fruits[i];
-    const float penetration = f.previousAndRadius.w - f.positionAndInverseMass.z;
-    if (!isfinite(penetration)) {
-        recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
-        return;
-    }
-    if (penetration > 0.0f) {
-        f.velocityAndGroundImpulse.w += penetration / (f.positionAndInverseMass.w * cfg->gravityAndTimestep.w);
-        f.positionAndInverseMass.z = f.previousAndRadius.w;
-        fruits[i] = f;
-    }
+    fruitGround(cfg, fruits, i, failure);
 }
 
 struct PointSample {

@@ -645,12 +645,35 @@ __device__ __forceinline__ void knotBox(const float4& p, bool has, V3& lo, V3& h
     hi = has ? xyz(p) : v3(-INFINITY, -INFINITY, -INFINITY);
 }
 
+// Fruit part of solve_ground: z >= radius with the impulse that removes the
+// penetration over the substep.
+__device__ __forceinline__ void fruitGround(const NumiClothBagGPUConfig* __restrict__ cfg,
+                                            NumiClothBagGPUFruit* __restrict__ fruits, uint32_t i,
+                                            uint32_t* __restrict__ failure) {
+    NumiClothBagGPUFruit f = fruits[i];
+    const float penetration = f.previousAndRadius.w - f.positionAndInverseMass.z;
+    if (!isfinite(penetration)) {
+        recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
+        return;
+    }
+    if (penetration > 0.0f) {
+        f.velocityAndGroundImpulse.w += penetration / (f.positionAndInverseMass.w * cfg->gravityAndTimestep.w);
+        f.positionAndInverseMass.z = f.previousAndRadius.w;
+        fruits[i] = f;
+    }
+}
+
 // Ground and static spheres for every knot. Dynamic shared memory: the
-// spheres, then the window's list (2 * sphereCount float4).
+// spheres, then the window's list (2 * sphereCount float4). With fruits
+// (fruitCount > 0) the first workgroup's first threads also run the fruit
+// part of the ground pass (the schedule's next launch, on the fruits only).
 __global__ void k_collide(const NumiClothBagGPUConfig* __restrict__ cfg, const uint32_t* __restrict__ ok,
                           float4* __restrict__ pos, uint32_t particles, const float4* __restrict__ spheres,
-                          uint32_t sphereCount, uint32_t windowMin, uint32_t* __restrict__ failure) {
+                          uint32_t sphereCount, uint32_t windowMin, uint32_t* __restrict__ failure,
+                          NumiClothBagGPUFruit* __restrict__ fruits = nullptr, uint32_t fruitCount = 0u) {
     extern __shared__ float4 sphereCache[];
+    if (blockIdx.x == 0 && threadIdx.x < fruitCount && *ok != 0u && cfg->constraintCounts.z != 0u)
+        fruitGround(cfg, fruits, threadIdx.x, failure);
     __shared__ SphereWindow window;
     __shared__ float4 scratch[2 * kMaxBlockWaves];
     float4* culled = sphereCache + sphereCount;

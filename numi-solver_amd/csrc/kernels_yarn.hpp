@@ -71,6 +71,11 @@ struct YarnArgs {
     float margin;                        // present mode candidate margin
     uint32_t* stats;                     // [reruns visit by visit, most candidates], or null
     uint32_t* failure;
+    // The fruit-pair colours the schedule runs just before (0: none): the
+    // solve runs them first, in its own workgroup.
+    const uint2* pairColours;
+    uint32_t pairColourCount;
+    NumiClothBagGPUFruitPair* fruitPairs;
 };
 
 __device__ __forceinline__ uint32_t phaseBit(uint32_t f, uint32_t s, uint2 batch) {
@@ -237,6 +242,8 @@ __global__ void __launch_bounds__(kYarnThreads) k_yarn_solve(YarnArgs a) {
     if (*a.ok == 0u) return;
     const uint32_t tid = threadIdx.x, threads = blockDim.x;
     const uint32_t P = a.particles, D = a.segments, F = a.fruitCount;
+    if (a.pairColourCount != 0u)
+        fruitPairColours(a.cfg, a.pairColours, a.pairColourCount, a.fruits, F, a.fruitPairs, a.failure, fruitStart);
     float4* knots = yarnLds;
     float* moved = reinterpret_cast<float*>(knots + P);
     uint32_t* counts = reinterpret_cast<uint32_t*>(moved + P);

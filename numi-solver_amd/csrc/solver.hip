@@ -367,12 +367,33 @@ struct NumiClothHipSolver {
         return a;
     }
 
-    void launchYarn(uint32_t mode) {
+    void launchYarn(uint32_t mode, bool withPairs = false) {
         if (Y == 0) return;
         flushCollide();
         const size_t lds = static_cast<size_t>(P) * (sizeof(float4) + sizeof(float)) + static_cast<size_t>(D) * 12;
-        k_yarn_solve<<<1, kYarnThreads, lds, stream>>>(yarnArgs(mode, posCur));
+        YarnArgs a = yarnArgs(mode, posCur);
+        if (withPairs) {
+            a.pairColours = pairColourRanges.ptr;
+            a.pairColourCount = static_cast<uint32_t>(pairColours.size());
+            a.fruitPairs = fruitPairs.ptr;
+        }
+        k_yarn_solve<<<1, kYarnThreads, lds, stream>>>(a);
     }
+
+    // The fruit-pair colours and the present fruit/yarn solve that follows
+    // them, in one workgroup launch when the scene has both.
+    void launchFruitPairsAndYarn() {
+        if (Y != 0 && !pairColours.empty()) {
+            launchYarn(0, true);
+            return;
+        }
+        launchFruitPairs();
+        launchYarn(0);
+    }
+
+    // Whether a ground/sphere launch also runs the fruit ground pass (fruits
+    // and the ground on): k_collide takes it then.
+    bool collideTakesFruitGround() const { return F != 0 && initialConfig.constraintCounts.z != 0u; }
 
     void launchBuildYarnContacts() {
         if (Y == 0) return;
@@ -387,24 +408,31 @@ struct NumiClothHipSolver {
         k_fruit_ground<<<blocksFor(F), kBlock, 0, stream>>>(cfg(), ok(), fruits.ptr, F, failure.ptr);
     }
 
-    // solveFinalContacts of the reference: fruit pairs, fruit/yarn, ground.
+    // solveFinalContacts of the reference: fruit pairs, fruit/yarn, ground
+    // (cloth, then fruits).
     void launchFinalContacts() {
-        launchFruitPairs();
-        launchYarn(0);
-        launchCollide();
-        launchFruitGround();
+        launchFruitPairsAndYarn();
+        launchCollide(true);
+        if (!collideTakesFruitGround()) launchFruitGround();
     }
 
-    void launchCollide() {
+    // fruitGround: the fruit ground pass that follows joins the launch (the
+    // caller then skips launchFruitGround when collideTakesFruitGround()).
+    void launchCollide(bool fruitGround = false) {
         flushCollide();
-        collideNow();
+        collideNow(fruitGround);
     }
 
-    void collideNow() {
-        if (!groundOrColliders()) return;
+    void collideNow(bool fruitGround = false) {
+        const bool withFruits = fruitGround && collideTakesFruitGround();
+        if (!groundOrColliders()) {
+            if (withFruits) launchFruitGround();
+            return;
+        }
         const uint32_t n = extension.staticSphereCount;
-        k_collide<<<blocksFor(P), kBlock, 2 * n * sizeof(float4), stream>>>(cfg(), ok(), posCur, P, spheres.ptr, n,
-                                                                            sphereWindowMin, failure.ptr);
+        k_collide<<<blocksFor(P), kBlock, 2 * n * sizeof(float4), stream>>>(
+            cfg(), ok(), posCur, P, spheres.ptr, n, sphereWindowMin, failure.ptr, withFruits ? fruits.ptr : nullptr,
+            withFruits ? F : 0u);
     }
 
     // A ground/sphere pass that follows self-contact can run at the start of
@@ -548,7 +576,8 @@ struct NumiClothHipSolver {
 
     // collide: run the ground/sphere pass that follows; defer: leave it to the
     // next launch (the next tile program takes it as its leading pass).
-    void launchSelf(uint32_t mode, bool collide = false, bool defer = false) {
+    // fruitGround: a collide run here also takes the fruit ground pass.
+    void launchSelf(uint32_t mode, bool collide = false, bool defer = false, bool fruitGround = false) {
         flushCollide();
         if (!self) return;
         const uint32_t b = selfBuffer;
@@ -573,7 +602,8 @@ struct NumiClothHipSolver {
             return;
         }
         applySelf(b, posCur);
-        if (colliding) collideNow();
+        if (colliding) collideNow(fruitGround);
+        else if (fruitGround && collideTakesFruitGround()) launchFruitGround();
     }
 
     void applySelf(uint32_t b, float4* target) {
@@ -823,19 +853,26 @@ struct NumiClothHipSolver {
                 if (it % 2 == 0) launchBends();
             }
             launchGrips();
-            launchFruitPairs();
-            launchYarn(0);
+            launchFruitPairsAndYarn();
+            // The fruit ground pass joins the ground/sphere launch where one
+            // follows here, or runs on its own.
+            bool fruitGroundDone = false;
             if (pointSelf) {
                 // segment self-contact, point self-collision, then ground
                 if (self) launchSelf(0, false, false);
                 launchPointSelf();
-                if (colliders) launchCollide();
+                if (colliders) {
+                    launchCollide(true);
+                    fruitGroundDone = true;
+                }
             } else if (self) {
-                launchSelf(0, colliders, defer);
+                launchSelf(0, colliders, defer, !defer);
+                fruitGroundDone = !defer;
             } else if (!(tiled && trailing)) {
-                launchCollide();
+                launchCollide(true);
+                fruitGroundDone = true;
             }
-            launchFruitGround();
+            if (!fruitGroundDone || !collideTakesFruitGround()) launchFruitGround();
         }
         launchStrainAndCollide(sweeps, 0);
         if (sweeps != 0) {
@@ -846,8 +883,7 @@ struct NumiClothHipSolver {
             // input and program are the unchanged round's).
             const bool repeating = F == 0 && FP == 0 && Y == 0;
             for (uint32_t pass = 0; pass < kReconciliationPasses; ++pass) {
-                launchFruitPairs();
-                launchYarn(0);
+                launchFruitPairsAndYarn();
                 launchStrainAndCollide(sweeps, 1, repeating ? static_cast<int>(pass) : -1);
                 launchFruitGround();
             }
