@@ -436,11 +436,11 @@ __global__ void k_grip_attach(const NumiClothBagGPUConfig* __restrict__ cfg, con
 
 // solve_grip (reference src/metal/ClothBag.metal:1449-1501): vector XPBD
 // towards the handle-frame target g + q o.
-__global__ void k_grip(const NumiClothBagGPUConfig* __restrict__ cfg, const uint32_t* __restrict__ ok,
-                       float4* __restrict__ pos, NumiClothBagGPUGrip* __restrict__ grips, uint32_t gripCount,
-                       uint32_t particles, uint32_t* __restrict__ failure) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (*ok == 0u || i >= gripCount || i >= cfg->control.w || !(cfg->gripTargetAndActive.w > 0.0f)) return;
+// Grip i of solve_grip (the caller has checked the config is valid).
+__device__ __forceinline__ void projectGrip(const NumiClothBagGPUConfig* __restrict__ cfg, float4* __restrict__ pos,
+                                            NumiClothBagGPUGrip* __restrict__ grips, uint32_t i, uint32_t gripCount,
+                                            uint32_t particles, uint32_t* __restrict__ failure) {
+    if (i >= gripCount || i >= cfg->control.w || !(cfg->gripTargetAndActive.w > 0.0f)) return;
     NumiClothBagGPUGrip grip = grips[i];
     if (grip.particle.y != cfg->gripControl.x) return;
     const uint32_t index = grip.particle.x;
@@ -473,6 +473,13 @@ __global__ void k_grip(const NumiClothBagGPUConfig* __restrict__ cfg, const uint
     grip.lambda.z = nl.z;
     pos[index] = withXyz(p, np);
     grips[i] = grip;
+}
+
+__global__ void k_grip(const NumiClothBagGPUConfig* __restrict__ cfg, const uint32_t* __restrict__ ok,
+                       float4* __restrict__ pos, NumiClothBagGPUGrip* __restrict__ grips, uint32_t gripCount,
+                       uint32_t particles, uint32_t* __restrict__ failure) {
+    if (*ok == 0u) return;
+    projectGrip(cfg, pos, grips, blockIdx.x * blockDim.x + threadIdx.x, gripCount, particles, failure);
 }
 
 // ------------------------------------------------------ ground + spheres --
@@ -663,14 +670,26 @@ __device__ __forceinline__ void fruitGround(const NumiClothBagGPUConfig* __restr
     }
 }
 
+// The pending self-contact corrections of one knot (the mean of its
+// fixed-point sum, when its touched bit is set in `word`) and the
+// accumulator cleared for reuse; live = false clears without moving.
+struct FixedAccumulator;
+__device__ __forceinline__ void applyPendingCorrection(float4& p, uint32_t i, uint32_t word, bool live,
+                                                       FixedAccumulator* __restrict__ acc,
+                                                       uint32_t* __restrict__ failure);
+
 // Ground and static spheres for every knot. Dynamic shared memory: the
 // spheres, then the window's list (2 * sphereCount float4). With fruits
 // (fruitCount > 0) the first workgroup's first threads also run the fruit
 // part of the ground pass (the schedule's next launch, on the fruits only).
+// With selfAcc the self-contact corrections of the solve before it are
+// applied first (its accumulator and touched mask cleared), as the
+// schedule's apply pass would.
 __global__ void k_collide(const NumiClothBagGPUConfig* __restrict__ cfg, const uint32_t* __restrict__ ok,
                           float4* __restrict__ pos, uint32_t particles, const float4* __restrict__ spheres,
                           uint32_t sphereCount, uint32_t windowMin, uint32_t* __restrict__ failure,
-                          NumiClothBagGPUFruit* __restrict__ fruits = nullptr, uint32_t fruitCount = 0u) {
+                          NumiClothBagGPUFruit* __restrict__ fruits = nullptr, uint32_t fruitCount = 0u,
+                          FixedAccumulator* __restrict__ selfAcc = nullptr, uint32_t* __restrict__ selfTouched = nullptr) {
     extern __shared__ float4 sphereCache[];
     if (blockIdx.x == 0 && threadIdx.x < fruitCount && *ok != 0u && cfg->constraintCounts.z != 0u)
         fruitGround(cfg, fruits, threadIdx.x, failure);
@@ -681,7 +700,18 @@ __global__ void k_collide(const NumiClothBagGPUConfig* __restrict__ cfg, const u
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = *ok != 0u;
     const bool has = live && i < particles;
-    const float4 p0 = has ? pos[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float4 p0 = has ? pos[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    bool corrected = false;
+    if (selfAcc != nullptr && i < particles) {
+        // Every lane of the 32-knot word reads it before its first lane
+        // clears it (the lanes of one wave, in order).
+        const uint32_t word = selfTouched[i >> 5];
+        if ((word >> (i & 31u)) & 1u) {
+            applyPendingCorrection(p0, i, word, live, selfAcc, failure);
+            corrected = live;
+        }
+        if ((i & 31u) == 0u && word != 0u) selfTouched[i >> 5] = 0u;
+    }
     const float r = cfg->clothMaterial.x;
     const bool windowed = sphereCount >= windowMin;
     if (windowed) {
@@ -700,7 +730,7 @@ __global__ void k_collide(const NumiClothBagGPUConfig* __restrict__ cfg, const u
     const bool ground = cfg->constraintCounts.z != 0u;
     const bool moved = windowed ? collideKnotWindow(p, ground, r, window, culled, sphereCache, sphereCount)
                                 : collideKnot(p, ground, r, sphereCache, sphereCount);
-    if (moved) pos[i] = p;
+    if (moved || corrected) pos[i] = p;
 }
 
 // ------------------------------------------------------------- finalize --

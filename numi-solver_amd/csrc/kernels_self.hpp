@@ -1073,6 +1073,18 @@ __global__ void k_self_solve(const NumiClothBagGPUConfig* __restrict__ cfg, cons
 // 32-knot word of the touched mask.
 // pos = nullptr: clear only (a buffer whose corrections were applied by the
 // tile programs that loaded the knots).
+__device__ __forceinline__ void applyPendingCorrection(float4& p, uint32_t i, uint32_t word, bool live,
+                                                       FixedAccumulator* __restrict__ acc,
+                                                       uint32_t* __restrict__ failure) {
+    (void)word;
+    if (live) {
+        const V3 np = xyz(p) + fixedMean(acc[i]);
+        if (!finite3(np)) recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
+        else p = withXyz(p, np);
+    }
+    acc[i] = FixedAccumulator{0, 0, 0, 0, 0u, 0u};
+}
+
 __global__ void k_self_apply(const uint32_t* __restrict__ ok, float4* __restrict__ pos,
                              FixedAccumulator* __restrict__ acc, uint32_t* __restrict__ touched, uint32_t words,
                              uint32_t* __restrict__ failure) {

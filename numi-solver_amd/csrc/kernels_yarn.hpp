@@ -71,8 +71,10 @@ struct YarnArgs {
     float margin;                        // present mode candidate margin
     uint32_t* stats;                     // [reruns visit by visit, most candidates], or null
     uint32_t* failure;
-    // The fruit-pair colours the schedule runs just before (0: none): the
-    // solve runs them first, in its own workgroup.
+    // The grips and fruit-pair colours the schedule runs just before (0:
+    // none): the solve runs them first, in its own workgroup.
+    NumiClothBagGPUGrip* grips;
+    uint32_t gripCount;
     const uint2* pairColours;
     uint32_t pairColourCount;
     NumiClothBagGPUFruitPair* fruitPairs;
@@ -242,6 +244,12 @@ __global__ void __launch_bounds__(kYarnThreads) k_yarn_solve(YarnArgs a) {
     if (*a.ok == 0u) return;
     const uint32_t tid = threadIdx.x, threads = blockDim.x;
     const uint32_t P = a.particles, D = a.segments, F = a.fruitCount;
+    if (a.gripCount != 0u) {
+        // solve_grip: one grip per thread, each on its own knot.
+        for (uint32_t i = tid; i < a.gripCount; i += threads)
+            projectGrip(a.cfg, a.pos, a.grips, i, a.gripCount, P, a.failure);
+        __syncthreads();
+    }
     if (a.pairColourCount != 0u)
         fruitPairColours(a.cfg, a.pairColours, a.pairColourCount, a.fruits, F, a.fruitPairs, a.failure, fruitStart);
     float4* knots = yarnLds;

@@ -367,11 +367,15 @@ struct NumiClothHipSolver {
         return a;
     }
 
-    void launchYarn(uint32_t mode, bool withPairs = false) {
+    void launchYarn(uint32_t mode, bool withPairs = false, bool withGrips = false) {
         if (Y == 0) return;
         flushCollide();
         const size_t lds = static_cast<size_t>(P) * (sizeof(float4) + sizeof(float)) + static_cast<size_t>(D) * 12;
         YarnArgs a = yarnArgs(mode, posCur);
+        if (withGrips && G != 0) {
+            a.grips = grips.ptr;
+            a.gripCount = G;
+        }
         if (withPairs) {
             a.pairColours = pairColourRanges.ptr;
             a.pairColourCount = static_cast<uint32_t>(pairColours.size());
@@ -380,13 +384,15 @@ struct NumiClothHipSolver {
         k_yarn_solve<<<1, kYarnThreads, lds, stream>>>(a);
     }
 
-    // The fruit-pair colours and the present fruit/yarn solve that follows
-    // them, in one workgroup launch when the scene has both.
-    void launchFruitPairsAndYarn() {
-        if (Y != 0 && !pairColours.empty()) {
-            launchYarn(0, true);
+    // The grips (withGrips), the fruit-pair colours and the present
+    // fruit/yarn solve that follows them, in one workgroup launch when the
+    // scene has fruit/yarn records.
+    void launchFruitPairsAndYarn(bool withGrips = false) {
+        if (Y != 0) {
+            launchYarn(0, !pairColours.empty(), withGrips);
             return;
         }
+        if (withGrips) launchGrips();
         launchFruitPairs();
         launchYarn(0);
     }
@@ -423,16 +429,21 @@ struct NumiClothHipSolver {
         collideNow(fruitGround);
     }
 
-    void collideNow(bool fruitGround = false) {
+    // selfBufferFirst >= 0: apply (and clear) that self-contact buffer's
+    // corrections in the same launch, before the pass.
+    void collideNow(bool fruitGround = false, int selfBufferFirst = -1) {
         const bool withFruits = fruitGround && collideTakesFruitGround();
         if (!groundOrColliders()) {
+            if (selfBufferFirst >= 0) applySelf(static_cast<uint32_t>(selfBufferFirst), posCur);
             if (withFruits) launchFruitGround();
             return;
         }
         const uint32_t n = extension.staticSphereCount;
+        FixedAccumulator* acc = selfBufferFirst >= 0 ? accumulator[selfBufferFirst].ptr : nullptr;
+        uint32_t* mask = selfBufferFirst >= 0 ? touched[selfBufferFirst].ptr : nullptr;
         k_collide<<<blocksFor(P), kBlock, 2 * n * sizeof(float4), stream>>>(
             cfg(), ok(), posCur, P, spheres.ptr, n, sphereWindowMin, failure.ptr, withFruits ? fruits.ptr : nullptr,
-            withFruits ? F : 0u);
+            withFruits ? F : 0u, acc, mask);
     }
 
     // A ground/sphere pass that follows self-contact can run at the start of
@@ -601,9 +612,12 @@ struct NumiClothHipSolver {
             pendingCollide = colliding;
             return;
         }
-        applySelf(b, posCur);
-        if (colliding) collideNow(fruitGround);
-        else if (fruitGround && collideTakesFruitGround()) launchFruitGround();
+        if (colliding) {
+            collideNow(fruitGround, static_cast<int>(b));  // the corrections first, in the same launch
+        } else {
+            applySelf(b, posCur);
+            if (fruitGround && collideTakesFruitGround()) launchFruitGround();
+        }
     }
 
     void applySelf(uint32_t b, float4* target) {
@@ -852,8 +866,7 @@ struct NumiClothHipSolver {
                 launchKnots();
                 if (it % 2 == 0) launchBends();
             }
-            launchGrips();
-            launchFruitPairsAndYarn();
+            launchFruitPairsAndYarn(true);
             // The fruit ground pass joins the ground/sphere launch where one
             // follows here, or runs on its own.
             bool fruitGroundDone = false;
