@@ -536,6 +536,7 @@ struct PairArgs {
     const uint32_t* pairTotal;
     uint32_t capacity;
     uint2* pairs;
+    uint4* pairKnots;  // the candidates' four endpoint knots (s's two, then t's), for the solves
     NumiClothBagGPUSelfImpulse* impulses;  // per-candidate impulse aggregates, cleared here
     uint32_t* failure;
     // Run walks (W > 0): the candidates the count pass found, appended as
@@ -563,6 +564,8 @@ constexpr uint32_t kPairThreads = 256;  // workgroup size of the candidate count
 __device__ __forceinline__ void emitCandidate(const PairArgs& a, uint32_t& out, uint32_t p, uint32_t q) {
     if (out < a.capacity) {
         a.pairs[out] = make_uint2(p, q);
+        const uint2 ep = a.ij[p], eq = a.ij[q];
+        a.pairKnots[out] = make_uint4(ep.x, ep.y, eq.x, eq.y);
         NumiClothBagGPUSelfImpulse empty{};
         empty.identity = mr_uint4{out, p, q, 0u};
         a.impulses[out] = empty;
@@ -923,9 +926,11 @@ __device__ __forceinline__ void addFixed(FixedAccumulator* acc, uint32_t* touche
 // solve_self_contact (reference src/metal/ClothBag.metal:2125-2448) for every
 // candidate against one snapshot; mode 0 present overlap, mode 1 swept
 // conservative advancement (80 steps, 0.9 safety, 1e-10 minimum advance).
+// pairKnots: each candidate's endpoint knots, written with the candidate (one
+// dependent load per candidate instead of the pair, then its segments).
 __global__ void k_self_solve(const NumiClothBagGPUConfig* __restrict__ cfg, const uint32_t* __restrict__ ok,
-                             uint32_t mode, const SelfGrid* __restrict__ grid, const uint2* __restrict__ pairs,
-                             const uint2* __restrict__ ij, const float4* __restrict__ pos,
+                             uint32_t mode, const SelfGrid* __restrict__ grid, const uint4* __restrict__ pairKnots,
+                             const float4* __restrict__ pos,
                              const float4* __restrict__ prev, FixedAccumulator* __restrict__ acc,
                              uint32_t* __restrict__ touched, NumiClothBagGPUSelfImpulse* __restrict__ impulses,
                              NumiClothBagGPUSelfStatus* __restrict__ status) {
@@ -941,9 +946,9 @@ __global__ void k_self_solve(const NumiClothBagGPUConfig* __restrict__ cfg, cons
         bool accepted = false;
         float correction = 0.0f;
         if (k < count) {
-            const uint2 pr = pairs[k];
-            const uint2 ea = ij[pr.x];
-            const uint2 eb = ij[pr.y];
+            const uint4 kk = pairKnots[k];
+            const uint2 ea = make_uint2(kk.x, kk.y);
+            const uint2 eb = make_uint2(kk.z, kk.w);
             const float4 A0 = pos[ea.x], A1 = pos[ea.y], B0 = pos[eb.x], B1 = pos[eb.y];
             // The start positions: the swept mode reads them always, the
             // present mode only for a contact normal of coincident points.

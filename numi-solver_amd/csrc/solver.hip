@@ -175,6 +175,7 @@ struct NumiClothHipSolver {
     DeviceArray<uint32_t> segmentCounts, entryCounts, scanSums, totals;
     DeviceArray<uint32_t> keysA, valsA, keysB, valsB, histogram;
     DeviceArray<uint2> pairs;
+    DeviceArray<uint4> pairKnots;  // per candidate: the endpoint knots of both segments
     DeviceArray<uint4> foundPairs;  // run walks: (entry, rank, s, t) of every candidate, in any order
     // Two fixed-point accumulators and touched masks (one bit per knot with a
     // pending correction), used by alternate self-contact solves: a tile
@@ -561,6 +562,7 @@ struct NumiClothHipSolver {
         pa.pairTotal = totals.ptr + 1;
         pa.capacity = pairCapacity;
         pa.pairs = pairs.ptr;
+        pa.pairKnots = pairKnots.ptr;
         pa.impulses = selfImpulses.ptr;
         pa.failure = failure.ptr;
         pa.found = foundPairs.ptr;
@@ -602,7 +604,7 @@ struct NumiClothHipSolver {
         // endpoints, positions) once, and a call with no candidate still
         // costs little more than its launch.
         const uint32_t blocks = std::min<uint32_t>(1024, std::max(1u, (pairCapacity + kBlock - 1) / kBlock));
-        k_self_solve<<<blocks, kBlock, 0, stream>>>(cfg(), ok(), mode, grid.ptr, pairs.ptr, dIJ.ptr, posCur,
+        k_self_solve<<<blocks, kBlock, 0, stream>>>(cfg(), ok(), mode, grid.ptr, pairKnots.ptr, posCur,
                                                      prev.ptr, accumulator[b].ptr, touched[b].ptr, selfImpulses.ptr,
                                                      selfStatus.ptr);
         selfBuffer ^= 1u;
@@ -1490,6 +1492,7 @@ void setupSelfContact(const NumiClothHipSceneDesc& scene, NumiClothHipSolver& s)
     s.valsB.allocate(s.entryCapacity);
     s.histogram.allocate(histLength);
     s.pairs.allocate(s.pairCapacity);
+    s.pairKnots.allocate(s.pairCapacity);
     if (s.localWidth != 0) s.foundPairs.allocate(s.pairCapacity);
     s.selfImpulses.allocate(s.pairCapacity);  // written by the emit pass before any use
     for (int b = 0; b < 2; ++b) {
