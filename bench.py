@@ -324,7 +324,7 @@ def diagnostics_of(solver, state) -> dict:
             "self_candidates_last_substep": d.selfCandidateCount, "self_candidate_overflow": bool(d.selfCandidateOverflow),
             "self_entries_last_substep": d.selfEntryCount, "self_grid_cells": list(d.selfGridCells),
             "self_contacts_last_step": {"present": int(self_status[0]), "swept": int(self_status[1])},
-            "reconciliation_tile_rounds_since_create": {"skipped": int(d.tileRoundsSkipped),
+            "repeated_round_tiles_since_create": {"skipped": int(d.tileRoundsSkipped),
                                                         "run": int(d.tileRoundsRun)}}
 
 

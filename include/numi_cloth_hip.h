@@ -192,10 +192,11 @@ typedef struct NumiClothHipDiagnostics {
     /* Certificate passes the last step() call ran (0 without
      * extension.certificatePasses). */
     uint32_t certificatePasses;
-    /* Tile strain programs of the reconciliation rounds since create: tiles
-     * that returned at once because no tile owning one of their knots
-     * changed them in the previous round (the round would write the same
-     * bytes), and tiles that ran. */
+    /* Tile strain programs of the reconciliation rounds (and of the
+     * certificate rounds of scenes without fruits or self-contact) since
+     * create: tiles that returned at once because no tile owning one of
+     * their knots changed them in the previous round (the round would
+     * write the same bytes), and tiles that ran. */
     uint64_t tileRoundsSkipped;
     uint64_t tileRoundsRun;
 } NumiClothHipDiagnostics;

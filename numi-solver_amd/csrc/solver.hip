@@ -133,7 +133,7 @@ struct NumiClothHipSolver {
     DeviceArray<double> times;      // per substep: time at the substep end
     DeviceArray<SubstepSlot> slot;
     DeviceArray<uint32_t> failure;
-    DeviceArray<uint32_t> roundFlags;  // per-tile change flags of the reconciliation rounds' strain programs
+    DeviceArray<uint32_t> roundFlags;  // per-tile change flags of the reconciliation and certificate rounds' strain programs
     DeviceArray<unsigned long long> roundStats;  // reconciliation tile-rounds since create: skipped, run
     DeviceArray<NumiClothBagGPUSelfStatus> selfStatus;
     DeviceArray<NumiClothBagGPUFrictionStatus> frictionStatus;
@@ -766,16 +766,17 @@ struct NumiClothHipSolver {
                            (!tileStrain.valid || tileStrain.windows.count != 0);
     }
 
-    // Strain sweeps followed by `collides` ground/sphere passes. round >= 0:
-    // the reconciliation round of that index, whose strain program records
-    // which tiles changed their knots and, when it repeats the previous
-    // round's program, skips the tiles whose loaded knots that round left
-    // unchanged.
-    void launchStrainAndCollide(uint32_t sweeps, uint32_t collides, int round = -1) {
+    // Strain sweeps followed by `collides` ground/sphere passes. flags >= 0:
+    // the round's slot of per-tile change flags, which its strain program
+    // records; repeat: the program repeats the previous slot's round with
+    // nothing in between, so it skips the tiles whose loaded knots that round
+    // left unchanged.
+    void launchStrainAndCollide(uint32_t sweeps, uint32_t collides, int flags = -1, bool repeat = false) {
         if (useTiles && sweeps != 0 && tileStrain.valid) {  // consumes a deferred collide
             const size_t tiles = tileStrain.tiles;
-            uint32_t* changed = round >= 0 ? roundFlags.ptr + static_cast<size_t>(round) * tiles : nullptr;
-            const uint32_t* prev = round > 0 ? roundFlags.ptr + static_cast<size_t>(round - 1) * tiles : nullptr;
+            uint32_t* changed = flags >= 0 ? roundFlags.ptr + static_cast<size_t>(flags) * tiles : nullptr;
+            const uint32_t* prev =
+                flags > 0 && repeat ? roundFlags.ptr + static_cast<size_t>(flags - 1) * tiles : nullptr;
             launchTileProgram(tileStrain, collides, cfg(), ok(), false, changed, prev);
             return;
         }
@@ -899,10 +900,16 @@ struct NumiClothHipSolver {
             const bool repeating = F == 0 && FP == 0 && Y == 0;
             for (uint32_t pass = 0; pass < kReconciliationPasses; ++pass) {
                 launchFruitPairsAndYarn();
-                launchStrainAndCollide(sweeps, 1, repeating ? static_cast<int>(pass) : -1);
+                launchStrainAndCollide(sweeps, 1, repeating ? static_cast<int>(pass) : -1, pass > 0);
                 launchFruitGround();
             }
-            for (uint32_t round = 0; round <= kCertificatePasses; ++round) certificateRound(sweeps, defer);
+            // Without self-contact the certificate rounds are the same strain
+            // program again and again (their self-contact solve is none).
+            const bool repeatingCertificates = repeating && !self && !pointSelf;
+            for (uint32_t round = 0; round <= kCertificatePasses; ++round)
+                certificateRound(sweeps, defer,
+                                 repeatingCertificates ? static_cast<int>(kReconciliationPasses + round) : -1,
+                                 round > 0);
             if (certify) runCertificates(sweeps, defer);
         }
         restorePrimaryBuffers();
@@ -933,12 +940,13 @@ struct NumiClothHipSolver {
 
     // One certificate round of the schedule: segment self-contact, the
     // strain sweeps, the final contact passes.
-    void certificateRound(uint32_t sweeps, bool defer) {
+    // flags / repeat: as launchStrainAndCollide (scenes without fruits).
+    void certificateRound(uint32_t sweeps, bool defer, int flags = -1, bool repeat = false) {
         launchSelf(0, false, defer);
         if (F == 0) {
             // Without fruits the final contact passes are ground passes only
             // and fuse into the strain program.
-            launchStrainAndCollide(sweeps, kFinalContactPasses);
+            launchStrainAndCollide(sweeps, kFinalContactPasses, flags, repeat);
         } else {
             launchStrainAndCollide(sweeps, 0);
             for (uint32_t pass = 0; pass < kFinalContactPasses; ++pass) launchFinalContacts();
@@ -2185,7 +2193,7 @@ void NumiClothHipSolver::ensureStrainProgram(uint32_t sweeps) {
         tileStrain.valid = false;
     } else {
         // Every tile of every round writes its flag, so they need no clearing.
-        roundFlags.allocate(static_cast<size_t>(kReconciliationPasses) * tileStrain.tiles);
+        roundFlags.allocate(static_cast<size_t>(kReconciliationPasses + kCertificatePasses + 1) * tileStrain.tiles);
     }
     if (graph != nullptr) {
         (void)hipGraphExecDestroy(graph);

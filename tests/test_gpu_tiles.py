@@ -79,7 +79,8 @@ def test_large_sheet_uses_several_tiles_with_halo(cloth):
 
 def test_reconciliation_rounds_skip_unchanged_tiles(cloth):
     """A reconciliation round's strain program skips the tiles whose loaded
-    knots the previous round left unchanged; the state equals the
+    knots the previous round left unchanged, and so does a certificate
+    round's on a scene without self-contact; the state equals the
     per-colour schedule's, which runs every round in full."""
     scene = SCENES["large"](cloth)
     saved = os.environ.get("NUMI_CLOTH_HIP_TILES")
@@ -96,10 +97,11 @@ def test_reconciliation_rounds_skip_unchanged_tiles(cloth):
         else:
             os.environ["NUMI_CLOTH_HIP_TILES"] = saved
     plain, _ = _run(cloth, scene, 24, 9, 3, False)
-    print(f"tiles={info.tiles}: reconciliation tile-rounds skipped {d.tileRoundsSkipped}, run {d.tileRoundsRun}")
-    assert info.tiles > 1
-    # 24 substeps x 8 rounds, every tile counted once per round.
-    assert d.tileRoundsSkipped + d.tileRoundsRun == 24 * 8 * info.tiles
+    print(f"tiles={info.tiles}: repeated-round tiles skipped {d.tileRoundsSkipped}, run {d.tileRoundsRun}")
+    assert info.tiles > 1 and not scene.selfContact
+    # 24 substeps x (8 reconciliation + 9 certificate rounds), every tile
+    # counted once per round.
+    assert d.tileRoundsSkipped + d.tileRoundsRun == 24 * (8 + 9) * info.tiles
     assert d.tileRoundsSkipped > 0
     for key in ["particles", "distances", "bends", "selfStatus", "frictionStatus"]:
         assert tiled[key].tobytes() == plain[key].tobytes(), key
