@@ -76,14 +76,14 @@ struct SortedEntry {
 
 // Per-knot sums of one Jacobi pass (self-contact corrections, point
 // self-collision corrections or self-friction velocity changes), all order
-// independent: the corrections and their lengths in 2^-32 fixed point, the
-// longest length as float bits (integer max orders non-negative floats), and
-// the number of contributions.
+// independent: the corrections and their lengths in 2^-32 fixed point and
+// the longest length as float bits (integer max orders non-negative floats).
+// A knot's bit in the pass's touched mask says it has contributions.
 struct FixedAccumulator {
     long long x, y, z;
     long long lengthSum;
     uint32_t maxLength;
-    uint32_t count;
+    uint32_t pad;
 };
 
 // ---------------------------------------------------------- broad phase --
@@ -905,9 +905,8 @@ __device__ __forceinline__ V3 fixedMean(const FixedAccumulator& a) {
               static_cast<float>(static_cast<double>(a.z) * scale));
 }
 
-// Adds d to knot i's fixed-point sum; the first contribution of a call also
-// sets the knot's bit in the touched mask, so the apply pass visits only the
-// knots that have corrections.
+// Adds d to knot i's fixed-point sum and sets the knot's bit in the touched
+// mask, so the apply pass visits only the knots that have corrections.
 __device__ __forceinline__ void addFixed(FixedAccumulator* acc, uint32_t* touched, uint32_t i, V3 d) {
     FixedAccumulator* a = acc + i;
     atomicAdd(reinterpret_cast<unsigned long long*>(&a->x),
@@ -920,7 +919,9 @@ __device__ __forceinline__ void addFixed(FixedAccumulator* acc, uint32_t* touche
     atomicAdd(reinterpret_cast<unsigned long long*>(&a->lengthSum),
               static_cast<unsigned long long>(__double2ll_rn(static_cast<double>(len) * kFixedScale)));
     atomicMax(&a->maxLength, __float_as_uint(len));
-    if (atomicAdd(&a->count, 1u) == 0u) atomicOr(touched + (i >> 5), 1u << (i & 31u));
+    // Every contribution sets the bit (idempotent; no atomic waits for a
+    // returned value).
+    atomicOr(touched + (i >> 5), 1u << (i & 31u));
 }
 
 // solve_self_contact (reference src/metal/ClothBag.metal:2125-2448) for every
