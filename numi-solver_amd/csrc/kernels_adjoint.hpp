@@ -70,7 +70,7 @@ struct SpecimenArgs {
     uint32_t trackedTotal;
     uint32_t linear;               // 1: loss = sum w . x, 0: sum |x - target|^2
     double* loss;                  // per specimen
-    uint32_t record;
+    uint32_t record;               // 0: loss only, 1: tape only, 2: tape and loss
     uint32_t passes;               // passes per substep (tape stride)
     uint32_t particles, distances, bends;
     float4* tapeX;                 // [substep in launch][pass][particles]
@@ -319,12 +319,12 @@ __global__ void __launch_bounds__(kSpecimenThreads) k_specimen_forward(SpecimenA
                 out[2] = gx2;
             }
         }
-        if (!a.record) {
+        if (a.record != 1u) {
             for (int offset = 32; offset > 0; offset >>= 1) part += __shfl_xor(part, offset);
             if ((tid & 63u) == 0u) lossParts[tid / 64] = part;
         }
         __syncthreads();
-        if (!a.record && tid == 0) {
+        if (a.record != 1u && tid == 0) {
             double sum = 0.0;
             for (uint32_t w = 0; w < threads / 64; ++w) sum += lossParts[w];
             a.loss[spec] += sum;

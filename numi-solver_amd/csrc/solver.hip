@@ -2653,8 +2653,23 @@ void runAdjoint(NumiClothHipBatch& b, const NumiClothHipAdjointProblem& pr) {
     a.gradBend = A.gradBend.ptr;
     a.strainTaped = A.strainTaped.ptr;
 
+    // The tape arguments of set `which` for a recording forward.
+    auto taped = [&](SpecimenArgs f, uint32_t which) {
+        AdjointBuffers::TapeSet& set = A.sets[which];
+        f.record = 1u;
+        f.tapeX = set.tapeX.ptr;
+        f.tapeLambdaD = set.tapeLambdaD.ptr;
+        f.tapeLambdaB = set.tapeLambdaB.ptr;
+        f.tapeV0 = set.tapeV0.ptr;
+        f.tapeActive = set.tapeActive.ptr;
+        f.tapeStrainActs = set.tapeStrainActs.ptr;
+        f.lossGrad = set.lossGrad.ptr;
+        return f;
+    };
     // Forward over the trajectory on the solver's own state, a checkpoint of
-    // (x, v) at the start of every segment.
+    // (x, v) at the start of every segment. The last segment records its tape
+    // as it goes (into the set its reverse reads first), so the reverse
+    // starts without recomputing it.
     for (uint32_t k = 0; k < segments; ++k) {
         HIP_OK(hipMemcpyAsync(A.checkpointPos.ptr + static_cast<size_t>(k) * s.P, s.pos.ptr, s.P * sizeof(float4),
                               hipMemcpyDeviceToDevice, st));
@@ -2669,6 +2684,10 @@ void runAdjoint(NumiClothHipBatch& b, const NumiClothHipAdjointProblem& pr) {
         f.vel = s.vel.ptr;
         f.dLambda = s.dLambda.ptr;
         f.bLambda = s.bLambda.ptr;
+        if (k + 1 == segments) {
+            f = taped(f, 0u);
+            f.record = 2u;
+        }
         k_specimen_forward<<<S, kSpecimenThreads, A.ldsBytes, st>>>(f);
         HIP_OK(hipGetLastError());
     }
@@ -2683,6 +2702,21 @@ void runAdjoint(NumiClothHipBatch& b, const NumiClothHipAdjointProblem& pr) {
     for (uint32_t k = segments; k-- > 0;) {
         const uint32_t which = (segments - 1 - k) % setsUsed;
         AdjointBuffers::TapeSet& set = A.sets[which];
+        if (k + 1 == segments) {
+            // Taped by the forward itself (set 0, on the solver stream).
+            SpecimenArgs f = taped(a, which);
+            f.firstSubstep = k * L;
+            f.substeps = std::min(L, T - k * L);
+            f.dLambda = s.dLambda.ptr;
+            f.bLambda = s.bLambda.ptr;
+            f.pos = s.pos.ptr;  // inverse masses for the reverse
+            f.prev = s.prev.ptr;
+            f.vel = s.vel.ptr;
+            k_specimen_reverse<<<S, kSpecimenThreads, A.ldsBytes, st>>>(f);
+            HIP_OK(hipGetLastError());
+            HIP_OK(hipEventRecord(A.released[which], st));
+            continue;
+        }
         hipStream_t rs = setsUsed > 1 ? A.side : st;
         HIP_OK(hipStreamWaitEvent(rs, A.released[which], 0));
         HIP_OK(hipMemcpyAsync(set.scratchPos.ptr, A.checkpointPos.ptr + static_cast<size_t>(k) * s.P,
