@@ -325,7 +325,10 @@ def diagnostics_of(solver, state) -> dict:
             "self_entries_last_substep": d.selfEntryCount, "self_grid_cells": list(d.selfGridCells),
             "self_contacts_last_step": {"present": int(self_status[0]), "swept": int(self_status[1])},
             "repeated_round_tiles_since_create": {"skipped": int(d.tileRoundsSkipped),
-                                                        "run": int(d.tileRoundsRun)}}
+                                                        "run": int(d.tileRoundsRun)},
+            "fruit_yarn_solves_since_create": {"reruns": int(d.yarnReruns), "most_candidates": int(d.yarnCandidatesMax),
+                                               "list_builds": int(d.yarnListBuilds),
+                                               "from_kept_list": int(d.yarnListSolves)}}
 
 
 # ------------------------------------------------------------- workloads --

@@ -199,6 +199,13 @@ typedef struct NumiClothHipDiagnostics {
      * write the same bytes), and tiles that ran. */
     uint64_t tileRoundsSkipped;
     uint64_t tileRoundsRun;
+    /* Present-mode fruit/yarn solves since create that tested every
+     * (fruit, segment) visit and rebuilt the kept list of visits near
+     * contact, and those that took their candidates from the kept list
+     * (its knots and fruits had drifted too little for a visit off it to
+     * reach contact). */
+    uint32_t yarnListBuilds;
+    uint32_t yarnListSolves;
 } NumiClothHipDiagnostics;
 
 typedef struct NumiClothHipSolver NumiClothHipSolver;

@@ -115,7 +115,8 @@ class Diagnostics(C.Structure):
                 ("yarnReruns", C.c_uint32), ("yarnCandidatesMax", C.c_uint32),
                 ("selfEntryCount", C.c_uint32), ("selfGridCells", C.c_uint32 * 3),
                 ("pointSelfContacts", C.c_uint32), ("certificatePasses", C.c_uint32),
-                ("tileRoundsSkipped", C.c_uint64), ("tileRoundsRun", C.c_uint64)]
+                ("tileRoundsSkipped", C.c_uint64), ("tileRoundsRun", C.c_uint64),
+                ("yarnListBuilds", C.c_uint32), ("yarnListSolves", C.c_uint32)]
 
 
 MATERIAL_FIELDS = ["ordinaryNodeMassKg", "hemNodeMassKg", "yarnRadiusM", "axialBodyComplianceMPerN",

@@ -277,47 +277,95 @@ __device__ __forceinline__ void projectFruitPair(NumiClothBagGPUFruit& a, NumiCl
     wrote = true;
 }
 
-// All fruit-pair colours of one solve pass in one workgroup: fruits staged
-// in LDS, a barrier between colours (the reference launches one dispatch per
-// colour, :4519-4528).
-// All fruit-pair colours in one workgroup (every thread calls it): the
-// fruits in LDS (`local`, at least fruitCount records), one pair per thread
-// within a colour (a colour's pairs share no fruit), a barrier between
-// colours; the fruits are written back and the workgroup synchronised.
+// All fruit-pair colours of one solve pass in one workgroup (the reference
+// launches one dispatch per colour, :4519-4528); every thread calls it. The
+// fruits are staged in LDS (`local`, at least fruitCount records), a
+// colour's pairs share no fruit, and a barrier separates colours; the
+// fruits are written back and the workgroup synchronised.
+// A pair acts only when its fruits overlap, and until one acts the fruits
+// stay as they start: the colours before the first colour holding a pair
+// that overlaps at the start act on nothing and are skipped (the non-finite
+// checks they would make see the start state and are made up front). With
+// at most one pair per thread, thread g keeps pair g (colourOf[g]: its
+// colour) in registers for the whole pass.
 __device__ void fruitPairColours(const NumiClothBagGPUConfig* __restrict__ cfg, const uint2* __restrict__ colours,
-                                 uint32_t colourCount, NumiClothBagGPUFruit* __restrict__ fruits, uint32_t fruitCount,
-                                 NumiClothBagGPUFruitPair* __restrict__ pairs, uint32_t* __restrict__ failure,
-                                 NumiClothBagGPUFruit* local) {
-    for (uint32_t i = threadIdx.x; i < fruitCount; i += blockDim.x) local[i] = fruits[i];
+                                 const uint32_t* __restrict__ colourOf, uint32_t colourCount,
+                                 NumiClothBagGPUFruit* __restrict__ fruits, uint32_t fruitCount,
+                                 NumiClothBagGPUFruitPair* __restrict__ pairs, uint32_t pairCount,
+                                 uint32_t* __restrict__ failure, NumiClothBagGPUFruit* local) {
+    __shared__ uint32_t firstColour;
+    const uint32_t tid = threadIdx.x, threads = blockDim.x;
+    for (uint32_t i = tid; i < fruitCount; i += threads) local[i] = fruits[i];
+    if (tid == 0) firstColour = colourCount;
+    const bool held = pairCount <= threads;
+    NumiClothBagGPUFruitPair mine{};
+    uint32_t myColour = ~0u;
+    if (held && tid < pairCount) {
+        mine = pairs[tid];
+        myColour = colourOf[tid];
+    }
+    __syncthreads();
+    for (uint32_t g = tid; g < pairCount; g += threads) {
+        const uint32_t c = held ? myColour : colourOf[g];
+        if (c >= colourCount) continue;
+        const NumiClothBagGPUFruitPair pair = held ? mine : pairs[g];
+        const NumiClothBagGPUFruit& a = local[pair.fruitsAndColor.x];
+        const NumiClothBagGPUFruit& b = local[pair.fruitsAndColor.y];
+        const float len = length(xyz(b.positionAndInverseMass) - xyz(a.positionAndInverseMass));
+        const float target = a.previousAndRadius.w + b.previousAndRadius.w;
+        if (!isfinite(len) || !isfinite(target) || !finite4(pair.contact)) {
+            recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
+        } else if (len < target && len > 1.0e-12f) {
+            atomicMin(&firstColour, c);
+        }
+    }
     __syncthreads();
     const float dt = cfg->gravityAndTimestep.w;
-    for (uint32_t c = 0; c < colourCount; ++c) {
-        const uint2 range = colours[c];
-        for (uint32_t t = threadIdx.x; t < range.y; t += blockDim.x) {
-            NumiClothBagGPUFruitPair pair = pairs[range.x + t];
-            NumiClothBagGPUFruit a = local[pair.fruitsAndColor.x];
-            NumiClothBagGPUFruit b = local[pair.fruitsAndColor.y];
-            bool wrote = false;
-            projectFruitPair(a, b, pair, dt, failure, wrote);
-            if (wrote) {
-                local[pair.fruitsAndColor.x] = a;
-                local[pair.fruitsAndColor.y] = b;
-                pairs[range.x + t] = pair;
+    for (uint32_t c = firstColour; c < colourCount; ++c) {
+        if (held) {
+            if (myColour == c) {
+                NumiClothBagGPUFruit a = local[mine.fruitsAndColor.x];
+                NumiClothBagGPUFruit b = local[mine.fruitsAndColor.y];
+                bool wrote = false;
+                projectFruitPair(a, b, mine, dt, failure, wrote);
+                if (wrote) {
+                    local[mine.fruitsAndColor.x] = a;
+                    local[mine.fruitsAndColor.y] = b;
+                    pairs[tid] = mine;
+                }
+            }
+        } else {
+            const uint2 range = colours[c];
+            for (uint32_t t = tid; t < range.y; t += threads) {
+                NumiClothBagGPUFruitPair pair = pairs[range.x + t];
+                NumiClothBagGPUFruit a = local[pair.fruitsAndColor.x];
+                NumiClothBagGPUFruit b = local[pair.fruitsAndColor.y];
+                bool wrote = false;
+                projectFruitPair(a, b, pair, dt, failure, wrote);
+                if (wrote) {
+                    local[pair.fruitsAndColor.x] = a;
+                    local[pair.fruitsAndColor.y] = b;
+                    pairs[range.x + t] = pair;
+                }
             }
         }
         __syncthreads();
     }
-    for (uint32_t i = threadIdx.x; i < fruitCount; i += blockDim.x) fruits[i] = local[i];
+    for (uint32_t i = tid; i < fruitCount; i += threads) fruits[i] = local[i];
     __syncthreads();
 }
 
-__global__ void k_fruit_pairs(const NumiClothBagGPUConfig* __restrict__ cfg, const uint32_t* __restrict__ ok,
-                              const uint2* __restrict__ colours, uint32_t colourCount,
-                              NumiClothBagGPUFruit* __restrict__ fruits, uint32_t fruitCount,
-                              NumiClothBagGPUFruitPair* __restrict__ pairs, uint32_t* __restrict__ failure) {
+constexpr uint32_t kFruitPairThreads = 256;  // k_fruit_pairs workgroup: up to this many pairs held in registers
+
+__global__ void __launch_bounds__(kFruitPairThreads)
+k_fruit_pairs(const NumiClothBagGPUConfig* __restrict__ cfg, const uint32_t* __restrict__ ok,
+                              const uint2* __restrict__ colours, const uint32_t* __restrict__ colourOf,
+                              uint32_t colourCount, NumiClothBagGPUFruit* __restrict__ fruits, uint32_t fruitCount,
+                              NumiClothBagGPUFruitPair* __restrict__ pairs, uint32_t pairCount,
+                              uint32_t* __restrict__ failure) {
     __shared__ NumiClothBagGPUFruit local[32];
     if (*ok == 0u) return;
-    fruitPairColours(cfg, colours, colourCount, fruits, fruitCount, pairs, failure, local);
+    fruitPairColours(cfg, colours, colourOf, colourCount, fruits, fruitCount, pairs, pairCount, failure, local);
 }
 
 // Fruit part of solve_ground: z >= radius with the impulse that removes the

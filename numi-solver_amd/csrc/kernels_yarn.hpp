@@ -35,6 +35,10 @@ namespace numi_dev {
 
 constexpr uint32_t kYarnSlots = 1024;   // staged candidate visits per call
 constexpr uint32_t kYarnThreads = 1024;
+// Present-mode kept list: pairs within kYarnListMargins candidate margins of
+// contact at the build, at most kYarnListCapacity of them (more: no list).
+constexpr float kYarnListMargins = 6.0f;
+constexpr uint32_t kYarnListCapacity = 8192;
 
 // Orders LDS accesses of one wavefront (lanes of wave 0 exchange knots
 // through LDS between phases).
@@ -69,15 +73,26 @@ struct YarnArgs {
     NumiClothBagGPUYarnContact* contacts;
     NumiClothBagGPUFrictionStatus* status;  // friction only
     float margin;                        // present mode candidate margin
-    uint32_t* stats;                     // [reruns visit by visit, most candidates], or null
+    uint32_t* stats;                     // [reruns visit by visit, most candidates, list builds, kept-list
+                                         // solves], or null
     uint32_t* failure;
     // The grips and fruit-pair colours the schedule runs just before (0:
     // none): the solve runs them first, in its own workgroup.
     NumiClothBagGPUGrip* grips;
     uint32_t gripCount;
     const uint2* pairColours;
+    const uint32_t* pairColourOf;        // per fruit pair: its colour
     uint32_t pairColourCount;
     NumiClothBagGPUFruitPair* fruitPairs;
+    uint32_t fruitPairCount;
+    // Present-mode kept list (null: none): header (pairs, valid), the listed
+    // pair indices, the knots and fruits (centre, limit) of the build.
+    uint32_t* listHeader;
+    uint4* listPairs;                    // (pair index, segment knot a, knot b, -)
+    float4* listKnots;
+    float4* listFruits;
+    uint32_t listCapacity;
+    float listMargin;                    // listed: within this of contact at the build
 };
 
 __device__ __forceinline__ uint32_t phaseBit(uint32_t f, uint32_t s, uint2 batch) {
@@ -221,29 +236,72 @@ __device__ void yarnDense(const YarnArgs& a, float4* knots, float* moved) {
     }
 }
 
+// Staged candidates up to this count are put in phase order by ranking
+// (each counts the candidates before it); more go through the per-phase
+// counts and a scan over every segment.
+constexpr uint32_t kYarnRankMax = 128;
+
+// Largest value over the workgroup of a per-thread non-negative float (NaN
+// as +inf): wave maxima into perWave[], which the caller reads after its
+// next barrier.
+__device__ __forceinline__ void waveMaxInto(float v, float* perWave) {
+    v = v < INFINITY ? v : INFINITY;
+    for (int offset = 32; offset > 0; offset >>= 1) v = fmaxf(v, __shfl_xor(v, offset));
+    if ((threadIdx.x & 63u) == 0u) perWave[threadIdx.x / 64u] = v;
+}
+
+__device__ __forceinline__ float maxAbs3(V3 v) {
+    const float m = fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fabsf(v.z));
+    return m < INFINITY ? m : INFINITY;
+}
+
 // One solve_yarn_batch call over all distance colours. One workgroup of
 // kYarnThreads; dynamic LDS: knots[P], moved[P], counts[D], visitors[D],
 // visited[D] (per segment: fruits with a candidate visit on it, fruits whose
 // visit has run).
+//
+// Present mode with a kept list (listPairs non-null): the list holds the
+// (fruit, segment) visits that were within `listMargin` of contact when it
+// was built, with the knots and the fruit centres of then and, per fruit,
+// the limit it was listed at. The distance of a point to a segment changes
+// by no more than the point and the segment's endpoints move, so a visit
+// off the list is still beyond contact while its fruit's room (limit minus
+// the current contact distance) exceeds the largest knot drift plus the
+// largest fruit drift since the build, plus a rounding slack. A solve whose
+// start drift leaves half the room lists its candidates from the kept list
+// alone; otherwise it tests every visit, as without a list, and rebuilds the
+// list from its start state. The check after the run covers the visits off
+// the list with the same bound (now including the knots' and fruits' travel
+// in the call), and falls back to checking every visit when it fails.
 __global__ void __launch_bounds__(kYarnThreads) k_yarn_solve(YarnArgs a) {
     extern __shared__ float4 yarnLds[];
     __shared__ uint32_t scanScratch[kYarnThreads];
-    __shared__ uint32_t candidates[kYarnSlots];  // pair indices, unordered
-    __shared__ uint32_t slotPair[kYarnSlots];     // pair indices in phase order
-    __shared__ uint32_t slotPhase[kYarnSlots];    // the phase of each slot
-    __shared__ uint32_t slotColour[kYarnSlots];   // first segment of the slot's colour
-    __shared__ uint32_t slotEarlier[kYarnSlots];  // fruits visiting the slot's segment before it
-    __shared__ uint2 slotEnds[kYarnSlots];        // the slot's segment knots
+    __shared__ uint32_t candidates[kYarnSlots];     // pair indices, unordered
+    __shared__ uint32_t slotPair[kYarnSlots];        // pair indices in phase order
+    __shared__ uint32_t slotFruit[kYarnSlots];       // the fruit of each slot
+    __shared__ uint32_t slotColourEnd[kYarnSlots];   // end of the slots of the slot's colour
+    __shared__ uint32_t slotEarlier[kYarnSlots];     // fruits visiting the slot's segment before it
+    __shared__ uint2 slotEnds[kYarnSlots];           // the slot's segment knots
     __shared__ YarnStage slotStage[kYarnSlots];
     __shared__ NumiClothBagGPUFruit fruitStart[32];
+    __shared__ float4 keptFruit[32];                 // kept list: centre, limit per fruit at the build
     __shared__ float fruitMoved[32];
     __shared__ mr_float4 fruitStaged[32];
+    __shared__ float waveDrift[kYarnThreads / 64];   // kept list: knot drift since the build
+    __shared__ float waveScale[kYarnThreads / 64];   // largest knot coordinate magnitude
+    __shared__ float waveMoved[kYarnThreads / 64];   // largest knot travel in the call
+    __shared__ float fruitBounds[4];  // kept list: fruit drift, kept room, build room, fruit scale
+    __shared__ uint32_t fruitBoundFlags;  // bit 0: build room finite, bit 1: kept room finite
     __shared__ uint32_t candidateCount;
+    __shared__ uint32_t listCount;  // pairs a rebuild listed
+    __shared__ uint32_t keptCount;  // pairs on the kept list, ~0u when there is none
     __shared__ uint32_t verdict;    // 0 sparse result stands, 1 rerun visit by visit
-    __shared__ uint32_t movedBits;  // largest knot travel (float bits, non-negative)
     if (*a.ok == 0u) return;
     const uint32_t tid = threadIdx.x, threads = blockDim.x;
     const uint32_t P = a.particles, D = a.segments, F = a.fruitCount;
+    const bool listed = a.mode == 0u && a.listPairs != nullptr;
+    // The kept list's entry of this thread, read ahead (within capacity).
+    const uint4 prefetched = listed && tid < a.listCapacity ? a.listPairs[tid] : make_uint4(0u, 0u, 0u, 0u);
     if (a.gripCount != 0u) {
         // solve_grip: one grip per thread, each on its own knot.
         for (uint32_t i = tid; i < a.gripCount; i += threads)
@@ -251,92 +309,251 @@ __global__ void __launch_bounds__(kYarnThreads) k_yarn_solve(YarnArgs a) {
         __syncthreads();
     }
     if (a.pairColourCount != 0u)
-        fruitPairColours(a.cfg, a.pairColours, a.pairColourCount, a.fruits, F, a.fruitPairs, a.failure, fruitStart);
+        fruitPairColours(a.cfg, a.pairColours, a.pairColourOf, a.pairColourCount, a.fruits, F, a.fruitPairs,
+                         a.fruitPairCount, a.failure, fruitStart);
     float4* knots = yarnLds;
     float* moved = reinterpret_cast<float*>(knots + P);
     uint32_t* counts = reinterpret_cast<uint32_t*>(moved + P);
     uint32_t* visitors = counts + D;
     uint32_t* visited = visitors + D;
+    float drift = 0.0f, scale = 0.0f;
     for (uint32_t i = tid; i < P; i += threads) {
-        knots[i] = a.pos[i];
+        const float4 x = a.pos[i];
+        knots[i] = x;
         moved[i] = 0.0f;
+        if (listed) {
+            const float d = length(xyz(x) - xyz(a.listKnots[i]));
+            drift = fmaxf(drift, d < INFINITY ? d : INFINITY);
+            scale = fmaxf(scale, maxAbs3(xyz(x)));
+        }
+    }
+    if (listed) {
+        waveMaxInto(drift, waveDrift);
+        waveMaxInto(scale, waveScale);
     }
     for (uint32_t i = tid; i < D; i += threads) {
         counts[i] = 0u;
         visitors[i] = 0u;
         visited[i] = 0u;
     }
-    if (tid < F) {
-        fruitStart[tid] = a.fruits[tid];
-        fruitMoved[tid] = 0.0f;
+    const float yarnRadius = a.cfg->clothMaterial.x;
+    if (tid < 64) {
+        // Wave 0, one lane per fruit (the pair colours left the fruits in
+        // fruitStart); kept list: the fruits' drift since the build, the
+        // room per fruit (its listing limit minus its contact distance) of
+        // the kept list and of a rebuild, their coordinate magnitude.
+        float fruitDrift = 0.0f, keptRoom = INFINITY, buildRoom = INFINITY, fruitScale = 0.0f;
+        bool buildable = true, keepable = true;
+        if (tid < F) {
+            const NumiClothBagGPUFruit fr = a.pairColourCount != 0u ? fruitStart[tid] : a.fruits[tid];
+            fruitStart[tid] = fr;
+            fruitMoved[tid] = 0.0f;
+            if (listed) {
+                const float4 kept = a.listFruits[tid];
+                keptFruit[tid] = kept;
+                const V3 c = xyz(fr.positionAndInverseMass);
+                const float target = fr.previousAndRadius.w + yarnRadius;
+                buildRoom = (target + a.listMargin) - target;
+                keptRoom = kept.w - target;
+                buildable = buildRoom == buildRoom;
+                keepable = keptRoom == keptRoom;
+                fruitScale = maxAbs3(c);
+                const float d = length(c - xyz(kept));
+                fruitDrift = d < INFINITY ? d : INFINITY;
+            }
+        }
+        if (listed) {
+            for (int offset = 32; offset > 0; offset >>= 1) {
+                fruitDrift = fmaxf(fruitDrift, __shfl_xor(fruitDrift, offset));
+                fruitScale = fmaxf(fruitScale, __shfl_xor(fruitScale, offset));
+                keptRoom = fminf(keptRoom, __shfl_xor(keptRoom, offset));
+                buildRoom = fminf(buildRoom, __shfl_xor(buildRoom, offset));
+            }
+            const bool allBuildable = __all(buildable), allKeepable = __all(keepable);
+            if (tid == 0) {
+                fruitBounds[0] = fruitDrift;
+                fruitBounds[1] = keptRoom;
+                fruitBounds[2] = buildRoom;
+                fruitBounds[3] = fruitScale;
+                fruitBoundFlags = (allBuildable ? 1u : 0u) | (allKeepable ? 2u : 0u);
+            }
+        }
     }
     if (tid == 0) {
         candidateCount = 0u;
+        listCount = 0u;
+        keptCount = listed && a.listHeader[1] != 0u ? a.listHeader[0] : ~0u;
         verdict = 0u;
-        movedBits = 0u;
     }
     __syncthreads();
+    // Kept list: 0 none, 1 candidates from the kept list, 2 rebuild. Every
+    // thread reaches the same decision from the same LDS values.
+    uint32_t listState = 0u;
+    float keptDrift = 0.0f, room = INFINITY, slack = 0.0f;
+    if (listed) {
+        float knotDrift = 0.0f;
+        for (uint32_t w = 0; w < threads / 64u; ++w) {
+            knotDrift = fmaxf(knotDrift, waveDrift[w]);
+            scale = fmaxf(scale, waveScale[w]);
+        }
+        const float fruitDrift = fruitBounds[0], keptRoom = fruitBounds[1];
+        room = fruitBounds[2];
+        scale = fmaxf(scale, fruitBounds[3]);
+        const bool buildable = (fruitBoundFlags & 1u) != 0u;
+        const bool keepable = (fruitBoundFlags & 2u) != 0u && keptCount != ~0u;
+        slack = 1.0e-4f + 1.0e-5f * scale;
+        listState = buildable ? 2u : 0u;
+        if (buildable && keepable) {
+            keptDrift = knotDrift + fruitDrift;
+            // Off-list visits start beyond the candidate margin, and half
+            // the room is left for the travel of this call.
+            if (keptDrift + 2.0f * slack + a.margin <= 0.5f * keptRoom) {
+                listState = 1u;
+                room = keptRoom;
+            }
+        }
+    }
     // 1. candidate visits: counted per phase and listed (start state in LDS).
     // A visit can act in the swept mode when its record's impact flag is
     // set, in the present mode when the fruit is within margin of contact
-    // at the start of the call. A thread takes segments and tests every
-    // fruit against each, so a segment's colour and endpoints are read once
-    // (nothing depends on the order of the listing).
-    const float yarnRadius = a.cfg->clothMaterial.x;
-    for (uint32_t s = tid; s < D; s += threads) {
-        const uint2 batch = a.segmentBatch[s];
-        if (batch.y == 0u) continue;
-        const uint2 e = a.ij[s];
-        const V3 k0 = xyz(knots[e.x]), k1 = xyz(knots[e.y]);
-        const V3 mid = (k0 + k1) * 0.5f;
-        const float half = 0.5f * length(k1 - k0);
-        for (uint32_t f = 0; f < F; ++f) {
-            const uint32_t idx = f * D + s;
-            bool candidate;
-            if (a.mode == 1u) {
-                candidate = a.contacts[idx].control.y != 0u;
-            } else {
-                const NumiClothBagGPUFruit& fr = fruitStart[f];
-                const V3 c = xyz(fr.positionAndInverseMass);
-                const float limit = fr.previousAndRadius.w + yarnRadius + a.margin;
-                candidate = !beyondSegment(c, mid, half, limit) && samplePointSegment(c, k0, k1).distance < limit;
+    // at the start of the call. Nothing depends on the order of the listing.
+    auto stageCandidate = [&](uint32_t f, uint32_t s, uint2 batch, uint32_t idx) {
+        atomicAdd(&counts[phaseBit(f, s, batch)], 1u);
+        atomicOr(&visitors[s], 1u << f);
+        const uint32_t k = atomicAdd(&candidateCount, 1u);
+        if (k < kYarnSlots) candidates[k] = idx;
+    };
+    // Kept list: thread t keeps entry t and its segment's start knots in
+    // registers for the check after the run.
+    uint4 heldEntry = make_uint4(~0u, 0u, 0u, 0u);
+    V3 heldK0 = v3(0.0f, 0.0f, 0.0f), heldK1 = heldK0;
+    if (listState == 1u) {
+        for (uint32_t k = tid; k < keptCount; k += threads) {
+            const uint4 entry = k == tid ? prefetched : a.listPairs[k];
+            const uint32_t idx = entry.x;
+            const uint32_t f = idx / D, s = idx - f * D;
+            const V3 k0 = xyz(knots[entry.y]), k1 = xyz(knots[entry.z]);
+            if (k == tid) {
+                heldEntry = entry;
+                heldK0 = k0;
+                heldK1 = k1;
             }
-            if (!candidate) continue;
-            atomicAdd(&counts[phaseBit(f, s, batch)], 1u);
-            atomicOr(&visitors[s], 1u << f);
-            const uint32_t k = atomicAdd(&candidateCount, 1u);
-            if (k < kYarnSlots) candidates[k] = idx;
+            const NumiClothBagGPUFruit& fr = fruitStart[f];
+            const float limit = fr.previousAndRadius.w + yarnRadius + a.margin;
+            if (samplePointSegment(xyz(fr.positionAndInverseMass), k0, k1).distance < limit)
+                stageCandidate(f, s, a.segmentBatch[s], idx);
+        }
+    } else {
+        // A thread takes segments and tests every fruit against each, so a
+        // segment's colour and endpoints are read once.
+        for (uint32_t s = tid; s < D; s += threads) {
+            const uint2 batch = a.segmentBatch[s];
+            if (batch.y == 0u) continue;
+            const uint2 e = a.ij[s];
+            const V3 k0 = xyz(knots[e.x]), k1 = xyz(knots[e.y]);
+            const V3 mid = (k0 + k1) * 0.5f;
+            const float half = 0.5f * length(k1 - k0);
+            for (uint32_t f = 0; f < F; ++f) {
+                const uint32_t idx = f * D + s;
+                bool candidate;
+                if (a.mode == 1u) {
+                    candidate = a.contacts[idx].control.y != 0u;
+                } else {
+                    const NumiClothBagGPUFruit& fr = fruitStart[f];
+                    const V3 c = xyz(fr.positionAndInverseMass);
+                    const float limit = fr.previousAndRadius.w + yarnRadius + a.margin;
+                    if (listState == 2u) {
+                        const float target = fr.previousAndRadius.w + yarnRadius;
+                        const float buildLimit = target + a.listMargin;
+                        if (beyondSegment(c, mid, half, buildLimit)) continue;
+                        const float distance = samplePointSegment(c, k0, k1).distance;
+                        if (!(distance < buildLimit)) continue;
+                        const uint32_t k = atomicAdd(&listCount, 1u);
+                        if (k < a.listCapacity) a.listPairs[k] = make_uint4(idx, e.x, e.y, 0u);
+                        candidate = distance < limit;
+                    } else {
+                        candidate =
+                            !beyondSegment(c, mid, half, limit) && samplePointSegment(c, k0, k1).distance < limit;
+                    }
+                }
+                if (candidate) stageCandidate(f, s, batch, idx);
+            }
+        }
+        if (listState == 2u) {
+            for (uint32_t i = tid; i < P; i += threads) a.listKnots[i] = knots[i];
+            if (tid < F) {
+                const NumiClothBagGPUFruit& fr = fruitStart[tid];
+                const float target = fr.previousAndRadius.w + yarnRadius;
+                a.listFruits[tid] = withXyz(make_float4(0.0f, 0.0f, 0.0f, target + a.listMargin),
+                                            xyz(fr.positionAndInverseMass));
+            }
         }
     }
     __syncthreads();
+    // Staging: slots in phase order, each with its fruit, segment knots,
+    // record, the end of its colour's slots and the fruits whose visit of
+    // its segment comes first in the reference order (an earlier phase, or
+    // the same phase and a lower fruit: fruits f and f + n of a colour of n
+    // segments).
     const uint32_t total = candidateCount;
-    blockExclusiveScan(counts, D, scanScratch);
+    auto fillSlot = [&](uint32_t slot, uint32_t idx, uint32_t f, uint32_t s, uint2 batch, uint32_t phase) {
+        slotPair[slot] = idx;
+        slotFruit[slot] = f;
+        slotEnds[slot] = a.ij[s];
+        uint32_t earlier = 0u;
+        for (uint32_t others = visitors[s] & ~(1u << f); others != 0u; others &= others - 1u) {
+            const uint32_t g = static_cast<uint32_t>(__ffs(static_cast<int>(others))) - 1u;
+            const uint32_t pg = phaseBit(g, s, batch);
+            if (pg < phase || (pg == phase && g < f)) earlier |= 1u << g;
+        }
+        slotEarlier[slot] = earlier;
+        const NumiClothBagGPUYarnContact& c = a.contacts[idx];
+        slotStage[slot] = YarnStage{c.fruitNormalAndImpulse, c.segmentImpulse, c.control.w, 0u};
+    };
     if (total > kYarnSlots) {
         if (tid == 0) verdict = 1u;
+    } else if (total <= kYarnRankMax) {
+        // A fruit has one visit per phase, so (phase, fruit) orders the
+        // candidates; colours are consecutive phase ranges in segment order.
+        uint32_t* keys = scanScratch;
+        uint32_t* colours = scanScratch + kYarnRankMax;
+        uint32_t idx = 0u, f = 0u, s = 0u, phase = 0u;
+        uint2 batch = make_uint2(0u, 0u);
+        if (tid < total) {
+            idx = candidates[tid];
+            f = idx / D;
+            s = idx - f * D;
+            batch = a.segmentBatch[s];
+            phase = phaseBit(f, s, batch);
+            keys[tid] = phase * 32u + f;
+            colours[tid] = batch.x;
+        }
+        __syncthreads();
+        if (tid < total) {
+            const uint32_t key = keys[tid];
+            uint32_t rank = 0u, colourEnd = 0u;
+            for (uint32_t j = 0; j < total; ++j) {
+                rank += keys[j] < key ? 1u : 0u;
+                colourEnd += colours[j] <= batch.x ? 1u : 0u;
+            }
+            slotColourEnd[rank] = colourEnd;
+            fillSlot(rank, idx, f, s, batch, phase);
+        }
     } else {
+        blockExclusiveScan(counts, D, scanScratch);
         // counts[b] becomes the end of phase b's group
         for (uint32_t k = tid; k < total; k += threads) {
             const uint32_t idx = candidates[k];
-            const uint32_t f = idx / D, s = idx % D;
+            const uint32_t f = idx / D, s = idx - f * D;
             const uint2 batch = a.segmentBatch[s];
             const uint32_t phase = phaseBit(f, s, batch);
-            const uint32_t slot = atomicAdd(&counts[phase], 1u);
-            slotPair[slot] = idx;
-            slotPhase[slot] = phase;
-            slotColour[slot] = batch.x;
-            slotEnds[slot] = a.ij[s];
-            // The other fruits whose visit of segment s comes first in the
-            // reference order: an earlier phase, or the same phase and a
-            // lower fruit (fruits f and f + n of a colour of n segments).
-            uint32_t earlier = 0u;
-            for (uint32_t others = visitors[s] & ~(1u << f); others != 0u; others &= others - 1u) {
-                const uint32_t g = static_cast<uint32_t>(__ffs(static_cast<int>(others))) - 1u;
-                const uint32_t pg = phaseBit(g, s, batch);
-                if (pg < phase || (pg == phase && g < f)) earlier |= 1u << g;
-            }
-            slotEarlier[slot] = earlier;
-            const NumiClothBagGPUYarnContact& c = a.contacts[idx];
-            slotStage[slot] = YarnStage{c.fruitNormalAndImpulse, c.segmentImpulse, c.control.w, 0u};
+            fillSlot(atomicAdd(&counts[phase], 1u), idx, f, s, batch, phase);
+        }
+        __syncthreads();
+        for (uint32_t k = tid; k < total; k += threads) {
+            const uint32_t s = slotPair[k] - slotFruit[k] * D;
+            const uint2 batch = a.segmentBatch[s];
+            slotColourEnd[k] = counts[batch.x + batch.y - 1u];  // end of its last phase
         }
     }
     __syncthreads();
@@ -356,20 +573,19 @@ __global__ void __launch_bounds__(kYarnThreads) k_yarn_solve(YarnArgs a) {
         if (live) fruit = fruitStart[lane];
         float travel = 0.0f;
         for (uint32_t colourStart = 0; colourStart < total;) {
-            const uint32_t colour = slotColour[colourStart];
-            const uint32_t colourEnd = counts[colour + a.segmentBatch[colour].y - 1u];  // end of its last phase
+            const uint32_t colourEnd = slotColourEnd[colourStart];
             uint32_t cursor = colourStart;
             bool finished = !live;
             while (true) {
                 if (!finished) {
-                    while (cursor < colourEnd && slotPair[cursor] / D != lane) ++cursor;
+                    while (cursor < colourEnd && slotFruit[cursor] != lane) ++cursor;
                     finished = cursor >= colourEnd;
                 }
                 if (__all(finished)) break;
                 if (!finished) {
                     const uint32_t k = cursor;
                     const uint32_t idx = slotPair[k];
-                    const uint32_t s = idx % D;
+                    const uint32_t s = idx - lane * D;
                     if ((slotEarlier[k] & ~visited[s]) == 0u) {
                         YarnStage stage = slotStage[k];
                         bool acted = false;
@@ -391,21 +607,51 @@ __global__ void __launch_bounds__(kYarnThreads) k_yarn_solve(YarnArgs a) {
     }
     __syncthreads();
     // 3. present mode: every visit that was not run must have stayed a no-op.
-    // Non-candidates started at least `margin` beyond contact; if no fruit
-    // plus no knot travelled that far in this call, none can have reached
-    // contact. Otherwise check each such visit's own start distance.
+    // Non-candidates that were tested started at least `margin` beyond
+    // contact; if no fruit plus no knot travelled that far in this call,
+    // none can have reached contact. Visits off the kept list are covered by
+    // the drift bound. Otherwise check each such visit's own start distance.
     if (verdict == 0u && a.mode == 0u) {
         float local = 0.0f;
         for (uint32_t i = tid; i < P; i += threads) local = fmaxf(local, moved[i]);
-        atomicMax(&movedBits, __float_as_uint(local));
+        waveMaxInto(local, waveMoved);
         __syncthreads();
-        float fruitMax = 0.0f;
-        for (uint32_t f = 0; f < F; ++f) fruitMax = fmaxf(fruitMax, fruitMoved[f]);
-        const bool cheap = fruitMax + __uint_as_float(movedBits) < a.margin;
-        if (!cheap) {
+        float fruitMax = 0.0f, knotMax = 0.0f;
+        for (uint32_t f = 0; f < F; ++f) fruitMax = fmaxf(fruitMax, fruitMoved[f] < INFINITY ? fruitMoved[f] : INFINITY);
+        for (uint32_t w = 0; w < threads / 64u; ++w) knotMax = fmaxf(knotMax, waveMoved[w]);
+        const bool cheap = fruitMax + knotMax + slack < a.margin;
+        const bool haveList = listState == 1u || (listState == 2u && listCount <= a.listCapacity);
+        const float startDrift = listState == 1u ? keptDrift : 0.0f;
+        const bool offListHold = haveList && startDrift + knotMax + fruitMax + slack < room;
+        // One visit that did not run: false when it may have reached contact.
+        auto stayedOut = [&](uint32_t f, uint32_t s, V3 k0, V3 k1, V3 mid, float half, float knotReach) {
+            if ((visitors[s] >> f) & 1u) return true;  // ran as a candidate
+            const NumiClothBagGPUFruit& fr = fruitStart[f];
+            const V3 c = xyz(fr.positionAndInverseMass);
+            const float target = fr.previousAndRadius.w + yarnRadius;
+            const float reach = fruitMoved[f] + knotReach;
+            // Far beyond contact plus reach: stayed out of contact, without
+            // the exact distance.
+            if (beyondSegment(c, mid, half, target + reach)) return true;
+            return samplePointSegment(c, k0, k1).distance - reach >= target;
+        };
+        bool bad = false;
+        if (offListHold) {
+            if (!cheap) {
+                const uint32_t n = listState == 1u ? keptCount : listCount;
+                for (uint32_t k = tid; k < n; k += threads) {
+                    const bool held = listState == 1u && k == tid;
+                    const uint4 entry = held ? heldEntry : a.listPairs[k];
+                    const uint32_t f = entry.x / D, s = entry.x - f * D;
+                    const V3 k0 = held ? heldK0 : xyz(a.pos[entry.y]), k1 = held ? heldK1 : xyz(a.pos[entry.z]);
+                    if (!stayedOut(f, s, k0, k1, (k0 + k1) * 0.5f, 0.5f * length(k1 - k0),
+                                   fmaxf(moved[entry.y], moved[entry.z])))
+                        bad = true;
+                }
+            }
+        } else if (!(cheap && listState != 1u)) {
             // Segment-major like the listing: a segment's colour, endpoints
             // and start positions are read once for all fruits.
-            bool bad = false;
             for (uint32_t s = tid; s < D; s += threads) {
                 if (a.segmentBatch[s].y == 0u) continue;
                 const uint2 e = a.ij[s];
@@ -413,26 +659,24 @@ __global__ void __launch_bounds__(kYarnThreads) k_yarn_solve(YarnArgs a) {
                 const V3 mid = (k0 + k1) * 0.5f;
                 const float half = 0.5f * length(k1 - k0);
                 const float knotReach = fmaxf(moved[e.x], moved[e.y]);
-                for (uint32_t f = 0; f < F; ++f) {
-                    const NumiClothBagGPUFruit& fr = fruitStart[f];
-                    const V3 c = xyz(fr.positionAndInverseMass);
-                    const float target = fr.previousAndRadius.w + yarnRadius;
-                    const float reach = fruitMoved[f] + knotReach;
-                    // Far beyond contact plus reach: an unlisted visit that
-                    // stayed out of contact, without the exact distance.
-                    if (beyondSegment(c, mid, half, target + reach)) continue;
-                    const PointSample ps = samplePointSegment(c, k0, k1);
-                    if (ps.distance < fr.previousAndRadius.w + yarnRadius + a.margin) continue;  // listed above
-                    if (!(ps.distance - reach >= target)) bad = true;
-                }
+                for (uint32_t f = 0; f < F; ++f)
+                    if (!stayedOut(f, s, k0, k1, mid, half, knotReach)) bad = true;
             }
-            if (bad) verdict = 1u;
         }
+        if (bad) verdict = 1u;
     }
     __syncthreads();
-    if (tid == 0 && a.stats != nullptr) {
-        atomicAdd(&a.stats[0], verdict);
-        atomicMax(&a.stats[1], total);
+    if (tid == 0) {
+        if (a.stats != nullptr) {
+            atomicAdd(&a.stats[0], verdict);
+            atomicMax(&a.stats[1], total);
+            atomicAdd(&a.stats[2], listState == 2u ? 1u : 0u);
+            atomicAdd(&a.stats[3], listState == 1u ? 1u : 0u);
+        }
+        if (listState == 2u) {
+            a.listHeader[0] = min(listCount, a.listCapacity);
+            a.listHeader[1] = listCount <= a.listCapacity ? 1u : 0u;
+        }
     }
     if (verdict == 0u) {
         for (uint32_t i = tid; i < P; i += threads) a.pos[i] = knots[i];

@@ -152,8 +152,15 @@ struct NumiClothHipSolver {
     bool yarnAir = false;
     std::vector<Range> pairColours;
     DeviceArray<uint2> pairColourRanges;
+    DeviceArray<uint32_t> pairColourOf;  // per fruit pair: the index of its colour
     DeviceArray<uint2> segmentBatch;  // per distance: (first, count) of its colour
-    DeviceArray<uint32_t> yarnStats;  // sparse yarn solve: reruns, most candidates
+    DeviceArray<uint32_t> yarnStats;  // sparse yarn solve: reruns, most candidates, list builds, kept-list solves
+    // Present-mode fruit/yarn kept list: header (pairs, valid), pairs, the
+    // knots and fruits of its build; off with NUMI_CLOTH_HIP_YARN_LIST=0.
+    bool yarnList = true;
+    DeviceArray<uint32_t> yarnListHeader;
+    DeviceArray<uint4> yarnListPairs;
+    DeviceArray<float4> yarnListKnots, yarnListFruits;
     std::vector<uint32_t> rimHost;
     DeviceArray<NumiClothBagGPUFruit> fruits;
     DeviceArray<NumiClothBagGPUFruitPair> fruitPairs;
@@ -339,9 +346,9 @@ struct NumiClothHipSolver {
     void launchFruitPairs() {
         if (pairColours.empty()) return;
         flushCollide();
-        k_fruit_pairs<<<1, 64, 0, stream>>>(cfg(), ok(), pairColourRanges.ptr,
-                                            static_cast<uint32_t>(pairColours.size()), fruits.ptr, F,
-                                            fruitPairs.ptr, failure.ptr);
+        k_fruit_pairs<<<1, kFruitPairThreads, 0, stream>>>(cfg(), ok(), pairColourRanges.ptr, pairColourOf.ptr,
+                                                          static_cast<uint32_t>(pairColours.size()), fruits.ptr, F,
+                                                          fruitPairs.ptr, FP, failure.ptr);
     }
 
     YarnArgs yarnArgs(uint32_t mode, float4* positions) const {
@@ -365,6 +372,14 @@ struct NumiClothHipSolver {
         a.margin = initialConfig.clothMaterial.x;
         a.stats = yarnStats.ptr;
         a.failure = failure.ptr;
+        if (yarnList && mode == 0u) {
+            a.listHeader = yarnListHeader.ptr;
+            a.listPairs = yarnListPairs.ptr;
+            a.listKnots = yarnListKnots.ptr;
+            a.listFruits = yarnListFruits.ptr;
+            a.listCapacity = static_cast<uint32_t>(yarnListPairs.count);
+            a.listMargin = kYarnListMargins * a.margin;
+        }
         return a;
     }
 
@@ -379,8 +394,10 @@ struct NumiClothHipSolver {
         }
         if (withPairs) {
             a.pairColours = pairColourRanges.ptr;
+            a.pairColourOf = pairColourOf.ptr;
             a.pairColourCount = static_cast<uint32_t>(pairColours.size());
             a.fruitPairs = fruitPairs.ptr;
+            a.fruitPairCount = FP;
         }
         k_yarn_solve<<<1, kYarnThreads, lds, stream>>>(a);
     }
@@ -1267,20 +1284,27 @@ void createSolver(const NumiClothHipSceneDesc& scene, NumiClothHipSolver& s) {
         s.fruitPairs.allocate(s.FP);
         s.fruitPairs.upload(pairs.data(), s.FP, st);
         std::vector<uint2> ranges;
-        for (const Range& r : s.pairColours) ranges.push_back(make_uint2(r.first, r.count));
+        std::vector<uint32_t> colourOf(std::max(s.FP, 1u), ~0u);
+        for (uint32_t r = 0; r < s.pairColours.size(); ++r) {
+            ranges.push_back(make_uint2(s.pairColours[r].first, s.pairColours[r].count));
+            for (uint32_t k = 0; k < s.pairColours[r].count; ++k) colourOf[s.pairColours[r].first + k] = r;
+        }
         s.pairColourRanges.allocate(ranges.size());
         s.pairColourRanges.upload(ranges.data(), ranges.size(), st);
+        s.pairColourOf.allocate(colourOf.size());
+        s.pairColourOf.upload(colourOf.data(), colourOf.size(), st);
         s.yarnContacts.allocate(s.Y);
         s.yarnContacts.upload(scene.yarnContacts, s.Y, st);
         if (s.Y != 0) {
             if (s.F > 32) fail(NUMI_CLOTH_HIP_ERROR_UNSUPPORTED, "fruit/yarn solve runs one fruit per lane of one wave");
             int maxShared = 0;
             HIP_OK(hipDeviceGetAttribute(&maxShared, hipDeviceAttributeMaxSharedMemoryPerBlock, s.device));
-            // Static LDS of the yarn kernels (scan scratch, candidate list,
-            // staged visits with their phases, fruits) plus the knots, their
-            // travel and the phase counts.
-            const uint64_t fixed = 4ull * kYarnThreads + (28ull + sizeof(YarnStage)) * kYarnSlots +
-                                   32ull * (sizeof(NumiClothBagGPUFruit) + 20) + 256;
+            // Static LDS of the yarn solve (scan scratch, candidate list,
+            // staged visits, fruits) plus the knots, their travel and the
+            // per-segment phase counts and fruit masks.
+            hipFuncAttributes attributes{};
+            HIP_OK(hipFuncGetAttributes(&attributes, reinterpret_cast<const void*>(&k_yarn_solve)));
+            const uint64_t fixed = attributes.sharedSizeBytes;
             const uint64_t dynamic = static_cast<uint64_t>(s.P) * 20 + static_cast<uint64_t>(s.D) * 12;
             if (fixed + dynamic > static_cast<uint64_t>(maxShared)) {
                 fail(NUMI_CLOTH_HIP_ERROR_UNSUPPORTED, "fruit/yarn contact stages every knot in LDS");
@@ -1295,8 +1319,16 @@ void createSolver(const NumiClothHipSceneDesc& scene, NumiClothHipSolver& s) {
             s.distanceBatchesDevice.upload(batches.data(), batches.size(), st);
             s.segmentBatch.allocate(s.D);
             s.segmentBatch.upload(segmentBatch.data(), s.D, st);
-            s.yarnStats.allocate(2);
-            HIP_OK(hipMemsetAsync(s.yarnStats.ptr, 0, 2 * sizeof(uint32_t), st));
+            s.yarnStats.allocate(4);
+            HIP_OK(hipMemsetAsync(s.yarnStats.ptr, 0, 4 * sizeof(uint32_t), st));
+            if (const char* env = std::getenv("NUMI_CLOTH_HIP_YARN_LIST")) s.yarnList = std::strcmp(env, "0") != 0;
+            s.yarnListHeader.allocate(4);
+            s.yarnListPairs.allocate(kYarnListCapacity);
+            s.yarnListKnots.allocate(s.P);
+            s.yarnListFruits.allocate(32);
+            HIP_OK(hipMemsetAsync(s.yarnListHeader.ptr, 0, 4 * sizeof(uint32_t), st));
+            HIP_OK(hipMemsetAsync(s.yarnListKnots.ptr, 0, s.P * sizeof(float4), st));
+            HIP_OK(hipMemsetAsync(s.yarnListFruits.ptr, 0, 32 * sizeof(float4), st));
         }
         if (scene.mouthRimCount != 0) {
             if (scene.mouthRimParticles == nullptr) fail(NUMI_CLOTH_HIP_ERROR_INVALID_ARGUMENT, "null mouth rim");
@@ -2055,11 +2087,13 @@ void diagnoseSolver(NumiClothHipSolver& s, NumiClothHipDiagnostics* out) {
         out->tileRoundsRun = stats[1];
     }
     if (s.yarnStats.count != 0) {
-        uint32_t stats[2] = {0u, 0u};
-        s.yarnStats.download(stats, 2, s.stream);
+        uint32_t stats[4] = {0u, 0u, 0u, 0u};
+        s.yarnStats.download(stats, 4, s.stream);
         HIP_OK(hipStreamSynchronize(s.stream));
         out->yarnReruns = stats[0];
         out->yarnCandidatesMax = stats[1];
+        out->yarnListBuilds = stats[2];
+        out->yarnListSolves = stats[3];
     }
     if (s.self) {
         SelfGrid g{};

@@ -228,3 +228,53 @@ def test_bag_with_self_contact_frame_parity(cloth, oracle):
     assert abs(int(counters[0]) - twin["presentContacts"]) <= max(4, twin["presentContacts"] // 100)
     assert abs(int(counters[1]) - twin["sweptContacts"]) <= 4
     assert abs(int(metrics[3]) - twin["selfFrictionContacts"]) <= 4
+
+
+def _lifted_bag(cloth, frames):
+    """The bag with self-contact and 12 fruits of 2-3 cm, its cuff grip
+    lifted by 0.4 m * smoothstep(t / 2 s); 48 substeps per 1/120 s frame."""
+    scene = cloth.Scene.bag(stretchComplianceOverride=1.0e-7, bendComplianceOverride=1.0e-4,
+                            strainLimitOverride=0.10, fruitRadiusMin=0.02, fruitRadiusMax=0.03, seed=20261203,
+                            selfContact=1)
+    scene.config["constraintCounts"][0, 2] = 1
+    substeps, dt = 48, 1.0 / (120.0 * 48)
+    scene.config["gravityAndTimestep"][0, 3] = dt
+    scene.refresh()
+    base = scene.config[0]["gripTargetAndActive"].astype(np.float64)
+    configs = scene.configs(substeps * frames)
+    t = np.arange(1, substeps * frames + 1) * dt
+    u = np.clip(t / 2.0, 0.0, 1.0)
+    configs["gripTargetAndActive"][:, 2] = base[2] + 0.4 * u * u * (3.0 - 2.0 * u)
+    configs["gravityAndTimestep"][:, 3] = dt
+    return scene, configs
+
+
+def test_yarn_kept_list_gives_the_full_listing_bytes(cloth):
+    """Present-mode fruit/yarn solves that take their candidates from the
+    kept list of visits near contact (valid while the knots and fruits have
+    drifted less than its room) end in the bytes of solves that test every
+    (fruit, segment) visit."""
+    import os
+    scene, configs = _lifted_bag(cloth, 12)  # the fruits reach the yarn within these frames
+    out = {}
+    for mode in ("1", "0"):
+        saved = os.environ.get("NUMI_CLOTH_HIP_YARN_LIST")
+        os.environ["NUMI_CLOTH_HIP_YARN_LIST"] = mode
+        try:
+            with cloth.Solver(scene) as solver:
+                solver.step(configs, 32, 3)
+                out[mode] = (solver.read(), solver.diagnostics())
+        finally:
+            if saved is None:
+                os.environ.pop("NUMI_CLOTH_HIP_YARN_LIST", None)
+            else:
+                os.environ["NUMI_CLOTH_HIP_YARN_LIST"] = saved
+    (kept, d), (full, d0) = out["1"], out["0"]
+    print(f"kept list: {d.yarnListBuilds} builds, {d.yarnListSolves} solves from the list, most candidates "
+          f"{d.yarnCandidatesMax}, reruns {d.yarnReruns}; without: reruns {d0.yarnReruns}")
+    assert kept["failure"] == full["failure"] == 0
+    assert d.yarnListBuilds >= 1 and d.yarnListSolves > 10 * d.yarnListBuilds
+    assert d0.yarnListBuilds == 0 and d0.yarnListSolves == 0
+    assert d.yarnCandidatesMax > 0
+    for name in ("particles", "fruits", "fruitPairs", "yarnContacts", "frictionStatus", "selfStatus"):
+        assert kept[name].tobytes() == full[name].tobytes(), name
