@@ -125,31 +125,40 @@ __global__ void k_advance(const NumiClothBagGPUConfig* __restrict__ cfg, const u
 // src/metal/ClothBag.metal:1148-1213) on values already loaded: updates a, b
 // and lambda in place; returns false when nothing is written.
 // invDt2 = 1 / (dt * dt) of the substep: alpha = compliance * invDt2.
-__device__ __forceinline__ bool projectDistance(float4& a, float4& b, float2 rc, float& lam, float invDt2,
-                                                uint32_t* failure) {
+// The arithmetic without branches: every value is computed and the result
+// selected, failures are or-ed into `fail` (so that several independent
+// projections can be interleaved by the compiler).
+__device__ __forceinline__ bool projectDistanceCore(float4& a, float4& b, float2 rc, float& lam, float invDt2,
+                                                    uint32_t& fail) {
     const V3 d = xyz(b) - xyz(a);
     const float len = length(d);
     const float alpha = rc.y * invDt2;
     const float denom = a.w + b.w + alpha;
-    if (!isfinite(len) || !isfinite(alpha) || !isfinite(denom) || !isfinite(rc.x) || !isfinite(lam)) {
-        recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
-        return false;
-    }
-    if (!(len > 1.0e-12f) || !(denom > 0.0f)) return false;
+    const bool bad = !isfinite(len) || !isfinite(alpha) || !isfinite(denom) || !isfinite(rc.x) || !isfinite(lam);
+    const bool go = !bad && len > 1.0e-12f && denom > 0.0f;
     const float value = len - rc.x;
     const float dl = (-value - alpha * lam) / denom;
     const V3 corr = d * (dl / len);
     const float nl = lam + dl;
     const V3 na = xyz(a) - corr * a.w;
     const V3 nb = xyz(b) + corr * b.w;
-    if (!finite3(na) || !finite3(nb) || !isfinite(nl)) {
-        recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
-        return false;
+    const bool badOut = go && (!finite3(na) || !finite3(nb) || !isfinite(nl));
+    fail |= bad || badOut ? NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE : 0u;
+    const bool apply = go && !badOut;
+    if (apply) {
+        a = withXyz(a, na);
+        b = withXyz(b, nb);
+        lam = nl;
     }
-    a = withXyz(a, na);
-    b = withXyz(b, nb);
-    lam = nl;
-    return true;
+    return apply;
+}
+
+__device__ __forceinline__ bool projectDistance(float4& a, float4& b, float2 rc, float& lam, float invDt2,
+                                                uint32_t* failure) {
+    uint32_t fail = 0u;
+    const bool apply = projectDistanceCore(a, b, rc, lam, invDt2, fail);
+    if (fail != 0u) recordFailure(failure, fail);
+    return apply;
 }
 
 // solve_distance over one colour batch [first, first + count).
@@ -275,64 +284,70 @@ __global__ void k_knot(const NumiClothBagGPUConfig* __restrict__ cfg, const uint
 // grounded endpoint that would be pushed down keeps its height) and a step
 // fraction that stops free endpoints at the ground.
 // invDt2 = 1 / (dt * dt) of the substep: alpha = compliance * invDt2.
-__device__ __forceinline__ bool projectBend(float4& a, float4& b, float2 cc, float& lam, float invDt2, bool ground,
-                                            float floorZ, uint32_t* failure) {
+// Without branches, as projectDistanceCore: every value computed, the
+// result selected, failures or-ed into `fail`.
+__device__ __forceinline__ bool projectBendCore(float4& a, float4& b, float2 cc, float& lam, float invDt2, bool ground,
+                                                float floorZ, uint32_t& fail) {
     const V3 d = xyz(b) - xyz(a);
     const float chord = length(d);
-    if (!isfinite(chord) || !isfinite(cc.x) || !isfinite(cc.y) || !isfinite(lam)) {
-        recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
-        return false;
-    }
-    if (!(chord > 1.0e-12f)) return false;
+    const bool bad = !isfinite(chord) || !isfinite(cc.x) || !isfinite(cc.y) || !isfinite(lam);
+    bool go = !bad && chord > 1.0e-12f;
     const float value = chord - cc.x;
     const V3 dir = d * (1.0f / chord);
-    const V3 g[2] = {-dir, dir};
-    const float w[2] = {a.w, b.w};
+    const V3 g0 = -dir, g1 = dir;
+    const float w0 = a.w, w1 = b.w;
     const float alpha = cc.y * invDt2;
     float freeDenom = alpha;
-    for (int k = 0; k < 2; ++k) freeDenom += w[k] * dot(g[k], g[k]);
-    if (!(freeDenom >= 1.0e-16f) || !isfinite(freeDenom)) return false;
+    freeDenom += w0 * dot(g0, g0);
+    freeDenom += w1 * dot(g1, g1);
+    go = go && freeDenom >= 1.0e-16f && isfinite(freeDenom);
     const float numerator = -value - alpha * lam;
     const float freeDl = numerator / freeDenom;
-    const float z[2] = {a.z, b.z};
-    bool active[2];
+    const bool active0 = ground && a.z <= floorZ + 1.0e-9f && g0.z * freeDl < 0.0f;
+    const bool active1 = ground && b.z <= floorZ + 1.0e-9f && g1.z * freeDl < 0.0f;
     float denom = alpha;
-    for (int k = 0; k < 2; ++k) {
-        active[k] = ground && z[k] <= floorZ + 1.0e-9f && g[k].z * freeDl < 0.0f;
-        denom += w[k] * (dot(g[k], g[k]) - (active[k] ? g[k].z * g[k].z : 0.0f));
-    }
-    if (!(denom >= 1.0e-16f) || !isfinite(denom)) return false;
+    denom += w0 * (dot(g0, g0) - (active0 ? g0.z * g0.z : 0.0f));
+    denom += w1 * (dot(g1, g1) - (active1 ? g1.z * g1.z : 0.0f));
+    go = go && denom >= 1.0e-16f && isfinite(denom);
     // Without an active ground knot denom is freeDenom to the bit (the same
     // sums with zero subtracted), so the free multiplier is reused.
-    const float dl = active[0] || active[1] ? numerator / denom : freeDl;
+    const float dl = active0 || active1 ? numerator / denom : freeDl;
     float fraction = 1.0f;
     if (ground) {
-        for (int k = 0; k < 2; ++k) {
-            if (active[k]) continue;
-            const float down = g[k].z * w[k] * dl;
-            if (down < 0.0f) fraction = fminf(fraction, fmaxf(0.0f, z[k] - floorZ) / -down);
-        }
+        const float down0 = g0.z * w0 * dl;
+        if (!active0 && down0 < 0.0f) fraction = fminf(fraction, fmaxf(0.0f, a.z - floorZ) / -down0);
+        const float down1 = g1.z * w1 * dl;
+        if (!active1 && down1 < 0.0f) fraction = fminf(fraction, fmaxf(0.0f, b.z - floorZ) / -down1);
     }
     const float applied = dl * fraction;
     const float nl = lam + applied;
-    V3 ca = g[0] * (w[0] * applied);
-    V3 cb = g[1] * (w[1] * applied);
-    if (active[0]) ca.z = 0.0f;
-    if (active[1]) cb.z = 0.0f;
+    V3 ca = g0 * (w0 * applied);
+    V3 cb = g1 * (w1 * applied);
+    if (active0) ca.z = 0.0f;
+    if (active1) cb.z = 0.0f;
     V3 na = xyz(a) + ca;
     V3 nb = xyz(b) + cb;
     if (ground) {
         na.z = fmaxf(na.z, floorZ);
         nb.z = fmaxf(nb.z, floorZ);
     }
-    if (!finite3(na) || !finite3(nb) || !isfinite(nl)) {
-        recordFailure(failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
-        return false;
+    const bool badOut = go && (!finite3(na) || !finite3(nb) || !isfinite(nl));
+    fail |= bad || badOut ? NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE : 0u;
+    const bool apply = go && !badOut;
+    if (apply) {
+        a = withXyz(a, na);
+        b = withXyz(b, nb);
+        lam = nl;
     }
-    a = withXyz(a, na);
-    b = withXyz(b, nb);
-    lam = nl;
-    return true;
+    return apply;
+}
+
+__device__ __forceinline__ bool projectBend(float4& a, float4& b, float2 cc, float& lam, float invDt2, bool ground,
+                                            float floorZ, uint32_t* failure) {
+    uint32_t fail = 0u;
+    const bool apply = projectBendCore(a, b, cc, lam, invDt2, ground, floorZ, fail);
+    if (fail != 0u) recordFailure(failure, fail);
+    return apply;
 }
 
 __global__ void k_bend(const NumiClothBagGPUConfig* __restrict__ cfg, const uint32_t* __restrict__ ok,

@@ -137,6 +137,7 @@ __device__ __forceinline__ void tileCollide(float4* lds, uint32_t tile, uint32_t
 }
 
 constexpr uint32_t kTilePrefetch = 4;  // item slots per thread and pass held in registers
+static_assert(kTilePrefetch % 2 == 0, "register slots are processed in pairs");
 constexpr uint32_t kTileKnotSlots = 4; // loaded knots per thread whose ids stay in registers
 
 // Ids of a thread's first kTileKnotSlots loaded knots (load list entries
@@ -312,6 +313,57 @@ __device__ __forceinline__ bool tileItem(float4* lds, uint32_t kind, const uint4
         if (tileItemOwner(kind, item)) out[c] = lam;
     }
     return false;
+}
+
+// The register slots of a distance or bend pass, two at a time where any
+// lane has both: both slots' knots are read, both projections computed
+// (branch-free, so the compiler interleaves the two dependency chains) and
+// both written back. A pass is one colour, whose constraints share no knot,
+// so reading the second slot's knots before the first slot's are written
+// changes nothing. Slots past the pass hold zero items (knots 0 and 0).
+__device__ __forceinline__ void tilePairSlots(float4* lds, bool bend, const uint4* items, float* lam, uint32_t j0,
+                                              uint32_t last, uint32_t stride, float* out, uint32_t* failure,
+                                              bool live, float invDt2, bool ground, float floorZ) {
+    uint32_t fail = 0u;
+    auto project = [&](float4& p, float4& q, const uint4& item, float& l, uint32_t& f) {
+        const float2 c = make_float2(__uint_as_float(item.z), __uint_as_float(item.w));
+        return bend ? projectBendCore(p, q, c, l, invDt2, ground, floorZ, f)
+                    : projectDistanceCore(p, q, c, l, invDt2, f);
+    };
+    auto finish = [&](bool valid, bool applied, const uint4& item, const float4& p, const float4& q, float l,
+                      uint32_t f, uint32_t m) {
+        const bool run = live && valid;
+        fail |= run ? f : 0u;
+        if (run && applied) {
+            lds[item.x & 0xffffu] = p;
+            lds[item.x >> 16] = q;
+            lam[m] = l;
+        }
+        if (valid && (item.y >> 31) != 0u) out[item.y & 0x7fffffffu] = lam[m];
+    };
+#pragma unroll
+    for (uint32_t m0 = 0; m0 < kTilePrefetch; m0 += 2) {
+        const bool v0 = j0 + m0 * stride < last, v1 = j0 + (m0 + 1) * stride < last;
+        if (!__any(v0)) break;
+        const uint4 i0 = items[m0];
+        float4 p0 = lds[i0.x & 0xffffu], q0 = lds[i0.x >> 16];
+        float l0 = lam[m0];
+        uint32_t f0 = 0u;
+        if (__any(v1)) {
+            const uint4 i1 = items[m0 + 1];
+            float4 p1 = lds[i1.x & 0xffffu], q1 = lds[i1.x >> 16];
+            float l1 = lam[m0 + 1];
+            uint32_t f1 = 0u;
+            const bool a0 = project(p0, q0, i0, l0, f0);
+            const bool a1 = project(p1, q1, i1, l1, f1);
+            finish(v0, a0, i0, p0, q0, l0, f0, m0);
+            finish(v1, a1, i1, p1, q1, l1, f1, m0 + 1);
+        } else {
+            const bool a0 = project(p0, q0, i0, l0, f0);
+            finish(v0, a0, i0, p0, q0, l0, f0, m0);
+        }
+    }
+    if (fail != 0u) recordFailure(failure, fail);
 }
 
 // The first kTilePrefetch item slots of a thread in every pass are software
@@ -494,11 +546,16 @@ __global__ void k_tile_program(TileProgramArgs a) {
                                     nextLam);
             tileLoadItems<Strain>(items, itemStart, passes, k + 2, t, stride, ahead);
             const float* in = Strain ? nullptr : lambdaIn[kind];
+            if (!Strain && (kind == kTileDistanceKind || kind == kTileBendKind)) {
+                tilePairSlots(lds, kind == kTileBendKind, cur, lam, first + t, last, stride, out, failure, live,
+                              invDt2, ground, floorZ);
+            } else {
 #pragma unroll
-            for (uint32_t m = 0; m < kTilePrefetch; ++m)
-                if (first + t + m * stride < last)
-                    moved |= tileItem<Strain>(lds, kind, cur[m], lam[m], out, knotCC, failure, live, dt, invDt2,
-                                              ground, floorZ);
+                for (uint32_t m = 0; m < kTilePrefetch; ++m)
+                    if (first + t + m * stride < last)
+                        moved |= tileItem<Strain>(lds, kind, cur[m], lam[m], out, knotCC, failure, live, dt,
+                                                  invDt2, ground, floorZ);
+            }
             for (uint32_t j = first + t + kTilePrefetch * stride; j < last; j += stride) {
                 const uint4 item = tileItemAt<Strain>(items, j);
                 const float lj = Strain ? 0.0f : in[tileItemConstraint(kind, item)];
