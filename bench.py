@@ -451,6 +451,7 @@ def run_pickup(args, cloth, hip) -> dict:
                      "max_self_penetration_over_candidates": d.maximumSelfPenetration,
                      "reference_published": "480 frames: released_mask=1537 released_count=3 "
                                             "grounded_released_count=2, violations 0"}
+    diagnostics = diagnostics_of(solver, state)
     line, tile_config = roofline(cloth, solver, args.workload)
     solver.close()
     baseline = None
@@ -473,7 +474,7 @@ def run_pickup(args, cloth, hip) -> dict:
                                       f"1-{args.steps}")
     return {"metric": "particle-substeps/s", "value": value, "unit": "particle-substeps/s",
             "ms_per_step": seconds * 1000.0 / args.steps, "dtype": "f32", "config": config, "roofline": line,
-            "cpu_baseline": baseline, "vs_baseline": vs,
+            "cpu_baseline": baseline, "vs_baseline": vs, "diagnostics": diagnostics,
             "data": "reference pickup trajectory (deterministic), default bag materials"}
 
 

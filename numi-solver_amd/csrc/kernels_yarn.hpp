@@ -93,6 +93,7 @@ struct YarnArgs {
     float4* listFruits;
     uint32_t listCapacity;
     float listMargin;                    // listed: within this of contact at the build
+    uint32_t visitByVisit;               // 1: run every call visit by visit (the reference's literal order)
 };
 
 __device__ __forceinline__ uint32_t phaseBit(uint32_t f, uint32_t s, uint2 batch) {
@@ -236,11 +237,6 @@ __device__ void yarnDense(const YarnArgs& a, float4* knots, float* moved) {
     }
 }
 
-// Staged candidates up to this count are put in phase order by ranking
-// (each counts the candidates before it); more go through the per-phase
-// counts and a scan over every segment.
-constexpr uint32_t kYarnRankMax = 128;
-
 // Largest value over the workgroup of a per-thread non-negative float (NaN
 // as +inf): wave maxima into perWave[], which the caller reads after its
 // next barrier.
@@ -256,9 +252,9 @@ __device__ __forceinline__ float maxAbs3(V3 v) {
 }
 
 // One solve_yarn_batch call over all distance colours. One workgroup of
-// kYarnThreads; dynamic LDS: knots[P], moved[P], counts[D], visitors[D],
-// visited[D] (per segment: fruits with a candidate visit on it, fruits whose
-// visit has run).
+// kYarnThreads; dynamic LDS: knots[P], moved[P], visitors[D], visited[D]
+// (per segment: fruits with a candidate visit on it, fruits whose visit has
+// run).
 //
 // Present mode with a kept list (listPairs non-null): the list holds the
 // (fruit, segment) visits that were within `listMargin` of contact when it
@@ -280,6 +276,7 @@ __global__ void __launch_bounds__(kYarnThreads) k_yarn_solve(YarnArgs a) {
     __shared__ uint32_t slotPair[kYarnSlots];        // pair indices in phase order
     __shared__ uint32_t slotFruit[kYarnSlots];       // the fruit of each slot
     __shared__ uint32_t slotColourEnd[kYarnSlots];   // end of the slots of the slot's colour
+    __shared__ uint32_t firstOwn[32];                // walk: each fruit's first slot in the colour
     __shared__ uint32_t slotEarlier[kYarnSlots];     // fruits visiting the slot's segment before it
     __shared__ uint2 slotEnds[kYarnSlots];           // the slot's segment knots
     __shared__ YarnStage slotStage[kYarnSlots];
@@ -313,8 +310,7 @@ __global__ void __launch_bounds__(kYarnThreads) k_yarn_solve(YarnArgs a) {
                          a.fruitPairCount, a.failure, fruitStart);
     float4* knots = yarnLds;
     float* moved = reinterpret_cast<float*>(knots + P);
-    uint32_t* counts = reinterpret_cast<uint32_t*>(moved + P);
-    uint32_t* visitors = counts + D;
+    uint32_t* visitors = reinterpret_cast<uint32_t*>(moved + P);
     uint32_t* visited = visitors + D;
     float drift = 0.0f, scale = 0.0f;
     for (uint32_t i = tid; i < P; i += threads) {
@@ -332,7 +328,6 @@ __global__ void __launch_bounds__(kYarnThreads) k_yarn_solve(YarnArgs a) {
         waveMaxInto(scale, waveScale);
     }
     for (uint32_t i = tid; i < D; i += threads) {
-        counts[i] = 0u;
         visitors[i] = 0u;
         visited[i] = 0u;
     }
@@ -417,8 +412,7 @@ __global__ void __launch_bounds__(kYarnThreads) k_yarn_solve(YarnArgs a) {
     // A visit can act in the swept mode when its record's impact flag is
     // set, in the present mode when the fruit is within margin of contact
     // at the start of the call. Nothing depends on the order of the listing.
-    auto stageCandidate = [&](uint32_t f, uint32_t s, uint2 batch, uint32_t idx) {
-        atomicAdd(&counts[phaseBit(f, s, batch)], 1u);
+    auto stageCandidate = [&](uint32_t f, uint32_t s, uint32_t idx) {
         atomicOr(&visitors[s], 1u << f);
         const uint32_t k = atomicAdd(&candidateCount, 1u);
         if (k < kYarnSlots) candidates[k] = idx;
@@ -441,7 +435,7 @@ __global__ void __launch_bounds__(kYarnThreads) k_yarn_solve(YarnArgs a) {
             const NumiClothBagGPUFruit& fr = fruitStart[f];
             const float limit = fr.previousAndRadius.w + yarnRadius + a.margin;
             if (samplePointSegment(xyz(fr.positionAndInverseMass), k0, k1).distance < limit)
-                stageCandidate(f, s, a.segmentBatch[s], idx);
+                stageCandidate(f, s, idx);
         }
     } else {
         // A thread takes segments and tests every fruit against each, so a
@@ -476,7 +470,7 @@ __global__ void __launch_bounds__(kYarnThreads) k_yarn_solve(YarnArgs a) {
                             !beyondSegment(c, mid, half, limit) && samplePointSegment(c, k0, k1).distance < limit;
                     }
                 }
-                if (candidate) stageCandidate(f, s, batch, idx);
+                if (candidate) stageCandidate(f, s, idx);
             }
         }
         if (listState == 2u) {
@@ -490,33 +484,19 @@ __global__ void __launch_bounds__(kYarnThreads) k_yarn_solve(YarnArgs a) {
         }
     }
     __syncthreads();
-    // Staging: slots in phase order, each with its fruit, segment knots,
-    // record, the end of its colour's slots and the fruits whose visit of
-    // its segment comes first in the reference order (an earlier phase, or
-    // the same phase and a lower fruit: fruits f and f + n of a colour of n
-    // segments).
+    // Staging: slots ordered by colour, then fruit, then phase (a fruit has
+    // one visit per phase, and colours are consecutive phase ranges in
+    // segment order), each with its fruit, segment knots, record, the end of
+    // its colour's slots and the fruits whose visit of its segment comes
+    // first in the reference order (an earlier phase, or the same phase and a
+    // lower fruit: fruits f and f + n of a colour of n segments). Candidate t
+    // is ranked by thread t, which counts the candidates ordered before it.
     const uint32_t total = candidateCount;
-    auto fillSlot = [&](uint32_t slot, uint32_t idx, uint32_t f, uint32_t s, uint2 batch, uint32_t phase) {
-        slotPair[slot] = idx;
-        slotFruit[slot] = f;
-        slotEnds[slot] = a.ij[s];
-        uint32_t earlier = 0u;
-        for (uint32_t others = visitors[s] & ~(1u << f); others != 0u; others &= others - 1u) {
-            const uint32_t g = static_cast<uint32_t>(__ffs(static_cast<int>(others))) - 1u;
-            const uint32_t pg = phaseBit(g, s, batch);
-            if (pg < phase || (pg == phase && g < f)) earlier |= 1u << g;
-        }
-        slotEarlier[slot] = earlier;
-        const NumiClothBagGPUYarnContact& c = a.contacts[idx];
-        slotStage[slot] = YarnStage{c.fruitNormalAndImpulse, c.segmentImpulse, c.control.w, 0u};
-    };
-    if (total > kYarnSlots) {
+    if (total > kYarnSlots || a.visitByVisit != 0u) {
         if (tid == 0) verdict = 1u;
-    } else if (total <= kYarnRankMax) {
-        // A fruit has one visit per phase, so (phase, fruit) orders the
-        // candidates; colours are consecutive phase ranges in segment order.
-        uint32_t* keys = scanScratch;
-        uint32_t* colours = scanScratch + kYarnRankMax;
+    } else {
+        uint32_t* groupKeys = scanScratch;  // colour's first segment * 32 + fruit
+        uint32_t* phaseKeys = candidates;   // the phase (each thread has read its candidate first)
         uint32_t idx = 0u, f = 0u, s = 0u, phase = 0u;
         uint2 batch = make_uint2(0u, 0u);
         if (tid < total) {
@@ -525,47 +505,47 @@ __global__ void __launch_bounds__(kYarnThreads) k_yarn_solve(YarnArgs a) {
             s = idx - f * D;
             batch = a.segmentBatch[s];
             phase = phaseBit(f, s, batch);
-            keys[tid] = phase * 32u + f;
-            colours[tid] = batch.x;
         }
         __syncthreads();
         if (tid < total) {
-            const uint32_t key = keys[tid];
-            uint32_t rank = 0u, colourEnd = 0u;
-            for (uint32_t j = 0; j < total; ++j) {
-                rank += keys[j] < key ? 1u : 0u;
-                colourEnd += colours[j] <= batch.x ? 1u : 0u;
-            }
-            slotColourEnd[rank] = colourEnd;
-            fillSlot(rank, idx, f, s, batch, phase);
-        }
-    } else {
-        blockExclusiveScan(counts, D, scanScratch);
-        // counts[b] becomes the end of phase b's group
-        for (uint32_t k = tid; k < total; k += threads) {
-            const uint32_t idx = candidates[k];
-            const uint32_t f = idx / D, s = idx - f * D;
-            const uint2 batch = a.segmentBatch[s];
-            const uint32_t phase = phaseBit(f, s, batch);
-            fillSlot(atomicAdd(&counts[phase], 1u), idx, f, s, batch, phase);
+            groupKeys[tid] = batch.x * 32u + f;
+            phaseKeys[tid] = phase;
         }
         __syncthreads();
-        for (uint32_t k = tid; k < total; k += threads) {
-            const uint32_t s = slotPair[k] - slotFruit[k] * D;
-            const uint2 batch = a.segmentBatch[s];
-            slotColourEnd[k] = counts[batch.x + batch.y - 1u];  // end of its last phase
+        if (tid < total) {
+            const uint32_t group = batch.x * 32u + f;
+            uint32_t rank = 0u, colourEnd = 0u;
+            for (uint32_t j = 0; j < total; ++j) {
+                const uint32_t g = groupKeys[j];
+                rank += g < group || (g == group && phaseKeys[j] < phase) ? 1u : 0u;
+                colourEnd += g / 32u <= batch.x ? 1u : 0u;
+            }
+            slotColourEnd[rank] = colourEnd;
+            slotPair[rank] = idx;
+            slotFruit[rank] = f;
+            slotEnds[rank] = a.ij[s];
+            uint32_t earlier = 0u;
+            for (uint32_t others = visitors[s] & ~(1u << f); others != 0u; others &= others - 1u) {
+                const uint32_t g = static_cast<uint32_t>(__ffs(static_cast<int>(others))) - 1u;
+                const uint32_t pg = phaseBit(g, s, batch);
+                if (pg < phase || (pg == phase && g < f)) earlier |= 1u << g;
+            }
+            slotEarlier[rank] = earlier;
+            const NumiClothBagGPUYarnContact& c = a.contacts[idx];
+            slotStage[rank] = YarnStage{c.fruitNormalAndImpulse, c.segmentImpulse, c.control.w, 0u};
         }
     }
     __syncthreads();
     // 2. candidate visits on wave 0, one lane per fruit, colour by colour
-    // (the slots of a colour are contiguous and in phase order). A visit
-    // reads and writes only its fruit, the two knots of its segment and its
-    // own staged record, and the segments of one colour share no knot; so
-    // within a colour each lane runs its own visits in phase order and a
-    // visit waits only for the visits of its segment that the reference
-    // runs before it (slotEarlier, against visited[s]). The earliest
-    // pending visit is always ready, and the result is the reference's
-    // sequential one. Everything a round reads is in LDS.
+    // (a colour's slots are contiguous, each fruit's within them contiguous
+    // and in phase order). A visit reads and writes only its fruit, the two
+    // knots of its segment and its own staged record, and the segments of
+    // one colour share no knot; so within a colour each lane runs its own
+    // visits in phase order and a visit waits only for the visits of its
+    // segment that the reference runs before it (slotEarlier, against
+    // visited[s]). The earliest pending visit is always ready, and the
+    // result is the reference's sequential one. Everything a round reads is
+    // in LDS.
     if (verdict == 0u && tid < 64) {
         const uint32_t lane = tid;
         const bool live = lane < F;
@@ -574,13 +554,18 @@ __global__ void __launch_bounds__(kYarnThreads) k_yarn_solve(YarnArgs a) {
         float travel = 0.0f;
         for (uint32_t colourStart = 0; colourStart < total;) {
             const uint32_t colourEnd = slotColourEnd[colourStart];
-            uint32_t cursor = colourStart;
-            bool finished = !live;
+            // Each fruit's first slot in the colour: the slot that starts
+            // its run.
+            if (lane < 32) firstOwn[lane] = ~0u;
+            waveSync();
+            for (uint32_t j = colourStart + lane; j < colourEnd; j += 64) {
+                const uint32_t g = slotFruit[j];
+                if (j == colourStart || slotFruit[j - 1] != g) firstOwn[g] = j;
+            }
+            waveSync();
+            uint32_t cursor = live ? firstOwn[lane] : ~0u;
+            bool finished = cursor >= colourEnd;
             while (true) {
-                if (!finished) {
-                    while (cursor < colourEnd && slotFruit[cursor] != lane) ++cursor;
-                    finished = cursor >= colourEnd;
-                }
                 if (__all(finished)) break;
                 if (!finished) {
                     const uint32_t k = cursor;
@@ -594,6 +579,7 @@ __global__ void __launch_bounds__(kYarnThreads) k_yarn_solve(YarnArgs a) {
                         slotStage[k] = stage;
                         visited[s] |= 1u << lane;
                         ++cursor;
+                        finished = cursor >= colourEnd || slotFruit[cursor] != lane;
                     }
                 }
                 waveSync();

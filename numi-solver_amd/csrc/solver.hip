@@ -158,6 +158,7 @@ struct NumiClothHipSolver {
     // Present-mode fruit/yarn kept list: header (pairs, valid), pairs, the
     // knots and fruits of its build; off with NUMI_CLOTH_HIP_YARN_LIST=0.
     bool yarnList = true;
+    bool yarnVisitByVisit = false;  // NUMI_CLOTH_HIP_YARN_VISIT_BY_VISIT=1: every solve in the reference's order
     DeviceArray<uint32_t> yarnListHeader;
     DeviceArray<uint4> yarnListPairs;
     DeviceArray<float4> yarnListKnots, yarnListFruits;
@@ -372,6 +373,7 @@ struct NumiClothHipSolver {
         a.margin = initialConfig.clothMaterial.x;
         a.stats = yarnStats.ptr;
         a.failure = failure.ptr;
+        a.visitByVisit = yarnVisitByVisit ? 1u : 0u;
         if (yarnList && mode == 0u) {
             a.listHeader = yarnListHeader.ptr;
             a.listPairs = yarnListPairs.ptr;
@@ -386,7 +388,7 @@ struct NumiClothHipSolver {
     void launchYarn(uint32_t mode, bool withPairs = false, bool withGrips = false) {
         if (Y == 0) return;
         flushCollide();
-        const size_t lds = static_cast<size_t>(P) * (sizeof(float4) + sizeof(float)) + static_cast<size_t>(D) * 12;
+        const size_t lds = static_cast<size_t>(P) * (sizeof(float4) + sizeof(float)) + static_cast<size_t>(D) * 8;
         YarnArgs a = yarnArgs(mode, posCur);
         if (withGrips && G != 0) {
             a.grips = grips.ptr;
@@ -1299,13 +1301,13 @@ void createSolver(const NumiClothHipSceneDesc& scene, NumiClothHipSolver& s) {
             if (s.F > 32) fail(NUMI_CLOTH_HIP_ERROR_UNSUPPORTED, "fruit/yarn solve runs one fruit per lane of one wave");
             int maxShared = 0;
             HIP_OK(hipDeviceGetAttribute(&maxShared, hipDeviceAttributeMaxSharedMemoryPerBlock, s.device));
-            // Static LDS of the yarn solve (scan scratch, candidate list,
+            // Static LDS of the yarn solve (ranking scratch, candidate list,
             // staged visits, fruits) plus the knots, their travel and the
-            // per-segment phase counts and fruit masks.
+            // per-segment fruit masks.
             hipFuncAttributes attributes{};
             HIP_OK(hipFuncGetAttributes(&attributes, reinterpret_cast<const void*>(&k_yarn_solve)));
             const uint64_t fixed = attributes.sharedSizeBytes;
-            const uint64_t dynamic = static_cast<uint64_t>(s.P) * 20 + static_cast<uint64_t>(s.D) * 12;
+            const uint64_t dynamic = static_cast<uint64_t>(s.P) * 20 + static_cast<uint64_t>(s.D) * 8;
             if (fixed + dynamic > static_cast<uint64_t>(maxShared)) {
                 fail(NUMI_CLOTH_HIP_ERROR_UNSUPPORTED, "fruit/yarn contact stages every knot in LDS");
             }
@@ -1322,6 +1324,8 @@ void createSolver(const NumiClothHipSceneDesc& scene, NumiClothHipSolver& s) {
             s.yarnStats.allocate(4);
             HIP_OK(hipMemsetAsync(s.yarnStats.ptr, 0, 4 * sizeof(uint32_t), st));
             if (const char* env = std::getenv("NUMI_CLOTH_HIP_YARN_LIST")) s.yarnList = std::strcmp(env, "0") != 0;
+            if (const char* env = std::getenv("NUMI_CLOTH_HIP_YARN_VISIT_BY_VISIT"))
+                s.yarnVisitByVisit = std::strcmp(env, "0") != 0;
             s.yarnListHeader.allocate(4);
             s.yarnListPairs.allocate(kYarnListCapacity);
             s.yarnListKnots.allocate(s.P);

@@ -278,3 +278,39 @@ def test_yarn_kept_list_gives_the_full_listing_bytes(cloth):
     assert d.yarnCandidatesMax > 0
     for name in ("particles", "fruits", "fruitPairs", "yarnContacts", "frictionStatus", "selfStatus"):
         assert kept[name].tobytes() == full[name].tobytes(), name
+
+
+def test_sparse_yarn_solve_equals_visit_by_visit_order(cloth):
+    """Frames 21-22 of the reference's pickup replay, after the fruits have
+    come to rest on the yarn (hundreds of candidate visits per solve), with
+    the sparse fruit/yarn solve and with every solve run visit by visit in
+    the reference's literal order: the same bytes. Both start from the state
+    the sparse solve reaches after 20 frames."""
+    import os
+    c = cloth.cloth
+    scene = cloth.Scene.bag_replay(c.REPLAY_PICKUP, selfContact=1)
+    with cloth.Solver(scene) as solver:
+        for f in range(20):
+            solver.step(c.replay_frame_configs(scene.config, c.REPLAY_PICKUP, f), 32, 3)
+        later = scene.continued(solver.read())
+    out = {}
+    for mode in ("0", "1"):
+        saved = os.environ.get("NUMI_CLOTH_HIP_YARN_VISIT_BY_VISIT")
+        os.environ["NUMI_CLOTH_HIP_YARN_VISIT_BY_VISIT"] = mode
+        try:
+            with cloth.Solver(later) as solver:
+                for f in (20, 21):
+                    solver.step(c.replay_frame_configs(scene.config, c.REPLAY_PICKUP, f), 32, 3)
+                out[mode] = (solver.read(), solver.diagnostics())
+        finally:
+            if saved is None:
+                os.environ.pop("NUMI_CLOTH_HIP_YARN_VISIT_BY_VISIT", None)
+            else:
+                os.environ["NUMI_CLOTH_HIP_YARN_VISIT_BY_VISIT"] = saved
+    (sparse, d), (dense, d1) = out["0"], out["1"]
+    print(f"sparse: most candidates {d.yarnCandidatesMax}, reruns {d.yarnReruns}, kept-list solves "
+          f"{d.yarnListSolves}; visit by visit: reruns {d1.yarnReruns}")
+    assert sparse["failure"] == dense["failure"] == 0
+    assert d.yarnCandidatesMax > 128 and d.yarnReruns == 0 and d1.yarnReruns > 0
+    for name in ("particles", "fruits", "fruitPairs", "yarnContacts", "frictionStatus", "selfStatus"):
+        assert sparse[name].tobytes() == dense[name].tobytes(), name
