@@ -271,9 +271,9 @@ __device__ __forceinline__ float maxAbs3(V3 v) {
 // in the call), and falls back to checking every visit when it fails.
 __global__ void __launch_bounds__(kYarnThreads) k_yarn_solve(YarnArgs a) {
     extern __shared__ float4 yarnLds[];
-    __shared__ uint32_t scanScratch[kYarnThreads];
+    __shared__ unsigned long long rankKeys[kYarnSlots];  // staging: (colour, fruit, phase) per candidate
     __shared__ uint32_t candidates[kYarnSlots];     // pair indices, unordered
-    __shared__ uint32_t slotPair[kYarnSlots];        // pair indices in phase order
+    __shared__ uint32_t slotPair[kYarnSlots];        // pair indices by colour, fruit, phase
     __shared__ uint32_t slotFruit[kYarnSlots];       // the fruit of each slot
     __shared__ uint32_t slotColourEnd[kYarnSlots];   // end of the slots of the slot's colour
     __shared__ uint32_t firstOwn[32];                // walk: each fruit's first slot in the colour
@@ -495,30 +495,31 @@ __global__ void __launch_bounds__(kYarnThreads) k_yarn_solve(YarnArgs a) {
     if (total > kYarnSlots || a.visitByVisit != 0u) {
         if (tid == 0) verdict = 1u;
     } else {
-        uint32_t* groupKeys = scanScratch;  // colour's first segment * 32 + fruit
-        uint32_t* phaseKeys = candidates;   // the phase (each thread has read its candidate first)
+        // key: the colour's first segment, the fruit, the phase within the
+        // colour (a colour's first segment is below 2^27 and its phases below
+        // 2^32 - 1 for any scene the LDS holds).
         uint32_t idx = 0u, f = 0u, s = 0u, phase = 0u;
         uint2 batch = make_uint2(0u, 0u);
+        unsigned long long key = ~0ull;
         if (tid < total) {
             idx = candidates[tid];
             f = idx / D;
             s = idx - f * D;
             batch = a.segmentBatch[s];
             phase = phaseBit(f, s, batch);
+            key = (static_cast<unsigned long long>(batch.x) << 37) | (static_cast<unsigned long long>(f) << 32) |
+                  (phase - batch.x);
+            rankKeys[tid] = key;
         }
         __syncthreads();
         if (tid < total) {
-            groupKeys[tid] = batch.x * 32u + f;
-            phaseKeys[tid] = phase;
-        }
-        __syncthreads();
-        if (tid < total) {
-            const uint32_t group = batch.x * 32u + f;
+            const unsigned long long colourLimit = (static_cast<unsigned long long>(batch.x) + 1ull) << 37;
             uint32_t rank = 0u, colourEnd = 0u;
+#pragma unroll 8
             for (uint32_t j = 0; j < total; ++j) {
-                const uint32_t g = groupKeys[j];
-                rank += g < group || (g == group && phaseKeys[j] < phase) ? 1u : 0u;
-                colourEnd += g / 32u <= batch.x ? 1u : 0u;
+                const unsigned long long other = rankKeys[j];
+                rank += other < key ? 1u : 0u;
+                colourEnd += other < colourLimit ? 1u : 0u;
             }
             slotColourEnd[rank] = colourEnd;
             slotPair[rank] = idx;
