@@ -102,15 +102,26 @@ __device__ __forceinline__ uint32_t phaseBit(uint32_t f, uint32_t s, uint2 batch
     return batch.x + (j + batch.y - fm) % batch.y;
 }
 
+// The substep constants a visit reads, loaded once per call.
+struct YarnVisitConstants {
+    bool ground;
+    float floorZ;  // the yarn radius: the knots' floor and the contact offset
+    float dt;
+};
+
+__device__ __forceinline__ YarnVisitConstants yarnVisitConstants(const YarnArgs& a) {
+    return YarnVisitConstants{a.cfg->constraintCounts.z != 0u, a.cfg->clothMaterial.x, a.cfg->gravityAndTimestep.w};
+}
+
 // The visit (fruit, segment) of the solve: updates fruit, knots a and b (in
 // LDS) and the staged record; returns the correction lengths it applied.
-__device__ __forceinline__ float yarnVisit(const YarnArgs& a, NumiClothBagGPUFruit& fruit,
-                                           const NumiClothBagGPUYarnContact& record, YarnStage& stage, float4* knots,
-                                           uint2 e, float* moved, bool& acted) {
+__device__ __forceinline__ float yarnVisit(const YarnArgs& a, const YarnVisitConstants& k,
+                                           NumiClothBagGPUFruit& fruit, const NumiClothBagGPUYarnContact& record,
+                                           YarnStage& stage, float4* knots, uint2 e, float* moved, bool& acted) {
     acted = false;
-    const bool ground = a.cfg->constraintCounts.z != 0u;
-    const float floorZ = a.cfg->clothMaterial.x;
-    const float dt = a.cfg->gravityAndTimestep.w;
+    const bool ground = k.ground;
+    const float floorZ = k.floorZ;
+    const float dt = k.dt;
     V3 n = v3(0.0f, 0.0f, 0.0f);
     float wa = 0.0f, wb = 0.0f, correction = 0.0f;
     if (a.mode == 1u) {
@@ -124,7 +135,7 @@ __device__ __forceinline__ float yarnVisit(const YarnArgs& a, NumiClothBagGPUFru
         const V3 ka = xyz(knots[e.x]), kb = xyz(knots[e.y]);
         const V3 centre = xyz(fruit.positionAndInverseMass);
         const PointSample s = samplePointSegment(centre, ka, kb);
-        const float target = fruit.previousAndRadius.w + a.cfg->clothMaterial.x;
+        const float target = fruit.previousAndRadius.w + floorZ;
         if (s.distance < target) {
             n = s.distance < 1.0e-12f ? degenerateYarnNormal(ka, kb) : (s.closest - centre) / s.distance;
             wb = s.weight;
@@ -207,6 +218,7 @@ __device__ uint32_t blockExclusiveScan(uint32_t* counts, uint32_t n, uint32_t* s
 __device__ void yarnDense(const YarnArgs& a, float4* knots, float* moved) {
     const uint32_t lane = threadIdx.x;
     const bool live = lane < a.fruitCount;
+    const YarnVisitConstants constants = yarnVisitConstants(a);
     NumiClothBagGPUFruit fruit{};
     if (live) fruit = a.fruits[lane];
     for (uint32_t batch = 0; batch < a.batchCount; ++batch) {
@@ -220,7 +232,7 @@ __device__ void yarnDense(const YarnArgs& a, float4* knots, float* moved) {
                 NumiClothBagGPUYarnContact c = a.contacts[index];
                 YarnStage stage{c.fruitNormalAndImpulse, c.segmentImpulse, c.control.w, 0u};
                 bool acted = false;
-                yarnVisit(a, fruit, c, stage, knots, a.ij[segment], moved, acted);
+                yarnVisit(a, constants, fruit, c, stage, knots, a.ij[segment], moved, acted);
                 if (acted) {
                     c.fruitNormalAndImpulse = stage.normalImpulse;
                     c.segmentImpulse = stage.segment;
@@ -550,6 +562,7 @@ __global__ void __launch_bounds__(kYarnThreads) k_yarn_solve(YarnArgs a) {
     if (verdict == 0u && tid < 64) {
         const uint32_t lane = tid;
         const bool live = lane < F;
+        const YarnVisitConstants constants = yarnVisitConstants(a);
         NumiClothBagGPUFruit fruit{};
         if (live) fruit = fruitStart[lane];
         float travel = 0.0f;
@@ -575,7 +588,8 @@ __global__ void __launch_bounds__(kYarnThreads) k_yarn_solve(YarnArgs a) {
                     if ((slotEarlier[k] & ~visited[s]) == 0u) {
                         YarnStage stage = slotStage[k];
                         bool acted = false;
-                        travel += yarnVisit(a, fruit, a.contacts[idx], stage, knots, slotEnds[k], moved, acted);
+                        travel += yarnVisit(a, constants, fruit, a.contacts[idx], stage, knots, slotEnds[k], moved,
+                                            acted);
                         stage.acted = acted ? 1u : 0u;
                         slotStage[k] = stage;
                         visited[s] |= 1u << lane;
