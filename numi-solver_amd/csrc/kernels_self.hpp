@@ -929,21 +929,23 @@ __device__ __forceinline__ void addFixed(FixedAccumulator* acc, uint32_t* touche
 // conservative advancement (80 steps, 0.9 safety, 1e-10 minimum advance).
 // pairKnots: each candidate's endpoint knots, written with the candidate (one
 // dependent load per candidate instead of the pair, then its segments).
-__global__ void k_self_solve(const NumiClothBagGPUConfig* __restrict__ cfg, const uint32_t* __restrict__ ok,
-                             uint32_t mode, const SelfGrid* __restrict__ grid, const uint4* __restrict__ pairKnots,
-                             const float4* __restrict__ pos,
-                             const float4* __restrict__ prev, FixedAccumulator* __restrict__ acc,
-                             uint32_t* __restrict__ touched, NumiClothBagGPUSelfImpulse* __restrict__ impulses,
-                             NumiClothBagGPUSelfStatus* __restrict__ status) {
-    if (*ok == 0u) return;
+// The solve of k_self_solve by workgroup `block` of `blocks` (the fruit/yarn
+// solve runs it as the one workgroup of its launch).
+__device__ __forceinline__ void selfSolve(const NumiClothBagGPUConfig* __restrict__ cfg, uint32_t mode,
+                                          const SelfGrid* __restrict__ grid, const uint4* __restrict__ pairKnots,
+                                          const float4* __restrict__ pos, const float4* __restrict__ prev,
+                                          FixedAccumulator* __restrict__ acc, uint32_t* __restrict__ touched,
+                                          NumiClothBagGPUSelfImpulse* __restrict__ impulses,
+                                          NumiClothBagGPUSelfStatus* __restrict__ status, uint32_t block,
+                                          uint32_t blocks) {
     const uint32_t count = grid->valid ? grid->pairCount : 0u;
     const float r = cfg->clothMaterial.x;
     const float target = 2.0f * r;
     const float tolerance = 1.0e-9f;
-    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t stride = blocks * blockDim.x;
     const uint32_t rounds = (count + stride - 1) / stride;
     for (uint32_t round = 0; round < rounds; ++round) {
-        const uint32_t k = round * stride + blockIdx.x * blockDim.x + threadIdx.x;
+        const uint32_t k = round * stride + block * blockDim.x + threadIdx.x;
         bool accepted = false;
         float correction = 0.0f;
         if (k < count) {
@@ -1072,6 +1074,16 @@ __global__ void k_self_solve(const NumiClothBagGPUConfig* __restrict__ cfg, cons
             atomicMax(&status->counters.z, __float_as_uint(m));
         }
     }
+}
+
+__global__ void k_self_solve(const NumiClothBagGPUConfig* __restrict__ cfg, const uint32_t* __restrict__ ok,
+                             uint32_t mode, const SelfGrid* __restrict__ grid, const uint4* __restrict__ pairKnots,
+                             const float4* __restrict__ pos,
+                             const float4* __restrict__ prev, FixedAccumulator* __restrict__ acc,
+                             uint32_t* __restrict__ touched, NumiClothBagGPUSelfImpulse* __restrict__ impulses,
+                             NumiClothBagGPUSelfStatus* __restrict__ status) {
+    if (*ok == 0u) return;
+    selfSolve(cfg, mode, grid, pairKnots, pos, prev, acc, touched, impulses, status, blockIdx.x, gridDim.x);
 }
 
 // Moves every touched knot by the mean of its fixed-point corrections and

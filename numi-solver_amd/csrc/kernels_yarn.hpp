@@ -30,6 +30,7 @@
 // global state. Either way the result is the reference's sequential one.
 
 #include "kernels_bag.hpp"
+#include "kernels_self.hpp"
 
 namespace numi_dev {
 
@@ -94,7 +95,25 @@ struct YarnArgs {
     uint32_t listCapacity;
     float listMargin;                    // listed: within this of contact at the build
     uint32_t visitByVisit;               // 1: run every call visit by visit (the reference's literal order)
+    // The present self-contact solve that follows in the schedule (null
+    // grid: none), run by this workgroup on the committed positions.
+    const SelfGrid* selfGrid;
+    const uint4* selfPairKnots;
+    const float4* selfPrev;
+    FixedAccumulator* selfAcc;
+    uint32_t* selfTouched;
+    NumiClothBagGPUSelfImpulse* selfImpulses;
+    NumiClothBagGPUSelfStatus* selfStatus;
 };
+
+// The self-contact solve a fruit/yarn call carries, after its positions are
+// written (every thread of the workgroup).
+__device__ __forceinline__ void yarnThenSelf(const YarnArgs& a) {
+    if (a.selfGrid == nullptr) return;
+    __syncthreads();
+    selfSolve(a.cfg, 0u, a.selfGrid, a.selfPairKnots, a.pos, a.selfPrev, a.selfAcc, a.selfTouched, a.selfImpulses,
+              a.selfStatus, 0u, 1u);
+}
 
 __device__ __forceinline__ uint32_t phaseBit(uint32_t f, uint32_t s, uint2 batch) {
     const uint32_t j = s - batch.x;
@@ -695,6 +714,7 @@ __global__ void __launch_bounds__(kYarnThreads) k_yarn_solve(YarnArgs a) {
             if (!finite3(xyz(fr.positionAndInverseMass))) recordFailure(a.failure, NUMI_CLOTH_BAG_GPU_FAILURE_NONFINITE);
             a.fruits[tid] = fr;
         }
+        yarnThenSelf(a);
         return;
     }
     // Rerun visit by visit from the unchanged global state.
@@ -706,6 +726,7 @@ __global__ void __launch_bounds__(kYarnThreads) k_yarn_solve(YarnArgs a) {
     if (tid < 64) yarnDense(a, knots, moved);
     __syncthreads();
     for (uint32_t i = tid; i < P; i += threads) a.pos[i] = knots[i];
+    yarnThenSelf(a);
 }
 
 // Friction of one fruit/yarn record (reference apply_yarn_friction,

@@ -159,6 +159,11 @@ struct NumiClothHipSolver {
     // knots and fruits of its build; off with NUMI_CLOTH_HIP_YARN_LIST=0.
     bool yarnList = true;
     bool yarnVisitByVisit = false;  // NUMI_CLOTH_HIP_YARN_VISIT_BY_VISIT=1: every solve in the reference's order
+    // Bag-sized scenes run each iteration's present self-contact solve in the
+    // fruit/yarn solve's workgroup (NUMI_CLOTH_HIP_SELF_IN_YARN=0: its own
+    // launch); selfSolvedInYarn: the last fruit/yarn launch carried it.
+    bool selfInYarn = false;
+    bool selfSolvedInYarn = false;
     DeviceArray<uint32_t> yarnListHeader;
     DeviceArray<uint4> yarnListPairs;
     DeviceArray<float4> yarnListKnots, yarnListFruits;
@@ -385,7 +390,7 @@ struct NumiClothHipSolver {
         return a;
     }
 
-    void launchYarn(uint32_t mode, bool withPairs = false, bool withGrips = false) {
+    void launchYarn(uint32_t mode, bool withPairs = false, bool withGrips = false, bool withSelf = false) {
         if (Y == 0) return;
         flushCollide();
         const size_t lds = static_cast<size_t>(P) * (sizeof(float4) + sizeof(float)) + static_cast<size_t>(D) * 8;
@@ -401,15 +406,32 @@ struct NumiClothHipSolver {
             a.fruitPairs = fruitPairs.ptr;
             a.fruitPairCount = FP;
         }
+        if (withSelf && self) {
+            const uint32_t b = selfBuffer;
+            if (dirtySelf == static_cast<int>(b)) {
+                clearSelf(b);
+                dirtySelf = -1;
+            }
+            a.selfGrid = grid.ptr;
+            a.selfPairKnots = pairKnots.ptr;
+            a.selfPrev = prev.ptr;
+            a.selfAcc = accumulator[b].ptr;
+            a.selfTouched = touched[b].ptr;
+            a.selfImpulses = selfImpulses.ptr;
+            a.selfStatus = selfStatus.ptr;
+            selfSolvedInYarn = true;
+        }
         k_yarn_solve<<<1, kYarnThreads, lds, stream>>>(a);
     }
 
     // The grips (withGrips), the fruit-pair colours and the present
     // fruit/yarn solve that follows them, in one workgroup launch when the
     // scene has fruit/yarn records.
-    void launchFruitPairsAndYarn(bool withGrips = false) {
+    // withSelf: the present self-contact solve that follows joins the launch
+    // (launchSelf then launches only what comes after the solve).
+    void launchFruitPairsAndYarn(bool withGrips = false, bool withSelf = false) {
         if (Y != 0) {
-            launchYarn(0, !pairColours.empty(), withGrips);
+            launchYarn(0, !pairColours.empty(), withGrips, withSelf);
             return;
         }
         if (withGrips) launchGrips();
@@ -613,19 +635,22 @@ struct NumiClothHipSolver {
         flushCollide();
         if (!self) return;
         const uint32_t b = selfBuffer;
-        if (dirtySelf == static_cast<int>(b)) {
-            clearSelf(b);
-            dirtySelf = -1;
+        if (!selfSolvedInYarn) {
+            if (dirtySelf == static_cast<int>(b)) {
+                clearSelf(b);
+                dirtySelf = -1;
+            }
+            // Four workgroups per CU: 262,144 candidates per round of the
+            // kernel's grid-stride loop, so a draped 1M-knot sheet (~230,000
+            // candidates) waits out its chain of dependent loads (pair,
+            // endpoints, positions) once, and a call with no candidate still
+            // costs little more than its launch.
+            const uint32_t blocks = std::min<uint32_t>(1024, std::max(1u, (pairCapacity + kBlock - 1) / kBlock));
+            k_self_solve<<<blocks, kBlock, 0, stream>>>(cfg(), ok(), mode, grid.ptr, pairKnots.ptr, posCur,
+                                                         prev.ptr, accumulator[b].ptr, touched[b].ptr,
+                                                         selfImpulses.ptr, selfStatus.ptr);
         }
-        // Four workgroups per CU: 262,144 candidates per round of the
-        // kernel's grid-stride loop, so a draped 1M-knot sheet (~230,000
-        // candidates) waits out its chain of dependent loads (pair,
-        // endpoints, positions) once, and a call with no candidate still
-        // costs little more than its launch.
-        const uint32_t blocks = std::min<uint32_t>(1024, std::max(1u, (pairCapacity + kBlock - 1) / kBlock));
-        k_self_solve<<<blocks, kBlock, 0, stream>>>(cfg(), ok(), mode, grid.ptr, pairKnots.ptr, posCur,
-                                                     prev.ptr, accumulator[b].ptr, touched[b].ptr, selfImpulses.ptr,
-                                                     selfStatus.ptr);
+        selfSolvedInYarn = false;
         selfBuffer ^= 1u;
         const bool colliding = collide && groundOrColliders();
         if (defer) {
@@ -888,7 +913,7 @@ struct NumiClothHipSolver {
                 launchKnots();
                 if (it % 2 == 0) launchBends();
             }
-            launchFruitPairsAndYarn(true);
+            launchFruitPairsAndYarn(true, self && !pointSelf && selfInYarn);
             // The fruit ground pass joins the ground/sphere launch where one
             // follows here, or runs on its own.
             bool fruitGroundDone = false;
@@ -1326,6 +1351,11 @@ void createSolver(const NumiClothHipSceneDesc& scene, NumiClothHipSolver& s) {
             if (const char* env = std::getenv("NUMI_CLOTH_HIP_YARN_LIST")) s.yarnList = std::strcmp(env, "0") != 0;
             if (const char* env = std::getenv("NUMI_CLOTH_HIP_YARN_VISIT_BY_VISIT"))
                 s.yarnVisitByVisit = std::strcmp(env, "0") != 0;
+            // One workgroup walks every self-contact candidate: bag-sized
+            // scenes only.
+            s.selfInYarn = s.P <= 8192u;
+            if (const char* env = std::getenv("NUMI_CLOTH_HIP_SELF_IN_YARN"))
+                s.selfInYarn = s.selfInYarn && std::strcmp(env, "0") != 0;
             s.yarnListHeader.allocate(4);
             s.yarnListPairs.allocate(kYarnListCapacity);
             s.yarnListKnots.allocate(s.P);

@@ -314,3 +314,31 @@ def test_sparse_yarn_solve_equals_visit_by_visit_order(cloth):
     assert d.yarnCandidatesMax > 128 and d.yarnReruns == 0 and d1.yarnReruns > 0
     for name in ("particles", "fruits", "fruitPairs", "yarnContacts", "frictionStatus", "selfStatus"):
         assert sparse[name].tobytes() == dense[name].tobytes(), name
+
+
+def test_self_solve_in_the_yarn_workgroup_gives_the_same_bytes(cloth):
+    """The bag runs each iteration's present self-contact solve in the
+    fruit/yarn solve's workgroup, after its positions are written; the bytes
+    equal those of the self-contact solve as a launch of its own."""
+    import os
+    scene, configs = _lifted_bag(cloth, 4)
+    out = {}
+    for mode in ("1", "0"):
+        saved = os.environ.get("NUMI_CLOTH_HIP_SELF_IN_YARN")
+        os.environ["NUMI_CLOTH_HIP_SELF_IN_YARN"] = mode
+        try:
+            with cloth.Solver(scene) as solver:
+                solver.step(configs, 32, 3)
+                out[mode] = solver.read()
+        finally:
+            if saved is None:
+                os.environ.pop("NUMI_CLOTH_HIP_SELF_IN_YARN", None)
+            else:
+                os.environ["NUMI_CLOTH_HIP_SELF_IN_YARN"] = saved
+    fused, own = out["1"], out["0"]
+    counters = fused["selfStatus"]["counters"][0]
+    print(f"self contacts (present, swept) {counters[:2]}")
+    assert fused["failure"] == own["failure"] == 0
+    assert counters[0] > 0
+    for name in ("particles", "fruits", "fruitPairs", "yarnContacts", "frictionStatus", "selfStatus"):
+        assert fused[name].tobytes() == own[name].tobytes(), name
