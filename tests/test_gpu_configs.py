@@ -3,7 +3,7 @@ by the same functions bench.py times.
 
 Two gates. Against the FP32 twin (the oracle's FP32 restatement in the
 device's operation order) the GPU must agree bit for bit on every frame.
-Against FP64 the north star's gate, positions within 1e-5 of the cloth
+Against FP64 the parity gate, positions within 1e-5 of the cloth
 extent after one frame, is asserted on the frames FP32 arithmetic can meet
 it: a workload's first frame from its seeded state and the frame in which a
 draping sheet first touches its spheres. Once a stiff sheet (axial

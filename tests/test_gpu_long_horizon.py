@@ -1,7 +1,7 @@
 """Long-horizon qualification of the produce bag on the GPU (needs a GPU):
 the reference's grounded (120 frames) and spin (60 frames) trajectory
 replays with yarn/yarn self-contact, gated as the reference gates them
-(tools/cloth_bag_metal.mm:6063-6074, :6148-6160) and with the north star's
+(tools/cloth_bag_metal.mm:6063-6074, :6148-6160) and with the path's
 invariant gates (strain-limit violation <= 1e-6 m, ground penetration
 <= 1 um), beside the values the reference publishes for its Metal runs
 (docs/METAL_CLOTH.md:426-442). Quantities set by kinematics or contact
